@@ -1,0 +1,79 @@
+/*
+ * iw_oracle.h -- CPU FP64 restatement of one time step of the reference
+ * stratified-Boussinesq solver (jmsegret/IWrepo, mode1_bbl).
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load this library, and only as the checker
+ * or the timed CPU baseline; the product never links it.
+ *
+ * Every function cites the reference file:line it restates.  The parameter
+ * block has the same layout as iw_params in include/iwgpu.h so one ctypes
+ * structure feeds both libraries (checked by the tests through
+ * iwo_params_size / iw_params_size).
+ */
+#ifndef IW_ORACLE_H
+#define IW_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define IWO_MAXSUB 64
+
+typedef struct iwo_params {
+    int nx, ny, nzloc, nsubd;
+    int bc_uv_bottom;   /* 0 Neumann (calcuvwt.f:291-295), 1 Dirichlet (calcuvwt_noslip.f:294-299) */
+    int ispecfilternl, ispecfilterexp, nspecfilterexp, navgintrfc;
+    int pad0;
+    double xlen, ylen, zlen;
+    double xnu, xkappa, brunt, grav, rho0, omz;
+    double dt;
+    double amp, p, pf, fackc;
+    double facdir, facneumn, facrobin;
+    double zcen, xbcrit;
+    double xacrit, umax, fwave;
+    double z0[IWO_MAXSUB], zh[IWO_MAXSUB];
+} iwo_params;
+
+typedef struct iwo_state iwo_state;
+
+int iwo_params_size(void);
+iwo_state *iwo_create(const iwo_params *p);
+void iwo_destroy(iwo_state *s);
+
+/* 24 state arrays (nxpp,ny,nz): 0..3 u,v,w,temp; 4..7 un..tn; 8..11 unm..tnm;
+   12..15 su..st; 16..19 sun..stn; 20..23 sunm..stnm. */
+double *iwo_field(iwo_state *s, int idx);
+/* setup tables: 0 zpts, 1 wg, 2 d (row-major d[k1*nzloc+k2]), 3 d2, 4 alpol
+   (alpol[j*nzloc+k] = P_j(x_k)), 5 xw, 6 yw, 7 z, 8 x */
+double *iwo_table(iwo_state *s, int idx);
+double iwo_time(iwo_state *s);
+
+void iwo_horfft(iwo_state *s, double *b, int is, int nk);
+void iwo_dpdz(iwo_state *s, const double *a, double *b);
+void iwo_filter_3dlegenfourier(iwo_state *s, double *f, int ifourier, int ilegen);
+int  iwo_interface_avg3d(iwo_state *s, double *f);
+/* one column penalty Helmholtz solve (penalty.f:162-251 with penalty_setup);
+   rhsr/rhsi are overwritten (scaled by -epsilon) as in the reference. */
+void iwo_penalty_solve(iwo_state *s, double epsilon, int ifbcbot, int ifbctop,
+                       double bcrbot, double bcibot, double bcrtop, double bcitop,
+                       double *rhsr, double *rhsi, double *fr, double *fi);
+/* load physical u,v,w,temp (nxpp,ny,nz each) and transform them (istart=0) */
+void iwo_set_initial(iwo_state *s, const double *u, const double *v,
+                     const double *w, const double *t);
+/* advance nsteps; counts (4 per step, may be NULL) receive interface counts */
+void iwo_run(iwo_state *s, int nsteps, int *counts);
+int  iwo_nthreads(void);
+
+double iwo_abvar1(double, double, double);
+double iwo_abvar2(double, double, double);
+double iwo_abvar3(double, double, double);
+double iwo_bdfvarv(double, double, double);
+double iwo_bdfvar1(double, double, double);
+double iwo_bdfvar2(double, double, double);
+double iwo_bdfvar3(double, double, double);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

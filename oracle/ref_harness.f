@@ -1,0 +1,268 @@
+c=======================================================================
+c  Test driver for the reference Fortran (oracle/_ref).  TEST
+c  INFRASTRUCTURE ONLY: it is compiled against the reference sources
+c  where they lie (symlinked by oracle/build_ref.py) and is never part
+c  of the product.
+c
+c  It replays the step sequence of main.f:653-1066 (jmsegret/IWrepo,
+c  mode1_bbl) without main.f's file output:
+c    * maxcalc (various.f:665) is replaced by its state-changing part,
+c      the horfft(+1)/horfft(-1) round trip of u,v,w,temp
+c      (various.f:739-742, 854-857); its adaptive-dt branch
+c      (various.f:911-985) is left out, so dt stays fixed;
+c    * convec, calcuvwt, interface_avg3d are the reference routines;
+c    * the SS/BDF coefficient selection copies main.f:700-777 and
+c      main.f:813-856 for istart=0 without adaptive dt.
+c  various.f cannot be compiled by flang (REAL subscript in testdump,
+c  various.f:503), so the two one-line helpers that the hot path uses
+c  from it, zero and copy (various.f:2-26), are given below.
+c
+c  Modes (namelist &harness, file harness.nml in the working dir):
+c    mode=0  full run: read physical IC, nrun steps, dump state
+c    mode=1  horfft forward of the IC fields, dump spectral fields
+c    mode=2  dpdz of the IC fields (taken as given), dump
+c    mode=3  filter_3dlegenfourier of the IC fields (spectral), dump
+c    mode=4  interface_avg3d of the IC fields (spectral), dump + counts
+c=======================================================================
+      program iwref
+      include 'dim.h'
+      include 'comwave.h'
+      include 'comleng.h'
+      include 'comtime.h'
+      include 'comflow.h'
+      include 'comfft.h'
+      include 'commatrix.h'
+      include 'comparamphys.h'
+      include 'comsubd.h'
+      include 'comspecfilter.h'
+      include 'compencoeff.h'
+      include 'comtimeadvance.h'
+      include 'comsoliton.h'
+      include 'comgridptxyz.h'
+      include 'comcritlayer.h'
+      include 'comscratch.h'
+
+      logical cpenspecfilter
+      character*256 icfile, outfile, diagfile
+      dimension wrk(nxpp,ny,nz)
+
+      common /Dump/ Tdump,Tx
+      common /anim/ianim,ianimc
+      common /Misc/tzmode
+      common /domlength/ ZL,Zcur
+      common /dtanimation/dtanim,fwave
+
+      namelist/inputs/ istart,dtin,nsteps,tanim,iform,cdtin,waveperiods,
+     >fwave,dtanim
+      namelist/params/ xlen,ylen,zlen,brunt,xnu,xkappa,grav,
+     >expan,umax,rho0,xacrit,xbcrit,zcen,xacu,zsh
+      namelist/mflags/irelax,ispecfilternl,
+     >ispecfilterexp,nspecfilterexp,amp,p,pf,fackc,fackcbv,
+     >facdir,facneumn,facrobin,navgintrfc,ncalclegspec,
+     >ifilter,nstepfilter,
+     >tstart,npostp,itempreset,nrestartdump,n3doutput,i0
+      namelist/harness/ icfile, outfile, diagfile, nrun, mode, iprobe,
+     >ipx, kp1, kp2
+
+      mode = 0
+      iprobe = 0
+      ipx = 1
+      kp1 = 1
+      kp2 = 1
+      nrun = 1
+      diagfile = 'diag.txt'
+      open(7,file='input.dat',status='old')
+      read(7,inputs)
+      read(7,params)
+      read(7,mflags)
+      close(7)
+      open(7,file='harness.nml',status='old')
+      read(7,harness)
+      close(7)
+
+c-FFT tables (main.f:210-218)
+      call fftfax(nx,ifaxx,trigsx)
+      if (ndims.eq.3) then
+        call cftfax(ny,ifaxy,trigsy)
+        call fftfax(ny,ifaxy1d,trigsy1d)
+      endif
+      scaley = 1./float(ny)
+      pi2 = 8.*atan(1.0)
+      ZL = zlen
+      Zcur = zsh
+c-Derived parameters (main.f:328-352, 416-472)
+      alpha = pi2/xlen
+      beta  = pi2/ylen
+      gamma = pi2/zlen
+      zdim = xlen
+      xkcrit = pi2/xlen
+      xmcrit = pi2/(2.*zlen)
+      xomega = (bruntz(0.0,zlen,brunt)*xkcrit)
+     >           /sqrt(xkcrit**2+xmcrit**2)
+      uo = 0.07
+      fcshear = 0.
+      omz = 0.
+      pr = xnu
+      dt = dtin
+      cdt = cdtin
+      fact = 1./(xkappa*dt)
+      facvel = 1./(pr*dt)
+      ianim = 0
+      Tdump = 0.
+      izmode = 0
+      ispecfilter = 0
+
+      call setup
+
+c-Initial condition: 4 physical fields (nxpp,ny,nz), pads ignored
+      open(10,file=icfile,form='unformatted',access='stream',
+     >     status='old')
+      read(10) u
+      read(10) v
+      read(10) w
+      read(10) temp
+      close(10)
+
+      open(12,file=diagfile,status='unknown')
+
+      if (mode.eq.1) then
+        call horfft(u,-1,nz)
+        call horfft(v,-1,nz)
+        call horfft(w,-1,nz)
+        call horfft(temp,-1,nz)
+        goto 900
+      endif
+      if (mode.eq.2) then
+        call dpdz(u,wrk)
+        call copy(wrk,u,ntot)
+        call dpdz(v,wrk)
+        call copy(wrk,v,ntot)
+        call dpdz(w,wrk)
+        call copy(wrk,w,ntot)
+        call dpdz(temp,wrk)
+        call copy(wrk,temp,ntot)
+        goto 900
+      endif
+      if (mode.eq.3) then
+        call filter_3dlegenfourier(u,1,1,0)
+        call filter_3dlegenfourier(v,1,1,0)
+        call filter_3dlegenfourier(w,1,1,0)
+        call filter_3dlegenfourier(temp,1,1,0)
+        goto 900
+      endif
+      if (mode.eq.4) then
+        call interface_avg3d(u,icu)
+        call interface_avg3d(v,icv)
+        call interface_avg3d(w,icw)
+        call interface_avg3d(temp,ict)
+        write(12,'(4(1x,I10))') icu,icv,icw,ict
+        goto 900
+      endif
+
+c-mode 0: full run from a physical-space initial condition
+      call horfft(u,-1,nz)
+      call horfft(v,-1,nz)
+      call horfft(w,-1,nz)
+      call horfft(temp,-1,nz)
+
+      t = 0.
+      fac1 = 1.0
+      fac2 = 0.
+      fac3 = 0.
+      bdf1 = 1.
+      bdf2 = 0.
+      bdf3 = 0.
+      bdfv = 1.
+
+      do 1000 istep=1,nrun
+c-State-changing part of maxcalc (various.f:739-742, 854-857)
+        call horfft(u,1,nz)
+        call horfft(v,1,nz)
+        call horfft(w,1,nz)
+        call horfft(temp,1,nz)
+        if (iprobe.eq.1) write(12,'(3(1x,E24.16))')
+     >     t,u(ipx,1,kp1),u(ipx,1,kp2)
+        call horfft(u,-1,nz)
+        call horfft(v,-1,nz)
+        call horfft(w,-1,nz)
+        call horfft(temp,-1,nz)
+
+        call convec(istep,0,ispecfilternl)
+
+c-main.f:700-777 (istart=0, fixed dt)
+        if (istep.lt.2) then
+          fac1 = 2.
+          fac2 = -1.
+          fac3 = 0.
+          bdf1 = 2.
+          bdf2 = -0.5
+          bdf3 = 0.
+        else
+          fac1 = 3.
+          fac2 = -3.
+          fac3 = 1.
+          bdf1 = 3.
+          bdf2 = -1.5
+          bdf3 = 1./3.
+        endif
+
+        call copy(u,su,ntot)
+        call copy(v,sv,ntot)
+        call copy(w,sw,ntot)
+        call copy(temp,st,ntot)
+
+        cpenspecfilter = ((ispecfilterexp.eq.1).and.
+     >   (mod(istep,nspecfilterexp).eq.0))
+        call calcuvwt(izmode,ispecfilter,irelax,cpenspecfilter)
+
+c-main.f:815-856
+        if (istep.lt.2) then
+          bdfv = 1.5
+        else
+          bdfv = 11./6.
+        endif
+
+        t = t + dt
+
+        if (mod(istep,navgintrfc).eq.0) then
+          call interface_avg3d(u,icu)
+          call interface_avg3d(v,icv)
+          call interface_avg3d(w,icw)
+          call interface_avg3d(temp,ict)
+          write(12,'(I8,4(1x,I10))') istep,icu,icv,icw,ict
+        endif
+        call zero(su,ntot)
+ 1000 continue
+
+ 900  continue
+      close(12)
+      open(11,file=outfile,form='unformatted',access='stream',
+     >     status='replace')
+      write(11) u,v,w,temp
+      write(11) un,vn,wn,tn
+      write(11) unm,vnm,wnm,tnm
+      write(11) su,sv,sw,st
+      write(11) sun,svn,swn,stn
+      write(11) sunm,svnm,swnm,stnm
+      write(11) t
+      close(11)
+      end
+
+c-----------------------------------------------------------------------
+c  zero / copy: the two helpers of various.f:2-26 used on the hot path
+c-----------------------------------------------------------------------
+      subroutine zero(a,n)
+      dimension a(*)
+      do i=1,n
+        a(i) = 0.
+      enddo
+      return
+      end
+
+      subroutine copy(a,b,n)
+      dimension a(*), b(*)
+      do i=1,n
+        b(i) = a(i)
+      enddo
+      return
+      end
