@@ -1,0 +1,116 @@
+/*
+ * iwgpu.h -- C-ABI of the MI355X (gfx950) implementation of one time step of
+ * the stratified-Boussinesq solver of jmsegret/IWrepo (mode1_bbl): Fourier
+ * pseudo-spectral in x,y, Legendre Gauss-Lobatto multidomain penalty in z.
+ *
+ * Plain C types only.  Every array argument is a host array in the reference's
+ * own layout a(nxpp,ny,nz) (Fortran column order: x fastest), nxpp = nx+2,
+ * nz = nsubd*nzloc; in spectral space entries 2ii-1/2ii (Fortran, 1-based) are
+ * the real/imaginary parts of mode (kx=(ii-1)*2pi/xlen, ky=yw(j)).
+ *
+ * All entry points return 0 on success and a negative code on failure;
+ * iw_last_error() then describes the failure.  A solver handle is not
+ * thread-safe (the reference is single-threaded, comscratch.h).
+ */
+#ifndef IWGPU_H
+#define IWGPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define IW_MAXSUB 64
+
+/* Run parameters: dim.h:12-28 sizes plus the &inputs/&params/&mflags
+ * namelists of main.f:185-194 that the step reads. */
+typedef struct iw_params {
+    int nx, ny, nzloc, nsubd;      /* dim.h:12 (nzloc = GLL order + 1)            */
+    int bc_uv_bottom;              /* 0 free slip (calcuvwt.f:291-295),
+                                      1 no slip (calcuvwt_noslip.f:294-299)       */
+    int ispecfilternl;             /* main.f mflags: filter after NL step          */
+    int ispecfilterexp;            /* filter penalty solutions ...                 */
+    int nspecfilterexp;            /* ... every nspecfilterexp steps               */
+    int navgintrfc;                /* interface averaging period (main.f:920)      */
+    int pad0;
+    double xlen, ylen, zlen;
+    double xnu, xkappa, brunt, grav, rho0, omz;
+    double dt;
+    double amp, p, pf, fackc;      /* exponential filters (filtering.f:1225-1268)  */
+    double facdir, facneumn, facrobin; /* penalty factors (penalty.f:57-157)      */
+    double zcen, xbcrit;           /* mean-u damping (calcuvwt.f:343-361)          */
+    double xacrit, umax, fwave;    /* virtual paddle (forcing.f:325-447)           */
+    double z0[IW_MAXSUB], zh[IW_MAXSUB]; /* subdomains (subd.dat, setup.f:316-325) */
+} iw_params;
+
+typedef struct iw_solver iw_solver;
+
+int iw_params_size(void);
+const char *iw_last_error(void);
+
+/* Set-up of one solver instance on the current HIP device: GLL nodes,
+ * weights, Costa-Don matrices, wavenumbers and grid (replaces setup,
+ * setup.f:6-267, and fftfax/cftfax, main.f:210-218) and the device mirror
+ * of the COMMON blocks /flow/ and /matrix/ (comflow.h, commatrix.h). */
+int iw_create(const iw_params *p, iw_solver **out);
+int iw_destroy(iw_solver *s);
+
+/* Device mirror of the 24 state arrays: 0..3 u,v,w,temp; 4..7 un..tn;
+ * 8..11 unm..tnm; 12..15 su..st; 16..19 sun..stn; 20..23 sunm..stnm. */
+int iw_set_state(iw_solver *s, int idx, const double *host);
+int iw_get_state(iw_solver *s, int idx, double *host);
+/* /pressbc/ wnl, wviscrot (comrotpressbc.h), each (nxpp,ny,2) */
+int iw_get_pressbc(iw_solver *s, double *wnl, double *wviscrot);
+/* Step clock and the SS/BDF coefficients of /ab/, /bdf/ (comtimeadvance.h) */
+int iw_set_coeffs(iw_solver *s, double t, const double fac[3], const double bdf[3], double bdfv);
+int iw_get_coeffs(iw_solver *s, double *t, double fac[3], double bdf[3], double *bdfv);
+
+/* Start from rest-frame physical fields (istart=0): forward transform of the
+ * four physical fields and the start coefficients of main.f:543-552. */
+int iw_set_initial(iw_solver *s, const double *u, const double *v,
+                   const double *w, const double *temp);
+
+/* Device-resident step loop replaying main.f:653-1066 for nsteps steps with
+ * fixed dt: maxcalc round trip (various.f:739-742, 854-857), convec,
+ * SS/BDF coefficient selection, calcuvwt, interface_avg3d.  counts (may be
+ * NULL) receives 4 interface-average counts per step (icu,icv,icw,ict). */
+int iw_run(iw_solver *s, int nsteps, int *counts);
+double iw_time(iw_solver *s);
+int iw_sync(iw_solver *s);
+
+/* ---- reference routines on host arrays (parity / Fortran-shim mode) ---- */
+/* horfft.f:2    subroutine horfft(b,is,nk): is=-1 physical->spectral */
+int iw_horfft(iw_solver *s, double *b, int is, int nk);
+/* derv.f:212    subroutine dpdz(a,b) */
+int iw_dpdz(iw_solver *s, const double *a, double *b);
+/* filtering.f:811  subroutine filter_3dlegenfourier(fin3d,ifourier,ilegen,iout) */
+int iw_filter_3dlegenfourier(iw_solver *s, double *f, int ifourier, int ilegen);
+/* penalty.f:714  subroutine interface_avg3d(fin,ic) */
+int iw_interface_avg3d(iw_solver *s, double *f, int *ic);
+/* solver.f:6    subroutine convec(istep,istart,ispecfilternl), on the device
+ * mirror of /flow/,/matrix/,/pressbc/ (upload with iw_set_state first) */
+int iw_convec(iw_solver *s, int istep, int ispecfilternl);
+/* calcuvwt.f:2  subroutine calcuvwt(izmode,ispecfilter,irelax,cpenspecfilter) */
+int iw_calcuvwt(iw_solver *s, int cpenspecfilter);
+
+/* multistep.f:7-89 variable-step AB3/BDF3 coefficient functions */
+double iw_abvar1(double dt1, double dt2, double dt3);
+double iw_abvar2(double dt1, double dt2, double dt3);
+double iw_abvar3(double dt1, double dt2, double dt3);
+double iw_bdfvarv(double dt1, double dt2, double dt3);
+double iw_bdfvar1(double dt1, double dt2, double dt3);
+double iw_bdfvar2(double dt1, double dt2, double dt3);
+double iw_bdfvar3(double dt1, double dt2, double dt3);
+
+/* ---- measurement ---- */
+/* Average device time (ms) per launch of each kernel class recorded during
+ * the last iw_run with profiling enabled; names are written as a
+ * ';'-separated list into names (capacity cap bytes). */
+int iw_set_profiling(iw_solver *s, int on);
+int iw_kernel_stats(iw_solver *s, char *names, int cap, double *avg_ms, int *launches, int maxk);
+/* Bytes of device memory held by the instance. */
+long long iw_device_bytes(iw_solver *s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1009 @@
+// Host side of libiwgpu: set-up tables, factor cache, the device-resident step
+// driver and the C-ABI of include/iwgpu.h.
+//
+// Reference: jmsegret/IWrepo, mode1_bbl.  The step order replays
+// main.f:653-1066 for istart=0 and fixed dt.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "iw_internal.h"
+
+using namespace iw;
+
+static thread_local std::string g_err;
+
+static int fail(const std::string &m)
+{
+    g_err = m;
+    return -1;
+}
+
+#define HIPCHK(x)                                                                          \
+    do {                                                                                   \
+        hipError_t e_ = (x);                                                               \
+        if (e_ != hipSuccess)                                                              \
+            return fail(std::string(#x) + ": " + hipGetErrorString(e_));                   \
+    } while (0)
+
+extern "C" int iw_params_size(void) { return (int)sizeof(iw_params); }
+extern "C" const char *iw_last_error(void) { return g_err.c_str(); }
+
+// ---------------------------------------------------------------------------
+// multistep.f:7-89 variable-step coefficients
+// ---------------------------------------------------------------------------
+extern "C" double iw_abvar1(double dt1, double dt2, double dt3)
+{
+    double dt123 = 2. * dt1 + 6. * dt2 + 3. * dt3, dt23 = dt2 + dt3;
+    return 1. + dt1 * dt123 / (6. * dt2 * dt23);
+}
+extern "C" double iw_abvar2(double dt1, double dt2, double dt3)
+{
+    double dt123 = 2. * dt1 + 3. * dt2 + 3. * dt3;
+    return -dt1 * dt123 / (6. * dt2 * dt3);
+}
+extern "C" double iw_abvar3(double dt1, double dt2, double dt3)
+{
+    double dt12 = 2. * dt1 + 3. * dt2, dt23 = dt2 + dt3;
+    return dt1 * dt12 / (6. * dt3 * dt23);
+}
+extern "C" double iw_bdfvarv(double dt1, double dt2, double dt3)
+{
+    return 1. + dt1 / (dt1 + dt2) + dt1 / (dt1 + dt2 + dt3);
+}
+extern "C" double iw_bdfvar1(double dt1, double dt2, double dt3)
+{
+    double dt12 = dt1 + dt2, dt23 = dt2 + dt3, dt123 = dt1 + dt2 + dt3;
+    return -(-1. - dt1 / dt2 - dt1 / dt123 - dt1 * dt1 / (dt123 * dt2)
+             - dt1 * dt1 * dt12 / (dt123 * dt23 * dt2));
+}
+extern "C" double iw_bdfvar2(double dt1, double dt2, double dt3)
+{
+    double dt12 = dt1 + dt2, dt23 = dt2 + dt3, dt123 = dt1 + dt2 + dt3;
+    return -(dt1 * dt1 / (dt12 * dt2) + dt1 * dt1 / (dt2 * dt123)
+             + dt1 * dt1 * dt12 / (dt2 * dt23 * dt123) + dt1 * dt1 * dt12 / (dt3 * dt23 * dt123));
+}
+extern "C" double iw_bdfvar3(double dt1, double dt2, double dt3)
+{
+    double dt12 = dt1 + dt2, dt23 = dt2 + dt3, dt123 = dt1 + dt2 + dt3;
+    return dt1 * dt1 * dt12 / (dt3 * dt23 * dt123);
+}
+
+// ---------------------------------------------------------------------------
+// GLL tables (setup.f:37-42): Gauss-Lobatto points by the Jacobi-polynomial
+// Newton iteration with deflation of jacobl (setup.f:395-479), weights and
+// Legendre table of quad (setup.f:527-600), Costa-Don D and D2 of derv
+// (setup.f:636-798).
+// ---------------------------------------------------------------------------
+struct Jac {
+    double poly, pder, polym1, pderm1, polym2, pderm2;
+};
+
+// Jacobi recurrence for alpha = beta = 0 (jacobf, setup.f:486-518)
+static Jac jacobf(int n, double x)
+{
+    const double alp = 0., bet = 0., apb = 0., rv = 1.;
+    Jac r = {1.0, 0.0, 0., 0., 0., 0.};
+    if (n == 0) return r;
+    double plst = r.poly, pdlst = r.pder, psave = 0., pdsave = 0.;
+    double p = rv * x, pd = rv;
+    if (n == 1) {
+        r.poly = p;
+        r.pder = pd;
+        return r;
+    }
+    for (int k = 2; k <= n; k++) {
+        double a1 = 2. * k * (k + apb) * (2. * k + apb - 2.);
+        double a2 = (2. * k + apb - 1.) * (alp * alp - bet * bet);
+        double b3 = (2. * k + apb - 2.);
+        double a3 = b3 * (b3 + 1.) * (b3 + 2.);
+        double a4 = 2. * (k + alp - 1.) * (k + bet - 1.) * (2. * k + apb);
+        double pn = ((a2 + a3 * x) * p - a4 * plst) / a1;
+        double pdn = ((a2 + a3 * x) * pd - a4 * pdlst + a3 * p) / a1;
+        psave = plst;
+        pdsave = pdlst;
+        plst = p;
+        p = pn;
+        pdlst = pd;
+        pd = pdn;
+    }
+    r.poly = p; r.pder = pd;
+    r.polym1 = plst; r.pderm1 = pdlst;
+    r.polym2 = psave; r.pderm2 = pdsave;
+    return r;
+}
+
+// Gauss-Lobatto points from the largest down by Newton with deflation,
+// tolerance 1e-12 and at most 10 steps (jacobl, setup.f:395-479)
+static std::vector<double> gll_points(int n)
+{
+    const int np = n + 1;
+    std::vector<double> xj(np + 2, 0.0), out(n + 1);
+    Jac jp = jacobf(np, 1.0), jm = jacobf(np, -1.0);
+    double det = jp.polym1 * jm.polym2 - jm.polym1 * jp.polym2;
+    double rp = -jp.poly, rm = -jm.poly;
+    double a = (rp * jm.polym2 - rm * jp.polym2) / det;
+    double b = (rm * jp.polym1 - rp * jm.polym1) / det;
+    xj[1] = 1.0;
+    int nh = (n + 1) / 2;
+    double dth = 3.14159265 / (2 * n + 1); // the reference's truncated pi
+    double cd = cos(2. * dth), sd = sin(2. * dth), cs = cos(dth), ss = sin(dth);
+    for (int j = 2; j <= nh; j++) {
+        double x = cs;
+        for (int k = 1; k <= 10; k++) {
+            Jac q = jacobf(np, x);
+            double poly = q.poly + a * q.polym1 + b * q.polym2;
+            double pder = q.pder + a * q.pderm1 + b * q.pderm2;
+            double recsum = 0.0;
+            for (int i = 1; i <= j - 1; i++) recsum += 1.0 / (x - xj[i]);
+            double delx = -poly / (pder - recsum * poly);
+            x = x + delx;
+            if (fabs(delx) < 1.0e-12) break;
+        }
+        xj[j] = x;
+        double cssave = cs * cd - ss * sd;
+        ss = cs * sd + ss * cd;
+        cs = cssave;
+    }
+    xj[np] = -1.0;
+    for (int i = 2; i <= nh; i++) xj[n + 2 - i] = -xj[i];
+    if (n % 2 == 0) xj[nh + 1] = 0.0;
+    for (int k = 0; k <= n; k++) out[k] = xj[n - k + 1];
+    return out;
+}
+
+static void setup_tables(iw_solver *s)
+{
+    const int nl = s->P.nzloc, N = nl - 1;
+    s->zpts = gll_points(N);
+    s->wg.assign(nl, 0.);
+    s->alpol.assign((size_t)nl * nl, 0.);
+    std::vector<double> al(nl + 2), alp(nl + 2);
+    for (int k = 0; k <= N; k++) {
+        double xc = s->zpts[k];
+        al[0] = 1.;
+        al[1] = xc;
+        alp[0] = 0.;
+        alp[1] = 1.;
+        for (int m = 1; m <= N; m++) {
+            al[m + 1] = (2. * m + 1.) * xc * al[m] / (m + 1) - m * al[m - 1] / (m + 1);
+            alp[m + 1] = (2. * m + 1.) * (al[m] + xc * alp[m]) / (m + 1) - m * alp[m - 1] / (m + 1);
+        }
+        s->wg[k] = 2. / (N * (N + 1) * al[N] * al[N] + 1.0e-30);
+        for (int j = 0; j <= N; j++) s->alpol[j * nl + k] = al[j];
+    }
+    std::vector<double> c(nl);
+    for (int k = 0; k <= N; k++) {
+        double prod = 1.;
+        for (int l = 0; l <= N; l++)
+            if (l != k) prod = prod * (s->zpts[k] - s->zpts[l]);
+        c[k] = prod;
+    }
+    s->d.assign((size_t)nl * nl, 0.);
+    s->d2.assign((size_t)nl * nl, 0.);
+    for (int k = 0; k <= N; k++)
+        for (int j = 0; j <= N; j++)
+            if (k != j) s->d[k * nl + j] = c[k] / (c[j] * (s->zpts[k] - s->zpts[j]));
+    for (int k = 0; k <= N; k++) {
+        double sum = 0.;
+        for (int j = 0; j <= N; j++)
+            if (k != j) sum += s->d[k * nl + j];
+        s->d[k * nl + k] = -sum;
+    }
+    for (int k = 0; k <= N; k++)
+        for (int j = 0; j <= N; j++)
+            if (k != j)
+                s->d2[k * nl + j] = 2.0 * (s->d[k * nl + k] * s->d[k * nl + j]
+                                           - s->d[k * nl + j] / (s->zpts[k] - s->zpts[j]));
+    for (int k = 0; k <= N; k++) {
+        double sum = 0.;
+        for (int j = 0; j <= N; j++)
+            if (k != j) sum += s->d2[k * nl + j];
+        s->d2[k * nl + k] = -sum;
+    }
+    // wavenumbers (setup.f:77-89) and grid (setup.f:126-160)
+    const double pi2 = 8. * atan(1.0);
+    const double alpha = pi2 / s->P.xlen, beta = pi2 / s->P.ylen;
+    const int nxhp = s->P.nx / 2 + 1, ny = s->P.ny, nyh = ny / 2;
+    s->xw.assign(nxhp, 0.);
+    s->xsq.assign(nxhp, 0.);
+    s->yw.assign(ny, 0.);
+    s->ysq.assign(ny, 0.);
+    for (int i = 0; i < nxhp; i++) {
+        s->xw[i] = i * alpha;
+        s->xsq[i] = s->xw[i] * s->xw[i];
+    }
+    for (int j = 1; j <= nyh; j++) {
+        s->yw[j - 1] = (j - 1) * beta;
+        s->yw[ny - j] = -j * beta;
+    }
+    for (int j = 0; j < ny; j++) s->ysq[j] = s->yw[j] * s->yw[j];
+    s->z.assign((size_t)nl * s->P.nsubd, 0.);
+    for (int ks = 0; ks < s->P.nsubd; ks++)
+        for (int k = 0; k < nl; k++) s->z[ks * nl + k] = s->P.z0[ks] + 0.5 * s->P.zh[ks] * (s->zpts[k] + 1.0);
+}
+
+// twiddle e^{-2 pi i k/n} in extended precision, rounded once
+static double2 twiddle(long k, long n)
+{
+    long double a = -2.0L * 3.141592653589793238462643383279502884L * (long double)k / (long double)n;
+    long double cr = cosl(a), sr = sinl(a);
+    if (4 * k == n) { cr = 0; sr = -1; }
+    if (2 * k == n) { cr = -1; sr = 0; }
+    if (4 * k == 3 * n) { cr = 0; sr = 1; }
+    return make_double2((double)cr, (double)sr);
+}
+
+template <typename T>
+static int upload(T **dst, const std::vector<T> &src, long long &acc)
+{
+    size_t b = sizeof(T) * (src.size() ? src.size() : 1);
+    HIPCHK(hipMalloc((void **)dst, b));
+    acc += (long long)b;
+    if (!src.empty()) HIPCHK(hipMemcpy(*dst, src.data(), sizeof(T) * src.size(), hipMemcpyHostToDevice));
+    return 0;
+}
+
+static int ilog2(int v)
+{
+    int l = 0;
+    while ((1 << l) < v) l++;
+    return (1 << l) == v ? l : -1;
+}
+
+// ---------------------------------------------------------------------------
+// penalty parameters (penalty.f:9-157) and per-subdomain assembly
+// coefficients of matrix_elmsetup (penalty.f:403-634)
+// ---------------------------------------------------------------------------
+enum { BC_DIR = 0, BC_NEU = 1, BC_ROB = 2 };
+
+struct Pen {
+    std::vector<double> alpha, beta, gamma, delta, tau1, tau2;
+};
+
+static void param_set(const iw_params &P, double omega, int ifbc, double eps, double ybar, double &a0,
+                      double &b0, double &t0)
+{
+    if (ifbc == BC_DIR) {
+        a0 = 1.;
+        b0 = 0.;
+        double fac = P.facdir * ((2. / ybar) * (2. / ybar));
+        t0 = fac * eps / (a0 * (omega * omega));
+    } else if (ifbc == BC_NEU) {
+        a0 = 0.;
+        b0 = 1.;
+        double fac = P.facneumn * 2. / ybar;
+        t0 = fac / (b0 * omega);
+    } else {
+        a0 = 1.;
+        b0 = 1.;
+        double ck = a0 * omega / b0;
+        double dn = 1. / (omega * eps * b0);
+        double t1 = sqrt(ck * ck + eps * ck);
+        double t2 = 2. * ck - 2. * t1;
+        double facm = ((1. / eps) >= 1.e5) ? P.facrobin : 1.;
+        t0 = facm * dn * (eps + t2) * 2. / ybar;
+    }
+}
+
+static Pen penalty_params(const iw_params &P, double eps, int bcbot, int bctop)
+{
+    const int ns = P.nsubd;
+    const double omega = 2. / ((P.nzloc + 1) * (P.nzloc + 2));
+    Pen p;
+    p.alpha.assign(ns, 0.);
+    p.beta = p.gamma = p.delta = p.tau1 = p.tau2 = p.alpha;
+    double a0, b0, t0;
+    for (int ks = 1; ks < ns - 1; ks++) {
+        param_set(P, omega, BC_ROB, eps, P.zh[ks], a0, b0, t0);
+        p.alpha[ks] = a0; p.beta[ks] = b0; p.tau1[ks] = t0;
+        p.gamma[ks] = a0; p.delta[ks] = b0; p.tau2[ks] = t0;
+    }
+    param_set(P, omega, bcbot, eps, P.zh[0], a0, b0, t0);
+    p.alpha[0] = a0; p.beta[0] = b0; p.tau1[0] = t0;
+    if (ns > 1) {
+        param_set(P, omega, BC_ROB, eps, P.zh[0], a0, b0, t0);
+        p.gamma[0] = a0; p.delta[0] = b0; p.tau2[0] = t0;
+    }
+    param_set(P, omega, bctop, eps, P.zh[ns - 1], a0, b0, t0);
+    p.gamma[ns - 1] = a0; p.delta[ns - 1] = b0; p.tau2[ns - 1] = t0;
+    if (ns > 1) {
+        param_set(P, omega, BC_ROB, eps, P.zh[ns - 1], a0, b0, t0);
+        p.alpha[ns - 1] = a0; p.beta[ns - 1] = b0; p.tau1[ns - 1] = t0;
+    }
+    return p;
+}
+
+static void assembly_coef(const iw_params &P, const Pen &p, double eps, double *q /*12*ns*/)
+{
+    const int ns = P.nsubd;
+    for (int ks = 0; ks < ns; ks++) {
+        double ybar = P.zh[ks];
+        double ybarl = ks > 0 ? P.zh[ks - 1] : 1., ybarr = ks < ns - 1 ? P.zh[ks + 1] : 1.;
+        double *c = q + 12 * ks;
+        double c2 = (2. / ybar) * (2. / ybar);
+        c[0] = eps * c2;
+        c[1] = p.tau1[ks] * p.alpha[ks];
+        c[2] = -p.tau1[ks] * p.beta[ks] * eps * 2. / ybar;
+        c[3] = p.tau2[ks] * p.gamma[ks];
+        c[4] = +p.tau2[ks] * p.delta[ks] * eps * 2. / ybar;
+        c[5] = 0.5 * p.tau1[ks] * p.alpha[ks];
+        c[6] = +0.5 * p.tau1[ks] * p.beta[ks] * eps * 2. / ybar;
+        c[7] = 0.5 * p.tau2[ks] * p.gamma[ks];
+        c[8] = -0.5 * p.tau2[ks] * p.delta[ks] * eps * 2. / ybar;
+        c[9] = +0.5 * p.tau1[ks] * p.beta[ks] * eps * 2. / ybarl;
+        c[10] = -0.5 * p.tau2[ks] * p.delta[ks] * eps * 2. / ybarr;
+        c[11] = 0.;
+    }
+}
+
+// Build the system table for the current (bdfv, dt): every mode needs a
+// pressure system (eps = 1/k^2, Neumann/Neumann, calcuvwt.f:114-152), a u/v
+// system (eps = 1/(k^2 + bdfv/(nu dt)), calcuvwt.f:276-295), a w system
+// (Dirichlet/Dirichlet, calcuvwt.f:516-532) and a density system
+// (eps = 1/(k^2 + bdfv/(kappa dt)), calcuvwt.f:634-653).  The matrices depend
+// only on (eps, bottom BC, top BC), so identical triples (same k^2 bits)
+// share one factor.  Returns the first system index that needs factoring.
+static int build_systems(iw_solver *s, bool refactor_p, int &first, int &count)
+{
+    const iw_params &P = s->P;
+    const int nxhp = s->g.nxhp, ny = s->g.ny, nmode = nxhp * ny, ns = P.nsubd;
+    const double facvel = 1. / (P.xnu * P.dt), fact = 1. / (P.xkappa * P.dt);
+    const int ubot = P.bc_uv_bottom == 1 ? BC_DIR : BC_NEU;
+    typedef std::tuple<unsigned long long, int, int> Key;
+    std::map<Key, int> idx;
+    std::vector<double> eps_of;
+    std::vector<int> bb, bt;
+    std::vector<int> modesys(4 * (size_t)nmode, -1);
+    auto key_of = [](double e, int b, int t) {
+        unsigned long long u;
+        memcpy(&u, &e, 8);
+        return Key(u, b, t);
+    };
+    auto get = [&](double e, int b, int t) {
+        Key k = key_of(e, b, t);
+        auto it = idx.find(k);
+        if (it != idx.end()) return it->second;
+        int id = (int)eps_of.size();
+        idx[k] = id;
+        eps_of.push_back(e);
+        bb.push_back(b);
+        bt.push_back(t);
+        return id;
+    };
+    // pressure systems first so that they keep their slots across refactors
+    for (int m = 0; m < nmode; m++) {
+        int ii = m % nxhp, j = m / nxhp;
+        if (m == 0) continue;
+        double ak = s->xsq[ii] + s->ysq[j];
+        modesys[4 * m] = get(1. / ak, BC_NEU, BC_NEU);
+    }
+    int np = (int)eps_of.size();
+    for (int m = 0; m < nmode; m++) {
+        int ii = m % nxhp, j = m / nxhp;
+        double ak = s->xsq[ii] + s->ysq[j];
+        double ev = 1. / (ak + s->bdfv * facvel);
+        double et = 1. / (ak + s->bdfv * fact);
+        modesys[4 * m + 1] = get(ev, ubot, BC_NEU);
+        if (m != 0) modesys[4 * m + 2] = get(ev, BC_DIR, BC_DIR);
+        modesys[4 * m + 3] = get(et, BC_NEU, BC_NEU);
+    }
+    const int nsys = (int)eps_of.size();
+    SysTables &S = s->sys;
+    if ((size_t)nsys > S.cap_sys) {
+        if (S.AB) hipFree(S.AB);
+        if (S.ipiv) hipFree(S.ipiv);
+        if (S.coef) hipFree(S.coef);
+        if (S.sysinfo) hipFree(S.sysinfo);
+        size_t cap = (size_t)nsys;
+        HIPCHK(hipMalloc((void **)&S.AB, sizeof(double) * cap * S.ldab * s->g.nz));
+        HIPCHK(hipMalloc((void **)&S.ipiv, sizeof(int) * cap * s->g.nz));
+        HIPCHK(hipMalloc((void **)&S.coef, sizeof(double) * cap * 12 * ns));
+        HIPCHK(hipMalloc((void **)&S.sysinfo, sizeof(double) * cap * 4));
+        s->dev_bytes += (long long)(sizeof(double) * cap * S.ldab * s->g.nz + sizeof(int) * cap * s->g.nz
+                                    + sizeof(double) * cap * 12 * ns + sizeof(double) * cap * 4);
+        S.cap_sys = cap;
+        refactor_p = true;
+    }
+    std::vector<double> coef((size_t)nsys * 12 * ns), info((size_t)nsys * 4);
+    for (int q = 0; q < nsys; q++) {
+        Pen p = penalty_params(P, eps_of[q], bb[q], bt[q]);
+        assembly_coef(P, p, eps_of[q], &coef[(size_t)q * 12 * ns]);
+        info[4 * q] = eps_of[q];
+        info[4 * q + 1] = p.tau1[0];
+        info[4 * q + 2] = p.tau2[ns - 1];
+        info[4 * q + 3] = bb[q] * 4 + bt[q];
+    }
+    HIPCHK(hipMemcpy(S.coef, coef.data(), sizeof(double) * coef.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(S.sysinfo, info.data(), sizeof(double) * info.size(), hipMemcpyHostToDevice));
+    if (!S.modesys) {
+        HIPCHK(hipMalloc((void **)&S.modesys, sizeof(int) * 4 * (size_t)nmode));
+        s->dev_bytes += (long long)(sizeof(int) * 4 * (size_t)nmode);
+    }
+    HIPCHK(hipMemcpy(S.modesys, modesys.data(), sizeof(int) * modesys.size(), hipMemcpyHostToDevice));
+    S.nsys = nsys;
+    first = refactor_p ? 0 : np;
+    count = nsys - first;
+    return 0;
+}
+
+static int ensure_factors(iw_solver *s)
+{
+    if (s->factored_bdfv == s->bdfv && s->factored_dt == s->P.dt && s->p_factored) return 0;
+    int first, count;
+    if (build_systems(s, !s->p_factored, first, count)) return -1;
+    launch_factor(s->g, s->sys, first, count, s->stream);
+    HIPCHK(hipGetLastError());
+    s->factored_bdfv = s->bdfv;
+    s->factored_dt = s->P.dt;
+    s->p_factored = true;
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// profiling: event pairs per kernel class, resolved after the run
+// ---------------------------------------------------------------------------
+struct ProfRec {
+    std::string name;
+    hipEvent_t a, b;
+};
+static thread_local std::vector<ProfRec> g_prec;
+static thread_local std::vector<hipEvent_t> g_pool;
+static thread_local size_t g_pool_next = 0;
+
+static hipEvent_t pool_event()
+{
+    if (g_pool_next == g_pool.size()) {
+        hipEvent_t e;
+        hipEventCreate(&e);
+        g_pool.push_back(e);
+    }
+    return g_pool[g_pool_next++];
+}
+
+struct Scope {
+    iw_solver *s;
+    const char *name;
+    hipEvent_t a;
+    Scope(iw_solver *s_, const char *n) : s(s_), name(n), a(nullptr)
+    {
+        if (s->profiling) {
+            a = pool_event();
+            hipEventRecord(a, s->stream);
+        }
+    }
+    ~Scope()
+    {
+        if (s->profiling) {
+            hipEvent_t b = pool_event();
+            hipEventRecord(b, s->stream);
+            g_prec.push_back({name, a, b});
+        }
+    }
+};
+
+static void prof_resolve(iw_solver *s)
+{
+    if (g_prec.empty()) return;
+    hipStreamSynchronize(s->stream);
+    for (auto &r : g_prec) {
+        float ms = 0;
+        hipEventElapsedTime(&ms, r.a, r.b);
+        size_t k = 0;
+        for (; k < s->prof.names.size(); k++)
+            if (s->prof.names[k] == r.name) break;
+        if (k == s->prof.names.size()) {
+            s->prof.names.push_back(r.name);
+            s->prof.ms.push_back(0.);
+            s->prof.n.push_back(0);
+        }
+        s->prof.ms[k] += ms;
+        s->prof.n[k] += 1;
+    }
+    g_prec.clear();
+    g_pool_next = 0;
+}
+
+// ---------------------------------------------------------------------------
+// creation
+// ---------------------------------------------------------------------------
+extern "C" int iw_create(const iw_params *p, iw_solver **out)
+{
+    if (!p || !out) return fail("null argument");
+    const iw_params &P = *p;
+    if (P.nsubd < 1 || P.nsubd > IW_MAXSUB) return fail("nsubd out of range");
+    if (P.nzloc < 3 || P.nzloc > kMaxNl - 1) return fail("nzloc must lie in [3, 31]");
+    int lx = ilog2(P.nx), ly = ilog2(P.ny);
+    if (lx < 2 || P.nx > 1024) return fail("nx must be a power of two in [4, 1024]");
+    if (ly < 0 || P.ny > 1024) return fail("ny must be 1 or a power of two <= 1024");
+    if (P.dt <= 0) return fail("dt must be positive");
+    iw_solver *s = new iw_solver();
+    s->P = P;
+    Geo &g = s->g;
+    g.nx = P.nx;
+    g.ny = P.ny;
+    g.nzloc = P.nzloc;
+    g.nsubd = P.nsubd;
+    g.nz = P.nzloc * P.nsubd;
+    g.nxpp = P.nx + 2;
+    g.nxh = P.nx / 2;
+    g.nxhp = g.nxh + 1;
+    g.nyh = P.ny / 2;
+    g.ndims = P.ny > 1 ? 3 : 2;
+    g.nxppy = (long)g.nxpp * g.ny;
+    g.ntot = g.nxppy * g.nz;
+    g.M = P.nx / 2;
+    g.logM = lx - 1;
+    g.logNy = ly;
+    g.scaley = 1. / (double)P.ny;
+    HIPCHK(hipGetDevice(&s->device));
+    HIPCHK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    setup_tables(s);
+    long long &acc = s->dev_bytes;
+    // filter tables: Legendre sigma (filtering.f:1225-1243), Fourier factor
+    // per mode (filtering.f:849, 868-901), mean-u damping (calcuvwt.f:343-361)
+    const int nzm = P.nzloc - 1;
+    const int kc = (int)(P.fackc * nzm);
+    std::vector<double> sig(P.nzloc);
+    for (int k = 0; k <= nzm; k++)
+        sig[k] = (k >= kc) ? exp(-P.amp * pow((double)(k - kc) / (double)(nzm - kc), 2. * P.p)) : 1.;
+    std::vector<double> fcf((size_t)g.nxhp * g.ny);
+    double akmax = sqrt(s->xsq[g.nxhp - 1] + s->ysq[g.nyh]);
+    for (int j = 0; j < g.ny; j++)
+        for (int ii = 0; ii < g.nxhp; ii++) {
+            double ak = sqrt(s->xsq[ii] + s->ysq[j]);
+            double theta = (ak - akmax * 0.) / (akmax - akmax * 0.);
+            fcf[(size_t)j * g.nxhp + ii] = exp(-P.amp * pow(theta, 2. * P.pf));
+        }
+    std::vector<double> damp(g.nz);
+    for (int k = 0; k < g.nz; k++) {
+        double q = (s->z[k] - P.zcen) / P.xlen;
+        damp[k] = 1.0 - exp(-P.xbcrit * (q * q));
+    }
+    std::vector<double> zh(P.zh, P.zh + P.nsubd);
+    std::vector<double> xg(P.nx);
+    const double pi2 = 8. * atan(1.0);
+    for (int i = 0; i < P.nx; i++) xg[i] = i * (pi2 / (pi2 / P.xlen)) / P.nx;
+    std::vector<double2> twM(g.M / 2 > 0 ? g.M / 2 : 1), twN(g.M + 1), twY(g.ny / 2 > 0 ? g.ny / 2 : 1);
+    for (int t = 0; t < g.M / 2; t++) twM[t] = twiddle(t, g.M);
+    for (int k = 0; k <= g.M; k++) twN[k] = twiddle(k, 2 * g.M);
+    for (int t = 0; t < g.ny / 2; t++) twY[t] = twiddle(t, g.ny);
+    if (upload(&g.D, s->d, acc) || upload(&g.D2, s->d2, acc) || upload(&g.AL, s->alpol, acc)
+        || upload(&g.wg, s->wg, acc) || upload(&g.sig, sig, acc) || upload(&g.xw, s->xw, acc)
+        || upload(&g.yw, s->yw, acc) || upload(&g.zh, zh, acc) || upload(&g.z, s->z, acc)
+        || upload(&g.damp, damp, acc) || upload(&g.fcf, fcf, acc) || upload(&g.x, xg, acc)
+        || upload(&g.twM, twM, acc) || upload(&g.twN, twN, acc) || upload(&g.twY, twY, acc)) {
+        delete s;
+        return -1;
+    }
+    size_t fb = sizeof(double) * (size_t)g.ntot;
+    for (int q = 0; q < 24; q++) {
+        HIPCHK(hipMalloc((void **)&s->buf[q], fb));
+        HIPCHK(hipMemset(s->buf[q], 0, fb));
+    }
+    for (int q = 0; q < 9; q++) HIPCHK(hipMalloc((void **)&s->grad[q], fb));
+    for (int q = 0; q < 7; q++) HIPCHK(hipMalloc((void **)&s->prod[q], fb));
+    HIPCHK(hipMalloc((void **)&s->ox, fb));
+    HIPCHK(hipMalloc((void **)&s->hbuf, fb));
+    HIPCHK(hipMalloc((void **)&s->wnl, 2 * sizeof(double) * g.nxppy));
+    HIPCHK(hipMalloc((void **)&s->wviscrot, 2 * sizeof(double) * g.nxppy));
+    HIPCHK(hipMemset(s->wnl, 0, 2 * sizeof(double) * g.nxppy));
+    HIPCHK(hipMemset(s->wviscrot, 0, 2 * sizeof(double) * g.nxppy));
+    HIPCHK(hipMalloc((void **)&s->d_count, 4 * sizeof(int)));
+    HIPCHK(hipHostMalloc((void **)&s->h_count, 4 * sizeof(int)));
+    acc += (long long)fb * (24 + 9 + 7 + 2) + 4 * sizeof(double) * g.nxppy;
+    s->sys.kl = P.nzloc;
+    s->sys.kv = 2 * P.nzloc;
+    s->sys.ldab = 3 * P.nzloc + 1;
+    HIPCHK(hipEventCreate(&s->ev0));
+    HIPCHK(hipEventCreate(&s->ev1));
+    s->t = 0.;
+    s->fac[0] = 1.; s->fac[1] = 0.; s->fac[2] = 0.;
+    s->bdf[0] = 1.; s->bdf[1] = 0.; s->bdf[2] = 0.;
+    s->bdfv = 1.;
+    s->istep = 0;
+    *out = s;
+    return 0;
+}
+
+extern "C" int iw_destroy(iw_solver *s)
+{
+    if (!s) return 0;
+    hipStreamSynchronize(s->stream);
+    Geo &g = s->g;
+    double *tabs[] = {g.D, g.D2, g.AL, g.wg, g.sig, g.xw, g.yw, g.zh, g.z, g.damp, g.fcf, g.x};
+    for (double *t : tabs) hipFree(t);
+    hipFree(g.twM);
+    hipFree(g.twN);
+    hipFree(g.twY);
+    for (int q = 0; q < 24; q++) hipFree(s->buf[q]);
+    for (int q = 0; q < 9; q++) hipFree(s->grad[q]);
+    for (int q = 0; q < 7; q++) hipFree(s->prod[q]);
+    hipFree(s->ox);
+    hipFree(s->hbuf);
+    hipFree(s->wnl);
+    hipFree(s->wviscrot);
+    hipFree(s->d_count);
+    hipHostFree(s->h_count);
+    hipFree(s->sys.AB);
+    hipFree(s->sys.ipiv);
+    hipFree(s->sys.coef);
+    hipFree(s->sys.sysinfo);
+    hipFree(s->sys.modesys);
+    hipEventDestroy(s->ev0);
+    hipEventDestroy(s->ev1);
+    hipStreamDestroy(s->stream);
+    delete s;
+    return 0;
+}
+
+extern "C" long long iw_device_bytes(iw_solver *s) { return s ? s->dev_bytes : 0; }
+
+// ---------------------------------------------------------------------------
+// state transfer
+// ---------------------------------------------------------------------------
+extern "C" int iw_set_state(iw_solver *s, int idx, const double *host)
+{
+    if (!s || idx < 0 || idx >= 24 || !host) return fail("iw_set_state: bad argument");
+    HIPCHK(hipMemcpyAsync(s->buf[idx], host, sizeof(double) * s->g.ntot, hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return 0;
+}
+
+extern "C" int iw_get_state(iw_solver *s, int idx, double *host)
+{
+    if (!s || idx < 0 || idx >= 24 || !host) return fail("iw_get_state: bad argument");
+    HIPCHK(hipMemcpyAsync(host, s->buf[idx], sizeof(double) * s->g.ntot, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return 0;
+}
+
+extern "C" int iw_get_pressbc(iw_solver *s, double *wnl, double *wviscrot)
+{
+    if (!s) return fail("null solver");
+    size_t b = 2 * sizeof(double) * s->g.nxppy;
+    if (wnl) HIPCHK(hipMemcpyAsync(wnl, s->wnl, b, hipMemcpyDeviceToHost, s->stream));
+    if (wviscrot) HIPCHK(hipMemcpyAsync(wviscrot, s->wviscrot, b, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return 0;
+}
+
+extern "C" int iw_set_coeffs(iw_solver *s, double t, const double fac[3], const double bdf[3], double bdfv)
+{
+    if (!s) return fail("null solver");
+    s->t = t;
+    for (int q = 0; q < 3; q++) {
+        s->fac[q] = fac[q];
+        s->bdf[q] = bdf[q];
+    }
+    s->bdfv = bdfv;
+    return 0;
+}
+
+extern "C" int iw_get_coeffs(iw_solver *s, double *t, double fac[3], double bdf[3], double *bdfv)
+{
+    if (!s) return fail("null solver");
+    if (t) *t = s->t;
+    for (int q = 0; q < 3; q++) {
+        if (fac) fac[q] = s->fac[q];
+        if (bdf) bdf[q] = s->bdf[q];
+    }
+    if (bdfv) *bdfv = s->bdfv;
+    return 0;
+}
+
+extern "C" double iw_time(iw_solver *s) { return s ? s->t : 0.; }
+
+extern "C" int iw_sync(iw_solver *s)
+{
+    if (!s) return fail("null solver");
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return 0;
+}
+
+extern "C" int iw_set_profiling(iw_solver *s, int on)
+{
+    if (!s) return fail("null solver");
+    s->profiling = on != 0;
+    s->prof = Prof();
+    return 0;
+}
+
+extern "C" int iw_kernel_stats(iw_solver *s, char *names, int cap, double *avg_ms, int *launches, int maxk)
+{
+    if (!s) return fail("null solver");
+    prof_resolve(s);
+    std::string all;
+    int k = 0;
+    for (; k < (int)s->prof.names.size() && k < maxk; k++) {
+        if (k) all += ";";
+        all += s->prof.names[k];
+        if (avg_ms) avg_ms[k] = s->prof.n[k] ? s->prof.ms[k] / s->prof.n[k] : 0.;
+        if (launches) launches[k] = s->prof.n[k];
+    }
+    if (names && cap > 0) {
+        strncpy(names, all.c_str(), (size_t)cap - 1);
+        names[cap - 1] = 0;
+    }
+    return k;
+}
+
+// ---------------------------------------------------------------------------
+// step pieces
+// ---------------------------------------------------------------------------
+static FieldBatch batch(std::initializer_list<double *> l)
+{
+    FieldBatch b;
+    b.n = 0;
+    for (double *p : l) b.f[b.n++] = p;
+    return b;
+}
+
+static void horfft_dev(iw_solver *s, FieldBatch b, int is, int nk)
+{
+    if (b.n == 0) return;
+    if (is == -1) {
+        {
+            Scope sc(s, "fft_x_fwd");
+            launch_fft_x(s->g, b, nk, -1, s->stream);
+        }
+        if (s->g.ndims == 3) {
+            Scope sc(s, "fft_y_fwd");
+            launch_fft_y(s->g, b, nk, -1, s->stream);
+        }
+    } else {
+        if (s->g.ndims == 3) {
+            Scope sc(s, "fft_y_inv");
+            launch_fft_y(s->g, b, nk, 1, s->stream);
+        }
+        Scope sc(s, "fft_x_inv");
+        launch_fft_x(s->g, b, nk, 1, s->stream);
+    }
+}
+
+static void batch_fft(iw_solver *s, std::vector<double *> v, int is)
+{
+    for (size_t i = 0; i < v.size(); i += kMaxFields) {
+        FieldBatch b;
+        b.n = 0;
+        for (size_t q = i; q < v.size() && q < i + kMaxFields; q++) b.f[b.n++] = v[q];
+        horfft_dev(s, b, is, s->g.nz);
+    }
+}
+
+// convec (solver.f:6-514) on the device state
+static int convec_dev(iw_solver *s, int ispecfilternl)
+{
+    const Geo &g = s->g;
+    double **B = s->buf;
+    double *uvwt[4] = {B[0], B[1], B[2], B[3]};
+    double *S[4] = {B[12], B[13], B[14], B[15]};
+    double **gr = s->grad;
+    {
+        Scope sc(s, "zero_meanw");
+        launch_zero_meanw(g, B[2], s->stream);
+    }
+    {
+        Scope sc(s, "dpdz");
+        double *o3[3] = {gr[2], gr[5], gr[8]};
+        launch_dpdz3(g, uvwt, o3, s->stream);
+    }
+    {
+        Scope sc(s, "grad");
+        launch_grad(g, uvwt, gr, s->stream);
+    }
+    batch_fft(s, {B[0], B[1], B[2], B[3], gr[0], gr[1], gr[2], gr[3], gr[4], gr[5], gr[6], gr[7], gr[8]}, 1);
+    {
+        Scope sc(s, "dpdz_flux");
+        double *o4[4] = {S[0], S[1], S[2], s->ox};
+        launch_dpdz_conv(g, uvwt, o4, s->stream);
+    }
+    {
+        Scope sc(s, "rhs_phys");
+        launch_rhs_phys(g, s->P, s->t, uvwt, gr, S, s->ox, s->prod, s->stream);
+    }
+    double **pr = s->prod;
+    batch_fft(s, {S[0], S[1], S[2], S[3], pr[0], pr[1], pr[2], pr[3], pr[4], pr[5], pr[6], B[3], B[0], B[1],
+                  B[2]},
+              -1);
+    {
+        Scope sc(s, "rhs_spec");
+        launch_rhs_spec(g, S, pr, s->stream);
+    }
+    {
+        Scope sc(s, "wnl");
+        launch_wnl(g, B[14], B[18], B[22], s->fac, s->wnl, s->stream);
+    }
+    {
+        Scope sc(s, "rotvisc");
+        double *lev[9] = {B[0], B[1], B[2], B[4], B[5], B[6], B[8], B[9], B[10]};
+        launch_rotvisc(g, s->P, lev, s->fac, s->wviscrot, s->stream);
+    }
+    {
+        Scope sc(s, "bdf_filter");
+        for (int q = 0; q < 4; q++) {
+            double *S3[3] = {B[12 + q], B[16 + q], B[20 + q]};
+            double *F3[3] = {B[q], B[4 + q], B[8 + q]};
+            launch_bdf_filter(g, s->P, S3, F3, B[8 + q], s->fac, s->bdf, s->P.dt, ispecfilternl == 1,
+                              s->stream);
+        }
+    }
+    for (int q = 0; q < 4; q++) { // history rotation (solver.f:447-478) by pointer swap
+        double *nw = B[8 + q];
+        B[8 + q] = B[4 + q];
+        B[4 + q] = B[q];
+        B[q] = nw;
+        double *fr = B[20 + q];
+        B[20 + q] = B[16 + q];
+        B[16 + q] = B[12 + q];
+        B[12 + q] = fr;
+    }
+    return hipGetLastError() == hipSuccess ? 0 : fail("convec launch failed");
+}
+
+static int calcuvwt_dev(iw_solver *s, int cpen)
+{
+    if (ensure_factors(s)) return -1;
+    {
+        Scope sc(s, "calc_mode");
+        double *uvwt[4] = {s->buf[0], s->buf[1], s->buf[2], s->buf[3]};
+        launch_calc_mode(s->g, s->P, s->sys, uvwt, s->wnl, s->wviscrot, s->P.dt, cpen, s->stream);
+    }
+    return hipGetLastError() == hipSuccess ? 0 : fail("calcuvwt launch failed");
+}
+
+static int iface_dev(iw_solver *s, double *const *f, int nf)
+{
+    launch_zero_int(s->d_count, 4, s->stream);
+    std::vector<double *> v(f, f + nf);
+    batch_fft(s, v, 1);
+    {
+        Scope sc(s, "iface");
+        FieldBatch b;
+        b.n = nf;
+        for (int q = 0; q < nf; q++) b.f[q] = f[q];
+        launch_iface(s->g, b, s->d_count, s->stream);
+    }
+    batch_fft(s, v, -1);
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// device-resident step loop (main.f:653-1066)
+// ---------------------------------------------------------------------------
+extern "C" int iw_set_initial(iw_solver *s, const double *u, const double *v, const double *w,
+                              const double *temp)
+{
+    if (!s || !u || !v || !w || !temp) return fail("iw_set_initial: null argument");
+    size_t fb = sizeof(double) * s->g.ntot;
+    const double *src[4] = {u, v, w, temp};
+    for (int q = 0; q < 24; q++) HIPCHK(hipMemsetAsync(s->buf[q], 0, fb, s->stream));
+    for (int q = 0; q < 4; q++)
+        HIPCHK(hipMemcpyAsync(s->buf[q], src[q], fb, hipMemcpyHostToDevice, s->stream));
+    batch_fft(s, {s->buf[0], s->buf[1], s->buf[2], s->buf[3]}, -1);
+    HIPCHK(hipStreamSynchronize(s->stream));
+    s->t = 0.;
+    s->fac[0] = 1.; s->fac[1] = 0.; s->fac[2] = 0.; // main.f:543-552
+    s->bdf[0] = 1.; s->bdf[1] = 0.; s->bdf[2] = 0.;
+    s->bdfv = 1.;
+    s->istep = 0;
+    return 0;
+}
+
+extern "C" int iw_run(iw_solver *s, int nsteps, int *counts)
+{
+    if (!s || nsteps < 0) return fail("iw_run: bad argument");
+    for (int n = 0; n < nsteps; n++) {
+        int istep = ++s->istep;
+        // maxcalc round trip (various.f:739-742, 854-857)
+        batch_fft(s, {s->buf[0], s->buf[1], s->buf[2], s->buf[3]}, 1);
+        batch_fft(s, {s->buf[0], s->buf[1], s->buf[2], s->buf[3]}, -1);
+        if (convec_dev(s, s->P.ispecfilternl)) return -1;
+        if (istep < 2) { // main.f:700-777
+            s->fac[0] = 2.; s->fac[1] = -1.; s->fac[2] = 0.;
+            s->bdf[0] = 2.; s->bdf[1] = -0.5; s->bdf[2] = 0.;
+        } else {
+            s->fac[0] = 3.; s->fac[1] = -3.; s->fac[2] = 1.;
+            s->bdf[0] = 3.; s->bdf[1] = -1.5; s->bdf[2] = 1. / 3.;
+        }
+        int cpen = (s->P.ispecfilterexp == 1) && (istep % s->P.nspecfilterexp == 0);
+        if (calcuvwt_dev(s, cpen)) return -1;
+        s->bdfv = (istep < 2) ? 1.5 : 11. / 6.; // main.f:815-856
+        s->t = s->t + s->P.dt;
+        bool avg = s->P.navgintrfc > 0 && istep % s->P.navgintrfc == 0;
+        if (avg) {
+            if (iface_dev(s, s->buf, 4)) return -1;
+        }
+        if (counts) {
+            if (avg) {
+                HIPCHK(hipMemcpyAsync(s->h_count, s->d_count, 4 * sizeof(int), hipMemcpyDeviceToHost,
+                                      s->stream));
+                HIPCHK(hipStreamSynchronize(s->stream));
+                for (int q = 0; q < 4; q++) counts[4 * n + q] = s->h_count[q];
+            } else {
+                for (int q = 0; q < 4; q++) counts[4 * n + q] = 0;
+            }
+        }
+    }
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// reference routines on host arrays
+// ---------------------------------------------------------------------------
+extern "C" int iw_horfft(iw_solver *s, double *b, int is, int nk)
+{
+    if (!s || !b) return fail("iw_horfft: null argument");
+    if (nk < 1 || nk > s->g.nz) return fail("iw_horfft: nk out of range");
+    if (is != 1 && is != -1) return fail("iw_horfft: is must be +1 or -1");
+    size_t bytes = sizeof(double) * s->g.nxppy * nk;
+    HIPCHK(hipMemcpyAsync(s->hbuf, b, bytes, hipMemcpyHostToDevice, s->stream));
+    horfft_dev(s, batch({s->hbuf}), is, nk);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(b, s->hbuf, bytes, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return 0;
+}
+
+extern "C" int iw_dpdz(iw_solver *s, const double *a, double *b)
+{
+    if (!s || !a || !b) return fail("iw_dpdz: null argument");
+    size_t bytes = sizeof(double) * s->g.ntot;
+    HIPCHK(hipMemcpyAsync(s->hbuf, a, bytes, hipMemcpyHostToDevice, s->stream));
+    launch_dpdz(s->g, s->hbuf, s->ox, s->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(b, s->ox, bytes, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return 0;
+}
+
+extern "C" int iw_filter_3dlegenfourier(iw_solver *s, double *f, int ifourier, int ilegen)
+{
+    if (!s || !f) return fail("iw_filter_3dlegenfourier: null argument");
+    size_t bytes = sizeof(double) * s->g.ntot;
+    HIPCHK(hipMemcpyAsync(s->hbuf, f, bytes, hipMemcpyHostToDevice, s->stream));
+    launch_filter3d(s->g, s->hbuf, ifourier == 1, ilegen == 1, s->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(f, s->hbuf, bytes, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return 0;
+}
+
+extern "C" int iw_interface_avg3d(iw_solver *s, double *f, int *ic)
+{
+    if (!s || !f) return fail("iw_interface_avg3d: null argument");
+    size_t bytes = sizeof(double) * s->g.ntot;
+    HIPCHK(hipMemcpyAsync(s->hbuf, f, bytes, hipMemcpyHostToDevice, s->stream));
+    double *one[1] = {s->hbuf};
+    if (iface_dev(s, one, 1)) return -1;
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(f, s->hbuf, bytes, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipMemcpyAsync(s->h_count, s->d_count, sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    if (ic) *ic = s->h_count[0];
+    return 0;
+}
+
+extern "C" int iw_convec(iw_solver *s, int istep, int ispecfilternl)
+{
+    if (!s) return fail("null solver");
+    (void)istep;
+    if (convec_dev(s, ispecfilternl)) return -1;
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return 0;
+}
+
+extern "C" int iw_calcuvwt(iw_solver *s, int cpenspecfilter)
+{
+    if (!s) return fail("null solver");
+    if (calcuvwt_dev(s, cpenspecfilter != 0)) return -1;
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return 0;
+}

@@ -1,0 +1,109 @@
+// Internal declarations shared by the kernels and the host driver of libiwgpu.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "iwgpu.h"
+
+namespace iw {
+
+constexpr int kMaxNl = 32;     // GLL points per subdomain supported by the kernels
+constexpr int kMaxFields = 16; // fields per batched transform launch
+
+// Field pointer batch passed by value to batched kernels.
+struct FieldBatch {
+    double *f[kMaxFields];
+    int n;
+};
+
+// Problem geometry and device-resident tables.
+struct Geo {
+    int nx, ny, nz, nzloc, nsubd, nxpp, nxh, nxhp, nyh, ndims;
+    long nxppy, ntot;
+    int M, logM, logNy; // x complex length nx/2 and logs
+    double scaley;
+    // device tables
+    double *D, *D2, *AL, *wg, *sig, *xw, *yw, *zh, *z, *damp, *fcf, *x;
+    double2 *twM;   // e^{-2 pi i t/M}, t < M/2
+    double2 *twN;   // e^{-2 pi i k/nx}, k <= M
+    double2 *twY;   // e^{-2 pi i t/ny}, t < ny/2
+};
+
+struct SysTables {
+    int nsys = 0;
+    int ldab = 0, kl = 0, kv = 0;
+    double *AB = nullptr;   // nsys * ldab * nz, LAPACK band columns
+    int *ipiv = nullptr;    // nsys * nz (0-based)
+    double *coef = nullptr; // nsys * nsubd * 12 assembly coefficients
+    double *sysinfo = nullptr; // nsys * 4: eps, tau1(bottom), tau2(top), bc flags
+    int *modesys = nullptr; // nmode * 4: P, UV, W, T system ids (-1: none)
+    size_t cap_sys = 0;
+};
+
+struct Prof {
+    std::vector<std::string> names;
+    std::vector<double> ms;
+    std::vector<int> n;
+};
+
+} // namespace iw
+
+struct iw_solver {
+    iw_params P;
+    iw::Geo g;
+    hipStream_t stream = nullptr;
+    int device = 0;
+    // state: rotating index tables into buf
+    double *buf[24];   // u v w t | un.. | unm.. | S(4) | Sn(4) | Snm(4)
+    double *grad[9];   // dudx dudy dudz dvdx dvdy dvdz dwdx dwdy dwdz
+    double *prod[7];   // -uu/2 -uv/2 -vv/2 -uw/2 -vw/2 -uT -vT
+    double *ox;
+    double *wnl, *wviscrot;
+    double *hbuf;      // device scratch for host-array entry points
+    int *d_count;      // 4 interface counters
+    int *h_count;      // pinned
+    // time stepping
+    double t, fac[3], bdf[3], bdfv;
+    int istep;
+    double factored_bdfv = -1, factored_dt = -1;
+    bool p_factored = false;
+    iw::SysTables sys;
+    // host copies of tables
+    std::vector<double> zpts, wg, d, d2, alpol, xw, xsq, yw, ysq, z;
+    // profiling
+    bool profiling = false;
+    iw::Prof prof;
+    hipEvent_t ev0, ev1;
+    long long dev_bytes = 0;
+};
+
+namespace iw {
+// kernels launchers (iw_kernels.hip)
+void launch_fft_x(const Geo &g, FieldBatch b, int nk, int dir, hipStream_t st);
+void launch_fft_y(const Geo &g, FieldBatch b, int nk, int dir, hipStream_t st);
+void launch_dpdz(const Geo &g, const double *a, double *b, hipStream_t st);
+void launch_dpdz_conv(const Geo &g, double *const *uvwt, double *const *out4, hipStream_t st);
+void launch_dpdz3(const Geo &g, double *const *in3, double *const *out3, hipStream_t st);
+void launch_grad(const Geo &g, double *const *uvw, double *const *gr, hipStream_t st);
+void launch_rhs_phys(const Geo &g, const iw_params &P, double t, double *const *uvwt,
+                     double *const *gr, double *const *S, const double *ox, double *const *pr,
+                     hipStream_t st);
+void launch_rhs_spec(const Geo &g, double *const *S, double *const *pr, hipStream_t st);
+void launch_wnl(const Geo &g, const double *sw, const double *swn, const double *swnm,
+                const double *fac, double *wnl, hipStream_t st);
+void launch_rotvisc(const Geo &g, const iw_params &P, double *const *lev /*9*/, const double *fac,
+                    double *wviscrot, hipStream_t st);
+void launch_bdf_filter(const Geo &g, const iw_params &P, double *const *S3, double *const *F3,
+                       double *out, const double *fac, const double *bdf, double dt, int filt,
+                       hipStream_t st);
+void launch_filter3d(const Geo &g, double *f, int ifourier, int ilegen, hipStream_t st);
+void launch_iface(const Geo &g, FieldBatch b, int *count, hipStream_t st);
+void launch_factor(const Geo &g, const SysTables &S, int first, int count, hipStream_t st);
+void launch_calc_mode(const Geo &g, const iw_params &P, const SysTables &S, double *const *uvwt,
+                      const double *wnl, const double *wviscrot, double dt, int cpen,
+                      hipStream_t st);
+void launch_zero_meanw(const Geo &g, double *w, hipStream_t st);
+void launch_zero_int(int *p, int n, hipStream_t st);
+} // namespace iw

@@ -1,0 +1,195 @@
+"""ctypes binding of libiwgpu.so (include/iwgpu.h) and a host-side solver handle.
+
+This is the Python face of the C-ABI.  There is no CPU fallback: if the HIP
+library is missing or cannot initialise a device, every entry point raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from .params import IWParams
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(PKG, "libiwgpu.so")
+
+# every symbol declared in include/iwgpu.h
+EXPORTS = [
+    "iw_params_size", "iw_last_error", "iw_create", "iw_destroy", "iw_set_state", "iw_get_state",
+    "iw_get_pressbc", "iw_set_coeffs", "iw_get_coeffs", "iw_set_initial", "iw_run", "iw_time",
+    "iw_sync", "iw_horfft", "iw_dpdz", "iw_filter_3dlegenfourier", "iw_interface_avg3d",
+    "iw_convec", "iw_calcuvwt", "iw_abvar1", "iw_abvar2", "iw_abvar3", "iw_bdfvarv",
+    "iw_bdfvar1", "iw_bdfvar2", "iw_bdfvar3", "iw_set_profiling", "iw_kernel_stats",
+    "iw_device_bytes",
+]
+
+_lib = None
+
+
+class IWError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libiwgpu.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIBPATH):
+        raise IWError("libiwgpu.so not built: run `python -m iwrepo_amd.build`")
+    L = ctypes.CDLL(LIBPATH)
+    P, D, I = ctypes.c_void_p, ctypes.c_double, ctypes.c_int
+    L.iw_last_error.restype = ctypes.c_char_p
+    L.iw_create.argtypes = [P, ctypes.POINTER(P)]
+    L.iw_destroy.argtypes = [P]
+    L.iw_set_state.argtypes = [P, I, P]
+    L.iw_get_state.argtypes = [P, I, P]
+    L.iw_get_pressbc.argtypes = [P, P, P]
+    L.iw_set_coeffs.argtypes = [P, D, P, P, D]
+    L.iw_get_coeffs.argtypes = [P, P, P, P, P]
+    L.iw_set_initial.argtypes = [P, P, P, P, P]
+    L.iw_run.argtypes = [P, I, P]
+    L.iw_time.restype = D
+    L.iw_time.argtypes = [P]
+    L.iw_sync.argtypes = [P]
+    L.iw_horfft.argtypes = [P, P, I, I]
+    L.iw_dpdz.argtypes = [P, P, P]
+    L.iw_filter_3dlegenfourier.argtypes = [P, P, I, I]
+    L.iw_interface_avg3d.argtypes = [P, P, ctypes.POINTER(I)]
+    L.iw_convec.argtypes = [P, I, I]
+    L.iw_calcuvwt.argtypes = [P, I]
+    for n in ("abvar1", "abvar2", "abvar3", "bdfvarv", "bdfvar1", "bdfvar2", "bdfvar3"):
+        f = getattr(L, "iw_" + n)
+        f.restype = D
+        f.argtypes = [D, D, D]
+    L.iw_set_profiling.argtypes = [P, I]
+    L.iw_kernel_stats.argtypes = [P, ctypes.c_char_p, I, P, P, I]
+    L.iw_device_bytes.restype = ctypes.c_longlong
+    L.iw_device_bytes.argtypes = [P]
+    if L.iw_params_size() != ctypes.sizeof(IWParams):
+        raise IWError("iw_params layout mismatch")
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc != 0:
+        raise IWError(lib().iw_last_error().decode())
+
+
+def _ptr(a):
+    if a.dtype != np.float64 or not a.flags.c_contiguous:
+        raise ValueError("arrays must be C-contiguous float64")
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class Solver:
+    """One device instance of the step for parameter block ``P``."""
+
+    def __init__(self, P):
+        self.L = lib()
+        self.P = P
+        self.shape = (P.nzloc * P.nsubd, P.ny, P.nx + 2)
+        h = ctypes.c_void_p()
+        _check(self.L.iw_create(ctypes.byref(P), ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.iw_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _arr(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        if a.size != int(np.prod(self.shape)):
+            raise ValueError("field has %d values, expected %d" % (a.size, int(np.prod(self.shape))))
+        return a
+
+    def set_initial(self, u, v, w, t):
+        arrs = [self._arr(a) for a in (u, v, w, t)]
+        _check(self.L.iw_set_initial(self.h, *[_ptr(a) for a in arrs]))
+
+    def run(self, nsteps, counts=True):
+        c = np.zeros(4 * max(nsteps, 1), dtype=np.int32)
+        _check(self.L.iw_run(self.h, nsteps, c.ctypes.data_as(ctypes.c_void_p) if counts else None))
+        return c[:4 * nsteps].reshape(nsteps, 4)
+
+    def sync(self):
+        _check(self.L.iw_sync(self.h))
+
+    def time(self):
+        return self.L.iw_time(self.h)
+
+    def get_state(self, idx):
+        out = np.empty(self.shape)
+        _check(self.L.iw_get_state(self.h, idx, _ptr(out)))
+        return out
+
+    def set_state(self, idx, a):
+        a = self._arr(a)
+        _check(self.L.iw_set_state(self.h, idx, _ptr(a)))
+
+    def state(self):
+        return np.stack([self.get_state(i) for i in range(24)])
+
+    def pressbc(self):
+        n = self.shape[1] * self.shape[2] * 2
+        wnl, wv = np.empty(n), np.empty(n)
+        _check(self.L.iw_get_pressbc(self.h, _ptr(wnl), _ptr(wv)))
+        return wnl, wv
+
+    def set_coeffs(self, t, fac, bdf, bdfv):
+        f = np.ascontiguousarray(fac, dtype=np.float64)
+        b = np.ascontiguousarray(bdf, dtype=np.float64)
+        _check(self.L.iw_set_coeffs(self.h, t, _ptr(f), _ptr(b), bdfv))
+
+    def horfft(self, b, is_):
+        b = np.ascontiguousarray(b, dtype=np.float64).copy()
+        nk = b.size // (self.shape[1] * self.shape[2])
+        _check(self.L.iw_horfft(self.h, _ptr(b), is_, nk))
+        return b
+
+    def dpdz(self, a):
+        a = self._arr(a)
+        b = np.empty_like(a)
+        _check(self.L.iw_dpdz(self.h, _ptr(a), _ptr(b)))
+        return b
+
+    def filter3d(self, f, ifourier=1, ilegen=1):
+        f = self._arr(f).copy()
+        _check(self.L.iw_filter_3dlegenfourier(self.h, _ptr(f), ifourier, ilegen))
+        return f
+
+    def interface_avg(self, f):
+        f = self._arr(f).copy()
+        ic = ctypes.c_int(0)
+        _check(self.L.iw_interface_avg3d(self.h, _ptr(f), ctypes.byref(ic)))
+        return f, ic.value
+
+    def convec(self, istep, ispecfilternl=1):
+        _check(self.L.iw_convec(self.h, istep, ispecfilternl))
+
+    def calcuvwt(self, cpen=1):
+        _check(self.L.iw_calcuvwt(self.h, cpen))
+
+    def set_profiling(self, on=True):
+        _check(self.L.iw_set_profiling(self.h, 1 if on else 0))
+
+    def kernel_stats(self):
+        names = ctypes.create_string_buffer(4096)
+        avg = np.zeros(64)
+        n = np.zeros(64, dtype=np.int32)
+        k = self.L.iw_kernel_stats(self.h, names, 4096, _ptr(avg), n.ctypes.data_as(ctypes.c_void_p), 64)
+        if k < 0:
+            raise IWError(self.L.iw_last_error().decode())
+        nm = names.value.decode().split(";") if k else []
+        return {nm[i]: (float(avg[i]), int(n[i])) for i in range(k)}
+
+    def device_bytes(self):
+        return int(self.L.iw_device_bytes(self.h))
