@@ -102,11 +102,13 @@ double iw_bdfvar2(double dt1, double dt2, double dt3);
 double iw_bdfvar3(double dt1, double dt2, double dt3);
 
 /* ---- measurement ---- */
-/* Average device time (ms) per launch of each kernel class recorded during
- * the last iw_run with profiling enabled; names are written as a
- * ';'-separated list into names (capacity cap bytes). */
+/* Per kernel class recorded while profiling is on: average device time (ms,
+ * HIP events on the solver's stream) and average algorithmic bytes per
+ * launch; names are written as a ';'-separated list into names (capacity
+ * cap bytes).  Returns the number of classes. */
 int iw_set_profiling(iw_solver *s, int on);
-int iw_kernel_stats(iw_solver *s, char *names, int cap, double *avg_ms, int *launches, int maxk);
+int iw_kernel_stats(iw_solver *s, char *names, int cap, double *avg_ms, int *launches,
+                    double *avg_bytes, int maxk);
 /* Bytes of device memory held by the instance. */
 long long iw_device_bytes(iw_solver *s);
 

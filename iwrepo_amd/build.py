@@ -16,7 +16,7 @@ LIB = os.path.join(PKG, "libiwgpu.so")
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 FLANG = "/opt/rocm/lib/llvm/bin/flang"
 
-HIPFLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",
+HIPFLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-Wno-unused-value",
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
             "-Wno-unused-result", "-I", os.path.join(ROOT, "include")]
 

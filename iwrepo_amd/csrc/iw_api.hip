@@ -452,6 +452,7 @@ static int ensure_factors(iw_solver *s)
 struct ProfRec {
     std::string name;
     hipEvent_t a, b;
+    double bytes;
 };
 static thread_local std::vector<ProfRec> g_prec;
 static thread_local std::vector<hipEvent_t> g_pool;
@@ -471,7 +472,8 @@ struct Scope {
     iw_solver *s;
     const char *name;
     hipEvent_t a;
-    Scope(iw_solver *s_, const char *n) : s(s_), name(n), a(nullptr)
+    double bytes;
+    Scope(iw_solver *s_, const char *n, double b = 0.) : s(s_), name(n), a(nullptr), bytes(b)
     {
         if (s->profiling) {
             a = pool_event();
@@ -483,7 +485,7 @@ struct Scope {
         if (s->profiling) {
             hipEvent_t b = pool_event();
             hipEventRecord(b, s->stream);
-            g_prec.push_back({name, a, b});
+            g_prec.push_back({name, a, b, bytes});
         }
     }
 };
@@ -502,9 +504,11 @@ static void prof_resolve(iw_solver *s)
             s->prof.names.push_back(r.name);
             s->prof.ms.push_back(0.);
             s->prof.n.push_back(0);
+            s->prof.bytes.push_back(0.);
         }
         s->prof.ms[k] += ms;
         s->prof.n[k] += 1;
+        s->prof.bytes[k] += r.bytes;
     }
     g_prec.clear();
     g_pool_next = 0;
@@ -715,7 +719,8 @@ extern "C" int iw_set_profiling(iw_solver *s, int on)
     return 0;
 }
 
-extern "C" int iw_kernel_stats(iw_solver *s, char *names, int cap, double *avg_ms, int *launches, int maxk)
+extern "C" int iw_kernel_stats(iw_solver *s, char *names, int cap, double *avg_ms, int *launches,
+                               double *avg_bytes, int maxk)
 {
     if (!s) return fail("null solver");
     prof_resolve(s);
@@ -726,6 +731,7 @@ extern "C" int iw_kernel_stats(iw_solver *s, char *names, int cap, double *avg_m
         all += s->prof.names[k];
         if (avg_ms) avg_ms[k] = s->prof.n[k] ? s->prof.ms[k] / s->prof.n[k] : 0.;
         if (launches) launches[k] = s->prof.n[k];
+        if (avg_bytes) avg_bytes[k] = s->prof.n[k] ? s->prof.bytes[k] / s->prof.n[k] : 0.;
     }
     if (names && cap > 0) {
         strncpy(names, all.c_str(), (size_t)cap - 1);
@@ -748,21 +754,23 @@ static FieldBatch batch(std::initializer_list<double *> l)
 static void horfft_dev(iw_solver *s, FieldBatch b, int is, int nk)
 {
     if (b.n == 0) return;
+    const double bx = 16. * (double)s->g.nxppy * nk * b.n;         // each double read + written
+    const double by = 32. * (double)s->g.nxh * s->g.ny * nk * b.n; // columns ii < nxh
     if (is == -1) {
         {
-            Scope sc(s, "fft_x_fwd");
+            Scope sc(s, "fft_x_fwd", bx);
             launch_fft_x(s->g, b, nk, -1, s->stream);
         }
         if (s->g.ndims == 3) {
-            Scope sc(s, "fft_y_fwd");
+            Scope sc(s, "fft_y_fwd", by);
             launch_fft_y(s->g, b, nk, -1, s->stream);
         }
     } else {
         if (s->g.ndims == 3) {
-            Scope sc(s, "fft_y_inv");
+            Scope sc(s, "fft_y_inv", by);
             launch_fft_y(s->g, b, nk, 1, s->stream);
         }
-        Scope sc(s, "fft_x_inv");
+        Scope sc(s, "fft_x_inv", bx);
         launch_fft_x(s->g, b, nk, 1, s->stream);
     }
 }
@@ -785,27 +793,28 @@ static int convec_dev(iw_solver *s, int ispecfilternl)
     double *uvwt[4] = {B[0], B[1], B[2], B[3]};
     double *S[4] = {B[12], B[13], B[14], B[15]};
     double **gr = s->grad;
+    const double nt = (double)g.ntot;
     {
         Scope sc(s, "zero_meanw");
         launch_zero_meanw(g, B[2], s->stream);
     }
     {
-        Scope sc(s, "dpdz");
+        Scope sc(s, "dpdz", 48. * nt);
         double *o3[3] = {gr[2], gr[5], gr[8]};
         launch_dpdz3(g, uvwt, o3, s->stream);
     }
     {
-        Scope sc(s, "grad");
+        Scope sc(s, "grad", 72. * nt);
         launch_grad(g, uvwt, gr, s->stream);
     }
     batch_fft(s, {B[0], B[1], B[2], B[3], gr[0], gr[1], gr[2], gr[3], gr[4], gr[5], gr[6], gr[7], gr[8]}, 1);
     {
-        Scope sc(s, "dpdz_flux");
+        Scope sc(s, "dpdz_flux", 64. * nt);
         double *o4[4] = {S[0], S[1], S[2], s->ox};
         launch_dpdz_conv(g, uvwt, o4, s->stream);
     }
     {
-        Scope sc(s, "rhs_phys");
+        Scope sc(s, "rhs_phys", 224. * nt);
         launch_rhs_phys(g, s->P, s->t, uvwt, gr, S, s->ox, s->prod, s->stream);
     }
     double **pr = s->prod;
@@ -813,11 +822,11 @@ static int convec_dev(iw_solver *s, int ispecfilternl)
                   B[2]},
               -1);
     {
-        Scope sc(s, "rhs_spec");
+        Scope sc(s, "rhs_spec", 120. * nt);
         launch_rhs_spec(g, S, pr, s->stream);
     }
     {
-        Scope sc(s, "wnl");
+        Scope sc(s, "wnl", 64. * (double)g.nxppy);
         launch_wnl(g, B[14], B[18], B[22], s->fac, s->wnl, s->stream);
     }
     {
@@ -826,7 +835,7 @@ static int convec_dev(iw_solver *s, int ispecfilternl)
         launch_rotvisc(g, s->P, lev, s->fac, s->wviscrot, s->stream);
     }
     {
-        Scope sc(s, "bdf_filter");
+        Scope sc(s, "bdf_filter", 4 * 56. * nt);
         for (int q = 0; q < 4; q++) {
             double *S3[3] = {B[12 + q], B[16 + q], B[20 + q]};
             double *F3[3] = {B[q], B[4 + q], B[8 + q]};
@@ -851,7 +860,9 @@ static int calcuvwt_dev(iw_solver *s, int cpen)
 {
     if (ensure_factors(s)) return -1;
     {
-        Scope sc(s, "calc_mode");
+        // state columns in and out (64 B per point) + every cached factor once
+        double fb = (double)s->sys.nsys * ((double)s->sys.ldab * s->g.nz * 8. + s->g.nz * 4.);
+        Scope sc(s, "calc_mode", 64. * (double)s->g.ntot + fb);
         double *uvwt[4] = {s->buf[0], s->buf[1], s->buf[2], s->buf[3]};
         launch_calc_mode(s->g, s->P, s->sys, uvwt, s->wnl, s->wviscrot, s->P.dt, cpen, s->stream);
     }
@@ -864,7 +875,7 @@ static int iface_dev(iw_solver *s, double *const *f, int nf)
     std::vector<double *> v(f, f + nf);
     batch_fft(s, v, 1);
     {
-        Scope sc(s, "iface");
+        Scope sc(s, "iface", 48. * (double)s->g.nxppy * (s->g.nsubd - 1) * nf);
         FieldBatch b;
         b.n = nf;
         for (int q = 0; q < nf; q++) b.f[q] = f[q];

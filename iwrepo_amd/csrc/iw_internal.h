@@ -46,6 +46,7 @@ struct Prof {
     std::vector<std::string> names;
     std::vector<double> ms;
     std::vector<int> n;
+    std::vector<double> bytes;
 };
 
 } // namespace iw

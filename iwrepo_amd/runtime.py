@@ -63,7 +63,7 @@ def lib():
         f.restype = D
         f.argtypes = [D, D, D]
     L.iw_set_profiling.argtypes = [P, I]
-    L.iw_kernel_stats.argtypes = [P, ctypes.c_char_p, I, P, P, I]
+    L.iw_kernel_stats.argtypes = [P, ctypes.c_char_p, I, P, P, P, I]
     L.iw_device_bytes.restype = ctypes.c_longlong
     L.iw_device_bytes.argtypes = [P]
     if L.iw_params_size() != ctypes.sizeof(IWParams):
@@ -184,12 +184,15 @@ class Solver:
     def kernel_stats(self):
         names = ctypes.create_string_buffer(4096)
         avg = np.zeros(64)
+        byt = np.zeros(64)
         n = np.zeros(64, dtype=np.int32)
-        k = self.L.iw_kernel_stats(self.h, names, 4096, _ptr(avg), n.ctypes.data_as(ctypes.c_void_p), 64)
+        k = self.L.iw_kernel_stats(self.h, names, 4096, _ptr(avg), n.ctypes.data_as(ctypes.c_void_p),
+                                   _ptr(byt), 64)
         if k < 0:
             raise IWError(self.L.iw_last_error().decode())
         nm = names.value.decode().split(";") if k else []
-        return {nm[i]: (float(avg[i]), int(n[i])) for i in range(k)}
+        return {nm[i]: {"avg_ms": float(avg[i]), "launches": int(n[i]), "avg_bytes": float(byt[i])}
+                for i in range(k)}
 
     def device_bytes(self):
         return int(self.L.iw_device_bytes(self.h))
