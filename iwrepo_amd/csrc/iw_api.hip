@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <algorithm>
 #include <map>
 #include <string>
 #include <tuple>
@@ -427,7 +428,49 @@ static int build_systems(iw_solver *s, bool refactor_p, int &first, int &count)
         s->dev_bytes += (long long)(sizeof(int) * 4 * (size_t)nmode);
     }
     HIPCHK(hipMemcpy(S.modesys, modesys.data(), sizeof(int) * modesys.size(), hipMemcpyHostToDevice));
+    // XCD-aware task order: group tasks by the system they solve, deal the
+    // groups round-robin to the 8 XCDs, and interleave the per-XCD queues so
+    // that block b runs on XCD b % 8 (the hardware's round-robin dispatch)
+    auto make_order = [&](const std::vector<std::pair<int, int>> &tasks, std::vector<int> &order) {
+        std::vector<std::pair<int, int>> t = tasks; // (system, task)
+        std::stable_sort(t.begin(), t.end());
+        const int nx = 8;
+        std::vector<std::vector<int>> q(nx);
+        int g = -1, lastsys = -2;
+        for (auto &e : t) {
+            if (e.first != lastsys) {
+                lastsys = e.first;
+                g++;
+            }
+            q[g % nx].push_back(e.second);
+        }
+        size_t len = 0;
+        for (auto &v : q) len = std::max(len, v.size());
+        order.assign(len * nx, -1);
+        for (int x = 0; x < nx; x++)
+            for (size_t i = 0; i < q[x].size(); i++) order[i * nx + x] = q[x][i];
+    };
+    std::vector<std::pair<int, int>> tp, tv;
+    for (int m = 0; m < nmode; m++) {
+        tp.push_back({m == 0 ? -1 : modesys[4 * m], m});
+        tv.push_back({modesys[4 * m + 1], 4 * m});
+        tv.push_back({modesys[4 * m + 1], 4 * m + 1});
+        if (m != 0) tv.push_back({modesys[4 * m + 2], 4 * m + 2});
+        tv.push_back({modesys[4 * m + 3], 4 * m + 3});
+    }
+    std::vector<int> op, ov;
+    make_order(tp, op);
+    make_order(tv, ov);
+    if (S.order_p) hipFree(S.order_p);
+    if (S.order_v) hipFree(S.order_v);
+    HIPCHK(hipMalloc((void **)&S.order_p, sizeof(int) * op.size()));
+    HIPCHK(hipMalloc((void **)&S.order_v, sizeof(int) * ov.size()));
+    HIPCHK(hipMemcpy(S.order_p, op.data(), sizeof(int) * op.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(S.order_v, ov.data(), sizeof(int) * ov.size(), hipMemcpyHostToDevice));
+    S.nblk_p = (int)op.size();
+    S.nblk_v = (int)ov.size();
     S.nsys = nsys;
+    S.np = np;
     first = refactor_p ? 0 : np;
     count = nsys - first;
     return 0;
@@ -440,6 +483,40 @@ static int ensure_factors(iw_solver *s)
     if (build_systems(s, !s->p_factored, first, count)) return -1;
     launch_factor(s->g, s->sys, first, count, s->stream);
     HIPCHK(hipGetLastError());
+    // columns of systems factored without row interchanges go to the
+    // lanes-as-columns solver; if any u/v, w or density system pivoted, all
+    // of them take the wavefront path
+    SysTables &S = s->sys;
+    const int nmode = s->g.nxhp * s->g.ny;
+    std::vector<double> info((size_t)S.nsys * 4);
+    HIPCHK(hipMemcpyAsync(info.data(), S.sysinfo, sizeof(double) * info.size(), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    std::vector<int> ms(4 * (size_t)nmode);
+    HIPCHK(hipMemcpy(ms.data(), S.modesys, sizeof(int) * ms.size(), hipMemcpyDeviceToHost));
+    bool ok = solve_np_supported(s->P.nzloc, s->P.nsubd);
+    for (int q = S.np; q < S.nsys && ok; q++)
+        if (info[4 * q + 3] != 0.) ok = false;
+    S.np_ok = ok;
+    if (ok) {
+        std::vector<std::pair<int, int>> cols;
+        for (int f = 0; f < 4; f++)
+            for (int m = 0; m < nmode; m++) {
+                if (f == 2 && m == 0) continue;
+                const int sys = ms[4 * m + (f < 2 ? 1 : f == 2 ? 2 : 3)];
+                for (int c = 0; c < 2; c++) cols.push_back({sys, (f * nmode + m) * 2 + c});
+            }
+        std::stable_sort(cols.begin(), cols.end());
+        std::vector<int> tasks((cols.size() + 63) / 64 * 64, -1);
+        for (size_t i = 0; i < cols.size(); i++) tasks[i] = cols[i].second;
+        if (S.np_tasks) hipFree(S.np_tasks);
+        HIPCHK(hipMalloc((void **)&S.np_tasks, sizeof(int) * tasks.size()));
+        HIPCHK(hipMemcpy(S.np_tasks, tasks.data(), sizeof(int) * tasks.size(), hipMemcpyHostToDevice));
+        S.np_ntasks = (int)tasks.size();
+        if (!s->cm) {
+            HIPCHK(hipMalloc((void **)&s->cm, 4 * sizeof(double) * (size_t)s->g.ntot));
+            s->dev_bytes += 4 * sizeof(double) * (long long)s->g.ntot;
+        }
+    }
     s->factored_bdfv = s->bdfv;
     s->factored_dt = s->P.dt;
     s->p_factored = true;
@@ -595,6 +672,23 @@ extern "C" int iw_create(const iw_params *p, iw_solver **out)
     for (int q = 0; q < 7; q++) HIPCHK(hipMalloc((void **)&s->prod[q], fb));
     HIPCHK(hipMalloc((void **)&s->ox, fb));
     HIPCHK(hipMalloc((void **)&s->hbuf, fb));
+    for (int q = 0; q < 4; q++) HIPCHK(hipMalloc((void **)&s->phys[q], fb));
+    {
+        const int nint = P.nsubd > 1 ? P.nsubd - 1 : 1, nl = P.nzloc;
+        HIPCHK(hipMalloc((void **)&s->icomp, 4 * sizeof(double) * 4 * (size_t)nint * g.nxppy));
+        std::vector<int> src, msrc, mdst;
+        for (int i = 0; i < P.nsubd - 1; i++) {
+            const int k = (i + 1) * nl; // first plane of subdomain i+1
+            src.push_back(k - 2); src.push_back(k - 1); src.push_back(k); src.push_back(k + 1);
+            msrc.push_back(4 * i + 1); msrc.push_back(4 * i + 2);
+            mdst.push_back(k - 1); mdst.push_back(k);
+        }
+        if (src.empty()) { src.push_back(0); msrc.push_back(0); mdst.push_back(0); }
+        if (upload(&s->iface_src, src, acc) || upload(&s->iface_mid_src, msrc, acc)
+            || upload(&s->iface_mid_dst, mdst, acc))
+            return -1;
+        acc += (long long)fb * 4 + 4LL * sizeof(double) * 4 * nint * g.nxppy;
+    }
     HIPCHK(hipMalloc((void **)&s->wnl, 2 * sizeof(double) * g.nxppy));
     HIPCHK(hipMalloc((void **)&s->wviscrot, 2 * sizeof(double) * g.nxppy));
     HIPCHK(hipMemset(s->wnl, 0, 2 * sizeof(double) * g.nxppy));
@@ -631,6 +725,11 @@ extern "C" int iw_destroy(iw_solver *s)
     for (int q = 0; q < 7; q++) hipFree(s->prod[q]);
     hipFree(s->ox);
     hipFree(s->hbuf);
+    for (int q = 0; q < 4; q++) hipFree(s->phys[q]);
+    hipFree(s->icomp);
+    hipFree(s->iface_src);
+    hipFree(s->iface_mid_src);
+    hipFree(s->iface_mid_dst);
     hipFree(s->wnl);
     hipFree(s->wviscrot);
     hipFree(s->d_count);
@@ -640,6 +739,10 @@ extern "C" int iw_destroy(iw_solver *s)
     hipFree(s->sys.coef);
     hipFree(s->sys.sysinfo);
     hipFree(s->sys.modesys);
+    hipFree(s->sys.order_p);
+    hipFree(s->sys.order_v);
+    hipFree(s->sys.np_tasks);
+    hipFree(s->cm);
     hipEventDestroy(s->ev0);
     hipEventDestroy(s->ev1);
     hipStreamDestroy(s->stream);
@@ -747,42 +850,77 @@ static FieldBatch batch(std::initializer_list<double *> l)
 {
     FieldBatch b;
     b.n = 0;
-    for (double *p : l) b.f[b.n++] = p;
+    b.smap = b.dmap = nullptr;
+    for (double *p : l) {
+        b.f[b.n] = p;
+        b.g[b.n] = p;
+        b.n++;
+    }
     return b;
 }
 
-static void horfft_dev(iw_solver *s, FieldBatch b, int is, int nk)
+// horfft (horfft.f:2-69) over nk logical planes of a batch: the first pass
+// reads b.f (through smap) and writes b.g (through dmap), the second pass
+// works in place on b.g.  is = -1: physical -> spectral.
+static void fft_dev(iw_solver *s, FieldBatch b, int is, int nk)
 {
-    if (b.n == 0) return;
+    if (b.n == 0 || nk == 0) return;
     const double bx = 16. * (double)s->g.nxppy * nk * b.n;         // each double read + written
     const double by = 32. * (double)s->g.nxh * s->g.ny * nk * b.n; // columns ii < nxh
+    FieldBatch b2 = b;
+    for (int q = 0; q < b.n; q++) b2.f[q] = b.g[q];
+    b2.smap = b.dmap;
+    const bool y = s->g.ndims == 3;
     if (is == -1) {
         {
             Scope sc(s, "fft_x_fwd", bx);
             launch_fft_x(s->g, b, nk, -1, s->stream);
         }
-        if (s->g.ndims == 3) {
+        if (y) {
             Scope sc(s, "fft_y_fwd", by);
-            launch_fft_y(s->g, b, nk, -1, s->stream);
+            launch_fft_y(s->g, b2, nk, -1, s->stream);
         }
     } else {
-        if (s->g.ndims == 3) {
-            Scope sc(s, "fft_y_inv", by);
-            launch_fft_y(s->g, b, nk, 1, s->stream);
+        if (y) {
+            {
+                Scope sc(s, "fft_y_inv", by);
+                launch_fft_y(s->g, b, nk, 1, s->stream);
+            }
+            Scope sc(s, "fft_x_inv", bx);
+            launch_fft_x(s->g, b2, nk, 1, s->stream);
+        } else {
+            Scope sc(s, "fft_x_inv", bx);
+            launch_fft_x(s->g, b, nk, 1, s->stream);
         }
-        Scope sc(s, "fft_x_inv", bx);
-        launch_fft_x(s->g, b, nk, 1, s->stream);
     }
 }
+
+static void horfft_dev(iw_solver *s, FieldBatch b, int is, int nk) { fft_dev(s, b, is, nk); }
 
 static void batch_fft(iw_solver *s, std::vector<double *> v, int is)
 {
     for (size_t i = 0; i < v.size(); i += kMaxFields) {
         FieldBatch b;
         b.n = 0;
-        for (size_t q = i; q < v.size() && q < i + kMaxFields; q++) b.f[b.n++] = v[q];
-        horfft_dev(s, b, is, s->g.nz);
+        b.smap = b.dmap = nullptr;
+        for (size_t q = i; q < v.size() && q < i + kMaxFields; q++) {
+            b.f[b.n] = v[q];
+            b.g[b.n] = v[q];
+            b.n++;
+        }
+        fft_dev(s, b, is, s->g.nz);
     }
+}
+
+// spectral effect of an inverse+forward horfft round trip on u,v,w,temp
+static void project_dev(iw_solver *s, double *const *f, int nf)
+{
+    FieldBatch b;
+    b.n = nf;
+    b.smap = b.dmap = nullptr;
+    for (int q = 0; q < nf; q++) b.f[q] = b.g[q] = f[q];
+    Scope sc(s, "project", 16. * (double)(s->g.ny + s->g.ny) * s->g.nz * nf);
+    launch_project(s->g, b, s->stream);
 }
 
 // convec (solver.f:6-514) on the device state
@@ -807,20 +945,30 @@ static int convec_dev(iw_solver *s, int ispecfilternl)
         Scope sc(s, "grad", 72. * nt);
         launch_grad(g, uvwt, gr, s->stream);
     }
-    batch_fft(s, {B[0], B[1], B[2], B[3], gr[0], gr[1], gr[2], gr[3], gr[4], gr[5], gr[6], gr[7], gr[8]}, 1);
+    // spectral-to-physical (solver.f:107-123); u,v,w,temp keep their spectral
+    // values and are transformed into separate physical buffers
+    {
+        FieldBatch b = batch({B[0], B[1], B[2], B[3], gr[0], gr[1], gr[2], gr[3], gr[4], gr[5], gr[6], gr[7],
+                              gr[8]});
+        for (int q = 0; q < 4; q++) b.g[q] = s->phys[q];
+        fft_dev(s, b, 1, g.nz);
+    }
+    double *ph[4] = {s->phys[0], s->phys[1], s->phys[2], s->phys[3]};
     {
         Scope sc(s, "dpdz_flux", 64. * nt);
         double *o4[4] = {S[0], S[1], S[2], s->ox};
-        launch_dpdz_conv(g, uvwt, o4, s->stream);
+        launch_dpdz_conv(g, ph, o4, s->stream);
     }
     {
         Scope sc(s, "rhs_phys", 224. * nt);
-        launch_rhs_phys(g, s->P, s->t, uvwt, gr, S, s->ox, s->prod, s->stream);
+        launch_rhs_phys(g, s->P, s->t, ph, gr, S, s->ox, s->prod, s->stream);
     }
     double **pr = s->prod;
-    batch_fft(s, {S[0], S[1], S[2], S[3], pr[0], pr[1], pr[2], pr[3], pr[4], pr[5], pr[6], B[3], B[0], B[1],
-                  B[2]},
-              -1);
+    batch_fft(s, {S[0], S[1], S[2], S[3], pr[0], pr[1], pr[2], pr[3], pr[4], pr[5], pr[6]}, -1);
+    // u,v,w,temp back to spectral space (solver.f:1180, 1201-1203): their
+    // physical values were not changed, so the round trip reduces to its
+    // spectral effect on the kx=0 and kx-Nyquist columns
+    project_dev(s, uvwt, 4);
     {
         Scope sc(s, "rhs_spec", 120. * nt);
         launch_rhs_spec(g, S, pr, s->stream);
@@ -859,29 +1007,75 @@ static int convec_dev(iw_solver *s, int ispecfilternl)
 static int calcuvwt_dev(iw_solver *s, int cpen)
 {
     if (ensure_factors(s)) return -1;
+    double *uvwt[4] = {s->buf[0], s->buf[1], s->buf[2], s->buf[3]};
+    // algorithmic bytes: state columns in and out + every distinct cached factor once
+    const double fcol = (double)s->sys.ldab * s->g.nz * 8. + s->g.nz * 4.;
+    const double nt = (double)s->g.ntot;
     {
-        // state columns in and out (64 B per point) + every cached factor once
-        double fb = (double)s->sys.nsys * ((double)s->sys.ldab * s->g.nz * 8. + s->g.nz * 4.);
-        Scope sc(s, "calc_mode", 64. * (double)s->g.ntot + fb);
-        double *uvwt[4] = {s->buf[0], s->buf[1], s->buf[2], s->buf[3]};
-        launch_calc_mode(s->g, s->P, s->sys, uvwt, s->wnl, s->wviscrot, s->P.dt, cpen, s->stream);
+        Scope sc(s, "pressure", 56. * nt + s->sys.np * fcol); // u,v,w in; u,v,w out; w re-read
+        launch_pressure(s->g, s->P, s->sys, uvwt, s->wnl, s->wviscrot, s->P.dt, cpen, s->stream);
+    }
+    if (s->sys.np_ok) {
+        {
+            Scope sc(s, "to_modemajor", 64. * nt);
+            launch_to_modemajor(s->g, s->P, s->sys, uvwt, s->cm, s->P.dt, s->stream);
+        }
+        {
+            // no-pivot factors: L multipliers and U band (kl + ku + 1 rows per column)
+            const double fnp = (double)(2 * s->sys.kl + 1) * s->g.nz * 8.;
+            Scope sc(s, "solve_np", 4 * 16. * nt + (s->sys.nsys - s->sys.np) * fnp);
+            launch_solve_np(s->g, s->sys, s->cm, s->stream);
+        }
+        {
+            Scope sc(s, "from_modemajor", 64. * nt);
+            launch_from_modemajor(s->g, s->cm, uvwt, cpen, s->stream);
+        }
+    } else {
+        Scope sc(s, "viscous", 64. * nt + (s->sys.nsys - s->sys.np) * fcol);
+        launch_viscous(s->g, s->P, s->sys, uvwt, s->P.dt, cpen, s->stream);
     }
     return hipGetLastError() == hipSuccess ? 0 : fail("calcuvwt launch failed");
 }
 
+// interface_avg3d (penalty.f:714-775): only the four planes km-1, km, k, k+1
+// around each interface are taken to physical space; after the averaging the
+// two modified planes are transformed back, and every other plane receives
+// the spectral effect of the round trip
 static int iface_dev(iw_solver *s, double *const *f, int nf)
 {
+    const int nint = s->g.nsubd - 1;
     launch_zero_int(s->d_count, 4, s->stream);
-    std::vector<double *> v(f, f + nf);
-    batch_fft(s, v, 1);
+    if (nint <= 0) {
+        project_dev(s, f, nf);
+        return 0;
+    }
+    FieldBatch bi;
+    bi.n = nf;
+    bi.smap = s->iface_src;
+    bi.dmap = nullptr;
+    for (int q = 0; q < nf; q++) {
+        bi.f[q] = f[q];
+        bi.g[q] = s->icomp + (size_t)q * 4 * nint * s->g.nxppy;
+    }
+    fft_dev(s, bi, 1, 4 * nint);
     {
-        Scope sc(s, "iface", 48. * (double)s->g.nxppy * (s->g.nsubd - 1) * nf);
+        Scope sc(s, "iface", 48. * (double)s->g.nxppy * nint * nf);
         FieldBatch b;
         b.n = nf;
-        for (int q = 0; q < nf; q++) b.f[q] = f[q];
+        b.smap = b.dmap = nullptr;
+        for (int q = 0; q < nf; q++) b.f[q] = b.g[q] = bi.g[q];
         launch_iface(s->g, b, s->d_count, s->stream);
     }
-    batch_fft(s, v, -1);
+    project_dev(s, f, nf);
+    FieldBatch bf;
+    bf.n = nf;
+    bf.smap = s->iface_mid_src;
+    bf.dmap = s->iface_mid_dst;
+    for (int q = 0; q < nf; q++) {
+        bf.f[q] = bi.g[q];
+        bf.g[q] = f[q];
+    }
+    fft_dev(s, bf, -1, 2 * nint);
     return 0;
 }
 
@@ -912,9 +1106,9 @@ extern "C" int iw_run(iw_solver *s, int nsteps, int *counts)
     if (!s || nsteps < 0) return fail("iw_run: bad argument");
     for (int n = 0; n < nsteps; n++) {
         int istep = ++s->istep;
-        // maxcalc round trip (various.f:739-742, 854-857)
-        batch_fft(s, {s->buf[0], s->buf[1], s->buf[2], s->buf[3]}, 1);
-        batch_fft(s, {s->buf[0], s->buf[1], s->buf[2], s->buf[3]}, -1);
+        // maxcalc's horfft round trip (various.f:739-742, 854-857) changes the
+        // spectrum only through the kx=0 and kx-Nyquist columns
+        project_dev(s, s->buf, 4);
         if (convec_dev(s, s->P.ispecfilternl)) return -1;
         if (istep < 2) { // main.f:700-777
             s->fac[0] = 2.; s->fac[1] = -1.; s->fac[2] = 0.;

@@ -12,10 +12,14 @@ namespace iw {
 constexpr int kMaxNl = 32;     // GLL points per subdomain supported by the kernels
 constexpr int kMaxFields = 16; // fields per batched transform launch
 
-// Field pointer batch passed by value to batched kernels.
+// Field pointer batch passed by value to batched kernels: source and
+// destination of each field (equal for an in-place transform) and optional
+// plane maps (logical plane -> stored plane index; null = identity).
 struct FieldBatch {
     double *f[kMaxFields];
+    double *g[kMaxFields];
     int n;
+    const int *smap, *dmap;
 };
 
 // Problem geometry and device-resident tables.
@@ -33,12 +37,22 @@ struct Geo {
 
 struct SysTables {
     int nsys = 0;
+    int np = 0; // pressure systems occupy ids 0..np-1
     int ldab = 0, kl = 0, kv = 0;
     double *AB = nullptr;   // nsys * ldab * nz, LAPACK band columns
     int *ipiv = nullptr;    // nsys * nz (0-based)
     double *coef = nullptr; // nsys * nsubd * 12 assembly coefficients
     double *sysinfo = nullptr; // nsys * 4: eps, tau1(bottom), tau2(top), bc flags
     int *modesys = nullptr; // nmode * 4: P, UV, W, T system ids (-1: none)
+    // block -> task tables: tasks that share a factor are placed on one XCD
+    // (blocks b, b+8, b+16, ...) next to each other so the factor streams
+    // from that XCD's L2 once for the whole group
+    int *order_p = nullptr, *order_v = nullptr;
+    int nblk_p = 0, nblk_v = 0;
+    // lanes-as-columns path for systems without row interchanges
+    int *np_tasks = nullptr;
+    int np_ntasks = 0;
+    bool np_ok = false;
     size_t cap_sys = 0;
 };
 
@@ -63,6 +77,10 @@ struct iw_solver {
     double *ox;
     double *wnl, *wviscrot;
     double *hbuf;      // device scratch for host-array entry points
+    double *cm = nullptr; // mode-major right-hand sides of the u,v,w,rho solves
+    double *phys[4];      // physical u,v,w,rho during convec
+    double *icomp;        // 4 planes per interface x 4 fields (interface averaging)
+    int *iface_src, *iface_mid_src, *iface_mid_dst; // plane maps
     int *d_count;      // 4 interface counters
     int *h_count;      // pinned
     // time stepping
@@ -101,10 +119,17 @@ void launch_bdf_filter(const Geo &g, const iw_params &P, double *const *S3, doub
                        hipStream_t st);
 void launch_filter3d(const Geo &g, double *f, int ifourier, int ilegen, hipStream_t st);
 void launch_iface(const Geo &g, FieldBatch b, int *count, hipStream_t st);
+void launch_project(const Geo &g, FieldBatch b, hipStream_t st);
 void launch_factor(const Geo &g, const SysTables &S, int first, int count, hipStream_t st);
-void launch_calc_mode(const Geo &g, const iw_params &P, const SysTables &S, double *const *uvwt,
-                      const double *wnl, const double *wviscrot, double dt, int cpen,
-                      hipStream_t st);
+void launch_pressure(const Geo &g, const iw_params &P, const SysTables &S, double *const *uvwt,
+                     const double *wnl, const double *wviscrot, double dt, int cpen, hipStream_t st);
+void launch_viscous(const Geo &g, const iw_params &P, const SysTables &S, double *const *uvwt, double dt,
+                    int cpen, hipStream_t st);
 void launch_zero_meanw(const Geo &g, double *w, hipStream_t st);
 void launch_zero_int(int *p, int n, hipStream_t st);
+bool solve_np_supported(int nl, int nsubd);
+void launch_solve_np(const Geo &g, const SysTables &S, double *cm, hipStream_t st);
+void launch_to_modemajor(const Geo &g, const iw_params &P, const SysTables &S, double *const *uvwt, double *cm,
+                         double dt, hipStream_t st);
+void launch_from_modemajor(const Geo &g, const double *cm, double *const *uvwt, int cpen, hipStream_t st);
 } // namespace iw

@@ -55,21 +55,31 @@ __device__ double2 *stockham(double2 *in, double2 *out, int M, int logM, int nro
 // even/odd-packed row with LDS-staged radix-2 passes.
 // ---------------------------------------------------------------------------
 template <int DIR>
-__global__ __launch_bounds__(256) void k_fft_x(FieldBatch fb, int nrows, int nxpp, int M,
+__global__ __launch_bounds__(256) void k_fft_x(FieldBatch fb, int nrows, int ny, int nxpp, int M,
                                                int logM, int R, const double2 *__restrict__ twM,
                                                const double2 *__restrict__ twN, double scaley)
 {
     extern __shared__ double2 smx[];
     double2 *b0 = smx, *b1 = smx + R * M;
-    double *base = fb.f[blockIdx.y];
+    const double *base = fb.f[blockIdx.y];
+    double *obase = fb.g[blockIdx.y];
     const int row0 = blockIdx.x * R;
     const int N = 2 * M;
+    // logical row -> stored row of the source / destination (plane maps)
+    auto srow = [&](int row) -> long {
+        int lp = row / ny, j = row - lp * ny;
+        return (long)((fb.smap ? fb.smap[lp] : lp) * ny + j) * nxpp;
+    };
+    auto drow = [&](int row) -> long {
+        int lp = row / ny, j = row - lp * ny;
+        return (long)((fb.dmap ? fb.dmap[lp] : lp) * ny + j) * nxpp;
+    };
     if (DIR < 0) {
         for (int idx = threadIdx.x; idx < R * M; idx += blockDim.x) {
             int r = idx / M, m = idx - r * M, row = row0 + r;
             double2 z = make_double2(0., 0.);
             if (row < nrows) {
-                const double *p = base + (long)row * nxpp + 2 * m;
+                const double *p = base + srow(row) + 2 * m;
                 z = make_double2(p[0], p[1]);
             }
             b0[idx] = z;
@@ -79,7 +89,7 @@ __global__ __launch_bounds__(256) void k_fft_x(FieldBatch fb, int nrows, int nxp
             int r = idx / M, k = idx - r * M, row = row0 + r;
             double2 Z = make_double2(0., 0.);
             if (row < nrows) {
-                const double *p = base + (long)row * nxpp;
+                const double *p = base + srow(row);
                 double ar, ai, br, bi;
                 if (k == 0) {
                     ar = p[0]; ai = 0.; br = p[2 * M]; bi = 0.;
@@ -115,7 +125,7 @@ __global__ __launch_bounds__(256) void k_fft_x(FieldBatch fb, int nrows, int nxp
             double A = sc * xr, B = sc * xi;
             if (k == 0 || k == M) B = 0.0;
             if (k == M) A = A * scaley; // horfft.f:28-30 on the untransformed Nyquist column
-            double *p = base + (long)row * nxpp;
+            double *p = obase + drow(row);
             p[2 * k] = A;
             p[2 * k + 1] = B;
         }
@@ -123,14 +133,14 @@ __global__ __launch_bounds__(256) void k_fft_x(FieldBatch fb, int nrows, int nxp
         for (int idx = threadIdx.x; idx < R * M; idx += blockDim.x) {
             int r = idx / M, m = idx - r * M, row = row0 + r;
             if (row >= nrows) continue;
-            double *p = base + (long)row * nxpp;
+            double *p = obase + drow(row);
             p[2 * m] = res[idx].x;
             p[2 * m + 1] = res[idx].y;
         }
         for (int r = threadIdx.x; r < R; r += blockDim.x) {
             int row = row0 + r;
             if (row < nrows) {
-                double *p = base + (long)row * nxpp;
+                double *p = obase + drow(row);
                 p[N] = 0.;
                 p[N + 1] = 0.;
             }
@@ -149,7 +159,9 @@ __global__ __launch_bounds__(256) void k_fft_y(FieldBatch fb, int nxpp, int nxh,
 {
     extern __shared__ double2 smy[];
     double2 *b0 = smy, *b1 = smy + CB * ny;
-    double *plane = fb.f[blockIdx.z] + (long)blockIdx.y * nxpp * ny;
+    const int lp = blockIdx.y;
+    const double *plane = fb.f[blockIdx.z] + (long)(fb.smap ? fb.smap[lp] : lp) * nxpp * ny;
+    double *oplane = fb.g[blockIdx.z] + (long)(fb.dmap ? fb.dmap[lp] : lp) * nxpp * ny;
     const int ii0 = blockIdx.x * CB;
     for (int idx = threadIdx.x; idx < CB * ny; idx += blockDim.x) {
         int j = idx / CB, c = idx - j * CB, ii = ii0 + c;
@@ -170,10 +182,18 @@ __global__ __launch_bounds__(256) void k_fft_y(FieldBatch fb, int nxpp, int nxh,
             z.x = scaley * z.x;
             z.y = scaley * z.y;
         }
-        double *p = plane + (long)j * nxpp + 2 * ii;
+        double *p = oplane + (long)j * nxpp + 2 * ii;
         p[0] = z.x;
         p[1] = z.y;
     }
+    // out of place: the untransformed kx-Nyquist column travels unchanged
+    if (blockIdx.x == 0 && plane != oplane)
+        for (int j = threadIdx.x; j < ny; j += blockDim.x) {
+            const double *p = plane + (long)j * nxpp + 2 * nxh;
+            double *q = oplane + (long)j * nxpp + 2 * nxh;
+            q[0] = p[0];
+            q[1] = p[1];
+        }
 }
 
 void launch_fft_x(const Geo &g, FieldBatch b, int nk, int dir, hipStream_t st)
@@ -184,10 +204,10 @@ void launch_fft_x(const Geo &g, FieldBatch b, int nk, int dir, hipStream_t st)
     dim3 grid((nrows + R - 1) / R, b.n);
     size_t sh = 2 * (size_t)R * g.M * sizeof(double2);
     if (dir < 0)
-        hipLaunchKernelGGL(k_fft_x<-1>, grid, dim3(256), sh, st, b, nrows, g.nxpp, g.M, g.logM, R,
+        hipLaunchKernelGGL(k_fft_x<-1>, grid, dim3(256), sh, st, b, nrows, g.ny, g.nxpp, g.M, g.logM, R,
                            g.twM, g.twN, g.scaley);
     else
-        hipLaunchKernelGGL(k_fft_x<1>, grid, dim3(256), sh, st, b, nrows, g.nxpp, g.M, g.logM, R,
+        hipLaunchKernelGGL(k_fft_x<1>, grid, dim3(256), sh, st, b, nrows, g.ny, g.nxpp, g.M, g.logM, R,
                            g.twM, g.twN, g.scaley);
 }
 
@@ -697,15 +717,18 @@ void launch_filter3d(const Geo &g, double *f, int ifourier, int ilegen, hipStrea
 // interface_avg3d physical-space pass (penalty.f:729-772): one thread per
 // (i,j) point and interface; counts by wavefront ballot + one atomic.
 // ---------------------------------------------------------------------------
+// The field holds, per interface, the four physical planes km-1, km, k, k+1
+// (compact: interface i at planes 4i..4i+3).
 __global__ __launch_bounds__(256) void k_iface(FieldBatch fb, int *count, long nxppy, int nl, int nint)
 {
     long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
     double *f = fb.f[blockIdx.y];
     bool hit = false;
+    (void)nl;
     if (id < nxppy * nint) {
-        int ks = 1 + (int)(id / nxppy);
+        int it = (int)(id / nxppy);
         long ij = id % nxppy;
-        long k = (long)ks * nl, km = (long)(ks - 1) * nl + nl - 1;
+        long k = 4L * it + 2, km = 4L * it + 1;
         double fsp = f[k * nxppy + ij], fsm = f[km * nxppy + ij];
         double favg2 = fsp + fsm;
         double c0 = fabs(fsp - fsm) / fabs(favg2);
@@ -783,7 +806,8 @@ __device__ __forceinline__ double assemble(int r, int c, int nl, int ns, const d
 
 __global__ __launch_bounds__(64) void k_factor(int first, int n, int nl, int ns, int ldab,
                                                const double *__restrict__ coef, const double *__restrict__ D,
-                                               const double *__restrict__ D2, double *AB, int *ipiv)
+                                               const double *__restrict__ D2, double *AB, int *ipiv,
+                                               double *sysinfo)
 {
     extern __shared__ double shf[];
     double *Wm = shf;                    // kWR * kWS
@@ -806,6 +830,7 @@ __global__ __launch_bounds__(64) void k_factor(int first, int n, int nl, int ns,
         if (lane <= kv) WM(r, lane) = (lane < n) ? assemble(r, lane, nl, ns, cf, Dl, D2l) : 0.;
     __syncthreads();
     int ju = 0;
+    bool swapped = false;
     for (int j = 0; j < n; j++) {
         const int km = (L < n - 1 - j) ? L : n - 1 - j;
         double av = (lane <= km) ? fabs(WM(j + lane, j)) : -1.0;
@@ -819,6 +844,7 @@ __global__ __launch_bounds__(64) void k_factor(int first, int n, int nl, int ns,
             }
         }
         const int p = ai;
+        swapped = swapped || (p != 0);
         if (lane == 0) ip[j] = j + p;
         const double piv = WM(j + p, j);
         __syncthreads();
@@ -857,6 +883,7 @@ __global__ __launch_bounds__(64) void k_factor(int first, int n, int nl, int ns,
         }
         __syncthreads();
     }
+    if (lane == 0) sysinfo[4L * sys + 3] = swapped ? 1. : 0.;
 #undef WM
 }
 
@@ -865,312 +892,294 @@ void launch_factor(const Geo &g, const SysTables &S, int first, int count, hipSt
     if (count <= 0) return;
     size_t sh = sizeof(double) * ((size_t)kWR * kWS + 2 * g.nzloc * g.nzloc + 12 * g.nsubd);
     hipLaunchKernelGGL(k_factor, dim3(count), dim3(64), sh, st, first, g.nz, g.nzloc, g.nsubd, S.ldab,
-                       S.coef, g.D, g.D2, S.AB, S.ipiv);
+                       S.coef, g.D, g.D2, S.AB, S.ipiv, S.sysinfo);
 }
 
 // ---------------------------------------------------------------------------
-// Implicit step for one Fourier mode (calcuvwt.f:87-741 with update,
-// calcuvwt.f:826-901): pressure Poisson, projection, u/v (shared factor),
-// w and density Helmholtz solves, each through SGBTRS (bandludcmp.f:883-1069)
-// on the cached factor, Legendre filter (filter_legenpen, filtering.f:729-808)
-// and mean-u damping.  One wavefront per mode; columns staged in LDS; the
-// band window of the forward/backward sweeps rides in the lanes.
+// Implicit step (calcuvwt.f:87-741 with update, calcuvwt.f:826-901) in two
+// kernels over the cached factors:
+//   k_pressure: one wavefront per mode -- divergence (divcolc, derv.f:35-85),
+//     pressure Poisson solve with the wnl+wviscrot Neumann data, Legendre
+//     filter, dphi/dz and the projection u* - dt grad(phi), written in place;
+//   k_viscous: one wavefront per (mode, field) -- u, v, w and density
+//     Helmholtz solves, filter, mean-u damping.
+// Each solve is SGBTRS (bandludcmp.f:883-1069): the band window of the
+// forward and backward sweeps rides in the lanes, the right-hand side column
+// sits in LDS, and factor columns are streamed from HBM through a register
+// ring kPF columns ahead of use.
 // ---------------------------------------------------------------------------
-constexpr int kCH = 16; // factor columns staged per chunk
+constexpr int kPF = 8;
+
+__device__ __forceinline__ double rdl(double v, int l)
+{
+    long long b = __double_as_longlong(v);
+    int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), l);
+    int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
 
 struct ModeArgs {
     double *u, *v, *w, *t;
     const double *wnl, *wviscrot;
     const double *AB;
     const int *ipiv;
-    const double *sysinfo; // eps, tau1 bottom, tau2 top, unused
+    const double *sysinfo; // eps, tau1 bottom, tau2 top, bc code
     const int *modesys;
+    const int *order; // block -> task (mode, or 4*mode+field); -1 = idle
     const double *D, *AL, *wg, *sig, *zh, *xw, *yw, *damp;
     double dt, xnu, xkappa;
     long nxppy;
     int nz, nl, ns, nxhp, ldab, kl, kv, cpen, noslip;
 };
 
-template <int NR>
-__device__ void band_solve(const ModeArgs &A, int sys, double *const *b, double *fch, int *ich)
+// Solve A x = b for the two columns b0, b1 (LDS) of one system; x overwrites b.
+// Factor columns are streamed from global memory through a register ring kPF
+// columns ahead of use (each lane holds its own row of the window).
+__device__ void band_solve2(const ModeArgs &A, int sys, double *b0, double *b1)
 {
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x & 63;
     const int n = A.nz, kl = A.kl, kv = A.kv, ldab = A.ldab;
-    const double *ab = A.AB + (long)sys * ldab * n;
-    const int *ip = A.ipiv + (long)sys * n;
-    double v[NR];
+    const double *__restrict__ ab = A.AB + (long)sys * ldab * n;
+    const int *__restrict__ ip = A.ipiv + (long)sys * n;
     // forward: L with row interchanges (bandludcmp.f:1020-1030)
+    {
+        const bool lact = lane >= 1 && lane <= kl;
+        double v0 = (lane <= kl && lane < n) ? b0[lane] : 0.;
+        double v1 = (lane <= kl && lane < n) ? b1[lane] : 0.;
+        double ring[kPF];
 #pragma unroll
-    for (int r = 0; r < NR; r++) v[r] = (lane <= kl && lane < n) ? b[r][lane] : 0.;
-    for (int j = 0; j < n; j++) {
-        const int c = j % kCH;
-        if (c == 0) {
-            __syncthreads();
-            for (int e = lane; e < kCH * kl; e += 64) {
-                int cc = e / kl, t = e - cc * kl;
-                if (j + cc < n) fch[cc * kl + t] = ab[(kv + 1 + t) + (long)ldab * (j + cc)];
-            }
-            for (int e = lane; e < kCH; e += 64)
-                if (j + e < n) ich[e] = ip[j + e];
-            __syncthreads();
-        }
-        if (j < n - 1) {
-            const int p = ich[c] - j;
-            const int lm = (kl < n - 1 - j) ? kl : n - 1 - j;
+        for (int q = 0; q < kPF; q++) ring[q] = (lact && q < n) ? ab[(kv + lane) + (long)ldab * q] : 0.;
+        int ipv = (lane < n) ? ip[lane] : 0;
+        for (int jb = 0; jb < n; jb += kPF) {
 #pragma unroll
-            for (int r = 0; r < NR; r++) {
-                if (p != 0) {
-                    double x0 = __shfl(v[r], 0), xp = __shfl(v[r], p);
-                    if (lane == 0) v[r] = xp;
-                    else if (lane == p) v[r] = x0;
+            for (int q = 0; q < kPF; q++) {
+                const int j = jb + q;
+                if (j >= n) break;
+                if (q == 0 && jb > 0 && (jb & 63) == 0) ipv = (j + lane < n) ? ip[j + lane] : 0;
+                const int lm = (kl < n - 1 - j) ? kl : n - 1 - j;
+                const double lt = (lane <= lm) ? ring[q] : 0.;
+                if (lact && j + kPF < n) ring[q] = ab[(kv + lane) + (long)ldab * (j + kPF)];
+                const int nj = (j + kl + 1 < n) ? j + kl + 1 : n - 1;
+                const double nb0 = b0[nj], nb1 = b1[nj];
+                const int p = __builtin_amdgcn_readlane(ipv, j & 63) - j;
+                const double x00 = rdl(v0, 0), x01 = rdl(v1, 0);
+                const double bj0 = rdl(v0, p), bj1 = rdl(v1, p);
+                if (lane == p) {
+                    v0 = x00;
+                    v1 = x01;
                 }
-                double bj = __shfl(v[r], 0);
-                if (lane >= 1 && lane <= lm) v[r] = v[r] + fch[c * kl + lane - 1] * (-bj);
+                v0 = v0 + lt * (-bj0);
+                v1 = v1 + lt * (-bj1);
+                if (lane == 0) {
+                    b0[j] = bj0;
+                    b1[j] = bj1;
+                }
+                v0 = __shfl_down(v0, 1);
+                v1 = __shfl_down(v1, 1);
+                if (lane == kl) {
+                    v0 = (j + kl + 1 < n) ? nb0 : 0.;
+                    v1 = (j + kl + 1 < n) ? nb1 : 0.;
+                }
             }
-        }
-#pragma unroll
-        for (int r = 0; r < NR; r++) {
-            if (lane == 0) b[r][j] = v[r];
-            double nv = __shfl_down(v[r], 1);
-            if (lane == kl) nv = (j + kl + 1 < n) ? b[r][j + kl + 1] : 0.;
-            v[r] = nv;
         }
     }
     __syncthreads();
-    // backward: upper band triangle with kv superdiagonals (STBSV, ludcmp.f:4698-5046)
+    // backward: upper triangle with kv superdiagonals (STBSV, ludcmp.f:4698-5046)
+    {
+        const bool uact = lane <= kv;
+        double v0 = (uact && n - 1 - lane >= 0) ? b0[n - 1 - lane] : 0.;
+        double v1 = (uact && n - 1 - lane >= 0) ? b1[n - 1 - lane] : 0.;
+        double ring[kPF];
 #pragma unroll
-    for (int r = 0; r < NR; r++) v[r] = (lane <= kv && n - 1 - lane >= 0) ? b[r][n - 1 - lane] : 0.;
-    const int W = kv + 1;
-    for (int j = n - 1; j >= 0; j--) {
-        const int c = (n - 1 - j) % kCH;
-        if (c == 0) {
-            __syncthreads();
-            for (int e = lane; e < kCH * W; e += 64) {
-                int cc = e / W, t = e - cc * W;
-                if (j - cc >= 0) fch[cc * W + t] = ab[t + (long)ldab * (j - cc)];
-            }
-            __syncthreads();
-        }
-        const double diag = fch[c * W + kv];
+        for (int q = 0; q < kPF; q++)
+            ring[q] = (uact && n - 1 - q >= 0) ? ab[(kv - lane) + (long)ldab * (n - 1 - q)] : 0.;
+        for (int jb = 0; jb < n; jb += kPF) {
 #pragma unroll
-        for (int r = 0; r < NR; r++) {
-            double bj = __shfl(v[r], 0);
-            double x = (bj != 0.) ? bj / diag : bj;
-            if (lane == 0) {
-                v[r] = x;
-                b[r][j] = x;
-            } else if (lane <= kv && lane <= j) {
-                v[r] = v[r] - x * fch[c * W + kv - lane];
+            for (int q = 0; q < kPF; q++) {
+                const int j = n - 1 - (jb + q);
+                if (j < 0) break;
+                const double uc0 = ring[q];
+                if (uact && j - kPF >= 0) ring[q] = ab[(kv - lane) + (long)ldab * (j - kPF)];
+                const int lu = (kv < j) ? kv : j;
+                const double uc = (lane >= 1 && lane <= lu) ? uc0 : 0.;
+                const int nj = (j - kv - 1 >= 0) ? j - kv - 1 : 0;
+                const double nb0 = b0[nj], nb1 = b1[nj];
+                const double diag = rdl(uc0, 0);
+                const double bj0 = rdl(v0, 0), bj1 = rdl(v1, 0);
+                const double x0 = (bj0 != 0.) ? bj0 / diag : bj0;
+                const double x1 = (bj1 != 0.) ? bj1 / diag : bj1;
+                v0 = v0 - x0 * uc;
+                v1 = v1 - x1 * uc;
+                if (lane == 0) {
+                    b0[j] = x0;
+                    b1[j] = x1;
+                }
+                v0 = __shfl_down(v0, 1);
+                v1 = __shfl_down(v1, 1);
+                if (lane == kv) {
+                    v0 = (j - kv - 1 >= 0) ? nb0 : 0.;
+                    v1 = (j - kv - 1 >= 0) ? nb1 : 0.;
+                }
             }
-            double nv = __shfl_down(v[r], 1);
-            if (lane == kv) nv = (j - 1 - kv >= 0) ? b[r][j - 1 - kv] : 0.;
-            v[r] = nv;
         }
     }
     __syncthreads();
 }
 
-// Legendre filter of NC columns of nz values held in LDS, subdomain by subdomain
-__device__ void filter_cols(const ModeArgs &A, double *const *col, int NC, double *tmp)
+// Legendre filter (filter_legenpen, filtering.f:729-808) of two columns of
+// nz values in LDS, subdomain by subdomain (lanes 0-31 real, 32-63 imaginary)
+__device__ void filter_pair(const ModeArgs &A, double *c0, double *c1, double *tmp)
 {
-    const int lane = threadIdx.x, nl = A.nl, nzm = nl - 1;
-    for (int q = 0; q < NC; q += 2) {
-        const int nq = (NC - q) < 2 ? (NC - q) : 2;
-        for (int ks = 0; ks < A.ns; ks++) {
-            const int comp = lane >> 5, k = lane & 31;
-            if (comp < nq && k < nl) {
-                const double *f = col[q + comp] + ks * nl;
-                double sum = 0.;
-                for (int t = 0; t < nl; t++) sum = sum + A.wg[t] * f[t] * A.AL[k * nl + t];
-                double a = (k == nzm) ? 0.5 * nzm * sum : (k + 0.5) * sum;
-                tmp[comp * 32 + k] = a * A.sig[k];
-            }
-            __syncthreads();
-            if (comp < nq && k < nl) {
-                double sum = 0.;
-                for (int kk = 0; kk < nl; kk++) sum = sum + tmp[comp * 32 + kk] * A.AL[kk * nl + k];
-                col[q + comp][ks * nl + k] = sum;
-            }
-            __syncthreads();
+    const int lane = threadIdx.x & 63, nl = A.nl, nzm = nl - 1;
+    const int comp = lane >> 5, k = lane & 31;
+    double *col = comp ? c1 : c0;
+    for (int ks = 0; ks < A.ns; ks++) {
+        if (k < nl) {
+            const double *f = col + ks * nl;
+            double sum = 0.;
+            for (int t = 0; t < nl; t++) sum = sum + A.wg[t] * f[t] * A.AL[k * nl + t];
+            double a = (k == nzm) ? 0.5 * nzm * sum : (k + 0.5) * sum;
+            tmp[comp * 32 + k] = a * A.sig[k];
         }
+        __syncthreads();
+        if (k < nl) {
+            double sum = 0.;
+            for (int kk = 0; kk < nl; kk++) sum = sum + tmp[comp * 32 + kk] * A.AL[kk * nl + k];
+            col[ks * nl + k] = sum;
+        }
+        __syncthreads();
     }
 }
 
-__global__ __launch_bounds__(64) void k_calc_mode(ModeArgs A)
+__global__ __launch_bounds__(64) void k_pressure(ModeArgs A)
 {
     extern __shared__ double shm[];
     const int nz = A.nz, nl = A.nl;
-    double *wr = shm, *wi = wr + nz, *rr = wi + nz, *ri = rr + nz, *ar = ri + nz, *ai = ar + nz;
-    double *ubr = ar, *ubi = ai, *vbr = ai + nz, *vbi = vbr + nz; // ub reuses the dphidz columns
-    double *tmp = vbi + nz;              // 64
-    double *fch = tmp + 64;              // kCH * (kv+1)
-    int *ich = (int *)(fch + kCH * (A.kv + 1));
-    const int m = blockIdx.x;
-    const int lane = threadIdx.x;
+    double *rr = shm, *ri = rr + nz, *tmp = ri + nz;
+    const int task = A.order[blockIdx.x];
+    if (task < 0) return;
+    const int m = task, lane = threadIdx.x;
     const int ii = m % A.nxhp, j = m / A.nxhp;
     const long o = 2L * m;
     const bool zero = (m == 0);
     const double xw = A.xw[ii], yw = A.yw[j];
-    const int sP = A.modesys[4 * m], sUV = A.modesys[4 * m + 1];
-    const int sW = A.modesys[4 * m + 2], sT = A.modesys[4 * m + 3];
-    for (int k = lane; k < nz; k += 64) {
-        wr[k] = A.w[(long)k * A.nxppy + o];
-        wi[k] = A.w[(long)k * A.nxppy + o + 1];
-    }
-    __syncthreads();
-    // ---- pressure (calcuvwt.f:87-222) ----
     if (!zero) {
+        const int sP = A.modesys[4 * m];
         const double eps = A.sysinfo[4 * sP];
-        for (int k = lane; k < nz; k += 64) {
-            int ks = k / nl, k1 = k - ks * nl;
-            double fac = 2. / A.zh[ks];
+        for (int k = lane; k < nz; k += 64) { // divergence, calcuvwt.f:112-136
+            const int ks = k / nl, k1 = k - ks * nl;
+            const double fac = 2. / A.zh[ks];
             double sr = 0., si = 0.;
             for (int t = 0; t < nl; t++) {
-                double dd = A.D[k1 * nl + t];
-                sr = sr + dd * wr[ks * nl + t];
-                si = si + dd * wi[ks * nl + t];
+                const double dd = A.D[k1 * nl + t];
+                const long nw = (long)(ks * nl + t) * A.nxppy + o;
+                sr = sr + dd * A.w[nw];
+                si = si + dd * A.w[nw + 1];
             }
-            double dfr = sr * fac, dfi = si * fac;
-            long n = (long)k * A.nxppy + o;
+            const long n = (long)k * A.nxppy + o;
             double dr = (0. + (-xw * A.u[n + 1])) + (-yw * A.v[n + 1]);
             double di = (0. + xw * A.u[n]) + yw * A.v[n];
-            dr = dr + dfr;
-            di = di + dfi;
+            dr = dr + sr * fac;
+            di = di + si * fac;
             rr[k] = -(dr / A.dt) * eps; // penalty.f:191-194
             ri[k] = -(di / A.dt) * eps;
         }
         __syncthreads();
-        if (lane == 0) { // Neumann boundary data (penalty.f:583-598)
-            double t1 = A.sysinfo[4 * sP + 1], t2 = A.sysinfo[4 * sP + 2];
-            long nb = A.nxppy;
-            double bcrtop = A.wnl[nb + o] + A.wviscrot[nb + o];
-            double bcrbot = A.wnl[o] + A.wviscrot[o];
-            double bcitop = A.wnl[nb + o + 1] + A.wviscrot[nb + o + 1];
-            double bcibot = A.wnl[o + 1] + A.wviscrot[o + 1];
+        if (lane == 0) { // Neumann data (calcuvwt.f:154-159, penalty.f:583-598)
+            const double t1 = A.sysinfo[4 * sP + 1], t2 = A.sysinfo[4 * sP + 2];
+            const long nb = A.nxppy;
+            const double bcrtop = A.wnl[nb + o] + A.wviscrot[nb + o];
+            const double bcrbot = A.wnl[o] + A.wviscrot[o];
+            const double bcitop = A.wnl[nb + o + 1] + A.wviscrot[nb + o + 1];
+            const double bcibot = A.wnl[o + 1] + A.wviscrot[o + 1];
             rr[0] = rr[0] - t1 * bcrbot * eps;
             ri[0] = ri[0] - t1 * bcibot * eps;
             rr[nz - 1] = rr[nz - 1] + t2 * bcrtop * eps;
             ri[nz - 1] = ri[nz - 1] + t2 * bcitop * eps;
         }
         __syncthreads();
-        double *bp[2] = {rr, ri};
-        band_solve<2>(A, sP, bp, fch, ich);
-        if (A.cpen) filter_cols(A, bp, 2, tmp);
-        for (int k = lane; k < nz; k += 64) { // dphidz (calcuvwt.f:853)
-            int ks = k / nl, k1 = k - ks * nl;
-            double fac = 2. / A.zh[ks];
+        band_solve2(A, sP, rr, ri);
+        if (A.cpen) filter_pair(A, rr, ri, tmp);
+    }
+    // projection (update, calcuvwt.f:826-901), in place
+    const double cx = A.dt * xw, cy = A.dt * yw;
+    const int k0 = A.noslip ? 1 : 0;
+    for (int k = lane; k < nz; k += 64) {
+        const long n = (long)k * A.nxppy + o;
+        if (k < k0) { // rows the no-slip variant never assigns (calcuvwt_noslip.f:851)
+            A.u[n] = 0.; A.u[n + 1] = 0.;
+            A.v[n] = 0.; A.v[n + 1] = 0.;
+            A.w[n] = 0.; A.w[n + 1] = 0.;
+            continue;
+        }
+        double tr = 0., ti = 0., dzr = 0., dzi = 0.;
+        if (!zero) {
+            tr = rr[k];
+            ti = ri[k];
+            const int ks = k / nl, k1 = k - ks * nl;
+            const double fac = 2. / A.zh[ks];
             double sr = 0., si = 0.;
             for (int t = 0; t < nl; t++) {
-                double dd = A.D[k1 * nl + t];
+                const double dd = A.D[k1 * nl + t];
                 sr = sr + dd * rr[ks * nl + t];
                 si = si + dd * ri[ks * nl + t];
             }
-            ar[k] = sr * fac;
-            ai[k] = si * fac;
+            dzr = sr * fac;
+            dzi = si * fac;
         }
-    } else {
-        for (int k = lane; k < nz; k += 64) {
-            rr[k] = 0.; ri[k] = 0.; ar[k] = 0.; ai[k] = 0.;
-        }
-    }
-    __syncthreads();
-    // ---- projection (update, calcuvwt.f:826-901) ----
-    {
-        const double cx = A.dt * xw, cy = A.dt * yw;
-        const int k0 = A.noslip ? 1 : 0;
-        for (int k = lane; k < nz; k += 64) {
-            long n = (long)k * A.nxppy + o;
-            if (k < k0) {
-                ubr[k] = 0.; ubi[k] = 0.; vbr[k] = 0.; vbi[k] = 0.; wr[k] = 0.; wi[k] = 0.;
-                continue;
-            }
-            double tr = rr[k], ti = ri[k];
-            double dzr = ar[k], dzi = ai[k];
-            ubr[k] = A.u[n] - (0. * tr - cx * ti);
-            ubi[k] = A.u[n + 1] - (0. * ti + cx * tr);
-            vbr[k] = A.v[n] - (0. * tr - cy * ti);
-            vbi[k] = A.v[n + 1] - (0. * ti + cy * tr);
-            if (zero) {
-                wr[k] = 0.; wi[k] = 0.;
-            } else {
-                wr[k] = wr[k] - A.dt * dzr;
-                wi[k] = wi[k] - A.dt * dzi;
-            }
-        }
-    }
-    __syncthreads();
-    const double xnudt = A.xnu * A.dt;
-    // ---- w (calcuvwt.f:496-585) ----
-    if (zero) {
-        for (int k = lane; k < nz; k += 64) {
-            A.w[(long)k * A.nxppy + o] = 0.;
-            A.w[(long)k * A.nxppy + o + 1] = 0.;
-        }
-    } else {
-        const double eps = A.sysinfo[4 * sW];
-        for (int k = lane; k < nz; k += 64) {
-            rr[k] = -(-wr[k] / xnudt) * eps;
-            ri[k] = -(-wi[k] / xnudt) * eps;
-        }
-        __syncthreads();
-        double *bp[2] = {rr, ri};
-        band_solve<2>(A, sW, bp, fch, ich);
-        if (A.cpen) filter_cols(A, bp, 2, tmp);
-        for (int k = lane; k < nz; k += 64) {
-            A.w[(long)k * A.nxppy + o] = rr[k];
-            A.w[(long)k * A.nxppy + o + 1] = ri[k];
-        }
-    }
-    // ---- u, v (calcuvwt.f:247-470): one factor, four right-hand sides ----
-    {
-        const double eps = A.sysinfo[4 * sUV];
-        for (int k = lane; k < nz; k += 64) {
-            ubr[k] = -(-ubr[k] / xnudt) * eps;
-            ubi[k] = -(-ubi[k] / xnudt) * eps;
-            vbr[k] = -(-vbr[k] / xnudt) * eps;
-            vbi[k] = -(-vbi[k] / xnudt) * eps;
-        }
-        __syncthreads();
-        double *bp[4] = {ubr, ubi, vbr, vbi};
-        band_solve<4>(A, sUV, bp, fch, ich);
-        if (A.cpen) filter_cols(A, bp, 4, tmp);
-        for (int k = lane; k < nz; k += 64) {
-            double a = ubr[k], b = ubi[k];
-            if (zero) { // calcuvwt.f:343-361
-                a = A.damp[k] * a;
-                b = A.damp[k] * b;
-            }
-            long n = (long)k * A.nxppy + o;
-            A.u[n] = a;
-            A.u[n + 1] = b;
-            A.v[n] = vbr[k];
-            A.v[n + 1] = vbi[k];
-        }
-    }
-    __syncthreads();
-    // ---- density (calcuvwt.f:609-741) ----
-    {
-        const double eps = A.sysinfo[4 * sT];
-        const double dk = A.dt * A.xkappa;
-        for (int k = lane; k < nz; k += 64) {
-            long n = (long)k * A.nxppy + o;
-            rr[k] = -(-A.t[n] / dk) * eps;
-            ri[k] = -(-A.t[n + 1] / dk) * eps;
-        }
-        __syncthreads();
-        double *bp[2] = {rr, ri};
-        band_solve<2>(A, sT, bp, fch, ich);
-        if (A.cpen) filter_cols(A, bp, 2, tmp);
-        for (int k = lane; k < nz; k += 64) {
-            long n = (long)k * A.nxppy + o;
-            A.t[n] = rr[k];
-            A.t[n + 1] = ri[k];
+        const double ur = A.u[n], ui = A.u[n + 1], vr = A.v[n], vi = A.v[n + 1];
+        A.u[n] = ur - (0. * tr - cx * ti);
+        A.u[n + 1] = ui - (0. * ti + cx * tr);
+        A.v[n] = vr - (0. * tr - cy * ti);
+        A.v[n + 1] = vi - (0. * ti + cy * tr);
+        if (zero) {
+            A.w[n] = 0.;
+            A.w[n + 1] = 0.;
+        } else {
+            A.w[n] = A.w[n] - A.dt * dzr;
+            A.w[n + 1] = A.w[n + 1] - A.dt * dzi;
         }
     }
 }
 
-void launch_calc_mode(const Geo &g, const iw_params &P, const SysTables &S, double *const *uvwt,
-                      const double *wnl, const double *wviscrot, double dt, int cpen, hipStream_t st)
+__global__ __launch_bounds__(64) void k_viscous(ModeArgs A)
+{
+    extern __shared__ double shm[];
+    const int nz = A.nz;
+    double *rr = shm, *ri = rr + nz, *tmp = ri + nz;
+    const int task = A.order[blockIdx.x];
+    if (task < 0) return;
+    const int m = task >> 2, q = task & 3;
+    const int lane = threadIdx.x;
+    const long o = 2L * m;
+    const bool zero = (m == 0);
+    if (q == 2 && zero) return; // w(0,0) = 0 already (calcuvwt.f:496-503)
+    double *f = q == 0 ? A.u : q == 1 ? A.v : q == 2 ? A.w : A.t;
+    const int sys = A.modesys[4 * m + (q == 0 ? 1 : q == 1 ? 1 : q == 2 ? 2 : 3)];
+    const double eps = A.sysinfo[4 * sys];
+    const double den = (q < 3) ? A.xnu * A.dt : A.dt * A.xkappa;
+    for (int k = lane; k < nz; k += 64) { // rhs = -f/(nu dt), scaled by -eps (penalty.f:191-194)
+        const long n = (long)k * A.nxppy + o;
+        rr[k] = -(-f[n] / den) * eps;
+        ri[k] = -(-f[n + 1] / den) * eps;
+    }
+    __syncthreads();
+    band_solve2(A, sys, rr, ri);
+    if (A.cpen) filter_pair(A, rr, ri, tmp);
+    for (int k = lane; k < nz; k += 64) {
+        const long n = (long)k * A.nxppy + o;
+        double a = rr[k], b = ri[k];
+        if (q == 0 && zero) { // mean-u damping, calcuvwt.f:343-361
+            a = A.damp[k] * a;
+            b = A.damp[k] * b;
+        }
+        f[n] = a;
+        f[n + 1] = b;
+    }
+}
+
+static ModeArgs mode_args(const Geo &g, const iw_params &P, const SysTables &S, double *const *uvwt,
+                          const double *wnl, const double *wviscrot, double dt, int cpen)
 {
     ModeArgs A;
     A.u = uvwt[0]; A.v = uvwt[1]; A.w = uvwt[2]; A.t = uvwt[3];
@@ -1184,9 +1193,303 @@ void launch_calc_mode(const Geo &g, const iw_params &P, const SysTables &S, doub
     A.ldab = S.ldab; A.kl = S.kl; A.kv = S.kv;
     A.cpen = cpen;
     A.noslip = P.bc_uv_bottom == 1;
-    int nmode = g.nxhp * g.ny;
-    size_t sh = sizeof(double) * (8 * (size_t)g.nz + 64 + (size_t)kCH * (S.kv + 1)) + sizeof(int) * kCH;
-    hipLaunchKernelGGL(k_calc_mode, dim3(nmode), dim3(64), sh, st, A);
+    return A;
+}
+
+static size_t mode_lds(const Geo &g)
+{
+    return sizeof(double) * (2 * (size_t)g.nz + 64);
+}
+
+void launch_pressure(const Geo &g, const iw_params &P, const SysTables &S, double *const *uvwt,
+                     const double *wnl, const double *wviscrot, double dt, int cpen, hipStream_t st)
+{
+    ModeArgs A = mode_args(g, P, S, uvwt, wnl, wviscrot, dt, cpen);
+    A.order = S.order_p;
+    hipLaunchKernelGGL(k_pressure, dim3(S.nblk_p), dim3(64), mode_lds(g), st, A);
+}
+
+void launch_viscous(const Geo &g, const iw_params &P, const SysTables &S, double *const *uvwt, double dt,
+                    int cpen, hipStream_t st)
+{
+    ModeArgs A = mode_args(g, P, S, uvwt, nullptr, nullptr, dt, cpen);
+    A.order = S.order_v;
+    hipLaunchKernelGGL(k_viscous, dim3(S.nblk_v), dim3(64), mode_lds(g), st, A);
+}
+
+// ---------------------------------------------------------------------------
+// Solves without row interchanges (the u/v, w and density operators are
+// diagonally dominant: SGBTF2 never swaps rows for them, so ipiv(j) = j and
+// U keeps ku superdiagonals).  One lane per right-hand-side column: the lane
+// runs the forward (bandludcmp.f:1020-1030) and backward (ludcmp.f:4698-5046)
+// substitutions itself with the band window in registers; lanes are grouped
+// by system so a factor column is fetched once per wave.  Columns live in a
+// mode-major scratch cm[col][k], col = (field*nmode + mode)*2 + comp.
+// ---------------------------------------------------------------------------
+struct NpArgs {
+    double *cm;
+    const int *tasks;      // lane -> column id (-1 idle)
+    const int *modesys;
+    const double *AB;
+    int n, ldab, kv, nmode;
+};
+
+template <int KL>
+__global__ __launch_bounds__(64) void k_solve_np(NpArgs A)
+{
+    // requires n > 2*KL+2 (checked on the host): every read below stays inside
+    // the column (factor rows 0..ldab-1, right-hand side 0..n-1); values that
+    // lie outside the band are masked to zero after an unconditional load
+    const int col = A.tasks[blockIdx.x * 64 + threadIdx.x];
+    if (col < 0) return;
+    const int n = A.n, kv = A.kv, ldab = A.ldab;
+    const int f = col / (2 * A.nmode), m = (col >> 1) % A.nmode;
+    const int sys = A.modesys[4 * m + (f < 2 ? 1 : f == 2 ? 2 : 3)];
+    const double *__restrict__ ab = A.AB + (long)sys * ldab * n;
+    double *__restrict__ b = A.cm + (long)col * n;
+    double w[KL + 1], lc[KL], ln[KL];
+    // forward sweep; multipliers and the next right-hand-side entry of step
+    // j+1 are loaded before step j's arithmetic and store
+#pragma unroll
+    for (int t = 0; t <= KL; t++) w[t] = b[t];
+#pragma unroll
+    for (int t = 1; t <= KL; t++) lc[t - 1] = ab[kv + t];
+    double nbn = b[KL + 1];
+#pragma unroll 2
+    for (int j = 0; j < n; j++) {
+        const double nb = (j + KL + 1 < n) ? nbn : 0.;
+        const int jn = (j + 1 < n) ? j + 1 : j;
+        const double *lcn = ab + (long)ldab * jn + kv;
+        const int lmn = n - 2 - j;
+#pragma unroll
+        for (int t = 1; t <= KL; t++) {
+            const double lv = lcn[t];
+            ln[t - 1] = (t <= lmn) ? lv : 0.;
+        }
+        const int in = (j + KL + 2 < n) ? j + KL + 2 : n - 1;
+        nbn = b[in];
+        const double y = w[0];
+#pragma unroll
+        for (int t = 1; t <= KL; t++) w[t - 1] = w[t] + lc[t - 1] * (-y);
+        w[KL] = nb;
+        b[j] = y;
+#pragma unroll
+        for (int t = 0; t < KL; t++) lc[t] = ln[t];
+    }
+    // backward sweep (U with ku = KL superdiagonals)
+#pragma unroll
+    for (int t = 0; t <= KL; t++) w[t] = b[n - 1 - t];
+    double dc = ab[(long)ldab * (n - 1) + kv], dn;
+#pragma unroll
+    for (int t = 1; t <= KL; t++) lc[t - 1] = ab[(long)ldab * (n - 1) + kv - t];
+    nbn = b[n - 2 - KL];
+#pragma unroll 2
+    for (int j = n - 1; j >= 0; j--) {
+        const double nb = (j - KL - 1 >= 0) ? nbn : 0.;
+        const int jn = (j >= 1) ? j - 1 : 0;
+        const double *ucn = ab + (long)ldab * jn + kv;
+        dn = ucn[0];
+        const int lun = j - 1;
+#pragma unroll
+        for (int t = 1; t <= KL; t++) {
+            const double uv = ucn[-t];
+            ln[t - 1] = (t <= lun) ? uv : 0.;
+        }
+        const int in = (j - KL - 2 >= 0) ? j - KL - 2 : 0;
+        nbn = b[in];
+        const double x = (w[0] != 0.) ? w[0] / dc : w[0];
+#pragma unroll
+        for (int t = 1; t <= KL; t++) w[t - 1] = w[t] - x * lc[t - 1];
+        w[KL] = nb;
+        b[j] = x;
+        dc = dn;
+#pragma unroll
+        for (int t = 0; t < KL; t++) lc[t] = ln[t];
+    }
+}
+
+bool solve_np_supported(int nl, int nsubd)
+{
+    return (nl == 9 || nl == 13 || nl == 17 || nl == 21 || nl == 25) && nsubd >= 3;
+}
+
+void launch_solve_np(const Geo &g, const SysTables &S, double *cm, hipStream_t st)
+{
+    NpArgs A;
+    A.cm = cm;
+    A.tasks = S.np_tasks;
+    A.modesys = S.modesys;
+    A.AB = S.AB;
+    A.n = g.nz;
+    A.ldab = S.ldab;
+    A.kv = S.kv;
+    A.nmode = g.nxhp * g.ny;
+    dim3 grid(S.np_ntasks / 64);
+    switch (g.nzloc) {
+    case 9: hipLaunchKernelGGL(k_solve_np<9>, grid, dim3(64), 0, st, A); break;
+    case 13: hipLaunchKernelGGL(k_solve_np<13>, grid, dim3(64), 0, st, A); break;
+    case 17: hipLaunchKernelGGL(k_solve_np<17>, grid, dim3(64), 0, st, A); break;
+    case 21: hipLaunchKernelGGL(k_solve_np<21>, grid, dim3(64), 0, st, A); break;
+    case 25: hipLaunchKernelGGL(k_solve_np<25>, grid, dim3(64), 0, st, A); break;
+    default: break;
+    }
+}
+
+// native a(nxpp,ny,nz) of u,v,w,rho -> mode-major right-hand sides
+// -(-f/(nu dt)) eps (calcuvwt.f:278-281, 612-615 and penalty.f:191-194),
+// through a 32 mode x 32 level LDS tile
+__global__ __launch_bounds__(256) void k_to_modemajor(const double *u, const double *v, const double *w,
+                                                      const double *t, double *cm, const int *__restrict__ modesys,
+                                                      const double *__restrict__ sysinfo, double xnudt,
+                                                      double dtkap, int nz, int nmode, long nxppy)
+{
+    __shared__ double tile[2][32][33];
+    const int f = blockIdx.z;
+    const double *src = f == 0 ? u : f == 1 ? v : f == 2 ? w : t;
+    const double den = f < 3 ? xnudt : dtkap;
+    const int m0 = blockIdx.x * 32, k0 = blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5; // 32 x 8
+    for (int kk = ty; kk < 32; kk += 8) {
+        const int k = k0 + kk, m = m0 + tx;
+        double re = 0., im = 0.;
+        if (k < nz && m < nmode) {
+            const long o = (long)k * nxppy + 2L * m;
+            re = src[o];
+            im = src[o + 1];
+        }
+        tile[0][kk][tx] = re;
+        tile[1][kk][tx] = im;
+    }
+    __syncthreads();
+    for (int mm = ty; mm < 32; mm += 8) {
+        const int m = m0 + mm, k = k0 + tx;
+        if (m >= nmode || k >= nz) continue;
+        const int sys = modesys[4 * m + (f < 2 ? 1 : f == 2 ? 2 : 3)];
+        const double eps = sys >= 0 ? sysinfo[4 * sys] : 0.;
+        const long c = ((long)f * nmode + m) * 2;
+        cm[c * nz + k] = -(-tile[0][tx][mm] / den) * eps;
+        cm[(c + 1) * nz + k] = -(-tile[1][tx][mm] / den) * eps;
+    }
+}
+
+void launch_to_modemajor(const Geo &g, const iw_params &P, const SysTables &S, double *const *uvwt, double *cm,
+                         double dt, hipStream_t st)
+{
+    const int nmode = g.nxhp * g.ny;
+    dim3 grid((nmode + 31) / 32, (g.nz + 31) / 32, 4);
+    hipLaunchKernelGGL(k_to_modemajor, grid, dim3(256), 0, st, uvwt[0], uvwt[1], uvwt[2], uvwt[3], cm,
+                       S.modesys, S.sysinfo, P.xnu * dt, dt * P.xkappa, g.nz, nmode, g.nxppy);
+}
+
+// mode-major solutions -> native layout with the Legendre filter of every
+// subdomain (filter_legenpen, filtering.f:729-808) and the mean-u damping
+// (calcuvwt.f:343-361); the tables are staged in LDS
+__global__ __launch_bounds__(256) void k_from_modemajor(const double *cm, double *u, double *v, double *w,
+                                                        double *t, int nz, int nl, int nmode, long nxppy,
+                                                        int cpen, const double *__restrict__ wg_,
+                                                        const double *__restrict__ AL_,
+                                                        const double *__restrict__ sig_,
+                                                        const double *__restrict__ damp)
+{
+    __shared__ double tile[kMaxNl][2 * kMB + 1];
+    __shared__ double coef[kMaxNl][2 * kMB + 1];
+    __shared__ double AL[kMaxNl * kMaxNl], wg[kMaxNl], sig[kMaxNl];
+    const int f = blockIdx.z, ks = blockIdx.y;
+    double *dst = f == 0 ? u : f == 1 ? v : f == 2 ? w : t;
+    const int m0 = blockIdx.x * kMB;
+    for (int e = threadIdx.x; e < nl * nl; e += blockDim.x) AL[e] = AL_[e];
+    for (int e = threadIdx.x; e < nl; e += blockDim.x) {
+        wg[e] = wg_[e];
+        sig[e] = sig_[e];
+    }
+    for (int e = threadIdx.x; e < nl * 2 * kMB; e += blockDim.x) { // e -> (column cc, level kk)
+        const int cc = e / nl, kk = e - cc * nl;
+        const int m = m0 + (cc >> 1);
+        double val = 0.;
+        if (m < nmode) val = cm[(((long)f * nmode + m) * 2 + (cc & 1)) * nz + ks * nl + kk];
+        tile[kk][cc] = val;
+    }
+    __syncthreads();
+    if (cpen) {
+        const int nzm = nl - 1;
+        for (int e = threadIdx.x; e < nl * 2 * kMB; e += blockDim.x) {
+            const int k = e / (2 * kMB), cc = e - k * 2 * kMB;
+            double sum = 0.;
+            for (int tt = 0; tt < nl; tt++) sum = sum + wg[tt] * tile[tt][cc] * AL[k * nl + tt];
+            const double a = (k == nzm) ? 0.5 * nzm * sum : (k + 0.5) * sum;
+            coef[k][cc] = a * sig[k];
+        }
+        __syncthreads();
+        for (int e = threadIdx.x; e < nl * 2 * kMB; e += blockDim.x) {
+            const int tt = e / (2 * kMB), cc = e - tt * 2 * kMB;
+            double sum = 0.;
+            for (int k = 0; k < nl; k++) sum = sum + coef[k][cc] * AL[k * nl + tt];
+            tile[tt][cc] = sum;
+        }
+        __syncthreads();
+    }
+    for (int e = threadIdx.x; e < nl * 2 * kMB; e += blockDim.x) {
+        const int kk = e / (2 * kMB), cc = e - kk * 2 * kMB;
+        const int m = m0 + (cc >> 1);
+        if (m >= nmode) continue;
+        if (f == 2 && m == 0) continue; // w(0,0) stays zero (calcuvwt.f:496-503)
+        double val = tile[kk][cc];
+        const int k = ks * nl + kk;
+        if (f == 0 && m == 0) val = damp[k] * val;
+        dst[(long)k * nxppy + 2L * m0 + cc] = val;
+    }
+}
+
+void launch_from_modemajor(const Geo &g, const double *cm, double *const *uvwt, int cpen, hipStream_t st)
+{
+    const int nmode = g.nxhp * g.ny;
+    dim3 grid((nmode + kMB - 1) / kMB, g.nsubd, 4);
+    hipLaunchKernelGGL(k_from_modemajor, grid, dim3(256), 0, st, cm, uvwt[0], uvwt[1], uvwt[2], uvwt[3], g.nz,
+                       g.nzloc, nmode, g.nxppy, cpen, g.wg, g.AL, g.sig, g.damp);
+}
+
+// ---------------------------------------------------------------------------
+// Spectral effect of a horfft inverse+forward round trip (horfft.f:14-64):
+// the inverse x transform keeps only Re of the kx=0 and kx-Nyquist entries of
+// every row (fft.f:1218-1230) and the y transform skips the Nyquist column
+// while 1/ny scales every entry (horfft.f:24-30).  Hence fwd(inv(X)) = X for
+// 0 < kx < Nyquist, the kx=0 column becomes its ky-Hermitian part
+// (X(ky) + conj X(-ky))/2, and the kx-Nyquist column becomes Re(X)/ny.
+// ---------------------------------------------------------------------------
+__global__ void k_project(FieldBatch fb, int nk, int ny, int nxpp, int nxh, double scaley)
+{
+    const int id = blockIdx.x * blockDim.x + threadIdx.x;
+    double *f = fb.f[blockIdx.y];
+    const int nh = ny / 2;
+    const int per = nh + 1 + ny; // Hermitian pairs j=0..ny/2, then Nyquist rows
+    if (id >= nk * per) return;
+    const int k = id / per, r = id - k * per;
+    double *plane = f + (long)k * nxpp * ny;
+    if (r <= nh) {
+        const int j = r, jm = (ny - j) % ny;
+        double *a = plane + (long)j * nxpp, *b = plane + (long)jm * nxpp;
+        const double ar = a[0], ai = a[1], br = b[0], bi = b[1];
+        const double hr = 0.5 * (ar + br), hi = 0.5 * (ai - bi);
+        a[0] = hr;
+        a[1] = hi;
+        if (jm != j) {
+            b[0] = hr;
+            b[1] = -hi;
+        }
+    } else {
+        const int j = r - nh - 1;
+        double *a = plane + (long)j * nxpp + 2 * nxh;
+        a[0] = scaley * a[0];
+        a[1] = 0.;
+    }
+}
+
+void launch_project(const Geo &g, FieldBatch b, hipStream_t st)
+{
+    const int per = g.ny / 2 + 1 + g.ny;
+    const int n = g.nz * per;
+    hipLaunchKernelGGL(k_project, dim3((n + 255) / 256, b.n), dim3(256), 0, st, b, g.nz, g.ny, g.nxpp, g.nxh,
+                       g.scaley);
 }
 
 } // namespace iw
