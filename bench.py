@@ -119,7 +119,7 @@ def main():
     cpu = None
     if not a.no_cpu_baseline:
         log("bench: cpu baseline")
-        cpu = cpu_baseline(P, fields, 1)
+        cpu = cpu_baseline(P, fields, 5)
 
     out = {"metric": "grid-point-steps/s on 1 GPU, mixed/learned vs plain FP64 at fixed error",
            "value": gps, "unit": "grid-point-steps/s", "n_gpus": 1, "steps": a.steps,

@@ -60,6 +60,10 @@ int iw_set_state(iw_solver *s, int idx, const double *host);
 int iw_get_state(iw_solver *s, int idx, double *host);
 /* /pressbc/ wnl, wviscrot (comrotpressbc.h), each (nxpp,ny,2) */
 int iw_get_pressbc(iw_solver *s, double *wnl, double *wviscrot);
+int iw_set_pressbc(iw_solver *s, const double *wnl, const double *wviscrot);
+/* dt of /timevars/ (comtime.h:1); the u,v,w,rho operators are refactored
+ * at the next calcuvwt when dt or bdfv has changed */
+int iw_set_dt(iw_solver *s, double dt);
 /* Step clock and the SS/BDF coefficients of /ab/, /bdf/ (comtimeadvance.h) */
 int iw_set_coeffs(iw_solver *s, double t, const double fac[3], const double bdf[3], double bdfv);
 int iw_get_coeffs(iw_solver *s, double *t, double fac[3], double bdf[3], double *bdfv);

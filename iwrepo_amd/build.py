@@ -1,8 +1,9 @@
-"""Build libiwgpu.so (HIP, gfx950) in-tree, plus the Fortran drop-in shims.
+"""Build libiwgpu.so (HIP, gfx950) in-tree.
 
 ``python -m iwrepo_amd.build`` compiles every HIP source for gfx950 with hipcc
-(cross-compiles without a GPU).  The Fortran shims (iwrepo_amd/fortran) are
-compiled with ROCm flang when it is present.
+(cross-compiles without a GPU).  The Fortran drop-in shims
+(iwrepo_amd/fortran/iwshim.F) are compiled inside the caller's own Fortran
+build, because they include the caller's dim.h and COMMON headers.
 """
 import os
 import shutil
@@ -14,7 +15,6 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libiwgpu.so")
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
-FLANG = "/opt/rocm/lib/llvm/bin/flang"
 
 HIPFLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-Wno-unused-value",
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -51,27 +51,8 @@ def build_hip(verbose=False):
     return LIB
 
 
-def build_fortran():
-    """Fortran drop-in shims (same names/arguments as the reference routines)."""
-    fdir = os.path.join(PKG, "fortran")
-    if not os.path.exists(FLANG) or not os.path.isdir(fdir):
-        return None
-    out = os.path.join(PKG, "libiwfort.so")
-    srcs = [os.path.join(fdir, f) for f in sorted(os.listdir(fdir)) if f.endswith(".f90")]
-    if not srcs:
-        return None
-    _run([FLANG, "-O2", "-fPIC", "-shared", "-J", fdir, "-o", out] + srcs +
-         ["-L" + PKG, "-liwgpu", "-Wl,-rpath,$ORIGIN"])
-    return out
-
-
 def build_all(verbose=False):
-    lib = build_hip(verbose)
-    try:
-        build_fortran()
-    except RuntimeError as e:  # the shims are optional for the product path
-        sys.stderr.write("fortran shims not built: %s\n" % e)
-    return lib
+    return build_hip(verbose)
 
 
 if __name__ == "__main__":

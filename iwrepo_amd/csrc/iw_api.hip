@@ -781,6 +781,26 @@ extern "C" int iw_get_pressbc(iw_solver *s, double *wnl, double *wviscrot)
     return 0;
 }
 
+extern "C" int iw_set_pressbc(iw_solver *s, const double *wnl, const double *wviscrot)
+{
+    if (!s) return fail("null solver");
+    size_t b = 2 * sizeof(double) * s->g.nxppy;
+    if (wnl) HIPCHK(hipMemcpyAsync(s->wnl, wnl, b, hipMemcpyHostToDevice, s->stream));
+    if (wviscrot) HIPCHK(hipMemcpyAsync(s->wviscrot, wviscrot, b, hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return 0;
+}
+
+// dt of /timevars/ (comtime.h); the implicit operators are refactored on
+// the next calcuvwt when it changes
+extern "C" int iw_set_dt(iw_solver *s, double dt)
+{
+    if (!s) return fail("null solver");
+    if (!(dt > 0.)) return fail("iw_set_dt: dt must be positive");
+    s->P.dt = dt;
+    return 0;
+}
+
 extern "C" int iw_set_coeffs(iw_solver *s, double t, const double fac[3], const double bdf[3], double bdfv)
 {
     if (!s) return fail("null solver");

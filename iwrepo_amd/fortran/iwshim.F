@@ -1,0 +1,444 @@
+c-----------------------------------------------------------------------
+c     Fortran drop-in shims of the mode1_bbl hot-path routines.
+c
+c     Each subroutine keeps the name, argument list and COMMON-block
+c     contract of the reference routine it replaces (file:line below),
+c     so a caller such as main.f links against these objects in place
+c     of the reference's and runs unchanged.  The work is done on the
+c     GPU by libiwgpu.so through its C-ABI (include/iwgpu.h).
+c
+c     The shims are built with the caller's own include path, because
+c     the reference sizes its COMMON blocks at compile time (dim.h), and
+c     with the caller's REAL size:
+c        flang -cpp -fdefault-real-8 -fdefault-double-8 -I<src> -c iwshim.F
+c     Define IW_NOSLIP when the caller is built with calcuvwt_noslip.f
+c     (no-slip bottom) rather than calcuvwt.f (free slip).
+c
+c     Data flow: each call copies the host arrays the reference routine
+c     reads to the device instance and copies back those it writes, so
+c     on return the caller's arrays hold what the reference routine
+c     would have left there.  One device instance is created on the
+c     first call from the parameters then held in COMMON.
+c-----------------------------------------------------------------------
+      module iwshim
+      use iso_c_binding
+      implicit none
+      integer, parameter :: iw_maxsub = 64
+c     layout of iw_params (include/iwgpu.h)
+      type, bind(c) :: iw_params_t
+        integer(c_int) :: nx, ny, nzloc, nsubd, bc_uv_bottom
+        integer(c_int) :: ispecfilternl, ispecfilterexp
+        integer(c_int) :: nspecfilterexp, navgintrfc, pad0
+        real(c_double) :: xlen, ylen, zlen
+        real(c_double) :: xnu, xkappa, brunt, grav, rho0, omz
+        real(c_double) :: dt
+        real(c_double) :: amp, p, pf, fackc
+        real(c_double) :: facdir, facneumn, facrobin
+        real(c_double) :: zcen, xbcrit
+        real(c_double) :: xacrit, umax, fwave
+        real(c_double) :: z0(iw_maxsub), zh(iw_maxsub)
+      end type
+      type(c_ptr), save :: iwh = c_null_ptr
+      interface
+        integer(c_int) function iw_params_size() bind(c)
+          import
+        end function
+        type(c_ptr) function iw_last_error() bind(c)
+          import
+        end function
+        integer(c_int) function iw_create(prm, h) bind(c)
+          import
+          type(iw_params_t), intent(in) :: prm
+          type(c_ptr), intent(out) :: h
+        end function
+        integer(c_int) function iw_set_state(h, idx, a) bind(c)
+          import
+          type(c_ptr), value :: h
+          integer(c_int), value :: idx
+          real(c_double) :: a(*)
+        end function
+        integer(c_int) function iw_get_state(h, idx, a) bind(c)
+          import
+          type(c_ptr), value :: h
+          integer(c_int), value :: idx
+          real(c_double) :: a(*)
+        end function
+        integer(c_int) function iw_set_pressbc(h, a, b) bind(c)
+          import
+          type(c_ptr), value :: h
+          real(c_double) :: a(*), b(*)
+        end function
+        integer(c_int) function iw_get_pressbc(h, a, b) bind(c)
+          import
+          type(c_ptr), value :: h
+          real(c_double) :: a(*), b(*)
+        end function
+        integer(c_int) function iw_set_coeffs(h, t, fac, bdf, bdfv)
+     &       bind(c)
+          import
+          type(c_ptr), value :: h
+          real(c_double), value :: t, bdfv
+          real(c_double) :: fac(3), bdf(3)
+        end function
+        integer(c_int) function iw_set_dt(h, dt) bind(c)
+          import
+          type(c_ptr), value :: h
+          real(c_double), value :: dt
+        end function
+        integer(c_int) function iw_horfft(h, b, is, nk) bind(c)
+          import
+          type(c_ptr), value :: h
+          real(c_double) :: b(*)
+          integer(c_int), value :: is, nk
+        end function
+        integer(c_int) function iw_dpdz(h, a, b) bind(c)
+          import
+          type(c_ptr), value :: h
+          real(c_double) :: a(*), b(*)
+        end function
+        integer(c_int) function iw_filter_3dlegenfourier(h, f, ifo, ile)
+     &       bind(c)
+          import
+          type(c_ptr), value :: h
+          real(c_double) :: f(*)
+          integer(c_int), value :: ifo, ile
+        end function
+        integer(c_int) function iw_interface_avg3d(h, f, ic) bind(c)
+          import
+          type(c_ptr), value :: h
+          real(c_double) :: f(*)
+          integer(c_int) :: ic
+        end function
+        integer(c_int) function iw_convec(h, istep, inl) bind(c)
+          import
+          type(c_ptr), value :: h
+          integer(c_int), value :: istep, inl
+        end function
+        integer(c_int) function iw_calcuvwt(h, icpen) bind(c)
+          import
+          type(c_ptr), value :: h
+          integer(c_int), value :: icpen
+        end function
+        real(c_double) function iw_abvar1(a, b, c) bind(c)
+          import
+          real(c_double), value :: a, b, c
+        end function
+        real(c_double) function iw_abvar2(a, b, c) bind(c)
+          import
+          real(c_double), value :: a, b, c
+        end function
+        real(c_double) function iw_abvar3(a, b, c) bind(c)
+          import
+          real(c_double), value :: a, b, c
+        end function
+        real(c_double) function iw_bdfvarv(a, b, c) bind(c)
+          import
+          real(c_double), value :: a, b, c
+        end function
+        real(c_double) function iw_bdfvar1(a, b, c) bind(c)
+          import
+          real(c_double), value :: a, b, c
+        end function
+        real(c_double) function iw_bdfvar2(a, b, c) bind(c)
+          import
+          real(c_double), value :: a, b, c
+        end function
+        real(c_double) function iw_bdfvar3(a, b, c) bind(c)
+          import
+          real(c_double), value :: a, b, c
+        end function
+      end interface
+      contains
+c     a non-zero return code of the C-ABI stops the program with the
+c     library's message (the reference has no error returns)
+        subroutine iwshim_check(rc, what)
+        integer(c_int), intent(in) :: rc
+        character(*), intent(in) :: what
+        character(kind=c_char), pointer :: msg(:)
+        integer :: n
+        if (rc == 0) return
+        call c_f_pointer(iw_last_error(), msg, [1024])
+        n = 0
+        do while (n < 1024)
+          if (msg(n + 1) == c_null_char) exit
+          n = n + 1
+        end do
+        write(*, '(3a,1024a1)') 'iwshim: ', what, ': ', msg(1:n)
+        error stop 1
+        end subroutine
+      end module iwshim
+
+c-----------------------------------------------------------------------
+c     Device instance from the caller's dim.h and COMMON parameters
+c     (created once; the same values setup and main.f hand to the
+c     reference routines).
+c-----------------------------------------------------------------------
+      subroutine iwshim_init
+      use iwshim
+      include 'dim.h'
+      include 'comparamphys.h'
+      include 'comsubd.h'
+      include 'comtime.h'
+      include 'comspecfilter.h'
+      include 'compencoeff.h'
+      include 'comcritlayer.h'
+      common /dtanimation/ dtanim, fwave
+      type(iw_params_t) :: iwp
+      integer ks
+      if (c_associated(iwh)) return
+      if (iw_params_size() .ne. c_sizeof(iwp)) then
+        write(*,*) 'iwshim: iw_params layout differs from libiwgpu'
+        error stop 1
+      endif
+      if (nsubd .gt. iw_maxsub) then
+        write(*,*) 'iwshim: nsubd exceeds ', iw_maxsub
+        error stop 1
+      endif
+c     mean-u damping width: calcuvwt.f:343-361 uses zdim, set to xlen
+c     by main.f:339
+      if (zdim .ne. xlen) then
+        write(*,*) 'iwshim: zdim must equal xlen (main.f:339)'
+        error stop 1
+      endif
+      iwp%nx = nx
+      iwp%ny = ny
+      iwp%nzloc = nzloc
+      iwp%nsubd = nsubd
+#ifdef IW_NOSLIP
+      iwp%bc_uv_bottom = 1
+#else
+      iwp%bc_uv_bottom = 0
+#endif
+c     the step-loop flags are arguments of convec/calcuvwt here
+      iwp%ispecfilternl = 1
+      iwp%ispecfilterexp = 1
+      iwp%nspecfilterexp = 1
+      iwp%navgintrfc = 1
+      iwp%pad0 = 0
+      iwp%xlen = xlen
+      iwp%ylen = ylen
+      iwp%zlen = zlen
+      iwp%xnu = xnu
+      iwp%xkappa = xkappa
+      iwp%brunt = brunt
+      iwp%grav = grav
+      iwp%rho0 = rho0
+      iwp%omz = omz
+      iwp%dt = dt
+      iwp%amp = amp
+      iwp%p = p
+      iwp%pf = pf
+      iwp%fackc = fackc
+      iwp%facdir = facdir
+      iwp%facneumn = facneumn
+      iwp%facrobin = facrobin
+      iwp%zcen = zcen
+      iwp%xbcrit = xbcrit
+      iwp%xacrit = xacrit
+      iwp%umax = umax
+      iwp%fwave = fwave
+      iwp%z0 = 0.d0
+      iwp%zh = 0.d0
+      do ks = 1, nsubd
+        iwp%z0(ks) = z0(ks)
+        iwp%zh(ks) = zh(ks)
+      enddo
+      call iwshim_check(iw_create(iwp, iwh), 'iw_create')
+      end
+
+c     /flow/ and /matrix/ (comflow.h, commatrix.h) to the device
+      subroutine iwshim_put_state
+      use iwshim
+      include 'dim.h'
+      include 'comflow.h'
+      include 'commatrix.h'
+      call iwshim_check(iw_set_state(iwh, 0, u), 'state')
+      call iwshim_check(iw_set_state(iwh, 1, v), 'state')
+      call iwshim_check(iw_set_state(iwh, 2, w), 'state')
+      call iwshim_check(iw_set_state(iwh, 3, temp), 'state')
+      call iwshim_check(iw_set_state(iwh, 4, un), 'state')
+      call iwshim_check(iw_set_state(iwh, 5, vn), 'state')
+      call iwshim_check(iw_set_state(iwh, 6, wn), 'state')
+      call iwshim_check(iw_set_state(iwh, 7, tn), 'state')
+      call iwshim_check(iw_set_state(iwh, 8, unm), 'state')
+      call iwshim_check(iw_set_state(iwh, 9, vnm), 'state')
+      call iwshim_check(iw_set_state(iwh, 10, wnm), 'state')
+      call iwshim_check(iw_set_state(iwh, 11, tnm), 'state')
+      call iwshim_check(iw_set_state(iwh, 12, su), 'state')
+      call iwshim_check(iw_set_state(iwh, 13, sv), 'state')
+      call iwshim_check(iw_set_state(iwh, 14, sw), 'state')
+      call iwshim_check(iw_set_state(iwh, 15, st), 'state')
+      call iwshim_check(iw_set_state(iwh, 16, sun), 'state')
+      call iwshim_check(iw_set_state(iwh, 17, svn), 'state')
+      call iwshim_check(iw_set_state(iwh, 18, swn), 'state')
+      call iwshim_check(iw_set_state(iwh, 19, stn), 'state')
+      call iwshim_check(iw_set_state(iwh, 20, sunm), 'state')
+      call iwshim_check(iw_set_state(iwh, 21, svnm), 'state')
+      call iwshim_check(iw_set_state(iwh, 22, swnm), 'state')
+      call iwshim_check(iw_set_state(iwh, 23, stnm), 'state')
+      end
+
+c     device to /flow/ and /matrix/
+      subroutine iwshim_get_state
+      use iwshim
+      include 'dim.h'
+      include 'comflow.h'
+      include 'commatrix.h'
+      call iwshim_check(iw_get_state(iwh, 0, u), 'state')
+      call iwshim_check(iw_get_state(iwh, 1, v), 'state')
+      call iwshim_check(iw_get_state(iwh, 2, w), 'state')
+      call iwshim_check(iw_get_state(iwh, 3, temp), 'state')
+      call iwshim_check(iw_get_state(iwh, 4, un), 'state')
+      call iwshim_check(iw_get_state(iwh, 5, vn), 'state')
+      call iwshim_check(iw_get_state(iwh, 6, wn), 'state')
+      call iwshim_check(iw_get_state(iwh, 7, tn), 'state')
+      call iwshim_check(iw_get_state(iwh, 8, unm), 'state')
+      call iwshim_check(iw_get_state(iwh, 9, vnm), 'state')
+      call iwshim_check(iw_get_state(iwh, 10, wnm), 'state')
+      call iwshim_check(iw_get_state(iwh, 11, tnm), 'state')
+      call iwshim_check(iw_get_state(iwh, 12, su), 'state')
+      call iwshim_check(iw_get_state(iwh, 13, sv), 'state')
+      call iwshim_check(iw_get_state(iwh, 14, sw), 'state')
+      call iwshim_check(iw_get_state(iwh, 15, st), 'state')
+      call iwshim_check(iw_get_state(iwh, 16, sun), 'state')
+      call iwshim_check(iw_get_state(iwh, 17, svn), 'state')
+      call iwshim_check(iw_get_state(iwh, 18, swn), 'state')
+      call iwshim_check(iw_get_state(iwh, 19, stn), 'state')
+      call iwshim_check(iw_get_state(iwh, 20, sunm), 'state')
+      call iwshim_check(iw_get_state(iwh, 21, svnm), 'state')
+      call iwshim_check(iw_get_state(iwh, 22, swnm), 'state')
+      call iwshim_check(iw_get_state(iwh, 23, stnm), 'state')
+      end
+
+c     clock, dt and the SS/BDF coefficients of /timevars/, /ab/, /bdf/
+      subroutine iwshim_put_coeffs
+      use iwshim
+      include 'dim.h'
+      include 'comtime.h'
+      include 'comtimeadvance.h'
+      real(c_double) :: fac(3), bdf(3)
+      fac(1) = fac1
+      fac(2) = fac2
+      fac(3) = fac3
+      bdf(1) = bdf1
+      bdf(2) = bdf2
+      bdf(3) = bdf3
+      call iwshim_check(iw_set_dt(iwh, dt), 'dt')
+      call iwshim_check(iw_set_coeffs(iwh, t, fac, bdf, bdfv), 'coeffs')
+      end
+
+c-----------------------------------------------------------------------
+c     horfft.f:2   subroutine horfft(b,is,nk)
+c-----------------------------------------------------------------------
+      subroutine horfft(b, is, nk)
+      use iwshim
+      dimension b(*)
+      call iwshim_init
+      call iwshim_check(iw_horfft(iwh, b, is, nk), 'horfft')
+      end
+
+c-----------------------------------------------------------------------
+c     derv.f:212   subroutine dpdz(a,b)
+c-----------------------------------------------------------------------
+      subroutine dpdz(a, b)
+      use iwshim
+      dimension a(*), b(*)
+      call iwshim_init
+      call iwshim_check(iw_dpdz(iwh, a, b), 'dpdz')
+      end
+
+c-----------------------------------------------------------------------
+c     filtering.f:811   subroutine filter_3dlegenfourier(fin3d,
+c     ifourier,ilegen,iout); iout only selects printing there
+c-----------------------------------------------------------------------
+      subroutine filter_3dlegenfourier(fin3d, ifourier, ilegen, iout)
+      use iwshim
+      dimension fin3d(*)
+      call iwshim_init
+      call iwshim_check(iw_filter_3dlegenfourier(iwh, fin3d, ifourier,
+     &     ilegen), 'filter_3dlegenfourier')
+      end
+
+c-----------------------------------------------------------------------
+c     penalty.f:714   subroutine interface_avg3d(fin,ic)
+c-----------------------------------------------------------------------
+      subroutine interface_avg3d(fin, ic)
+      use iwshim
+      dimension fin(*)
+      integer(c_int) :: icc
+      call iwshim_init
+      call iwshim_check(iw_interface_avg3d(iwh, fin, icc),
+     &     'interface_avg3d')
+      ic = icc
+      end
+
+c-----------------------------------------------------------------------
+c     solver.f:6   subroutine convec(istep,istart,ispecfilternl)
+c     reads /flow/, /matrix/, /timevars/, /ab/, /bdf/; writes /flow/,
+c     /matrix/ (new NL terms, BDF-assembled fields, shifted histories)
+c     and /pressbc/ (comrotpressbc.h).  istart only selects printing.
+c-----------------------------------------------------------------------
+      subroutine convec(istep, istart, ispecfilternl)
+      use iwshim
+      include 'dim.h'
+      include 'comrotpressbc.h'
+      call iwshim_init
+      call iwshim_put_state
+      call iwshim_put_coeffs
+      call iwshim_check(iw_convec(iwh, istep, ispecfilternl), 'convec')
+      call iwshim_get_state
+      call iwshim_check(iw_get_pressbc(iwh, wnl, wviscrot), 'convec')
+      end
+
+c-----------------------------------------------------------------------
+c     calcuvwt.f:2 / calcuvwt_noslip.f:2
+c     subroutine calcuvwt(izmode,ispecfilter,irelax,cpenspecfilter)
+c     izmode, ispecfilter and irelax are not read by the reference.
+c-----------------------------------------------------------------------
+      subroutine calcuvwt(izmode, ispecfilter, irelax, cpenspecfilter)
+      use iwshim
+      include 'dim.h'
+      include 'comrotpressbc.h'
+      logical cpenspecfilter
+      integer(c_int) :: icpen
+      call iwshim_init
+      call iwshim_put_state
+      call iwshim_put_coeffs
+      call iwshim_check(iw_set_pressbc(iwh, wnl, wviscrot), 'calcuvwt')
+      icpen = 0
+      if (cpenspecfilter) icpen = 1
+      call iwshim_check(iw_calcuvwt(iwh, icpen), 'calcuvwt')
+      call iwshim_get_state
+      end
+
+c-----------------------------------------------------------------------
+c     multistep.f:7-89 variable-step AB3/BDF3 coefficients
+c-----------------------------------------------------------------------
+      real function abvar1(dt1, dt2, dt3)
+      use iwshim
+      abvar1 = iw_abvar1(dt1, dt2, dt3)
+      end
+      real function abvar2(dt1, dt2, dt3)
+      use iwshim
+      abvar2 = iw_abvar2(dt1, dt2, dt3)
+      end
+      real function abvar3(dt1, dt2, dt3)
+      use iwshim
+      abvar3 = iw_abvar3(dt1, dt2, dt3)
+      end
+      real function bdfvarv(dt1, dt2, dt3)
+      use iwshim
+      bdfvarv = iw_bdfvarv(dt1, dt2, dt3)
+      end
+      real function bdfvar1(dt1, dt2, dt3)
+      use iwshim
+      bdfvar1 = iw_bdfvar1(dt1, dt2, dt3)
+      end
+      real function bdfvar2(dt1, dt2, dt3)
+      use iwshim
+      bdfvar2 = iw_bdfvar2(dt1, dt2, dt3)
+      end
+      real function bdfvar3(dt1, dt2, dt3)
+      use iwshim
+      bdfvar3 = iw_bdfvar3(dt1, dt2, dt3)
+      end

@@ -16,7 +16,7 @@ LIBPATH = os.path.join(PKG, "libiwgpu.so")
 # every symbol declared in include/iwgpu.h
 EXPORTS = [
     "iw_params_size", "iw_last_error", "iw_create", "iw_destroy", "iw_set_state", "iw_get_state",
-    "iw_get_pressbc", "iw_set_coeffs", "iw_get_coeffs", "iw_set_initial", "iw_run", "iw_time",
+    "iw_get_pressbc", "iw_set_pressbc", "iw_set_dt", "iw_set_coeffs", "iw_get_coeffs", "iw_set_initial", "iw_run", "iw_time",
     "iw_sync", "iw_horfft", "iw_dpdz", "iw_filter_3dlegenfourier", "iw_interface_avg3d",
     "iw_convec", "iw_calcuvwt", "iw_abvar1", "iw_abvar2", "iw_abvar3", "iw_bdfvarv",
     "iw_bdfvar1", "iw_bdfvar2", "iw_bdfvar3", "iw_set_profiling", "iw_kernel_stats",
@@ -45,6 +45,8 @@ def lib():
     L.iw_set_state.argtypes = [P, I, P]
     L.iw_get_state.argtypes = [P, I, P]
     L.iw_get_pressbc.argtypes = [P, P, P]
+    L.iw_set_pressbc.argtypes = [P, P, P]
+    L.iw_set_dt.argtypes = [P, D]
     L.iw_set_coeffs.argtypes = [P, D, P, P, D]
     L.iw_get_coeffs.argtypes = [P, P, P, P, P]
     L.iw_set_initial.argtypes = [P, P, P, P, P]

@@ -23,6 +23,7 @@ import sys
 
 REF = "/root/reference/mode1_bbl"
 HERE = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(os.path.dirname(HERE), "iwrepo_amd")
 OUT = os.path.join(HERE, "_ref")
 FLANG = "/opt/rocm/lib/llvm/bin/flang"
 FFLAGS = ["-fdefault-real-8", "-fdefault-double-8", "-fno-automatic",
@@ -46,7 +47,7 @@ DIM_TEMPLATE = """\
 """
 
 
-def build(name, nx, ny, nzloc, nsubd, noslip=False, ref_dim=False, jobs=8):
+def build(name, nx, ny, nzloc, nsubd, noslip=False, ref_dim=False, jobs=8, shim=False):
     if not os.path.isdir(REF):
         print("reference not mounted; skipping", name)
         return None
@@ -89,6 +90,33 @@ def build(name, nx, ny, nzloc, nsubd, noslip=False, ref_dim=False, jobs=8):
     exe = os.path.join(bdir, "iwref")
     subprocess.run([FLANG] + FFLAGS + [ho] + objs + ["-o", exe], check=True)
     print("built", exe)
+    if shim:
+        build_shim(bdir, src, ho, objs, noslip)
+    return exe
+
+
+def build_shim(bdir, src, ho, objs, noslip):
+    """The same harness with the product's Fortran shims (iwshim.F) linked in
+    front of the reference objects: every call of horfft, dpdz, convec,
+    calcuvwt, interface_avg3d, filter_3dlegenfourier and the multistep
+    functions then reaches libiwgpu.so, while setup and the step loop stay
+    the reference's own.  The shim object comes first on the link line and
+    ``--allow-multiple-definition`` keeps the first definition."""
+    lib = os.path.join(PKG, "libiwgpu.so")
+    if not os.path.exists(lib):
+        print("libiwgpu.so not built; skipping shim harness")
+        return None
+    so = os.path.join(bdir, "iwshim.o")
+    cmd = [FLANG] + FFLAGS + ["-cpp", "-I", src, "-J", bdir]
+    if noslip:
+        cmd.append("-DIW_NOSLIP")
+    subprocess.run(cmd + ["-c", os.path.join(PKG, "fortran", "iwshim.F"), "-o", so], check=True)
+    exe = os.path.join(bdir, "iwshim")
+    rel = os.path.relpath(PKG, bdir)
+    subprocess.run([FLANG] + FFLAGS + [ho, so] + objs +
+                   ["-o", exe, "-Wl,--allow-multiple-definition", "-L" + PKG, "-liwgpu",
+                    "-Wl,-rpath,$ORIGIN/" + rel, "-Wl,-rpath,/opt/rocm/lib"], check=True)
+    print("built", exe)
     return exe
 
 
@@ -104,9 +132,9 @@ def _wait(procs):
 # Named reference builds used by the tests (sizes of the BASELINE-derived
 # parity cases plus the reference's own shipped configuration).
 BUILDS = {
-    "c1": dict(nx=64, ny=16, nzloc=17, nsubd=4),
-    "c1ns": dict(nx=64, ny=16, nzloc=17, nsubd=4, noslip=True),
-    "small": dict(nx=16, ny=8, nzloc=9, nsubd=3),
+    "c1": dict(nx=64, ny=16, nzloc=17, nsubd=4, shim=True),
+    "c1ns": dict(nx=64, ny=16, nzloc=17, nsubd=4, noslip=True, shim=True),
+    "small": dict(nx=16, ny=8, nzloc=9, nsubd=3, shim=True),
     "smallns": dict(nx=16, ny=8, nzloc=9, nsubd=3, noslip=True),
     "g3": dict(nx=256, ny=1, nzloc=25, nsubd=12, ref_dim=True),
 }

@@ -151,11 +151,13 @@ def _input_dat(P, nsteps):
 """
 
 
-def run_ref(build, P, fields, nsteps=1, mode=0, probe=None):
+def run_ref(build, P, fields, nsteps=1, mode=0, probe=None, exe="iwref"):
     """Run the reference build on physical ``fields`` (u,v,w,rho').
 
+    ``exe="iwshim"`` runs the same harness linked with the product's Fortran
+    shims (the reference step loop calling libiwgpu.so).
     Returns (state[24], t, diag_lines)."""
-    exe = os.path.join(REFDIR, build, "iwref")
+    exe = os.path.join(REFDIR, build, exe)
     nz, ny, nxpp = P.nzloc * P.nsubd, P.ny, P.nx + 2
     with tempfile.TemporaryDirectory() as d:
         with open(os.path.join(d, "input.dat"), "w") as f:

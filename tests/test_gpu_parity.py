@@ -156,3 +156,28 @@ def test_parity_mode_entry_points_replay_the_step(rt, oracle_mod):
         assert rel(b.get_state(q), a.get_state(q)) < 1e-13, STATE[q]
     a.close()
     b.close()
+
+
+@pytest.mark.parametrize("name,steps", [("small", 10), ("c1", 40), ("c1ns", 40)])
+def test_fortran_shims_under_the_reference_step_loop(oracle_mod, name, steps):
+    """Drop-in check of the Fortran boundary: the reference's own set-up and
+    step loop (the harness replaying main.f:653-1066) linked with the shims of
+    iwrepo_amd/fortran/iwshim.F in front of the reference objects, so that
+    horfft, convec, calcuvwt, interface_avg3d and the multistep functions run
+    on the GPU, against the same harness built from the reference alone."""
+    for exe in ("iwref", "iwshim"):
+        if not os.path.exists(os.path.join(oracle_mod.REFDIR, name, exe)):
+            pytest.skip("%s/%s not built (built where the reference sources are)" % (name, exe))
+    P, flds = case(name)
+    print("shim harness", name, steps, flush=True)
+    sr, tr, dr = oracle_mod.run_ref(name, P, flds, nsteps=steps)
+    ss, ts, ds = oracle_mod.run_ref(name, P, flds, nsteps=steps, exe="iwshim")
+    errs = {STATE[q]: rel(ss[q], sr[q]) for q in SOLUTION}
+    print(errs, flush=True)
+    _record("shim_%s_%d" % (name, steps), errs)
+    for q in SOLUTION:
+        assert errs[STATE[q]] < (FIELD_TOL if q < 12 else HIST_TOL), STATE[q]
+    assert abs(ts - tr) < 1e-15
+    cr = [l.split()[1:] for l in dr if l.strip()]
+    cs = [l.split()[1:] for l in ds if l.strip()]
+    assert cr == cs  # interface-average counts of every step
