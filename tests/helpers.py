@@ -49,3 +49,16 @@ def g2_params():
     for k, (z0, _) in enumerate(sub):
         P.z0[k] = z0
     return P, d
+
+
+def routine_inputs(O, P, flds):
+    """Physical fields for the horfft check and spectral fields (rough, with
+    interface jumps) for the dpdz, filter and interface-averaging checks."""
+    o = O.Oracle(P)
+    rng = np.random.default_rng(11)
+    spec = []
+    for q in range(4):
+        f = flds[q] + 0.05 * rng.standard_normal(o.shape)
+        f[:, :, P.nx:] = 0.0
+        spec.append(o.horfft(f, -1))
+    return o, spec

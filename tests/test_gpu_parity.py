@@ -6,7 +6,7 @@ import os
 import numpy as np
 import pytest
 
-from helpers import SOLUTION, STATE, case, g3_params, rel
+from helpers import SOLUTION, STATE, case, g3_params, rel, routine_inputs
 
 pytestmark = pytest.mark.gpu
 
@@ -181,3 +181,25 @@ def test_fortran_shims_under_the_reference_step_loop(oracle_mod, name, steps):
     cr = [l.split()[1:] for l in dr if l.strip()]
     cs = [l.split()[1:] for l in ds if l.strip()]
     assert cr == cs  # interface-average counts of every step
+
+
+@pytest.mark.parametrize("name", ["small", "c1"])
+def test_fortran_shim_routines_against_reference(oracle_mod, name):
+    """horfft, dpdz, filter_3dlegenfourier and interface_avg3d reached through
+    the Fortran shims, called one at a time after the reference set-up,
+    against the reference routines."""
+    for exe in ("iwref", "iwshim"):
+        if not os.path.exists(os.path.join(oracle_mod.REFDIR, name, exe)):
+            pytest.skip("%s/%s not built (built where the reference sources are)" % (name, exe))
+    P, flds = case(name)
+    o, spec = routine_inputs(oracle_mod, P, flds)
+    tol = {1: 1e-14, 2: 1e-13, 3: 1e-12, 4: 1e-13}
+    for mode in (1, 2, 3, 4):
+        print("shim routine", name, "mode", mode, flush=True)
+        inp = flds if mode == 1 else spec
+        sr, _, dr = oracle_mod.run_ref(name, P, inp, nsteps=0, mode=mode)
+        ss, _, ds = oracle_mod.run_ref(name, P, inp, nsteps=0, mode=mode, exe="iwshim")
+        for q in range(4):
+            assert rel(ss[q], sr[q]) < tol[mode], (mode, q)
+        if mode == 4:
+            assert dr[0].split() == ds[0].split()

@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 from numpy.polynomial import legendre as L
 
-from helpers import SOLUTION, STATE, case, g2_params, g3_params, rel
+from helpers import SOLUTION, STATE, case, g2_params, g3_params, rel, routine_inputs
 from iwrepo_amd.params import make_params
 
 # ---------------------------------------------------------------- GLL tables
@@ -233,3 +233,29 @@ def test_g3_forced_2d_run_against_reference_and_golden(oracle_mod):
         st, t, diag = oracle_mod.run_ref("g3", P, [zero] * 4, nsteps=9, probe=(32, 10, 50))
         ref_probe = np.array([[float(x) for x in l.split()] for l in diag if len(l.split()) == 3])
         assert np.abs(probes[:, 1:] - ref_probe[:, 1:]).max() < 1e-11 * np.abs(ref_probe[:, 1:]).max()
+
+
+@pytest.mark.parametrize("name", ["small", "c1"])
+def test_routines_match_compiled_reference(oracle_mod, name):
+    """horfft, dpdz, filter_3dlegenfourier and interface_avg3d of the oracle
+    against the same reference routines called one at a time (harness modes
+    1-4 after the reference set-up)."""
+    if not oracle_mod.ref_available(name):
+        pytest.skip("reference build %s not present (needs /root/reference)" % name)
+    P, flds = case(name)
+    o, spec = routine_inputs(oracle_mod, P, flds)
+    st, _, _ = oracle_mod.run_ref(name, P, flds, nsteps=0, mode=1)
+    for q in range(4):
+        assert rel(o.horfft(flds[q], -1), st[q]) < 1e-14
+    st, _, _ = oracle_mod.run_ref(name, P, spec, nsteps=0, mode=2)
+    for q in range(4):
+        assert rel(o.dpdz(spec[q]), st[q]) < 1e-13
+    st, _, _ = oracle_mod.run_ref(name, P, spec, nsteps=0, mode=3)
+    for q in range(4):
+        assert rel(o.filter3d(spec[q]), st[q]) < 1e-12
+    st, _, diag = oracle_mod.run_ref(name, P, spec, nsteps=0, mode=4)
+    ref_counts = [int(v) for v in diag[0].split()]
+    for q in range(4):
+        f, ic = o.interface_avg(spec[q])
+        assert ic == ref_counts[q] and ic > 0
+        assert rel(f, st[q]) < 1e-13
