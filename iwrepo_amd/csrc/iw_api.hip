@@ -404,6 +404,9 @@ static int build_systems(iw_solver *s, bool refactor_p, int &first, int &count)
         if (S.sysinfo) hipFree(S.sysinfo);
         size_t cap = (size_t)nsys;
         HIPCHK(hipMalloc((void **)&S.AB, sizeof(double) * cap * S.ldab * s->g.nz));
+        // band positions outside the matrix are never written by the
+        // factorisation and must read as zero in the lane solvers
+        HIPCHK(hipMemsetAsync(S.AB, 0, sizeof(double) * cap * S.ldab * s->g.nz, s->stream));
         HIPCHK(hipMalloc((void **)&S.ipiv, sizeof(int) * cap * s->g.nz));
         HIPCHK(hipMalloc((void **)&S.coef, sizeof(double) * cap * 12 * ns));
         HIPCHK(hipMalloc((void **)&S.sysinfo, sizeof(double) * cap * 4));
@@ -497,25 +500,44 @@ static int ensure_factors(iw_solver *s)
     for (int q = S.np; q < S.nsys && ok; q++)
         if (info[4 * q + 3] != 0.) ok = false;
     S.np_ok = ok;
-    if (ok) {
-        std::vector<std::pair<int, int>> cols;
-        for (int f = 0; f < 4; f++)
-            for (int m = 0; m < nmode; m++) {
-                if (f == 2 && m == 0) continue;
-                const int sys = ms[4 * m + (f < 2 ? 1 : f == 2 ? 2 : 3)];
-                for (int c = 0; c < 2; c++) cols.push_back({sys, (f * nmode + m) * 2 + c});
+    if (ok && !S.np_tasks) {
+        // lane order of the native-layout solver: a wavefront takes 16
+        // consecutive kx of the rows +ky and -ky (equal k^2, so equal
+        // systems; each row segment is 4 whole 128-byte lines), then the
+        // self-paired rows ky = 0 and ky = -ny/2, then the kx-Nyquist column
+        const Geo &g = s->g;
+        std::vector<int> cols;
+        auto push = [&](int jy, int ii) {
+            const int m = jy * g.nxhp + ii;
+            cols.push_back(2 * m);
+            cols.push_back(2 * m + 1);
+        };
+        const int B = 16;
+        for (int p = 1; p < (g.ny + 1) / 2; p++) {
+            const int q = g.ny - p;
+            if (q == p) continue;
+            for (int i0 = 0; i0 < g.nxh; i0 += B) {
+                for (int ii = i0; ii < std::min(i0 + B, g.nxh); ii++) push(p, ii);
+                for (int ii = i0; ii < std::min(i0 + B, g.nxh); ii++) push(q, ii);
             }
-        std::stable_sort(cols.begin(), cols.end());
+        }
+        std::vector<int> single = {0};
+        if (g.ny > 1) single.push_back(g.ny / 2);
+        for (int jy : single)
+            for (int ii = 0; ii < g.nxh; ii++) push(jy, ii);
+        for (int jy : single) push(jy, g.nxh);
+        for (int p = 1; p < (g.ny + 1) / 2; p++) {
+            const int q = g.ny - p;
+            if (q == p) continue;
+            push(p, g.nxh);
+            push(q, g.nxh);
+        }
+        if ((long)cols.size() != g.nxppy) return fail("internal: solver lane table does not cover the plane");
         std::vector<int> tasks((cols.size() + 63) / 64 * 64, -1);
-        for (size_t i = 0; i < cols.size(); i++) tasks[i] = cols[i].second;
-        if (S.np_tasks) hipFree(S.np_tasks);
+        std::copy(cols.begin(), cols.end(), tasks.begin());
         HIPCHK(hipMalloc((void **)&S.np_tasks, sizeof(int) * tasks.size()));
         HIPCHK(hipMemcpy(S.np_tasks, tasks.data(), sizeof(int) * tasks.size(), hipMemcpyHostToDevice));
         S.np_ntasks = (int)tasks.size();
-        if (!s->cm) {
-            HIPCHK(hipMalloc((void **)&s->cm, 4 * sizeof(double) * (size_t)s->g.ntot));
-            s->dev_bytes += 4 * sizeof(double) * (long long)s->g.ntot;
-        }
     }
     s->factored_bdfv = s->bdfv;
     s->factored_dt = s->P.dt;
@@ -742,7 +764,6 @@ extern "C" int iw_destroy(iw_solver *s)
     hipFree(s->sys.order_p);
     hipFree(s->sys.order_v);
     hipFree(s->sys.np_tasks);
-    hipFree(s->cm);
     hipEventDestroy(s->ev0);
     hipEventDestroy(s->ev1);
     hipStreamDestroy(s->stream);
@@ -1037,18 +1058,15 @@ static int calcuvwt_dev(iw_solver *s, int cpen)
     }
     if (s->sys.np_ok) {
         {
-            Scope sc(s, "to_modemajor", 64. * nt);
-            launch_to_modemajor(s->g, s->P, s->sys, uvwt, s->cm, s->P.dt, s->stream);
-        }
-        {
-            // no-pivot factors: L multipliers and U band (kl + ku + 1 rows per column)
+            // right-hand sides in, solutions out (16 B per point and field)
+            // + each distinct no-interchange factor once ((2 kl + 1) rows)
             const double fnp = (double)(2 * s->sys.kl + 1) * s->g.nz * 8.;
-            Scope sc(s, "solve_np", 4 * 16. * nt + (s->sys.nsys - s->sys.np) * fnp);
-            launch_solve_np(s->g, s->sys, s->cm, s->stream);
+            Scope sc(s, "solve_nat", 4 * 16. * nt + (s->sys.nsys - s->sys.np) * fnp);
+            launch_solve_nat(s->g, s->P, s->sys, uvwt, s->P.dt, s->stream);
         }
         {
-            Scope sc(s, "from_modemajor", 64. * nt);
-            launch_from_modemajor(s->g, s->cm, uvwt, cpen, s->stream);
+            Scope sc(s, "lfilter", 4 * 16. * nt);
+            launch_lfilter(s->g, uvwt, cpen, s->stream);
         }
     } else {
         Scope sc(s, "viscous", 64. * nt + (s->sys.nsys - s->sys.np) * fcol);

@@ -77,7 +77,6 @@ struct iw_solver {
     double *ox;
     double *wnl, *wviscrot;
     double *hbuf;      // device scratch for host-array entry points
-    double *cm = nullptr; // mode-major right-hand sides of the u,v,w,rho solves
     double *phys[4];      // physical u,v,w,rho during convec
     double *icomp;        // 4 planes per interface x 4 fields (interface averaging)
     int *iface_src, *iface_mid_src, *iface_mid_dst; // plane maps
@@ -128,8 +127,7 @@ void launch_viscous(const Geo &g, const iw_params &P, const SysTables &S, double
 void launch_zero_meanw(const Geo &g, double *w, hipStream_t st);
 void launch_zero_int(int *p, int n, hipStream_t st);
 bool solve_np_supported(int nl, int nsubd);
-void launch_solve_np(const Geo &g, const SysTables &S, double *cm, hipStream_t st);
-void launch_to_modemajor(const Geo &g, const iw_params &P, const SysTables &S, double *const *uvwt, double *cm,
-                         double dt, hipStream_t st);
-void launch_from_modemajor(const Geo &g, const double *cm, double *const *uvwt, int cpen, hipStream_t st);
+void launch_solve_nat(const Geo &g, const iw_params &P, const SysTables &S, double *const *uvwt, double dt,
+                      hipStream_t st);
+void launch_lfilter(const Geo &g, double *const *uvwt, int cpen, hipStream_t st);
 } // namespace iw

@@ -1218,234 +1218,221 @@ void launch_viscous(const Geo &g, const iw_params &P, const SysTables &S, double
 }
 
 // ---------------------------------------------------------------------------
-// Solves without row interchanges (the u/v, w and density operators are
-// diagonally dominant: SGBTF2 never swaps rows for them, so ipiv(j) = j and
-// U keeps ku superdiagonals).  One lane per right-hand-side column: the lane
-// runs the forward (bandludcmp.f:1020-1030) and backward (ludcmp.f:4698-5046)
-// substitutions itself with the band window in registers; lanes are grouped
-// by system so a factor column is fetched once per wave.  Columns live in a
-// mode-major scratch cm[col][k], col = (field*nmode + mode)*2 + comp.
+// Solves without row interchanges: the u/v, w and density operators are
+// diagonally dominant, so SGBTF2 never swaps rows for them (ipiv(j) = j) and
+// U keeps ku = kl superdiagonals.  Lane = one right-hand-side column
+// (re or im of one mode) of one field, read and written in place at
+// f[k*nxppy + col]; the host orders the lanes so that a wavefront holds 16
+// consecutive kx of the rows +ky and -ky (modes with equal k^2, hence equal
+// systems), so every row access is 4 contiguous 128-byte lines and lanes
+// sharing a system fetch the same factor segment.  The right-hand side
+// -(-f/(nu dt)) eps (calcuvwt.f:278-281, 612-615, penalty.f:191-194) is
+// formed as each row enters the band window.  Factor segments are read as
+// 16-byte loads one step ahead of use; band positions outside the matrix
+// are zero in AB, so no masking of the factors is needed.
 // ---------------------------------------------------------------------------
-struct NpArgs {
-    double *cm;
-    const int *tasks;      // lane -> column id (-1 idle)
+struct NatArgs {
+    double *f[4];
+    const int *tasks;
     const int *modesys;
     const double *AB;
-    int n, ldab, kv, nmode;
+    const double *sysinfo;
+    double den[4];
+    long nxppy;
+    int n, ldab, kv;
 };
 
+template <int NV>
+__device__ __forceinline__ void ld_seg(const double *__restrict__ p, double (&c)[NV])
+{
+    const double2 *q = reinterpret_cast<const double2 *>(p);
+#pragma unroll
+    for (int i = 0; i < NV / 2; i++) {
+        const double2 v = q[i];
+        c[2 * i] = v.x;
+        c[2 * i + 1] = v.y;
+    }
+}
+
 template <int KL>
-__global__ __launch_bounds__(64) void k_solve_np(NpArgs A)
+__global__ __launch_bounds__(64) void k_solve_nat(NatArgs A)
 {
-    // requires n > 2*KL+2 (checked on the host): every read below stays inside
-    // the column (factor rows 0..ldab-1, right-hand side 0..n-1); values that
-    // lie outside the band are masked to zero after an unconditional load
     const int col = A.tasks[blockIdx.x * 64 + threadIdx.x];
+    const int fld = blockIdx.y;
     if (col < 0) return;
+    const int m = col >> 1;
+    if (fld == 2 && m == 0) return; // w(0,0) = 0 (calcuvwt.f:496-503)
+    const int sys = A.modesys[4 * m + (fld < 2 ? 1 : fld == 2 ? 2 : 3)];
+    const double eps = A.sysinfo[4 * sys], den = A.den[fld];
     const int n = A.n, kv = A.kv, ldab = A.ldab;
-    const int f = col / (2 * A.nmode), m = (col >> 1) % A.nmode;
-    const int sys = A.modesys[4 * m + (f < 2 ? 1 : f == 2 ? 2 : 3)];
+    const long st = A.nxppy;
     const double *__restrict__ ab = A.AB + (long)sys * ldab * n;
-    double *__restrict__ b = A.cm + (long)col * n;
-    double w[KL + 1], lc[KL], ln[KL];
-    // forward sweep; multipliers and the next right-hand-side entry of step
-    // j+1 are loaded before step j's arithmetic and store
+    double *__restrict__ b = A.f[fld] + col;
+    constexpr int NF = KL + 1; // forward segment: rows kv..kv+KL of column j
+    constexpr int NB = KL + 3; // backward segment: rows kv-KL-1..kv+1 of column j
+    double w[KL + 1];
+    // ---- forward: unit lower triangle (bandludcmp.f:1020-1030); the factor
+    // segment and the entering right-hand-side row of step j+1 are loaded
+    // before the arithmetic of step j
 #pragma unroll
-    for (int t = 0; t <= KL; t++) w[t] = b[t];
-#pragma unroll
-    for (int t = 1; t <= KL; t++) lc[t - 1] = ab[kv + t];
-    double nbn = b[KL + 1];
-#pragma unroll 2
-    for (int j = 0; j < n; j++) {
-        const double nb = (j + KL + 1 < n) ? nbn : 0.;
-        const int jn = (j + 1 < n) ? j + 1 : j;
-        const double *lcn = ab + (long)ldab * jn + kv;
-        const int lmn = n - 2 - j;
-#pragma unroll
-        for (int t = 1; t <= KL; t++) {
-            const double lv = lcn[t];
-            ln[t - 1] = (t <= lmn) ? lv : 0.;
-        }
-        const int in = (j + KL + 2 < n) ? j + KL + 2 : n - 1;
-        nbn = b[in];
-        const double y = w[0];
-#pragma unroll
-        for (int t = 1; t <= KL; t++) w[t - 1] = w[t] + lc[t - 1] * (-y);
-        w[KL] = nb;
-        b[j] = y;
-#pragma unroll
-        for (int t = 0; t < KL; t++) lc[t] = ln[t];
+    for (int t = 0; t <= KL; t++) {
+        const double v = b[(long)(t < n ? t : n - 1) * st];
+        w[t] = (t < n) ? -(-v / den) * eps : 0.;
     }
-    // backward sweep (U with ku = KL superdiagonals)
+    {
+        double f0[NF], f1[NF];
+        double r0, r1;
+        ld_seg<NF>(ab + kv, f0);
+        r0 = b[(long)(KL + 1 < n ? KL + 1 : n - 1) * st];
+        int j = 0;
+        auto step = [&](double (&fc)[NF], double (&fn)[NF], double &rc, double &rn) {
+            const int jp = j + 1 < n ? j + 1 : n - 1;
+            ld_seg<NF>(ab + (long)ldab * jp + kv, fn);
+            const int rp = j + KL + 2 < n ? j + KL + 2 : n - 1;
+            rn = b[(long)rp * st];
+            const double y = w[0];
 #pragma unroll
-    for (int t = 0; t <= KL; t++) w[t] = b[n - 1 - t];
-    double dc = ab[(long)ldab * (n - 1) + kv], dn;
-#pragma unroll
-    for (int t = 1; t <= KL; t++) lc[t - 1] = ab[(long)ldab * (n - 1) + kv - t];
-    nbn = b[n - 2 - KL];
-#pragma unroll 2
-    for (int j = n - 1; j >= 0; j--) {
-        const double nb = (j - KL - 1 >= 0) ? nbn : 0.;
-        const int jn = (j >= 1) ? j - 1 : 0;
-        const double *ucn = ab + (long)ldab * jn + kv;
-        dn = ucn[0];
-        const int lun = j - 1;
-#pragma unroll
-        for (int t = 1; t <= KL; t++) {
-            const double uv = ucn[-t];
-            ln[t - 1] = (t <= lun) ? uv : 0.;
+            for (int t = 1; t <= KL; t++) w[t - 1] = w[t] + fc[t] * (-y);
+            w[KL] = (j + KL + 1 < n) ? -(-rc / den) * eps : 0.;
+            b[(long)j * st] = y;
+            j++;
+        };
+        while (j < n) {
+            step(f0, f1, r0, r1);
+            if (j >= n) break;
+            step(f1, f0, r1, r0);
         }
-        const int in = (j - KL - 2 >= 0) ? j - KL - 2 : 0;
-        nbn = b[in];
-        const double x = (w[0] != 0.) ? w[0] / dc : w[0];
+    }
+    // ---- backward: upper band with KL superdiagonals (STBSV, ludcmp.f:4698-5046)
 #pragma unroll
-        for (int t = 1; t <= KL; t++) w[t - 1] = w[t] - x * lc[t - 1];
-        w[KL] = nb;
-        b[j] = x;
-        dc = dn;
+    for (int t = 0; t <= KL; t++) {
+        const int r = n - 1 - t;
+        const double v = b[(long)(r >= 0 ? r : 0) * st];
+        w[t] = (r >= 0) ? v : 0.;
+    }
+    {
+        double g0[NB], g1[NB];
+        double r0, r1;
+        const int o = kv - KL - 1;
+        ld_seg<NB>(ab + (long)ldab * (n - 1) + o, g0);
+        r0 = b[(long)(n - 2 - KL >= 0 ? n - 2 - KL : 0) * st];
+        int j = n - 1;
+        auto step = [&](double (&gc)[NB], double (&gn)[NB], double &rc, double &rn) {
+            const int jp = j - 1 >= 0 ? j - 1 : 0;
+            ld_seg<NB>(ab + (long)ldab * jp + o, gn);
+            const int rp = j - KL - 2 >= 0 ? j - KL - 2 : 0;
+            rn = b[(long)rp * st];
+            const double dc = gc[KL + 1];
+            const double x = (w[0] != 0.) ? w[0] / dc : w[0];
 #pragma unroll
-        for (int t = 0; t < KL; t++) lc[t] = ln[t];
+            for (int t = 1; t <= KL; t++) w[t - 1] = w[t] - x * gc[KL + 1 - t];
+            w[KL] = (j - KL - 1 >= 0) ? rc : 0.;
+            b[(long)j * st] = x;
+            j--;
+        };
+        while (j >= 0) {
+            step(g0, g1, r0, r1);
+            if (j < 0) break;
+            step(g1, g0, r1, r0);
+        }
     }
 }
 
-bool solve_np_supported(int nl, int nsubd)
+void launch_solve_nat(const Geo &g, const iw_params &P, const SysTables &S, double *const *uvwt, double dt,
+                      hipStream_t st)
 {
-    return (nl == 9 || nl == 13 || nl == 17 || nl == 21 || nl == 25) && nsubd >= 3;
-}
-
-void launch_solve_np(const Geo &g, const SysTables &S, double *cm, hipStream_t st)
-{
-    NpArgs A;
-    A.cm = cm;
+    NatArgs A;
+    for (int q = 0; q < 4; q++) A.f[q] = uvwt[q];
     A.tasks = S.np_tasks;
     A.modesys = S.modesys;
     A.AB = S.AB;
+    A.sysinfo = S.sysinfo;
+    A.den[0] = A.den[1] = A.den[2] = P.xnu * dt;
+    A.den[3] = dt * P.xkappa;
+    A.nxppy = g.nxppy;
     A.n = g.nz;
     A.ldab = S.ldab;
     A.kv = S.kv;
-    A.nmode = g.nxhp * g.ny;
-    dim3 grid(S.np_ntasks / 64);
+    dim3 grid(S.np_ntasks / 64, 4);
     switch (g.nzloc) {
-    case 9: hipLaunchKernelGGL(k_solve_np<9>, grid, dim3(64), 0, st, A); break;
-    case 13: hipLaunchKernelGGL(k_solve_np<13>, grid, dim3(64), 0, st, A); break;
-    case 17: hipLaunchKernelGGL(k_solve_np<17>, grid, dim3(64), 0, st, A); break;
-    case 21: hipLaunchKernelGGL(k_solve_np<21>, grid, dim3(64), 0, st, A); break;
-    case 25: hipLaunchKernelGGL(k_solve_np<25>, grid, dim3(64), 0, st, A); break;
+    case 9: hipLaunchKernelGGL(k_solve_nat<9>, grid, dim3(64), 0, st, A); break;
+    case 13: hipLaunchKernelGGL(k_solve_nat<13>, grid, dim3(64), 0, st, A); break;
+    case 17: hipLaunchKernelGGL(k_solve_nat<17>, grid, dim3(64), 0, st, A); break;
+    case 21: hipLaunchKernelGGL(k_solve_nat<21>, grid, dim3(64), 0, st, A); break;
+    case 25: hipLaunchKernelGGL(k_solve_nat<25>, grid, dim3(64), 0, st, A); break;
     default: break;
     }
 }
 
-// native a(nxpp,ny,nz) of u,v,w,rho -> mode-major right-hand sides
-// -(-f/(nu dt)) eps (calcuvwt.f:278-281, 612-615 and penalty.f:191-194),
-// through a 32 mode x 32 level LDS tile
-__global__ __launch_bounds__(256) void k_to_modemajor(const double *u, const double *v, const double *w,
-                                                      const double *t, double *cm, const int *__restrict__ modesys,
-                                                      const double *__restrict__ sysinfo, double xnudt,
-                                                      double dtkap, int nz, int nmode, long nxppy)
+// Legendre filter of every subdomain of the four solutions (filter_legenpen,
+// filtering.f:729-808) and the mean-u damping (calcuvwt.f:343-361), in place
+// on the native layout: lane = one column, the nl values of its subdomain in
+// registers, the (uniform) tables read through the scalar cache.
+template <int NL>
+__global__ __launch_bounds__(256) void k_lfilter(double *u, double *v, double *w, double *t, long nxppy,
+                                                int cpen, const double *__restrict__ wg,
+                                                const double *__restrict__ AL,
+                                                const double *__restrict__ sig,
+                                                const double *__restrict__ damp)
 {
-    __shared__ double tile[2][32][33];
-    const int f = blockIdx.z;
-    const double *src = f == 0 ? u : f == 1 ? v : f == 2 ? w : t;
-    const double den = f < 3 ? xnudt : dtkap;
-    const int m0 = blockIdx.x * 32, k0 = blockIdx.y * 32;
-    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5; // 32 x 8
-    for (int kk = ty; kk < 32; kk += 8) {
-        const int k = k0 + kk, m = m0 + tx;
-        double re = 0., im = 0.;
-        if (k < nz && m < nmode) {
-            const long o = (long)k * nxppy + 2L * m;
-            re = src[o];
-            im = src[o + 1];
-        }
-        tile[0][kk][tx] = re;
-        tile[1][kk][tx] = im;
-    }
-    __syncthreads();
-    for (int mm = ty; mm < 32; mm += 8) {
-        const int m = m0 + mm, k = k0 + tx;
-        if (m >= nmode || k >= nz) continue;
-        const int sys = modesys[4 * m + (f < 2 ? 1 : f == 2 ? 2 : 3)];
-        const double eps = sys >= 0 ? sysinfo[4 * sys] : 0.;
-        const long c = ((long)f * nmode + m) * 2;
-        cm[c * nz + k] = -(-tile[0][tx][mm] / den) * eps;
-        cm[(c + 1) * nz + k] = -(-tile[1][tx][mm] / den) * eps;
-    }
-}
-
-void launch_to_modemajor(const Geo &g, const iw_params &P, const SysTables &S, double *const *uvwt, double *cm,
-                         double dt, hipStream_t st)
-{
-    const int nmode = g.nxhp * g.ny;
-    dim3 grid((nmode + 31) / 32, (g.nz + 31) / 32, 4);
-    hipLaunchKernelGGL(k_to_modemajor, grid, dim3(256), 0, st, uvwt[0], uvwt[1], uvwt[2], uvwt[3], cm,
-                       S.modesys, S.sysinfo, P.xnu * dt, dt * P.xkappa, g.nz, nmode, g.nxppy);
-}
-
-// mode-major solutions -> native layout with the Legendre filter of every
-// subdomain (filter_legenpen, filtering.f:729-808) and the mean-u damping
-// (calcuvwt.f:343-361); the tables are staged in LDS
-__global__ __launch_bounds__(256) void k_from_modemajor(const double *cm, double *u, double *v, double *w,
-                                                        double *t, int nz, int nl, int nmode, long nxppy,
-                                                        int cpen, const double *__restrict__ wg_,
-                                                        const double *__restrict__ AL_,
-                                                        const double *__restrict__ sig_,
-                                                        const double *__restrict__ damp)
-{
-    __shared__ double tile[kMaxNl][2 * kMB + 1];
-    __shared__ double coef[kMaxNl][2 * kMB + 1];
-    __shared__ double AL[kMaxNl * kMaxNl], wg[kMaxNl], sig[kMaxNl];
-    const int f = blockIdx.z, ks = blockIdx.y;
-    double *dst = f == 0 ? u : f == 1 ? v : f == 2 ? w : t;
-    const int m0 = blockIdx.x * kMB;
-    for (int e = threadIdx.x; e < nl * nl; e += blockDim.x) AL[e] = AL_[e];
-    for (int e = threadIdx.x; e < nl; e += blockDim.x) {
-        wg[e] = wg_[e];
-        sig[e] = sig_[e];
-    }
-    for (int e = threadIdx.x; e < nl * 2 * kMB; e += blockDim.x) { // e -> (column cc, level kk)
-        const int cc = e / nl, kk = e - cc * nl;
-        const int m = m0 + (cc >> 1);
-        double val = 0.;
-        if (m < nmode) val = cm[(((long)f * nmode + m) * 2 + (cc & 1)) * nz + ks * nl + kk];
-        tile[kk][cc] = val;
-    }
-    __syncthreads();
+    const long col = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int ks = blockIdx.y, fld = blockIdx.z;
+    if (col >= nxppy) return;
+    const int m = (int)(col >> 1);
+    if (fld == 2 && m == 0) return;
+    double *f = (fld == 0 ? u : fld == 1 ? v : fld == 2 ? w : t) + (long)ks * NL * nxppy + col;
+    double x[NL];
+#pragma unroll
+    for (int k = 0; k < NL; k++) x[k] = f[(long)k * nxppy];
     if (cpen) {
-        const int nzm = nl - 1;
-        for (int e = threadIdx.x; e < nl * 2 * kMB; e += blockDim.x) {
-            const int k = e / (2 * kMB), cc = e - k * 2 * kMB;
+        double a[NL];
+        constexpr int nzm = NL - 1;
+#pragma unroll
+        for (int q = 0; q < NL; q++) x[q] = wg[q] * x[q];
+#pragma unroll
+        for (int k = 0; k < NL; k++) {
             double sum = 0.;
-            for (int tt = 0; tt < nl; tt++) sum = sum + wg[tt] * tile[tt][cc] * AL[k * nl + tt];
-            const double a = (k == nzm) ? 0.5 * nzm * sum : (k + 0.5) * sum;
-            coef[k][cc] = a * sig[k];
+#pragma unroll
+            for (int q = 0; q < NL; q++) sum = sum + x[q] * AL[k * NL + q];
+            const double c = (k == nzm) ? 0.5 * nzm * sum : (k + 0.5) * sum;
+            a[k] = c * sig[k];
         }
-        __syncthreads();
-        for (int e = threadIdx.x; e < nl * 2 * kMB; e += blockDim.x) {
-            const int tt = e / (2 * kMB), cc = e - tt * 2 * kMB;
+#pragma unroll
+        for (int q = 0; q < NL; q++) {
             double sum = 0.;
-            for (int k = 0; k < nl; k++) sum = sum + coef[k][cc] * AL[k * nl + tt];
-            tile[tt][cc] = sum;
+#pragma unroll
+            for (int k = 0; k < NL; k++) sum = sum + a[k] * AL[k * NL + q];
+            x[q] = sum;
         }
-        __syncthreads();
     }
-    for (int e = threadIdx.x; e < nl * 2 * kMB; e += blockDim.x) {
-        const int kk = e / (2 * kMB), cc = e - kk * 2 * kMB;
-        const int m = m0 + (cc >> 1);
-        if (m >= nmode) continue;
-        if (f == 2 && m == 0) continue; // w(0,0) stays zero (calcuvwt.f:496-503)
-        double val = tile[kk][cc];
-        const int k = ks * nl + kk;
-        if (f == 0 && m == 0) val = damp[k] * val;
-        dst[(long)k * nxppy + 2L * m0 + cc] = val;
-    }
+    if (fld == 0 && m == 0)
+#pragma unroll
+        for (int k = 0; k < NL; k++) x[k] = damp[ks * NL + k] * x[k];
+#pragma unroll
+    for (int k = 0; k < NL; k++) f[(long)k * nxppy] = x[k];
 }
 
-void launch_from_modemajor(const Geo &g, const double *cm, double *const *uvwt, int cpen, hipStream_t st)
+void launch_lfilter(const Geo &g, double *const *uvwt, int cpen, hipStream_t st)
 {
-    const int nmode = g.nxhp * g.ny;
-    dim3 grid((nmode + kMB - 1) / kMB, g.nsubd, 4);
-    hipLaunchKernelGGL(k_from_modemajor, grid, dim3(256), 0, st, cm, uvwt[0], uvwt[1], uvwt[2], uvwt[3], g.nz,
-                       g.nzloc, nmode, g.nxppy, cpen, g.wg, g.AL, g.sig, g.damp);
+    dim3 grid((unsigned)((g.nxppy + 255) / 256), g.nsubd, 4);
+#define LF(NL)                                                                                             \
+    hipLaunchKernelGGL(k_lfilter<NL>, grid, dim3(256), 0, st, uvwt[0], uvwt[1], uvwt[2], uvwt[3], g.nxppy, \
+                       cpen, g.wg, g.AL, g.sig, g.damp)
+    switch (g.nzloc) {
+    case 9: LF(9); break;
+    case 13: LF(13); break;
+    case 17: LF(17); break;
+    case 21: LF(21); break;
+    case 25: LF(25); break;
+    default: break;
+    }
+#undef LF
+}
+
+bool solve_np_supported(int nl, int nsubd)
+{
+    (void)nsubd;
+    return nl == 9 || nl == 13 || nl == 17 || nl == 21 || nl == 25;
 }
 
 // ---------------------------------------------------------------------------
