@@ -500,7 +500,7 @@ static int ensure_factors(iw_solver *s)
     for (int q = S.np; q < S.nsys && ok; q++)
         if (info[4 * q + 3] != 0.) ok = false;
     S.np_ok = ok;
-    if (ok && !S.np_tasks) {
+    if (solve_np_supported(s->P.nzloc, s->P.nsubd) && !S.np_tasks) {
         // lane order of the native-layout solver: a wavefront takes 16
         // consecutive kx of the rows +ky and -ky (equal k^2, so equal
         // systems; each row segment is 4 whole 128-byte lines), then the
@@ -1052,7 +1052,13 @@ static int calcuvwt_dev(iw_solver *s, int cpen)
     // algorithmic bytes: state columns in and out + every distinct cached factor once
     const double fcol = (double)s->sys.ldab * s->g.nz * 8. + s->g.nz * 4.;
     const double nt = (double)s->g.ntot;
-    {
+    if (s->sys.np_tasks) {
+        // u,v,w read twice and written once, phi written, read and written
+        // by the solve and read again: 80 B per point; each P factor once
+        Scope sc(s, "pressure", 80. * nt + s->sys.np * fcol);
+        launch_pressure_nat(s->g, s->P, s->sys, uvwt, s->grad[0], s->wnl, s->wviscrot, s->P.dt, cpen,
+                            s->stream);
+    } else {
         Scope sc(s, "pressure", 56. * nt + s->sys.np * fcol); // u,v,w in; u,v,w out; w re-read
         launch_pressure(s->g, s->P, s->sys, uvwt, s->wnl, s->wviscrot, s->P.dt, cpen, s->stream);
     }

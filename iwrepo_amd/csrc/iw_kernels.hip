@@ -1429,6 +1429,274 @@ void launch_lfilter(const Geo &g, double *const *uvwt, int cpen, hipStream_t st)
 #undef LF
 }
 
+// ---------------------------------------------------------------------------
+// Pressure step on the native layout (calcuvwt.f:87-200 with update,
+// calcuvwt.f:826-901), three kernels:
+//   k_prhs:     divergence (divcolc, derv.f:35-85) of u*, scaled right-hand
+//               side -(div/dt) eps and the Neumann data wnl + wviscrot
+//               (calcuvwt.f:112-159, penalty.f:191-194, 583-598) -> phi;
+//   k_solve_p:  SGBTRS with row interchanges (bandludcmp.f:1020-1036), lane =
+//               one column, the same lane order as k_solve_nat;
+//   k_pproject: Legendre filter of phi, dphi/dz and u* - dt grad(phi).
+// ---------------------------------------------------------------------------
+struct PArgs {
+    const double *u, *v, *w;
+    double *phi;
+    const double *wnl, *wviscrot;
+    const int *modesys;
+    const double *sysinfo;
+    const double *D, *zh, *xw, *yw;
+    double dt;
+    long nxppy;
+    int nxhp, nz;
+};
+
+template <int NL>
+__global__ __launch_bounds__(256) void k_prhs(PArgs A)
+{
+    const long col = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int ks = blockIdx.y;
+    if (col >= A.nxppy) return;
+    const int m = (int)(col >> 1), comp = (int)(col & 1);
+    const long st = A.nxppy, base = (long)ks * NL * st;
+    double *phi = A.phi + base + col;
+    if (m == 0) { // no pressure for kx = ky = 0 (calcuvwt.f:105-109)
+#pragma unroll
+        for (int k = 0; k < NL; k++) phi[(long)k * st] = 0.;
+        return;
+    }
+    const int sP = A.modesys[4 * m];
+    const double eps = A.sysinfo[4 * sP];
+    const double xw = A.xw[m % A.nxhp], yw = A.yw[m / A.nxhp];
+    const double fac = 2. / A.zh[ks];
+    double wc[NL];
+#pragma unroll
+    for (int t = 0; t < NL; t++) wc[t] = A.w[base + (long)t * st + col];
+    const long pc = col ^ 1;
+#pragma unroll
+    for (int k = 0; k < NL; k++) {
+        double sd = 0.;
+#pragma unroll
+        for (int t = 0; t < NL; t++) sd = sd + A.D[k * NL + t] * wc[t];
+        const long o = base + (long)k * st;
+        double d;
+        if (comp == 0)
+            d = (0. + (-xw * A.u[o + pc])) + (-yw * A.v[o + pc]);
+        else
+            d = (0. + xw * A.u[o + pc]) + yw * A.v[o + pc];
+        d = d + sd * fac;
+        double r = -(d / A.dt) * eps;
+        const int kg = ks * NL + k;
+        if (kg == 0) r = r - A.sysinfo[4 * sP + 1] * (A.wnl[col] + A.wviscrot[col]) * eps;
+        if (kg == A.nz - 1) r = r + A.sysinfo[4 * sP + 2] * (A.wnl[st + col] + A.wviscrot[st + col]) * eps;
+        phi[(long)k * st] = r;
+    }
+}
+
+struct SolPArgs {
+    double *phi;
+    const int *tasks;
+    const int *modesys;
+    const double *AB;
+    const int *ipiv;
+    long nxppy;
+    int n, ldab, kv;
+};
+
+template <int KL>
+__global__ __launch_bounds__(64) void k_solve_p(SolPArgs A)
+{
+    const int col = A.tasks[blockIdx.x * 64 + threadIdx.x];
+    if (col < 0) return;
+    const int m = col >> 1;
+    if (m == 0) return;
+    const int sys = A.modesys[4 * m];
+    const int n = A.n, kv = A.kv, ldab = A.ldab;
+    const long st = A.nxppy;
+    const double *__restrict__ ab = A.AB + (long)sys * ldab * n;
+    const int *__restrict__ ip = A.ipiv + (long)sys * n;
+    double *__restrict__ b = A.phi + col;
+    constexpr int NF = KL + 1;     // rows kv..kv+KL of column j (L multipliers)
+    constexpr int KV = 2 * KL;     // superdiagonals of U after interchanges
+    constexpr int NB = KV + 2;     // rows 0..kv+1 of column j (U column)
+    // ---- forward: L with row interchanges (bandludcmp.f:1020-1030)
+    {
+        double w[KL + 1];
+#pragma unroll
+        for (int t = 0; t <= KL; t++) {
+            const double v = b[(long)(t < n ? t : n - 1) * st];
+            w[t] = (t < n) ? v : 0.;
+        }
+        double f0[NF], f1[NF];
+        double r0, r1;
+        int p0, p1;
+        ld_seg<NF>(ab + kv, f0);
+        r0 = b[(long)(KL + 1 < n ? KL + 1 : n - 1) * st];
+        p0 = ip[0];
+        int j = 0;
+        auto step = [&](double (&fc)[NF], double (&fn)[NF], double &rc, double &rn, int &pc, int &pn) {
+            const int jp = j + 1 < n ? j + 1 : n - 1;
+            ld_seg<NF>(ab + (long)ldab * jp + kv, fn);
+            pn = ip[jp];
+            const int rp = j + KL + 2 < n ? j + KL + 2 : n - 1;
+            rn = b[(long)rp * st];
+            const int d = pc - j; // 0 <= d <= KL
+            double y = w[0];
+#pragma unroll
+            for (int t = 1; t <= KL; t++) y = (d == t) ? w[t] : y;
+            const double w0 = w[0];
+#pragma unroll
+            for (int t = 1; t <= KL; t++) {
+                const double wt = (d == t) ? w0 : w[t];
+                w[t - 1] = wt + fc[t] * (-y);
+            }
+            w[KL] = (j + KL + 1 < n) ? rc : 0.;
+            b[(long)j * st] = y;
+            j++;
+        };
+        while (j < n) {
+            step(f0, f1, r0, r1, p0, p1);
+            if (j >= n) break;
+            step(f1, f0, r1, r0, p1, p0);
+        }
+    }
+    // ---- backward: U with kv = 2 kl superdiagonals (STBSV, ludcmp.f:4698-5046)
+    {
+        double w[KV + 1];
+#pragma unroll
+        for (int t = 0; t <= KV; t++) {
+            const int r = n - 1 - t;
+            const double v = b[(long)(r >= 0 ? r : 0) * st];
+            w[t] = (r >= 0) ? v : 0.;
+        }
+        double g0[NB], g1[NB];
+        double r0, r1;
+        ld_seg<NB>(ab + (long)ldab * (n - 1), g0);
+        r0 = b[(long)(n - 2 - KV >= 0 ? n - 2 - KV : 0) * st];
+        int j = n - 1;
+        auto step = [&](double (&gc)[NB], double (&gn)[NB], double &rc, double &rn) {
+            const int jp = j - 1 >= 0 ? j - 1 : 0;
+            ld_seg<NB>(ab + (long)ldab * jp, gn);
+            const int rp = j - KV - 2 >= 0 ? j - KV - 2 : 0;
+            rn = b[(long)rp * st];
+            const double dc = gc[KV];
+            const double x = (w[0] != 0.) ? w[0] / dc : w[0];
+#pragma unroll
+            for (int t = 1; t <= KV; t++) w[t - 1] = w[t] - x * gc[KV - t];
+            w[KV] = (j - KV - 1 >= 0) ? rc : 0.;
+            b[(long)j * st] = x;
+            j--;
+        };
+        while (j >= 0) {
+            step(g0, g1, r0, r1);
+            if (j < 0) break;
+            step(g1, g0, r1, r0);
+        }
+    }
+}
+
+template <int NL>
+__global__ __launch_bounds__(256) void k_pproject(PArgs A, double *u, double *v, double *w, int cpen,
+                                                 int noslip, const double *__restrict__ wg,
+                                                 const double *__restrict__ AL,
+                                                 const double *__restrict__ sig)
+{
+    const long col = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int ks = blockIdx.y;
+    const bool live = col < A.nxppy;
+    const long c = live ? col : 0;
+    const int m = (int)(c >> 1), comp = (int)(c & 1);
+    const long st = A.nxppy, base = (long)ks * NL * st;
+    double x[NL];
+#pragma unroll
+    for (int k = 0; k < NL; k++) x[k] = (m == 0) ? 0. : A.phi[base + (long)k * st + c];
+    if (cpen && m != 0) { // filter_legenpen (filtering.f:729-808)
+        double a[NL];
+        constexpr int nzm = NL - 1;
+        double y[NL];
+#pragma unroll
+        for (int q = 0; q < NL; q++) y[q] = wg[q] * x[q];
+#pragma unroll
+        for (int k = 0; k < NL; k++) {
+            double sum = 0.;
+#pragma unroll
+            for (int q = 0; q < NL; q++) sum = sum + y[q] * AL[k * NL + q];
+            const double cc = (k == nzm) ? 0.5 * nzm * sum : (k + 0.5) * sum;
+            a[k] = cc * sig[k];
+        }
+#pragma unroll
+        for (int q = 0; q < NL; q++) {
+            double sum = 0.;
+#pragma unroll
+            for (int k = 0; k < NL; k++) sum = sum + a[k] * AL[k * NL + q];
+            x[q] = sum;
+        }
+    }
+    const double xw = A.xw[m % A.nxhp], yw = A.yw[m / A.nxhp];
+    const double cx = A.dt * xw, cy = A.dt * yw;
+    const double fac = 2. / A.zh[ks];
+#pragma unroll
+    for (int k = 0; k < NL; k++) {
+        const double other = __shfl_xor(x[k], 1); // phi of the partner component
+        double sd = 0.;
+#pragma unroll
+        for (int t = 0; t < NL; t++) sd = sd + A.D[k * NL + t] * x[t];
+        if (!live) continue;
+        const long o = base + (long)k * st + c;
+        const int kg = ks * NL + k;
+        if (noslip && kg == 0) { // rows the no-slip variant never assigns (calcuvwt_noslip.f:851)
+            u[o] = 0.;
+            v[o] = 0.;
+            w[o] = 0.;
+            continue;
+        }
+        const double tr = comp == 0 ? x[k] : other, ti = comp == 0 ? other : x[k];
+        if (comp == 0) {
+            u[o] = u[o] - (0. * tr - cx * ti);
+            v[o] = v[o] - (0. * tr - cy * ti);
+        } else {
+            u[o] = u[o] - (0. * ti + cx * tr);
+            v[o] = v[o] - (0. * ti + cy * tr);
+        }
+        w[o] = (m == 0) ? 0. : w[o] - A.dt * (sd * fac);
+    }
+}
+
+void launch_pressure_nat(const Geo &g, const iw_params &P, const SysTables &S, double *const *uvwt, double *phi,
+                         const double *wnl, const double *wviscrot, double dt, int cpen, hipStream_t st)
+{
+    PArgs A;
+    A.u = uvwt[0]; A.v = uvwt[1]; A.w = uvwt[2];
+    A.phi = phi;
+    A.wnl = wnl; A.wviscrot = wviscrot;
+    A.modesys = S.modesys; A.sysinfo = S.sysinfo;
+    A.D = g.D; A.zh = g.zh; A.xw = g.xw; A.yw = g.yw;
+    A.dt = dt;
+    A.nxppy = g.nxppy;
+    A.nxhp = g.nxhp;
+    A.nz = g.nz;
+    SolPArgs B;
+    B.phi = phi; B.tasks = S.np_tasks; B.modesys = S.modesys; B.AB = S.AB; B.ipiv = S.ipiv;
+    B.nxppy = g.nxppy; B.n = g.nz; B.ldab = S.ldab; B.kv = S.kv;
+    dim3 gc((unsigned)((g.nxppy + 255) / 256), g.nsubd);
+    dim3 gs(S.np_ntasks / 64);
+    const int ns = P.bc_uv_bottom == 1;
+#define PSTEP(NL)                                                                                       \
+    hipLaunchKernelGGL(k_prhs<NL>, gc, dim3(256), 0, st, A);                                            \
+    hipLaunchKernelGGL(k_solve_p<NL>, gs, dim3(64), 0, st, B);                                          \
+    hipLaunchKernelGGL(k_pproject<NL>, gc, dim3(256), 0, st, A, uvwt[0], uvwt[1], uvwt[2], cpen, ns, g.wg, \
+                       g.AL, g.sig)
+    switch (g.nzloc) {
+    case 9: PSTEP(9); break;
+    case 13: PSTEP(13); break;
+    case 17: PSTEP(17); break;
+    case 21: PSTEP(21); break;
+    case 25: PSTEP(25); break;
+    default: break;
+    }
+#undef PSTEP
+}
+
 bool solve_np_supported(int nl, int nsubd)
 {
     (void)nsubd;
