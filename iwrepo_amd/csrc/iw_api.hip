@@ -673,10 +673,10 @@ extern "C" int iw_create(const iw_params *p, iw_solver **out)
     std::vector<double> xg(P.nx);
     const double pi2 = 8. * atan(1.0);
     for (int i = 0; i < P.nx; i++) xg[i] = i * (pi2 / (pi2 / P.xlen)) / P.nx;
-    std::vector<double2> twM(g.M / 2 > 0 ? g.M / 2 : 1), twN(g.M + 1), twY(g.ny / 2 > 0 ? g.ny / 2 : 1);
-    for (int t = 0; t < g.M / 2; t++) twM[t] = twiddle(t, g.M);
+    std::vector<double2> twM(g.M), twN(g.M + 1), twY(g.ny);
+    for (int t = 0; t < g.M; t++) twM[t] = twiddle(t, g.M);
     for (int k = 0; k <= g.M; k++) twN[k] = twiddle(k, 2 * g.M);
-    for (int t = 0; t < g.ny / 2; t++) twY[t] = twiddle(t, g.ny);
+    for (int t = 0; t < g.ny; t++) twY[t] = twiddle(t, g.ny);
     if (upload(&g.D, s->d, acc) || upload(&g.D2, s->d2, acc) || upload(&g.AL, s->alpol, acc)
         || upload(&g.wg, s->wg, acc) || upload(&g.sig, sig, acc) || upload(&g.xw, s->xw, acc)
         || upload(&g.yw, s->yw, acc) || upload(&g.zh, zh, acc) || upload(&g.z, s->z, acc)

@@ -15,35 +15,108 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 b)
     return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
-// Radix-2 Stockham autosort FFT on nrows rows of length M held in LDS.
-// sign -1: sum g_j e^{-2 pi i jk/M}; +1: e^{+...}.  tw[t] = e^{-2 pi i t/M}.
-// Returns the buffer holding the result (natural order).
-__device__ double2 *stockham(double2 *in, double2 *out, int M, int logM, int nrows,
-                             const double2 *__restrict__ tw, int sign)
+// ---------------------------------------------------------------------------
+// In-place multi-radix Stockham FFT on LDS.  A block holds kFE = 2048 complex
+// values (nrows sequences of length N, N a power of two), so every pass of
+// radix R has exactly 8/R butterflies per thread of a 256-thread block.
+// Passes are radix 8 while at least three factors of two remain, then one
+// radix-4 or radix-2 pass.  Each pass reads its butterflies into registers,
+// synchronises, and writes the outputs to their autosort positions; LDS
+// indices carry one pad slot per 16 values against bank conflicts.
+// sign -1: sum g_j e^{-2 pi i jk/N}; +1: e^{+...}.  tw[t] = e^{-2 pi i t/N}.
+// ---------------------------------------------------------------------------
+constexpr int kFE = 2048;
+
+__host__ __device__ __forceinline__ int lp(int i) { return i + (i >> 4); }
+
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+// multiply by sign*i
+__device__ __forceinline__ double2 cmuli(double2 a, int sign)
 {
-    const int half = M >> 1;
-    const int nb = nrows * half;
-    for (int s = 0; s < logM; s++) {
-        const int Ns = 1 << s;
-        for (int b = threadIdx.x; b < nb; b += blockDim.x) {
-            int r = b / half;
-            int j = b - r * half;
-            int k = j & (Ns - 1);
-            double2 a = in[r * M + j];
-            double2 c = in[r * M + j + half];
-            double2 w = tw[k << (logM - 1 - s)];
-            if (sign > 0) w.y = -w.y;
-            double2 cw = cmul(c, w);
-            int o = r * M + ((j - k) << 1) + k;
-            out[o] = make_double2(a.x + cw.x, a.y + cw.y);
-            out[o + Ns] = make_double2(a.x - cw.x, a.y - cw.y);
-        }
-        __syncthreads();
-        double2 *t = in;
-        in = out;
-        out = t;
+    return sign < 0 ? make_double2(a.y, -a.x) : make_double2(-a.y, a.x);
+}
+
+template <int R>
+__device__ __forceinline__ void dft(double2 *a, int sign)
+{
+    if (R == 2) {
+        const double2 t = a[0];
+        a[0] = cadd(t, a[1]);
+        a[1] = csub(t, a[1]);
+    } else if (R == 4) {
+        const double2 t0 = cadd(a[0], a[2]), t1 = csub(a[0], a[2]);
+        const double2 t2 = cadd(a[1], a[3]), t3 = cmuli(csub(a[1], a[3]), sign);
+        a[0] = cadd(t0, t2);
+        a[2] = csub(t0, t2);
+        a[1] = cadd(t1, t3);
+        a[3] = csub(t1, t3);
+    } else { // R == 8: two 4-point DFTs of the even and odd inputs
+        constexpr double c = 0.70710678118654752440084436210484904; // sqrt(2)/2
+        double2 e[4] = {a[0], a[2], a[4], a[6]}, o[4] = {a[1], a[3], a[5], a[7]};
+        dft<4>(e, sign);
+        dft<4>(o, sign);
+        // o_q *= e^{sign 2 pi i q/8}
+        const double2 o1 = make_double2(c * (o[1].x - sign * o[1].y), c * (o[1].y + sign * o[1].x));
+        const double2 o2 = cmuli(o[2], sign);
+        const double2 o3 = make_double2(c * (-o[3].x - sign * o[3].y), c * (-o[3].y + sign * o[3].x));
+        a[0] = cadd(e[0], o[0]);
+        a[4] = csub(e[0], o[0]);
+        a[1] = cadd(e[1], o1);
+        a[5] = csub(e[1], o1);
+        a[2] = cadd(e[2], o2);
+        a[6] = csub(e[2], o2);
+        a[3] = cadd(e[3], o3);
+        a[7] = csub(e[3], o3);
     }
-    return in;
+}
+
+template <int R>
+__device__ __forceinline__ void lds_pass(double2 *buf, int N, int Ns, int nrows, const double2 *tw, int sign)
+{
+    constexpr int BPT = 8 / R;
+    const int NR = N / R, nb = nrows * NR, step = N / (Ns * R);
+    double2 a[BPT][R];
+    int base[BPT];
+#pragma unroll
+    for (int i = 0; i < BPT; i++) {
+        const int b = threadIdx.x + i * 256;
+        base[i] = -1;
+        if (b < nb) {
+            const int r = b / NR, j = b - r * NR;
+            const int k = j & (Ns - 1);
+#pragma unroll
+            for (int m = 0; m < R; m++) a[i][m] = buf[lp(r * N + j + m * NR)];
+#pragma unroll
+            for (int m = 1; m < R; m++) {
+                double2 w = tw[m * k * step];
+                if (sign > 0) w.y = -w.y;
+                a[i][m] = cmul(a[i][m], w);
+            }
+            dft<R>(a[i], sign);
+            base[i] = r * N + (j - k) * R + k;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < BPT; i++)
+        if (base[i] >= 0)
+#pragma unroll
+            for (int q = 0; q < R; q++) buf[lp(base[i] + q * Ns)] = a[i][q];
+    __syncthreads();
+}
+
+// nrows sequences of length N = 2^logN in buf (padded indexing), in place
+__device__ void fft_lds(double2 *buf, int N, int logN, int nrows, const double2 *tw, int sign)
+{
+    int Ns = 1, left = logN;
+    while (left >= 3) {
+        lds_pass<8>(buf, N, Ns, nrows, tw, sign);
+        Ns *= 8;
+        left -= 3;
+    }
+    if (left == 2) lds_pass<4>(buf, N, Ns, nrows, tw, sign);
+    if (left == 1) lds_pass<2>(buf, N, Ns, nrows, tw, sign);
 }
 
 // ---------------------------------------------------------------------------
@@ -55,25 +128,27 @@ __device__ double2 *stockham(double2 *in, double2 *out, int M, int logM, int nro
 // even/odd-packed row with LDS-staged radix-2 passes.
 // ---------------------------------------------------------------------------
 template <int DIR>
-__global__ __launch_bounds__(256) void k_fft_x(FieldBatch fb, int nrows, int ny, int nxpp, int M,
-                                               int logM, int R, const double2 *__restrict__ twM,
+__global__ __launch_bounds__(256) void k_fft_x(FieldBatch fb, int nrows, int ny, int nxpp, int M, int logM,
+                                               int R, const double2 *__restrict__ twM,
                                                const double2 *__restrict__ twN, double scaley)
 {
     extern __shared__ double2 smx[];
-    double2 *b0 = smx, *b1 = smx + R * M;
+    double2 *buf = smx;
+    double2 *tw = smx + lp(R * M) + 1;
     const double *base = fb.f[blockIdx.y];
     double *obase = fb.g[blockIdx.y];
     const int row0 = blockIdx.x * R;
     const int N = 2 * M;
     // logical row -> stored row of the source / destination (plane maps)
     auto srow = [&](int row) -> long {
-        int lp = row / ny, j = row - lp * ny;
-        return (long)((fb.smap ? fb.smap[lp] : lp) * ny + j) * nxpp;
+        int lpl = row / ny, j = row - lpl * ny;
+        return (long)((fb.smap ? fb.smap[lpl] : lpl) * ny + j) * nxpp;
     };
     auto drow = [&](int row) -> long {
-        int lp = row / ny, j = row - lp * ny;
-        return (long)((fb.dmap ? fb.dmap[lp] : lp) * ny + j) * nxpp;
+        int lpl = row / ny, j = row - lpl * ny;
+        return (long)((fb.dmap ? fb.dmap[lpl] : lpl) * ny + j) * nxpp;
     };
+    for (int t = threadIdx.x; t < M; t += blockDim.x) tw[t] = twM[t];
     if (DIR < 0) {
         for (int idx = threadIdx.x; idx < R * M; idx += blockDim.x) {
             int r = idx / M, m = idx - r * M, row = row0 + r;
@@ -82,7 +157,7 @@ __global__ __launch_bounds__(256) void k_fft_x(FieldBatch fb, int nrows, int ny,
                 const double *p = base + srow(row) + 2 * m;
                 z = make_double2(p[0], p[1]);
             }
-            b0[idx] = z;
+            buf[lp(idx)] = z;
         }
     } else {
         for (int idx = threadIdx.x; idx < R * M; idx += blockDim.x) {
@@ -104,18 +179,18 @@ __global__ __launch_bounds__(256) void k_fft_x(FieldBatch fb, int nrows, int ny,
                 double orr = dr * c - di * s, oi = dr * s + di * c;
                 Z = make_double2(er - oi, ei + orr);
             }
-            b0[idx] = Z;
+            buf[lp(idx)] = Z;
         }
     }
     __syncthreads();
-    double2 *res = stockham(b0, b1, M, logM, R, twM, DIR);
+    fft_lds(buf, M, logM, R, tw, DIR);
     if (DIR < 0) {
         const double sc = 1.0 / (double)N;
         for (int idx = threadIdx.x; idx < R * (M + 1); idx += blockDim.x) {
             int r = idx / (M + 1), k = idx - r * (M + 1), row = row0 + r;
             if (row >= nrows) continue;
-            double2 Zk = res[r * M + (k == M ? 0 : k)];
-            double2 Zm = res[r * M + (k == 0 ? 0 : M - k)];
+            double2 Zk = buf[lp(r * M + (k == M ? 0 : k))];
+            double2 Zm = buf[lp(r * M + (k == 0 ? 0 : M - k))];
             double er = 0.5 * (Zk.x + Zm.x), ei = 0.5 * (Zk.y - Zm.y);
             double dr = Zk.x - Zm.x, di = Zk.y + Zm.y;
             double orr = 0.5 * di, oi = -0.5 * dr;
@@ -134,8 +209,9 @@ __global__ __launch_bounds__(256) void k_fft_x(FieldBatch fb, int nrows, int ny,
             int r = idx / M, m = idx - r * M, row = row0 + r;
             if (row >= nrows) continue;
             double *p = obase + drow(row);
-            p[2 * m] = res[idx].x;
-            p[2 * m + 1] = res[idx].y;
+            const double2 z = buf[lp(idx)];
+            p[2 * m] = z.x;
+            p[2 * m + 1] = z.y;
         }
         for (int r = threadIdx.x; r < R; r += blockDim.x) {
             int row = row0 + r;
@@ -158,11 +234,13 @@ __global__ __launch_bounds__(256) void k_fft_y(FieldBatch fb, int nxpp, int nxh,
                                                int CB, const double2 *__restrict__ twY, double scaley)
 {
     extern __shared__ double2 smy[];
-    double2 *b0 = smy, *b1 = smy + CB * ny;
-    const int lp = blockIdx.y;
-    const double *plane = fb.f[blockIdx.z] + (long)(fb.smap ? fb.smap[lp] : lp) * nxpp * ny;
-    double *oplane = fb.g[blockIdx.z] + (long)(fb.dmap ? fb.dmap[lp] : lp) * nxpp * ny;
+    double2 *buf = smy;
+    double2 *tw = smy + lp(CB * ny) + 1;
+    const int lpl = blockIdx.y;
+    const double *plane = fb.f[blockIdx.z] + (long)(fb.smap ? fb.smap[lpl] : lpl) * nxpp * ny;
+    double *oplane = fb.g[blockIdx.z] + (long)(fb.dmap ? fb.dmap[lpl] : lpl) * nxpp * ny;
     const int ii0 = blockIdx.x * CB;
+    for (int t = threadIdx.x; t < ny; t += blockDim.x) tw[t] = twY[t];
     for (int idx = threadIdx.x; idx < CB * ny; idx += blockDim.x) {
         int j = idx / CB, c = idx - j * CB, ii = ii0 + c;
         double2 z = make_double2(0., 0.);
@@ -170,14 +248,14 @@ __global__ __launch_bounds__(256) void k_fft_y(FieldBatch fb, int nxpp, int nxh,
             const double *p = plane + (long)j * nxpp + 2 * ii;
             z = make_double2(p[0], p[1]);
         }
-        b0[c * ny + j] = z;
+        buf[lp(c * ny + j)] = z;
     }
     __syncthreads();
-    double2 *res = stockham(b0, b1, ny, logNy, CB, twY, DIR);
+    fft_lds(buf, ny, logNy, CB, tw, DIR);
     for (int idx = threadIdx.x; idx < CB * ny; idx += blockDim.x) {
         int j = idx / CB, c = idx - j * CB, ii = ii0 + c;
         if (ii >= nxh) continue;
-        double2 z = res[c * ny + j];
+        double2 z = buf[lp(c * ny + j)];
         if (DIR < 0) {
             z.x = scaley * z.x;
             z.y = scaley * z.y;
@@ -198,11 +276,10 @@ __global__ __launch_bounds__(256) void k_fft_y(FieldBatch fb, int nxpp, int nxh,
 
 void launch_fft_x(const Geo &g, FieldBatch b, int nk, int dir, hipStream_t st)
 {
-    int R = 512 / g.M;
-    if (R < 1) R = 1;
+    const int R = kFE / g.M;
     int nrows = g.ny * nk;
     dim3 grid((nrows + R - 1) / R, b.n);
-    size_t sh = 2 * (size_t)R * g.M * sizeof(double2);
+    size_t sh = sizeof(double2) * ((size_t)lp(R * g.M) + 1 + g.M);
     if (dir < 0)
         hipLaunchKernelGGL(k_fft_x<-1>, grid, dim3(256), sh, st, b, nrows, g.ny, g.nxpp, g.M, g.logM, R,
                            g.twM, g.twN, g.scaley);
@@ -214,11 +291,10 @@ void launch_fft_x(const Geo &g, FieldBatch b, int nk, int dir, hipStream_t st)
 void launch_fft_y(const Geo &g, FieldBatch b, int nk, int dir, hipStream_t st)
 {
     if (g.ndims != 3) return;
-    int CB = 1024 / g.ny;
-    if (CB < 1) CB = 1;
+    int CB = kFE / g.ny;
     if (CB > g.nxh) CB = g.nxh;
     dim3 grid((g.nxh + CB - 1) / CB, nk, b.n);
-    size_t sh = 2 * (size_t)CB * g.ny * sizeof(double2);
+    size_t sh = sizeof(double2) * ((size_t)lp(CB * g.ny) + 1 + g.ny);
     if (dir < 0)
         hipLaunchKernelGGL(k_fft_y<-1>, grid, dim3(256), sh, st, b, g.nxpp, g.nxh, g.ny, g.logNy, CB,
                            g.twY, g.scaley);
