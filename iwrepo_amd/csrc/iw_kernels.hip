@@ -153,10 +153,7 @@ __global__ __launch_bounds__(256) void k_fft_x(FieldBatch fb, int nrows, int ny,
         for (int idx = threadIdx.x; idx < R * M; idx += blockDim.x) {
             int r = idx / M, m = idx - r * M, row = row0 + r;
             double2 z = make_double2(0., 0.);
-            if (row < nrows) {
-                const double *p = base + srow(row) + 2 * m;
-                z = make_double2(p[0], p[1]);
-            }
+            if (row < nrows) z = *reinterpret_cast<const double2 *>(base + srow(row) + 2 * m);
             buf[lp(idx)] = z;
         }
     } else {
@@ -166,11 +163,13 @@ __global__ __launch_bounds__(256) void k_fft_x(FieldBatch fb, int nrows, int ny,
             if (row < nrows) {
                 const double *p = base + srow(row);
                 double ar, ai, br, bi;
+                const double2 va = *reinterpret_cast<const double2 *>(p + 2 * k);
+                const double2 vb = *reinterpret_cast<const double2 *>(p + 2 * (M - k));
                 if (k == 0) {
-                    ar = p[0]; ai = 0.; br = p[2 * M]; bi = 0.;
+                    ar = va.x; ai = 0.; br = vb.x; bi = 0.;
                 } else {
-                    ar = p[2 * k]; ai = p[2 * k + 1];
-                    br = p[2 * (M - k)]; bi = -p[2 * (M - k) + 1];
+                    ar = va.x; ai = va.y;
+                    br = vb.x; bi = -vb.y;
                 }
                 double er = ar + br, ei = ai + bi;
                 double dr = ar - br, di = ai - bi;
@@ -200,26 +199,17 @@ __global__ __launch_bounds__(256) void k_fft_x(FieldBatch fb, int nrows, int ny,
             double A = sc * xr, B = sc * xi;
             if (k == 0 || k == M) B = 0.0;
             if (k == M) A = A * scaley; // horfft.f:28-30 on the untransformed Nyquist column
-            double *p = obase + drow(row);
-            p[2 * k] = A;
-            p[2 * k + 1] = B;
+            *reinterpret_cast<double2 *>(obase + drow(row) + 2 * k) = make_double2(A, B);
         }
     } else {
         for (int idx = threadIdx.x; idx < R * M; idx += blockDim.x) {
             int r = idx / M, m = idx - r * M, row = row0 + r;
             if (row >= nrows) continue;
-            double *p = obase + drow(row);
-            const double2 z = buf[lp(idx)];
-            p[2 * m] = z.x;
-            p[2 * m + 1] = z.y;
+            *reinterpret_cast<double2 *>(obase + drow(row) + 2 * m) = buf[lp(idx)];
         }
         for (int r = threadIdx.x; r < R; r += blockDim.x) {
             int row = row0 + r;
-            if (row < nrows) {
-                double *p = obase + drow(row);
-                p[N] = 0.;
-                p[N + 1] = 0.;
-            }
+            if (row < nrows) *reinterpret_cast<double2 *>(obase + drow(row) + N) = make_double2(0., 0.);
         }
     }
 }
@@ -244,10 +234,7 @@ __global__ __launch_bounds__(256) void k_fft_y(FieldBatch fb, int nxpp, int nxh,
     for (int idx = threadIdx.x; idx < CB * ny; idx += blockDim.x) {
         int j = idx / CB, c = idx - j * CB, ii = ii0 + c;
         double2 z = make_double2(0., 0.);
-        if (ii < nxh) {
-            const double *p = plane + (long)j * nxpp + 2 * ii;
-            z = make_double2(p[0], p[1]);
-        }
+        if (ii < nxh) z = *reinterpret_cast<const double2 *>(plane + (long)j * nxpp + 2 * ii);
         buf[lp(c * ny + j)] = z;
     }
     __syncthreads();
@@ -260,9 +247,7 @@ __global__ __launch_bounds__(256) void k_fft_y(FieldBatch fb, int nxpp, int nxh,
             z.x = scaley * z.x;
             z.y = scaley * z.y;
         }
-        double *p = oplane + (long)j * nxpp + 2 * ii;
-        p[0] = z.x;
-        p[1] = z.y;
+        *reinterpret_cast<double2 *>(oplane + (long)j * nxpp + 2 * ii) = z;
     }
     // out of place: the untransformed kx-Nyquist column travels unchanged
     if (blockIdx.x == 0 && plane != oplane)
@@ -345,10 +330,53 @@ __global__ __launch_bounds__(256) void k_dpdz(DpdzArgs A, long nxppy, int nl,
     }
 }
 
+// The same with the GLL order fixed at compile time: the column is loaded in
+// one burst and D is read from LDS row by row.
+template <int NL>
+__global__ __launch_bounds__(256) void k_dpdz_t(DpdzArgs A, long nxppy, const double *__restrict__ D,
+                                                const double *__restrict__ zh)
+{
+    __shared__ double Ds[NL * NL];
+    for (int e = threadIdx.x; e < NL * NL; e += blockDim.x) Ds[e] = D[e];
+    __syncthreads();
+    const long ij = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (ij >= nxppy) return;
+    const int ks = blockIdx.y;
+    const double fac = 2. / zh[ks];
+    const long b0 = (long)ks * NL * nxppy + ij;
+    for (int q = 0; q < A.nq; q++) {
+        double col[NL];
+        const double *a = A.a[q], *b = A.b[q];
+        if (b) {
+            const double c = A.c[q];
+#pragma unroll
+            for (int t = 0; t < NL; t++) col[t] = (c * a[b0 + t * nxppy]) * b[b0 + t * nxppy];
+        } else {
+#pragma unroll
+            for (int t = 0; t < NL; t++) col[t] = a[b0 + t * nxppy];
+        }
+        double *o = A.out[q] + b0;
+#pragma unroll 1
+        for (int k1 = 0; k1 < NL; k1++) {
+            double sm = 0.0;
+#pragma unroll
+            for (int t = 0; t < NL; t++) sm = sm + Ds[k1 * NL + t] * col[t];
+            o[k1 * nxppy] = sm * fac;
+        }
+    }
+}
+
 static void run_dpdz(const Geo &g, const DpdzArgs &a, hipStream_t st)
 {
     dim3 grid((unsigned)((g.nxppy + 255) / 256), g.nsubd);
-    hipLaunchKernelGGL(k_dpdz, grid, dim3(256), 0, st, a, g.nxppy, g.nzloc, g.D, g.zh);
+    switch (g.nzloc) {
+    case 9: hipLaunchKernelGGL(k_dpdz_t<9>, grid, dim3(256), 0, st, a, g.nxppy, g.D, g.zh); break;
+    case 13: hipLaunchKernelGGL(k_dpdz_t<13>, grid, dim3(256), 0, st, a, g.nxppy, g.D, g.zh); break;
+    case 17: hipLaunchKernelGGL(k_dpdz_t<17>, grid, dim3(256), 0, st, a, g.nxppy, g.D, g.zh); break;
+    case 21: hipLaunchKernelGGL(k_dpdz_t<21>, grid, dim3(256), 0, st, a, g.nxppy, g.D, g.zh); break;
+    case 25: hipLaunchKernelGGL(k_dpdz_t<25>, grid, dim3(256), 0, st, a, g.nxppy, g.D, g.zh); break;
+    default: hipLaunchKernelGGL(k_dpdz, grid, dim3(256), 0, st, a, g.nxppy, g.nzloc, g.D, g.zh); break;
+    }
 }
 
 void launch_dpdz(const Geo &g, const double *a, double *b, hipStream_t st)
@@ -1444,6 +1472,45 @@ void launch_solve_nat(const Geo &g, const iw_params &P, const SysTables &S, doub
 // filtering.f:729-808) and the mean-u damping (calcuvwt.f:343-361), in place
 // on the native layout: lane = one column, the nl values of its subdomain in
 // registers, the (uniform) tables read through the scalar cache.
+// filter_legenpen (filtering.f:729-808) of the NL values x of one column in
+// place: a_k = c_k sig_k sum_q w_q x_q P_k(x_q), x_q = sum_k a_k P_k(x_q);
+// the second sum is accumulated as each a_k becomes available (same order
+// of terms), so AL (LDS) is read once per k.
+template <int NL>
+__device__ __forceinline__ void legfilter(double (&x)[NL], const double *wgs, const double *ALs,
+                                          const double *sgs)
+{
+    constexpr int nzm = NL - 1;
+    double y[NL];
+#pragma unroll
+    for (int q = 0; q < NL; q++) {
+        y[q] = wgs[q] * x[q];
+        x[q] = 0.;
+    }
+#pragma unroll 1
+    for (int k = 0; k < NL; k++) {
+        double sum = 0.;
+#pragma unroll
+        for (int q = 0; q < NL; q++) sum = sum + y[q] * ALs[k * NL + q];
+        const double c = (k == nzm) ? 0.5 * nzm * sum : (k + 0.5) * sum;
+        const double a = c * sgs[k];
+#pragma unroll
+        for (int q = 0; q < NL; q++) x[q] = x[q] + a * ALs[k * NL + q];
+    }
+}
+
+template <int NL>
+__device__ __forceinline__ void stage_tables(double *wgs, double *ALs, double *sgs, const double *wg,
+                                             const double *AL, const double *sig)
+{
+    for (int e = threadIdx.x; e < NL * NL; e += blockDim.x) ALs[e] = AL[e];
+    for (int e = threadIdx.x; e < NL; e += blockDim.x) {
+        wgs[e] = wg[e];
+        sgs[e] = sig[e];
+    }
+    __syncthreads();
+}
+
 template <int NL>
 __global__ __launch_bounds__(256) void k_lfilter(double *u, double *v, double *w, double *t, long nxppy,
                                                 int cpen, const double *__restrict__ wg,
@@ -1451,6 +1518,8 @@ __global__ __launch_bounds__(256) void k_lfilter(double *u, double *v, double *w
                                                 const double *__restrict__ sig,
                                                 const double *__restrict__ damp)
 {
+    __shared__ double ALs[NL * NL], wgs[NL], sgs[NL];
+    stage_tables<NL>(wgs, ALs, sgs, wg, AL, sig);
     const long col = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const int ks = blockIdx.y, fld = blockIdx.z;
     if (col >= nxppy) return;
@@ -1460,27 +1529,7 @@ __global__ __launch_bounds__(256) void k_lfilter(double *u, double *v, double *w
     double x[NL];
 #pragma unroll
     for (int k = 0; k < NL; k++) x[k] = f[(long)k * nxppy];
-    if (cpen) {
-        double a[NL];
-        constexpr int nzm = NL - 1;
-#pragma unroll
-        for (int q = 0; q < NL; q++) x[q] = wg[q] * x[q];
-#pragma unroll
-        for (int k = 0; k < NL; k++) {
-            double sum = 0.;
-#pragma unroll
-            for (int q = 0; q < NL; q++) sum = sum + x[q] * AL[k * NL + q];
-            const double c = (k == nzm) ? 0.5 * nzm * sum : (k + 0.5) * sum;
-            a[k] = c * sig[k];
-        }
-#pragma unroll
-        for (int q = 0; q < NL; q++) {
-            double sum = 0.;
-#pragma unroll
-            for (int k = 0; k < NL; k++) sum = sum + a[k] * AL[k * NL + q];
-            x[q] = sum;
-        }
-    }
+    if (cpen) legfilter<NL>(x, wgs, ALs, sgs);
     if (fld == 0 && m == 0)
 #pragma unroll
         for (int k = 0; k < NL; k++) x[k] = damp[ks * NL + k] * x[k];
@@ -1530,6 +1579,9 @@ struct PArgs {
 template <int NL>
 __global__ __launch_bounds__(256) void k_prhs(PArgs A)
 {
+    __shared__ double Ds[NL * NL];
+    for (int e = threadIdx.x; e < NL * NL; e += blockDim.x) Ds[e] = A.D[e];
+    __syncthreads();
     const long col = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const int ks = blockIdx.y;
     if (col >= A.nxppy) return;
@@ -1553,7 +1605,7 @@ __global__ __launch_bounds__(256) void k_prhs(PArgs A)
     for (int k = 0; k < NL; k++) {
         double sd = 0.;
 #pragma unroll
-        for (int t = 0; t < NL; t++) sd = sd + A.D[k * NL + t] * wc[t];
+        for (int t = 0; t < NL; t++) sd = sd + Ds[k * NL + t] * wc[t];
         const long o = base + (long)k * st;
         double d;
         if (comp == 0)
@@ -1677,6 +1729,9 @@ __global__ __launch_bounds__(256) void k_pproject(PArgs A, double *u, double *v,
                                                  const double *__restrict__ AL,
                                                  const double *__restrict__ sig)
 {
+    __shared__ double ALs[NL * NL], wgs[NL], sgs[NL], Ds[NL * NL];
+    for (int e = threadIdx.x; e < NL * NL; e += blockDim.x) Ds[e] = A.D[e];
+    stage_tables<NL>(wgs, ALs, sgs, wg, AL, sig);
     const long col = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const int ks = blockIdx.y;
     const bool live = col < A.nxppy;
@@ -1686,28 +1741,7 @@ __global__ __launch_bounds__(256) void k_pproject(PArgs A, double *u, double *v,
     double x[NL];
 #pragma unroll
     for (int k = 0; k < NL; k++) x[k] = (m == 0) ? 0. : A.phi[base + (long)k * st + c];
-    if (cpen && m != 0) { // filter_legenpen (filtering.f:729-808)
-        double a[NL];
-        constexpr int nzm = NL - 1;
-        double y[NL];
-#pragma unroll
-        for (int q = 0; q < NL; q++) y[q] = wg[q] * x[q];
-#pragma unroll
-        for (int k = 0; k < NL; k++) {
-            double sum = 0.;
-#pragma unroll
-            for (int q = 0; q < NL; q++) sum = sum + y[q] * AL[k * NL + q];
-            const double cc = (k == nzm) ? 0.5 * nzm * sum : (k + 0.5) * sum;
-            a[k] = cc * sig[k];
-        }
-#pragma unroll
-        for (int q = 0; q < NL; q++) {
-            double sum = 0.;
-#pragma unroll
-            for (int k = 0; k < NL; k++) sum = sum + a[k] * AL[k * NL + q];
-            x[q] = sum;
-        }
-    }
+    if (cpen && m != 0) legfilter<NL>(x, wgs, ALs, sgs); // filter_legenpen of phi
     const double xw = A.xw[m % A.nxhp], yw = A.yw[m / A.nxhp];
     const double cx = A.dt * xw, cy = A.dt * yw;
     const double fac = 2. / A.zh[ks];
@@ -1716,7 +1750,7 @@ __global__ __launch_bounds__(256) void k_pproject(PArgs A, double *u, double *v,
         const double other = __shfl_xor(x[k], 1); // phi of the partner component
         double sd = 0.;
 #pragma unroll
-        for (int t = 0; t < NL; t++) sd = sd + A.D[k * NL + t] * x[t];
+        for (int t = 0; t < NL; t++) sd = sd + Ds[k * NL + t] * x[t];
         if (!live) continue;
         const long o = base + (long)k * st + c;
         const int kg = ks * NL + k;
