@@ -119,6 +119,45 @@ __device__ void fft_lds(double2 *buf, int N, int logN, int nrows, const double2 
     if (left == 1) lds_pass<2>(buf, N, Ns, nrows, tw, sign);
 }
 
+// filter_legenpen (filtering.f:729-808) of the NL values x of one column in
+// place: a_k = c_k sig_k sum_q w_q x_q P_k(x_q), x_q = sum_k a_k P_k(x_q);
+// the second sum is accumulated as each a_k becomes available (same order
+// of terms), so AL (LDS) is read once per k.
+template <int NL>
+__device__ __forceinline__ void legfilter(double (&x)[NL], const double *wgs, const double *ALs,
+                                          const double *sgs)
+{
+    constexpr int nzm = NL - 1;
+    double y[NL];
+#pragma unroll
+    for (int q = 0; q < NL; q++) {
+        y[q] = wgs[q] * x[q];
+        x[q] = 0.;
+    }
+#pragma unroll 1
+    for (int k = 0; k < NL; k++) {
+        double sum = 0.;
+#pragma unroll
+        for (int q = 0; q < NL; q++) sum = sum + y[q] * ALs[k * NL + q];
+        const double c = (k == nzm) ? 0.5 * nzm * sum : (k + 0.5) * sum;
+        const double a = c * sgs[k];
+#pragma unroll
+        for (int q = 0; q < NL; q++) x[q] = x[q] + a * ALs[k * NL + q];
+    }
+}
+
+template <int NL>
+__device__ __forceinline__ void stage_tables(double *wgs, double *ALs, double *sgs, const double *wg,
+                                             const double *AL, const double *sig)
+{
+    for (int e = threadIdx.x; e < NL * NL; e += blockDim.x) ALs[e] = AL[e];
+    for (int e = threadIdx.x; e < NL; e += blockDim.x) {
+        wgs[e] = wg[e];
+        sgs[e] = sig[e];
+    }
+    __syncthreads();
+}
+
 // ---------------------------------------------------------------------------
 // x transform (FFT991, fft.f:1361-1502, as used by horfft.f:21 / 58):
 // real length-N rows <-> N/2+1 coefficients A_k + i B_k, forward normalised
@@ -781,9 +820,93 @@ __global__ __launch_bounds__(256) void k_tile(TileArgs A, long ncplx_plane, long
     }
 }
 
+// The same per (mode, subdomain) with the GLL order fixed at compile time:
+// lane = one mode, its re/im columns in registers (16-byte loads of each
+// row), the Fourier filter with one sincos, the Legendre tables in LDS.
+template <int NL, int MODE>
+__global__ __launch_bounds__(256, 2) void k_tile_t(TileArgs A, long nmode, long nxppy,
+                                                const double *__restrict__ fcf, const double *__restrict__ wg,
+                                                const double *__restrict__ AL, const double *__restrict__ sig)
+{
+    __shared__ double ALs[NL * NL], wgs[NL], sgs[NL];
+    stage_tables<NL>(wgs, ALs, sgs, wg, AL, sig);
+    const long m = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int ks = blockIdx.y;
+    if (m >= nmode) return;
+    const long b0 = (long)ks * NL * nxppy + 2 * m;
+    double xr[NL], xi[NL];
+    if (MODE == 1) {
+#pragma unroll
+        for (int t = 0; t < NL; t++) {
+            const long o = b0 + (long)t * nxppy;
+            const double2 s0 = *reinterpret_cast<const double2 *>(A.S[0] + o);
+            const double2 s1 = *reinterpret_cast<const double2 *>(A.S[1] + o);
+            const double2 s2 = *reinterpret_cast<const double2 *>(A.S[2] + o);
+            const double2 f0 = *reinterpret_cast<const double2 *>(A.F[0] + o);
+            const double2 f1 = *reinterpret_cast<const double2 *>(A.F[1] + o);
+            const double2 f2 = *reinterpret_cast<const double2 *>(A.F[2] + o);
+            const double sr = A.f1 * s0.x + A.f2 * s1.x + A.f3 * s2.x;
+            const double si = A.f1 * s0.y + A.f2 * s1.y + A.f3 * s2.y;
+            xr[t] = A.dt * sr + A.b1 * f0.x + A.b2 * f1.x + A.b3 * f2.x;
+            xi[t] = A.dt * si + A.b1 * f0.y + A.b2 * f1.y + A.b3 * f2.y;
+        }
+    } else {
+#pragma unroll
+        for (int t = 0; t < NL; t++) {
+            const double2 v = *reinterpret_cast<const double2 *>(A.out + b0 + (long)t * nxppy);
+            xr[t] = v.x;
+            xi[t] = v.y;
+        }
+    }
+    if (A.ifourier) { // Fourier magnitude/phase filter, filtering.f:852-901
+        const double sigf = fcf[m];
+#pragma unroll
+        for (int t = 0; t < NL; t++) {
+            const double fr = xr[t], fi = xi[t];
+            const double fmag = sqrt(fr * fr + fi * fi);
+            double gr = 0., gi = 0.;
+            if (fmag != 0.) {
+                const double cosc = fr / fmag, sinc = fi / fmag;
+                const double phase = (sinc >= 0.) ? acos(cosc) : -acos(cosc);
+                const double fmagf = fmag * sigf;
+                double sn, cs;
+                sincos(phase, &sn, &cs);
+                gr = fmagf * cs;
+                gi = fmagf * sn;
+            }
+            xr[t] = gr;
+            xi[t] = gi;
+        }
+    }
+    if (A.ilegen) { // filtering.f:905-956
+        legfilter<NL>(xr, wgs, ALs, sgs);
+        legfilter<NL>(xi, wgs, ALs, sgs);
+    }
+#pragma unroll
+    for (int t = 0; t < NL; t++)
+        *reinterpret_cast<double2 *>(A.out + b0 + (long)t * nxppy) = make_double2(xr[t], xi[t]);
+}
+
 static void run_tile(const Geo &g, const TileArgs &A, hipStream_t st)
 {
     long ncp = (long)g.nxhp * g.ny;
+    dim3 gt((unsigned)((ncp + 255) / 256), g.nsubd);
+#define TT(NL)                                                                                           \
+    if (A.mode == 1)                                                                                     \
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_tile_t<NL, 1>), gt, dim3(256), 0, st, A, ncp, g.nxppy, g.fcf, \
+                           g.wg, g.AL, g.sig);                                                           \
+    else                                                                                                 \
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_tile_t<NL, 0>), gt, dim3(256), 0, st, A, ncp, g.nxppy, g.fcf, \
+                           g.wg, g.AL, g.sig)
+    switch (g.nzloc) {
+    case 9: TT(9); return;
+    case 13: TT(13); return;
+    case 17: TT(17); return;
+    case 21: TT(21); return;
+    case 25: TT(25); return;
+    default: break;
+    }
+#undef TT
     dim3 grid((unsigned)((ncp + kMB - 1) / kMB), g.nsubd);
     hipLaunchKernelGGL(k_tile, grid, dim3(256), 0, st, A, ncp, g.nxppy, g.nzloc, g.fcf, g.wg, g.AL, g.sig);
 }
@@ -1472,45 +1595,6 @@ void launch_solve_nat(const Geo &g, const iw_params &P, const SysTables &S, doub
 // filtering.f:729-808) and the mean-u damping (calcuvwt.f:343-361), in place
 // on the native layout: lane = one column, the nl values of its subdomain in
 // registers, the (uniform) tables read through the scalar cache.
-// filter_legenpen (filtering.f:729-808) of the NL values x of one column in
-// place: a_k = c_k sig_k sum_q w_q x_q P_k(x_q), x_q = sum_k a_k P_k(x_q);
-// the second sum is accumulated as each a_k becomes available (same order
-// of terms), so AL (LDS) is read once per k.
-template <int NL>
-__device__ __forceinline__ void legfilter(double (&x)[NL], const double *wgs, const double *ALs,
-                                          const double *sgs)
-{
-    constexpr int nzm = NL - 1;
-    double y[NL];
-#pragma unroll
-    for (int q = 0; q < NL; q++) {
-        y[q] = wgs[q] * x[q];
-        x[q] = 0.;
-    }
-#pragma unroll 1
-    for (int k = 0; k < NL; k++) {
-        double sum = 0.;
-#pragma unroll
-        for (int q = 0; q < NL; q++) sum = sum + y[q] * ALs[k * NL + q];
-        const double c = (k == nzm) ? 0.5 * nzm * sum : (k + 0.5) * sum;
-        const double a = c * sgs[k];
-#pragma unroll
-        for (int q = 0; q < NL; q++) x[q] = x[q] + a * ALs[k * NL + q];
-    }
-}
-
-template <int NL>
-__device__ __forceinline__ void stage_tables(double *wgs, double *ALs, double *sgs, const double *wg,
-                                             const double *AL, const double *sig)
-{
-    for (int e = threadIdx.x; e < NL * NL; e += blockDim.x) ALs[e] = AL[e];
-    for (int e = threadIdx.x; e < NL; e += blockDim.x) {
-        wgs[e] = wg[e];
-        sgs[e] = sig[e];
-    }
-    __syncthreads();
-}
-
 template <int NL>
 __global__ __launch_bounds__(256) void k_lfilter(double *u, double *v, double *w, double *t, long nxppy,
                                                 int cpen, const double *__restrict__ wg,
