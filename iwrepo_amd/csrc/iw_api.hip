@@ -1053,11 +1053,15 @@ static int calcuvwt_dev(iw_solver *s, int cpen)
     const double fcol = (double)s->sys.ldab * s->g.nz * 8. + s->g.nz * 4.;
     const double nt = (double)s->g.ntot;
     if (s->sys.np_tasks) {
-        // u,v,w read twice and written once, phi written, read and written
-        // by the solve and read again: 80 B per point; each P factor once
-        Scope sc(s, "pressure", 80. * nt + s->sys.np * fcol);
-        launch_pressure_nat(s->g, s->P, s->sys, uvwt, s->grad[0], s->wnl, s->wviscrot, s->P.dt, cpen,
-                            s->stream);
+        const char *name[3] = {"p_rhs", "p_solve", "p_project"};
+        // u,v,w in, phi out | phi in and out + each P factor once |
+        // phi, u,v,w in, u,v,w out
+        const double bytes[3] = {32. * nt, 16. * nt + s->sys.np * fcol, 56. * nt};
+        for (int part = 0; part < 3; part++) {
+            Scope sc(s, name[part], bytes[part]);
+            launch_pressure_nat(s->g, s->P, s->sys, uvwt, s->grad[0], s->wnl, s->wviscrot, s->P.dt, cpen, part,
+                                s->stream);
+        }
     } else {
         Scope sc(s, "pressure", 56. * nt + s->sys.np * fcol); // u,v,w in; u,v,w out; w re-read
         launch_pressure(s->g, s->P, s->sys, uvwt, s->wnl, s->wviscrot, s->P.dt, cpen, s->stream);

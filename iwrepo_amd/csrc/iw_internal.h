@@ -130,6 +130,8 @@ bool solve_np_supported(int nl, int nsubd);
 void launch_solve_nat(const Geo &g, const iw_params &P, const SysTables &S, double *const *uvwt, double dt,
                       hipStream_t st);
 void launch_lfilter(const Geo &g, double *const *uvwt, int cpen, hipStream_t st);
+// part 0: right-hand side, 1: solve, 2: filter + projection
 void launch_pressure_nat(const Geo &g, const iw_params &P, const SysTables &S, double *const *uvwt, double *phi,
-                         const double *wnl, const double *wviscrot, double dt, int cpen, hipStream_t st);
+                         const double *wnl, const double *wviscrot, double dt, int cpen, int part,
+                         hipStream_t st);
 } // namespace iw

@@ -71,11 +71,13 @@ __device__ __forceinline__ void dft(double2 *a, int sign)
     }
 }
 
-template <int R>
-__device__ __forceinline__ void lds_pass(double2 *buf, int N, int Ns, int nrows, const double2 *tw, int sign)
+template <int R, int LR>
+__device__ __forceinline__ void lds_pass(double2 *buf, int logN, int logNs, int nrows, const double2 *tw,
+                                         int sign)
 {
     constexpr int BPT = 8 / R;
-    const int NR = N / R, nb = nrows * NR, step = N / (Ns * R);
+    const int lNR = logN - LR, NR = 1 << lNR, Ns = 1 << logNs, nb = nrows << lNR;
+    const int lstep = logN - logNs - LR;
     double2 a[BPT][R];
     int base[BPT];
 #pragma unroll
@@ -83,18 +85,19 @@ __device__ __forceinline__ void lds_pass(double2 *buf, int N, int Ns, int nrows,
         const int b = threadIdx.x + i * 256;
         base[i] = -1;
         if (b < nb) {
-            const int r = b / NR, j = b - r * NR;
+            const int r = b >> lNR, j = b & (NR - 1);
             const int k = j & (Ns - 1);
+            const int rb = r << logN;
 #pragma unroll
-            for (int m = 0; m < R; m++) a[i][m] = buf[lp(r * N + j + m * NR)];
+            for (int m = 0; m < R; m++) a[i][m] = buf[lp(rb + j + m * NR)];
 #pragma unroll
             for (int m = 1; m < R; m++) {
-                double2 w = tw[m * k * step];
+                double2 w = tw[(m * k) << lstep];
                 if (sign > 0) w.y = -w.y;
                 a[i][m] = cmul(a[i][m], w);
             }
             dft<R>(a[i], sign);
-            base[i] = r * N + (j - k) * R + k;
+            base[i] = rb + (j - k) * R + k;
         }
     }
     __syncthreads();
@@ -102,21 +105,21 @@ __device__ __forceinline__ void lds_pass(double2 *buf, int N, int Ns, int nrows,
     for (int i = 0; i < BPT; i++)
         if (base[i] >= 0)
 #pragma unroll
-            for (int q = 0; q < R; q++) buf[lp(base[i] + q * Ns)] = a[i][q];
+            for (int q = 0; q < R; q++) buf[lp(base[i] + (q << logNs))] = a[i][q];
     __syncthreads();
 }
 
 // nrows sequences of length N = 2^logN in buf (padded indexing), in place
-__device__ void fft_lds(double2 *buf, int N, int logN, int nrows, const double2 *tw, int sign)
+__device__ void fft_lds(double2 *buf, int logN, int nrows, const double2 *tw, int sign)
 {
-    int Ns = 1, left = logN;
+    int lNs = 0, left = logN;
     while (left >= 3) {
-        lds_pass<8>(buf, N, Ns, nrows, tw, sign);
-        Ns *= 8;
+        lds_pass<8, 3>(buf, logN, lNs, nrows, tw, sign);
+        lNs += 3;
         left -= 3;
     }
-    if (left == 2) lds_pass<4>(buf, N, Ns, nrows, tw, sign);
-    if (left == 1) lds_pass<2>(buf, N, Ns, nrows, tw, sign);
+    if (left == 2) lds_pass<4, 2>(buf, logN, lNs, nrows, tw, sign);
+    if (left == 1) lds_pass<2, 1>(buf, logN, lNs, nrows, tw, sign);
 }
 
 // filter_legenpen (filtering.f:729-808) of the NL values x of one column in
@@ -179,25 +182,26 @@ __global__ __launch_bounds__(256) void k_fft_x(FieldBatch fb, int nrows, int ny,
     const int row0 = blockIdx.x * R;
     const int N = 2 * M;
     // logical row -> stored row of the source / destination (plane maps)
+    const int lny = __builtin_ctz(ny); // ny is a power of two
     auto srow = [&](int row) -> long {
-        int lpl = row / ny, j = row - lpl * ny;
+        int lpl = row >> lny, j = row & (ny - 1);
         return (long)((fb.smap ? fb.smap[lpl] : lpl) * ny + j) * nxpp;
     };
     auto drow = [&](int row) -> long {
-        int lpl = row / ny, j = row - lpl * ny;
+        int lpl = row >> lny, j = row & (ny - 1);
         return (long)((fb.dmap ? fb.dmap[lpl] : lpl) * ny + j) * nxpp;
     };
     for (int t = threadIdx.x; t < M; t += blockDim.x) tw[t] = twM[t];
     if (DIR < 0) {
         for (int idx = threadIdx.x; idx < R * M; idx += blockDim.x) {
-            int r = idx / M, m = idx - r * M, row = row0 + r;
+            int r = idx >> logM, m = idx & (M - 1), row = row0 + r;
             double2 z = make_double2(0., 0.);
             if (row < nrows) z = *reinterpret_cast<const double2 *>(base + srow(row) + 2 * m);
             buf[lp(idx)] = z;
         }
     } else {
         for (int idx = threadIdx.x; idx < R * M; idx += blockDim.x) {
-            int r = idx / M, k = idx - r * M, row = row0 + r;
+            int r = idx >> logM, k = idx & (M - 1), row = row0 + r;
             double2 Z = make_double2(0., 0.);
             if (row < nrows) {
                 const double *p = base + srow(row);
@@ -221,14 +225,14 @@ __global__ __launch_bounds__(256) void k_fft_x(FieldBatch fb, int nrows, int ny,
         }
     }
     __syncthreads();
-    fft_lds(buf, M, logM, R, tw, DIR);
+    fft_lds(buf, logM, R, tw, DIR);
     if (DIR < 0) {
         const double sc = 1.0 / (double)N;
-        for (int idx = threadIdx.x; idx < R * (M + 1); idx += blockDim.x) {
-            int r = idx / (M + 1), k = idx - r * (M + 1), row = row0 + r;
-            if (row >= nrows) continue;
-            double2 Zk = buf[lp(r * M + (k == M ? 0 : k))];
-            double2 Zm = buf[lp(r * M + (k == 0 ? 0 : M - k))];
+        // coefficient k from Z_k and Z_{M-k}; the thread of k = 0 also
+        // writes k = M (both from Z_0)
+        auto emit = [&](int r, int k, double *orow) {
+            double2 Zk = buf[lp((r << logM) + (k == M ? 0 : k))];
+            double2 Zm = buf[lp((r << logM) + (k == 0 ? 0 : M - k))];
             double er = 0.5 * (Zk.x + Zm.x), ei = 0.5 * (Zk.y - Zm.y);
             double dr = Zk.x - Zm.x, di = Zk.y + Zm.y;
             double orr = 0.5 * di, oi = -0.5 * dr;
@@ -238,11 +242,18 @@ __global__ __launch_bounds__(256) void k_fft_x(FieldBatch fb, int nrows, int ny,
             double A = sc * xr, B = sc * xi;
             if (k == 0 || k == M) B = 0.0;
             if (k == M) A = A * scaley; // horfft.f:28-30 on the untransformed Nyquist column
-            *reinterpret_cast<double2 *>(obase + drow(row) + 2 * k) = make_double2(A, B);
+            *reinterpret_cast<double2 *>(orow + 2 * k) = make_double2(A, B);
+        };
+        for (int idx = threadIdx.x; idx < R * M; idx += blockDim.x) {
+            int r = idx >> logM, k = idx & (M - 1), row = row0 + r;
+            if (row >= nrows) continue;
+            double *orow = obase + drow(row);
+            emit(r, k, orow);
+            if (k == 0) emit(r, M, orow);
         }
     } else {
         for (int idx = threadIdx.x; idx < R * M; idx += blockDim.x) {
-            int r = idx / M, m = idx - r * M, row = row0 + r;
+            int r = idx >> logM, m = idx & (M - 1), row = row0 + r;
             if (row >= nrows) continue;
             *reinterpret_cast<double2 *>(obase + drow(row) + 2 * m) = buf[lp(idx)];
         }
@@ -270,18 +281,19 @@ __global__ __launch_bounds__(256) void k_fft_y(FieldBatch fb, int nxpp, int nxh,
     double *oplane = fb.g[blockIdx.z] + (long)(fb.dmap ? fb.dmap[lpl] : lpl) * nxpp * ny;
     const int ii0 = blockIdx.x * CB;
     for (int t = threadIdx.x; t < ny; t += blockDim.x) tw[t] = twY[t];
+    const int lcb = __builtin_ctz(CB); // CB is a power of two
     for (int idx = threadIdx.x; idx < CB * ny; idx += blockDim.x) {
-        int j = idx / CB, c = idx - j * CB, ii = ii0 + c;
+        int j = idx >> lcb, c = idx & (CB - 1), ii = ii0 + c;
         double2 z = make_double2(0., 0.);
         if (ii < nxh) z = *reinterpret_cast<const double2 *>(plane + (long)j * nxpp + 2 * ii);
-        buf[lp(c * ny + j)] = z;
+        buf[lp((c << logNy) + j)] = z;
     }
     __syncthreads();
-    fft_lds(buf, ny, logNy, CB, tw, DIR);
+    fft_lds(buf, logNy, CB, tw, DIR);
     for (int idx = threadIdx.x; idx < CB * ny; idx += blockDim.x) {
-        int j = idx / CB, c = idx - j * CB, ii = ii0 + c;
+        int j = idx >> lcb, c = idx & (CB - 1), ii = ii0 + c;
         if (ii >= nxh) continue;
-        double2 z = buf[lp(c * ny + j)];
+        double2 z = buf[lp((c << logNy) + j)];
         if (DIR < 0) {
             z.x = scaley * z.x;
             z.y = scaley * z.y;
@@ -1857,7 +1869,8 @@ __global__ __launch_bounds__(256) void k_pproject(PArgs A, double *u, double *v,
 }
 
 void launch_pressure_nat(const Geo &g, const iw_params &P, const SysTables &S, double *const *uvwt, double *phi,
-                         const double *wnl, const double *wviscrot, double dt, int cpen, hipStream_t st)
+                         const double *wnl, const double *wviscrot, double dt, int cpen, int part,
+                         hipStream_t st)
 {
     PArgs A;
     A.u = uvwt[0]; A.v = uvwt[1]; A.w = uvwt[2];
@@ -1875,11 +1888,12 @@ void launch_pressure_nat(const Geo &g, const iw_params &P, const SysTables &S, d
     dim3 gc((unsigned)((g.nxppy + 255) / 256), g.nsubd);
     dim3 gs(S.np_ntasks / 64);
     const int ns = P.bc_uv_bottom == 1;
-#define PSTEP(NL)                                                                                       \
-    hipLaunchKernelGGL(k_prhs<NL>, gc, dim3(256), 0, st, A);                                            \
-    hipLaunchKernelGGL(k_solve_p<NL>, gs, dim3(64), 0, st, B);                                          \
-    hipLaunchKernelGGL(k_pproject<NL>, gc, dim3(256), 0, st, A, uvwt[0], uvwt[1], uvwt[2], cpen, ns, g.wg, \
-                       g.AL, g.sig)
+#define PSTEP(NL)                                                                                         \
+    if (part == 0) hipLaunchKernelGGL(k_prhs<NL>, gc, dim3(256), 0, st, A);                               \
+    if (part == 1) hipLaunchKernelGGL(k_solve_p<NL>, gs, dim3(64), 0, st, B);                             \
+    if (part == 2)                                                                                        \
+        hipLaunchKernelGGL(k_pproject<NL>, gc, dim3(256), 0, st, A, uvwt[0], uvwt[1], uvwt[2], cpen, ns, g.wg, \
+                           g.AL, g.sig)
     switch (g.nzloc) {
     case 9: PSTEP(9); break;
     case 13: PSTEP(13); break;
