@@ -538,6 +538,38 @@ static int ensure_factors(iw_solver *s)
         HIPCHK(hipMalloc((void **)&S.np_tasks, sizeof(int) * tasks.size()));
         HIPCHK(hipMemcpy(S.np_tasks, tasks.data(), sizeof(int) * tasks.size(), hipMemcpyHostToDevice));
         S.np_ntasks = (int)tasks.size();
+        // u and v share their systems: a wavefront takes 8 kx of the rows
+        // +ky and -ky for both fields, so 8 lanes use each factor segment
+        std::vector<int> uv;
+        auto push_uv = [&](int jy, int ii) {
+            const int m = jy * g.nxhp + ii;
+            for (int f = 0; f < 2; f++)
+                for (int c = 0; c < 2; c++) uv.push_back((2 * m + c) | (f << 28));
+        };
+        const int B8 = 8;
+        for (int p = 1; p < (g.ny + 1) / 2; p++) {
+            const int q = g.ny - p;
+            if (q == p) continue;
+            for (int i0 = 0; i0 < g.nxh; i0 += B8) {
+                for (int ii = i0; ii < std::min(i0 + B8, g.nxh); ii++) push_uv(p, ii);
+                for (int ii = i0; ii < std::min(i0 + B8, g.nxh); ii++) push_uv(q, ii);
+            }
+        }
+        for (int jy : single)
+            for (int ii = 0; ii < g.nxh; ii++) push_uv(jy, ii);
+        for (int jy : single) push_uv(jy, g.nxh);
+        for (int p = 1; p < (g.ny + 1) / 2; p++) {
+            const int q = g.ny - p;
+            if (q == p) continue;
+            push_uv(p, g.nxh);
+            push_uv(q, g.nxh);
+        }
+        if ((long)uv.size() != 2 * g.nxppy) return fail("internal: u/v lane table does not cover the plane");
+        std::vector<int> uvt((uv.size() + 63) / 64 * 64, -1);
+        std::copy(uv.begin(), uv.end(), uvt.begin());
+        HIPCHK(hipMalloc((void **)&S.uv_tasks, sizeof(int) * uvt.size()));
+        HIPCHK(hipMemcpy(S.uv_tasks, uvt.data(), sizeof(int) * uvt.size(), hipMemcpyHostToDevice));
+        S.uv_ntasks = (int)uvt.size();
     }
     s->factored_bdfv = s->bdfv;
     s->factored_dt = s->P.dt;
@@ -764,6 +796,7 @@ extern "C" int iw_destroy(iw_solver *s)
     hipFree(s->sys.order_p);
     hipFree(s->sys.order_v);
     hipFree(s->sys.np_tasks);
+    hipFree(s->sys.uv_tasks);
     hipEventDestroy(s->ev0);
     hipEventDestroy(s->ev1);
     hipStreamDestroy(s->stream);

@@ -50,8 +50,10 @@ struct SysTables {
     int *order_p = nullptr, *order_v = nullptr;
     int nblk_p = 0, nblk_v = 0;
     // lanes-as-columns path for systems without row interchanges
-    int *np_tasks = nullptr;
+    int *np_tasks = nullptr;  // lane -> column (2 m + comp), -1 idle
     int np_ntasks = 0;
+    int *uv_tasks = nullptr;  // lane -> column | field << 28 for the shared u/v systems
+    int uv_ntasks = 0;
     bool np_ok = false;
     size_t cap_sys = 0;
 };

@@ -193,14 +193,18 @@ __global__ __launch_bounds__(256) void k_fft_x(FieldBatch fb, int nrows, int ny,
     };
     for (int t = threadIdx.x; t < M; t += blockDim.x) tw[t] = twM[t];
     if (DIR < 0) {
-        for (int idx = threadIdx.x; idx < R * M; idx += blockDim.x) {
+#pragma unroll
+        for (int it = 0; it < kFE / 256; it++) {
+            const int idx = threadIdx.x + it * 256;
             int r = idx >> logM, m = idx & (M - 1), row = row0 + r;
             double2 z = make_double2(0., 0.);
             if (row < nrows) z = *reinterpret_cast<const double2 *>(base + srow(row) + 2 * m);
             buf[lp(idx)] = z;
         }
     } else {
-        for (int idx = threadIdx.x; idx < R * M; idx += blockDim.x) {
+#pragma unroll
+        for (int it = 0; it < kFE / 256; it++) {
+            const int idx = threadIdx.x + it * 256;
             int r = idx >> logM, k = idx & (M - 1), row = row0 + r;
             double2 Z = make_double2(0., 0.);
             if (row < nrows) {
@@ -244,7 +248,9 @@ __global__ __launch_bounds__(256) void k_fft_x(FieldBatch fb, int nrows, int ny,
             if (k == M) A = A * scaley; // horfft.f:28-30 on the untransformed Nyquist column
             *reinterpret_cast<double2 *>(orow + 2 * k) = make_double2(A, B);
         };
-        for (int idx = threadIdx.x; idx < R * M; idx += blockDim.x) {
+#pragma unroll
+        for (int it = 0; it < kFE / 256; it++) {
+            const int idx = threadIdx.x + it * 256;
             int r = idx >> logM, k = idx & (M - 1), row = row0 + r;
             if (row >= nrows) continue;
             double *orow = obase + drow(row);
@@ -252,7 +258,9 @@ __global__ __launch_bounds__(256) void k_fft_x(FieldBatch fb, int nrows, int ny,
             if (k == 0) emit(r, M, orow);
         }
     } else {
-        for (int idx = threadIdx.x; idx < R * M; idx += blockDim.x) {
+#pragma unroll
+        for (int it = 0; it < kFE / 256; it++) {
+            const int idx = threadIdx.x + it * 256;
             int r = idx >> logM, m = idx & (M - 1), row = row0 + r;
             if (row >= nrows) continue;
             *reinterpret_cast<double2 *>(obase + drow(row) + 2 * m) = buf[lp(idx)];
@@ -282,7 +290,10 @@ __global__ __launch_bounds__(256) void k_fft_y(FieldBatch fb, int nxpp, int nxh,
     const int ii0 = blockIdx.x * CB;
     for (int t = threadIdx.x; t < ny; t += blockDim.x) tw[t] = twY[t];
     const int lcb = __builtin_ctz(CB); // CB is a power of two
-    for (int idx = threadIdx.x; idx < CB * ny; idx += blockDim.x) {
+#pragma unroll
+    for (int it = 0; it < kFE / 256; it++) {
+        const int idx = threadIdx.x + it * 256;
+        if (idx >= CB * ny) break;
         int j = idx >> lcb, c = idx & (CB - 1), ii = ii0 + c;
         double2 z = make_double2(0., 0.);
         if (ii < nxh) z = *reinterpret_cast<const double2 *>(plane + (long)j * nxpp + 2 * ii);
@@ -290,7 +301,10 @@ __global__ __launch_bounds__(256) void k_fft_y(FieldBatch fb, int nxpp, int nxh,
     }
     __syncthreads();
     fft_lds(buf, logNy, CB, tw, DIR);
-    for (int idx = threadIdx.x; idx < CB * ny; idx += blockDim.x) {
+#pragma unroll
+    for (int it = 0; it < kFE / 256; it++) {
+        const int idx = threadIdx.x + it * 256;
+        if (idx >= CB * ny) break;
         int j = idx >> lcb, c = idx & (CB - 1), ii = ii0 + c;
         if (ii >= nxh) continue;
         double2 z = buf[lp((c << logNy) + j)];
@@ -1473,6 +1487,8 @@ void launch_viscous(const Geo &g, const iw_params &P, const SysTables &S, double
 struct NatArgs {
     double *f[4];
     const int *tasks;
+    const int *uv_tasks;
+    int ntask_blocks;
     const int *modesys;
     const double *AB;
     const double *sysinfo;
@@ -1496,8 +1512,18 @@ __device__ __forceinline__ void ld_seg(const double *__restrict__ p, double (&c)
 template <int KL>
 __global__ __launch_bounds__(64) void k_solve_nat(NatArgs A)
 {
-    const int col = A.tasks[blockIdx.x * 64 + threadIdx.x];
-    const int fld = blockIdx.y;
+    // blockIdx.y = 0: u and v lanes from the shared table; 1: w; 2: rho
+    int col, fld;
+    if (blockIdx.y == 0) {
+        const int e = A.uv_tasks[blockIdx.x * 64 + threadIdx.x];
+        if (e < 0) return;
+        col = e & ((1 << 28) - 1);
+        fld = e >> 28;
+    } else {
+        if ((int)blockIdx.x >= A.ntask_blocks) return;
+        col = A.tasks[blockIdx.x * 64 + threadIdx.x];
+        fld = blockIdx.y + 1;
+    }
     if (col < 0) return;
     const int m = col >> 1;
     if (fld == 2 && m == 0) return; // w(0,0) = 0 (calcuvwt.f:496-503)
@@ -1583,6 +1609,8 @@ void launch_solve_nat(const Geo &g, const iw_params &P, const SysTables &S, doub
     NatArgs A;
     for (int q = 0; q < 4; q++) A.f[q] = uvwt[q];
     A.tasks = S.np_tasks;
+    A.uv_tasks = S.uv_tasks;
+    A.ntask_blocks = S.np_ntasks / 64;
     A.modesys = S.modesys;
     A.AB = S.AB;
     A.sysinfo = S.sysinfo;
@@ -1592,7 +1620,7 @@ void launch_solve_nat(const Geo &g, const iw_params &P, const SysTables &S, doub
     A.n = g.nz;
     A.ldab = S.ldab;
     A.kv = S.kv;
-    dim3 grid(S.np_ntasks / 64, 4);
+    dim3 grid(S.uv_ntasks / 64, 3);
     switch (g.nzloc) {
     case 9: hipLaunchKernelGGL(k_solve_nat<9>, grid, dim3(64), 0, st, A); break;
     case 13: hipLaunchKernelGGL(k_solve_nat<13>, grid, dim3(64), 0, st, A); break;
