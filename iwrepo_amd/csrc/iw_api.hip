@@ -570,6 +570,64 @@ static int ensure_factors(iw_solver *s)
         HIPCHK(hipMalloc((void **)&S.uv_tasks, sizeof(int) * uvt.size()));
         HIPCHK(hipMemcpy(S.uv_tasks, uvt.data(), sizeof(int) * uvt.size(), hipMemcpyHostToDevice));
         S.uv_ntasks = (int)uvt.size();
+        // grouped layouts: G lanes per system, groups of one kx on the rows
+        // +ky and -ky (u and v together for G = 8); rows without a partner
+        // fill their group with idle lanes
+        auto groups = [&](int G, bool both, std::vector<int> &out) -> int {
+            auto ent = [&](int jy, int ii, std::vector<int> &v) {
+                const int m = jy * g.nxhp + ii;
+                for (int f = 0; f < (both ? 2 : 1); f++)
+                    for (int c = 0; c < 2; c++) v.push_back((2 * m + c) | (f << 28));
+            };
+            auto add = [&](std::vector<int> &v) {
+                const int first = v[0];
+                while ((int)v.size() < G) v.push_back(first | (1 << 29));
+                out.insert(out.end(), v.begin(), v.end());
+            };
+            for (int p = 1; p < (g.ny + 1) / 2; p++) {
+                const int q = g.ny - p;
+                if (q == p) continue;
+                for (int ii = 0; ii < g.nxh; ii++) {
+                    std::vector<int> v;
+                    ent(p, ii, v);
+                    ent(q, ii, v);
+                    add(v);
+                }
+            }
+            for (int jy : single)
+                for (int ii = 0; ii < g.nxh; ii++) {
+                    std::vector<int> v;
+                    ent(jy, ii, v);
+                    add(v);
+                }
+            for (int jy : single) {
+                std::vector<int> v;
+                ent(jy, g.nxh, v);
+                add(v);
+            }
+            for (int p = 1; p < (g.ny + 1) / 2; p++) {
+                const int q = g.ny - p;
+                if (q == p) continue;
+                std::vector<int> v;
+                ent(p, g.nxh, v);
+                ent(q, g.nxh, v);
+                add(v);
+            }
+            long live = 0;
+            for (int e : out) live += ((e >> 29) & 1) ? 0 : 1;
+            if (live != (both ? 2 : 1) * g.nxppy) return -1;
+            while (out.size() % 64) out.push_back(-1);
+            return 0;
+        };
+        std::vector<int> guv, gwt;
+        if (groups(8, true, guv) || groups(4, false, gwt))
+            return fail("internal: grouped lane tables do not cover the plane");
+        HIPCHK(hipMalloc((void **)&S.guv_tasks, sizeof(int) * guv.size()));
+        HIPCHK(hipMemcpy(S.guv_tasks, guv.data(), sizeof(int) * guv.size(), hipMemcpyHostToDevice));
+        S.guv_ntasks = (int)guv.size();
+        HIPCHK(hipMalloc((void **)&S.gwt_tasks, sizeof(int) * gwt.size()));
+        HIPCHK(hipMemcpy(S.gwt_tasks, gwt.data(), sizeof(int) * gwt.size(), hipMemcpyHostToDevice));
+        S.gwt_ntasks = (int)gwt.size();
     }
     s->factored_bdfv = s->bdfv;
     s->factored_dt = s->P.dt;
@@ -797,6 +855,8 @@ extern "C" int iw_destroy(iw_solver *s)
     hipFree(s->sys.order_v);
     hipFree(s->sys.np_tasks);
     hipFree(s->sys.uv_tasks);
+    hipFree(s->sys.guv_tasks);
+    hipFree(s->sys.gwt_tasks);
     hipEventDestroy(s->ev0);
     hipEventDestroy(s->ev1);
     hipStreamDestroy(s->stream);
@@ -1105,7 +1165,7 @@ static int calcuvwt_dev(iw_solver *s, int cpen)
             // + each distinct no-interchange factor once ((2 kl + 1) rows)
             const double fnp = (double)(2 * s->sys.kl + 1) * s->g.nz * 8.;
             Scope sc(s, "solve_nat", 4 * 16. * nt + (s->sys.nsys - s->sys.np) * fnp);
-            launch_solve_nat(s->g, s->P, s->sys, uvwt, s->P.dt, s->stream);
+            launch_solve_grp(s->g, s->P, s->sys, uvwt, s->P.dt, s->stream);
         }
         {
             Scope sc(s, "lfilter", 4 * 16. * nt);

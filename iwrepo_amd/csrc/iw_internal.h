@@ -54,6 +54,10 @@ struct SysTables {
     int np_ntasks = 0;
     int *uv_tasks = nullptr;  // lane -> column | field << 28 for the shared u/v systems
     int uv_ntasks = 0;
+    int *guv_tasks = nullptr; // grouped layouts (8 lanes per u/v system, 4 per w/rho system)
+    int guv_ntasks = 0;
+    int *gwt_tasks = nullptr;
+    int gwt_ntasks = 0;
     bool np_ok = false;
     size_t cap_sys = 0;
 };
@@ -132,6 +136,8 @@ bool solve_np_supported(int nl, int nsubd);
 void launch_solve_nat(const Geo &g, const iw_params &P, const SysTables &S, double *const *uvwt, double dt,
                       hipStream_t st);
 void launch_lfilter(const Geo &g, double *const *uvwt, int cpen, hipStream_t st);
+void launch_solve_grp(const Geo &g, const iw_params &P, const SysTables &S, double *const *uvwt, double dt,
+                      hipStream_t st);
 // part 0: right-hand side, 1: solve, 2: filter + projection
 void launch_pressure_nat(const Geo &g, const iw_params &P, const SysTables &S, double *const *uvwt, double *phi,
                          const double *wnl, const double *wviscrot, double dt, int cpen, int part,

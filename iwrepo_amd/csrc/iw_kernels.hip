@@ -6,6 +6,8 @@
 // spectral plane is nxhp*ny contiguous complex values.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "iw_internal.h"
 
 namespace iw {
@@ -71,25 +73,38 @@ __device__ __forceinline__ void dft(double2 *a, int sign)
     }
 }
 
-template <int R, int LR>
-__device__ __forceinline__ void lds_pass(double2 *buf, int logN, int logNs, int nrows, const double2 *tw,
-                                         int sign)
+// COL = false: sequence r occupies lp(r*N + i) and consecutive threads take
+// consecutive butterflies of one sequence.  COL = true: element i of
+// sequence r sits at i*nrows + r and consecutive threads take the same
+// butterfly of consecutive sequences (nrows a power of two), so every
+// 16-lane access is 16 consecutive slots.
+template <int R, int LR, bool COL>
+__device__ __forceinline__ void lds_pass(double2 *buf, int logN, int logNs, int nrows, int lrows,
+                                         const double2 *tw, int sign)
 {
     constexpr int BPT = 8 / R;
     const int lNR = logN - LR, NR = 1 << lNR, Ns = 1 << logNs, nb = nrows << lNR;
     const int lstep = logN - logNs - LR;
     double2 a[BPT][R];
-    int base[BPT];
+    int base[BPT], rr[BPT];
 #pragma unroll
     for (int i = 0; i < BPT; i++) {
         const int b = threadIdx.x + i * 256;
         base[i] = -1;
+        rr[i] = 0;
         if (b < nb) {
-            const int r = b >> lNR, j = b & (NR - 1);
+            int r, j;
+            if (COL) {
+                r = b & (nrows - 1);
+                j = b >> lrows;
+            } else {
+                r = b >> lNR;
+                j = b & (NR - 1);
+            }
             const int k = j & (Ns - 1);
-            const int rb = r << logN;
 #pragma unroll
-            for (int m = 0; m < R; m++) a[i][m] = buf[lp(rb + j + m * NR)];
+            for (int m = 0; m < R; m++)
+                a[i][m] = COL ? buf[((j + m * NR) << lrows) + r] : buf[lp((r << logN) + j + m * NR)];
 #pragma unroll
             for (int m = 1; m < R; m++) {
                 double2 w = tw[(m * k) << lstep];
@@ -97,7 +112,8 @@ __device__ __forceinline__ void lds_pass(double2 *buf, int logN, int logNs, int 
                 a[i][m] = cmul(a[i][m], w);
             }
             dft<R>(a[i], sign);
-            base[i] = rb + (j - k) * R + k;
+            base[i] = (j - k) * R + k;
+            rr[i] = r;
         }
     }
     __syncthreads();
@@ -105,21 +121,29 @@ __device__ __forceinline__ void lds_pass(double2 *buf, int logN, int logNs, int 
     for (int i = 0; i < BPT; i++)
         if (base[i] >= 0)
 #pragma unroll
-            for (int q = 0; q < R; q++) buf[lp(base[i] + (q << logNs))] = a[i][q];
+            for (int q = 0; q < R; q++) {
+                const int e = base[i] + (q << logNs);
+                if (COL)
+                    buf[(e << lrows) + rr[i]] = a[i][q];
+                else
+                    buf[lp((rr[i] << logN) + e)] = a[i][q];
+            }
     __syncthreads();
 }
 
-// nrows sequences of length N = 2^logN in buf (padded indexing), in place
+// nrows sequences of length N = 2^logN in buf, in place
+template <bool COL>
 __device__ void fft_lds(double2 *buf, int logN, int nrows, const double2 *tw, int sign)
 {
+    const int lrows = COL ? __builtin_ctz(nrows) : 0;
     int lNs = 0, left = logN;
     while (left >= 3) {
-        lds_pass<8, 3>(buf, logN, lNs, nrows, tw, sign);
+        lds_pass<8, 3, COL>(buf, logN, lNs, nrows, lrows, tw, sign);
         lNs += 3;
         left -= 3;
     }
-    if (left == 2) lds_pass<4, 2>(buf, logN, lNs, nrows, tw, sign);
-    if (left == 1) lds_pass<2, 1>(buf, logN, lNs, nrows, tw, sign);
+    if (left == 2) lds_pass<4, 2, COL>(buf, logN, lNs, nrows, lrows, tw, sign);
+    if (left == 1) lds_pass<2, 1, COL>(buf, logN, lNs, nrows, lrows, tw, sign);
 }
 
 // filter_legenpen (filtering.f:729-808) of the NL values x of one column in
@@ -229,7 +253,7 @@ __global__ __launch_bounds__(256) void k_fft_x(FieldBatch fb, int nrows, int ny,
         }
     }
     __syncthreads();
-    fft_lds(buf, logM, R, tw, DIR);
+    fft_lds<false>(buf, logM, R, tw, DIR);
     if (DIR < 0) {
         const double sc = 1.0 / (double)N;
         // coefficient k from Z_k and Z_{M-k}; the thread of k = 0 also
@@ -283,7 +307,7 @@ __global__ __launch_bounds__(256) void k_fft_y(FieldBatch fb, int nxpp, int nxh,
 {
     extern __shared__ double2 smy[];
     double2 *buf = smy;
-    double2 *tw = smy + lp(CB * ny) + 1;
+    double2 *tw = smy + CB * ny;
     const int lpl = blockIdx.y;
     const double *plane = fb.f[blockIdx.z] + (long)(fb.smap ? fb.smap[lpl] : lpl) * nxpp * ny;
     double *oplane = fb.g[blockIdx.z] + (long)(fb.dmap ? fb.dmap[lpl] : lpl) * nxpp * ny;
@@ -297,17 +321,17 @@ __global__ __launch_bounds__(256) void k_fft_y(FieldBatch fb, int nxpp, int nxh,
         int j = idx >> lcb, c = idx & (CB - 1), ii = ii0 + c;
         double2 z = make_double2(0., 0.);
         if (ii < nxh) z = *reinterpret_cast<const double2 *>(plane + (long)j * nxpp + 2 * ii);
-        buf[lp((c << logNy) + j)] = z;
+        buf[(j << lcb) + c] = z;
     }
     __syncthreads();
-    fft_lds(buf, logNy, CB, tw, DIR);
+    fft_lds<true>(buf, logNy, CB, tw, DIR);
 #pragma unroll
     for (int it = 0; it < kFE / 256; it++) {
         const int idx = threadIdx.x + it * 256;
         if (idx >= CB * ny) break;
         int j = idx >> lcb, c = idx & (CB - 1), ii = ii0 + c;
         if (ii >= nxh) continue;
-        double2 z = buf[lp((c << logNy) + j)];
+        double2 z = buf[(j << lcb) + c];
         if (DIR < 0) {
             z.x = scaley * z.x;
             z.y = scaley * z.y;
@@ -344,7 +368,7 @@ void launch_fft_y(const Geo &g, FieldBatch b, int nk, int dir, hipStream_t st)
     int CB = kFE / g.ny;
     if (CB > g.nxh) CB = g.nxh;
     dim3 grid((g.nxh + CB - 1) / CB, nk, b.n);
-    size_t sh = sizeof(double2) * ((size_t)lp(CB * g.ny) + 1 + g.ny);
+    size_t sh = sizeof(double2) * ((size_t)CB * g.ny + g.ny);
     if (dir < 0)
         hipLaunchKernelGGL(k_fft_y<-1>, grid, dim3(256), sh, st, b, g.nxpp, g.nxh, g.ny, g.logNy, CB,
                            g.twY, g.scaley);
@@ -1629,6 +1653,216 @@ void launch_solve_nat(const Geo &g, const iw_params &P, const SysTables &S, doub
     case 25: hipLaunchKernelGGL(k_solve_nat<25>, grid, dim3(64), 0, st, A); break;
     default: break;
     }
+}
+
+// ---------------------------------------------------------------------------
+// Grouped variant of the same solves.  The G lanes of a group share one
+// system (host layout: +ky and -ky rows of one kx, and for G = 8 both u and
+// v).  Each lane fetches 1/G of the factor segment DP steps ahead into a
+// register ring; one step before use the pieces are put together in LDS and
+// every lane of the group reads the whole segment from there.  This keeps
+// DP steps of distinct factor data in flight per wavefront instead of one
+// step of G identical copies.  Task entry: col | field << 28 | idle << 29
+// (an idle lane helps load its group's factors but owns no column); -1
+// marks an empty group.
+// ---------------------------------------------------------------------------
+// compile-time loop: f(std::integral_constant<int, I>) for I = B..E-1
+template <int B, int E, class F>
+__device__ __forceinline__ void static_for(F &&f)
+{
+    if constexpr (B < E) {
+        f(std::integral_constant<int, B>{});
+        static_for<B + 1, E>(f);
+    }
+}
+
+struct GrpArgs {
+    double *f[4];
+    const int *tasks;
+    const int *modesys;
+    const double *AB;
+    const double *sysinfo;
+    double den[4];
+    long nxppy;
+    int n, ldab, kv, fld_base;
+};
+
+template <int KL, int G, int DP>
+__global__ __launch_bounds__(64) void k_solve_grp(GrpArgs A)
+{
+    constexpr int NG = 64 / G, PPL = 16 / G;
+    constexpr int NPF = (KL + 1) / 2; // forward segment, rows kv..kv+KL of a column
+    constexpr int NPB = (KL + 3) / 2; // backward segment, rows kv-KL-1..kv+1
+    __shared__ double2 seg[2][NG][17]; // 17: groups start on different banks
+    const int lane = threadIdx.x, grp = lane / G, rk = lane % G;
+    const int e = A.tasks[blockIdx.x * 64 + lane];
+    if (e < 0) return; // the whole group is empty
+    const bool idle = (e >> 29) & 1;
+    const int col = e & ((1 << 28) - 1);
+    const int fld = A.fld_base + blockIdx.y + ((e >> 28) & 1);
+    const int m = col >> 1;
+    if (fld == 2 && m == 0) return; // w(0,0) = 0 (calcuvwt.f:496-503); its group holds only this mode
+    const int sys = A.modesys[4 * m + (fld < 2 ? 1 : fld == 2 ? 2 : 3)];
+    const double eps = A.sysinfo[4 * sys], den = A.den[fld];
+    const int n = A.n, kv = A.kv;
+    const long st = A.nxppy, cst = A.ldab / 2; // double2 stride between factor columns
+    const double2 *__restrict__ ab2 = reinterpret_cast<const double2 *>(A.AB + (long)sys * A.ldab * n);
+    double *__restrict__ b = A.f[fld] + col;
+    double w[KL + 1];
+    double rx[DP][PPL], ry[DP][PPL]; // ring of factor pieces (re/im halves of 16-byte loads)
+    double bring[DP];
+    // ---- forward: unit lower triangle (bandludcmp.f:1020-1030)
+    {
+        const double2 *f0 = ab2 + kv / 2;
+        auto ldp = [&](int sidx, double(&dx)[PPL], double(&dy)[PPL]) __attribute__((always_inline)) {
+            const long c = sidx < n ? sidx : n - 1;
+#pragma unroll
+            for (int q = 0; q < PPL; q++) {
+                const int p = rk + q * G;
+                const double2 v = f0[cst * c + (p < NPF ? p : 0)];
+                dx[q] = v.x;
+                dy[q] = v.y;
+            }
+        };
+        auto stp = [&](int buf, const double(&sx)[PPL], const double(&sy)[PPL]) __attribute__((always_inline)) {
+#pragma unroll
+            for (int q = 0; q < PPL; q++) {
+                const int p = rk + q * G;
+                if (p < NPF) seg[buf][grp][p] = make_double2(sx[q], sy[q]);
+            }
+        };
+#pragma unroll
+        for (int t = 0; t <= KL; t++) {
+            const double v = b[(long)(t < n ? t : n - 1) * st];
+            w[t] = (t < n) ? -(-v / den) * eps : 0.;
+        }
+        static_for<0, DP>([&](auto dc) __attribute__((always_inline)) {
+            constexpr int d = decltype(dc)::value;
+            ldp(d, rx[d], ry[d]);
+            const int r = KL + 1 + d;
+            bring[d] = b[(long)(r < n ? r : n - 1) * st];
+        });
+        stp(0, rx[0], ry[0]);
+        for (int j0 = 0; j0 < n; j0 += DP) {
+            static_for<0, DP>([&](auto dc) __attribute__((always_inline)) {
+                constexpr int d = decltype(dc)::value;
+                const int j = j0 + d;
+                if (j < n) {
+                stp((j + 1) & 1, rx[(d + 1) % DP], ry[(d + 1) % DP]);
+                ldp(j + DP, rx[d], ry[d]);
+                double fc[2 * NPF];
+#pragma unroll
+                for (int p = 0; p < NPF; p++) {
+                    const double2 v = seg[j & 1][grp][p];
+                    fc[2 * p] = v.x;
+                    fc[2 * p + 1] = v.y;
+                }
+                const double rc = bring[d];
+                const int r = j + KL + 1 + DP;
+                bring[d] = b[(long)(r < n ? r : n - 1) * st];
+                const double y = w[0];
+#pragma unroll
+                for (int t = 1; t <= KL; t++) w[t - 1] = w[t] + fc[t] * (-y);
+                w[KL] = (j + KL + 1 < n) ? -(-rc / den) * eps : 0.;
+                if (!idle) b[(long)j * st] = y;
+                }
+            });
+        }
+    }
+    // ---- backward: upper band with KL superdiagonals (STBSV, ludcmp.f:4698-5046)
+    {
+        const double2 *g0 = ab2 + (kv - KL - 1) / 2;
+        auto ldp = [&](int sidx, double(&dx)[PPL], double(&dy)[PPL]) __attribute__((always_inline)) {
+            const long c = sidx < n ? n - 1 - sidx : 0;
+#pragma unroll
+            for (int q = 0; q < PPL; q++) {
+                const int p = rk + q * G;
+                const double2 v = g0[cst * c + (p < NPB ? p : 0)];
+                dx[q] = v.x;
+                dy[q] = v.y;
+            }
+        };
+        auto stp = [&](int buf, const double(&sx)[PPL], const double(&sy)[PPL]) __attribute__((always_inline)) {
+#pragma unroll
+            for (int q = 0; q < PPL; q++) {
+                const int p = rk + q * G;
+                if (p < NPB) seg[buf][grp][p] = make_double2(sx[q], sy[q]);
+            }
+        };
+#pragma unroll
+        for (int t = 0; t <= KL; t++) {
+            const int r = n - 1 - t;
+            const double v = b[(long)(r >= 0 ? r : 0) * st];
+            w[t] = (r >= 0) ? v : 0.;
+        }
+        static_for<0, DP>([&](auto dc) __attribute__((always_inline)) {
+            constexpr int d = decltype(dc)::value;
+            ldp(d, rx[d], ry[d]);
+            const int r = n - 2 - KL - d;
+            bring[d] = b[(long)(r >= 0 ? r : 0) * st];
+        });
+        stp(0, rx[0], ry[0]);
+        for (int s0 = 0; s0 < n; s0 += DP) {
+            static_for<0, DP>([&](auto dc) __attribute__((always_inline)) {
+                constexpr int d = decltype(dc)::value;
+                const int sidx = s0 + d, j = n - 1 - sidx;
+                if (sidx < n) {
+                stp((sidx + 1) & 1, rx[(d + 1) % DP], ry[(d + 1) % DP]);
+                ldp(sidx + DP, rx[d], ry[d]);
+                double gc[2 * NPB];
+#pragma unroll
+                for (int p = 0; p < NPB; p++) {
+                    const double2 v = seg[sidx & 1][grp][p];
+                    gc[2 * p] = v.x;
+                    gc[2 * p + 1] = v.y;
+                }
+                const double rc = bring[d];
+                const int r = j - KL - 1 - DP;
+                bring[d] = b[(long)(r >= 0 ? r : 0) * st];
+                const double dc = gc[KL + 1];
+                const double x = (w[0] != 0.) ? w[0] / dc : w[0];
+#pragma unroll
+                for (int t = 1; t <= KL; t++) w[t - 1] = w[t] - x * gc[KL + 1 - t];
+                w[KL] = (j - KL - 1 >= 0) ? rc : 0.;
+                if (!idle) b[(long)j * st] = x;
+                }
+            });
+        }
+    }
+}
+
+void launch_solve_grp(const Geo &g, const iw_params &P, const SysTables &S, double *const *uvwt, double dt,
+                      hipStream_t st)
+{
+    GrpArgs A;
+    for (int q = 0; q < 4; q++) A.f[q] = uvwt[q];
+    A.modesys = S.modesys;
+    A.AB = S.AB;
+    A.sysinfo = S.sysinfo;
+    A.den[0] = A.den[1] = A.den[2] = P.xnu * dt;
+    A.den[3] = dt * P.xkappa;
+    A.nxppy = g.nxppy;
+    A.n = g.nz;
+    A.ldab = S.ldab;
+    A.kv = S.kv;
+    GrpArgs B = A;
+    A.tasks = S.guv_tasks;
+    A.fld_base = 0;
+    B.tasks = S.gwt_tasks;
+    B.fld_base = 2;
+    dim3 ga(S.guv_ntasks / 64, 1), gb(S.gwt_ntasks / 64, 2);
+#define GS(KL)                                                                                  \
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_solve_grp<KL, 8, 8>), ga, dim3(64), 0, st, A);         \
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_solve_grp<KL, 4, 4>), gb, dim3(64), 0, st, B)
+    switch (g.nzloc) {
+    case 9: GS(9); break;
+    case 13: GS(13); break;
+    case 17: GS(17); break;
+    case 21: GS(21); break;
+    case 25: GS(25); break;
+    default: break;
+    }
+#undef GS
 }
 
 // Legendre filter of every subdomain of the four solutions (filter_legenpen,
