@@ -31,7 +31,9 @@ typedef struct iw_params {
     int ispecfilterexp;            /* filter penalty solutions ...                 */
     int nspecfilterexp;            /* ... every nspecfilterexp steps               */
     int navgintrfc;                /* interface averaging period (main.f:920)      */
-    int pad0;
+    int ifilter, nstepfilter;      /* estimation-model filter every nstepfilter
+                                      steps (main.f:880-895)                      */
+    int npostp;                    /* ... with the SGS tensors every npostp steps  */
     double xlen, ylen, zlen;
     double xnu, xkappa, brunt, grav, rho0, omz;
     double dt;
@@ -39,6 +41,7 @@ typedef struct iw_params {
     double facdir, facneumn, facrobin; /* penalty factors (penalty.f:57-157)      */
     double zcen, xbcrit;           /* mean-u damping (calcuvwt.f:343-361)          */
     double xacrit, umax, fwave;    /* virtual paddle (forcing.f:325-447)           */
+    double fackcbv;                /* Boyd-Vandeven cut-off fraction (filtering.f:1040) */
     double z0[IW_MAXSUB], zh[IW_MAXSUB]; /* subdomains (subd.dat, setup.f:316-325) */
 } iw_params;
 

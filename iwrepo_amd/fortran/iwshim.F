@@ -28,7 +28,8 @@ c     layout of iw_params (include/iwgpu.h)
       type, bind(c) :: iw_params_t
         integer(c_int) :: nx, ny, nzloc, nsubd, bc_uv_bottom
         integer(c_int) :: ispecfilternl, ispecfilterexp
-        integer(c_int) :: nspecfilterexp, navgintrfc, pad0
+        integer(c_int) :: nspecfilterexp, navgintrfc
+        integer(c_int) :: ifilter, nstepfilter, npostp
         real(c_double) :: xlen, ylen, zlen
         real(c_double) :: xnu, xkappa, brunt, grav, rho0, omz
         real(c_double) :: dt
@@ -36,6 +37,7 @@ c     layout of iw_params (include/iwgpu.h)
         real(c_double) :: facdir, facneumn, facrobin
         real(c_double) :: zcen, xbcrit
         real(c_double) :: xacrit, umax, fwave
+        real(c_double) :: fackcbv
         real(c_double) :: z0(iw_maxsub), zh(iw_maxsub)
       end type
       type(c_ptr), save :: iwh = c_null_ptr
@@ -214,7 +216,10 @@ c     the step-loop flags are arguments of convec/calcuvwt here
       iwp%ispecfilterexp = 1
       iwp%nspecfilterexp = 1
       iwp%navgintrfc = 1
-      iwp%pad0 = 0
+c     the estimation filter is an argument of filtering here
+      iwp%ifilter = 0
+      iwp%nstepfilter = 1
+      iwp%npostp = 1
       iwp%xlen = xlen
       iwp%ylen = ylen
       iwp%zlen = zlen
@@ -237,6 +242,7 @@ c     the step-loop flags are arguments of convec/calcuvwt here
       iwp%xacrit = xacrit
       iwp%umax = umax
       iwp%fwave = fwave
+      iwp%fackcbv = fackcbv
       iwp%z0 = 0.d0
       iwp%zh = 0.d0
       do ks = 1, nsubd

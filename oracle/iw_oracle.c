@@ -38,6 +38,9 @@ struct iwo_state {
     int istep;
     /* FFT twiddle tables: cos/sin(2*pi*k/n) for n = nx and n = ny */
     double *twx_c, *twx_s, *twy_c, *twy_s;
+    /* estimation filter: /comsgs/ t11,t12,t13,t22,t23,t33,th1,th2,th3
+       (comsgs.h) and the /comfiltertmp/ work arrays (comfiltertmp.h) */
+    double *sgs[9], *tmps[6];
 };
 
 #define U(s) ((s)->F[0])
@@ -319,6 +322,8 @@ iwo_state *iwo_create(const iwo_params *p)
     s->ss1 = dalloc(s->ntot);
     s->ss2 = dalloc(s->ntot);
     s->ox = dalloc(s->ntot);
+    for (int q = 0; q < 9; q++) s->sgs[q] = dalloc(s->ntot);
+    for (int q = 0; q < 6; q++) s->tmps[q] = dalloc(s->ntot);
     for (int f = 0; f < 8; f++) s->work[f] = dalloc(s->ntot);
     s->wnl = dalloc(s->nxppy * 2);
     s->wviscrot = dalloc(s->nxppy * 2);
@@ -344,6 +349,8 @@ void iwo_destroy(iwo_state *s)
     for (int f = 0; f < 24; f++) free(s->F[f]);
     for (int f = 0; f < 12; f++) free(s->g[f]);
     free(s->ss1); free(s->ss2); free(s->ox);
+    for (int q = 0; q < 9; q++) free(s->sgs[q]);
+    for (int q = 0; q < 6; q++) free(s->tmps[q]);
     for (int f = 0; f < 8; f++) free(s->work[f]);
     free(s->wnl); free(s->wviscrot);
     free(s->ub); free(s->vb); free(s->wb); free(s->phi); free(s->dphidz);
@@ -1349,6 +1356,232 @@ void iwo_set_initial(iwo_state *s, const double *u, const double *v, const doubl
     s->bdfv = 1.;
 }
 
+/* ------------------------------------------------------------------ */
+/* Estimation-model filter and SGS tensors: filtering.f:8-270,         */
+/* 989-1105, 1272-1310, sgscalc.f:4-223                                */
+/* ------------------------------------------------------------------ */
+
+/* Boyd-Vandeven filter (filtering.f:1272-1310).  Its erfc is the Fortran
+ * 2008 intrinsic (no EXTERNAL declaration in filtering.f), i.e. the C
+ * library's erfc: the compiled reference references libm's erfc there and
+ * leaves the Numerical Recipes version of erfc.f unused. */
+static double specfilter_bv(int k, int kc, int n, double p)
+{
+    if (k > kc && k < n) {
+        double theta = (double)(k - kc) / (double)(n - kc);
+        double omega = fabs(theta) - 0.5, chi;
+        if (theta == 0.5) {
+            chi = 1.;
+        } else {
+            double c1 = 4. * pow(omega, 2.);
+            double c2 = -log(1. - c1) / c1;
+            chi = sqrt(c2);
+        }
+        double arg = 2. * sqrt(p) * chi * omega;
+        return 0.5 * erfc(arg);
+    }
+    return (k == n) ? 0. : 1.;
+}
+
+/* 3-point periodic top-hat filters in x and y (filtering.f:162-270):
+ * uf = u everywhere, then uf = u(i-1)/4 + u(i)/2 + u(i+1)/4 for i <= nx */
+static void filterx3pv(const iwo_state *s, const double *u, double *uf)
+{
+    const double a = 1. / 4., b = 2. / 4., c = 1. / 4.;
+    const int nx = s->nx, nxpp = s->nxpp;
+    memcpy(uf, u, sizeof(double) * (size_t)s->ntot);
+#pragma omp parallel for schedule(static)
+    for (long r = 0; r < (long)s->ny * s->nz; r++) {
+        const double *row = u + r * nxpp;
+        double *o = uf + r * nxpp;
+        for (int i = 0; i < nx; i++) {
+            double um = row[i == 0 ? nx - 1 : i - 1], up = row[i == nx - 1 ? 0 : i + 1];
+            o[i] = a * um + b * row[i] + c * up;
+        }
+    }
+}
+
+static void filtery3pv(const iwo_state *s, const double *u, double *uf)
+{
+    const double a = 1. / 4., b = 2. / 4., c = 1. / 4.;
+    const int nx = s->nx, ny = s->ny, nxpp = s->nxpp;
+    memcpy(uf, u, sizeof(double) * (size_t)s->ntot);
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < s->nz; k++)
+        for (int j = 0; j < ny; j++) {
+            const double *rm = u + ((long)k * ny + (j == 0 ? ny - 1 : j - 1)) * nxpp;
+            const double *r0 = u + ((long)k * ny + j) * nxpp;
+            const double *rp = u + ((long)k * ny + (j == ny - 1 ? 0 : j + 1)) * nxpp;
+            double *o = uf + ((long)k * ny + j) * nxpp;
+            for (int i = 0; i < nx; i++) o[i] = a * rm[i] + b * r0[i] + c * rp[i];
+        }
+}
+
+/* filter_legensgsz (filtering.f:989-1105): Legendre transform of every
+ * column i <= nx of each subdomain, Boyd-Vandeven filter of order 2 with
+ * cut-off kc = int(fackcbv*nzm), back-transform */
+static void filter_legensgsz(const iwo_state *s, double *uin)
+{
+    const int nl = s->nzloc, nzm = s->nzm, nx = s->nx, nxpp = s->nxpp;
+    const int kc = (int)(s->P.fackcbv * nzm);
+    const long nxppy = s->nxppy;
+    double sig[128];
+    for (int k = 0; k <= nzm; k++) sig[k] = specfilter_bv(k, kc, nzm, 2.);
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j < s->ny; j++) {
+        double fin[128], a[128];
+        for (int i = 0; i < nx; i++)
+            for (int ks = 0; ks < s->nsubd; ks++) {
+                const long o = (long)ks * nl * nxppy + (long)j * nxpp + i;
+                for (int kl = 0; kl < nl; kl++) fin[kl] = uin[o + kl * nxppy];
+                for (int k = 0; k <= nzm; k++) {
+                    double sum = 0.;
+                    for (int k1 = 0; k1 < nl; k1++) sum = sum + s->wg[k1] * fin[k1] * s->alpol[k * nl + k1];
+                    a[k] = (k == nzm) ? 0.5 * nzm * sum : (k + 0.5) * sum;
+                }
+                for (int k = 0; k <= nzm; k++) a[k] = a[k] * sig[k];
+                for (int k1 = 0; k1 < nl; k1++) {
+                    double sum = 0.;
+                    for (int k = 0; k <= nzm; k++) sum = sum + a[k] * s->alpol[k * nl + k1];
+                    uin[o + k1 * nxppy] = sum;
+                }
+            }
+    }
+}
+
+/* modecut (filtering.f:100-156): u0 = 3 f(u) - 3 f(f(u)) + f(f(f(u))) with
+ * f = y-filter after x-filter, for i <= nx, then the vertical SGS filter */
+static void modecut(iwo_state *s, const double *u, double *u0)
+{
+    const long n = s->ntot;
+    double *tmp2 = dalloc(n), *g3 = dalloc(n), *g3s = dalloc(n), *g3c = dalloc(n);
+    filterx3pv(s, u, tmp2);
+    filtery3pv(s, tmp2, g3);
+    filterx3pv(s, g3, tmp2);
+    filtery3pv(s, tmp2, g3s);
+    filterx3pv(s, g3s, tmp2);
+    filtery3pv(s, tmp2, g3c);
+    for (long r = 0; r < (long)s->ny * s->nz; r++)
+        for (int i = 0; i < s->nx; i++) {
+            long o = r * s->nxpp + i;
+            u0[o] = 3. * g3[o] - 3. * g3s[o] + g3c[o];
+        }
+    free(tmp2);
+    free(g3);
+    free(g3s);
+    free(g3c);
+    filter_legensgsz(s, u0);
+}
+
+/* mean of temp per level through its horfft round trip (sgscalc.f:21-28,
+ * 129-136); the round trip itself rescales temp's kx-Nyquist content */
+static void temp_mean(iwo_state *s, double *tm)
+{
+    iwo_horfft(s, T(s), -1, s->nz);
+    for (int k = 0; k < s->nz; k++) tm[k] = T(s)[(long)k * s->nxppy];
+    iwo_horfft(s, T(s), 1, s->nz);
+}
+
+static void sgscalc_pre(iwo_state *s)
+{ /* sgscalc.f:4-57 */
+    double **t = s->sgs, tm[4096];
+    const long nxppy = s->nxppy;
+    temp_mean(s, tm);
+    for (int j = 0; j < s->ny; j++)
+        for (int i = 0; i < s->nx; i++) {
+            long o1 = (long)j * s->nxpp + i;
+            for (int q = 0; q < 9; q++) t[q][o1] = 0.;
+            for (int k = 1; k < s->nz; k++) {
+                long o = (long)k * nxppy + o1;
+                double u = U(s)[o], v = V(s)[o], w = W(s)[o];
+                t[0][o] = u * u;
+                t[3][o] = v * v;
+                t[5][o] = w * w;
+                t[1][o] = u * v;
+                t[2][o] = u * w;
+                t[4][o] = v * w;
+                s->ox[o] = T(s)[o] - tm[k];
+                t[6][o] = u * s->ox[o];
+                t[7][o] = v * s->ox[o];
+                t[8][o] = w * s->ox[o];
+            }
+        }
+}
+
+static void sgscalc_post(iwo_state *s)
+{ /* sgscalc.f:62-223 */
+    double **t = s->sgs, **tp = s->tmps, tm[4096];
+    const long nxppy = s->nxppy;
+    const int nz = s->nz;
+    modecut(s, t[0], tp[0]); /* t11 */
+    modecut(s, t[3], tp[1]); /* t22 */
+    modecut(s, t[5], tp[2]); /* t33 */
+    modecut(s, t[1], tp[3]); /* t12 */
+    modecut(s, t[2], tp[4]); /* t13 */
+    modecut(s, t[4], tp[5]); /* t23 */
+    for (int k = 1; k < nz; k++)
+        for (int j = 0; j < s->ny; j++)
+            for (int i = 0; i < s->nx; i++) {
+                long o = (long)k * nxppy + (long)j * s->nxpp + i;
+                double u = U(s)[o], v = V(s)[o], w = W(s)[o];
+                t[0][o] = tp[0][o] - u * u;
+                t[3][o] = tp[1][o] - v * v;
+                t[1][o] = tp[3][o] - u * v;
+                if (k == nz - 1) {
+                    t[5][o] = 0.;
+                    t[2][o] = 0.;
+                    t[4][o] = 0.;
+                } else {
+                    t[5][o] = tp[2][o] - w * w;
+                    t[2][o] = tp[4][o] - u * w;
+                    t[4][o] = tp[5][o] - v * w;
+                }
+            }
+    temp_mean(s, tm);
+    modecut(s, t[6], tp[0]);
+    modecut(s, t[7], tp[1]);
+    modecut(s, t[8], tp[2]);
+    for (int k = 1; k < nz; k++)
+        for (int j = 0; j < s->ny; j++)
+            for (int i = 0; i < s->nx; i++) {
+                long o = (long)k * nxppy + (long)j * s->nxpp + i;
+                s->ox[o] = T(s)[o] - tm[k];
+                if (k == nz - 1) {
+                    t[6][o] = 0.;
+                    t[7][o] = 0.;
+                    t[8][o] = 0.;
+                } else {
+                    t[6][o] = tp[0][o] - U(s)[o] * s->ox[o];
+                    t[7][o] = tp[1][o] - V(s)[o] * s->ox[o];
+                    t[8][o] = tp[2][o] - W(s)[o] * s->ox[o];
+                }
+            }
+}
+
+void iwo_filtering(iwo_state *s, int isgscalc)
+{ /* filtering.f:8-93 */
+    double **tp = s->tmps;
+    for (int q = 0; q < 4; q++) iwo_horfft(s, s->F[q], 1, s->nz);
+    if (isgscalc) sgscalc_pre(s);
+    modecut(s, U(s), tp[0]);
+    modecut(s, V(s), tp[1]);
+    modecut(s, W(s), tp[2]);
+    modecut(s, T(s), tp[3]); /* computed, not used (filtering.f:48, 67) */
+    const int nz = s->nz;
+    for (int k = 1; k < nz; k++)
+        for (int j = 0; j < s->ny; j++)
+            for (int i = 0; i < s->nx; i++) {
+                long o = (long)k * s->nxppy + (long)j * s->nxpp + i;
+                U(s)[o] = tp[0][o];
+                V(s)[o] = tp[1][o];
+                if (k != nz - 1) W(s)[o] = tp[2][o];
+            }
+    if (isgscalc) sgscalc_post(s);
+    for (int q = 0; q < 4; q++) iwo_horfft(s, s->F[q], -1, s->nz);
+}
+
+double *iwo_sgs(iwo_state *s, int idx) { return (idx >= 0 && idx < 9) ? s->sgs[idx] : 0; }
+
 void iwo_run(iwo_state *s, int nsteps, int *counts)
 {
     for (int n = 0; n < nsteps; n++) {
@@ -1369,6 +1602,8 @@ void iwo_run(iwo_state *s, int nsteps, int *counts)
         int cpen = (s->P.ispecfilterexp == 1) && (istep % s->P.nspecfilterexp == 0);
         calcuvwt(s, cpen);
         s->bdfv = (istep < 2) ? 1.5 : 11. / 6.; /* main.f:815-856 */
+        if (s->P.ifilter == 1 && s->P.nstepfilter > 0 && istep % s->P.nstepfilter == 0) /* main.f:880-895 */
+            iwo_filtering(s, s->P.npostp > 0 && istep % s->P.npostp == 0);
         s->t = s->t + s->dt;
         int ic[4] = {0, 0, 0, 0};
         if (s->P.navgintrfc > 0 && istep % s->P.navgintrfc == 0) /* main.f:920-932 */

@@ -24,7 +24,7 @@ typedef struct iwo_params {
     int nx, ny, nzloc, nsubd;
     int bc_uv_bottom;   /* 0 Neumann (calcuvwt.f:291-295), 1 Dirichlet (calcuvwt_noslip.f:294-299) */
     int ispecfilternl, ispecfilterexp, nspecfilterexp, navgintrfc;
-    int pad0;
+    int ifilter, nstepfilter, npostp; /* estimation filter, main.f:880-895 */
     double xlen, ylen, zlen;
     double xnu, xkappa, brunt, grav, rho0, omz;
     double dt;
@@ -32,6 +32,7 @@ typedef struct iwo_params {
     double facdir, facneumn, facrobin;
     double zcen, xbcrit;
     double xacrit, umax, fwave;
+    double fackcbv;
     double z0[IWO_MAXSUB], zh[IWO_MAXSUB];
 } iwo_params;
 
@@ -64,6 +65,12 @@ void iwo_set_initial(iwo_state *s, const double *u, const double *v,
 /* advance nsteps; counts (4 per step, may be NULL) receive interface counts */
 void iwo_run(iwo_state *s, int nsteps, int *counts);
 int  iwo_nthreads(void);
+/* estimation-model filter of u,v,w (filtering.f:8-93) on the spectral
+   state; isgscalc also computes the SGS tensors (sgscalc.f:4-223) */
+void iwo_filtering(iwo_state *s, int isgscalc);
+/* SGS tensors in physical space: 0 t11, 1 t12, 2 t13, 3 t22, 4 t23,
+   5 t33, 6 th1, 7 th2, 8 th3 (comsgs.h) */
+double *iwo_sgs(iwo_state *s, int idx);
 
 double iwo_abvar1(double, double, double);
 double iwo_abvar2(double, double, double);

@@ -42,6 +42,9 @@ def lib():
         L.iwo_penalty_solve.argtypes = [P, D, ctypes.c_int, ctypes.c_int, D, D, D, D, P, P, P, P]
         L.iwo_set_initial.argtypes = [P, P, P, P, P]
         L.iwo_run.argtypes = [P, ctypes.c_int, P]
+        L.iwo_filtering.argtypes = [P, ctypes.c_int]
+        L.iwo_sgs.restype = ctypes.POINTER(D)
+        L.iwo_sgs.argtypes = [P, ctypes.c_int]
         for n in ("abvar1", "abvar2", "abvar3", "bdfvarv", "bdfvar1", "bdfvar2", "bdfvar3"):
             f = getattr(L, "iwo_" + n)
             f.restype = D
@@ -119,6 +122,19 @@ class Oracle:
         arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (u, v, w, t)]
         self.L.iwo_set_initial(self.h, *[_ptr(a) for a in arrs])
 
+    def set_field(self, idx, a):
+        self.field(idx)[...] = np.asarray(a, dtype=np.float64).reshape(self.shape)
+
+    def filtering(self, isgscalc=0):
+        """Estimation-model filter of the spectral u,v,w (filtering.f:8-93)."""
+        self.L.iwo_filtering(self.h, isgscalc)
+
+    def sgs(self, idx):
+        """SGS tensor idx (0 t11, 1 t12, 2 t13, 3 t22, 4 t23, 5 t33, 6-8 th1-3)."""
+        n = int(np.prod(self.shape))
+        p = self.L.iwo_sgs(self.h, idx)
+        return np.ctypeslib.as_array(p, shape=(n,)).reshape(self.shape).copy()
+
     def run(self, nsteps):
         counts = np.zeros(4 * nsteps, dtype=np.int32)
         self.L.iwo_run(self.h, nsteps, counts.ctypes.data_as(ctypes.c_void_p))
@@ -144,14 +160,14 @@ def _input_dat(P, nsteps):
  $mflags
     irelax = 0, ispecfilternl = {P.ispecfilternl}, ispecfilterexp = {P.ispecfilterexp},
     nspecfilterexp = {P.nspecfilterexp}, amp = {P.amp!r}, p = {P.p!r}, pf = {P.pf!r},
-    fackc = {P.fackc!r}, fackcbv = 0.0, facdir = {P.facdir!r}, facneumn = {P.facneumn!r},
+    fackc = {P.fackc!r}, fackcbv = {P.fackcbv!r}, facdir = {P.facdir!r}, facneumn = {P.facneumn!r},
     facrobin = {P.facrobin!r}, navgintrfc = {P.navgintrfc}, ncalclegspec = 100000000,
-    ifilter = 0, nstepfilter = 5, tstart = 0., npostp = 100000000, itempreset = 0,
+    ifilter = {P.ifilter}, nstepfilter = {P.nstepfilter}, tstart = 0., npostp = {P.npostp}, itempreset = 0,
     nrestartdump = 100000000, n3doutput = 100000000, i0 = 0 $
 """
 
 
-def run_ref(build, P, fields, nsteps=1, mode=0, probe=None, exe="iwref"):
+def run_ref(build, P, fields, nsteps=1, mode=0, probe=None, exe="iwref", isgs=0):
     """Run the reference build on physical ``fields`` (u,v,w,rho').
 
     ``exe="iwshim"`` runs the same harness linked with the product's Fortran
@@ -169,8 +185,8 @@ def run_ref(build, P, fields, nsteps=1, mode=0, probe=None, exe="iwref"):
         pr = probe or (1, 1, 1)
         with open(os.path.join(d, "harness.nml"), "w") as f:
             f.write("&harness icfile='ic.bin', outfile='out.bin', diagfile='diag.txt', "
-                    "nrun=%d, mode=%d, iprobe=%d, ipx=%d, kp1=%d, kp2=%d /\n"
-                    % (nsteps, mode, 1 if probe else 0, pr[0], pr[1], pr[2]))
+                    "nrun=%d, mode=%d, iprobe=%d, ipx=%d, kp1=%d, kp2=%d, isgs=%d /\n"
+                    % (nsteps, mode, 1 if probe else 0, pr[0], pr[1], pr[2], isgs))
         np.concatenate([np.ascontiguousarray(a, dtype=np.float64).ravel() for a in fields]).tofile(
             os.path.join(d, "ic.bin"))
         r = subprocess.run([exe], cwd=d, stdout=subprocess.DEVNULL, stderr=subprocess.PIPE)
@@ -180,6 +196,8 @@ def run_ref(build, P, fields, nsteps=1, mode=0, probe=None, exe="iwref"):
         n = nz * ny * nxpp
         state = raw[:24 * n].reshape(24, nz, ny, nxpp)
         t = raw[24 * n]
+        if mode == 5:  # filtering: the nine SGS tensors follow (comsgs.h order)
+            state = np.concatenate([state, raw[24 * n + 1:33 * n + 1].reshape(9, nz, ny, nxpp)])
         with open(os.path.join(d, "diag.txt")) as f:
             diag = f.read().split("\n")
     return state, t, diag

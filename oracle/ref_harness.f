@@ -23,6 +23,8 @@ c    mode=1  horfft forward of the IC fields, dump spectral fields
 c    mode=2  dpdz of the IC fields (taken as given), dump
 c    mode=3  filter_3dlegenfourier of the IC fields (spectral), dump
 c    mode=4  interface_avg3d of the IC fields (spectral), dump + counts
+c    mode=5  filtering(isgs) of the IC fields (spectral), dump + the
+c            nine SGS tensors of /comsgs/
 c=======================================================================
       program iwref
       include 'dim.h'
@@ -41,6 +43,7 @@ c=======================================================================
       include 'comgridptxyz.h'
       include 'comcritlayer.h'
       include 'comscratch.h'
+      include 'comsgs.h'
 
       logical cpenspecfilter
       character*256 icfile, outfile, diagfile
@@ -62,9 +65,10 @@ c=======================================================================
      >ifilter,nstepfilter,
      >tstart,npostp,itempreset,nrestartdump,n3doutput,i0
       namelist/harness/ icfile, outfile, diagfile, nrun, mode, iprobe,
-     >ipx, kp1, kp2
+     >ipx, kp1, kp2, isgs
 
       mode = 0
+      isgs = 0
       iprobe = 0
       ipx = 1
       kp1 = 1
@@ -150,6 +154,10 @@ c-Initial condition: 4 physical fields (nxpp,ny,nz), pads ignored
         call filter_3dlegenfourier(temp,1,1,0)
         goto 900
       endif
+      if (mode.eq.5) then
+        call filtering(isgs)
+        goto 900
+      endif
       if (mode.eq.4) then
         call interface_avg3d(u,icu)
         call interface_avg3d(v,icv)
@@ -222,6 +230,16 @@ c-main.f:815-856
           bdfv = 11./6.
         endif
 
+c-main.f:880-895
+        if ((ifilter.eq.1).and.(mod(istep,nstepfilter).eq.0)) then
+          if (mod(istep,npostp).eq.0) then
+            isgscalc = 1
+          else
+            isgscalc = 0
+          endif
+          call filtering(isgscalc)
+        endif
+
         t = t + dt
 
         if (mod(istep,navgintrfc).eq.0) then
@@ -245,6 +263,7 @@ c-main.f:815-856
       write(11) sun,svn,swn,stn
       write(11) sunm,svnm,swnm,stnm
       write(11) t
+      if (mode.eq.5) write(11) t11,t12,t13,t22,t23,t33,th1,th2,th3
       close(11)
       end
 

@@ -259,3 +259,45 @@ def test_routines_match_compiled_reference(oracle_mod, name):
         f, ic = o.interface_avg(spec[q])
         assert ic == ref_counts[q] and ic > 0
         assert rel(f, st[q]) < 1e-13
+
+
+@pytest.mark.parametrize("name,fackcbv", [("small", 0.0), ("small", 0.5), ("c1", 0.0)])
+def test_estimation_filter_matches_compiled_reference(oracle_mod, name, fackcbv):
+    """filtering(isgscalc) (filtering.f:8-93): modecut, the x/y top-hat
+    filters, the Boyd-Vandeven Legendre filter and, with isgscalc = 1, the SGS
+    tensors of sgscalc.f, against the reference routine (harness mode 5)."""
+    if not oracle_mod.ref_available(name):
+        pytest.skip("reference build %s not present (needs /root/reference)" % name)
+    P, flds = case(name)
+    P.fackcbv = fackcbv
+    _, spec = routine_inputs(oracle_mod, P, flds)
+    for isgs in (0, 1):
+        st, _, _ = oracle_mod.run_ref(name, P, spec, nsteps=0, mode=5, isgs=isgs)
+        o = oracle_mod.Oracle(P)
+        for q in range(4):
+            o.set_field(q, spec[q])
+        o.filtering(isgs)
+        for q in range(4):
+            assert rel(o.field(q), st[q]) < 1e-13, (isgs, q)
+        assert rel(o.field(0), spec[0]) > 1e-3  # the filter acts
+        if isgs:
+            for q in range(9):  # interior columns; the pads are never written
+                assert rel(o.sgs(q)[:, :, :P.nx], st[24 + q][:, :, :P.nx]) < 1e-13, q
+
+
+def test_step_loop_with_estimation_filter_matches_compiled_reference(oracle_mod):
+    """main.f:880-895: the filter every nstepfilter steps, the SGS tensors
+    every npostp steps, inside the replayed step loop."""
+    if not oracle_mod.ref_available("small"):
+        pytest.skip("reference build small not present (needs /root/reference)")
+    P, flds = case("small")
+    P.ifilter, P.nstepfilter, P.npostp = 1, 2, 4
+    o = oracle_mod.Oracle(P)
+    o.set_initial(*flds)
+    counts = o.run(6)
+    st, t, diag = oracle_mod.run_ref("small", P, flds, nsteps=6)
+    mine = o.state()
+    for q in SOLUTION:
+        assert rel(mine[q], st[q]) < (1e-11 if q < 12 else 1e-9), STATE[q]
+    ref_counts = np.array([[int(v) for v in l.split()[1:]] for l in diag if l.strip()])
+    assert np.array_equal(counts, ref_counts)
