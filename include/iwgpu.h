@@ -99,6 +99,14 @@ int iw_convec(iw_solver *s, int istep, int ispecfilternl);
 /* calcuvwt.f:2  subroutine calcuvwt(izmode,ispecfilter,irelax,cpenspecfilter) */
 int iw_calcuvwt(iw_solver *s, int cpenspecfilter);
 
+/* filtering.f:8  subroutine filtering(isgscalc): the estimation-model
+ * filter of u,v,w (modecut with the Boyd-Vandeven vertical filter) on the
+ * device mirror; isgscalc = 1 also forms the SGS tensors (sgscalc.f:4-223) */
+int iw_filtering(iw_solver *s, int isgscalc);
+/* /comsgs/ (comsgs.h) tensor idx in physical space: 0 t11, 1 t12, 2 t13,
+ * 3 t22, 4 t23, 5 t33, 6 th1, 7 th2, 8 th3 */
+int iw_get_sgs(iw_solver *s, int idx, double *host);
+
 /* multistep.f:7-89 variable-step AB3/BDF3 coefficient functions */
 double iw_abvar1(double dt1, double dt2, double dt3);
 double iw_abvar2(double dt1, double dt2, double dt3);

@@ -754,6 +754,29 @@ extern "C" int iw_create(const iw_params *p, iw_solver **out)
             double theta = (ak - akmax * 0.) / (akmax - akmax * 0.);
             fcf[(size_t)j * g.nxhp + ii] = exp(-P.amp * pow(theta, 2. * P.pf));
         }
+    // Boyd-Vandeven filter of order 2 with cut-off int(fackcbv*nzm)
+    // (filtering.f:1040, 1085, 1272-1310); erfc is the C library's, as the
+    // Fortran intrinsic the reference resolves to
+    std::vector<double> sigbv(P.nzloc);
+    {
+        const int kcb = (int)(P.fackcbv * nzm);
+        for (int k = 0; k <= nzm; k++) {
+            double v = 1.;
+            if (k > kcb && k < nzm) {
+                const double theta = (double)(k - kcb) / (double)(nzm - kcb);
+                const double omega = fabs(theta) - 0.5;
+                double chi = 1.;
+                if (theta != 0.5) {
+                    const double c1 = 4. * pow(omega, 2.);
+                    chi = sqrt(-log(1. - c1) / c1);
+                }
+                v = 0.5 * erfc(2. * sqrt(2.) * chi * omega);
+            } else if (k == nzm) {
+                v = 0.;
+            }
+            sigbv[k] = v;
+        }
+    }
     std::vector<double> damp(g.nz);
     for (int k = 0; k < g.nz; k++) {
         double q = (s->z[k] - P.zcen) / P.xlen;
@@ -771,6 +794,7 @@ extern "C" int iw_create(const iw_params *p, iw_solver **out)
         || upload(&g.wg, s->wg, acc) || upload(&g.sig, sig, acc) || upload(&g.xw, s->xw, acc)
         || upload(&g.yw, s->yw, acc) || upload(&g.zh, zh, acc) || upload(&g.z, s->z, acc)
         || upload(&g.damp, damp, acc) || upload(&g.fcf, fcf, acc) || upload(&g.x, xg, acc)
+        || upload(&g.sigbv, sigbv, acc)
         || upload(&g.twM, twM, acc) || upload(&g.twN, twN, acc) || upload(&g.twY, twY, acc)) {
         delete s;
         return -1;
@@ -827,7 +851,9 @@ extern "C" int iw_destroy(iw_solver *s)
     if (!s) return 0;
     hipStreamSynchronize(s->stream);
     Geo &g = s->g;
-    double *tabs[] = {g.D, g.D2, g.AL, g.wg, g.sig, g.xw, g.yw, g.zh, g.z, g.damp, g.fcf, g.x};
+    double *tabs[] = {g.D, g.D2, g.AL, g.wg, g.sig, g.xw, g.yw, g.zh, g.z, g.damp, g.fcf, g.x, g.sigbv};
+    for (int q = 0; q < 9; q++) hipFree(s->sgs[q]);
+    hipFree(s->tmean);
     for (double *t : tabs) hipFree(t);
     hipFree(g.twM);
     hipFree(g.twN);
@@ -1242,6 +1268,77 @@ extern "C" int iw_set_initial(iw_solver *s, const double *u, const double *v, co
     return 0;
 }
 
+// filtering(isgscalc) (filtering.f:8-93) on the device state: the
+// estimation model applied to u,v,w in physical space, and with isgscalc the
+// SGS tensors of sgscalc.f:4-223
+static int rho_mean(iw_solver *s)
+{ // sgscalc.f:21-28: rho's level means through its horfft round trip
+    batch_fft(s, {s->buf[3]}, -1);
+    launch_plane_mean(s->g, s->buf[3], s->tmean, s->stream);
+    batch_fft(s, {s->buf[3]}, 1);
+    return 0;
+}
+
+static int filtering_dev(iw_solver *s, int isgscalc)
+{
+    const Geo &g = s->g;
+    if (!solve_np_supported(s->P.nzloc, s->P.nsubd))
+        return fail("estimation filter: GLL order must be one of 8, 12, 16, 20, 24");
+    double **B = s->buf;
+    double *uvwt[4] = {B[0], B[1], B[2], B[3]};
+    double *u0 = s->grad[0];
+    if (isgscalc && !s->sgs[0]) {
+        const size_t fb = sizeof(double) * (size_t)g.ntot;
+        for (int q = 0; q < 9; q++) {
+            HIPCHK(hipMalloc((void **)&s->sgs[q], fb));
+            HIPCHK(hipMemset(s->sgs[q], 0, fb));
+        }
+        HIPCHK(hipMalloc((void **)&s->tmean, sizeof(double) * g.nz));
+        s->dev_bytes += 9 * (long long)fb + sizeof(double) * g.nz;
+    }
+    Scope sc(s, "estimation_filter");
+    batch_fft(s, {B[0], B[1], B[2], B[3]}, 1);
+    if (isgscalc) {
+        rho_mean(s);
+        launch_sgs_pre(g, uvwt, s->tmean, s->sgs, s->ox, s->stream);
+    }
+    // modecut of u, v, w (filtering.f:43-47) written on levels 2..nz
+    // (w: 2..nz-1) of the 1-based reference index (filtering.f:59-77)
+    launch_modecut(g, B[0], u0, B[0], 1, g.nz - 1, s->stream);
+    launch_modecut(g, B[1], u0, B[1], 1, g.nz - 1, s->stream);
+    launch_modecut(g, B[2], u0, B[2], 1, g.nz - 2, s->stream);
+    if (isgscalc) {
+        double **tp = s->grad + 1;
+        const int srcs[6] = {0, 3, 5, 1, 2, 4}; // t11 t22 t33 t12 t13 t23 (sgscalc.f:71-82)
+        for (int q = 0; q < 6; q++) launch_modecut(g, s->sgs[srcs[q]], u0, tp[q], 0, g.nz - 1, s->stream);
+        launch_sgs_post(g, uvwt, s->tmean, s->sgs, tp, s->ox, 0, s->stream);
+        rho_mean(s);
+        for (int q = 0; q < 3; q++) launch_modecut(g, s->sgs[6 + q], u0, tp[q], 0, g.nz - 1, s->stream);
+        launch_sgs_post(g, uvwt, s->tmean, s->sgs, tp, s->ox, 1, s->stream);
+    }
+    batch_fft(s, {B[0], B[1], B[2], B[3]}, -1);
+    return hipGetLastError() == hipSuccess ? 0 : fail("filtering launch failed");
+}
+
+extern "C" int iw_filtering(iw_solver *s, int isgscalc)
+{
+    if (!s) return fail("null solver");
+    if (filtering_dev(s, isgscalc)) return -1;
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return 0;
+}
+
+extern "C" int iw_get_sgs(iw_solver *s, int idx, double *host)
+{
+    if (!s || !host) return fail("iw_get_sgs: null argument");
+    if (idx < 0 || idx > 8) return fail("iw_get_sgs: index must lie in [0, 8]");
+    if (!s->sgs[idx]) return fail("iw_get_sgs: no SGS tensors computed yet (filtering with isgscalc = 1)");
+    HIPCHK(hipMemcpyAsync(host, s->sgs[idx], sizeof(double) * (size_t)s->g.ntot, hipMemcpyDeviceToHost,
+                          s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return 0;
+}
+
 extern "C" int iw_run(iw_solver *s, int nsteps, int *counts)
 {
     if (!s || nsteps < 0) return fail("iw_run: bad argument");
@@ -1261,6 +1358,8 @@ extern "C" int iw_run(iw_solver *s, int nsteps, int *counts)
         int cpen = (s->P.ispecfilterexp == 1) && (istep % s->P.nspecfilterexp == 0);
         if (calcuvwt_dev(s, cpen)) return -1;
         s->bdfv = (istep < 2) ? 1.5 : 11. / 6.; // main.f:815-856
+        if (s->P.ifilter == 1 && s->P.nstepfilter > 0 && istep % s->P.nstepfilter == 0) // main.f:880-895
+            if (filtering_dev(s, s->P.npostp > 0 && istep % s->P.npostp == 0)) return -1;
         s->t = s->t + s->P.dt;
         bool avg = s->P.navgintrfc > 0 && istep % s->P.navgintrfc == 0;
         if (avg) {

@@ -30,6 +30,7 @@ struct Geo {
     double scaley;
     // device tables
     double *D, *D2, *AL, *wg, *sig, *xw, *yw, *zh, *z, *damp, *fcf, *x;
+    double *sigbv; // Boyd-Vandeven filter of the estimation model (filtering.f:1272-1310)
     double2 *twM;   // e^{-2 pi i t/M}, t < M/2
     double2 *twN;   // e^{-2 pi i k/nx}, k <= M
     double2 *twY;   // e^{-2 pi i t/ny}, t < ny/2
@@ -84,6 +85,8 @@ struct iw_solver {
     double *wnl, *wviscrot;
     double *hbuf;      // device scratch for host-array entry points
     double *phys[4];      // physical u,v,w,rho during convec
+    double *sgs[9] = {};  // SGS tensors t11 t12 t13 t22 t23 t33 th1 th2 th3 (allocated on first use)
+    double *tmean = nullptr; // mean of rho per level
     double *icomp;        // 4 planes per interface x 4 fields (interface averaging)
     int *iface_src, *iface_mid_src, *iface_mid_dst; // plane maps
     int *d_count;      // 4 interface counters
@@ -136,6 +139,13 @@ bool solve_np_supported(int nl, int nsubd);
 void launch_solve_nat(const Geo &g, const iw_params &P, const SysTables &S, double *const *uvwt, double dt,
                       hipStream_t st);
 void launch_lfilter(const Geo &g, double *const *uvwt, int cpen, hipStream_t st);
+// estimation-model filter: modecut of f into out on levels kmin..kmax (u0 scratch)
+void launch_modecut(const Geo &g, const double *f, double *u0, double *out, int kmin, int kmax, hipStream_t st);
+void launch_plane_mean(const Geo &g, const double *tspec, double *tm, hipStream_t st);
+void launch_sgs_pre(const Geo &g, double *const *uvwt, const double *tm, double *const *ten, double *ox,
+                    hipStream_t st);
+void launch_sgs_post(const Geo &g, double *const *uvwt, const double *tm, double *const *ten,
+                     double *const *tmp, double *ox, int part, hipStream_t st);
 void launch_solve_grp(const Geo &g, const iw_params &P, const SysTables &S, double *const *uvwt, double dt,
                       hipStream_t st);
 // part 0: right-hand side, 1: solve, 2: filter + projection

@@ -2174,6 +2174,209 @@ bool solve_np_supported(int nl, int nsubd)
 }
 
 // ---------------------------------------------------------------------------
+// Estimation-model filter (filtering.f:8-270, 989-1105) and SGS tensors
+// (sgscalc.f:4-223), physical space.
+//   k_sgs_xy:  u0 = 3 f(u) - 3 f(f(u)) + f(f(f(u))), f = periodic 3-point
+//              top-hat in y after x (modecut, filtering.f:100-156;
+//              filterx3pv/filtery3pv, 162-270), one LDS tile of TX x TY
+//              points plus a halo of 3 per plane;
+//   k_sgs_z:   Boyd-Vandeven Legendre filter of every column i < nx of a
+//              subdomain (filter_legensgsz, filtering.f:989-1105), written to
+//              the target on levels kmin..kmax;
+//   k_sgs_pre / k_sgs_post: products and tensors of sgscalc.f.
+// ---------------------------------------------------------------------------
+constexpr int kSX = 32, kSY = 8, kSH = 3;
+
+__global__ __launch_bounds__(256) void k_sgs_xy(const double *__restrict__ f, double *__restrict__ u0, int nx,
+                                               int ny, int nxpp)
+{
+    constexpr int WX = kSX + 2 * kSH, WY = kSY + 2 * kSH;
+    __shared__ double A[WY][WX], B[WY][WX], G1[WY][WX], G2[WY][WX];
+    const int k = blockIdx.z;
+    const int i0 = blockIdx.x * kSX - kSH, j0 = blockIdx.y * kSY - kSH;
+    const double *plane = f + (long)k * nxpp * ny;
+    const double a = 1. / 4., b = 2. / 4., c = 1. / 4.;
+    for (int e = threadIdx.x; e < WX * WY; e += blockDim.x) {
+        const int y = e / WX, x = e - y * WX;
+        const int i = ((i0 + x) % nx + nx) % nx, j = ((j0 + y) % ny + ny) % ny;
+        A[y][x] = plane[(long)j * nxpp + i];
+    }
+    __syncthreads();
+    // three passes: x filter into B, then y filter into the next level
+    double(*src)[WX] = A;
+    double(*dst[3])[WX] = {G1, G2, A};
+    for (int pass = 0; pass < 3; pass++) {
+        const int lo = pass + 1, hix = WX - 1 - pass, hiy = WY - 1 - pass;
+        for (int e = threadIdx.x; e < WX * WY; e += blockDim.x) {
+            const int y = e / WX, x = e - y * WX;
+            if (x >= lo && x < hix) B[y][x] = a * src[y][x - 1] + b * src[y][x] + c * src[y][x + 1];
+        }
+        __syncthreads();
+        double(*d)[WX] = dst[pass];
+        // the third level overwrites A, whose values are no longer needed
+        for (int e = threadIdx.x; e < WX * WY; e += blockDim.x) {
+            const int y = e / WX, x = e - y * WX;
+            if (x >= lo && x < hix && y >= lo && y < hiy)
+                d[y][x] = a * B[y - 1][x] + b * B[y][x] + c * B[y + 1][x];
+        }
+        __syncthreads();
+        src = d;
+    }
+    double *out = u0 + (long)k * nxpp * ny;
+    for (int e = threadIdx.x; e < kSX * kSY; e += blockDim.x) {
+        const int y = e / kSX, x = e - y * kSX;
+        const int i = blockIdx.x * kSX + x, j = blockIdx.y * kSY + y;
+        if (i < nx && j < ny) {
+            const int X = x + kSH, Y = y + kSH;
+            out[(long)j * nxpp + i] = 3. * G1[Y][X] - 3. * G2[Y][X] + A[Y][X];
+        }
+    }
+}
+
+template <int NL>
+__global__ __launch_bounds__(256) void k_sgs_z(const double *__restrict__ u0, double *__restrict__ out, int nx,
+                                              int ny, int nxpp, int kmin, int kmax,
+                                              const double *__restrict__ wg, const double *__restrict__ AL,
+                                              const double *__restrict__ sig)
+{
+    __shared__ double ALs[NL * NL], wgs[NL], sgs[NL];
+    stage_tables<NL>(wgs, ALs, sgs, wg, AL, sig);
+    const long col = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int ks = blockIdx.y;
+    const long nxppy = (long)nxpp * ny;
+    if (col >= nxppy) return;
+    const int i = (int)(col % nxpp);
+    if (i >= nx) return;
+    const long b0 = (long)ks * NL * nxppy + col;
+    double x[NL];
+#pragma unroll
+    for (int t = 0; t < NL; t++) x[t] = u0[b0 + t * nxppy];
+    legfilter<NL>(x, wgs, ALs, sgs);
+#pragma unroll
+    for (int t = 0; t < NL; t++) {
+        const int kg = ks * NL + t;
+        if (kg >= kmin && kg <= kmax) out[b0 + t * nxppy] = x[t];
+    }
+}
+
+// mean of rho per level: its kx = ky = 0 coefficient after the forward transform
+__global__ void k_plane_mean(const double *__restrict__ tspec, double *__restrict__ tm, int nz, long nxppy)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < nz) tm[k] = tspec[(long)k * nxppy];
+}
+
+struct SgsArgs {
+    const double *u, *v, *w, *t, *tm;
+    double *ten[9]; // t11 t12 t13 t22 t23 t33 th1 th2 th3
+    const double *tmp[6];
+    double *ox;
+    long ntot, nxppy;
+    int nxpp, nx, nz;
+};
+
+// sgscalc.f:29-56: products on levels k >= 1 (0-based), zero on level 0
+__global__ void k_sgs_pre(SgsArgs A)
+{
+    const long o = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= A.ntot) return;
+    const int i = (int)(o % A.nxpp);
+    if (i >= A.nx) return;
+    const int k = (int)(o / A.nxppy);
+    if (k == 0) {
+        for (int q = 0; q < 9; q++) A.ten[q][o] = 0.;
+        return;
+    }
+    const double u = A.u[o], v = A.v[o], w = A.w[o];
+    const double ox = A.t[o] - A.tm[k];
+    A.ox[o] = ox;
+    A.ten[0][o] = u * u;
+    A.ten[3][o] = v * v;
+    A.ten[5][o] = w * w;
+    A.ten[1][o] = u * v;
+    A.ten[2][o] = u * w;
+    A.ten[4][o] = v * w;
+    A.ten[6][o] = u * ox;
+    A.ten[7][o] = v * ox;
+    A.ten[8][o] = w * ox;
+}
+
+// sgscalc.f:84-126 (part 0: stresses from tmp 0..5 = filtered t11 t22 t33
+// t12 t13 t23) and 137-165 (part 1: fluxes from tmp 0..2 = filtered th1..3)
+__global__ void k_sgs_post(SgsArgs A, int part)
+{
+    const long o = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= A.ntot) return;
+    const int i = (int)(o % A.nxpp);
+    if (i >= A.nx) return;
+    const int k = (int)(o / A.nxppy);
+    if (k == 0) return;
+    const double u = A.u[o], v = A.v[o], w = A.w[o];
+    const bool top = (k == A.nz - 1);
+    if (part == 0) {
+        A.ten[0][o] = A.tmp[0][o] - u * u;
+        A.ten[3][o] = A.tmp[1][o] - v * v;
+        A.ten[1][o] = A.tmp[3][o] - u * v;
+        A.ten[5][o] = top ? 0. : A.tmp[2][o] - w * w;
+        A.ten[2][o] = top ? 0. : A.tmp[4][o] - u * w;
+        A.ten[4][o] = top ? 0. : A.tmp[5][o] - v * w;
+    } else {
+        const double ox = A.t[o] - A.tm[k];
+        A.ox[o] = ox;
+        A.ten[6][o] = top ? 0. : A.tmp[0][o] - u * ox;
+        A.ten[7][o] = top ? 0. : A.tmp[1][o] - v * ox;
+        A.ten[8][o] = top ? 0. : A.tmp[2][o] - w * ox;
+    }
+}
+
+void launch_modecut(const Geo &g, const double *f, double *u0, double *out, int kmin, int kmax, hipStream_t st)
+{
+    dim3 gx((g.nx + kSX - 1) / kSX, (g.ny + kSY - 1) / kSY, g.nz);
+    hipLaunchKernelGGL(k_sgs_xy, gx, dim3(256), 0, st, f, u0, g.nx, g.ny, g.nxpp);
+    dim3 gz((unsigned)((g.nxppy + 255) / 256), g.nsubd);
+#define SZ(NL)                                                                                         \
+    hipLaunchKernelGGL(k_sgs_z<NL>, gz, dim3(256), 0, st, u0, out, g.nx, g.ny, g.nxpp, kmin, kmax, g.wg, \
+                       g.AL, g.sigbv)
+    switch (g.nzloc) {
+    case 9: SZ(9); break;
+    case 13: SZ(13); break;
+    case 17: SZ(17); break;
+    case 21: SZ(21); break;
+    case 25: SZ(25); break;
+    default: break;
+    }
+#undef SZ
+}
+
+void launch_plane_mean(const Geo &g, const double *tspec, double *tm, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_plane_mean, dim3((g.nz + 255) / 256), dim3(256), 0, st, tspec, tm, g.nz, g.nxppy);
+}
+
+void launch_sgs_pre(const Geo &g, double *const *uvwt, const double *tm, double *const *ten, double *ox,
+                    hipStream_t st)
+{
+    SgsArgs A = {};
+    A.u = uvwt[0]; A.v = uvwt[1]; A.w = uvwt[2]; A.t = uvwt[3]; A.tm = tm;
+    for (int q = 0; q < 9; q++) A.ten[q] = ten[q];
+    A.ox = ox;
+    A.ntot = g.ntot; A.nxppy = g.nxppy; A.nxpp = g.nxpp; A.nx = g.nx; A.nz = g.nz;
+    hipLaunchKernelGGL(k_sgs_pre, dim3((unsigned)((g.ntot + 255) / 256)), dim3(256), 0, st, A);
+}
+
+void launch_sgs_post(const Geo &g, double *const *uvwt, const double *tm, double *const *ten,
+                     double *const *tmp, double *ox, int part, hipStream_t st)
+{
+    SgsArgs A = {};
+    A.u = uvwt[0]; A.v = uvwt[1]; A.w = uvwt[2]; A.t = uvwt[3]; A.tm = tm;
+    for (int q = 0; q < 9; q++) A.ten[q] = ten[q];
+    for (int q = 0; q < 6; q++) A.tmp[q] = tmp[q];
+    A.ox = ox;
+    A.ntot = g.ntot; A.nxppy = g.nxppy; A.nxpp = g.nxpp; A.nx = g.nx; A.nz = g.nz;
+    hipLaunchKernelGGL(k_sgs_post, dim3((unsigned)((g.ntot + 255) / 256)), dim3(256), 0, st, A, part);
+}
+
+// ---------------------------------------------------------------------------
 // Spectral effect of a horfft inverse+forward round trip (horfft.f:14-64):
 // the inverse x transform keeps only Re of the kx=0 and kx-Nyquist entries of
 // every row (fft.f:1218-1230) and the y transform skips the Nyquist column

@@ -121,6 +121,17 @@ c     layout of iw_params (include/iwgpu.h)
           type(c_ptr), value :: h
           integer(c_int), value :: icpen
         end function
+        integer(c_int) function iw_filtering(h, isgs) bind(c)
+          import
+          type(c_ptr), value :: h
+          integer(c_int), value :: isgs
+        end function
+        integer(c_int) function iw_get_sgs(h, idx, a) bind(c)
+          import
+          type(c_ptr), value :: h
+          integer(c_int), value :: idx
+          real(c_double) :: a(*)
+        end function
         real(c_double) function iw_abvar1(a, b, c) bind(c)
           import
           real(c_double), value :: a, b, c
@@ -415,6 +426,32 @@ c-----------------------------------------------------------------------
       if (cpenspecfilter) icpen = 1
       call iwshim_check(iw_calcuvwt(iwh, icpen), 'calcuvwt')
       call iwshim_get_state
+      end
+
+c-----------------------------------------------------------------------
+c     filtering.f:8   subroutine filtering(isgscalc)
+c     estimation-model filter of /flow/ u,v,w; with isgscalc = 1 the SGS
+c     tensors of /comsgs/ (comsgs.h, sgscalc.f:4-223) as well
+c-----------------------------------------------------------------------
+      subroutine filtering(isgscalc)
+      use iwshim
+      include 'dim.h'
+      include 'comsgs.h'
+      call iwshim_init
+      call iwshim_put_state
+      call iwshim_check(iw_filtering(iwh, isgscalc), 'filtering')
+      call iwshim_get_state
+      if (isgscalc .eq. 1) then
+        call iwshim_check(iw_get_sgs(iwh, 0, t11), 'filtering')
+        call iwshim_check(iw_get_sgs(iwh, 1, t12), 'filtering')
+        call iwshim_check(iw_get_sgs(iwh, 2, t13), 'filtering')
+        call iwshim_check(iw_get_sgs(iwh, 3, t22), 'filtering')
+        call iwshim_check(iw_get_sgs(iwh, 4, t23), 'filtering')
+        call iwshim_check(iw_get_sgs(iwh, 5, t33), 'filtering')
+        call iwshim_check(iw_get_sgs(iwh, 6, th1), 'filtering')
+        call iwshim_check(iw_get_sgs(iwh, 7, th2), 'filtering')
+        call iwshim_check(iw_get_sgs(iwh, 8, th3), 'filtering')
+      endif
       end
 
 c-----------------------------------------------------------------------

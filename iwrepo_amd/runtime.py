@@ -18,7 +18,7 @@ EXPORTS = [
     "iw_params_size", "iw_last_error", "iw_create", "iw_destroy", "iw_set_state", "iw_get_state",
     "iw_get_pressbc", "iw_set_pressbc", "iw_set_dt", "iw_set_coeffs", "iw_get_coeffs", "iw_set_initial", "iw_run", "iw_time",
     "iw_sync", "iw_horfft", "iw_dpdz", "iw_filter_3dlegenfourier", "iw_interface_avg3d",
-    "iw_convec", "iw_calcuvwt", "iw_abvar1", "iw_abvar2", "iw_abvar3", "iw_bdfvarv",
+    "iw_convec", "iw_calcuvwt", "iw_filtering", "iw_get_sgs", "iw_abvar1", "iw_abvar2", "iw_abvar3", "iw_bdfvarv",
     "iw_bdfvar1", "iw_bdfvar2", "iw_bdfvar3", "iw_set_profiling", "iw_kernel_stats",
     "iw_device_bytes",
 ]
@@ -60,6 +60,8 @@ def lib():
     L.iw_interface_avg3d.argtypes = [P, P, ctypes.POINTER(I)]
     L.iw_convec.argtypes = [P, I, I]
     L.iw_calcuvwt.argtypes = [P, I]
+    L.iw_filtering.argtypes = [P, I]
+    L.iw_get_sgs.argtypes = [P, I, P]
     for n in ("abvar1", "abvar2", "abvar3", "bdfvarv", "bdfvar1", "bdfvar2", "bdfvar3"):
         f = getattr(L, "iw_" + n)
         f.restype = D
@@ -179,6 +181,14 @@ class Solver:
 
     def calcuvwt(self, cpen=1):
         _check(self.L.iw_calcuvwt(self.h, cpen))
+
+    def filtering(self, isgscalc=0):
+        _check(self.L.iw_filtering(self.h, isgscalc))
+
+    def sgs(self, idx):
+        out = np.empty(self.shape)
+        _check(self.L.iw_get_sgs(self.h, idx, _ptr(out)))
+        return out
 
     def set_profiling(self, on=True):
         _check(self.L.iw_set_profiling(self.h, 1 if on else 0))

@@ -204,3 +204,55 @@ def test_fortran_shim_routines_against_reference(oracle_mod, name):
             assert rel(ss[q], sr[q]) < tol[mode], (mode, q)
         if mode == 4:
             assert dr[0].split() == ds[0].split()
+
+
+@pytest.mark.parametrize("name,fackcbv", [("small", 0.0), ("small", 0.5), ("c1", 0.0)])
+def test_estimation_filter_matches_oracle(rt, oracle_mod, name, fackcbv):
+    """filtering(isgscalc) on the device (x/y top-hat tiles, Boyd-Vandeven
+    Legendre filter, SGS tensors) against the oracle on the same spectral
+    fields."""
+    P, flds = case(name)
+    P.fackcbv = fackcbv
+    _, spec = routine_inputs(oracle_mod, P, flds)
+    for isgs in (0, 1):
+        print("estimation filter", name, fackcbv, isgs, flush=True)
+        o, g = oracle_mod.Oracle(P), rt.Solver(P)
+        for q in range(4):
+            o.set_field(q, spec[q])
+            g.set_state(q, spec[q])
+        o.filtering(isgs)
+        g.filtering(isgs)
+        for q in range(4):
+            assert rel(g.get_state(q), o.field(q)) < 1e-13, (isgs, q)
+        if isgs:
+            for q in range(9):
+                assert rel(g.sgs(q)[:, :, :P.nx], o.sgs(q)[:, :, :P.nx]) < 1e-13, q
+        g.close()
+
+
+def test_step_loop_with_estimation_filter_matches_oracle(rt, oracle_mod):
+    """The filter every 2 steps and the SGS tensors every 4 inside iw_run."""
+    P, flds = case("small")
+    P.ifilter, P.nstepfilter, P.npostp = 1, 2, 4
+    co, cg, so, sg, t = _run_pair(rt, oracle_mod, P, flds, 6)
+    for q in SOLUTION:
+        assert rel(sg[q], so[q]) < (FIELD_TOL if q < 12 else HIST_TOL), STATE[q]
+    assert np.array_equal(co, cg)
+
+
+def test_fortran_shim_filtering_against_reference(oracle_mod):
+    """filtering reached through the Fortran shim after the reference set-up
+    (harness mode 5), fields and the /comsgs/ tensors."""
+    for exe in ("iwref", "iwshim"):
+        if not os.path.exists(os.path.join(oracle_mod.REFDIR, "small", exe)):
+            pytest.skip("small/%s not built (built where the reference sources are)" % exe)
+    P, flds = case("small")
+    _, spec = routine_inputs(oracle_mod, P, flds)
+    for isgs in (0, 1):
+        sr, _, _ = oracle_mod.run_ref("small", P, spec, nsteps=0, mode=5, isgs=isgs)
+        ss, _, _ = oracle_mod.run_ref("small", P, spec, nsteps=0, mode=5, isgs=isgs, exe="iwshim")
+        for q in range(4):
+            assert rel(ss[q], sr[q]) < 1e-13, (isgs, q)
+        if isgs:
+            for q in range(24, 33):
+                assert rel(ss[q][:, :, :P.nx], sr[q][:, :, :P.nx]) < 1e-13, q
