@@ -107,6 +107,20 @@ int iw_filtering(iw_solver *s, int isgscalc);
  * 3 t22, 4 t23, 5 t33, 6 th1, 7 th2, 8 th3 */
 int iw_get_sgs(iw_solver *s, int idx, double *host);
 
+/* ---- restart files in the record layout of output.f:27-104 and
+ * restart.f:26-115 (Fortran sequential unformatted, REAL*8, INTEGER*4) ---- */
+/* Device state to a file the reference can restart from (main.f:986-990). */
+int iw_save_restart(iw_solver *s, const char *path);
+/* Restart from a reference start.dat (main.f:563-634): the 20 saved fields,
+ * t and umax, then the variable-step first-step coefficients with cdtin =
+ * cdt; the next two steps of iw_run use the restart branches of
+ * main.f:703-734 and 818-825. */
+int iw_load_restart(iw_solver *s, const char *path, double cdt);
+/* The same file format on host arrays, no device needed: state holds the 24
+ * arrays in iw_set_state order (su..st are not stored and read as zero). */
+int iw_write_restart_file(const char *path, const iw_params *p, double t, const double *state);
+int iw_read_restart_file(const char *path, const iw_params *p, double *t, double *umax, double *state);
+
 /* multistep.f:7-89 variable-step AB3/BDF3 coefficient functions */
 double iw_abvar1(double dt1, double dt2, double dt3);
 double iw_abvar2(double dt1, double dt2, double dt3);

@@ -1265,6 +1265,221 @@ extern "C" int iw_set_initial(iw_solver *s, const double *u, const double *v, co
     s->bdf[0] = 1.; s->bdf[1] = 0.; s->bdf[2] = 0.;
     s->bdfv = 1.;
     s->istep = 0;
+    s->istart = 0;
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Restart files in the record layout of output.f:27-104 / restart.f:26-115:
+// Fortran sequential unformatted records (4-byte length, payload, 4-byte
+// length), REAL = 8 bytes, INTEGER = 4 bytes.  29 records of set-up data
+// (which restart.f reads and main.f:591 then recomputes with setup), the
+// time level, then 20 fields: u,v,w,temp, the n-1 and n-2 levels and the NL
+// terms at n-1 and n-2.
+// ---------------------------------------------------------------------------
+static const int kRestartMeta = 29;
+
+struct RecFile {
+    FILE *f = nullptr;
+    bool ok = true;
+    ~RecFile() { if (f) fclose(f); }
+    void put(const void *p, size_t n)
+    {
+        const int m = (int)n;
+        ok = ok && fwrite(&m, 4, 1, f) == 1 && (n == 0 || fwrite(p, 1, n, f) == n) && fwrite(&m, 4, 1, f) == 1;
+    }
+    void putd(const std::vector<double> &v) { put(v.data(), sizeof(double) * v.size()); }
+    // one record into out (its length must be known in advance unless out is null: skipped)
+    long get(std::vector<char> &buf)
+    {
+        int m = 0, m2 = 0;
+        if (fread(&m, 4, 1, f) != 1 || m < 0) return -1;
+        buf.resize((size_t)m);
+        if (m && fread(buf.data(), 1, (size_t)m, f) != (size_t)m) return -1;
+        if (fread(&m2, 4, 1, f) != 1 || m2 != m) return -1;
+        return m;
+    }
+};
+
+// host copy of the set-up tables of a parameter block (no device needed)
+static int host_tables(const iw_params *p, iw_solver &h)
+{
+    if (p->nzloc < 3 || p->nzloc > kMaxNl - 1 || p->nx < 4 || p->ny < 1 || p->nsubd < 1 || p->nsubd > IW_MAXSUB)
+        return fail("restart file: parameter block out of range");
+    h.P = *p;
+    setup_tables(&h);
+    return 0;
+}
+
+static int write_restart(const char *path, const iw_params *p, iw_solver &h, double t, double umax,
+                         const double *const *fields)
+{
+    const int nx = p->nx, ny = p->ny, nl = p->nzloc, ns = p->nsubd, nz = nl * ns, nzm = nl - 1;
+    const int nxpp = nx + 2, nxhp = nx / 2 + 1;
+    const long nxppy = (long)nxpp * ny, ntot = nxppy * nz;
+    const int nz2 = ns * (nl - 1) + 1, nsdtype = 1;
+    const double pi2 = 8. * atan(1.0);
+    RecFile r;
+    r.f = fopen(path, "wb");
+    if (!r.f) return fail(std::string("cannot write restart file ") + path);
+    r.put(&t, 8);
+    const double prra[2] = {p->xnu, 0.};
+    r.put(prra, 16);
+    const int dims[9] = {nx, ny, nz, nzm, (int)nxppy, (int)ntot, ns, nl, nsdtype};
+    r.put(dims, sizeof(dims));
+    const double ab[2] = {pi2 / p->xlen, pi2 / p->ylen};
+    r.put(ab, 16);
+    // (xw(i),xsq(i),i=1,nx): entries past nxhp lie outside xw in the
+    // reference's /wave/ block and are recomputed by setup; written as zero
+    std::vector<double> v(2 * (size_t)nx, 0.);
+    for (int i = 0; i < nxhp && i < nx; i++) { v[2 * i] = h.xw[i]; v[2 * i + 1] = h.xsq[i]; }
+    r.putd(v);
+    v.assign(2 * (size_t)ny, 0.);
+    for (int j = 0; j < ny; j++) { v[2 * j] = h.yw[j]; v[2 * j + 1] = h.ysq[j]; }
+    r.putd(v);
+    v.assign(nx, 0.);
+    for (int i = 0; i < nx; i++) v[i] = i * p->xlen / nx;
+    r.putd(v);
+    v.assign(ny, 0.);
+    for (int j = 0; j < ny; j++) v[j] = j * p->ylen / ny;
+    r.putd(v);
+    r.putd(h.z);
+    v.assign(nz2, 0.);
+    for (int ks = 0; ks < ns; ks++)
+        for (int k = 0; k < nl; k++) v[ks * (nl - 1) + k] = h.z[ks * nl + k];
+    r.putd(v);
+    std::vector<char> sub((size_t)ns * 20);
+    for (int ks = 0; ks < ns; ks++) {
+        const int it = 1;
+        memcpy(&sub[ks * 20], &p->z0[ks], 8);
+        memcpy(&sub[ks * 20 + 8], &p->zh[ks], 8);
+        memcpy(&sub[ks * 20 + 16], &it, 4);
+    }
+    r.put(sub.data(), sub.size());
+    v.assign(nsdtype, p->zlen / ns);
+    r.putd(v);
+    std::vector<double> al((size_t)nl * nl);
+    for (int k = 0; k < nl; k++)
+        for (int j = 0; j < nl; j++) al[(size_t)k * nl + j] = h.alpol[(size_t)j * nl + k]; // alpol(j,k), j fastest
+    r.putd(al);
+    r.putd(std::vector<double>((size_t)nl * nl, 0.)); // dalpol, recomputed by setup
+    r.putd(h.zpts);
+    r.putd(h.wg);
+    std::vector<double> dd((size_t)nl * nl);
+    for (int j = 0; j < nl; j++)
+        for (int i = 0; i < nl; i++) dd[(size_t)j * nl + i] = h.d[(size_t)i * nl + j]; // d(i,j), i fastest
+    r.putd(dd);
+    for (int j = 0; j < nl; j++)
+        for (int i = 0; i < nl; i++) dd[(size_t)j * nl + i] = h.d2[(size_t)i * nl + j];
+    r.putd(dd);
+    r.putd(std::vector<double>((size_t)nl * nl, 0.));          // d3
+    r.putd(std::vector<double>(3 * (size_t)nz, 0.));           // atpz, btpz, ctpz
+    r.putd(std::vector<double>(2 * (size_t)nxppy, 0.));        // tubc, tbbc (homogeneous walls)
+    r.putd(std::vector<double>(12 * (size_t)nxppy, 0.));       // uw, vw, ww
+    r.put(&umax, 8);
+    const double solks = 0.;
+    r.put(&solks, 8);
+    for (int q = 0; q < 2; q++) r.putd(std::vector<double>(2 * (size_t)nxpp * nz, 0.)); // mean-wave fields
+    r.putd(std::vector<double>(2 * (size_t)nxpp * nz, 0.));
+    r.putd(std::vector<double>((size_t)nxpp * nz, 0.));
+    r.putd(std::vector<double>(2 * (size_t)nxpp * nz, 0.));
+    for (int q = 0; q < 20; q++) r.put(fields[q], sizeof(double) * (size_t)ntot);
+    if (!r.ok) return fail(std::string("write error on restart file ") + path);
+    return 0;
+}
+
+// state index of restart field q (the file holds u..tnm, then sun..stnm)
+static int restart_state_index(int q) { return q < 12 ? q : q + 4; }
+
+static int read_restart(const char *path, const iw_params *p, double *t, double *umax, double *const *fields)
+{
+    const int nx = p->nx, ny = p->ny, nl = p->nzloc, nz = nl * p->nsubd;
+    const long ntot = (long)(nx + 2) * ny * nz;
+    RecFile r;
+    r.f = fopen(path, "rb");
+    if (!r.f) return fail(std::string("cannot open restart file ") + path);
+    std::vector<char> b;
+    for (int q = 0; q < kRestartMeta; q++) {
+        const long n = r.get(b);
+        if (n < 0) return fail(std::string("restart file ") + path + ": truncated set-up records");
+        if (q == 0 && n == 8) memcpy(t, b.data(), 8);
+        if (q == 2) {
+            int d[9];
+            if (n != 36) return fail("restart file: bad dimension record");
+            memcpy(d, b.data(), 36);
+            if (d[0] != nx || d[1] != ny || d[2] != nz || d[7] != nl)
+                return fail("restart file: dimensions differ from the parameter block");
+        }
+        if (q == 22 && n == 8) memcpy(umax, b.data(), 8);
+    }
+    for (int q = 0; q < 20; q++) {
+        const long n = r.get(b);
+        if (n != (long)sizeof(double) * ntot) return fail(std::string("restart file ") + path + ": bad field record");
+        memcpy(fields[q], b.data(), (size_t)n);
+    }
+    return 0;
+}
+
+extern "C" int iw_write_restart_file(const char *path, const iw_params *p, double t, const double *state)
+{
+    if (!path || !p || !state) return fail("iw_write_restart_file: null argument");
+    iw_solver h;
+    if (host_tables(p, h)) return -1;
+    const long ntot = (long)(p->nx + 2) * p->ny * p->nzloc * p->nsubd;
+    const double *f[20];
+    for (int q = 0; q < 20; q++) f[q] = state + (size_t)restart_state_index(q) * ntot;
+    return write_restart(path, p, h, t, p->umax, f);
+}
+
+extern "C" int iw_read_restart_file(const char *path, const iw_params *p, double *t, double *umax, double *state)
+{
+    if (!path || !p || !t || !umax || !state) return fail("iw_read_restart_file: null argument");
+    const long ntot = (long)(p->nx + 2) * p->ny * p->nzloc * p->nsubd;
+    double *f[20];
+    for (int q = 0; q < 20; q++) f[q] = state + (size_t)restart_state_index(q) * ntot;
+    for (int q = 12; q < 16; q++) memset(state + (size_t)q * ntot, 0, sizeof(double) * ntot);
+    return read_restart(path, p, t, umax, f);
+}
+
+extern "C" int iw_save_restart(iw_solver *s, const char *path)
+{
+    if (!s || !path) return fail("iw_save_restart: null argument");
+    const size_t fb = sizeof(double) * (size_t)s->g.ntot;
+    std::vector<double> host(20 * (size_t)s->g.ntot);
+    const double *f[20];
+    for (int q = 0; q < 20; q++) {
+        HIPCHK(hipMemcpyAsync(host.data() + (size_t)q * s->g.ntot, s->buf[restart_state_index(q)], fb,
+                              hipMemcpyDeviceToHost, s->stream));
+        f[q] = host.data() + (size_t)q * s->g.ntot;
+    }
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return write_restart(path, &s->P, *s, s->t, s->P.umax, f);
+}
+
+extern "C" int iw_load_restart(iw_solver *s, const char *path, double cdt)
+{
+    if (!s || !path) return fail("iw_load_restart: null argument");
+    if (!(cdt > 0.)) return fail("iw_load_restart: cdt must be positive");
+    const size_t fb = sizeof(double) * (size_t)s->g.ntot;
+    std::vector<double> host(20 * (size_t)s->g.ntot);
+    double *f[20];
+    for (int q = 0; q < 20; q++) f[q] = host.data() + (size_t)q * s->g.ntot;
+    double t = 0., umax = s->P.umax;
+    if (read_restart(path, &s->P, &t, &umax, f)) return -1;
+    for (int q = 0; q < 24; q++) HIPCHK(hipMemsetAsync(s->buf[q], 0, fb, s->stream));
+    for (int q = 0; q < 20; q++)
+        HIPCHK(hipMemcpyAsync(s->buf[restart_state_index(q)], f[q], fb, hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    // restart.f:69 overwrites umax; main.f:619-634 first-step coefficients
+    s->P.umax = umax;
+    s->t = t;
+    s->cdt = cdt;
+    const double dt = s->P.dt, d2 = cdt * dt;
+    s->fac[0] = iw_abvar1(dt, d2, d2); s->fac[1] = iw_abvar2(dt, d2, d2); s->fac[2] = iw_abvar3(dt, d2, d2);
+    s->bdf[0] = iw_bdfvar1(dt, d2, d2); s->bdf[1] = iw_bdfvar2(dt, d2, d2); s->bdf[2] = iw_bdfvar3(dt, d2, d2);
+    s->bdfv = iw_bdfvarv(dt, d2, d2);
+    s->istep = 0;
+    s->istart = 1;
     return 0;
 }
 
@@ -1348,16 +1563,24 @@ extern "C" int iw_run(iw_solver *s, int nsteps, int *counts)
         // spectrum only through the kx=0 and kx-Nyquist columns
         project_dev(s, s->buf, 4);
         if (convec_dev(s, s->P.ispecfilternl)) return -1;
-        if (istep < 2) { // main.f:700-777
+        if (istep < 2 && s->istart == 0) { // main.f:700-777
             s->fac[0] = 2.; s->fac[1] = -1.; s->fac[2] = 0.;
             s->bdf[0] = 2.; s->bdf[1] = -0.5; s->bdf[2] = 0.;
+        } else if (istep < 2) { // restart: variable-step AB3/BDF3 (main.f:719-734)
+            const double dt = s->P.dt, d3 = s->cdt * dt;
+            s->fac[0] = iw_abvar1(dt, dt, d3); s->fac[1] = iw_abvar2(dt, dt, d3); s->fac[2] = iw_abvar3(dt, dt, d3);
+            s->bdf[0] = iw_bdfvar1(dt, dt, d3); s->bdf[1] = iw_bdfvar2(dt, dt, d3);
+            s->bdf[2] = iw_bdfvar3(dt, dt, d3);
         } else {
             s->fac[0] = 3.; s->fac[1] = -3.; s->fac[2] = 1.;
             s->bdf[0] = 3.; s->bdf[1] = -1.5; s->bdf[2] = 1. / 3.;
         }
         int cpen = (s->P.ispecfilterexp == 1) && (istep % s->P.nspecfilterexp == 0);
         if (calcuvwt_dev(s, cpen)) return -1;
-        s->bdfv = (istep < 2) ? 1.5 : 11. / 6.; // main.f:815-856
+        if (istep >= 2) // main.f:815-856
+            s->bdfv = 11. / 6.;
+        else
+            s->bdfv = (s->istart == 0) ? 1.5 : iw_bdfvarv(s->P.dt, s->P.dt, s->cdt * s->P.dt);
         if (s->P.ifilter == 1 && s->P.nstepfilter > 0 && istep % s->P.nstepfilter == 0) // main.f:880-895
             if (filtering_dev(s, s->P.npostp > 0 && istep % s->P.npostp == 0)) return -1;
         s->t = s->t + s->P.dt;

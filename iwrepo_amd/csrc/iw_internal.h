@@ -94,6 +94,8 @@ struct iw_solver {
     // time stepping
     double t, fac[3], bdf[3], bdfv;
     int istep;
+    int istart = 0;   // 1 after iw_load_restart: first-step coefficients of a restart
+    double cdt = 1.;  // cdtin of the restart (main.f:619-634)
     double factored_bdfv = -1, factored_dt = -1;
     bool p_factored = false;
     iw::SysTables sys;

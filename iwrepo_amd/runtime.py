@@ -18,7 +18,8 @@ EXPORTS = [
     "iw_params_size", "iw_last_error", "iw_create", "iw_destroy", "iw_set_state", "iw_get_state",
     "iw_get_pressbc", "iw_set_pressbc", "iw_set_dt", "iw_set_coeffs", "iw_get_coeffs", "iw_set_initial", "iw_run", "iw_time",
     "iw_sync", "iw_horfft", "iw_dpdz", "iw_filter_3dlegenfourier", "iw_interface_avg3d",
-    "iw_convec", "iw_calcuvwt", "iw_filtering", "iw_get_sgs", "iw_abvar1", "iw_abvar2", "iw_abvar3", "iw_bdfvarv",
+    "iw_convec", "iw_calcuvwt", "iw_filtering", "iw_get_sgs",
+    "iw_save_restart", "iw_load_restart", "iw_write_restart_file", "iw_read_restart_file", "iw_abvar1", "iw_abvar2", "iw_abvar3", "iw_bdfvarv",
     "iw_bdfvar1", "iw_bdfvar2", "iw_bdfvar3", "iw_set_profiling", "iw_kernel_stats",
     "iw_device_bytes",
 ]
@@ -61,6 +62,10 @@ def lib():
     L.iw_convec.argtypes = [P, I, I]
     L.iw_calcuvwt.argtypes = [P, I]
     L.iw_filtering.argtypes = [P, I]
+    L.iw_save_restart.argtypes = [P, ctypes.c_char_p]
+    L.iw_load_restart.argtypes = [P, ctypes.c_char_p, D]
+    L.iw_write_restart_file.argtypes = [ctypes.c_char_p, P, D, P]
+    L.iw_read_restart_file.argtypes = [ctypes.c_char_p, P, P, P, P]
     L.iw_get_sgs.argtypes = [P, I, P]
     for n in ("abvar1", "abvar2", "abvar3", "bdfvarv", "bdfvar1", "bdfvar2", "bdfvar3"):
         f = getattr(L, "iw_" + n)
@@ -182,6 +187,12 @@ class Solver:
     def calcuvwt(self, cpen=1):
         _check(self.L.iw_calcuvwt(self.h, cpen))
 
+    def save_restart(self, path):
+        _check(self.L.iw_save_restart(self.h, os.fsencode(path)))
+
+    def load_restart(self, path, cdt=1.0):
+        _check(self.L.iw_load_restart(self.h, os.fsencode(path), cdt))
+
     def filtering(self, isgscalc=0):
         _check(self.L.iw_filtering(self.h, isgscalc))
 
@@ -208,3 +219,19 @@ class Solver:
 
     def device_bytes(self):
         return int(self.L.iw_device_bytes(self.h))
+
+
+def write_restart_file(path, P, t, state):
+    """Host-only writer of the reference restart layout (state: 24 arrays)."""
+    st = np.ascontiguousarray(state, dtype=np.float64)
+    _check(lib().iw_write_restart_file(os.fsencode(path), ctypes.byref(P), t, _ptr(st)))
+
+
+def read_restart_file(path, P):
+    """Host-only reader: returns (state[24], t, umax)."""
+    n = (P.nx + 2) * P.ny * P.nzloc * P.nsubd
+    st = np.zeros((24, n))
+    t, umax = ctypes.c_double(0.), ctypes.c_double(0.)
+    _check(lib().iw_read_restart_file(os.fsencode(path), ctypes.byref(P), ctypes.byref(t), ctypes.byref(umax),
+                                      _ptr(st)))
+    return st.reshape((24, P.nzloc * P.nsubd, P.ny, P.nx + 2)), t.value, umax.value

@@ -31,7 +31,7 @@ FFLAGS = ["-fdefault-real-8", "-fdefault-double-8", "-fno-automatic",
 # hot-path sources of the reference (SURVEY-listed files), minus main.f/various.f
 SOURCES = ["solver.f", "penalty.f", "band_lu.f", "bandludcmp.f", "ludcmp.f",
            "derv.f", "horfft.f", "fft.f", "filtering.f", "setup.f",
-           "forcing.f", "multistep.f", "erfc.f", "sgscalc.f"]
+           "forcing.f", "multistep.f", "erfc.f", "sgscalc.f", "output.f", "restart.f"]
 
 DIM_TEMPLATE = """\
       parameter   ( nx = {nx}, ny={ny}, nz= {nz}, nzloc = {nzloc}, nsubd = {nsubd})

@@ -36,6 +36,8 @@ struct iwo_state {
     double *ub, *vb, *wb, *phi, *dphidz;
     double fac1, fac2, fac3, bdf1, bdf2, bdf3, bdfv, t, dt, facvel, fact;
     int istep;
+    int istart;  /* 1 after iwo_set_restart (main.f:563-634) */
+    double cdt;
     /* FFT twiddle tables: cos/sin(2*pi*k/n) for n = nx and n = ny */
     double *twx_c, *twx_s, *twy_c, *twy_s;
     /* estimation filter: /comsgs/ t11,t12,t13,t22,t23,t33,th1,th2,th3
@@ -1351,6 +1353,7 @@ void iwo_set_initial(iwo_state *s, const double *u, const double *v, const doubl
     for (int q = 0; q < 4; q++) iwo_horfft(s, s->F[q], -1, s->nz);
     s->t = 0.;
     s->istep = 0;
+    s->istart = 0;
     s->fac1 = 1.0; s->fac2 = 0.; s->fac3 = 0.;
     s->bdf1 = 1.; s->bdf2 = 0.; s->bdf3 = 0.;
     s->bdfv = 1.;
@@ -1582,6 +1585,20 @@ void iwo_filtering(iwo_state *s, int isgscalc)
 
 double *iwo_sgs(iwo_state *s, int idx) { return (idx >= 0 && idx < 9) ? s->sgs[idx] : 0; }
 
+void iwo_set_restart(iwo_state *s, const double *state, double t, double cdt)
+{ /* restart.f:26-115 then main.f:619-634 */
+    for (int q = 0; q < 24; q++)
+        memcpy(s->F[q], state + (size_t)q * s->ntot, sizeof(double) * (size_t)s->ntot);
+    s->t = t;
+    s->cdt = cdt;
+    double d1 = s->dt, d2 = cdt * s->dt;
+    s->fac1 = iwo_abvar1(d1, d2, d2); s->fac2 = iwo_abvar2(d1, d2, d2); s->fac3 = iwo_abvar3(d1, d2, d2);
+    s->bdf1 = iwo_bdfvar1(d1, d2, d2); s->bdf2 = iwo_bdfvar2(d1, d2, d2); s->bdf3 = iwo_bdfvar3(d1, d2, d2);
+    s->bdfv = iwo_bdfvarv(d1, d2, d2);
+    s->istep = 0;
+    s->istart = 1;
+}
+
 void iwo_run(iwo_state *s, int nsteps, int *counts)
 {
     for (int n = 0; n < nsteps; n++) {
@@ -1590,9 +1607,13 @@ void iwo_run(iwo_state *s, int nsteps, int *counts)
         for (int q = 0; q < 4; q++) iwo_horfft(s, s->F[q], 1, s->nz);
         for (int q = 0; q < 4; q++) iwo_horfft(s, s->F[q], -1, s->nz);
         convec(s);
-        if (istep < 2) { /* main.f:700-777 */
+        if (istep < 2 && !s->istart) { /* main.f:700-777 */
             s->fac1 = 2.; s->fac2 = -1.; s->fac3 = 0.;
             s->bdf1 = 2.; s->bdf2 = -0.5; s->bdf3 = 0.;
+        } else if (istep < 2) { /* restart, main.f:719-734 */
+            double d1 = s->dt, d3 = s->cdt * s->dt;
+            s->fac1 = iwo_abvar1(d1, d1, d3); s->fac2 = iwo_abvar2(d1, d1, d3); s->fac3 = iwo_abvar3(d1, d1, d3);
+            s->bdf1 = iwo_bdfvar1(d1, d1, d3); s->bdf2 = iwo_bdfvar2(d1, d1, d3); s->bdf3 = iwo_bdfvar3(d1, d1, d3);
         } else {
             s->fac1 = 3.; s->fac2 = -3.; s->fac3 = 1.;
             s->bdf1 = 3.; s->bdf2 = -1.5; s->bdf3 = 1. / 3.;
@@ -1601,7 +1622,10 @@ void iwo_run(iwo_state *s, int nsteps, int *counts)
             memcpy(s->F[12 + q], s->F[q], sizeof(double) * (size_t)s->ntot);
         int cpen = (s->P.ispecfilterexp == 1) && (istep % s->P.nspecfilterexp == 0);
         calcuvwt(s, cpen);
-        s->bdfv = (istep < 2) ? 1.5 : 11. / 6.; /* main.f:815-856 */
+        if (istep >= 2) /* main.f:815-856 */
+            s->bdfv = 11. / 6.;
+        else
+            s->bdfv = s->istart ? iwo_bdfvarv(s->dt, s->dt, s->cdt * s->dt) : 1.5;
         if (s->P.ifilter == 1 && s->P.nstepfilter > 0 && istep % s->P.nstepfilter == 0) /* main.f:880-895 */
             iwo_filtering(s, s->P.npostp > 0 && istep % s->P.npostp == 0);
         s->t = s->t + s->dt;

@@ -64,6 +64,8 @@ void iwo_set_initial(iwo_state *s, const double *u, const double *v,
                      const double *w, const double *t);
 /* advance nsteps; counts (4 per step, may be NULL) receive interface counts */
 void iwo_run(iwo_state *s, int nsteps, int *counts);
+/* restart from 24 state arrays at time t with cdtin = cdt (main.f:563-634) */
+void iwo_set_restart(iwo_state *s, const double *state, double t, double cdt);
 int  iwo_nthreads(void);
 /* estimation-model filter of u,v,w (filtering.f:8-93) on the spectral
    state; isgscalc also computes the SGS tensors (sgscalc.f:4-223) */

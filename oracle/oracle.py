@@ -43,6 +43,7 @@ def lib():
         L.iwo_set_initial.argtypes = [P, P, P, P, P]
         L.iwo_run.argtypes = [P, ctypes.c_int, P]
         L.iwo_filtering.argtypes = [P, ctypes.c_int]
+        L.iwo_set_restart.argtypes = [P, P, D, D]
         L.iwo_sgs.restype = ctypes.POINTER(D)
         L.iwo_sgs.argtypes = [P, ctypes.c_int]
         for n in ("abvar1", "abvar2", "abvar3", "bdfvarv", "bdfvar1", "bdfvar2", "bdfvar3"):
@@ -125,6 +126,11 @@ class Oracle:
     def set_field(self, idx, a):
         self.field(idx)[...] = np.asarray(a, dtype=np.float64).reshape(self.shape)
 
+    def set_restart(self, state, t, cdt=1.0):
+        st = np.ascontiguousarray(state, dtype=np.float64)
+        assert st.size == 24 * int(np.prod(self.shape))
+        self.L.iwo_set_restart(self.h, _ptr(st), t, cdt)
+
     def filtering(self, isgscalc=0):
         """Estimation-model filter of the spectral u,v,w (filtering.f:8-93)."""
         self.L.iwo_filtering(self.h, isgscalc)
@@ -148,10 +154,10 @@ def ref_available(build):
     return os.path.exists(os.path.join(REFDIR, build, "iwref"))
 
 
-def _input_dat(P, nsteps):
+def _input_dat(P, nsteps, istart=0, cdt=1.0):
     return f""" $inputs
-   istart = 0, dtin = {P.dt!r}, nsteps = {nsteps}, tanim = 1e30, iform = 0,
-   cdtin = 1.0, waveperiods = 1e30, fwave = {P.fwave!r}, dtanim = 1e30 $
+   istart = {istart}, dtin = {P.dt!r}, nsteps = {nsteps}, tanim = 1e30, iform = 0,
+   cdtin = {cdt!r}, waveperiods = 1e30, fwave = {P.fwave!r}, dtanim = 1e30 $
  $params
     xlen = {P.xlen!r}, ylen = {P.ylen!r}, zlen = {P.zlen!r}, brunt = {P.brunt!r},
     xnu = {P.xnu!r}, xkappa = {P.xkappa!r}, grav = {P.grav!r}, expan = 0.0,
@@ -167,17 +173,24 @@ def _input_dat(P, nsteps):
 """
 
 
-def run_ref(build, P, fields, nsteps=1, mode=0, probe=None, exe="iwref", isgs=0):
+def run_ref(build, P, fields, nsteps=1, mode=0, probe=None, exe="iwref", isgs=0, start=None, cdt=1.0,
+            write_restart=False):
     """Run the reference build on physical ``fields`` (u,v,w,rho').
 
     ``exe="iwshim"`` runs the same harness linked with the product's Fortran
-    shims (the reference step loop calling libiwgpu.so).
-    Returns (state[24], t, diag_lines)."""
+    shims (the reference step loop calling libiwgpu.so).  ``start`` (bytes
+    of a restart file) restarts from it (istart=1) instead; with
+    ``write_restart`` the restart file written by output.f at the end is
+    returned as a fourth item.
+    Returns (state[24], t, diag_lines[, restart_bytes])."""
     exe = os.path.join(REFDIR, build, exe)
     nz, ny, nxpp = P.nzloc * P.nsubd, P.ny, P.nx + 2
     with tempfile.TemporaryDirectory() as d:
+        if start is not None:
+            with open(os.path.join(d, "start.dat"), "wb") as f:
+                f.write(start)
         with open(os.path.join(d, "input.dat"), "w") as f:
-            f.write(_input_dat(P, nsteps))
+            f.write(_input_dat(P, nsteps, 1 if start is not None else 0, cdt))
         with open(os.path.join(d, "subd.dat"), "w") as f:
             f.write("%d\n" % P.nsubd)
             for k in range(P.nsubd):
@@ -185,8 +198,8 @@ def run_ref(build, P, fields, nsteps=1, mode=0, probe=None, exe="iwref", isgs=0)
         pr = probe or (1, 1, 1)
         with open(os.path.join(d, "harness.nml"), "w") as f:
             f.write("&harness icfile='ic.bin', outfile='out.bin', diagfile='diag.txt', "
-                    "nrun=%d, mode=%d, iprobe=%d, ipx=%d, kp1=%d, kp2=%d, isgs=%d /\n"
-                    % (nsteps, mode, 1 if probe else 0, pr[0], pr[1], pr[2], isgs))
+                    "nrun=%d, mode=%d, iprobe=%d, ipx=%d, kp1=%d, kp2=%d, isgs=%d, iwrite=%d /\n"
+                    % (nsteps, mode, 1 if probe else 0, pr[0], pr[1], pr[2], isgs, 1 if write_restart else 0))
         np.concatenate([np.ascontiguousarray(a, dtype=np.float64).ravel() for a in fields]).tofile(
             os.path.join(d, "ic.bin"))
         r = subprocess.run([exe], cwd=d, stdout=subprocess.DEVNULL, stderr=subprocess.PIPE)
@@ -200,4 +213,7 @@ def run_ref(build, P, fields, nsteps=1, mode=0, probe=None, exe="iwref", isgs=0)
             state = np.concatenate([state, raw[24 * n + 1:33 * n + 1].reshape(9, nz, ny, nxpp)])
         with open(os.path.join(d, "diag.txt")) as f:
             diag = f.read().split("\n")
+        if write_restart:
+            with open(os.path.join(d, "restart_out.dat"), "rb") as f:
+                return state, t, diag, f.read()
     return state, t, diag

@@ -25,6 +25,9 @@ c    mode=3  filter_3dlegenfourier of the IC fields (spectral), dump
 c    mode=4  interface_avg3d of the IC fields (spectral), dump + counts
 c    mode=5  filtering(isgs) of the IC fields (spectral), dump + the
 c            nine SGS tensors of /comsgs/
+c  istart=1 (namelist &inputs) in mode 0 restarts from start.dat with
+c  restart.f and the first-step coefficients of main.f:619-634, 703-734
+c  and 818-825; iwrite=1 writes restart_out.dat with output.f at the end.
 c=======================================================================
       program iwref
       include 'dim.h'
@@ -65,10 +68,11 @@ c=======================================================================
      >ifilter,nstepfilter,
      >tstart,npostp,itempreset,nrestartdump,n3doutput,i0
       namelist/harness/ icfile, outfile, diagfile, nrun, mode, iprobe,
-     >ipx, kp1, kp2, isgs
+     >ipx, kp1, kp2, isgs, iwrite
 
       mode = 0
       isgs = 0
+      iwrite = 0
       iprobe = 0
       ipx = 1
       kp1 = 1
@@ -117,6 +121,26 @@ c-Derived parameters (main.f:328-352, 416-472)
       ispecfilter = 0
 
       call setup
+
+      if (istart.ne.0) then
+c-main.f:563-634: restart file, set-up again, first-step coefficients
+        open(8,file='start.dat',form='unformatted',status='old')
+        call restart(8,t)
+        close(8)
+        call setup
+        dt1 = dt
+        dt2 = cdt*dt
+        dt3 = cdt*dt
+        fac1 = abvar1(dt1,dt2,dt3)
+        fac2 = abvar2(dt1,dt2,dt3)
+        fac3 = abvar3(dt1,dt2,dt3)
+        bdf1 = bdfvar1(dt1,dt2,dt3)
+        bdf2 = bdfvar2(dt1,dt2,dt3)
+        bdf3 = bdfvar3(dt1,dt2,dt3)
+        bdfv = bdfvarv(dt1,dt2,dt3)
+        open(12,file=diagfile,status='unknown')
+        goto 500
+      endif
 
 c-Initial condition: 4 physical fields (nxpp,ny,nz), pads ignored
       open(10,file=icfile,form='unformatted',access='stream',
@@ -182,6 +206,7 @@ c-mode 0: full run from a physical-space initial condition
       bdf3 = 0.
       bdfv = 1.
 
+ 500  continue
       do 1000 istep=1,nrun
 c-State-changing part of maxcalc (various.f:739-742, 854-857)
         call horfft(u,1,nz)
@@ -197,14 +222,26 @@ c-State-changing part of maxcalc (various.f:739-742, 854-857)
 
         call convec(istep,0,ispecfilternl)
 
-c-main.f:700-777 (istart=0, fixed dt)
+c-main.f:700-777 (fixed dt)
         if (istep.lt.2) then
-          fac1 = 2.
-          fac2 = -1.
-          fac3 = 0.
-          bdf1 = 2.
-          bdf2 = -0.5
-          bdf3 = 0.
+          if (istart.eq.0) then
+            fac1 = 2.
+            fac2 = -1.
+            fac3 = 0.
+            bdf1 = 2.
+            bdf2 = -0.5
+            bdf3 = 0.
+          else
+            dt1 = dt
+            dt2 = dt
+            dt3 = cdt*dt
+            fac1 = abvar1(dt1,dt2,dt3)
+            fac2 = abvar2(dt1,dt2,dt3)
+            fac3 = abvar3(dt1,dt2,dt3)
+            bdf1 = bdfvar1(dt1,dt2,dt3)
+            bdf2 = bdfvar2(dt1,dt2,dt3)
+            bdf3 = bdfvar3(dt1,dt2,dt3)
+          endif
         else
           fac1 = 3.
           fac2 = -3.
@@ -225,7 +262,11 @@ c-main.f:700-777 (istart=0, fixed dt)
 
 c-main.f:815-856
         if (istep.lt.2) then
-          bdfv = 1.5
+          if (istart.eq.0) then
+            bdfv = 1.5
+          else
+            bdfv = bdfvarv(dt,dt,cdt*dt)
+          endif
         else
           bdfv = 11./6.
         endif
@@ -254,6 +295,12 @@ c-main.f:880-895
 
  900  continue
       close(12)
+      if (iwrite.eq.1) then
+        open(3,file='restart_out.dat',form='unformatted',
+     >       status='unknown')
+        call output(3,t)
+        close(3)
+      endif
       open(11,file=outfile,form='unformatted',access='stream',
      >     status='replace')
       write(11) u,v,w,temp

@@ -256,3 +256,50 @@ def test_fortran_shim_filtering_against_reference(oracle_mod):
         if isgs:
             for q in range(24, 33):
                 assert rel(ss[q][:, :, :P.nx], sr[q][:, :, :P.nx]) < 1e-13, q
+
+
+@pytest.mark.parametrize("cdt", [1.0, 2.0])
+def test_device_restart_from_reference_file(rt, oracle_mod, tmp_path, cdt):
+    """iw_load_restart of a start.dat written by the reference (output.f),
+    then 5 steps with the restart coefficients, against the reference's own
+    restart of the same file."""
+    if not oracle_mod.ref_available("small"):
+        pytest.skip("reference build small not present")
+    P, flds = case("small")
+    _, t, _, rs = oracle_mod.run_ref("small", P, flds, nsteps=5, write_restart=True)
+    ref, tr, diag = oracle_mod.run_ref("small", P, flds, nsteps=5, start=rs, cdt=cdt)
+    path = tmp_path / "start.dat"
+    path.write_bytes(rs)
+    g = rt.Solver(P)
+    g.load_restart(str(path), cdt)
+    cg = g.run(5)
+    sg = g.state()
+    g.close()
+    for q in SOLUTION:
+        assert rel(sg[q], ref[q]) < (FIELD_TOL if q < 12 else HIST_TOL), STATE[q]
+    ref_counts = np.array([[int(v) for v in l.split()[1:]] for l in diag if l.strip()])
+    assert np.array_equal(cg, ref_counts)
+
+
+def test_device_restart_file_read_by_reference(rt, oracle_mod, tmp_path):
+    """iw_save_restart after 5 device steps; the reference restarts from that
+    file and agrees with the device restarting from it."""
+    if not oracle_mod.ref_available("small"):
+        pytest.skip("reference build small not present")
+    P, flds = case("small")
+    g = rt.Solver(P)
+    g.set_initial(*flds)
+    g.run(5)
+    path = tmp_path / "dev.dat"
+    g.save_restart(str(path))
+    g.close()
+    ref, tr, _ = oracle_mod.run_ref("small", P, flds, nsteps=5, start=path.read_bytes(), cdt=1.0)
+    h = rt.Solver(P)
+    h.load_restart(str(path), 1.0)
+    h.run(5)
+    sh = h.state()
+    th = h.time()
+    h.close()
+    assert abs(th - tr) < 1e-15
+    for q in SOLUTION:
+        assert rel(sh[q], ref[q]) < (FIELD_TOL if q < 12 else HIST_TOL), STATE[q]

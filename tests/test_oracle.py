@@ -301,3 +301,66 @@ def test_step_loop_with_estimation_filter_matches_compiled_reference(oracle_mod)
         assert rel(mine[q], st[q]) < (1e-11 if q < 12 else 1e-9), STATE[q]
     ref_counts = np.array([[int(v) for v in l.split()[1:]] for l in diag if l.strip()])
     assert np.array_equal(counts, ref_counts)
+
+
+def _ref_restart_pair(oracle_mod, P, flds, n1):
+    """Reference run of n1 steps that writes a restart file with output.f."""
+    st, t, _, rs = oracle_mod.run_ref("small", P, flds, nsteps=n1, write_restart=True)
+    return st, t, rs
+
+
+def test_restart_file_layout_and_round_trip(oracle_mod, tmp_path):
+    """The library's host reader and writer of the output.f / restart.f record
+    layout: reading a restart file written by the reference gives its state
+    bit for bit, and the reference restarting from a file written by the
+    library continues exactly as from its own file."""
+    if not oracle_mod.ref_available("small"):
+        pytest.skip("reference build small not present (needs /root/reference)")
+    from iwrepo_amd import runtime
+    P, flds = case("small")
+    st, t, rs = _ref_restart_pair(oracle_mod, P, flds, 5)
+    path = tmp_path / "start.dat"
+    path.write_bytes(rs)
+    rst, rt, umax = runtime.read_restart_file(str(path), P)
+    assert rt == t and umax == P.umax
+    for q in list(range(12)) + list(range(16, 24)):
+        assert np.array_equal(rst[q], st[q]), STATE[q]
+    mine = tmp_path / "mine.dat"
+    runtime.write_restart_file(str(mine), P, t, st)
+    ours = mine.read_bytes()
+
+    def lengths(b):
+        out, i = [], 0
+        while i < len(b):
+            n = int(np.frombuffer(b[i:i + 4], dtype=np.int32)[0])
+            out.append(n)
+            i += 8 + n
+        return out
+    assert lengths(ours) == lengths(rs)
+    a, ta, _ = oracle_mod.run_ref("small", P, flds, nsteps=4, start=rs, cdt=1.0)
+    b, tb, _ = oracle_mod.run_ref("small", P, flds, nsteps=4, start=ours, cdt=1.0)
+    assert ta == tb
+    for q in SOLUTION:
+        assert np.array_equal(a[q], b[q]), STATE[q]
+
+
+@pytest.mark.parametrize("cdt", [1.0, 2.0])
+def test_oracle_restart_matches_compiled_reference(oracle_mod, cdt):
+    """Restart with the variable-step first-step coefficients (main.f:619-634,
+    703-734, 818-825), including cdtin != 1."""
+    if not oracle_mod.ref_available("small"):
+        pytest.skip("reference build small not present (needs /root/reference)")
+    P, flds = case("small")
+    st, t, rs = _ref_restart_pair(oracle_mod, P, flds, 5)
+    ref, tr, diag = oracle_mod.run_ref("small", P, flds, nsteps=5, start=rs, cdt=cdt)
+    o = oracle_mod.Oracle(P)
+    st0 = st.copy()
+    st0[12:16] = 0.
+    o.set_restart(st0, t, cdt)
+    counts = o.run(5)
+    mine = o.state()
+    for q in SOLUTION:
+        assert rel(mine[q], ref[q]) < (1e-11 if q < 12 else 1e-9), STATE[q]
+    assert abs(o.time() - tr) < 1e-15
+    ref_counts = np.array([[int(v) for v in l.split()[1:]] for l in diag if l.strip()])
+    assert np.array_equal(counts, ref_counts)
