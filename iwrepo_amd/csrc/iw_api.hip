@@ -1144,12 +1144,13 @@ static int convec_dev(iw_solver *s, int ispecfilternl)
     }
     {
         Scope sc(s, "bdf_filter", 4 * 56. * nt);
+        double *S12[12], *F12[12], *out4[4];
         for (int q = 0; q < 4; q++) {
-            double *S3[3] = {B[12 + q], B[16 + q], B[20 + q]};
-            double *F3[3] = {B[q], B[4 + q], B[8 + q]};
-            launch_bdf_filter(g, s->P, S3, F3, B[8 + q], s->fac, s->bdf, s->P.dt, ispecfilternl == 1,
-                              s->stream);
+            S12[3 * q] = B[12 + q]; S12[3 * q + 1] = B[16 + q]; S12[3 * q + 2] = B[20 + q];
+            F12[3 * q] = B[q]; F12[3 * q + 1] = B[4 + q]; F12[3 * q + 2] = B[8 + q];
+            out4[q] = B[8 + q];
         }
+        launch_bdf_filter(g, S12, F12, out4, s->fac, s->bdf, s->P.dt, ispecfilternl == 1, s->stream);
     }
     for (int q = 0; q < 4; q++) { // history rotation (solver.f:447-478) by pointer swap
         double *nw = B[8 + q];

@@ -124,9 +124,10 @@ void launch_wnl(const Geo &g, const double *sw, const double *swn, const double 
                 const double *fac, double *wnl, hipStream_t st);
 void launch_rotvisc(const Geo &g, const iw_params &P, double *const *lev /*9*/, const double *fac,
                     double *wviscrot, hipStream_t st);
-void launch_bdf_filter(const Geo &g, const iw_params &P, double *const *S3, double *const *F3,
-                       double *out, const double *fac, const double *bdf, double dt, int filt,
-                       hipStream_t st);
+// SS extrapolation + BDF assembly + filters of u,v,w,rho in one launch:
+// S12/F12 = (S, Sn, Snm) and (f, fn, fnm) per field, out4 the results
+void launch_bdf_filter(const Geo &g, double *const *S12, double *const *F12, double *const *out4, const double *fac,
+                       const double *bdf, double dt, int filt, hipStream_t st);
 void launch_filter3d(const Geo &g, double *f, int ifourier, int ilegen, hipStream_t st);
 void launch_iface(const Geo &g, FieldBatch b, int *count, hipStream_t st);
 void launch_project(const Geo &g, FieldBatch b, hipStream_t st);

@@ -791,9 +791,9 @@ void launch_rotvisc(const Geo &g, const iw_params &P, double *const *lev, const 
 constexpr int kMB = 32;
 
 struct TileArgs {
-    const double *S[3];  // S, Sn, Snm
-    const double *F[3];  // f, fn, fnm
-    double *out;
+    const double *S[4][3]; // per field (blockIdx.z): S, Sn, Snm
+    const double *F[4][3]; // f, fn, fnm
+    double *out[4];
     double f1, f2, f3, b1, b2, b3, dt;
     int mode, ifourier, ilegen;
 };
@@ -806,7 +806,9 @@ __global__ __launch_bounds__(256) void k_tile(TileArgs A, long ncplx_plane, long
 {
     __shared__ double tile[kMaxNl * 2 * kMB];
     __shared__ double coef[kMaxNl * 2 * kMB];
-    const int ks = blockIdx.y;
+    const int ks = blockIdx.y, fz = blockIdx.z;
+    const double *const *Sf = A.S[fz], *const *Ff = A.F[fz];
+    double *outf = A.out[fz];
     const long c0 = (long)blockIdx.x * kMB;
     const int nmb = (int)((ncplx_plane - c0) < kMB ? (ncplx_plane - c0) : kMB);
     const int W2 = 2 * kMB;
@@ -816,10 +818,10 @@ __global__ __launch_bounds__(256) void k_tile(TileArgs A, long ncplx_plane, long
         if (e < 2 * nmb) {
             long o = (long)(ks * nl + t) * nxppy + 2 * c0 + e;
             if (A.mode == 1) {
-                double s = A.f1 * A.S[0][o] + A.f2 * A.S[1][o] + A.f3 * A.S[2][o];
-                val = A.dt * s + A.b1 * A.F[0][o] + A.b2 * A.F[1][o] + A.b3 * A.F[2][o];
+                double s = A.f1 * Sf[0][o] + A.f2 * Sf[1][o] + A.f3 * Sf[2][o];
+                val = A.dt * s + A.b1 * Ff[0][o] + A.b2 * Ff[1][o] + A.b3 * Ff[2][o];
             } else {
-                val = A.out[o];
+                val = outf[o];
             }
         }
         tile[idx] = val;
@@ -866,7 +868,7 @@ __global__ __launch_bounds__(256) void k_tile(TileArgs A, long ncplx_plane, long
     }
     for (int idx = threadIdx.x; idx < nl * W2; idx += blockDim.x) {
         int t = idx / W2, e = idx - t * W2;
-        if (e < 2 * nmb) A.out[(long)(ks * nl + t) * nxppy + 2 * c0 + e] = tile[idx];
+        if (e < 2 * nmb) outf[(long)(ks * nl + t) * nxppy + 2 * c0 + e] = tile[idx];
     }
 }
 
@@ -881,20 +883,22 @@ __global__ __launch_bounds__(256, 2) void k_tile_t(TileArgs A, long nmode, long 
     __shared__ double ALs[NL * NL], wgs[NL], sgs[NL];
     stage_tables<NL>(wgs, ALs, sgs, wg, AL, sig);
     const long m = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int ks = blockIdx.y;
+    const int ks = blockIdx.y, fz = blockIdx.z;
     if (m >= nmode) return;
+    const double *const *Sf = A.S[fz], *const *Ff = A.F[fz];
+    double *outf = A.out[fz];
     const long b0 = (long)ks * NL * nxppy + 2 * m;
     double xr[NL], xi[NL];
     if (MODE == 1) {
 #pragma unroll
         for (int t = 0; t < NL; t++) {
             const long o = b0 + (long)t * nxppy;
-            const double2 s0 = *reinterpret_cast<const double2 *>(A.S[0] + o);
-            const double2 s1 = *reinterpret_cast<const double2 *>(A.S[1] + o);
-            const double2 s2 = *reinterpret_cast<const double2 *>(A.S[2] + o);
-            const double2 f0 = *reinterpret_cast<const double2 *>(A.F[0] + o);
-            const double2 f1 = *reinterpret_cast<const double2 *>(A.F[1] + o);
-            const double2 f2 = *reinterpret_cast<const double2 *>(A.F[2] + o);
+            const double2 s0 = *reinterpret_cast<const double2 *>(Sf[0] + o);
+            const double2 s1 = *reinterpret_cast<const double2 *>(Sf[1] + o);
+            const double2 s2 = *reinterpret_cast<const double2 *>(Sf[2] + o);
+            const double2 f0 = *reinterpret_cast<const double2 *>(Ff[0] + o);
+            const double2 f1 = *reinterpret_cast<const double2 *>(Ff[1] + o);
+            const double2 f2 = *reinterpret_cast<const double2 *>(Ff[2] + o);
             const double sr = A.f1 * s0.x + A.f2 * s1.x + A.f3 * s2.x;
             const double si = A.f1 * s0.y + A.f2 * s1.y + A.f3 * s2.y;
             xr[t] = A.dt * sr + A.b1 * f0.x + A.b2 * f1.x + A.b3 * f2.x;
@@ -903,7 +907,7 @@ __global__ __launch_bounds__(256, 2) void k_tile_t(TileArgs A, long nmode, long 
     } else {
 #pragma unroll
         for (int t = 0; t < NL; t++) {
-            const double2 v = *reinterpret_cast<const double2 *>(A.out + b0 + (long)t * nxppy);
+            const double2 v = *reinterpret_cast<const double2 *>(outf + b0 + (long)t * nxppy);
             xr[t] = v.x;
             xi[t] = v.y;
         }
@@ -934,13 +938,13 @@ __global__ __launch_bounds__(256, 2) void k_tile_t(TileArgs A, long nmode, long 
     }
 #pragma unroll
     for (int t = 0; t < NL; t++)
-        *reinterpret_cast<double2 *>(A.out + b0 + (long)t * nxppy) = make_double2(xr[t], xi[t]);
+        *reinterpret_cast<double2 *>(outf + b0 + (long)t * nxppy) = make_double2(xr[t], xi[t]);
 }
 
-static void run_tile(const Geo &g, const TileArgs &A, hipStream_t st)
+static void run_tile(const Geo &g, const TileArgs &A, int nf, hipStream_t st)
 {
     long ncp = (long)g.nxhp * g.ny;
-    dim3 gt((unsigned)((ncp + 255) / 256), g.nsubd);
+    dim3 gt((unsigned)((ncp + 255) / 256), g.nsubd, nf);
 #define TT(NL)                                                                                           \
     if (A.mode == 1)                                                                                     \
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_tile_t<NL, 1>), gt, dim3(256), 0, st, A, ncp, g.nxppy, g.fcf, \
@@ -957,37 +961,38 @@ static void run_tile(const Geo &g, const TileArgs &A, hipStream_t st)
     default: break;
     }
 #undef TT
-    dim3 grid((unsigned)((ncp + kMB - 1) / kMB), g.nsubd);
+    dim3 grid((unsigned)((ncp + kMB - 1) / kMB), g.nsubd, nf);
     hipLaunchKernelGGL(k_tile, grid, dim3(256), 0, st, A, ncp, g.nxppy, g.nzloc, g.fcf, g.wg, g.AL, g.sig);
 }
 
-void launch_bdf_filter(const Geo &g, const iw_params &P, double *const *S3, double *const *F3, double *out,
-                       const double *fac, const double *bdf, double dt, int filt, hipStream_t st)
+void launch_bdf_filter(const Geo &g, double *const *S12, double *const *F12, double *const *out4, const double *fac,
+                       const double *bdf, double dt, int filt, hipStream_t st)
 {
     TileArgs A = {};
-    for (int q = 0; q < 3; q++) {
-        A.S[q] = S3[q];
-        A.F[q] = F3[q];
+    for (int f = 0; f < 4; f++) {
+        for (int q = 0; q < 3; q++) {
+            A.S[f][q] = S12[3 * f + q];
+            A.F[f][q] = F12[3 * f + q];
+        }
+        A.out[f] = out4[f];
     }
-    A.out = out;
     A.f1 = fac[0]; A.f2 = fac[1]; A.f3 = fac[2];
     A.b1 = bdf[0]; A.b2 = bdf[1]; A.b3 = bdf[2];
     A.dt = dt;
     A.mode = 1;
     A.ifourier = filt;
     A.ilegen = filt;
-    (void)P;
-    run_tile(g, A, st);
+    run_tile(g, A, 4, st);
 }
 
 void launch_filter3d(const Geo &g, double *f, int ifourier, int ilegen, hipStream_t st)
 {
     TileArgs A = {};
-    A.out = f;
+    A.out[0] = f;
     A.mode = 0;
     A.ifourier = ifourier;
     A.ilegen = ilegen;
-    run_tile(g, A, st);
+    run_tile(g, A, 1, st);
 }
 
 // ---------------------------------------------------------------------------
