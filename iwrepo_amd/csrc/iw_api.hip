@@ -1101,13 +1101,37 @@ static int convec_dev(iw_solver *s, int ispecfilternl)
         double *o3[3] = {gr[2], gr[5], gr[8]};
         launch_dpdz3(g, uvwt, o3, s->stream);
     }
-    {
-        Scope sc(s, "grad", 72. * nt);
-        launch_grad(g, uvwt, gr, s->stream);
-    }
-    // spectral-to-physical (solver.f:107-123); u,v,w,temp keep their spectral
-    // values and are transformed into separate physical buffers
-    {
+    if (g.ndims == 3) {
+        // spectral-to-physical (solver.f:107-123) with the x/y gradients of
+        // u,v,w (solver.f:86-103) formed inside the transforms: the y pass
+        // turns u,v,w,temp (into the physical buffers) and the three d/dz
+        // fields, and emits i ky f of u,v,w; the x pass turns all ten and
+        // emits i kx f of u,v,w
+        FieldBatch by = batch({B[0], B[1], B[2], B[3], gr[2], gr[5], gr[8]});
+        for (int q = 0; q < 4; q++) by.g[q] = s->phys[q];
+        by.dy[0] = gr[1];
+        by.dy[1] = gr[4];
+        by.dy[2] = gr[7];
+        by.kw = g.yw;
+        {
+            Scope sc(s, "fft_y_inv", 32. * g.nxh * g.ny * g.nz * 10);
+            launch_fft_y(g, by, g.nz, 1, s->stream);
+        }
+        FieldBatch bx = batch({s->phys[0], s->phys[1], s->phys[2], s->phys[3], gr[2], gr[5], gr[8], gr[1], gr[4],
+                               gr[7]});
+        bx.dx[0] = gr[0];
+        bx.dx[1] = gr[3];
+        bx.dx[2] = gr[6];
+        bx.kw = g.xw;
+        {
+            Scope sc(s, "fft_x_inv", 16. * g.nxppy * g.nz * 13);
+            launch_fft_x(g, bx, g.nz, 1, s->stream);
+        }
+    } else {
+        {
+            Scope sc(s, "grad", 72. * nt);
+            launch_grad(g, uvwt, gr, s->stream);
+        }
         FieldBatch b = batch({B[0], B[1], B[2], B[3], gr[0], gr[1], gr[2], gr[3], gr[4], gr[5], gr[6], gr[7],
                               gr[8]});
         for (int q = 0; q < 4; q++) b.g[q] = s->phys[q];

@@ -18,8 +18,13 @@ constexpr int kMaxFields = 16; // fields per batched transform launch
 struct FieldBatch {
     double *f[kMaxFields];
     double *g[kMaxFields];
-    int n;
-    const int *smap, *dmap;
+    // inverse transforms only: optional second outputs, the transform of
+    // i ky f (y pass) or i kx f (x pass), written with the plane map of g
+    double *dy[kMaxFields] = {};
+    double *dx[kMaxFields] = {};
+    const double *kw = nullptr; // ky (y pass) or kx (x pass) per row / column
+    int n = 0;
+    const int *smap = nullptr, *dmap = nullptr;
 };
 
 // Problem geometry and device-resident tables.

@@ -205,8 +205,8 @@ __global__ __launch_bounds__(256) void k_fft_x(FieldBatch fb, int nrows, int ny,
     double *obase = fb.g[blockIdx.y];
     const int row0 = blockIdx.x * R;
     const int N = 2 * M;
-    // logical row -> stored row of the source / destination (plane maps)
     const int lny = __builtin_ctz(ny); // ny is a power of two
+    // logical row -> stored row of the source / destination (plane maps)
     auto srow = [&](int row) -> long {
         int lpl = row >> lny, j = row & (ny - 1);
         return (long)((fb.smap ? fb.smap[lpl] : lpl) * ny + j) * nxpp;
@@ -216,16 +216,11 @@ __global__ __launch_bounds__(256) void k_fft_x(FieldBatch fb, int nrows, int ny,
         return (long)((fb.dmap ? fb.dmap[lpl] : lpl) * ny + j) * nxpp;
     };
     for (int t = threadIdx.x; t < M; t += blockDim.x) tw[t] = twM[t];
-    if (DIR < 0) {
-#pragma unroll
-        for (int it = 0; it < kFE / 256; it++) {
-            const int idx = threadIdx.x + it * 256;
-            int r = idx >> logM, m = idx & (M - 1), row = row0 + r;
-            double2 z = make_double2(0., 0.);
-            if (row < nrows) z = *reinterpret_cast<const double2 *>(base + srow(row) + 2 * m);
-            buf[lp(idx)] = z;
-        }
-    } else {
+    // inverse pre-processing (FFT991 inverse, fft.f:1208-1281): the N/2+1
+    // coefficients A_k + i B_k of a row -> the packed complex sequence; with
+    // deriv the coefficients are first multiplied by i kx (the x gradient of
+    // solver.f:86-103)
+    auto load_inverse = [&](bool deriv) {
 #pragma unroll
         for (int it = 0; it < kFE / 256; it++) {
             const int idx = threadIdx.x + it * 256;
@@ -233,9 +228,14 @@ __global__ __launch_bounds__(256) void k_fft_x(FieldBatch fb, int nrows, int ny,
             double2 Z = make_double2(0., 0.);
             if (row < nrows) {
                 const double *p = base + srow(row);
+                double2 va = *reinterpret_cast<const double2 *>(p + 2 * k);
+                double2 vb = *reinterpret_cast<const double2 *>(p + 2 * (M - k));
+                if (deriv) {
+                    const double ka = fb.kw[k], kb = fb.kw[M - k];
+                    va = make_double2(-ka * va.y, ka * va.x);
+                    vb = make_double2(-kb * vb.y, kb * vb.x);
+                }
                 double ar, ai, br, bi;
-                const double2 va = *reinterpret_cast<const double2 *>(p + 2 * k);
-                const double2 vb = *reinterpret_cast<const double2 *>(p + 2 * (M - k));
                 if (k == 0) {
                     ar = va.x; ai = 0.; br = vb.x; bi = 0.;
                 } else {
@@ -251,6 +251,39 @@ __global__ __launch_bounds__(256) void k_fft_x(FieldBatch fb, int nrows, int ny,
             }
             buf[lp(idx)] = Z;
         }
+    };
+    auto store_inverse = [&](double *ob) {
+#pragma unroll
+        for (int it = 0; it < kFE / 256; it++) {
+            const int idx = threadIdx.x + it * 256;
+            int r = idx >> logM, m = idx & (M - 1), row = row0 + r;
+            if (row >= nrows) continue;
+            *reinterpret_cast<double2 *>(ob + drow(row) + 2 * m) = buf[lp(idx)];
+        }
+        for (int r = threadIdx.x; r < R; r += blockDim.x) {
+            int row = row0 + r;
+            if (row < nrows) *reinterpret_cast<double2 *>(ob + drow(row) + N) = make_double2(0., 0.);
+        }
+    };
+    if (DIR > 0 && fb.dx[blockIdx.y]) {
+        // i kx f first: a transform in place overwrites its input rows
+        load_inverse(true);
+        __syncthreads();
+        fft_lds<false>(buf, logM, R, tw, DIR);
+        store_inverse(fb.dx[blockIdx.y]);
+        __syncthreads();
+    }
+    if (DIR < 0) {
+#pragma unroll
+        for (int it = 0; it < kFE / 256; it++) {
+            const int idx = threadIdx.x + it * 256;
+            int r = idx >> logM, m = idx & (M - 1), row = row0 + r;
+            double2 z = make_double2(0., 0.);
+            if (row < nrows) z = *reinterpret_cast<const double2 *>(base + srow(row) + 2 * m);
+            buf[lp(idx)] = z;
+        }
+    } else {
+        load_inverse(false);
     }
     __syncthreads();
     fft_lds<false>(buf, logM, R, tw, DIR);
@@ -282,17 +315,7 @@ __global__ __launch_bounds__(256) void k_fft_x(FieldBatch fb, int nrows, int ny,
             if (k == 0) emit(r, M, orow);
         }
     } else {
-#pragma unroll
-        for (int it = 0; it < kFE / 256; it++) {
-            const int idx = threadIdx.x + it * 256;
-            int r = idx >> logM, m = idx & (M - 1), row = row0 + r;
-            if (row >= nrows) continue;
-            *reinterpret_cast<double2 *>(obase + drow(row) + 2 * m) = buf[lp(idx)];
-        }
-        for (int r = threadIdx.x; r < R; r += blockDim.x) {
-            int row = row0 + r;
-            if (row < nrows) *reinterpret_cast<double2 *>(obase + drow(row) + N) = make_double2(0., 0.);
-        }
+        store_inverse(obase);
     }
 }
 
@@ -310,41 +333,72 @@ __global__ __launch_bounds__(256) void k_fft_y(FieldBatch fb, int nxpp, int nxh,
     double2 *tw = smy + CB * ny;
     const int lpl = blockIdx.y;
     const double *plane = fb.f[blockIdx.z] + (long)(fb.smap ? fb.smap[lpl] : lpl) * nxpp * ny;
-    double *oplane = fb.g[blockIdx.z] + (long)(fb.dmap ? fb.dmap[lpl] : lpl) * nxpp * ny;
+    const long opl = (long)(fb.dmap ? fb.dmap[lpl] : lpl) * nxpp * ny;
+    double *oplane = fb.g[blockIdx.z] + opl;
     const int ii0 = blockIdx.x * CB;
-    for (int t = threadIdx.x; t < ny; t += blockDim.x) tw[t] = twY[t];
     const int lcb = __builtin_ctz(CB); // CB is a power of two
+    for (int t = threadIdx.x; t < ny; t += blockDim.x) tw[t] = twY[t];
+    // columns ii < nxh of the plane into LDS; with deriv multiplied by i ky
+    // (the y gradient of solver.f:86-103)
+    auto load = [&](bool deriv) {
 #pragma unroll
-    for (int it = 0; it < kFE / 256; it++) {
-        const int idx = threadIdx.x + it * 256;
-        if (idx >= CB * ny) break;
-        int j = idx >> lcb, c = idx & (CB - 1), ii = ii0 + c;
-        double2 z = make_double2(0., 0.);
-        if (ii < nxh) z = *reinterpret_cast<const double2 *>(plane + (long)j * nxpp + 2 * ii);
-        buf[(j << lcb) + c] = z;
+        for (int it = 0; it < kFE / 256; it++) {
+            const int idx = threadIdx.x + it * 256;
+            if (idx >= CB * ny) break;
+            int j = idx >> lcb, c = idx & (CB - 1), ii = ii0 + c;
+            double2 z = make_double2(0., 0.);
+            if (ii < nxh) z = *reinterpret_cast<const double2 *>(plane + (long)j * nxpp + 2 * ii);
+            if (deriv) {
+                const double ky = fb.kw[j];
+                z = make_double2(-ky * z.y, ky * z.x);
+            }
+            buf[(j << lcb) + c] = z;
+        }
+    };
+    auto store = [&](double *op) {
+#pragma unroll
+        for (int it = 0; it < kFE / 256; it++) {
+            const int idx = threadIdx.x + it * 256;
+            if (idx >= CB * ny) break;
+            int j = idx >> lcb, c = idx & (CB - 1), ii = ii0 + c;
+            if (ii >= nxh) continue;
+            double2 z = buf[(j << lcb) + c];
+            if (DIR < 0) {
+                z.x = scaley * z.x;
+                z.y = scaley * z.y;
+            }
+            *reinterpret_cast<double2 *>(op + (long)j * nxpp + 2 * ii) = z;
+        }
+    };
+    double *dyp = (DIR > 0 && fb.dy[blockIdx.z]) ? fb.dy[blockIdx.z] + opl : nullptr;
+    if (dyp) { // i ky f first: a transform in place overwrites its input
+        load(true);
+        __syncthreads();
+        fft_lds<true>(buf, logNy, CB, tw, DIR);
+        store(dyp);
+        __syncthreads();
     }
+    load(false);
     __syncthreads();
     fft_lds<true>(buf, logNy, CB, tw, DIR);
-#pragma unroll
-    for (int it = 0; it < kFE / 256; it++) {
-        const int idx = threadIdx.x + it * 256;
-        if (idx >= CB * ny) break;
-        int j = idx >> lcb, c = idx & (CB - 1), ii = ii0 + c;
-        if (ii >= nxh) continue;
-        double2 z = buf[(j << lcb) + c];
-        if (DIR < 0) {
-            z.x = scaley * z.x;
-            z.y = scaley * z.y;
-        }
-        *reinterpret_cast<double2 *>(oplane + (long)j * nxpp + 2 * ii) = z;
-    }
-    // out of place: the untransformed kx-Nyquist column travels unchanged
-    if (blockIdx.x == 0 && plane != oplane)
+    store(oplane);
+    // the untransformed kx-Nyquist column: copied when out of place, and
+    // times i ky into the derivative output
+    if (blockIdx.x == 0 && (plane != oplane || dyp))
         for (int j = threadIdx.x; j < ny; j += blockDim.x) {
             const double *p = plane + (long)j * nxpp + 2 * nxh;
-            double *q = oplane + (long)j * nxpp + 2 * nxh;
-            q[0] = p[0];
-            q[1] = p[1];
+            const double pr = p[0], pi = p[1];
+            if (plane != oplane) {
+                double *q = oplane + (long)j * nxpp + 2 * nxh;
+                q[0] = pr;
+                q[1] = pi;
+            }
+            if (dyp) {
+                const double ky = fb.kw[j];
+                double *q = dyp + (long)j * nxpp + 2 * nxh;
+                q[0] = -ky * pi;
+                q[1] = ky * pr;
+            }
         }
 }
 
