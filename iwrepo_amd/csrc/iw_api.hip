@@ -1022,9 +1022,13 @@ static FieldBatch batch(std::initializer_list<double *> l)
 // horfft (horfft.f:2-69) over nk logical planes of a batch: the first pass
 // reads b.f (through smap) and writes b.g (through dmap), the second pass
 // works in place on b.g.  is = -1: physical -> spectral.
-static void fft_dev(iw_solver *s, FieldBatch b, int is, int nk)
+static void fft_dev(iw_solver *s, FieldBatch b, int is, int nk, bool iface = false)
 {
     if (b.n == 0 || nk == 0) return;
+    // timing scopes: the interface-plane transforms are kept apart from the
+    // whole-field ones
+    const char *nxf = iface ? "iface_fft_x_fwd" : "fft_x_fwd", *nyf = iface ? "iface_fft_y_fwd" : "fft_y_fwd";
+    const char *nxi = iface ? "iface_fft_x_inv" : "fft_x_inv", *nyi = iface ? "iface_fft_y_inv" : "fft_y_inv";
     const double bx = 16. * (double)s->g.nxppy * nk * b.n;         // each double read + written
     const double by = 32. * (double)s->g.nxh * s->g.ny * nk * b.n; // columns ii < nxh
     FieldBatch b2 = b;
@@ -1033,23 +1037,23 @@ static void fft_dev(iw_solver *s, FieldBatch b, int is, int nk)
     const bool y = s->g.ndims == 3;
     if (is == -1) {
         {
-            Scope sc(s, "fft_x_fwd", bx);
+            Scope sc(s, nxf, bx);
             launch_fft_x(s->g, b, nk, -1, s->stream);
         }
         if (y) {
-            Scope sc(s, "fft_y_fwd", by);
+            Scope sc(s, nyf, by);
             launch_fft_y(s->g, b2, nk, -1, s->stream);
         }
     } else {
         if (y) {
             {
-                Scope sc(s, "fft_y_inv", by);
+                Scope sc(s, nyi, by);
                 launch_fft_y(s->g, b, nk, 1, s->stream);
             }
-            Scope sc(s, "fft_x_inv", bx);
+            Scope sc(s, nxi, bx);
             launch_fft_x(s->g, b2, nk, 1, s->stream);
         } else {
-            Scope sc(s, "fft_x_inv", bx);
+            Scope sc(s, nxi, bx);
             launch_fft_x(s->g, b, nk, 1, s->stream);
         }
     }
@@ -1148,15 +1152,40 @@ static int convec_dev(iw_solver *s, int ispecfilternl)
         launch_rhs_phys(g, s->P, s->t, ph, gr, S, s->ox, s->prod, s->stream);
     }
     double **pr = s->prod;
-    batch_fft(s, {S[0], S[1], S[2], S[3], pr[0], pr[1], pr[2], pr[3], pr[4], pr[5], pr[6]}, -1);
+    if (g.ndims == 3) {
+        // physical-to-spectral of the four right-hand sides and the seven
+        // flux products (solver.f:1167-1180) with the flux divergence
+        // i kx P_a + i ky P_b (solver.f:985-1063, 1163-1196) added inside the
+        // transforms: the x pass turns S_u,S_v,S_w,S_T adding i kx of
+        // P0,P1,P3,P5 and writes the x transforms of P1,P2,P4,P6 to free
+        // gradient buffers; the y pass turns S_u.. adding i ky of those
+        FieldBatch bx = batch({S[0], S[1], S[2], S[3], pr[1], pr[2], pr[4], pr[6]});
+        const int ia[4] = {0, 1, 3, 5};
+        for (int q = 0; q < 4; q++) {
+            bx.ax[q] = pr[ia[q]];
+            bx.g[4 + q] = gr[q];
+        }
+        bx.kw = g.xw;
+        {
+            Scope sc(s, "fft_x_fwd", 16. * g.nxppy * g.nz * 10);
+            launch_fft_x(g, bx, g.nz, -1, s->stream);
+        }
+        FieldBatch by = batch({S[0], S[1], S[2], S[3]});
+        for (int q = 0; q < 4; q++) by.ay[q] = gr[q];
+        by.kw = g.yw;
+        {
+            Scope sc(s, "fft_y_fwd", 32. * g.nxh * g.ny * g.nz * 6);
+            launch_fft_y(g, by, g.nz, -1, s->stream);
+        }
+    } else {
+        batch_fft(s, {S[0], S[1], S[2], S[3], pr[0], pr[1], pr[2], pr[3], pr[4], pr[5], pr[6]}, -1);
+        Scope sc(s, "rhs_spec", 120. * nt);
+        launch_rhs_spec(g, S, pr, s->stream);
+    }
     // u,v,w,temp back to spectral space (solver.f:1180, 1201-1203): their
     // physical values were not changed, so the round trip reduces to its
     // spectral effect on the kx=0 and kx-Nyquist columns
     project_dev(s, uvwt, 4);
-    {
-        Scope sc(s, "rhs_spec", 120. * nt);
-        launch_rhs_spec(g, S, pr, s->stream);
-    }
     {
         Scope sc(s, "wnl", 64. * (double)g.nxppy);
         launch_wnl(g, B[14], B[18], B[22], s->fac, s->wnl, s->stream);
@@ -1249,7 +1278,7 @@ static int iface_dev(iw_solver *s, double *const *f, int nf)
         bi.f[q] = f[q];
         bi.g[q] = s->icomp + (size_t)q * 4 * nint * s->g.nxppy;
     }
-    fft_dev(s, bi, 1, 4 * nint);
+    fft_dev(s, bi, 1, 4 * nint, true);
     {
         Scope sc(s, "iface", 48. * (double)s->g.nxppy * nint * nf);
         FieldBatch b;
@@ -1267,7 +1296,7 @@ static int iface_dev(iw_solver *s, double *const *f, int nf)
         bf.f[q] = bi.g[q];
         bf.g[q] = f[q];
     }
-    fft_dev(s, bf, -1, 2 * nint);
+    fft_dev(s, bf, -1, 2 * nint, true);
     return 0;
 }
 

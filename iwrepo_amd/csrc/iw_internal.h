@@ -22,6 +22,11 @@ struct FieldBatch {
     // i ky f (y pass) or i kx f (x pass), written with the plane map of g
     double *dy[kMaxFields] = {};
     double *dx[kMaxFields] = {};
+    // forward transforms only: optional second inputs P whose transform is
+    // added as i kx P (x pass) or i ky P (y pass) to the field's own (the
+    // horizontal flux terms of rhsfm); read with the plane map of f
+    const double *ax[kMaxFields] = {};
+    const double *ay[kMaxFields] = {};
     const double *kw = nullptr; // ky (y pass) or kx (x pass) per row / column
     int n = 0;
     const int *smap = nullptr, *dmap = nullptr;

@@ -201,6 +201,7 @@ __global__ __launch_bounds__(256) void k_fft_x(FieldBatch fb, int nrows, int ny,
     extern __shared__ double2 smx[];
     double2 *buf = smx;
     double2 *tw = smx + lp(R * M) + 1;
+    double2 *nyq = tw + M; // R entries: k = M term of each row (forward)
     const double *base = fb.f[blockIdx.y];
     double *obase = fb.g[blockIdx.y];
     const int row0 = blockIdx.x * R;
@@ -274,24 +275,21 @@ __global__ __launch_bounds__(256) void k_fft_x(FieldBatch fb, int nrows, int ny,
         __syncthreads();
     }
     if (DIR < 0) {
-#pragma unroll
-        for (int it = 0; it < kFE / 256; it++) {
-            const int idx = threadIdx.x + it * 256;
-            int r = idx >> logM, m = idx & (M - 1), row = row0 + r;
-            double2 z = make_double2(0., 0.);
-            if (row < nrows) z = *reinterpret_cast<const double2 *>(base + srow(row) + 2 * m);
-            buf[lp(idx)] = z;
-        }
-    } else {
-        load_inverse(false);
-    }
-    __syncthreads();
-    fft_lds<false>(buf, logM, R, tw, DIR);
-    if (DIR < 0) {
+        // forward: FFT991 normalised by 1/N with B_0 = B_M = 0 (fft.f:1293-1305)
         const double sc = 1.0 / (double)N;
+        auto load_fwd = [&](const double *src) {
+#pragma unroll
+            for (int it = 0; it < kFE / 256; it++) {
+                const int idx = threadIdx.x + it * 256;
+                int r = idx >> logM, m = idx & (M - 1), row = row0 + r;
+                double2 z = make_double2(0., 0.);
+                if (row < nrows) z = *reinterpret_cast<const double2 *>(src + srow(row) + 2 * m);
+                buf[lp(idx)] = z;
+            }
+        };
         // coefficient k from Z_k and Z_{M-k}; the thread of k = 0 also
-        // writes k = M (both from Z_0)
-        auto emit = [&](int r, int k, double *orow) {
+        // forms k = M (both from Z_0)
+        auto coef = [&](int r, int k) -> double2 {
             double2 Zk = buf[lp((r << logM) + (k == M ? 0 : k))];
             double2 Zm = buf[lp((r << logM) + (k == 0 ? 0 : M - k))];
             double er = 0.5 * (Zk.x + Zm.x), ei = 0.5 * (Zk.y - Zm.y);
@@ -303,7 +301,31 @@ __global__ __launch_bounds__(256) void k_fft_x(FieldBatch fb, int nrows, int ny,
             double A = sc * xr, B = sc * xi;
             if (k == 0 || k == M) B = 0.0;
             if (k == M) A = A * scaley; // horfft.f:28-30 on the untransformed Nyquist column
-            *reinterpret_cast<double2 *>(orow + 2 * k) = make_double2(A, B);
+            return make_double2(A, B);
+        };
+        const double *acc = fb.ax[blockIdx.y];
+        double2 pa[kFE / 256];
+        if (acc) { // transform of the flux product first: coefficients k < M
+                   // kept in registers, k = M of each row in LDS
+            load_fwd(acc);
+            __syncthreads();
+            fft_lds<false>(buf, logM, R, tw, DIR);
+#pragma unroll
+            for (int it = 0; it < kFE / 256; it++) {
+                const int idx = threadIdx.x + it * 256;
+                const int r = idx >> logM, k = idx & (M - 1);
+                pa[it] = coef(r, k);
+                if (k == 0) nyq[r] = coef(r, M);
+            }
+            __syncthreads();
+        }
+        load_fwd(base);
+        __syncthreads();
+        fft_lds<false>(buf, logM, R, tw, DIR);
+        // + i kx P (the x part of the flux divergence, solver.f:985-1063, 1163-1196)
+        auto addk = [&](double2 c, double2 q, int k) {
+            const double kx = fb.kw[k];
+            return make_double2(c.x + (-kx * q.y), c.y + kx * q.x);
         };
 #pragma unroll
         for (int it = 0; it < kFE / 256; it++) {
@@ -311,10 +333,19 @@ __global__ __launch_bounds__(256) void k_fft_x(FieldBatch fb, int nrows, int ny,
             int r = idx >> logM, k = idx & (M - 1), row = row0 + r;
             if (row >= nrows) continue;
             double *orow = obase + drow(row);
-            emit(r, k, orow);
-            if (k == 0) emit(r, M, orow);
+            double2 c = coef(r, k);
+            if (acc) c = addk(c, pa[it], k);
+            *reinterpret_cast<double2 *>(orow + 2 * k) = c;
+            if (k == 0) {
+                double2 cm = coef(r, M);
+                if (acc) cm = addk(cm, nyq[r], M);
+                *reinterpret_cast<double2 *>(orow + 2 * M) = cm;
+            }
         }
     } else {
+        load_inverse(false);
+        __syncthreads();
+        fft_lds<false>(buf, logM, R, tw, DIR);
         store_inverse(obase);
     }
 }
@@ -340,14 +371,14 @@ __global__ __launch_bounds__(256) void k_fft_y(FieldBatch fb, int nxpp, int nxh,
     for (int t = threadIdx.x; t < ny; t += blockDim.x) tw[t] = twY[t];
     // columns ii < nxh of the plane into LDS; with deriv multiplied by i ky
     // (the y gradient of solver.f:86-103)
-    auto load = [&](bool deriv) {
+    auto load = [&](const double *src, bool deriv) {
 #pragma unroll
         for (int it = 0; it < kFE / 256; it++) {
             const int idx = threadIdx.x + it * 256;
             if (idx >= CB * ny) break;
             int j = idx >> lcb, c = idx & (CB - 1), ii = ii0 + c;
             double2 z = make_double2(0., 0.);
-            if (ii < nxh) z = *reinterpret_cast<const double2 *>(plane + (long)j * nxpp + 2 * ii);
+            if (ii < nxh) z = *reinterpret_cast<const double2 *>(src + (long)j * nxpp + 2 * ii);
             if (deriv) {
                 const double ky = fb.kw[j];
                 z = make_double2(-ky * z.y, ky * z.x);
@@ -355,7 +386,13 @@ __global__ __launch_bounds__(256) void k_fft_y(FieldBatch fb, int nxpp, int nxh,
             buf[(j << lcb) + c] = z;
         }
     };
-    auto store = [&](double *op) {
+    // forward with a second input P: + i ky P-hat (the y part of the flux
+    // divergence, solver.f:985-1063, 1163-1196), P-hat kept in registers
+    const double *accp = (DIR < 0 && fb.ay[blockIdx.z])
+                             ? fb.ay[blockIdx.z] + (long)(fb.smap ? fb.smap[lpl] : lpl) * nxpp * ny
+                             : nullptr;
+    double2 pb[kFE / 256];
+    auto store = [&](double *op, bool add) {
 #pragma unroll
         for (int it = 0; it < kFE / 256; it++) {
             const int idx = threadIdx.x + it * 256;
@@ -366,29 +403,57 @@ __global__ __launch_bounds__(256) void k_fft_y(FieldBatch fb, int nxpp, int nxh,
             if (DIR < 0) {
                 z.x = scaley * z.x;
                 z.y = scaley * z.y;
+                if (add) {
+                    const double ky = fb.kw[j];
+                    z.x = z.x + (-ky * pb[it].y);
+                    z.y = z.y + ky * pb[it].x;
+                }
             }
             *reinterpret_cast<double2 *>(op + (long)j * nxpp + 2 * ii) = z;
         }
     };
     double *dyp = (DIR > 0 && fb.dy[blockIdx.z]) ? fb.dy[blockIdx.z] + opl : nullptr;
     if (dyp) { // i ky f first: a transform in place overwrites its input
-        load(true);
+        load(plane, true);
         __syncthreads();
         fft_lds<true>(buf, logNy, CB, tw, DIR);
-        store(dyp);
+        store(dyp, false);
         __syncthreads();
     }
-    load(false);
+    if (accp) {
+        load(accp, false);
+        __syncthreads();
+        fft_lds<true>(buf, logNy, CB, tw, DIR);
+#pragma unroll
+        for (int it = 0; it < kFE / 256; it++) {
+            const int idx = threadIdx.x + it * 256;
+            double2 z = make_double2(0., 0.);
+            if (idx < CB * ny) {
+                z = buf[((idx >> lcb) << lcb) + (idx & (CB - 1))];
+                z.x = scaley * z.x;
+                z.y = scaley * z.y;
+            }
+            pb[it] = z;
+        }
+        __syncthreads();
+    }
+    load(plane, false);
     __syncthreads();
     fft_lds<true>(buf, logNy, CB, tw, DIR);
-    store(oplane);
-    // the untransformed kx-Nyquist column: copied when out of place, and
-    // times i ky into the derivative output
-    if (blockIdx.x == 0 && (plane != oplane || dyp))
+    store(oplane, accp != nullptr);
+    // the untransformed kx-Nyquist column: copied when out of place, plus
+    // i ky P when accumulating, and times i ky into the derivative output
+    if (blockIdx.x == 0 && (plane != oplane || dyp || accp))
         for (int j = threadIdx.x; j < ny; j += blockDim.x) {
             const double *p = plane + (long)j * nxpp + 2 * nxh;
-            const double pr = p[0], pi = p[1];
-            if (plane != oplane) {
+            double pr = p[0], pi = p[1];
+            if (accp) {
+                const double ky = fb.kw[j];
+                const double *a = accp + (long)j * nxpp + 2 * nxh;
+                pr = pr + (-ky * a[1]);
+                pi = pi + ky * a[0];
+            }
+            if (plane != oplane || accp) {
                 double *q = oplane + (long)j * nxpp + 2 * nxh;
                 q[0] = pr;
                 q[1] = pi;
@@ -407,7 +472,7 @@ void launch_fft_x(const Geo &g, FieldBatch b, int nk, int dir, hipStream_t st)
     const int R = kFE / g.M;
     int nrows = g.ny * nk;
     dim3 grid((nrows + R - 1) / R, b.n);
-    size_t sh = sizeof(double2) * ((size_t)lp(R * g.M) + 1 + g.M);
+    size_t sh = sizeof(double2) * ((size_t)lp(R * g.M) + 1 + g.M + R);
     if (dir < 0)
         hipLaunchKernelGGL(k_fft_x<-1>, grid, dim3(256), sh, st, b, nrows, g.ny, g.nxpp, g.M, g.logM, R,
                            g.twM, g.twN, g.scaley);
