@@ -34,6 +34,11 @@ typedef struct iw_params {
     int ifilter, nstepfilter;      /* estimation-model filter every nstepfilter
                                       steps (main.f:880-895)                      */
     int npostp;                    /* ... with the SGS tensors every npostp steps  */
+    int imaxcalc;                  /* 1: min/max/CFL reductions of maxcalc every
+                                      step (main.f:47, various.f:749-851)          */
+    int dtadapt;                   /* adaptive dt of various.f:911-985: 0 off
+                                      (fixed dt), 1 with Tso kept from the last
+                                      change, 2 with Tso read as 0 at every call  */
     double xlen, ylen, zlen;
     double xnu, xkappa, brunt, grav, rho0, omz;
     double dt;

@@ -30,6 +30,7 @@ c     layout of iw_params (include/iwgpu.h)
         integer(c_int) :: ispecfilternl, ispecfilterexp
         integer(c_int) :: nspecfilterexp, navgintrfc
         integer(c_int) :: ifilter, nstepfilter, npostp
+        integer(c_int) :: imaxcalc, dtadapt
         real(c_double) :: xlen, ylen, zlen
         real(c_double) :: xnu, xkappa, brunt, grav, rho0, omz
         real(c_double) :: dt
@@ -231,6 +232,10 @@ c     the estimation filter is an argument of filtering here
       iwp%ifilter = 0
       iwp%nstepfilter = 1
       iwp%npostp = 1
+c     maxcalc is its own shim; adaptive dt as in various.f:911-985 with
+c     Tso kept between calls
+      iwp%imaxcalc = 1
+      iwp%dtadapt = 1
       iwp%xlen = xlen
       iwp%ylen = ylen
       iwp%zlen = zlen

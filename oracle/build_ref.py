@@ -9,9 +9,14 @@ several variants of it, e.g. ``dim_65.h``, ``dim_multi.h``).  Flags follow
 ``makefile:24-31``: ``-autodouble`` becomes ``-fdefault-real-8
 -fdefault-double-8`` and ifort's static local arrays become ``-fno-automatic``.
 
-``various.f`` is not compiled (flang rejects its REAL subscript at
-``various.f:503``); ``ref_harness.f`` supplies its ``zero``/``copy`` helpers and
-replaces ``main.f`` (which flang rejects at ``main.f:571``).
+``ref_harness.f`` replaces ``main.f`` (which flang rejects at ``main.f:571``).
+``various.f`` (``maxcalc``, ``various.f:665-987``) is compiled with the
+preprocessor renaming the three REAL loop counters of ``testdump``
+(``various.f:493-523``, which flang rejects as subscripts) to INTEGER names;
+that routine is not called.  flang also rejects constant subscripts outside the
+arrays (``various.f:194-204`` for ny = 1, ``various.f:646, 745`` for nxpp < 32 or
+nz < 58), so builds where it does not compile leave it out and the harness
+supplies its ``zero``/``copy`` helpers instead.
 
 Outputs go only under ``oracle/_ref/`` (git-ignored).  Nothing here runs when
 ``/root/reference`` is absent.
@@ -32,6 +37,9 @@ FFLAGS = ["-fdefault-real-8", "-fdefault-double-8", "-fno-automatic",
 SOURCES = ["solver.f", "penalty.f", "band_lu.f", "bandludcmp.f", "ludcmp.f",
            "derv.f", "horfft.f", "fft.f", "filtering.f", "setup.f",
            "forcing.f", "multistep.f", "erfc.f", "sgscalc.f", "output.f", "restart.f"]
+
+# testdump's REAL loop counters used as subscripts (various.f:493-523)
+VARIOUS_FLAGS = ["-cpp", "-Dcounter=icounter", "-Dcounter2=icounter2", "-Dcounter3=icounter3"]
 
 DIM_TEMPLATE = """\
       parameter   ( nx = {nx}, ny={ny}, nz= {nz}, nzloc = {nzloc}, nsubd = {nsubd})
@@ -74,7 +82,13 @@ def build(name, nx, ny, nzloc, nsubd, noslip=False, ref_dim=False, jobs=8, shim=
         with open(dpath, "w") as fh:
             fh.write(txt)
     srcs = SOURCES + ["calcuvwt_noslip.f" if noslip else "calcuvwt.f"]
-    objs = []
+    vo = os.path.join(bdir, "various.o")
+    r = subprocess.run([FLANG] + FFLAGS + VARIOUS_FLAGS + ["-c", os.path.join(src, "various.f"), "-o", vo],
+                       stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+    with_maxcalc = r.returncode == 0
+    if not with_maxcalc:
+        print(name, ": various.f does not compile at these sizes; maxcalc left out")
+    objs = [vo] if with_maxcalc else []
     procs = []
     for s in srcs:
         o = os.path.join(bdir, s.replace(".f", ".o"))
@@ -85,17 +99,34 @@ def build(name, nx, ny, nzloc, nsubd, noslip=False, ref_dim=False, jobs=8, shim=
             _wait(procs)
     _wait(procs)
     ho = os.path.join(bdir, "ref_harness.o")
-    subprocess.run([FLANG] + FFLAGS + ["-I", src, "-c", os.path.join(HERE, "ref_harness.f"),
-                                       "-o", ho], check=True)
+    hdef = ["-DIW_MAXCALC"] if with_maxcalc else []
+    subprocess.run([FLANG] + FFLAGS + ["-cpp"] + hdef + ["-I", src, "-c", os.path.join(HERE, "ref_harness.f"),
+                                                         "-o", ho], check=True)
     exe = os.path.join(bdir, "iwref")
-    subprocess.run([FLANG] + FFLAGS + [ho] + objs + ["-o", exe], check=True)
+    link([FLANG] + FFLAGS + [ho] + objs + ["-o", exe], with_maxcalc)
     print("built", exe)
     if shim:
-        build_shim(bdir, src, ho, objs, noslip)
+        build_shim(bdir, src, ho, objs, noslip, with_maxcalc)
     return exe
 
 
-def build_shim(bdir, src, ho, objs, noslip):
+def link(cmd, with_maxcalc):
+    """Link; with various.f in the build, its timer routine timecall
+    (various.f:990-1010) references the ifort library routine time, which
+    flang's runtime lacks.  timecall is never called here, so that one
+    reference is left unresolved (reported as a warning); any other
+    unresolved symbol fails the build."""
+    if with_maxcalc:
+        cmd = cmd + ["-Wl,--warn-unresolved-symbols"]
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
+    out = r.stdout.decode()
+    bad = [l for l in out.splitlines() if "undefined symbol" in l and "time_" not in l]
+    if r.returncode != 0 or bad:
+        sys.stderr.write(out)
+        raise SystemExit("link failed")
+
+
+def build_shim(bdir, src, ho, objs, noslip, with_maxcalc=False):
     """The same harness with the product's Fortran shims (iwshim.F) linked in
     front of the reference objects: every call of horfft, dpdz, convec,
     calcuvwt, interface_avg3d, filter_3dlegenfourier and the multistep
@@ -113,9 +144,9 @@ def build_shim(bdir, src, ho, objs, noslip):
     subprocess.run(cmd + ["-c", os.path.join(PKG, "fortran", "iwshim.F"), "-o", so], check=True)
     exe = os.path.join(bdir, "iwshim")
     rel = os.path.relpath(PKG, bdir)
-    subprocess.run([FLANG] + FFLAGS + [ho, so] + objs +
-                   ["-o", exe, "-Wl,--allow-multiple-definition", "-L" + PKG, "-liwgpu",
-                    "-Wl,-rpath,$ORIGIN/" + rel, "-Wl,-rpath,/opt/rocm/lib"], check=True)
+    link([FLANG] + FFLAGS + [ho, so] + objs +
+         ["-o", exe, "-Wl,--allow-multiple-definition", "-L" + PKG, "-liwgpu",
+          "-Wl,-rpath,$ORIGIN/" + rel, "-Wl,-rpath,/opt/rocm/lib"], with_maxcalc)
     print("built", exe)
     return exe
 

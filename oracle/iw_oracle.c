@@ -43,6 +43,13 @@ struct iwo_state {
     /* estimation filter: /comsgs/ t11,t12,t13,t22,t23,t33,th1,th2,th3
        (comsgs.h) and the /comfiltertmp/ work arrays (comfiltertmp.h) */
     double *sgs[9], *tmps[6];
+    /* maxcalc (various.f:665-987): Tso, the /vardtflags/ ivardt, idtstep
+       (comtime.h:3), itmin/jtmin and dz, which the reference keeps from
+       the previous call, and the per-step records of the last iwo_run */
+    double Tso, dzlast;
+    int ivardt, idtstep, itmin, jtmin;
+    double *mcrec;
+    int mcn, mccap;
 };
 
 #define U(s) ((s)->F[0])
@@ -290,6 +297,7 @@ iwo_state *iwo_create(const iwo_params *p)
     s->xomega = (p->brunt * s->xkcrit) / sqrt(s->xkcrit * s->xkcrit + s->xmcrit * s->xmcrit);
     s->twave = s->pi2 / s->xomega;
     s->dt = p->dt;
+    s->cdt = 1.; /* cdtin (input.dat) */
     s->fact = 1. / (p->xkappa * s->dt);
     s->facvel = 1. / (p->xnu * s->dt);
     int nl = s->nzloc;
@@ -355,6 +363,7 @@ void iwo_destroy(iwo_state *s)
     for (int q = 0; q < 6; q++) free(s->tmps[q]);
     for (int f = 0; f < 8; f++) free(s->work[f]);
     free(s->wnl); free(s->wviscrot);
+    free(s->mcrec);
     free(s->ub); free(s->vb); free(s->wb); free(s->phi); free(s->dphidz);
     free(s->twx_c); free(s->twx_s); free(s->twy_c); free(s->twy_s);
     free(s);
@@ -1599,33 +1608,170 @@ void iwo_set_restart(iwo_state *s, const double *state, double t, double cdt)
     s->istart = 1;
 }
 
+/* variable-step coefficients for the step sizes (dt1, dt2, dt3), the
+   blocks of main.f:619-634 / various.f:945-958 */
+static void var_coeffs(iwo_state *s, double d1, double d2, double d3)
+{
+    s->fac1 = iwo_abvar1(d1, d2, d3); s->fac2 = iwo_abvar2(d1, d2, d3); s->fac3 = iwo_abvar3(d1, d2, d3);
+    s->bdf1 = iwo_bdfvar1(d1, d2, d3); s->bdf2 = iwo_bdfvar2(d1, d2, d3); s->bdf3 = iwo_bdfvar3(d1, d2, d3);
+}
+
+/* maxcalc (various.f:665-987): the physical u,v,w,temp (horfft +1), the
+   running extrema and CFL numbers in the reference's loop order (i, j, then
+   z; a later point wins a tie, various.f:755-845), the return to spectral
+   space (the state-changing round trip) and, with dtadapt, the adaptive
+   step of various.f:887-985.  rec (IWO_NREC doubles) receives the outputs. */
+void iwo_maxcalc(iwo_state *s, int istep, double *rec)
+{
+    const int nx = s->nx, ny = s->ny, nz = s->nz, nzl = s->nzloc, ns = s->nsubd;
+    for (int q = 0; q < 4; q++) iwo_horfft(s, s->F[q], 1, nz);
+    const double pi2 = 8. * atan(1.0); /* various.f:696-700 */
+    const double xl = pi2 / s->alpha, yl = pi2 / s->beta;
+    const double dy = yl / ny, dx = xl / nx;
+    /* zf (comgridptxyz.h) as setup.f:150-161 fills it (the upper subdomain
+       writes an interface point last); zf(0) is the word before zf in
+       /comgridptxyz/, z(nz) */
+    const int nz2 = ns * (nzl - 1) + 1;
+    double *zf = (double *)malloc(sizeof(double) * (size_t)(nz2 + 1));
+    zf[0] = s->z[nz - 1];
+    for (int ks = 0; ks < ns; ks++)
+        for (int kl = 0; kl < nzl; kl++) zf[ks * (nzl - 1) + kl + 1] = s->z[ks * nzl + kl];
+    const double b = s->P.brunt;
+    const double rhograd = -(s->P.rho0 * (b * b)) / s->P.grav; /* forcing.f:115-120 */
+    double umin = 0., umaxt = 0., vmin = 0., vmax = 0., wmin = 0., wmax = 0., tmin = 1., tmax = 0.;
+    double cflxmax = 0., cflymax = 0., cflzmax = 0., dzmax = s->dzlast, dz = s->dzlast;
+    int kumin = 0, kumax = 0, kvmin = 0, kvmax = 0, kwmin = 0, kwmax = 0, ktmin = 0, ktmax = 0;
+    int itmax = 0, jtmax = 0, icflx = 0, icfly = 0, icflz = 0, jcflx = 0, jcfly = 0, jcflz = 0;
+    int kcflx = 0, kcfly = 0, kcflz = 0;
+    const double *u = U(s), *v = V(s), *w = W(s), *tp = T(s);
+    for (int i = 0; i < nx; i++)
+        for (int j = 0; j < ny; j++)
+            for (int ks = 0; ks < ns; ks++)
+                for (int kl = 0; kl < nzl; kl++) {
+                    const int k = ks * nzl + kl, kg = ks * (nzl - 1) + kl + 1;
+                    const long o = (long)k * s->nxppy + (long)j * s->nxpp + i;
+                    const double uu = u[o] + 0.0, vv = v[o], ww = w[o] + 0.0; /* umwv = wmwv = 0 */
+                    if (uu >= umaxt) { umaxt = uu + s->P.umax; kumax = k + 1; }
+                    if (uu <= umin) { umin = uu; kumin = k + 1; }
+                    if (vv >= vmax) { vmax = vv; kvmax = k + 1; }
+                    if (vv <= vmin) { vmin = vv; kvmin = k + 1; }
+                    if (ww >= wmax) { wmax = ww; kwmax = k + 1; }
+                    if (ww <= wmin) { wmin = ww; kwmin = k + 1; }
+                    const double rm = 1. + rhograd * s->z[k]; /* rhomean, forcing.f:134-141 */
+                    const double tt = (tp[o] + 0.0) + rm;     /* rhomwv = 0 */
+                    if (tt >= tmax) { tmax = tt; ktmax = k + 1; itmax = i + 1; jtmax = j + 1; }
+                    if (tt <= tmin) { tmin = tt; ktmin = k + 1; s->itmin = i + 1; s->jtmin = j + 1; }
+                    /* uo*fcshear*fshear = 0: fcshear = 0 (main.f:120) */
+                    const double cflx = fabs((u[o] * s->dt) / dx);
+                    const double cfly = fabs((v[o] * s->dt) / dy);
+                    double cflz;
+                    if (k + 1 == nz) {
+                        cflz = 0.;
+                    } else {
+                        dz = zf[kg] - zf[kg - 1];
+                        cflz = fabs((w[o] * s->dt) / dz);
+                    }
+                    if (cflx >= cflxmax) { cflxmax = cflx; icflx = i + 1; jcflx = j + 1; kcflx = k + 1; }
+                    if (cfly >= cflymax) { cflymax = cfly; icfly = i + 1; jcfly = j + 1; kcfly = k + 1; }
+                    if (cflz >= cflzmax) {
+                        cflzmax = cflz; icflz = i + 1; jcflz = j + 1; kcflz = k + 1;
+                        dzmax = dz;
+                    }
+                }
+    free(zf);
+    s->dzlast = dz;
+    /* uveltest.dat probe (various.f:744-746): u(32,1,10), u(32,1,50) by
+       their linear position in u */
+    const long p1 = 31 + (long)s->nxpp * ((long)ny * 9), p2 = 31 + (long)s->nxpp * ((long)ny * 49);
+    const double pr1 = p1 < s->ntot ? u[p1] : NAN, pr2 = p2 < s->ntot ? u[p2] : NAN;
+    for (int q = 0; q < 4; q++) iwo_horfft(s, s->F[q], -1, nz);
+    double r[IWO_NREC] = {0};
+    r[0] = istep; r[1] = s->t;
+    r[2] = umin; r[3] = umaxt; r[4] = kumin; r[5] = kumax;
+    r[6] = vmin; r[7] = vmax; r[8] = kvmin; r[9] = kvmax;
+    r[10] = wmin; r[11] = wmax; r[12] = kwmin; r[13] = kwmax;
+    r[14] = tmin; r[15] = tmax; r[16] = ktmin; r[17] = ktmax;
+    r[18] = s->itmin; r[19] = s->jtmin; r[20] = itmax; r[21] = jtmax;
+    r[22] = cflxmax; r[23] = icflx; r[24] = jcflx; r[25] = kcflx;
+    r[26] = cflymax; r[27] = icfly; r[28] = jcfly; r[29] = kcfly;
+    r[30] = cflzmax; r[31] = icflz; r[32] = jcflz; r[33] = kcflz;
+    r[34] = dzmax;
+    r[40] = pr1; r[41] = pr2;
+    int changed = 0;
+    if (s->P.dtadapt) { /* various.f:887-985 */
+        const int change_dt = (cflzmax > 0.7) || (cflzmax < 0.1) || (cflxmax > 0.16);
+        const double tlim = pi2 / (s->xomega * 50.);
+        const double tso = s->P.dtadapt == 2 ? 0. : s->Tso;
+        const int nst = (int)((s->t - tso) / s->dt);
+        if (nst > 20 && change_dt) {
+            s->Tso = s->t;
+            if (cflzmax < 0.1) s->cdt = fmax(0.8, s->dt / tlim);
+            if ((cflzmax > 0.7) || (cflxmax > 0.16)) s->cdt = 1.25;
+            s->dt = s->dt / s->cdt;
+            var_coeffs(s, s->dt, s->cdt * s->dt, s->cdt * s->dt);
+            s->bdfv = iwo_bdfvarv(s->dt, s->cdt * s->dt, s->cdt * s->dt);
+            s->fact = 1. / (s->P.xkappa * s->dt); /* pr = xnu */
+            s->facvel = 1. / (s->P.xnu * s->dt);
+            s->ivardt = 1;
+            s->idtstep = 0;
+            changed = 1;
+        }
+    }
+    r[35] = s->dt; r[36] = s->cdt; r[37] = changed; r[38] = s->ivardt; r[39] = s->idtstep;
+    if (rec) memcpy(rec, r, sizeof r);
+}
+
+double *iwo_maxcalc_records(iwo_state *s, int *n)
+{
+    if (n) *n = s->mcn;
+    return s->mcrec;
+}
+
 void iwo_run(iwo_state *s, int nsteps, int *counts)
 {
+    s->mcn = 0;
+    if (nsteps > s->mccap) {
+        free(s->mcrec);
+        s->mcrec = (double *)calloc((size_t)nsteps * IWO_NREC, sizeof(double));
+        s->mccap = nsteps;
+    }
     for (int n = 0; n < nsteps; n++) {
         int istep = ++s->istep;
-        /* state-changing part of maxcalc, various.f:739-742, 854-857 */
-        for (int q = 0; q < 4; q++) iwo_horfft(s, s->F[q], 1, s->nz);
-        for (int q = 0; q < 4; q++) iwo_horfft(s, s->F[q], -1, s->nz);
+        if (s->P.imaxcalc || s->P.dtadapt) { /* main.f:657 */
+            iwo_maxcalc(s, istep, s->mcrec + (size_t)n * IWO_NREC);
+            s->mcn = n + 1;
+        } else { /* only the state-changing part, various.f:739-742, 854-857 */
+            for (int q = 0; q < 4; q++) iwo_horfft(s, s->F[q], 1, s->nz);
+            for (int q = 0; q < 4; q++) iwo_horfft(s, s->F[q], -1, s->nz);
+        }
+        if (s->ivardt == 1) s->idtstep++; /* main.f:660-664 */
         convec(s);
-        if (istep < 2 && !s->istart) { /* main.f:700-777 */
-            s->fac1 = 2.; s->fac2 = -1.; s->fac3 = 0.;
-            s->bdf1 = 2.; s->bdf2 = -0.5; s->bdf3 = 0.;
-        } else if (istep < 2) { /* restart, main.f:719-734 */
-            double d1 = s->dt, d3 = s->cdt * s->dt;
-            s->fac1 = iwo_abvar1(d1, d1, d3); s->fac2 = iwo_abvar2(d1, d1, d3); s->fac3 = iwo_abvar3(d1, d1, d3);
-            s->bdf1 = iwo_bdfvar1(d1, d1, d3); s->bdf2 = iwo_bdfvar2(d1, d1, d3); s->bdf3 = iwo_bdfvar3(d1, d1, d3);
-        } else {
+        if (istep < 2) { /* main.f:700-734 */
+            if (!s->istart) {
+                s->fac1 = 2.; s->fac2 = -1.; s->fac3 = 0.;
+                s->bdf1 = 2.; s->bdf2 = -0.5; s->bdf3 = 0.;
+            } else {
+                var_coeffs(s, s->dt, s->dt, s->cdt * s->dt);
+            }
+        } else if (!(s->ivardt == 1 && s->idtstep < 2)) { /* main.f:741-756 */
             s->fac1 = 3.; s->fac2 = -3.; s->fac3 = 1.;
             s->bdf1 = 3.; s->bdf2 = -1.5; s->bdf3 = 1. / 3.;
+        } else { /* first step after a change of dt, main.f:757-776 */
+            var_coeffs(s, s->dt, s->dt, s->cdt * s->dt);
         }
         for (int q = 0; q < 4; q++) /* su = u ... (main.f:785-795) */
             memcpy(s->F[12 + q], s->F[q], sizeof(double) * (size_t)s->ntot);
         int cpen = (s->P.ispecfilterexp == 1) && (istep % s->P.nspecfilterexp == 0);
         calcuvwt(s, cpen);
-        if (istep >= 2) /* main.f:815-856 */
-            s->bdfv = 11. / 6.;
-        else
+        if (istep < 2) { /* main.f:813-826 */
             s->bdfv = s->istart ? iwo_bdfvarv(s->dt, s->dt, s->cdt * s->dt) : 1.5;
+        } else if (!(s->ivardt == 1 && s->idtstep < 2)) { /* main.f:832-837 */
+            s->bdfv = 11. / 6.;
+        } else { /* main.f:838-854 */
+            s->bdfv = iwo_bdfvarv(s->dt, s->dt, s->cdt * s->dt);
+            s->cdt = 1.;
+            s->ivardt = 0;
+        }
         if (s->P.ifilter == 1 && s->P.nstepfilter > 0 && istep % s->P.nstepfilter == 0) /* main.f:880-895 */
             iwo_filtering(s, s->P.npostp > 0 && istep % s->P.npostp == 0);
         s->t = s->t + s->dt;

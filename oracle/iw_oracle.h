@@ -25,6 +25,7 @@ typedef struct iwo_params {
     int bc_uv_bottom;   /* 0 Neumann (calcuvwt.f:291-295), 1 Dirichlet (calcuvwt_noslip.f:294-299) */
     int ispecfilternl, ispecfilterexp, nspecfilterexp, navgintrfc;
     int ifilter, nstepfilter, npostp; /* estimation filter, main.f:880-895 */
+    int imaxcalc, dtadapt;            /* maxcalc reductions, adaptive dt (various.f:665-987) */
     double xlen, ylen, zlen;
     double xnu, xkappa, brunt, grav, rho0, omz;
     double dt;
@@ -74,6 +75,19 @@ void iwo_filtering(iwo_state *s, int isgscalc);
    5 t33, 6 th1, 7 th2, 8 th3 (comsgs.h) */
 double *iwo_sgs(iwo_state *s, int idx);
 
+/* maxcalc (various.f:665-987) on the spectral state: the round trip, the
+   extrema/CFL record and, with dtadapt, the adaptive step.  Record layout
+   (IWO_NREC doubles; indices 1-based as the reference prints them, 0 =
+   none): 0 istep, 1 t, 2 umin, 3 umaxt, 4 kumin, 5 kumax, 6 vmin, 7 vmax,
+   8 kvmin, 9 kvmax, 10 wmin, 11 wmax, 12 kwmin, 13 kwmax, 14 tmin, 15 tmax,
+   16 ktmin, 17 ktmax, 18 itmin, 19 jtmin, 20 itmax, 21 jtmax, 22 cflxmax,
+   23-25 i,j,k, 26 cflymax, 27-29 i,j,k, 30 cflzmax, 31-33 i,j,k, 34 dzmax,
+   35 dt (after the decision), 36 cdt, 37 dt changed, 38 ivardt, 39 idtstep,
+   40 u(32,1,10), 41 u(32,1,50) (physical, uveltest.dat). */
+#define IWO_NREC 42
+void iwo_maxcalc(iwo_state *s, int istep, double *rec);
+/* records of the steps of the last iwo_run (n of them) */
+double *iwo_maxcalc_records(iwo_state *s, int *n);
 double iwo_abvar1(double, double, double);
 double iwo_abvar2(double, double, double);
 double iwo_abvar3(double, double, double);

@@ -15,6 +15,7 @@ LIB = os.path.join(HERE, "_ref", "liboracle.so")
 REFDIR = os.path.join(HERE, "_ref")
 
 _lib = None
+NREC = 42  # maxcalc record length (IWO_NREC)
 
 
 def lib():
@@ -44,6 +45,9 @@ def lib():
         L.iwo_run.argtypes = [P, ctypes.c_int, P]
         L.iwo_filtering.argtypes = [P, ctypes.c_int]
         L.iwo_set_restart.argtypes = [P, P, D, D]
+        L.iwo_maxcalc.argtypes = [P, ctypes.c_int, P]
+        L.iwo_maxcalc_records.restype = ctypes.POINTER(D)
+        L.iwo_maxcalc_records.argtypes = [P, ctypes.POINTER(ctypes.c_int)]
         L.iwo_sgs.restype = ctypes.POINTER(D)
         L.iwo_sgs.argtypes = [P, ctypes.c_int]
         for n in ("abvar1", "abvar2", "abvar3", "bdfvarv", "bdfvar1", "bdfvar2", "bdfvar3"):
@@ -146,6 +150,21 @@ class Oracle:
         self.L.iwo_run(self.h, nsteps, counts.ctypes.data_as(ctypes.c_void_p))
         return counts.reshape(nsteps, 4)
 
+    def maxcalc(self, istep=1):
+        """One maxcalc call (various.f:665-987) on the current state; its
+        record (NREC values, layout in iw_oracle.h)."""
+        r = np.zeros(NREC)
+        self.L.iwo_maxcalc(self.h, istep, _ptr(r))
+        return r
+
+    def maxcalc_records(self):
+        """(nsteps, NREC) maxcalc records of the last run."""
+        n = ctypes.c_int(0)
+        p = self.L.iwo_maxcalc_records(self.h, ctypes.byref(n))
+        if n.value == 0:
+            return np.zeros((0, NREC))
+        return np.ctypeslib.as_array(p, shape=(n.value * NREC,)).reshape(n.value, NREC).copy()
+
 
 # ---------------------------------------------------------------------------
 # the compiled reference (oracle/_ref/<build>/iwref), driven by ref_harness.f
@@ -173,16 +192,58 @@ def _input_dat(P, nsteps, istart=0, cdt=1.0):
 """
 
 
+def ref_has_maxcalc(build):
+    """True when the reference build includes various.f (maxcalc)."""
+    return os.path.exists(os.path.join(REFDIR, build, "various.o"))
+
+
+def _parse_maxcalc(d):
+    """Outputs of the reference maxcalc (various.f:744-885) and of the
+    harness's per-step log: dtlog (istep, t, dt, cdt, ivardt, idtstep after
+    maxcalc), fort50 (istep-1, umin, umaxt, kumin, kumax, v.., w.., t..;
+    E12.6), fort51 (t, cflx, i, j, k, cfly, .., cflz, ..; E12.6), cflz
+    (dzmax, cflzmax of the 'At max CFLZ' line, full precision), tloc
+    (itmin, jtmin, ktmin, itmax, jtmax, ktmax), uvel (t, u(32,1,10),
+    u(32,1,50))."""
+    out = {}
+
+    def rows(name):
+        fn = os.path.join(d, name)
+        if not os.path.exists(fn):
+            return []
+        with open(fn) as f:
+            return [[float(x.replace("D", "E")) for x in l.split()] for l in f if l.strip()]
+    out["dtlog"] = rows("dtlog.txt")
+    out["fort50"] = rows("fort.50")
+    out["fort51"] = rows("fort.51")
+    out["uvel"] = rows("uveltest.dat")
+    with open(os.path.join(d, "stdout.txt")) as f:
+        lines = f.read().split("\n")
+    cflz, tmin, tmax = [], [], []
+    for l in lines:
+        if "At max CFLZ dz=" in l:
+            cflz.append([float(x) for x in l.split("=")[1].split()])
+        elif "Temp. min at i,j,k" in l:
+            tmin.append([int(x) for x in l.split("k")[-1].split()])
+        elif "Temp. max at i,j,k" in l:
+            tmax.append([int(x) for x in l.split("k")[-1].split()])
+    out["cflz"] = cflz
+    out["tloc"] = [a + b for a, b in zip(tmin, tmax)]
+    return out
+
+
 def run_ref(build, P, fields, nsteps=1, mode=0, probe=None, exe="iwref", isgs=0, start=None, cdt=1.0,
-            write_restart=False):
+            write_restart=False, maxcalc=False):
     """Run the reference build on physical ``fields`` (u,v,w,rho').
 
     ``exe="iwshim"`` runs the same harness linked with the product's Fortran
     shims (the reference step loop calling libiwgpu.so).  ``start`` (bytes
     of a restart file) restarts from it (istart=1) instead; with
     ``write_restart`` the restart file written by output.f at the end is
-    returned as a fourth item.
-    Returns (state[24], t, diag_lines[, restart_bytes])."""
+    returned as a fourth item.  With ``maxcalc`` the reference maxcalc runs
+    every step (adaptive dt included) and the parsed outputs
+    (``_parse_maxcalc``) are returned as the last item.
+    Returns (state[24], t, diag_lines[, restart_bytes][, maxcalc_outputs])."""
     exe = os.path.join(REFDIR, build, exe)
     nz, ny, nxpp = P.nzloc * P.nsubd, P.ny, P.nx + 2
     with tempfile.TemporaryDirectory() as d:
@@ -198,11 +259,13 @@ def run_ref(build, P, fields, nsteps=1, mode=0, probe=None, exe="iwref", isgs=0,
         pr = probe or (1, 1, 1)
         with open(os.path.join(d, "harness.nml"), "w") as f:
             f.write("&harness icfile='ic.bin', outfile='out.bin', diagfile='diag.txt', "
-                    "nrun=%d, mode=%d, iprobe=%d, ipx=%d, kp1=%d, kp2=%d, isgs=%d, iwrite=%d /\n"
-                    % (nsteps, mode, 1 if probe else 0, pr[0], pr[1], pr[2], isgs, 1 if write_restart else 0))
+                    "nrun=%d, mode=%d, iprobe=%d, ipx=%d, kp1=%d, kp2=%d, isgs=%d, iwrite=%d, imaxc=%d /\n"
+                    % (nsteps, mode, 1 if probe else 0, pr[0], pr[1], pr[2], isgs, 1 if write_restart else 0,
+                       1 if maxcalc else 0))
         np.concatenate([np.ascontiguousarray(a, dtype=np.float64).ravel() for a in fields]).tofile(
             os.path.join(d, "ic.bin"))
-        r = subprocess.run([exe], cwd=d, stdout=subprocess.DEVNULL, stderr=subprocess.PIPE)
+        with open(os.path.join(d, "stdout.txt"), "w") as so:
+            r = subprocess.run([exe], cwd=d, stdout=so, stderr=subprocess.PIPE)
         if r.returncode != 0:
             raise RuntimeError("reference run failed: " + r.stderr.decode()[-2000:])
         raw = np.fromfile(os.path.join(d, "out.bin"), dtype=np.float64)
@@ -213,7 +276,10 @@ def run_ref(build, P, fields, nsteps=1, mode=0, probe=None, exe="iwref", isgs=0,
             state = np.concatenate([state, raw[24 * n + 1:33 * n + 1].reshape(9, nz, ny, nxpp)])
         with open(os.path.join(d, "diag.txt")) as f:
             diag = f.read().split("\n")
+        res = [state, t, diag]
         if write_restart:
             with open(os.path.join(d, "restart_out.dat"), "rb") as f:
-                return state, t, diag, f.read()
-    return state, t, diag
+                res.append(f.read())
+        if maxcalc:
+            res.append(_parse_maxcalc(d))
+    return tuple(res)

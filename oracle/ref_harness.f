@@ -6,16 +6,18 @@ c  of the product.
 c
 c  It replays the step sequence of main.f:653-1066 (jmsegret/IWrepo,
 c  mode1_bbl) without main.f's file output:
-c    * maxcalc (various.f:665) is replaced by its state-changing part,
-c      the horfft(+1)/horfft(-1) round trip of u,v,w,temp
-c      (various.f:739-742, 854-857); its adaptive-dt branch
-c      (various.f:911-985) is left out, so dt stays fixed;
+c    * with imaxc=0, maxcalc (various.f:665) is replaced by its
+c      state-changing part, the horfft(+1)/horfft(-1) round trip of
+c      u,v,w,temp (various.f:739-742, 854-857), so dt stays fixed; with
+c      imaxc=1 (builds compiled with -DIW_MAXCALC, where various.f is
+c      part of the build) the reference maxcalc itself runs, adaptive dt
+c      included, and t, dt, cdt after it go to dtlog.txt;
 c    * convec, calcuvwt, interface_avg3d are the reference routines;
-c    * the SS/BDF coefficient selection copies main.f:700-777 and
-c      main.f:813-856 for istart=0 without adaptive dt.
-c  various.f cannot be compiled by flang (REAL subscript in testdump,
-c  various.f:503), so the two one-line helpers that the hot path uses
-c  from it, zero and copy (various.f:2-26), are given below.
+c    * the SS/BDF coefficient selection follows main.f:660-664,
+c      700-777 and 813-856.
+c  Without -DIW_MAXCALC, various.f is not compiled (flang rejects it for
+c  ny=1, various.f:194-204), so the two one-line helpers that the hot
+c  path uses from it, zero and copy (various.f:2-26), are given below.
 c
 c  Modes (namelist &harness, file harness.nml in the working dir):
 c    mode=0  full run: read physical IC, nrun steps, dump state
@@ -48,7 +50,7 @@ c=======================================================================
       include 'comscratch.h'
       include 'comsgs.h'
 
-      logical cpenspecfilter
+      logical cpenspecfilter, ldtadapt
       character*256 icfile, outfile, diagfile
       dimension wrk(nxpp,ny,nz)
 
@@ -68,8 +70,9 @@ c=======================================================================
      >ifilter,nstepfilter,
      >tstart,npostp,itempreset,nrestartdump,n3doutput,i0
       namelist/harness/ icfile, outfile, diagfile, nrun, mode, iprobe,
-     >ipx, kp1, kp2, isgs, iwrite
+     >ipx, kp1, kp2, isgs, iwrite, imaxc
 
+      imaxc = 0
       mode = 0
       isgs = 0
       iwrite = 0
@@ -207,22 +210,37 @@ c-mode 0: full run from a physical-space initial condition
       bdfv = 1.
 
  500  continue
+      if (imaxc.eq.1) open(13,file='dtlog.txt',status='unknown')
       do 1000 istep=1,nrun
+        if (imaxc.eq.1) then
+#ifdef IW_MAXCALC
+          call maxcalc(istep)
+          write(13,'(I8,3(1x,E24.16),2(1x,I8))') istep,t,dt,cdt,
+     >      ivardt,idtstep
+#else
+          write(*,*) 'imaxc=1 needs a build with various.f'
+          stop 2
+#endif
+        else
 c-State-changing part of maxcalc (various.f:739-742, 854-857)
-        call horfft(u,1,nz)
-        call horfft(v,1,nz)
-        call horfft(w,1,nz)
-        call horfft(temp,1,nz)
-        if (iprobe.eq.1) write(12,'(3(1x,E24.16))')
-     >     t,u(ipx,1,kp1),u(ipx,1,kp2)
-        call horfft(u,-1,nz)
-        call horfft(v,-1,nz)
-        call horfft(w,-1,nz)
-        call horfft(temp,-1,nz)
+          call horfft(u,1,nz)
+          call horfft(v,1,nz)
+          call horfft(w,1,nz)
+          call horfft(temp,1,nz)
+          if (iprobe.eq.1) write(12,'(3(1x,E24.16))')
+     >       t,u(ipx,1,kp1),u(ipx,1,kp2)
+          call horfft(u,-1,nz)
+          call horfft(v,-1,nz)
+          call horfft(w,-1,nz)
+          call horfft(temp,-1,nz)
+        endif
+c-main.f:660-664
+        if (ivardt.eq.1) idtstep = idtstep+1
 
         call convec(istep,0,ispecfilternl)
 
-c-main.f:700-777 (fixed dt)
+c-main.f:700-777
+        ldtadapt = ((ivardt.eq.1).and.(idtstep.lt.2))
         if (istep.lt.2) then
           if (istart.eq.0) then
             fac1 = 2.
@@ -242,13 +260,23 @@ c-main.f:700-777 (fixed dt)
             bdf2 = bdfvar2(dt1,dt2,dt3)
             bdf3 = bdfvar3(dt1,dt2,dt3)
           endif
-        else
+        else if (.not.ldtadapt) then
           fac1 = 3.
           fac2 = -3.
           fac3 = 1.
           bdf1 = 3.
           bdf2 = -1.5
           bdf3 = 1./3.
+        else
+          dt1 = dt
+          dt2 = dt
+          dt3 = cdt*dt
+          fac1 = abvar1(dt1,dt2,dt3)
+          fac2 = abvar2(dt1,dt2,dt3)
+          fac3 = abvar3(dt1,dt2,dt3)
+          bdf1 = bdfvar1(dt1,dt2,dt3)
+          bdf2 = bdfvar2(dt1,dt2,dt3)
+          bdf3 = bdfvar3(dt1,dt2,dt3)
         endif
 
         call copy(u,su,ntot)
@@ -260,15 +288,20 @@ c-main.f:700-777 (fixed dt)
      >   (mod(istep,nspecfilterexp).eq.0))
         call calcuvwt(izmode,ispecfilter,irelax,cpenspecfilter)
 
-c-main.f:815-856
+c-main.f:813-856
+        ldtadapt = ((ivardt.eq.1).and.(idtstep.lt.2))
         if (istep.lt.2) then
           if (istart.eq.0) then
             bdfv = 1.5
           else
             bdfv = bdfvarv(dt,dt,cdt*dt)
           endif
-        else
+        else if (.not.ldtadapt) then
           bdfv = 11./6.
+        else
+          bdfv = bdfvarv(dt,dt,cdt*dt)
+          cdt = 1.
+          ivardt = 0
         endif
 
 c-main.f:880-895
@@ -314,6 +347,7 @@ c-main.f:880-895
       close(11)
       end
 
+#ifndef IW_MAXCALC
 c-----------------------------------------------------------------------
 c  zero / copy: the two helpers of various.f:2-26 used on the hot path
 c-----------------------------------------------------------------------
@@ -332,3 +366,4 @@ c-----------------------------------------------------------------------
       enddo
       return
       end
+#endif

@@ -8,6 +8,11 @@ Run in a container where /root/reference is mounted.  Each fixture is data only
   on the G1 grid: 10 subdomains (output:78-88), nzloc = 25 (output:59),
   epsilon = 1e-2, Dirichlet/Dirichlet, right-hand side -1 (testhelm.f:59-99),
   facdir = 1, facrobin = 1e4 (output:1-48).
+* g1_run.json -- G1: the 400-step 2-D forced run logged in ``output``: its
+  namelists (output:2-48), subdomains (output:78-88), the per-step maxcalc
+  lines (istep-1, tmin, tmax, ktmin, ktmax; the temperature extremum
+  locations; dz and CFLZ at the maximum, 16 digits) and the adaptive-dt
+  events (dtchange.dat:1-18: time, new dt).
 * g3_run.json -- G3: the last 9-step run of the shipped configuration
   (dim.h 256x1, 12x25; input.dat; subd.dat): u at two points after each step
   (uveltest.dat:1041-1049), interface-average fractions (avgint.dat:2-9) and
@@ -62,7 +67,42 @@ def g3():
             "probe": probe, "avgint": avg, "z_grid": zg}
 
 
+def g1():
+    with open(os.path.join(REF, "output")) as f:
+        lines = f.read().split("\n")
+    params = {}
+    for l in lines[1:48]:
+        m = re.match(r"^\s*([A-Z0-9]+)\s*=\s*([-+0-9.Ee]+)", l)
+        if m:
+            params[m.group(1).lower()] = float(m.group(2))
+    sub = []
+    for l in lines[77:88]:
+        m = re.match(r"\s+(\d+)\s+(\S+)\s+(\S+)\s*$", l)
+        if m:
+            sub.append([float(m.group(2)), float(m.group(3))])
+    grid = re.findall(r"\d+", lines[58])  # output:59 grid points in x and z
+    minmax, tmin, tmax, cflz = [], [], [], []
+    for l in lines:
+        m = re.match(r"^\s+(\d+)\s+(\S+E[+-]\d\d)\s+(\S+E[+-]\d\d)\s+(\d+)\s+(\d+)\s*$", l)
+        if m:
+            minmax.append([float(m.group(i)) for i in range(1, 6)])
+        elif "Temp. min at i,j,k" in l:
+            tmin.append([int(x) for x in l.split("k")[-1].split()])
+        elif "Temp. max at i,j,k" in l:
+            tmax.append([int(x) for x in l.split("k")[-1].split()])
+        elif "At max CFLZ dz=" in l:
+            cflz.append([float(x) for x in l.split("=")[1].split()])
+    with open(os.path.join(REF, "dtchange.dat")) as f:
+        events = [[float(x) for x in l.split()] for l in f if l.strip()]
+    return {"source": "output:2-48, 59, 78-88 and its per-step maxcalc lines (various.f:862-885); "
+                      "dtchange.dat:1-18",
+            "nx": int(grid[0]), "nz": int(grid[1]), "nzloc": 25, "params": params, "subdomains": sub,
+            "minmax": minmax, "tmin_loc": tmin, "tmax_loc": tmax, "cflz": cflz, "dt_events": events}
+
+
 if __name__ == "__main__":
+    with open(os.path.join(HERE, "g1_run.json"), "w") as f:
+        json.dump(g1(), f, indent=0)
     with open(os.path.join(HERE, "g2_testhelm.json"), "w") as f:
         json.dump(g2(), f, indent=0)
     with open(os.path.join(HERE, "g3_run.json"), "w") as f:

@@ -40,6 +40,24 @@ def g3_params():
     return P, d
 
 
+def g1_params(dtadapt=2):
+    """G1: the logged 400-step 2-D forced run (tests/golden/g1_run.json)."""
+    d = json.load(open(os.path.join(GOLDEN, "g1_run.json")))
+    q = d["params"]
+    sub = d["subdomains"]
+    P = make_params(nx=d["nx"], ny=1, nsubd=len(sub), order=d["nzloc"] - 1, xlen=q["xlen"], ylen=q["ylen"],
+                    zlen=q["zlen"], re=1.0 / q["xnu"], pr=q["xnu"] / q["xkappa"], n2=q["brunt"] ** 2,
+                    dt=q["dtin"], amp=q["amp"], p=q["p"], pf=q["pf"], fackc=q["fackc"], facdir=q["facdir"],
+                    facneumn=q["facneumn"], facrobin=q["facrobin"], navgintrfc=int(q["navgintrfc"]),
+                    zcen=q["zcen"], xbcrit=q["xbcrit"], xacrit=q["xacrit"], umax=q["umax"], fwave=8.0,
+                    heights=[h for _, h in sub], grav=q["grav"], rho0=q["rho0"], fackcbv=q["fackcbv"],
+                    dtadapt=dtadapt)
+    P.brunt = q["brunt"]
+    for k, (z0, _) in enumerate(sub):
+        P.z0[k] = z0
+    return P, d
+
+
 def g2_params():
     d = json.load(open(os.path.join(GOLDEN, "g2_testhelm.json")))
     sub = d["subdomains"]

@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 from numpy.polynomial import legendre as L
 
-from helpers import SOLUTION, STATE, case, g2_params, g3_params, rel, routine_inputs
+from helpers import SOLUTION, STATE, case, g1_params, g2_params, g3_params, rel, routine_inputs
 from iwrepo_amd.params import make_params
 
 # ---------------------------------------------------------------- GLL tables
@@ -364,3 +364,72 @@ def test_oracle_restart_matches_compiled_reference(oracle_mod, cdt):
     assert abs(o.time() - tr) < 1e-15
     ref_counts = np.array([[int(v) for v in l.split()[1:]] for l in diag if l.strip()])
     assert np.array_equal(counts, ref_counts)
+
+
+# ------------------------------------------- maxcalc and adaptive dt (row f2)
+
+
+def test_maxcalc_adaptive_dt_matches_compiled_reference(oracle_mod):
+    """c1 for 50 steps with the reference's own maxcalc (various.f:665-987,
+    Tso kept between calls as the static-local build keeps it): two dt
+    changes (steps 22 and 43), the variable-step coefficients around them,
+    the CFL/extremum records and the state."""
+    name, n = "c1", 50
+    if not oracle_mod.ref_available(name) or not oracle_mod.ref_has_maxcalc(name):
+        pytest.skip("reference build with various.f not present (needs /root/reference)")
+    P, flds = case(name)
+    P.dtadapt = 1
+    o = oracle_mod.Oracle(P)
+    o.set_initial(*flds)
+    o.run(n)
+    rec = o.maxcalc_records()
+    st, t, _, mc = oracle_mod.run_ref(name, P, flds, nsteps=n, maxcalc=True)
+    dl = np.array(mc["dtlog"])
+    assert [int(r[0]) for r in rec if r[37] == 1] == [22, 43]
+    assert [int(r[0]) for r in dl if r[4] == 1 and r[5] == 0] == [22, 43]
+    # t, dt, cdt after maxcalc (printed E24.16) and the /vardtflags/
+    assert np.abs(dl[:, 1:4] - rec[:, [1, 35, 36]]).max() < 1e-15
+    assert np.array_equal(dl[:, 4:6], rec[:, 38:40])
+    cz = np.array(mc["cflz"])
+    assert np.abs(cz[:, 0] - rec[:, 34]).max() == 0.0             # dz at the CFLZ maximum
+    assert rel(rec[:, 30], cz[:, 1]) < 1e-13                      # cflzmax, 16 digits
+    f50, f51 = np.array(mc["fort50"]), np.array(mc["fort51"])    # E12.6 records
+    assert np.array_equal(f50[:, 0], rec[:, 0])
+    assert np.abs(f50[:, 1:] - rec[:, 2:18]).max() < 5e-6
+    assert (np.abs(f51[:, [1, 5, 9]] - rec[:, [22, 26, 30]]) <= 5.1e-6 * rec[:, [22, 26, 30]]).all()
+    assert np.array_equal(f51[:, [2, 3, 4, 6, 7, 8, 10, 11, 12]], rec[:, [23, 24, 25, 27, 28, 29, 31, 32, 33]])
+    assert np.array_equal(np.array(mc["tloc"]), rec[:, [18, 19, 16, 20, 21, 17]])
+    assert np.abs(np.array(mc["uvel"])[:, 1:] - rec[:, 40:42]).max() < 1e-14
+    mine = o.state()
+    for q in SOLUTION:
+        assert rel(mine[q], st[q]) < (1e-11 if q < 12 else 1e-9), STATE[q]
+    assert abs(t - o.time()) < 1e-15
+
+
+def test_g1_adaptive_dt_events_match_shipped_log(oracle_mod):
+    """G1, the reference's logged 400-step forced 2-D run: its 18 dt changes
+    (dtchange.dat: 14 increases by 5/4 while CFLZ < 0.1, then four CFL-driven
+    decreases) are reproduced with Tso read as zero at every call, which is
+    what the logged times require (each change comes at the first step with
+    int(t/dt) > 20).  The log's CFLZ maxima (16 digits) agree to 2e-4: the
+    log came from an earlier, unrecorded revision of the sources."""
+    P, d = g1_params(dtadapt=2)
+    o = oracle_mod.Oracle(P)
+    zero = np.zeros(o.shape)
+    o.set_initial(zero, zero, zero, zero)
+    n = len(d["cflz"])
+    o.run(n)
+    rec = o.maxcalc_records()
+    ev = np.array([[r[1], r[35]] for r in rec if r[37] == 1])
+    gold = np.array(d["dt_events"])
+    assert ev.shape == gold.shape
+    assert np.abs(ev - gold).max() < 1e-12 * np.abs(gold).max()
+    cz = np.array(d["cflz"])
+    assert np.abs(cz[:, 0] - rec[:, 34]).max() < 1e-15            # dz at the maximum, incl. t = 0
+    assert cz[0, 1] == rec[0, 30] == 0.0
+    assert (np.abs(cz[1:, 1] - rec[1:, 30]) / cz[1:, 1]).max() < 5e-4
+    mm = np.array(d["minmax"])
+    assert np.array_equal(mm[:, 0], rec[:, 0] - 1)
+    assert np.abs(mm[:, 1:3] - rec[:, 14:16]).max() < 1e-4         # E12.6 of values near -19.39 and 1
+    assert np.array_equal(mm[:, 3:5], rec[:, 16:18])
+    assert np.array_equal(np.array(d["tmin_loc"]), rec[:, [18, 19, 16]])
