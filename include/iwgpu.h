@@ -81,13 +81,39 @@ int iw_get_coeffs(iw_solver *s, double *t, double fac[3], double bdf[3], double 
 int iw_set_initial(iw_solver *s, const double *u, const double *v,
                    const double *w, const double *temp);
 
-/* Device-resident step loop replaying main.f:653-1066 for nsteps steps with
- * fixed dt: maxcalc round trip (various.f:739-742, 854-857), convec,
- * SS/BDF coefficient selection, calcuvwt, interface_avg3d.  counts (may be
- * NULL) receives 4 interface-average counts per step (icu,icv,icw,ict). */
+/* Device-resident step loop replaying main.f:653-1066 for nsteps steps:
+ * maxcalc (its round trip; with imaxcalc its records, see iw_maxcalc; with
+ * dtadapt the adaptive step), convec, SS/BDF coefficient selection with the
+ * variable-step branches after a restart or a change of dt, calcuvwt, the
+ * estimation filter, interface_avg3d.  counts (may be NULL) receives 4
+ * interface-average counts per step (icu,icv,icw,ict). */
 int iw_run(iw_solver *s, int nsteps, int *counts);
 double iw_time(iw_solver *s);
 int iw_sync(iw_solver *s);
+
+/* various.f:665  subroutine maxcalc(istep), called by main.f:657 every step:
+ * on the device state, the physical-space extrema and CFL maxima in the
+ * reference's loop order (a later point wins a tie), the horfft round trip
+ * of u,v,w,temp, and, with dtadapt, the adaptive step of various.f:887-985
+ * (new dt, cdt, variable-step coefficients, ivardt = 1, idtstep = 0).
+ * rec (may be NULL) receives IW_MAXCALC_NREC doubles, indices 1-based as the
+ * reference prints them (0 = none): 0 istep, 1 t, 2 umin, 3 umaxt, 4 kumin,
+ * 5 kumax, 6 vmin, 7 vmax, 8 kvmin, 9 kvmax, 10 wmin, 11 wmax, 12 kwmin,
+ * 13 kwmax, 14 tmin, 15 tmax, 16 ktmin, 17 ktmax, 18 itmin, 19 jtmin,
+ * 20 itmax, 21 jtmax, 22 cflxmax, 23-25 its i,j,k, 26 cflymax, 27-29 i,j,k,
+ * 30 cflzmax, 31-33 i,j,k, 34 dz at the CFLZ maximum, 35 dt after the
+ * decision, 36 cdt, 37 1 if dt changed, 38 ivardt, 39 idtstep,
+ * 40 u(32,1,10), 41 u(32,1,50) (physical, the uveltest.dat probe; NaN when
+ * that position lies outside u). */
+#define IW_MAXCALC_NREC 42
+int iw_maxcalc(iw_solver *s, int istep, double *rec);
+/* The maxcalc records of the steps of the last iw_run (imaxcalc or dtadapt
+ * set): n records of IW_MAXCALC_NREC doubles. */
+int iw_maxcalc_records(iw_solver *s, double *out, int n);
+/* dt, cdt of /timevars/ and ivardt, idtstep of /vardtflags/ (comtime.h:1-3),
+ * which main.f:660-664, 838-854 change between maxcalc calls */
+int iw_get_dtstate(iw_solver *s, double *dt, double *cdt, int *ivardt, int *idtstep);
+int iw_set_dtstate(iw_solver *s, double cdt, int ivardt, int idtstep);
 
 /* ---- reference routines on host arrays (parity / Fortran-shim mode) ---- */
 /* horfft.f:2    subroutine horfft(b,is,nk): is=-1 physical->spectral */

@@ -831,6 +831,41 @@ extern "C" int iw_create(const iw_params *p, iw_solver **out)
     HIPCHK(hipMemset(s->wviscrot, 0, 2 * sizeof(double) * g.nxppy));
     HIPCHK(hipMalloc((void **)&s->d_count, 4 * sizeof(int)));
     HIPCHK(hipHostMalloc((void **)&s->h_count, 4 * sizeof(int)));
+    {   // maxcalc tables (various.f:696-700, 790-822; forcing.f:115-141)
+        if ((double)P.nx * P.ny * g.nz >= 2147483647.) return fail("grid too large for maxcalc positions");
+        const double pi2 = 8. * atan(1.0);
+        s->mc_dx = (pi2 / (pi2 / P.xlen)) / P.nx;
+        s->mc_dy = (pi2 / (pi2 / P.ylen)) / P.ny;
+        const double xkc = pi2 / P.xlen, xmc = pi2 / (2. * P.zlen);
+        s->xomega = (P.brunt * xkc) / sqrt(xkc * xkc + xmc * xmc); // main.f:341-346
+        const double rhograd = -(P.rho0 * (P.brunt * P.brunt)) / P.grav;
+        std::vector<double> rm(g.nz), dz(g.nz);
+        // zf as setup.f:150-161 fills it (the upper subdomain writes an
+        // interface point last); zf(0) is the word before zf in
+        // /comgridptxyz/, z(nz)
+        const int nl = P.nzloc, ns = P.nsubd;
+        std::vector<double> zf(ns * (nl - 1) + 2);
+        zf[0] = s->z[g.nz - 1];
+        for (int ks = 0; ks < ns; ks++)
+            for (int kl = 0; kl < nl; kl++) zf[ks * (nl - 1) + kl + 1] = s->z[ks * nl + kl];
+        for (int ks = 0; ks < ns; ks++)
+            for (int kl = 0; kl < nl; kl++) {
+                const int k = ks * nl + kl, kg = ks * (nl - 1) + kl + 1;
+                rm[k] = 1. + rhograd * s->z[k];
+                dz[k] = zf[kg] - zf[kg - 1];
+            }
+        // the top point has no CFLZ and keeps the dz of the point below
+        // it (various.f:817-822)
+        if (g.nz > 1) dz[g.nz - 1] = dz[g.nz - 2];
+        if (upload(&s->mc_rm, rm, acc) || upload(&s->mc_dz, dz, acc)) return -1;
+        const int nxy = P.nx * P.ny;
+        HIPCHK(hipMalloc((void **)&s->mc_part, sizeof(double) * 2 * kMcQ * (size_t)mc_blocks(g)));
+        HIPCHK(hipMalloc((void **)&s->mc_rec, sizeof(double) * kMcRec));
+        HIPCHK(hipMemset(s->mc_rec, 0, sizeof(double) * kMcRec));
+        HIPCHK(hipMalloc((void **)&s->mc_colmax, sizeof(double) * (size_t)nxy));
+        HIPCHK(hipMalloc((void **)&s->mc_gmax, sizeof(double) * (size_t)((nxy + 255) / 256)));
+        HIPCHK(hipHostMalloc((void **)&s->h_mc, sizeof(double) * kMcRec));
+    }
     acc += (long long)fb * (24 + 9 + 7 + 2) + 4 * sizeof(double) * g.nxppy;
     s->sys.kl = P.nzloc;
     s->sys.kv = 2 * P.nzloc;
@@ -872,6 +907,14 @@ extern "C" int iw_destroy(iw_solver *s)
     hipFree(s->wviscrot);
     hipFree(s->d_count);
     hipHostFree(s->h_count);
+    hipFree(s->mc_rm);
+    hipFree(s->mc_dz);
+    hipFree(s->mc_part);
+    hipFree(s->mc_rec);
+    hipFree(s->mc_colmax);
+    hipFree(s->mc_gmax);
+    hipFree(s->mc_ring);
+    hipHostFree(s->h_mc);
     hipFree(s->sys.AB);
     hipFree(s->sys.ipiv);
     hipFree(s->sys.coef);
@@ -1022,13 +1065,17 @@ static FieldBatch batch(std::initializer_list<double *> l)
 // horfft (horfft.f:2-69) over nk logical planes of a batch: the first pass
 // reads b.f (through smap) and writes b.g (through dmap), the second pass
 // works in place on b.g.  is = -1: physical -> spectral.
-static void fft_dev(iw_solver *s, FieldBatch b, int is, int nk, bool iface = false)
+enum FftUse { kFftMain = 0, kFftIface = 1, kFftMaxcalc = 2 };
+
+static void fft_dev(iw_solver *s, FieldBatch b, int is, int nk, int use = kFftMain)
 {
     if (b.n == 0 || nk == 0) return;
-    // timing scopes: the interface-plane transforms are kept apart from the
-    // whole-field ones
-    const char *nxf = iface ? "iface_fft_x_fwd" : "fft_x_fwd", *nyf = iface ? "iface_fft_y_fwd" : "fft_y_fwd";
-    const char *nxi = iface ? "iface_fft_x_inv" : "fft_x_inv", *nyi = iface ? "iface_fft_y_inv" : "fft_y_inv";
+    // timing scopes: the interface-plane and maxcalc transforms are kept
+    // apart from the others
+    static const char *names[3][4] = {{"fft_x_fwd", "fft_y_fwd", "fft_x_inv", "fft_y_inv"},
+                                      {"iface_fft_x_fwd", "iface_fft_y_fwd", "iface_fft_x_inv", "iface_fft_y_inv"},
+                                      {"mc_fft_x_fwd", "mc_fft_y_fwd", "mc_fft_x_inv", "mc_fft_y_inv"}};
+    const char *nxf = names[use][0], *nyf = names[use][1], *nxi = names[use][2], *nyi = names[use][3];
     const double bx = 16. * (double)s->g.nxppy * nk * b.n;         // each double read + written
     const double by = 32. * (double)s->g.nxh * s->g.ny * nk * b.n; // columns ii < nxh
     FieldBatch b2 = b;
@@ -1278,7 +1325,7 @@ static int iface_dev(iw_solver *s, double *const *f, int nf)
         bi.f[q] = f[q];
         bi.g[q] = s->icomp + (size_t)q * 4 * nint * s->g.nxppy;
     }
-    fft_dev(s, bi, 1, 4 * nint, true);
+    fft_dev(s, bi, 1, 4 * nint, kFftIface);
     {
         Scope sc(s, "iface", 48. * (double)s->g.nxppy * nint * nf);
         FieldBatch b;
@@ -1296,7 +1343,116 @@ static int iface_dev(iw_solver *s, double *const *f, int nf)
         bf.f[q] = bi.g[q];
         bf.g[q] = f[q];
     }
-    fft_dev(s, bf, -1, 2 * nint, true);
+    fft_dev(s, bf, -1, 2 * nint, kFftIface);
+    return 0;
+}
+
+// maxcalc (various.f:665-987) on the device state: physical u,v,w,temp
+// (horfft +1) into the physical scratch fields, the extrema and CFL record
+// (s->mc_rec), and the return to spectral space, whose only effect on the
+// spectrum is on the kx=0 and kx-Nyquist columns
+static void maxcalc_dev(iw_solver *s, int istep)
+{
+    const Geo &g = s->g;
+    FieldBatch b = batch({s->buf[0], s->buf[1], s->buf[2], s->buf[3]});
+    for (int q = 0; q < 4; q++) b.g[q] = s->phys[q];
+    fft_dev(s, b, 1, g.nz, kFftMaxcalc);
+    {
+        Scope sc(s, "maxcalc", 32. * (double)g.ntot);
+        McArgs A;
+        A.u = s->phys[0]; A.v = s->phys[1]; A.w = s->phys[2]; A.t = s->phys[3];
+        A.rm = s->mc_rm; A.dz = s->mc_dz;
+        A.dt = s->P.dt; A.dx = s->mc_dx; A.dy = s->mc_dy;
+        A.nx = g.nx; A.ny = g.ny; A.nz = g.nz; A.nxpp = g.nxpp; A.nxppy = g.nxppy;
+        A.part = s->mc_part;
+        launch_maxcalc(g, A, istep, s->t, s->P.umax, s->mc_colmax, s->mc_gmax, s->mc_rec, s->stream);
+    }
+    project_dev(s, s->buf, 4);
+}
+
+// variable-step AB3/BDF3 coefficients for step sizes (d1, d2, d3)
+static void var_coeffs(iw_solver *s, double d1, double d2, double d3)
+{
+    s->fac[0] = iw_abvar1(d1, d2, d3); s->fac[1] = iw_abvar2(d1, d2, d3); s->fac[2] = iw_abvar3(d1, d2, d3);
+    s->bdf[0] = iw_bdfvar1(d1, d2, d3); s->bdf[1] = iw_bdfvar2(d1, d2, d3); s->bdf[2] = iw_bdfvar3(d1, d2, d3);
+}
+
+// adaptive step (various.f:887-985) from the record on the host; fills
+// the record's dt, cdt, change flag, ivardt, idtstep
+static void adapt_dt(iw_solver *s, double *rec)
+{
+    int changed = 0;
+    if (s->P.dtadapt) {
+        const double cflx = rec[22], cflz = rec[30];
+        const bool change_dt = (cflz > 0.7) || (cflz < 0.1) || (cflx > 0.16);
+        const double pi2 = 8. * atan(1.0);
+        const double tlim = pi2 / (s->xomega * 50.);
+        const double tso = s->P.dtadapt == 2 ? 0. : s->Tso;
+        const int nst = (int)((s->t - tso) / s->P.dt);
+        if (nst > 20 && change_dt) {
+            s->Tso = s->t;
+            if (cflz < 0.1) s->cdt = fmax(0.8, s->P.dt / tlim);
+            if ((cflz > 0.7) || (cflx > 0.16)) s->cdt = 1.25;
+            s->P.dt = s->P.dt / s->cdt;
+            const double dt = s->P.dt, d2 = s->cdt * dt;
+            var_coeffs(s, dt, d2, d2);
+            s->bdfv = iw_bdfvarv(dt, d2, d2);
+            s->ivardt = 1;
+            s->idtstep = 0;
+            changed = 1;
+        }
+    }
+    rec[35] = s->P.dt;
+    rec[36] = s->cdt;
+    rec[37] = changed;
+    rec[38] = s->ivardt;
+    rec[39] = s->idtstep;
+}
+
+// maxcalc record to the host (waits for the stream)
+static int mc_fetch(iw_solver *s)
+{
+    HIPCHK(hipMemcpyAsync(s->h_mc, s->mc_rec, sizeof(double) * kMcRec, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return 0;
+}
+
+extern "C" int iw_maxcalc(iw_solver *s, int istep, double *rec)
+{
+    if (!s) return fail("null solver");
+    maxcalc_dev(s, istep);
+    HIPCHK(hipGetLastError());
+    if (mc_fetch(s)) return -1;
+    adapt_dt(s, s->h_mc);
+    if (rec) memcpy(rec, s->h_mc, sizeof(double) * kMcRec);
+    return 0;
+}
+
+extern "C" int iw_maxcalc_records(iw_solver *s, double *out, int n)
+{
+    if (!s || (n > 0 && !out)) return fail("iw_maxcalc_records: null argument");
+    if (n < 0 || (size_t)n * kMcRec > s->mc_hist.size())
+        return fail("iw_maxcalc_records: more records asked for than the last iw_run produced");
+    memcpy(out, s->mc_hist.data(), sizeof(double) * kMcRec * (size_t)n);
+    return 0;
+}
+
+extern "C" int iw_get_dtstate(iw_solver *s, double *dt, double *cdt, int *ivardt, int *idtstep)
+{
+    if (!s) return fail("null solver");
+    if (dt) *dt = s->P.dt;
+    if (cdt) *cdt = s->cdt;
+    if (ivardt) *ivardt = s->ivardt;
+    if (idtstep) *idtstep = s->idtstep;
+    return 0;
+}
+
+extern "C" int iw_set_dtstate(iw_solver *s, double cdt, int ivardt, int idtstep)
+{
+    if (!s) return fail("null solver");
+    s->cdt = cdt;
+    s->ivardt = ivardt;
+    s->idtstep = idtstep;
     return 0;
 }
 
@@ -1611,30 +1767,59 @@ extern "C" int iw_get_sgs(iw_solver *s, int idx, double *host)
 extern "C" int iw_run(iw_solver *s, int nsteps, int *counts)
 {
     if (!s || nsteps < 0) return fail("iw_run: bad argument");
+    const bool mc = s->P.imaxcalc || s->P.dtadapt;
+    const bool adapt = s->P.dtadapt != 0;
+    s->mc_hist.clear();
+    if (mc && !adapt && nsteps > s->mc_ring_cap) { // device ring of the step records
+        hipFree(s->mc_ring);
+        s->mc_ring = nullptr;
+        s->mc_ring_cap = 0;
+        HIPCHK(hipMalloc((void **)&s->mc_ring, sizeof(double) * kMcRec * (size_t)nsteps));
+        s->mc_ring_cap = nsteps;
+    }
     for (int n = 0; n < nsteps; n++) {
         int istep = ++s->istep;
-        // maxcalc's horfft round trip (various.f:739-742, 854-857) changes the
-        // spectrum only through the kx=0 and kx-Nyquist columns
-        project_dev(s, s->buf, 4);
+        if (mc) { // main.f:657
+            maxcalc_dev(s, istep);
+            if (adapt) { // the decision needs the record on the host
+                if (mc_fetch(s)) return -1;
+                adapt_dt(s, s->h_mc);
+                s->mc_hist.insert(s->mc_hist.end(), s->h_mc, s->h_mc + kMcRec);
+            } else {
+                HIPCHK(hipMemcpyAsync(s->mc_ring + (size_t)n * kMcRec, s->mc_rec, sizeof(double) * kMcRec,
+                                      hipMemcpyDeviceToDevice, s->stream));
+            }
+        } else {
+            // maxcalc's horfft round trip (various.f:739-742, 854-857) changes
+            // the spectrum only through the kx=0 and kx-Nyquist columns
+            project_dev(s, s->buf, 4);
+        }
+        if (s->ivardt == 1) s->idtstep++; // main.f:660-664
         if (convec_dev(s, s->P.ispecfilternl)) return -1;
+        const bool ldtadapt = s->ivardt == 1 && s->idtstep < 2;
         if (istep < 2 && s->istart == 0) { // main.f:700-777
             s->fac[0] = 2.; s->fac[1] = -1.; s->fac[2] = 0.;
             s->bdf[0] = 2.; s->bdf[1] = -0.5; s->bdf[2] = 0.;
-        } else if (istep < 2) { // restart: variable-step AB3/BDF3 (main.f:719-734)
-            const double dt = s->P.dt, d3 = s->cdt * dt;
-            s->fac[0] = iw_abvar1(dt, dt, d3); s->fac[1] = iw_abvar2(dt, dt, d3); s->fac[2] = iw_abvar3(dt, dt, d3);
-            s->bdf[0] = iw_bdfvar1(dt, dt, d3); s->bdf[1] = iw_bdfvar2(dt, dt, d3);
-            s->bdf[2] = iw_bdfvar3(dt, dt, d3);
+        } else if (istep < 2 || ldtadapt) {
+            // variable-step AB3/BDF3 after a restart (main.f:719-734) or on
+            // the step after a change of dt (main.f:757-776)
+            const double dt = s->P.dt;
+            var_coeffs(s, dt, dt, s->cdt * dt);
         } else {
             s->fac[0] = 3.; s->fac[1] = -3.; s->fac[2] = 1.;
             s->bdf[0] = 3.; s->bdf[1] = -1.5; s->bdf[2] = 1. / 3.;
         }
         int cpen = (s->P.ispecfilterexp == 1) && (istep % s->P.nspecfilterexp == 0);
         if (calcuvwt_dev(s, cpen)) return -1;
-        if (istep >= 2) // main.f:815-856
-            s->bdfv = 11. / 6.;
-        else
+        if (istep < 2) { // main.f:813-856
             s->bdfv = (s->istart == 0) ? 1.5 : iw_bdfvarv(s->P.dt, s->P.dt, s->cdt * s->P.dt);
+        } else if (!(s->ivardt == 1 && s->idtstep < 2)) {
+            s->bdfv = 11. / 6.;
+        } else {
+            s->bdfv = iw_bdfvarv(s->P.dt, s->P.dt, s->cdt * s->P.dt);
+            s->cdt = 1.;
+            s->ivardt = 0;
+        }
         if (s->P.ifilter == 1 && s->P.nstepfilter > 0 && istep % s->P.nstepfilter == 0) // main.f:880-895
             if (filtering_dev(s, s->P.npostp > 0 && istep % s->P.npostp == 0)) return -1;
         s->t = s->t + s->P.dt;
@@ -1654,6 +1839,13 @@ extern "C" int iw_run(iw_solver *s, int nsteps, int *counts)
         }
     }
     HIPCHK(hipGetLastError());
+    if (mc && !adapt && nsteps > 0) { // records of a fixed-dt run
+        s->mc_hist.resize((size_t)nsteps * kMcRec);
+        HIPCHK(hipMemcpyAsync(s->mc_hist.data(), s->mc_ring, sizeof(double) * kMcRec * (size_t)nsteps,
+                              hipMemcpyDeviceToHost, s->stream));
+        HIPCHK(hipStreamSynchronize(s->stream));
+        for (int n = 0; n < nsteps; n++) adapt_dt(s, s->mc_hist.data() + (size_t)n * kMcRec);
+    }
     return 0;
 }
 

@@ -73,6 +73,20 @@ struct SysTables {
     size_t cap_sys = 0;
 };
 
+// maxcalc reductions (various.f:749-851): 11 (value, position) quantities,
+// the record layout of iw_maxcalc (iwgpu.h)
+constexpr int kMcQ = 11;
+constexpr int kMcRec = 42;
+struct McArgs {
+    const double *u, *v, *w, *t; // physical fields
+    const double *rm;            // nz: rhomean(z(k)) (forcing.f:134-141)
+    const double *dz;            // nz: dz of the CFLZ (various.f:818-822)
+    double dt, dx, dy;
+    int nx, ny, nz, nxpp;
+    long nxppy;
+    double *part;                // mc_blocks * 2 kMcQ partial results
+};
+
 struct Prof {
     std::vector<std::string> names;
     std::vector<double> ms;
@@ -111,6 +125,16 @@ struct iw_solver {
     iw::SysTables sys;
     // host copies of tables
     std::vector<double> zpts, wg, d, d2, alpol, xw, xsq, yw, ysq, z;
+    // maxcalc: tables, partials, the record (device and pinned host), the
+    // records of the last iw_run, Tso and /vardtflags/ (comtime.h:3)
+    double *mc_rm = nullptr, *mc_dz = nullptr, *mc_part = nullptr, *mc_rec = nullptr;
+    double *mc_colmax = nullptr, *mc_gmax = nullptr, *h_mc = nullptr;
+    double *mc_ring = nullptr;
+    int mc_ring_cap = 0;
+    double mc_dx = 0, mc_dy = 0, xomega = 0;
+    std::vector<double> mc_hist;
+    double Tso = 0.;
+    int ivardt = 0, idtstep = 0;
     // profiling
     bool profiling = false;
     iw::Prof prof;
@@ -120,6 +144,9 @@ struct iw_solver {
 
 namespace iw {
 // kernels launchers (iw_kernels.hip)
+int mc_blocks(const Geo &g);
+void launch_maxcalc(const Geo &g, const McArgs &A, int istep, double t, double umax, double *colmax,
+                    double *groupmax, double *rec, hipStream_t st);
 void launch_fft_x(const Geo &g, FieldBatch b, int nk, int dir, hipStream_t st);
 void launch_fft_y(const Geo &g, FieldBatch b, int nk, int dir, hipStream_t st);
 void launch_dpdz(const Geo &g, const double *a, double *b, hipStream_t st);

@@ -1,0 +1,337 @@
+// maxcalc (jmsegret/IWrepo mode1_bbl, various.f:665-987) on the device: the
+// running extrema and CFL maxima of the physical u, v, w, temp.
+//
+// The reference scans i (outer), j, then z (inner) and keeps a later point
+// on a tie (.ge./.le. comparisons, various.f:755-845).  With L = (i ny + j)
+// nz + k that is: the extreme value, at the largest L among the points that
+// reach it, provided it passes the start value (0, or 1 for tmin).  That
+// makes every quantity but one an ordinary (value, L) reduction.  The
+// exception is umaxt (various.f:755-758): a new maximum stores u + umax, so
+// later points are compared against the last record plus umax.  With umax = 0
+// this is again a plain maximum; otherwise the chain of records is followed
+// exactly by one work-group walking the columns (i,j) in L order, skipping
+// columns and groups of 256 columns whose maximum cannot start a record.
+#include "iw_internal.h"
+
+#include <cfloat>
+
+namespace iw {
+
+namespace {
+
+constexpr int kMcE = 2048;   // plane elements per block (8 per thread)
+constexpr int kMcThreads = 256;
+
+// quantities: 0 umin 1 umax 2 vmin 3 vmax 4 wmin 5 wmax 6 tmin 7 tmax
+// 8 cflx 9 cfly 10 cflz; even indices below 8 are minima
+__device__ __forceinline__ bool mc_is_min(int q) { return q < 8 && (q & 1) == 0; }
+
+// does (x, lx) replace (b, lb)?  NaN never replaces (Fortran comparisons
+// with NaN are false)
+__device__ __forceinline__ bool mc_better(bool mn, double x, int lx, double b, int lb)
+{
+    return mn ? (x < b || (x == b && lx > lb)) : (x > b || (x == b && lx > lb));
+}
+
+struct McAcc {
+    double v[kMcQ];
+    int l[kMcQ];
+    __device__ void init()
+    {
+#pragma unroll
+        for (int q = 0; q < kMcQ; q++) {
+            v[q] = mc_is_min(q) ? DBL_MAX : -DBL_MAX;
+            l[q] = -1;
+        }
+    }
+    __device__ __forceinline__ void add(int q, double x, int lx)
+    {
+        if (mc_better(mc_is_min(q), x, lx, v[q], l[q])) {
+            v[q] = x;
+            l[q] = lx;
+        }
+    }
+};
+
+// combine the accumulators of a work-group of kMcThreads threads; the result
+// is valid in thread 0
+__device__ void mc_block_reduce(McAcc &a)
+{
+    __shared__ double sv[kMcThreads / 64][kMcQ];
+    __shared__ int sl[kMcThreads / 64][kMcQ];
+#pragma unroll
+    for (int q = 0; q < kMcQ; q++) {
+        const bool mn = mc_is_min(q);
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            double x = __shfl_down(a.v[q], off, 64);
+            int lx = __shfl_down(a.l[q], off, 64);
+            if (mc_better(mn, x, lx, a.v[q], a.l[q])) {
+                a.v[q] = x;
+                a.l[q] = lx;
+            }
+        }
+    }
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0)
+#pragma unroll
+        for (int q = 0; q < kMcQ; q++) {
+            sv[wv][q] = a.v[q];
+            sl[wv][q] = a.l[q];
+        }
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int w = 1; w < kMcThreads / 64; w++)
+#pragma unroll
+            for (int q = 0; q < kMcQ; q++)
+                if (mc_better(mc_is_min(q), sv[w][q], sl[w][q], a.v[q], a.l[q])) {
+                    a.v[q] = sv[w][q];
+                    a.l[q] = sl[w][q];
+                }
+}
+
+// one block: kMcE points of plane k = blockIdx.y
+__global__ __launch_bounds__(kMcThreads) void k_mc_reduce(McArgs A)
+{
+    const int k = blockIdx.y;
+    const int nxy = A.nx * A.ny;
+    const double rm = A.rm[k];
+    const bool top = k == A.nz - 1;
+    const double dz = A.dz[k];
+    const long pk = (long)k * A.nxppy;
+    McAcc acc;
+    acc.init();
+#pragma unroll
+    for (int it = 0; it < kMcE / kMcThreads; it++) {
+        const int e = blockIdx.x * kMcE + it * kMcThreads + threadIdx.x;
+        if (e >= nxy) break;
+        const int j = e / A.nx, i = e - j * A.nx;
+        const long o = pk + (long)j * A.nxpp + i;
+        const int L = (i * A.ny + j) * A.nz + k;
+        const double u = A.u[o], v = A.v[o], w = A.w[o], t = A.t[o];
+        const double uu = u + 0.0, ww = w + 0.0; // + umwv, wmwv (zero)
+        acc.add(0, uu, L);
+        acc.add(1, uu, L);
+        acc.add(2, v, L);
+        acc.add(3, v, L);
+        acc.add(4, ww, L);
+        acc.add(5, ww, L);
+        const double tt = (t + 0.0) + rm; // + rhomwv (zero) + rhomean
+        acc.add(6, tt, L);
+        acc.add(7, tt, L);
+        // uo*fcshear*fshear = 0 (fcshear = 0, main.f:120)
+        acc.add(8, fabs((u * A.dt) / A.dx), L);
+        acc.add(9, fabs((v * A.dt) / A.dy), L);
+        acc.add(10, top ? 0.0 : fabs((w * A.dt) / dz), L);
+    }
+    mc_block_reduce(acc);
+    if (threadIdx.x == 0) {
+        double *p = A.part + (size_t)(blockIdx.y * gridDim.x + blockIdx.x) * (2 * kMcQ);
+        for (int q = 0; q < kMcQ; q++) {
+            p[2 * q] = acc.v[q];
+            p[2 * q + 1] = (double)acc.l[q];
+        }
+    }
+}
+
+// partials -> the record (layout of iw_maxcalc in iwgpu.h); entries the
+// reference keeps from the previous call (itmin, jtmin, dzmax) stay when
+// there is no new record
+__global__ __launch_bounds__(kMcThreads) void k_mc_final(McArgs A, int nblk, int istep, double t, double umax,
+                                                        double *rec)
+{
+    McAcc acc;
+    acc.init();
+    for (int b = threadIdx.x; b < nblk; b += kMcThreads) {
+        const double *p = A.part + (size_t)b * (2 * kMcQ);
+        for (int q = 0; q < kMcQ; q++) acc.add(q, p[2 * q], (int)p[2 * q + 1]);
+    }
+    mc_block_reduce(acc);
+    if (threadIdx.x != 0) return;
+    const int nz = A.nz, ny = A.ny;
+    auto kk = [&](int L) { return L % nz; };
+    auto jj = [&](int L) { return (L / nz) % ny; };
+    auto ii = [&](int L) { return L / (nz * ny); };
+    // did the running extremum with start value s move at all?
+    auto moved = [&](int q, double s) {
+        return acc.l[q] >= 0 && (mc_is_min(q) ? acc.v[q] <= s : acc.v[q] >= s);
+    };
+    rec[0] = istep;
+    rec[1] = t;
+    const int pairs[6][3] = {{0, 2, 4}, {1, 3, 5}, {2, 6, 8}, {3, 7, 9}, {4, 10, 12}, {5, 11, 13}};
+    for (int m = 0; m < 6; m++) { // u, v, w extrema with their k
+        const int q = pairs[m][0];
+        const bool r = moved(q, 0.0);
+        rec[pairs[m][1]] = r ? acc.v[q] : 0.0;
+        rec[pairs[m][2]] = r ? kk(acc.l[q]) + 1 : 0;
+    }
+    if (umax == 0.0 && moved(1, 0.0)) rec[3] = acc.v[1] + umax; // umaxt; the walk sets it otherwise
+    if (moved(6, 1.0)) {
+        rec[14] = acc.v[6];
+        rec[16] = kk(acc.l[6]) + 1;
+        rec[18] = ii(acc.l[6]) + 1;
+        rec[19] = jj(acc.l[6]) + 1;
+    } else {
+        rec[14] = 1.0;
+        rec[16] = 0;
+    }
+    if (moved(7, 0.0)) {
+        rec[15] = acc.v[7];
+        rec[17] = kk(acc.l[7]) + 1;
+        rec[20] = ii(acc.l[7]) + 1;
+        rec[21] = jj(acc.l[7]) + 1;
+    } else {
+        rec[15] = 0.0;
+        rec[17] = rec[20] = rec[21] = 0;
+    }
+    for (int m = 0; m < 3; m++) {
+        const int q = 8 + m, o = 22 + 4 * m;
+        if (moved(q, 0.0)) {
+            rec[o] = acc.v[q];
+            rec[o + 1] = ii(acc.l[q]) + 1;
+            rec[o + 2] = jj(acc.l[q]) + 1;
+            rec[o + 3] = kk(acc.l[q]) + 1;
+        } else {
+            rec[o] = 0.0;
+            rec[o + 1] = rec[o + 2] = rec[o + 3] = 0;
+        }
+    }
+    if (moved(10, 0.0)) rec[34] = A.dz[kk(acc.l[10])]; // dz at the CFLZ maximum
+    // uveltest.dat probe u(32,1,10), u(32,1,50) by linear position
+    const long ntot = (long)A.nxppy * nz;
+    const long p1 = 31 + (long)A.nxpp * ny * 9, p2 = 31 + (long)A.nxpp * ny * 49;
+    rec[40] = p1 < ntot ? A.u[p1] : __builtin_nan("");
+    rec[41] = p2 < ntot ? A.u[p2] : __builtin_nan("");
+}
+
+// column maxima of u in L order: cm[i ny + j] = max_k u(i,j,k)
+__global__ __launch_bounds__(256) void k_mc_colmax(const double *__restrict__ u, int nx, int ny, int nz, int nxpp,
+                                                   long nxppy, double *cm)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nx * ny) return;
+    const int j = e / nx, i = e - j * nx;
+    const double *p = u + (long)j * nxpp + i;
+    double m = -DBL_MAX;
+    for (int k = 0; k < nz; k++) m = fmax(m, p[(long)k * nxppy] + 0.0);
+    cm[i * ny + j] = m;
+}
+
+// maxima of groups of 256 consecutive columns
+__global__ __launch_bounds__(256) void k_mc_groupmax(const double *__restrict__ cm, int ncol, double *gm)
+{
+    __shared__ double s[4];
+    const int c = blockIdx.x * 256 + threadIdx.x;
+    double m = c < ncol ? cm[c] : -DBL_MAX;
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_down(m, off, 64));
+    if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) gm[blockIdx.x] = fmax(fmax(s[0], s[1]), fmax(s[2], s[3]));
+}
+
+// smallest index in [0, 256) whose flag is set, or 256; all threads get it
+__device__ int mc_block_first(bool flag)
+{
+    __shared__ int sw[4];
+    const unsigned long long b = __ballot(flag);
+    if ((threadIdx.x & 63) == 0) sw[threadIdx.x >> 6] = b ? (threadIdx.x & ~63) + __ffsll((long long)b) - 1 : 256;
+    __syncthreads();
+    const int r = min(min(sw[0], sw[1]), min(sw[2], sw[3]));
+    __syncthreads();
+    return r;
+}
+
+// the umaxt records (various.f:755-758): start value 0, a point with
+// u >= umaxt becomes the record and umaxt = u + umax.  One work-group.
+// Every loop advances (c, k0) or ends, so the walk finishes after at most
+// ncol column steps plus one step per record.
+__global__ __launch_bounds__(256) void k_mc_walk(const double *__restrict__ u, const double *__restrict__ cm,
+                                                 const double *__restrict__ gm, int nx, int ny, int nz, int nxpp,
+                                                 long nxppy, double umax, double *rec)
+{
+    const int ncol = nx * ny, ngrp = (ncol + 255) / 256;
+    double thr = 0.0;
+    int kmax = 0;
+    int c = 0, k0 = 0;
+    // first k >= k0 in column c with u >= thr, or nz
+    auto scan_column = [&](int cc, int kb) -> int {
+        const int i = cc / ny, j = cc - i * ny;
+        const double *p = u + (long)j * nxpp + i;
+        for (int k = kb; k < nz; k += 256) {
+            const int kt = k + (int)threadIdx.x;
+            const bool f = kt < nz && (p[(long)kt * nxppy] + 0.0) >= thr;
+            const int r = mc_block_first(f);
+            if (r < 256) return k + r;
+        }
+        return nz;
+    };
+    while (c < ncol) {
+        if (k0 == 0) { // next column that can hold a record: groups first
+            int g = c >> 8;
+            int found = -1;
+            // rest of the current group
+            {
+                const int cc = c + (int)threadIdx.x;
+                const bool f = cc < min(ncol, (g + 1) * 256) && cm[cc] >= thr;
+                const int r = mc_block_first(f);
+                if (r < 256) found = c + r;
+            }
+            for (g = g + 1; found < 0 && g < ngrp; g += 256) {
+                const int gg = g + (int)threadIdx.x;
+                const int r = mc_block_first(gg < ngrp && gm[gg] >= thr);
+                if (r == 256) continue;
+                const int g0 = g + r; // first group with a candidate column
+                const int cc = g0 * 256 + (int)threadIdx.x;
+                const int r2 = mc_block_first(cc < ncol && cm[cc] >= thr);
+                if (r2 < 256) {
+                    found = g0 * 256 + r2;
+                } else { // only when a group maximum is NaN-free but no column passes
+                    g = g0 + 1 - 256;
+                }
+            }
+            if (found < 0) break;
+            c = found;
+        }
+        const int k = scan_column(c, k0);
+        if (k < nz) {
+            const int i = c / ny, j = c - i * ny;
+            thr = (u[(long)k * nxppy + (long)j * nxpp + i] + 0.0) + umax;
+            kmax = k + 1;
+            k0 = k + 1;
+            if (k0 == nz) {
+                c++;
+                k0 = 0;
+            }
+        } else {
+            c++;
+            k0 = 0;
+        }
+    }
+    if (threadIdx.x == 0) {
+        rec[3] = thr;
+        rec[5] = kmax;
+    }
+}
+
+} // namespace
+
+int mc_blocks(const Geo &g) { return ((g.nx * g.ny + kMcE - 1) / kMcE) * g.nz; }
+
+void launch_maxcalc(const Geo &g, const McArgs &A, int istep, double t, double umax, double *colmax,
+                    double *groupmax, double *rec, hipStream_t st)
+{
+    const int nxy = g.nx * g.ny;
+    dim3 grid((nxy + kMcE - 1) / kMcE, g.nz);
+    hipLaunchKernelGGL(k_mc_reduce, grid, dim3(kMcThreads), 0, st, A);
+    hipLaunchKernelGGL(k_mc_final, dim3(1), dim3(kMcThreads), 0, st, A, (int)(grid.x * grid.y), istep, t, umax,
+                       rec);
+    if (umax != 0.0) {
+        hipLaunchKernelGGL(k_mc_colmax, dim3((nxy + 255) / 256), dim3(256), 0, st, A.u, g.nx, g.ny, g.nz, g.nxpp,
+                           g.nxppy, colmax);
+        hipLaunchKernelGGL(k_mc_groupmax, dim3((nxy + 255) / 256), dim3(256), 0, st, colmax, nxy, groupmax);
+        hipLaunchKernelGGL(k_mc_walk, dim3(1), dim3(256), 0, st, A.u, colmax, groupmax, g.nx, g.ny, g.nz, g.nxpp,
+                           g.nxppy, umax, rec);
+    }
+}
+
+} // namespace iw

@@ -133,6 +133,33 @@ c     layout of iw_params (include/iwgpu.h)
           integer(c_int), value :: idx
           real(c_double) :: a(*)
         end function
+        integer(c_int) function iw_get_coeffs(h, t, fac, bdf, bdfv)
+     &       bind(c)
+          import
+          type(c_ptr), value :: h
+          real(c_double) :: t, bdfv
+          real(c_double) :: fac(3), bdf(3)
+        end function
+        integer(c_int) function iw_maxcalc(h, istep, rec) bind(c)
+          import
+          type(c_ptr), value :: h
+          integer(c_int), value :: istep
+          real(c_double) :: rec(*)
+        end function
+        integer(c_int) function iw_get_dtstate(h, dt, cdt, iv, idt)
+     &       bind(c)
+          import
+          type(c_ptr), value :: h
+          real(c_double) :: dt, cdt
+          integer(c_int) :: iv, idt
+        end function
+        integer(c_int) function iw_set_dtstate(h, cdt, iv, idt)
+     &       bind(c)
+          import
+          type(c_ptr), value :: h
+          real(c_double), value :: cdt
+          integer(c_int), value :: iv, idt
+        end function
         real(c_double) function iw_abvar1(a, b, c) bind(c)
           import
           real(c_double), value :: a, b, c
@@ -431,6 +458,83 @@ c-----------------------------------------------------------------------
       if (cpenspecfilter) icpen = 1
       call iwshim_check(iw_calcuvwt(iwh, icpen), 'calcuvwt')
       call iwshim_get_state
+      end
+
+c-----------------------------------------------------------------------
+c     various.f:665   subroutine maxcalc(istep)
+c     the extrema and CFL maxima of the physical u,v,w,temp, the horfft
+c     round trip of /flow/ u,v,w,temp and the adaptive step of
+c     various.f:887-985 (dt, cdt, /ab/, /bdf/, /fachelm/, /vardtflags/)
+c     on the device; the reference's outputs (uveltest.dat, units 50, 51
+c     and 65, the printed lines, various.f:744-985) in its formats
+c-----------------------------------------------------------------------
+      subroutine maxcalc(istep)
+      use iwshim
+      include 'dim.h'
+      include 'comflow.h'
+      include 'comtime.h'
+      include 'comtimeadvance.h'
+      include 'comparamphys.h'
+      real(c_double) :: rec(42), fac(3), bdf(3), tt, bv, dtn, cdtn
+      integer(c_int) :: iv, idt
+      integer ir(42), m
+      call iwshim_init
+      call iwshim_check(iw_set_state(iwh, 0, u), 'maxcalc')
+      call iwshim_check(iw_set_state(iwh, 1, v), 'maxcalc')
+      call iwshim_check(iw_set_state(iwh, 2, w), 'maxcalc')
+      call iwshim_check(iw_set_state(iwh, 3, temp), 'maxcalc')
+      call iwshim_put_coeffs
+      call iwshim_check(iw_set_dtstate(iwh, cdt, ivardt, idtstep),
+     &     'maxcalc')
+      call iwshim_check(iw_maxcalc(iwh, istep, rec), 'maxcalc')
+      call iwshim_check(iw_get_state(iwh, 0, u), 'maxcalc')
+      call iwshim_check(iw_get_state(iwh, 1, v), 'maxcalc')
+      call iwshim_check(iw_get_state(iwh, 2, w), 'maxcalc')
+      call iwshim_check(iw_get_state(iwh, 3, temp), 'maxcalc')
+      do m = 1, 42
+        ir(m) = 0
+        if (rec(m) .eq. rec(m)) ir(m) = nint(rec(m))
+      enddo
+      open(700, file='uveltest.dat', position='append')
+      write(700,*) t, rec(41), rec(42)
+      close(700)
+      write(50,'(1x,I5,2x,4(E12.6,1x,E12.6,1x,I3,1x,I3,2x))')
+     &  istep, rec(3), rec(4), ir(5), ir(6), rec(7), rec(8), ir(9),
+     &  ir(10), rec(11), rec(12), ir(13), ir(14), rec(15), rec(16),
+     &  ir(17), ir(18)
+      write(*,'(1x,I5,2x,e12.6,2x,e12.6,2x,i3,2x,i3)')
+     &  istep-1, rec(15), rec(16), ir(17), ir(18)
+      write(51,'(1x,e12.6,2x,3(1x,e12.6,2x,3(I5,2x)))')
+     &  t, rec(23), ir(24), ir(25), ir(26), rec(27), ir(28), ir(29),
+     &  ir(30), rec(31), ir(32), ir(33), ir(34)
+      write(*,*) 'Temp. min at i,j,k', ir(19), ir(20), ir(17)
+      write(*,*) 'Temp. max at i,j,k', ir(21), ir(22), ir(18)
+      write(*,*) 'At max CFLZ dz= ', rec(35), rec(31)
+      if (ir(38) .ne. 0) then
+        call iwshim_check(iw_get_dtstate(iwh, dtn, cdtn, iv, idt),
+     &       'maxcalc')
+        call iwshim_check(iw_get_coeffs(iwh, tt, fac, bdf, bv),
+     &       'maxcalc')
+        dt = dtn
+        cdt = cdtn
+        ivardt = iv
+        idtstep = idt
+        fac1 = fac(1)
+        fac2 = fac(2)
+        fac3 = fac(3)
+        bdf1 = bdf(1)
+        bdf2 = bdf(2)
+        bdf3 = bdf(3)
+        bdfv = bv
+        fact = 1./(xkappa*dt)
+        facvel = 1./(pr*dt)
+        write(65,*) t, dt
+        write(*,*) 'At time= ', t
+        if (cdt .gt. 1.) write(*,*)
+     &    'TIMESTEP *REDUCED* by a factor of 4/5 to dt=', dt
+        if (cdt .lt. 1.) write(*,*)
+     &    'TIMESTEP *INCREASED* by a factor of 5/4 to dt=', dt
+      endif
       end
 
 c-----------------------------------------------------------------------

@@ -21,8 +21,10 @@ EXPORTS = [
     "iw_convec", "iw_calcuvwt", "iw_filtering", "iw_get_sgs",
     "iw_save_restart", "iw_load_restart", "iw_write_restart_file", "iw_read_restart_file", "iw_abvar1", "iw_abvar2", "iw_abvar3", "iw_bdfvarv",
     "iw_bdfvar1", "iw_bdfvar2", "iw_bdfvar3", "iw_set_profiling", "iw_kernel_stats",
-    "iw_device_bytes",
+    "iw_device_bytes", "iw_maxcalc", "iw_maxcalc_records", "iw_get_dtstate", "iw_set_dtstate",
 ]
+
+MAXCALC_NREC = 42  # IW_MAXCALC_NREC
 
 _lib = None
 
@@ -67,6 +69,10 @@ def lib():
     L.iw_write_restart_file.argtypes = [ctypes.c_char_p, P, D, P]
     L.iw_read_restart_file.argtypes = [ctypes.c_char_p, P, P, P, P]
     L.iw_get_sgs.argtypes = [P, I, P]
+    L.iw_maxcalc.argtypes = [P, I, P]
+    L.iw_maxcalc_records.argtypes = [P, P, I]
+    L.iw_get_dtstate.argtypes = [P, P, P, P, P]
+    L.iw_set_dtstate.argtypes = [P, D, I, I]
     for n in ("abvar1", "abvar2", "abvar3", "bdfvarv", "bdfvar1", "bdfvar2", "bdfvar3"):
         f = getattr(L, "iw_" + n)
         f.restype = D
@@ -200,6 +206,25 @@ class Solver:
         out = np.empty(self.shape)
         _check(self.L.iw_get_sgs(self.h, idx, _ptr(out)))
         return out
+
+    def maxcalc(self, istep=1):
+        """maxcalc (various.f:665-987) on the device state; returns its record."""
+        r = np.zeros(MAXCALC_NREC)
+        _check(self.L.iw_maxcalc(self.h, istep, _ptr(r)))
+        return r
+
+    def maxcalc_records(self, n):
+        """(n, MAXCALC_NREC) records of the last run."""
+        r = np.zeros((n, MAXCALC_NREC))
+        _check(self.L.iw_maxcalc_records(self.h, _ptr(r), n))
+        return r
+
+    def dtstate(self):
+        dt, cdt = ctypes.c_double(), ctypes.c_double()
+        iv, idt = ctypes.c_int(), ctypes.c_int()
+        _check(self.L.iw_get_dtstate(self.h, ctypes.byref(dt), ctypes.byref(cdt), ctypes.byref(iv),
+                                     ctypes.byref(idt)))
+        return dt.value, cdt.value, iv.value, idt.value
 
     def set_profiling(self, on=True):
         _check(self.L.iw_set_profiling(self.h, 1 if on else 0))

@@ -86,7 +86,12 @@ def build(name, nx, ny, nzloc, nsubd, noslip=False, ref_dim=False, jobs=8, shim=
     r = subprocess.run([FLANG] + FFLAGS + VARIOUS_FLAGS + ["-c", os.path.join(src, "various.f"), "-o", vo],
                        stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
     with_maxcalc = r.returncode == 0
-    if not with_maxcalc:
+    marker = os.path.join(bdir, "has_maxcalc")
+    if with_maxcalc:
+        open(marker, "w").close()
+    else:
+        if os.path.exists(marker):
+            os.remove(marker)
         print(name, ": various.f does not compile at these sizes; maxcalc left out")
     objs = [vo] if with_maxcalc else []
     procs = []

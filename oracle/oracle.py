@@ -194,7 +194,7 @@ def _input_dat(P, nsteps, istart=0, cdt=1.0):
 
 def ref_has_maxcalc(build):
     """True when the reference build includes various.f (maxcalc)."""
-    return os.path.exists(os.path.join(REFDIR, build, "various.o"))
+    return os.path.exists(os.path.join(REFDIR, build, "has_maxcalc"))
 
 
 def _parse_maxcalc(d):

@@ -6,7 +6,7 @@ import os
 import numpy as np
 import pytest
 
-from helpers import SOLUTION, STATE, case, g3_params, rel, routine_inputs
+from helpers import SOLUTION, STATE, case, g1_params, g3_params, rel, routine_inputs
 
 pytestmark = pytest.mark.gpu
 
@@ -303,3 +303,142 @@ def test_device_restart_file_read_by_reference(rt, oracle_mod, tmp_path):
     assert abs(th - tr) < 1e-15
     for q in SOLUTION:
         assert rel(sh[q], ref[q]) < (FIELD_TOL if q < 12 else HIST_TOL), STATE[q]
+
+
+# ------------------------------------------------ maxcalc and adaptive dt
+
+# record columns (layout of iw_maxcalc): positions and flags must agree
+# exactly, values to rounding
+REC_EXACT = [0, 4, 5, 8, 9, 12, 13, 16, 17, 18, 19, 20, 21, 23, 24, 25, 27, 28, 29, 31, 32, 33, 37, 38, 39]
+REC_VALUES = [1, 2, 3, 6, 7, 10, 11, 14, 15, 22, 26, 30, 34, 35, 36, 40, 41]
+REC_SCALE = {2: [2, 3], 3: [2, 3], 6: [6, 7], 7: [6, 7], 10: [10, 11], 11: [10, 11], 14: [14, 15],
+             15: [14, 15], 40: [2, 3], 41: [2, 3]}
+
+
+def _cmp_records(rg, ro, tol=1e-12):
+    assert rg.shape == ro.shape
+    assert np.array_equal(rg[:, REC_EXACT], ro[:, REC_EXACT])
+    for c in REC_VALUES:
+        a, b = rg[:, c], ro[:, c]
+        assert np.array_equal(np.isnan(a), np.isnan(b)), c
+        m = ~np.isnan(b)
+        if m.any():
+            # a minimum can be a small difference of O(1) terms (temp plus
+            # the mean density): every extremum is compared on the scale of
+            # its min/max pair, the two point probes of u on that of u
+            ref = ro[:, REC_SCALE[c]] if c in REC_SCALE else b[m]
+            scale = max(np.abs(ref).max(), 1e-300)
+            assert np.abs(a[m] - b[m]).max() <= tol * scale, c
+
+
+@pytest.mark.parametrize("name,steps,umax", [("small", 10, None), ("c1", 20, None), ("c1", 20, 0.0)])
+def test_maxcalc_records_match_oracle(rt, oracle_mod, name, steps, umax):
+    """The per-step maxcalc records of iw_run (extrema, CFL maxima and their
+    positions, the umaxt record chain or, with umax = 0, the plain maximum)
+    against the oracle's sequential scan in the reference loop order."""
+    P, flds = case(name)
+    if umax is not None:
+        P.umax = umax
+    print("maxcalc records", name, steps, umax, flush=True)
+    o, g = oracle_mod.Oracle(P), rt.Solver(P)
+    o.set_initial(*flds)
+    g.set_initial(*flds)
+    o.run(steps)
+    g.run(steps, counts=False)
+    _cmp_records(g.maxcalc_records(steps), o.maxcalc_records())
+    # one more call through the parity-mode entry point
+    _cmp_records(g.maxcalc(steps + 1)[None], o.maxcalc(steps + 1)[None])
+    for q in range(4):
+        assert rel(g.get_state(q), o.field(q)) < FIELD_TOL, STATE[q]
+    g.close()
+
+
+def test_adaptive_dt_run_matches_oracle(rt, oracle_mod):
+    """c1 with adaptive dt (Tso kept between calls): the device takes the same
+    two dt changes as the oracle (bit-identical dt), the variable-step
+    coefficients and refactored operators around them, and stays within
+    the field tolerance after 50 steps."""
+    P, flds = case("c1")
+    P.dtadapt = 1
+    print("adaptive c1 50", flush=True)
+    o, g = oracle_mod.Oracle(P), rt.Solver(P)
+    o.set_initial(*flds)
+    g.set_initial(*flds)
+    o.run(50)
+    g.run(50, counts=False)
+    ro, rg = o.maxcalc_records(), g.maxcalc_records(50)
+    assert [int(r[0]) for r in ro if r[37] == 1] == [22, 43]
+    assert np.array_equal(rg[:, 35], ro[:, 35])
+    _cmp_records(rg, ro)
+    sg = g.state()
+    so = o.state()
+    errs = {STATE[q]: rel(sg[q], so[q]) for q in SOLUTION}
+    _record("adaptive_c1_50", errs)
+    for q in SOLUTION:
+        assert errs[STATE[q]] < (FIELD_TOL if q < 12 else HIST_TOL), STATE[q]
+    assert abs(g.time() - o.time()) < 1e-15
+    g.close()
+
+
+def test_g1_forced_run_with_adaptive_dt_on_device(rt, oracle_mod):
+    """G1, the reference's logged 400-step 2-D forced run, on the device with
+    Tso read as zero at every call: its 18 dt changes (dtchange.dat) and the
+    oracle's run."""
+    P, d = g1_params(dtadapt=2)
+    n = len(d["cflz"])
+    zero = np.zeros((P.nzloc * P.nsubd, 1, P.nx + 2))
+    print("G1 device run", n, flush=True)
+    g = rt.Solver(P)
+    g.set_initial(zero, zero, zero, zero)
+    g.run(n, counts=False)
+    rg = g.maxcalc_records(n)
+    ev = np.array([[r[1], r[35]] for r in rg if r[37] == 1])
+    gold = np.array(d["dt_events"])
+    assert ev.shape == gold.shape
+    assert np.abs(ev - gold).max() < 1e-12 * np.abs(gold).max()
+    print("G1 oracle run", n, flush=True)
+    o = oracle_mod.Oracle(P)
+    o.set_initial(zero, zero, zero, zero)
+    o.run(n)
+    ro = o.maxcalc_records()
+    assert np.array_equal(rg[:, 35], ro[:, 35])
+    errs = {}
+    for q in range(12):
+        so = o.field(q)
+        if np.linalg.norm(so) > 0:
+            errs[STATE[q]] = rel(g.get_state(q), so)
+    _record("g1_400", errs)
+    for k, e in errs.items():
+        assert e < FIELD_TOL, k
+    g.close()
+
+
+def test_fortran_maxcalc_shim_under_the_reference_step_loop(oracle_mod):
+    """maxcalc reached through the Fortran shim (device extrema, round trip
+    and adaptive step, the reference's output lines) under the reference
+    step loop with every hot-path routine shimmed, against the reference
+    harness calling the reference maxcalc: 50 steps of c1 with two dt
+    changes."""
+    name, n = "c1", 50
+    for exe in ("iwref", "iwshim"):
+        if not os.path.exists(os.path.join(oracle_mod.REFDIR, name, exe)):
+            pytest.skip("%s/%s not built (built where the reference sources are)" % (name, exe))
+    if not oracle_mod.ref_has_maxcalc(name):
+        pytest.skip("reference build without various.f")
+    P, flds = case(name)
+    print("shim maxcalc", name, n, flush=True)
+    sr, tr, _, mr = oracle_mod.run_ref(name, P, flds, nsteps=n, maxcalc=True)
+    ss, ts, _, ms = oracle_mod.run_ref(name, P, flds, nsteps=n, exe="iwshim", maxcalc=True)
+    dr, ds = np.array(mr["dtlog"]), np.array(ms["dtlog"])
+    assert np.array_equal(dr, ds)  # t, dt, cdt, ivardt, idtstep after every maxcalc
+    cr, cs = np.array(mr["cflz"]), np.array(ms["cflz"])
+    assert np.array_equal(cr[:, 0], cs[:, 0])
+    assert rel(cs[:, 1], cr[:, 1]) < 1e-12
+    assert mr["tloc"] == ms["tloc"]
+    f50r, f50s = np.array(mr["fort50"]), np.array(ms["fort50"])
+    assert (np.abs(f50r - f50s) <= 1.1e-5 * np.abs(f50r)).all()  # at most the last printed digit
+    errs = {STATE[q]: rel(ss[q], sr[q]) for q in SOLUTION}
+    _record("shim_maxcalc_%s_%d" % (name, n), errs)
+    for q in SOLUTION:
+        assert errs[STATE[q]] < (FIELD_TOL if q < 12 else HIST_TOL), STATE[q]
+    assert abs(ts - tr) < 1e-15
