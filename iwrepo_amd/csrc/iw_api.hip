@@ -857,12 +857,15 @@ extern "C" int iw_create(const iw_params *p, iw_solver **out)
         // the top point has no CFLZ and keeps the dz of the point below
         // it (various.f:817-822)
         if (g.nz > 1) dz[g.nz - 1] = dz[g.nz - 2];
-        if (upload(&s->mc_rm, rm, acc) || upload(&s->mc_dz, dz, acc)) return -1;
+        std::vector<double> rdz(g.nz);
+        for (int k = 0; k < g.nz; k++) rdz[k] = 1. / dz[k];
+        if (upload(&s->mc_rm, rm, acc) || upload(&s->mc_dz, dz, acc) || upload(&s->mc_rdz, rdz, acc)) return -1;
         const int nxy = P.nx * P.ny;
         HIPCHK(hipMalloc((void **)&s->mc_part, sizeof(double) * 2 * kMcQ * (size_t)mc_blocks(g)));
         HIPCHK(hipMalloc((void **)&s->mc_rec, sizeof(double) * kMcRec));
         HIPCHK(hipMemset(s->mc_rec, 0, sizeof(double) * kMcRec));
-        HIPCHK(hipMalloc((void **)&s->mc_colmax, sizeof(double) * (size_t)nxy));
+        HIPCHK(hipMalloc((void **)&s->mc_w00, sizeof(double) * g.nz));
+        HIPCHK(hipMalloc((void **)&s->mc_colmax, sizeof(double) * (size_t)(kMcSlices + 1) * nxy));
         HIPCHK(hipMalloc((void **)&s->mc_gmax, sizeof(double) * (size_t)((nxy + 255) / 256)));
         HIPCHK(hipHostMalloc((void **)&s->h_mc, sizeof(double) * kMcRec));
     }
@@ -909,9 +912,11 @@ extern "C" int iw_destroy(iw_solver *s)
     hipHostFree(s->h_count);
     hipFree(s->mc_rm);
     hipFree(s->mc_dz);
+    hipFree(s->mc_rdz);
     hipFree(s->mc_part);
     hipFree(s->mc_rec);
     hipFree(s->mc_colmax);
+    hipFree(s->mc_w00);
     hipFree(s->mc_gmax);
     hipFree(s->mc_ring);
     hipHostFree(s->h_mc);
@@ -1067,7 +1072,7 @@ static FieldBatch batch(std::initializer_list<double *> l)
 // works in place on b.g.  is = -1: physical -> spectral.
 enum FftUse { kFftMain = 0, kFftIface = 1, kFftMaxcalc = 2 };
 
-static void fft_dev(iw_solver *s, FieldBatch b, int is, int nk, int use = kFftMain)
+static void fft_dev(iw_solver *s, FieldBatch b, int is, int nk, int use = kFftMain, const McFuse *mf = nullptr)
 {
     if (b.n == 0 || nk == 0) return;
     // timing scopes: the interface-plane and maxcalc transforms are kept
@@ -1098,10 +1103,10 @@ static void fft_dev(iw_solver *s, FieldBatch b, int is, int nk, int use = kFftMa
                 launch_fft_y(s->g, b, nk, 1, s->stream);
             }
             Scope sc(s, nxi, bx);
-            launch_fft_x(s->g, b2, nk, 1, s->stream);
+            launch_fft_x(s->g, b2, nk, 1, s->stream, mf);
         } else {
             Scope sc(s, nxi, bx);
-            launch_fft_x(s->g, b, nk, 1, s->stream);
+            launch_fft_x(s->g, b, nk, 1, s->stream, mf);
         }
     }
 }
@@ -1134,8 +1139,51 @@ static void project_dev(iw_solver *s, double *const *f, int nf)
     launch_project(s->g, b, s->stream);
 }
 
-// convec (solver.f:6-514) on the device state
-static int convec_dev(iw_solver *s, int ispecfilternl)
+// maxcalc reductions in the epilogue of an inverse x transform of u,v,w,
+// temp: cny = ny-1 and the removed mean of w when the transform is convec's
+static McFuse mc_fuse(iw_solver *s, double cny, const double *w00)
+{
+    McFuse f;
+    f.part = s->mc_part;
+    f.rm = s->mc_rm;
+    f.rdz = s->mc_rdz;
+    f.w00 = w00;
+    f.dt = s->P.dt;
+    f.cx = s->P.dt / s->mc_dx;
+    f.cy = s->P.dt / s->mc_dy;
+    f.cny = cny;
+    return f;
+}
+
+// the maxcalc record (s->mc_rec) from the partials of that transform
+static void mc_record(iw_solver *s, int istep, double cny)
+{
+    const Geo &g = s->g;
+    McArgs A;
+    A.u = s->phys[0];
+    A.spec = s->buf[0];
+    A.cny = cny;
+    A.dz = s->mc_dz;
+    A.nx = g.nx; A.ny = g.ny; A.nz = g.nz; A.nxpp = g.nxpp; A.nxppy = g.nxppy;
+    A.part = s->mc_part;
+    A.nblk = mc_blocks(g);
+    {
+        Scope sc(s, "maxcalc");
+        launch_maxcalc_final(g, A, istep, s->t, s->P.umax, s->mc_rec, s->stream);
+    }
+    if (s->P.umax != 0.0) {
+        Scope sc(s, "mc_umaxt", 8. * (double)g.ntot);
+        launch_maxcalc_umaxt(g, A, s->P.umax, s->mc_colmax, s->mc_gmax, s->mc_rec, s->stream);
+    }
+}
+
+static int mc_after(iw_solver *s, int slot);
+
+// convec (solver.f:6-514) on the device state.  mcstep > 0: maxcalc of that
+// step (its record from convec's inverse transforms, then the adaptive step
+// and main.f:660-664) before anything that reads dt or the coefficients;
+// its record goes to slot mcslot of the run.
+static int convec_dev(iw_solver *s, int ispecfilternl, int mcstep = 0, int mcslot = 0)
 {
     const Geo &g = s->g;
     double **B = s->buf;
@@ -1145,8 +1193,9 @@ static int convec_dev(iw_solver *s, int ispecfilternl)
     const double nt = (double)g.ntot;
     {
         Scope sc(s, "zero_meanw");
-        launch_zero_meanw(g, B[2], s->stream);
+        launch_zero_meanw(g, B[2], s->mc_w00, s->stream);
     }
+    const McFuse mf = mc_fuse(s, g.ndims == 3 ? (double)(g.ny - 1) : 0.0, s->mc_w00);
     {
         Scope sc(s, "dpdz", 48. * nt);
         double *o3[3] = {gr[2], gr[5], gr[8]};
@@ -1176,7 +1225,7 @@ static int convec_dev(iw_solver *s, int ispecfilternl)
         bx.kw = g.xw;
         {
             Scope sc(s, "fft_x_inv", 16. * g.nxppy * g.nz * 13);
-            launch_fft_x(g, bx, g.nz, 1, s->stream);
+            launch_fft_x(g, bx, g.nz, 1, s->stream, mcstep > 0 ? &mf : nullptr);
         }
     } else {
         {
@@ -1186,7 +1235,11 @@ static int convec_dev(iw_solver *s, int ispecfilternl)
         FieldBatch b = batch({B[0], B[1], B[2], B[3], gr[0], gr[1], gr[2], gr[3], gr[4], gr[5], gr[6], gr[7],
                               gr[8]});
         for (int q = 0; q < 4; q++) b.g[q] = s->phys[q];
-        fft_dev(s, b, 1, g.nz);
+        fft_dev(s, b, 1, g.nz, kFftMain, mcstep > 0 ? &mf : nullptr);
+    }
+    if (mcstep > 0) {
+        mc_record(s, mcstep, mf.cny);
+        if (mc_after(s, mcslot)) return -1;
     }
     double *ph[4] = {s->phys[0], s->phys[1], s->phys[2], s->phys[3]};
     {
@@ -1356,17 +1409,9 @@ static void maxcalc_dev(iw_solver *s, int istep)
     const Geo &g = s->g;
     FieldBatch b = batch({s->buf[0], s->buf[1], s->buf[2], s->buf[3]});
     for (int q = 0; q < 4; q++) b.g[q] = s->phys[q];
-    fft_dev(s, b, 1, g.nz, kFftMaxcalc);
-    {
-        Scope sc(s, "maxcalc", 32. * (double)g.ntot);
-        McArgs A;
-        A.u = s->phys[0]; A.v = s->phys[1]; A.w = s->phys[2]; A.t = s->phys[3];
-        A.rm = s->mc_rm; A.dz = s->mc_dz;
-        A.dt = s->P.dt; A.dx = s->mc_dx; A.dy = s->mc_dy;
-        A.nx = g.nx; A.ny = g.ny; A.nz = g.nz; A.nxpp = g.nxpp; A.nxppy = g.nxppy;
-        A.part = s->mc_part;
-        launch_maxcalc(g, A, istep, s->t, s->P.umax, s->mc_colmax, s->mc_gmax, s->mc_rec, s->stream);
-    }
+    const McFuse mf = mc_fuse(s, 0.0, nullptr);
+    fft_dev(s, b, 1, g.nz, kFftMaxcalc, &mf);
+    mc_record(s, istep, 0.0);
     project_dev(s, s->buf, 4);
 }
 
@@ -1414,6 +1459,22 @@ static int mc_fetch(iw_solver *s)
 {
     HIPCHK(hipMemcpyAsync(s->h_mc, s->mc_rec, sizeof(double) * kMcRec, hipMemcpyDeviceToHost, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
+    return 0;
+}
+
+// after the record of a step of iw_run: the adaptive step (the record to
+// the host) or the record into the run's ring; then main.f:660-664
+static int mc_after(iw_solver *s, int slot)
+{
+    if (s->P.dtadapt) {
+        if (mc_fetch(s)) return -1;
+        adapt_dt(s, s->h_mc);
+        s->mc_hist.insert(s->mc_hist.end(), s->h_mc, s->h_mc + kMcRec);
+    } else {
+        HIPCHK(hipMemcpyAsync(s->mc_ring + (size_t)slot * kMcRec, s->mc_rec, sizeof(double) * kMcRec,
+                              hipMemcpyDeviceToDevice, s->stream));
+    }
+    if (s->ivardt == 1) s->idtstep++;
     return 0;
 }
 
@@ -1779,23 +1840,14 @@ extern "C" int iw_run(iw_solver *s, int nsteps, int *counts)
     }
     for (int n = 0; n < nsteps; n++) {
         int istep = ++s->istep;
-        if (mc) { // main.f:657
-            maxcalc_dev(s, istep);
-            if (adapt) { // the decision needs the record on the host
-                if (mc_fetch(s)) return -1;
-                adapt_dt(s, s->h_mc);
-                s->mc_hist.insert(s->mc_hist.end(), s->h_mc, s->h_mc + kMcRec);
-            } else {
-                HIPCHK(hipMemcpyAsync(s->mc_ring + (size_t)n * kMcRec, s->mc_rec, sizeof(double) * kMcRec,
-                                      hipMemcpyDeviceToDevice, s->stream));
-            }
-        } else {
-            // maxcalc's horfft round trip (various.f:739-742, 854-857) changes
-            // the spectrum only through the kx=0 and kx-Nyquist columns
-            project_dev(s, s->buf, 4);
-        }
-        if (s->ivardt == 1) s->idtstep++; // main.f:660-664
-        if (convec_dev(s, s->P.ispecfilternl)) return -1;
+        // maxcalc (main.f:657): its horfft round trip (various.f:739-742,
+        // 854-857) changes the spectrum only through the kx=0 and kx-Nyquist
+        // columns; its reductions come from convec's own inverse transforms
+        // of the same fields, and the adaptive step and main.f:660-664 are
+        // taken there before dt or the coefficients are read
+        project_dev(s, s->buf, 4);
+        if (!mc && s->ivardt == 1) s->idtstep++; // main.f:660-664
+        if (convec_dev(s, s->P.ispecfilternl, mc ? istep : 0, n)) return -1;
         const bool ldtadapt = s->ivardt == 1 && s->idtstep < 2;
         if (istep < 2 && s->istart == 0) { // main.f:700-777
             s->fac[0] = 2.; s->fac[1] = -1.; s->fac[2] = 0.;

@@ -77,14 +77,33 @@ struct SysTables {
 // the record layout of iw_maxcalc (iwgpu.h)
 constexpr int kMcQ = 11;
 constexpr int kMcRec = 42;
+constexpr int kMcSlices = 8; // z slices of the column maxima of the umaxt walk
+// The reductions run in the epilogue of the inverse x transform of u,v,w,
+// temp (fields 0..3 of its batch).  When that transform is convec's
+// (solver.f:107-123) rather than maxcalc's own, its input differs from
+// maxcalc's by the round trip (kx-Nyquist column scaled by 1/ny) and the
+// zeroed mean w (solver.f:66-69); maxcalc's values are then recovered as
+// f + (ny-1) A_M(j,k) (-1)^i (+ the removed mean for w), A_M the kx-Nyquist
+// coefficient of the row.
+struct McFuse {
+    double *part = nullptr;      // [blocks][kMcQ] (value, position) pairs; nullptr: off
+    const double *rm = nullptr;  // nz: rhomean(z(k)) (forcing.f:134-141)
+    const double *rdz = nullptr; // nz: 1/dz of the CFLZ (various.f:818-822)
+    const double *w00 = nullptr; // nz: mean w removed before the transform, or nullptr
+    double dt = 0, cx = 0, cy = 0; // dt/dx, dt/dy: the CFL numbers are |u| dt/dx ... to
+                                   // within an ulp of the reference's (u dt)/dx
+    double cny = 0;              // ny-1 for the Nyquist correction, 0: none
+};
+// what the record kernels read after the transform
 struct McArgs {
-    const double *u, *v, *w, *t; // physical fields
-    const double *rm;            // nz: rhomean(z(k)) (forcing.f:134-141)
-    const double *dz;            // nz: dz of the CFLZ (various.f:818-822)
-    double dt, dx, dy;
+    const double *u;             // physical u as transformed (convec's or maxcalc's)
+    const double *spec;          // spectral u (kx-Nyquist coefficients), used when cny != 0
+    double cny;
+    const double *dz;
     int nx, ny, nz, nxpp;
     long nxppy;
-    double *part;                // mc_blocks * 2 kMcQ partial results
+    const double *part;
+    int nblk;
 };
 
 struct Prof {
@@ -127,8 +146,8 @@ struct iw_solver {
     std::vector<double> zpts, wg, d, d2, alpol, xw, xsq, yw, ysq, z;
     // maxcalc: tables, partials, the record (device and pinned host), the
     // records of the last iw_run, Tso and /vardtflags/ (comtime.h:3)
-    double *mc_rm = nullptr, *mc_dz = nullptr, *mc_part = nullptr, *mc_rec = nullptr;
-    double *mc_colmax = nullptr, *mc_gmax = nullptr, *h_mc = nullptr;
+    double *mc_rm = nullptr, *mc_dz = nullptr, *mc_rdz = nullptr, *mc_part = nullptr, *mc_rec = nullptr;
+    double *mc_colmax = nullptr, *mc_gmax = nullptr, *h_mc = nullptr, *mc_w00 = nullptr;
     double *mc_ring = nullptr;
     int mc_ring_cap = 0;
     double mc_dx = 0, mc_dy = 0, xomega = 0;
@@ -145,9 +164,11 @@ struct iw_solver {
 namespace iw {
 // kernels launchers (iw_kernels.hip)
 int mc_blocks(const Geo &g);
-void launch_maxcalc(const Geo &g, const McArgs &A, int istep, double t, double umax, double *colmax,
-                    double *groupmax, double *rec, hipStream_t st);
-void launch_fft_x(const Geo &g, FieldBatch b, int nk, int dir, hipStream_t st);
+void launch_maxcalc_final(const Geo &g, const McArgs &A, int istep, double t, double umax, double *rec,
+                          hipStream_t st);
+void launch_maxcalc_umaxt(const Geo &g, const McArgs &A, double umax, double *colmax, double *groupmax,
+                          double *rec, hipStream_t st);
+void launch_fft_x(const Geo &g, FieldBatch b, int nk, int dir, hipStream_t st, const McFuse *mf = nullptr);
 void launch_fft_y(const Geo &g, FieldBatch b, int nk, int dir, hipStream_t st);
 void launch_dpdz(const Geo &g, const double *a, double *b, hipStream_t st);
 void launch_dpdz_conv(const Geo &g, double *const *uvwt, double *const *out4, hipStream_t st);
@@ -173,7 +194,7 @@ void launch_pressure(const Geo &g, const iw_params &P, const SysTables &S, doubl
                      const double *wnl, const double *wviscrot, double dt, int cpen, hipStream_t st);
 void launch_viscous(const Geo &g, const iw_params &P, const SysTables &S, double *const *uvwt, double dt,
                     int cpen, hipStream_t st);
-void launch_zero_meanw(const Geo &g, double *w, hipStream_t st);
+void launch_zero_meanw(const Geo &g, double *w, double *w00, hipStream_t st);
 void launch_zero_int(int *p, int n, hipStream_t st);
 bool solve_np_supported(int nl, int nsubd);
 void launch_solve_nat(const Geo &g, const iw_params &P, const SysTables &S, double *const *uvwt, double dt,

@@ -193,10 +193,19 @@ __device__ __forceinline__ void stage_tables(double *wgs, double *ALs, double *s
 // (fft.f:1483-1490).  Implemented as one complex FFT of length M=N/2 on the
 // even/odd-packed row with LDS-staged radix-2 passes.
 // ---------------------------------------------------------------------------
+// maxcalc extrema of one field (various.f:749-851), ties to the later point
+__device__ __forceinline__ void mc_take(bool mn, double x, int lx, double &b, int &lb)
+{
+    if (mn ? (x < b || (x == b && lx > lb)) : (x > b || (x == b && lx > lb))) {
+        b = x;
+        lb = lx;
+    }
+}
+
 template <int DIR>
 __global__ __launch_bounds__(256) void k_fft_x(FieldBatch fb, int nrows, int ny, int nxpp, int M, int logM,
                                                int R, const double2 *__restrict__ twM,
-                                               const double2 *__restrict__ twN, double scaley)
+                                               const double2 *__restrict__ twN, double scaley, McFuse mf)
 {
     extern __shared__ double2 smx[];
     double2 *buf = smx;
@@ -231,6 +240,7 @@ __global__ __launch_bounds__(256) void k_fft_x(FieldBatch fb, int nrows, int ny,
                 const double *p = base + srow(row);
                 double2 va = *reinterpret_cast<const double2 *>(p + 2 * k);
                 double2 vb = *reinterpret_cast<const double2 *>(p + 2 * (M - k));
+                if (!deriv && k == 0) nyq[r] = vb; // kx-Nyquist coefficient of the row
                 if (deriv) {
                     const double ka = fb.kw[k], kb = fb.kw[M - k];
                     va = make_double2(-ka * va.y, ka * va.x);
@@ -347,6 +357,75 @@ __global__ __launch_bounds__(256) void k_fft_x(FieldBatch fb, int nrows, int ny,
         __syncthreads();
         fft_lds<false>(buf, logM, R, tw, DIR);
         store_inverse(obase);
+        if (mf.part && blockIdx.y < 4) { // maxcalc reductions of u, v, w, temp
+            const int f = blockIdx.y, nz = nrows >> lny;
+            const double big = 1.79769313486231570e308;
+            double b0 = big, b1 = -big, b2 = -big; // min, max of the value; max of the CFL number
+            int l0 = -1, l1 = -1, l2 = -1;
+#pragma unroll
+            for (int it = 0; it < kFE / 256; it++) {
+                const int idx = threadIdx.x + it * 256;
+                const int r = idx >> logM, m = idx & (M - 1), row = row0 + r;
+                if (row >= nrows) continue;
+                const int k = row >> lny, j = row & (ny - 1);
+                const double2 z = buf[lp(idx)];
+                const double c = mf.cny * nyq[r].x;
+                double xe = z.x + c, xo = z.y - c; // points i = 2m, 2m+1
+                if (f == 2 && mf.w00) {
+                    xe = xe + mf.w00[k];
+                    xo = xo + mf.w00[k];
+                }
+                const int L = ((2 * m) * ny + j) * nz + k, L1 = L + ny * nz;
+                double ve = xe + 0.0, vo = xo + 0.0; // + umwv, wmwv, rhomwv (zero)
+                if (f == 3) {
+                    ve = ve + mf.rm[k]; // + rhomean
+                    vo = vo + mf.rm[k];
+                }
+                mc_take(true, ve, L, b0, l0);
+                mc_take(true, vo, L1, b0, l0);
+                mc_take(false, ve, L, b1, l1);
+                mc_take(false, vo, L1, b1, l1);
+                if (f < 3) { // uo*fcshear = 0 (main.f:120); no CFLZ at the top point
+                    const double cf = f == 0 ? mf.cx : f == 1 ? mf.cy : (k == nz - 1 ? 0.0 : mf.dt * mf.rdz[k]);
+                    mc_take(false, fabs(xe * cf), L, b2, l2);
+                    mc_take(false, fabs(xo * cf), L1, b2, l2);
+                }
+            }
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+                double x;
+                int lx;
+                x = __shfl_down(b0, off, 64); lx = __shfl_down(l0, off, 64); mc_take(true, x, lx, b0, l0);
+                x = __shfl_down(b1, off, 64); lx = __shfl_down(l1, off, 64); mc_take(false, x, lx, b1, l1);
+                x = __shfl_down(b2, off, 64); lx = __shfl_down(l2, off, 64); mc_take(false, x, lx, b2, l2);
+            }
+            __shared__ double rv[4][3];
+            __shared__ int rl[4][3];
+            const int wv = threadIdx.x >> 6;
+            if ((threadIdx.x & 63) == 0) {
+                rv[wv][0] = b0; rv[wv][1] = b1; rv[wv][2] = b2;
+                rl[wv][0] = l0; rl[wv][1] = l1; rl[wv][2] = l2;
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                for (int w = 1; w < 4; w++) {
+                    mc_take(true, rv[w][0], rl[w][0], b0, l0);
+                    mc_take(false, rv[w][1], rl[w][1], b1, l1);
+                    mc_take(false, rv[w][2], rl[w][2], b2, l2);
+                }
+                // quantities: 0 umin 1 umax 2 vmin 3 vmax 4 wmin 5 wmax 6 tmin
+                // 7 tmax 8 cflx 9 cfly 10 cflz
+                double *p = mf.part + (size_t)blockIdx.x * (2 * kMcQ);
+                p[4 * f] = b0;
+                p[4 * f + 1] = (double)l0;
+                p[4 * f + 2] = b1;
+                p[4 * f + 3] = (double)l1;
+                if (f < 3) {
+                    p[2 * (8 + f)] = b2;
+                    p[2 * (8 + f) + 1] = (double)l2;
+                }
+            }
+        }
     }
 }
 
@@ -467,19 +546,22 @@ __global__ __launch_bounds__(256) void k_fft_y(FieldBatch fb, int nxpp, int nxh,
         }
 }
 
-void launch_fft_x(const Geo &g, FieldBatch b, int nk, int dir, hipStream_t st)
+void launch_fft_x(const Geo &g, FieldBatch b, int nk, int dir, hipStream_t st, const McFuse *mf)
 {
     const int R = kFE / g.M;
     int nrows = g.ny * nk;
     dim3 grid((nrows + R - 1) / R, b.n);
     size_t sh = sizeof(double2) * ((size_t)lp(R * g.M) + 1 + g.M + R);
+    McFuse none;
     if (dir < 0)
         hipLaunchKernelGGL(k_fft_x<-1>, grid, dim3(256), sh, st, b, nrows, g.ny, g.nxpp, g.M, g.logM, R,
-                           g.twM, g.twN, g.scaley);
+                           g.twM, g.twN, g.scaley, none);
     else
         hipLaunchKernelGGL(k_fft_x<1>, grid, dim3(256), sh, st, b, nrows, g.ny, g.nxpp, g.M, g.logM, R,
-                           g.twM, g.twN, g.scaley);
+                           g.twM, g.twN, g.scaley, mf ? *mf : none);
 }
+
+int mc_blocks(const Geo &g) { return (g.ny * g.nz + kFE / g.M - 1) / (kFE / g.M); }
 
 void launch_fft_y(const Geo &g, FieldBatch b, int nk, int dir, hipStream_t st)
 {
@@ -623,10 +705,13 @@ void launch_dpdz_conv(const Geo &g, double *const *uvwt, double *const *out4, hi
 }
 
 // mean w = 0 (solver.f:66-69)
-__global__ void k_zero_meanw(double *w, long nxppy, int nz)
+// w(kx=0,ky=0) = 0 (solver.f:66-69); the removed real part is kept in w00
+// (maxcalc's values are recovered from convec's transform with it)
+__global__ void k_zero_meanw(double *w, double *w00, long nxppy, int nz)
 {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k < nz) {
+        w00[k] = w[(long)k * nxppy];
         w[(long)k * nxppy] = 0.;
         w[(long)k * nxppy + 1] = 0.;
     }
@@ -643,9 +728,9 @@ void launch_zero_int(int *p, int n, hipStream_t st)
     hipLaunchKernelGGL(k_zero_int, dim3((n + 63) / 64), dim3(64), 0, st, p, n);
 }
 
-void launch_zero_meanw(const Geo &g, double *w, hipStream_t st)
+void launch_zero_meanw(const Geo &g, double *w, double *w00, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_zero_meanw, dim3((g.nz + 63) / 64), dim3(64), 0, st, w, g.nxppy, g.nz);
+    hipLaunchKernelGGL(k_zero_meanw, dim3((g.nz + 63) / 64), dim3(64), 0, st, w, w00, g.nxppy, g.nz);
 }
 
 // spectral x/y gradients i kx f, i ky f of u,v,w (solver.f:86-103)

@@ -1,5 +1,8 @@
 // maxcalc (jmsegret/IWrepo mode1_bbl, various.f:665-987) on the device: the
-// running extrema and CFL maxima of the physical u, v, w, temp.
+// running extrema and CFL maxima of the physical u, v, w, temp.  The
+// per-block partial results come from the epilogue of the inverse x
+// transform (k_fft_x); the kernels here combine them into the record and
+// follow the umaxt chain.
 //
 // The reference scans i (outer), j, then z (inner) and keeps a later point
 // on a tie (.ge./.le. comparisons, various.f:755-845).  With L = (i ny + j)
@@ -19,7 +22,6 @@ namespace iw {
 
 namespace {
 
-constexpr int kMcE = 2048;   // plane elements per block (8 per thread)
 constexpr int kMcThreads = 256;
 
 // quantities: 0 umin 1 umax 2 vmin 3 vmax 4 wmin 5 wmax 6 tmin 7 tmax
@@ -90,59 +92,28 @@ __device__ void mc_block_reduce(McAcc &a)
                 }
 }
 
-// one block: kMcE points of plane k = blockIdx.y
-__global__ __launch_bounds__(kMcThreads) void k_mc_reduce(McArgs A)
+// physical u at (i,j,k) as maxcalc sees it: the transformed value plus,
+// after convec's transform, the kx-Nyquist term that the round trip scaled
+// by 1/ny (the epilogue of k_fft_x forms the same sum)
+__device__ __forceinline__ double mc_u(const McArgs &A, int i, int j, int k)
 {
-    const int k = blockIdx.y;
-    const int nxy = A.nx * A.ny;
-    const double rm = A.rm[k];
-    const bool top = k == A.nz - 1;
-    const double dz = A.dz[k];
-    const long pk = (long)k * A.nxppy;
-    McAcc acc;
-    acc.init();
-#pragma unroll
-    for (int it = 0; it < kMcE / kMcThreads; it++) {
-        const int e = blockIdx.x * kMcE + it * kMcThreads + threadIdx.x;
-        if (e >= nxy) break;
-        const int j = e / A.nx, i = e - j * A.nx;
-        const long o = pk + (long)j * A.nxpp + i;
-        const int L = (i * A.ny + j) * A.nz + k;
-        const double u = A.u[o], v = A.v[o], w = A.w[o], t = A.t[o];
-        const double uu = u + 0.0, ww = w + 0.0; // + umwv, wmwv (zero)
-        acc.add(0, uu, L);
-        acc.add(1, uu, L);
-        acc.add(2, v, L);
-        acc.add(3, v, L);
-        acc.add(4, ww, L);
-        acc.add(5, ww, L);
-        const double tt = (t + 0.0) + rm; // + rhomwv (zero) + rhomean
-        acc.add(6, tt, L);
-        acc.add(7, tt, L);
-        // uo*fcshear*fshear = 0 (fcshear = 0, main.f:120)
-        acc.add(8, fabs((u * A.dt) / A.dx), L);
-        acc.add(9, fabs((v * A.dt) / A.dy), L);
-        acc.add(10, top ? 0.0 : fabs((w * A.dt) / dz), L);
+    const long o = (long)k * A.nxppy + (long)j * A.nxpp;
+    double x = A.u[o + i];
+    if (A.cny != 0.0) {
+        const double c = A.cny * A.spec[o + A.nx];
+        x = (i & 1) ? x - c : x + c;
     }
-    mc_block_reduce(acc);
-    if (threadIdx.x == 0) {
-        double *p = A.part + (size_t)(blockIdx.y * gridDim.x + blockIdx.x) * (2 * kMcQ);
-        for (int q = 0; q < kMcQ; q++) {
-            p[2 * q] = acc.v[q];
-            p[2 * q + 1] = (double)acc.l[q];
-        }
-    }
+    return x;
 }
 
 // partials -> the record (layout of iw_maxcalc in iwgpu.h); entries the
 // reference keeps from the previous call (itmin, jtmin, dzmax) stay when
 // there is no new record
-__global__ __launch_bounds__(kMcThreads) void k_mc_final(McArgs A, int nblk, int istep, double t, double umax,
-                                                        double *rec)
+__global__ __launch_bounds__(kMcThreads) void k_mc_final(McArgs A, int istep, double t, double umax, double *rec)
 {
     McAcc acc;
     acc.init();
-    for (int b = threadIdx.x; b < nblk; b += kMcThreads) {
+    for (int b = threadIdx.x; b < A.nblk; b += kMcThreads) {
         const double *p = A.part + (size_t)b * (2 * kMcQ);
         for (int q = 0; q < kMcQ; q++) acc.add(q, p[2 * q], (int)p[2 * q + 1]);
     }
@@ -200,29 +171,41 @@ __global__ __launch_bounds__(kMcThreads) void k_mc_final(McArgs A, int nblk, int
     // uveltest.dat probe u(32,1,10), u(32,1,50) by linear position
     const long ntot = (long)A.nxppy * nz;
     const long p1 = 31 + (long)A.nxpp * ny * 9, p2 = 31 + (long)A.nxpp * ny * 49;
-    rec[40] = p1 < ntot ? A.u[p1] : __builtin_nan("");
-    rec[41] = p2 < ntot ? A.u[p2] : __builtin_nan("");
+    auto probe = [&](long p) {
+        if (p >= ntot) return __builtin_nan("");
+        const int i = (int)(p % A.nxpp), j = (int)((p / A.nxpp) % ny), k = (int)(p / A.nxppy);
+        return i < A.nx ? mc_u(A, i, j, k) : A.u[p]; // the pads hold zeros
+    };
+    rec[40] = probe(p1);
+    rec[41] = probe(p2);
 }
 
-// column maxima of u in L order: cm[i ny + j] = max_k u(i,j,k)
-__global__ __launch_bounds__(256) void k_mc_colmax(const double *__restrict__ u, int nx, int ny, int nz, int nxpp,
-                                                   long nxppy, double *cm)
+// partial column maxima of u over kMcSlices slices of z:
+// cp[kc][i ny + j] = max over the slice of u(i,j,k)
+__global__ __launch_bounds__(256) void k_mc_colmax(McArgs A, double *cp)
 {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= nx * ny) return;
-    const int j = e / nx, i = e - j * nx;
-    const double *p = u + (long)j * nxpp + i;
+    const int nxy = A.nx * A.ny;
+    if (e >= nxy) return;
+    const int j = e / A.nx, i = e - j * A.nx;
+    const int kc = blockIdx.y, k0 = kc * A.nz / kMcSlices, k1 = (kc + 1) * A.nz / kMcSlices;
     double m = -DBL_MAX;
-    for (int k = 0; k < nz; k++) m = fmax(m, p[(long)k * nxppy] + 0.0);
-    cm[i * ny + j] = m;
+    for (int k = k0; k < k1; k++) m = fmax(m, mc_u(A, i, j, k) + 0.0);
+    cp[(size_t)kc * nxy + i * A.ny + j] = m;
 }
 
-// maxima of groups of 256 consecutive columns
-__global__ __launch_bounds__(256) void k_mc_groupmax(const double *__restrict__ cm, int ncol, double *gm)
+// column maxima cm (from the slices) and the maxima of groups of 256
+// consecutive columns
+__global__ __launch_bounds__(256) void k_mc_groupmax(const double *__restrict__ cp, int ncol, double *cm,
+                                                     double *gm)
 {
     __shared__ double s[4];
     const int c = blockIdx.x * 256 + threadIdx.x;
-    double m = c < ncol ? cm[c] : -DBL_MAX;
+    double m = -DBL_MAX;
+    if (c < ncol) {
+        for (int kc = 0; kc < kMcSlices; kc++) m = fmax(m, cp[(size_t)kc * ncol + c]);
+        cm[c] = m;
+    }
     for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_down(m, off, 64));
     if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = m;
     __syncthreads();
@@ -245,21 +228,20 @@ __device__ int mc_block_first(bool flag)
 // u >= umaxt becomes the record and umaxt = u + umax.  One work-group.
 // Every loop advances (c, k0) or ends, so the walk finishes after at most
 // ncol column steps plus one step per record.
-__global__ __launch_bounds__(256) void k_mc_walk(const double *__restrict__ u, const double *__restrict__ cm,
-                                                 const double *__restrict__ gm, int nx, int ny, int nz, int nxpp,
-                                                 long nxppy, double umax, double *rec)
+__global__ __launch_bounds__(256) void k_mc_walk(McArgs A, const double *__restrict__ cm,
+                                                 const double *__restrict__ gm, double umax, double *rec)
 {
-    const int ncol = nx * ny, ngrp = (ncol + 255) / 256;
+    const int ny = A.ny, nz = A.nz;
+    const int ncol = A.nx * ny, ngrp = (ncol + 255) / 256;
     double thr = 0.0;
     int kmax = 0;
     int c = 0, k0 = 0;
     // first k >= k0 in column c with u >= thr, or nz
     auto scan_column = [&](int cc, int kb) -> int {
         const int i = cc / ny, j = cc - i * ny;
-        const double *p = u + (long)j * nxpp + i;
         for (int k = kb; k < nz; k += 256) {
             const int kt = k + (int)threadIdx.x;
-            const bool f = kt < nz && (p[(long)kt * nxppy] + 0.0) >= thr;
+            const bool f = kt < nz && (mc_u(A, i, j, kt) + 0.0) >= thr;
             const int r = mc_block_first(f);
             if (r < 256) return k + r;
         }
@@ -295,7 +277,7 @@ __global__ __launch_bounds__(256) void k_mc_walk(const double *__restrict__ u, c
         const int k = scan_column(c, k0);
         if (k < nz) {
             const int i = c / ny, j = c - i * ny;
-            thr = (u[(long)k * nxppy + (long)j * nxpp + i] + 0.0) + umax;
+            thr = (mc_u(A, i, j, k) + 0.0) + umax;
             kmax = k + 1;
             k0 = k + 1;
             if (k0 == nz) {
@@ -315,23 +297,21 @@ __global__ __launch_bounds__(256) void k_mc_walk(const double *__restrict__ u, c
 
 } // namespace
 
-int mc_blocks(const Geo &g) { return ((g.nx * g.ny + kMcE - 1) / kMcE) * g.nz; }
+void launch_maxcalc_final(const Geo &g, const McArgs &A, int istep, double t, double umax, double *rec,
+                          hipStream_t st)
+{
+    hipLaunchKernelGGL(k_mc_final, dim3(1), dim3(kMcThreads), 0, st, A, istep, t, umax, rec);
+}
 
-void launch_maxcalc(const Geo &g, const McArgs &A, int istep, double t, double umax, double *colmax,
-                    double *groupmax, double *rec, hipStream_t st)
+// umaxt chain (umax != 0); colmax holds (kMcSlices + 1) nx ny doubles
+void launch_maxcalc_umaxt(const Geo &g, const McArgs &A, double umax, double *colmax, double *groupmax,
+                          double *rec, hipStream_t st)
 {
     const int nxy = g.nx * g.ny;
-    dim3 grid((nxy + kMcE - 1) / kMcE, g.nz);
-    hipLaunchKernelGGL(k_mc_reduce, grid, dim3(kMcThreads), 0, st, A);
-    hipLaunchKernelGGL(k_mc_final, dim3(1), dim3(kMcThreads), 0, st, A, (int)(grid.x * grid.y), istep, t, umax,
-                       rec);
-    if (umax != 0.0) {
-        hipLaunchKernelGGL(k_mc_colmax, dim3((nxy + 255) / 256), dim3(256), 0, st, A.u, g.nx, g.ny, g.nz, g.nxpp,
-                           g.nxppy, colmax);
-        hipLaunchKernelGGL(k_mc_groupmax, dim3((nxy + 255) / 256), dim3(256), 0, st, colmax, nxy, groupmax);
-        hipLaunchKernelGGL(k_mc_walk, dim3(1), dim3(256), 0, st, A.u, colmax, groupmax, g.nx, g.ny, g.nz, g.nxpp,
-                           g.nxppy, umax, rec);
-    }
+    double *cm = colmax + (size_t)kMcSlices * nxy;
+    hipLaunchKernelGGL(k_mc_colmax, dim3((nxy + 255) / 256, kMcSlices), dim3(256), 0, st, A, colmax);
+    hipLaunchKernelGGL(k_mc_groupmax, dim3((nxy + 255) / 256), dim3(256), 0, st, colmax, nxy, cm, groupmax);
+    hipLaunchKernelGGL(k_mc_walk, dim3(1), dim3(256), 0, st, A, cm, groupmax, umax, rec);
 }
 
 } // namespace iw
