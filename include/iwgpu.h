@@ -47,6 +47,8 @@ typedef struct iw_params {
     double zcen, xbcrit;           /* mean-u damping (calcuvwt.f:343-361)          */
     double xacrit, umax, fwave;    /* virtual paddle (forcing.f:325-447)           */
     double fackcbv;                /* Boyd-Vandeven cut-off fraction (filtering.f:1040) */
+    double fcshear, uo;            /* shear current uo*fcshear*fshear(z) (main.f:120,
+                                      352; forcing.f:190-214): 0 and 0.07 there   */
     double z0[IW_MAXSUB], zh[IW_MAXSUB]; /* subdomains (subd.dat, setup.f:316-325) */
 } iw_params;
 

@@ -39,6 +39,7 @@ c     layout of iw_params (include/iwgpu.h)
         real(c_double) :: zcen, xbcrit
         real(c_double) :: xacrit, umax, fwave
         real(c_double) :: fackcbv
+        real(c_double) :: fcshear, uo
         real(c_double) :: z0(iw_maxsub), zh(iw_maxsub)
       end type
       type(c_ptr), save :: iwh = c_null_ptr
@@ -223,6 +224,7 @@ c-----------------------------------------------------------------------
       include 'comspecfilter.h'
       include 'compencoeff.h'
       include 'comcritlayer.h'
+      include 'comsoliton.h'
       common /dtanimation/ dtanim, fwave
       type(iw_params_t) :: iwp
       integer ks
@@ -286,6 +288,8 @@ c     Tso kept between calls
       iwp%umax = umax
       iwp%fwave = fwave
       iwp%fackcbv = fackcbv
+      iwp%fcshear = fcshear
+      iwp%uo = uo
       iwp%z0 = 0.d0
       iwp%zh = 0.d0
       do ks = 1, nsubd

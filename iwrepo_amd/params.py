@@ -27,7 +27,7 @@ class IWParams(ctypes.Structure):
         ("facdir", ctypes.c_double), ("facneumn", ctypes.c_double), ("facrobin", ctypes.c_double),
         ("zcen", ctypes.c_double), ("xbcrit", ctypes.c_double),
         ("xacrit", ctypes.c_double), ("umax", ctypes.c_double), ("fwave", ctypes.c_double),
-        ("fackcbv", ctypes.c_double),
+        ("fackcbv", ctypes.c_double), ("fcshear", ctypes.c_double), ("uo", ctypes.c_double),
         ("z0", ctypes.c_double * MAXSUB), ("zh", ctypes.c_double * MAXSUB),
     ]
 
@@ -53,7 +53,8 @@ def make_params(nx, ny, nsubd, order, xlen=2 * math.pi, ylen=math.pi / 2, zlen=1
                 fackc=0.0, facdir=1.0, facneumn=1.0, facrobin=1e4, navgintrfc=1,
                 ispecfilternl=1, ispecfilterexp=1, nspecfilterexp=1, zcen=0.13, xbcrit=0.84,
                 xacrit=0.0, umax=0.05, fwave=0.0, heights=None, grav=1.0, rho0=1.0,
-                ifilter=0, nstepfilter=5, npostp=100000000, fackcbv=0.0, imaxcalc=1, dtadapt=0):
+                ifilter=0, nstepfilter=5, npostp=100000000, fackcbv=0.0, imaxcalc=1, dtadapt=0,
+                fcshear=0.0, uo=0.07):
     """Build a parameter block.  ``order`` is the GLL polynomial order p
     (nzloc = p+1); z in [0,zlen] is split into equal subdomains unless
     ``heights`` gives them (subd.dat, setup.f:297-330)."""
@@ -76,6 +77,7 @@ def make_params(nx, ny, nsubd, order, xlen=2 * math.pi, ylen=math.pi / 2, zlen=1
     P.xacrit, P.umax, P.fwave = xacrit, umax, fwave
     P.ifilter, P.nstepfilter, P.npostp, P.fackcbv = ifilter, nstepfilter, npostp, fackcbv
     P.imaxcalc, P.dtadapt = imaxcalc, dtadapt
+    P.fcshear, P.uo = fcshear, uo
     if heights is None:
         heights = [zlen / nsubd] * nsubd
     z = 0.0

@@ -46,6 +46,10 @@ struct iwo_state {
     /* maxcalc (various.f:665-987): Tso, the /vardtflags/ ivardt, idtstep
        (comtime.h:3), itmin/jtmin and dz, which the reference keeps from
        the previous call, and the per-step records of the last iwo_run */
+    /* /sol2dvar/ (comsoliton.h:9-13), each (nxpp,nz): 0 umwv, 1 wmwv,
+       2 dumdx, 3 dumdz, 4 dwmdx, 5 dwmdz, 6 rhomwv, 7 drhomdx, 8 drhomdz;
+       zero unless set (wavesetup is not called, setup.f:191) */
+    double *mw[9];
     double Tso, dzlast;
     int ivardt, idtstep, itmin, jtmin;
     double *mcrec;
@@ -298,6 +302,7 @@ iwo_state *iwo_create(const iwo_params *p)
     s->twave = s->pi2 / s->xomega;
     s->dt = p->dt;
     s->cdt = 1.; /* cdtin (input.dat) */
+    for (int q = 0; q < 9; q++) s->mw[q] = (double *)calloc((size_t)s->nxpp * s->nz, sizeof(double));
     s->fact = 1. / (p->xkappa * s->dt);
     s->facvel = 1. / (p->xnu * s->dt);
     int nl = s->nzloc;
@@ -364,6 +369,7 @@ void iwo_destroy(iwo_state *s)
     for (int f = 0; f < 8; f++) free(s->work[f]);
     free(s->wnl); free(s->wviscrot);
     free(s->mcrec);
+    for (int q = 0; q < 9; q++) free(s->mw[q]);
     free(s->ub); free(s->vb); free(s->wb); free(s->phi); free(s->dphidz);
     free(s->twx_c); free(s->twx_s); free(s->twy_c); free(s->twy_s);
     free(s);
@@ -955,6 +961,27 @@ static double phit(const iwo_state *s, double t)
 { /* forcing.f:424-447 */
     return (t <= s->P.fwave * s->twave) ? 1. : 0.;
 }
+/* shear-current profile and its derivative, forcing.f:190-214 (their
+   zdim, zsh, xacu arguments are not used there) */
+static double fshear(double z)
+{
+    const double dt = 0.13, z0 = 0.2625, zdom = 0.45;
+    const double gam = 4.0 / (dt * zdom);
+    return 0.5 * (tanh(gam * (z - z0)) + 1);
+}
+static double dfsheardz(double z)
+{
+    const double dt = 0.13, z0 = 0.2625, zdom = 0.45;
+    const double gam = 4.0 / (dt * zdom);
+    const double c = 1.0 / cosh(gam * (z - z0));
+    return 0.5 * gam * (c * c);
+}
+
+void iwo_set_meanwave(iwo_state *s, const double *mw)
+{
+    const size_t n = (size_t)s->nxpp * s->nz;
+    for (int q = 0; q < 9; q++) memcpy(s->mw[q], mw + q * n, sizeof(double) * n);
+}
 
 /* ------------------------------------------------------------------ */
 /* rhsfm, solver.f:523-1206 (all arrays except u,v,w,temp physical)      */
@@ -988,7 +1015,10 @@ static void rhsfm(iwo_state *s)
         for (int ks = 0; ks < s->nsubd; ks++) {
             double ybar = s->P.zh[ks];
             long kb = (long)ks * nl, kt = (long)ks * nl + nl - 1;
-            double wb = w[kb * nxppy + ij], wt = w[kt * nxppy + ij];
+            /* total vertical velocity, perturbation + mean wave */
+            const int i = (int)(ij % nxpp);
+            double wb = w[kb * nxppy + ij] + s->mw[1][kb * nxpp + i];
+            double wt = w[kt * nxppy + ij] + s->mw[1][kt * nxpp + i];
             if (kb == 0) wb = 0.;
             if (kt == nz - 1) wt = 0.;
             double alpha0, tau1, gamma0, tau2;
@@ -1004,8 +1034,33 @@ static void rhsfm(iwo_state *s)
                 sf[kt * nxppy + ij] = sf[kt * nxppy + ij] - tau2 * (gamma0 * f[kt * nxppy + ij] - pct);
             }
         }
-    /* solver.f:848-931 mean-wave and shear terms vanish (umwv=wmwv=0,
-       fcshear=0, d2rhomeandz2=0); virtual paddle solver.f:941-956 */
+    /* mean-wave and shear-current terms, solver.f:848-931: advection of the
+       perturbation by the mean flow and of the mean flow by the
+       perturbation; d2rhomeandz2 = 0 (forcing.f:167-183) */
+    {
+        const double uofc = s->P.uo * s->P.fcshear;
+        double **mw = s->mw;
+        for (int k = 0; k < nz; k++) {
+            const double zz = s->z[k];
+            const double fsh = fshear(zz), dfsh = dfsheardz(zz);
+            for (int j = 0; j < s->ny; j++)
+                for (int i = 0; i < nxpp; i++) {
+                    const long n = (long)k * nxppy + (long)j * nxpp + i;
+                    const long m = (long)k * nxpp + i;
+                    const double advmean = mw[0][m] + uofc * fsh;
+                    su[n] = su[n] - advmean * g[0][n] - mw[1][m] * g[2][n];
+                    sv[n] = sv[n] - advmean * g[3][n] - mw[1][m] * g[5][n];
+                    sw[n] = sw[n] - advmean * g[6][n] - mw[1][m] * g[8][n];
+                    st[n] = st[n] - advmean * g[9][n] - mw[1][m] * g[11][n];
+                    const double dusheardz = uofc * dfsh;
+                    su[n] = su[n] - u[n] * mw[2][m] - w[n] * mw[3][m] - w[n] * dusheardz;
+                    sw[n] = sw[n] - u[n] * mw[4][m] - w[n] * mw[5][m];
+                    st[n] = st[n] - u[n] * mw[7][m] - w[n] * mw[8][m];
+                    st[n] = st[n] + s->P.xkappa * 0.0;
+                }
+        }
+    }
+    /* virtual paddle solver.f:941-956 */
     if (s->P.xacrit != 0.) {
         double ph = phit(s, s->t), A = s->P.xacrit * s->P.umax, um = s->P.umax;
         for (int k = 0; k < nz; k++) {
@@ -1650,7 +1705,8 @@ void iwo_maxcalc(iwo_state *s, int istep, double *rec)
                 for (int kl = 0; kl < nzl; kl++) {
                     const int k = ks * nzl + kl, kg = ks * (nzl - 1) + kl + 1;
                     const long o = (long)k * s->nxppy + (long)j * s->nxpp + i;
-                    const double uu = u[o] + 0.0, vv = v[o], ww = w[o] + 0.0; /* umwv = wmwv = 0 */
+                    const long m = (long)k * s->nxpp + i;
+                    const double uu = u[o] + s->mw[0][m], vv = v[o], ww = w[o] + s->mw[1][m];
                     if (uu >= umaxt) { umaxt = uu + s->P.umax; kumax = k + 1; }
                     if (uu <= umin) { umin = uu; kumin = k + 1; }
                     if (vv >= vmax) { vmax = vv; kvmax = k + 1; }
@@ -1658,11 +1714,10 @@ void iwo_maxcalc(iwo_state *s, int istep, double *rec)
                     if (ww >= wmax) { wmax = ww; kwmax = k + 1; }
                     if (ww <= wmin) { wmin = ww; kwmin = k + 1; }
                     const double rm = 1. + rhograd * s->z[k]; /* rhomean, forcing.f:134-141 */
-                    const double tt = (tp[o] + 0.0) + rm;     /* rhomwv = 0 */
+                    const double tt = (tp[o] + s->mw[6][m]) + rm;
                     if (tt >= tmax) { tmax = tt; ktmax = k + 1; itmax = i + 1; jtmax = j + 1; }
                     if (tt <= tmin) { tmin = tt; ktmin = k + 1; s->itmin = i + 1; s->jtmin = j + 1; }
-                    /* uo*fcshear*fshear = 0: fcshear = 0 (main.f:120) */
-                    const double cflx = fabs((u[o] * s->dt) / dx);
+                    const double cflx = fabs(((s->P.uo * s->P.fcshear * fshear(s->z[k]) + u[o]) * s->dt) / dx);
                     const double cfly = fabs((v[o] * s->dt) / dy);
                     double cflz;
                     if (k + 1 == nz) {

@@ -34,6 +34,7 @@ typedef struct iwo_params {
     double zcen, xbcrit;
     double xacrit, umax, fwave;
     double fackcbv;
+    double fcshear, uo; /* shear current, main.f:120, 352 */
     double z0[IWO_MAXSUB], zh[IWO_MAXSUB];
 } iwo_params;
 
@@ -75,6 +76,9 @@ void iwo_filtering(iwo_state *s, int isgscalc);
    5 t33, 6 th1, 7 th2, 8 th3 (comsgs.h) */
 double *iwo_sgs(iwo_state *s, int idx);
 
+/* the nine (nxpp,nz) arrays of /sol2dvar/ (comsoliton.h:9-13) in COMMON
+   order: umwv, wmwv, dumdx, dumdz, dwmdx, dwmdz, rhomwv, drhomdx, drhomdz */
+void iwo_set_meanwave(iwo_state *s, const double *mw);
 /* maxcalc (various.f:665-987) on the spectral state: the round trip, the
    extrema/CFL record and, with dtadapt, the adaptive step.  Record layout
    (IWO_NREC doubles; indices 1-based as the reference prints them, 0 =

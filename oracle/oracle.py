@@ -46,6 +46,7 @@ def lib():
         L.iwo_filtering.argtypes = [P, ctypes.c_int]
         L.iwo_set_restart.argtypes = [P, P, D, D]
         L.iwo_maxcalc.argtypes = [P, ctypes.c_int, P]
+        L.iwo_set_meanwave.argtypes = [P, P]
         L.iwo_maxcalc_records.restype = ctypes.POINTER(D)
         L.iwo_maxcalc_records.argtypes = [P, ctypes.POINTER(ctypes.c_int)]
         L.iwo_sgs.restype = ctypes.POINTER(D)
@@ -129,6 +130,12 @@ class Oracle:
 
     def set_field(self, idx, a):
         self.field(idx)[...] = np.asarray(a, dtype=np.float64).reshape(self.shape)
+
+    def set_meanwave(self, mw):
+        """/sol2dvar/ arrays: 9 x (nz, nxpp) in COMMON order."""
+        a = np.ascontiguousarray(mw, dtype=np.float64)
+        assert a.size == 9 * self.shape[0] * self.shape[2]
+        self.L.iwo_set_meanwave(self.h, _ptr(a))
 
     def set_restart(self, state, t, cdt=1.0):
         st = np.ascontiguousarray(state, dtype=np.float64)
@@ -233,7 +240,7 @@ def _parse_maxcalc(d):
 
 
 def run_ref(build, P, fields, nsteps=1, mode=0, probe=None, exe="iwref", isgs=0, start=None, cdt=1.0,
-            write_restart=False, maxcalc=False):
+            write_restart=False, maxcalc=False, meanwave=None):
     """Run the reference build on physical ``fields`` (u,v,w,rho').
 
     ``exe="iwshim"`` runs the same harness linked with the product's Fortran
@@ -242,7 +249,9 @@ def run_ref(build, P, fields, nsteps=1, mode=0, probe=None, exe="iwref", isgs=0,
     ``write_restart`` the restart file written by output.f at the end is
     returned as a fourth item.  With ``maxcalc`` the reference maxcalc runs
     every step (adaptive dt included) and the parsed outputs
-    (``_parse_maxcalc``) are returned as the last item.
+    (``_parse_maxcalc``) are returned as the last item.  ``P.fcshear`` sets
+    the shear current; ``meanwave`` (9 arrays (nz, nxpp) in /sol2dvar/
+    order) the mean-wave fields.
     Returns (state[24], t, diag_lines[, restart_bytes][, maxcalc_outputs])."""
     exe = os.path.join(REFDIR, build, exe)
     nz, ny, nxpp = P.nzloc * P.nsubd, P.ny, P.nx + 2
@@ -259,9 +268,12 @@ def run_ref(build, P, fields, nsteps=1, mode=0, probe=None, exe="iwref", isgs=0,
         pr = probe or (1, 1, 1)
         with open(os.path.join(d, "harness.nml"), "w") as f:
             f.write("&harness icfile='ic.bin', outfile='out.bin', diagfile='diag.txt', "
-                    "nrun=%d, mode=%d, iprobe=%d, ipx=%d, kp1=%d, kp2=%d, isgs=%d, iwrite=%d, imaxc=%d /\n"
+                    "nrun=%d, mode=%d, iprobe=%d, ipx=%d, kp1=%d, kp2=%d, isgs=%d, iwrite=%d, imaxc=%d, "
+                    "fcshr=%r, mwfile='%s' /\n"
                     % (nsteps, mode, 1 if probe else 0, pr[0], pr[1], pr[2], isgs, 1 if write_restart else 0,
-                       1 if maxcalc else 0))
+                       1 if maxcalc else 0, float(P.fcshear), "mw.bin" if meanwave is not None else " "))
+        if meanwave is not None:
+            np.ascontiguousarray(meanwave, dtype=np.float64).tofile(os.path.join(d, "mw.bin"))
         np.concatenate([np.ascontiguousarray(a, dtype=np.float64).ravel() for a in fields]).tofile(
             os.path.join(d, "ic.bin"))
         with open(os.path.join(d, "stdout.txt"), "w") as so:

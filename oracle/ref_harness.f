@@ -6,6 +6,8 @@ c  of the product.
 c
 c  It replays the step sequence of main.f:653-1066 (jmsegret/IWrepo,
 c  mode1_bbl) without main.f's file output:
+c    * fcshr sets the shear-current factor fcshear (main.f:120 has 0)
+c      and mwfile names a stream file with the nine /sol2dvar/ arrays;
 c    * with imaxc=0, maxcalc (various.f:665) is replaced by its
 c      state-changing part, the horfft(+1)/horfft(-1) round trip of
 c      u,v,w,temp (various.f:739-742, 854-857), so dt stays fixed; with
@@ -69,10 +71,13 @@ c=======================================================================
      >facdir,facneumn,facrobin,navgintrfc,ncalclegspec,
      >ifilter,nstepfilter,
      >tstart,npostp,itempreset,nrestartdump,n3doutput,i0
+      character*256 mwfile
       namelist/harness/ icfile, outfile, diagfile, nrun, mode, iprobe,
-     >ipx, kp1, kp2, isgs, iwrite, imaxc
+     >ipx, kp1, kp2, isgs, iwrite, imaxc, fcshr, mwfile
 
       imaxc = 0
+      fcshr = 0.
+      mwfile = ' '
       mode = 0
       isgs = 0
       iwrite = 0
@@ -111,7 +116,8 @@ c-Derived parameters (main.f:328-352, 416-472)
       xomega = (bruntz(0.0,zlen,brunt)*xkcrit)
      >           /sqrt(xkcrit**2+xmcrit**2)
       uo = 0.07
-      fcshear = 0.
+c-main.f:120 sets fcshear = 0; the harness may set the shear current
+      fcshear = fcshr
       omz = 0.
       pr = xnu
       dt = dtin
@@ -124,6 +130,16 @@ c-Derived parameters (main.f:328-352, 416-472)
       ispecfilter = 0
 
       call setup
+
+c-/sol2dvar/ mean-wave arrays (comsoliton.h:9-13), zero in the reference
+c-(wavesetup is not called, setup.f:191) unless a restart file holds them
+      if ((mwfile.ne.' ').and.(istart.eq.0)) then
+        open(10,file=mwfile,form='unformatted',access='stream',
+     >       status='old')
+        read(10) umwv,wmwv,dumdx,dumdz,dwmdx,dwmdz,rhomwv,drhomdx,
+     >           drhomdz
+        close(10)
+      endif
 
       if (istart.ne.0) then
 c-main.f:563-634: restart file, set-up again, first-step coefficients

@@ -80,3 +80,18 @@ def routine_inputs(O, P, flds):
         f[:, :, P.nx:] = 0.0
         spec.append(o.horfft(f, -1))
     return o, spec
+
+
+def meanwave_fields(P, z):
+    """Nine smooth (nz, nxpp) arrays for /sol2dvar/ (COMMON order: umwv,
+    wmwv, dumdx, dumdz, dwmdx, dwmdz, rhomwv, drhomdx, drhomdz), periodic in
+    x, O(1e-2)."""
+    nxpp = P.nx + 2
+    x = np.arange(nxpp) * (P.xlen / P.nx)
+    kx = 2 * np.pi / P.xlen
+    spec = [(0.02, np.cos, np.cos), (0.01, np.sin, np.sin), (0.02, np.sin, np.cos), (0.03, np.cos, np.sin),
+            (0.01, np.cos, np.sin), (0.01, np.sin, np.cos), (0.05, np.sin, np.sin), (0.02, np.cos, np.cos),
+            (0.04, np.sin, np.cos)]
+    z = np.asarray(z)
+    return np.stack([a * fz(np.pi * z)[:, None] * fx(kx * x)[None, :] for a, fx, fz in spec])
+

@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 from numpy.polynomial import legendre as L
 
-from helpers import SOLUTION, STATE, case, g1_params, g2_params, g3_params, rel, routine_inputs
+from helpers import SOLUTION, STATE, case, g1_params, g2_params, g3_params, meanwave_fields, rel, routine_inputs
 from iwrepo_amd.params import make_params
 
 # ---------------------------------------------------------------- GLL tables
@@ -433,3 +433,41 @@ def test_g1_adaptive_dt_events_match_shipped_log(oracle_mod):
     assert np.abs(mm[:, 1:3] - rec[:, 14:16]).max() < 1e-4         # E12.6 of values near -19.39 and 1
     assert np.array_equal(mm[:, 3:5], rec[:, 16:18])
     assert np.array_equal(np.array(d["tmin_loc"]), rec[:, [18, 19, 16]])
+
+
+# ------------------------------------- mean-wave and shear-current terms (row f3)
+
+
+@pytest.mark.parametrize("name", ["small", "c1"])
+def test_meanwave_and_shear_terms_match_compiled_reference(oracle_mod, name):
+    """solver.f:848-931 with the shear current on (fcshear = 0.5, main.f:120
+    has 0) and nonzero /sol2dvar/ mean-wave fields, which also enter the
+    interface upwind penalty (solver.f:723-724) and maxcalc
+    (various.f:757-808): 10 steps against the compiled reference, whose
+    fields the terms change by 1e-4 to 5e-3."""
+    if not oracle_mod.ref_available(name):
+        pytest.skip("reference build %s not present (needs /root/reference)" % name)
+    P, flds = case(name)
+    P.fcshear = 0.5
+    o = oracle_mod.Oracle(P)
+    mw = meanwave_fields(P, o.table(7, P.nzloc * P.nsubd))
+    o.set_meanwave(mw)
+    o.set_initial(*flds)
+    o.run(10)
+    mc = oracle_mod.ref_has_maxcalc(name)
+    res = oracle_mod.run_ref(name, P, flds, nsteps=10, meanwave=mw, maxcalc=mc)
+    st = res[0]
+    mine = o.state()
+    for q in SOLUTION:
+        assert rel(mine[q], st[q]) < (1e-11 if q < 12 else 1e-9), STATE[q]
+    base = oracle_mod.run_ref(name, P, flds, nsteps=10)[0] if name == "small" else None
+    if base is not None:  # the terms matter at this size
+        assert min(rel(base[q], st[q]) for q in range(4)) > 1e-5
+    if mc:
+        rec, m = o.maxcalc_records(), res[-1]
+        assert rel(rec[:, 30], np.array(m["cflz"])[:, 1]) < 1e-13
+        f51 = np.array(m["fort51"])
+        assert (np.abs(f51[:, [1, 5, 9]] - rec[:, [22, 26, 30]]) <= 5.1e-6 * rec[:, [22, 26, 30]]).all()
+        assert np.array_equal(f51[:, [2, 3, 4, 6, 7, 8, 10, 11, 12]], rec[:, [23, 24, 25, 27, 28, 29, 31, 32, 33]])
+        f50 = np.array(m["fort50"])
+        assert np.abs(f50[:, 1:] - rec[:, 2:18]).max() < 5e-6
