@@ -117,6 +117,14 @@ int iw_maxcalc_records(iw_solver *s, double *out, int n);
 int iw_get_dtstate(iw_solver *s, double *dt, double *cdt, int *ivardt, int *idtstep);
 int iw_set_dtstate(iw_solver *s, double cdt, int ivardt, int idtstep);
 
+/* /sol2dvar/ (comsoliton.h:9-13): the nine (nxpp,nz) mean-wave arrays in
+ * COMMON order, umwv, wmwv, dumdx, dumdz, dwmdx, dwmdz, rhomwv, drhomdx,
+ * drhomdz (9*nxpp*nz doubles).  They enter rhsfm (solver.f:848-931), the
+ * interface upwind penalty (solver.f:723-724) and maxcalc (various.f:757-808);
+ * zero unless set here or read from a restart file (restart.f:65-69). */
+int iw_set_meanwave(iw_solver *s, const double *mw);
+int iw_get_meanwave(iw_solver *s, double *mw);
+
 /* ---- reference routines on host arrays (parity / Fortran-shim mode) ---- */
 /* horfft.f:2    subroutine horfft(b,is,nk): is=-1 physical->spectral */
 int iw_horfft(iw_solver *s, double *b, int is, int nk);

@@ -868,6 +868,21 @@ extern "C" int iw_create(const iw_params *p, iw_solver **out)
         HIPCHK(hipMalloc((void **)&s->mc_colmax, sizeof(double) * (size_t)(kMcSlices + 1) * nxy));
         HIPCHK(hipMalloc((void **)&s->mc_gmax, sizeof(double) * (size_t)((nxy + 255) / 256)));
         HIPCHK(hipHostMalloc((void **)&s->h_mc, sizeof(double) * kMcRec));
+        // mean-wave arrays (zero) and the shear-current profiles
+        // (forcing.f:190-214; their zdim, zsh, xacu arguments are unused)
+        const size_t nmw = 9 * (size_t)g.nxpp * g.nz;
+        HIPCHK(hipMalloc((void **)&s->mw, sizeof(double) * nmw));
+        HIPCHK(hipMemset(s->mw, 0, sizeof(double) * nmw));
+        const double uofc = P.uo * P.fcshear;
+        const double sdt = 0.13, sz0 = 0.2625, szdom = 0.45, gam = 4.0 / (sdt * szdom);
+        std::vector<double> ush(g.nz), dush(g.nz);
+        for (int k = 0; k < g.nz; k++) {
+            const double zz = s->z[k];
+            const double c = 1.0 / cosh(gam * (zz - sz0));
+            ush[k] = uofc * (0.5 * (tanh(gam * (zz - sz0)) + 1));
+            dush[k] = uofc * (0.5 * gam * (c * c));
+        }
+        if (upload(&s->ush, ush, acc) || upload(&s->dush, dush, acc)) return -1;
     }
     acc += (long long)fb * (24 + 9 + 7 + 2) + 4 * sizeof(double) * g.nxppy;
     s->sys.kl = P.nzloc;
@@ -917,6 +932,11 @@ extern "C" int iw_destroy(iw_solver *s)
     hipFree(s->mc_rec);
     hipFree(s->mc_colmax);
     hipFree(s->mc_w00);
+    hipFree(s->mw);
+    hipFree(s->ush);
+    hipFree(s->dush);
+    hipFree(s->gT[0]);
+    hipFree(s->gT[1]);
     hipFree(s->mc_gmax);
     hipFree(s->mc_ring);
     hipHostFree(s->h_mc);
@@ -1139,11 +1159,21 @@ static void project_dev(iw_solver *s, double *const *f, int nf)
     launch_project(s->g, b, s->stream);
 }
 
+// are the mean-wave / shear-current terms of solver.f:848-931 active?
+static bool mean_active(const iw_solver *s) { return s->mw_on || s->P.uo * s->P.fcshear != 0.0; }
+
 // maxcalc reductions in the epilogue of an inverse x transform of u,v,w,
 // temp: cny = ny-1 and the removed mean of w when the transform is convec's
 static McFuse mc_fuse(iw_solver *s, double cny, const double *w00)
 {
     McFuse f;
+    if (s->mw_on) {
+        const size_t nm = (size_t)s->g.nxpp * s->g.nz;
+        f.umwv = s->mw;
+        f.wmwv = s->mw + nm;
+        f.rhomwv = s->mw + 6 * nm;
+    }
+    if (s->P.uo * s->P.fcshear != 0.0) f.ush = s->ush;
     f.part = s->mc_part;
     f.rm = s->mc_rm;
     f.rdz = s->mc_rdz;
@@ -1167,6 +1197,7 @@ static void mc_record(iw_solver *s, int istep, double cny)
     A.nx = g.nx; A.ny = g.ny; A.nz = g.nz; A.nxpp = g.nxpp; A.nxppy = g.nxppy;
     A.part = s->mc_part;
     A.nblk = mc_blocks(g);
+    A.umwv = s->mw_on ? s->mw : nullptr;
     {
         Scope sc(s, "maxcalc");
         launch_maxcalc_final(g, A, istep, s->t, s->P.umax, s->mc_rec, s->stream);
@@ -1201,6 +1232,17 @@ static int convec_dev(iw_solver *s, int ispecfilternl, int mcstep = 0, int mcslo
         double *o3[3] = {gr[2], gr[5], gr[8]};
         launch_dpdz3(g, uvwt, o3, s->stream);
     }
+    // the mean-wave / shear-current terms also need dtdx, dtdz
+    // (solver.f:74-103): dtdz spectral here, both transformed below
+    const bool mean = mean_active(s);
+    if (mean) {
+        if (!s->gT[0]) {
+            for (int q = 0; q < 2; q++) HIPCHK(hipMalloc((void **)&s->gT[q], sizeof(double) * (size_t)g.ntot));
+            s->dev_bytes += 2 * (long long)sizeof(double) * g.ntot;
+        }
+        Scope sc(s, "dpdz", 16. * nt);
+        launch_dpdz(g, B[3], s->gT[1], s->stream);
+    }
     if (g.ndims == 3) {
         // spectral-to-physical (solver.f:107-123) with the x/y gradients of
         // u,v,w (solver.f:86-103) formed inside the transforms: the y pass
@@ -1208,6 +1250,10 @@ static int convec_dev(iw_solver *s, int ispecfilternl, int mcstep = 0, int mcslo
         // fields, and emits i ky f of u,v,w; the x pass turns all ten and
         // emits i kx f of u,v,w
         FieldBatch by = batch({B[0], B[1], B[2], B[3], gr[2], gr[5], gr[8]});
+        if (mean) {
+            by.f[by.n] = by.g[by.n] = s->gT[1];
+            by.n++;
+        }
         for (int q = 0; q < 4; q++) by.g[q] = s->phys[q];
         by.dy[0] = gr[1];
         by.dy[1] = gr[4];
@@ -1222,6 +1268,11 @@ static int convec_dev(iw_solver *s, int ispecfilternl, int mcstep = 0, int mcslo
         bx.dx[0] = gr[0];
         bx.dx[1] = gr[3];
         bx.dx[2] = gr[6];
+        if (mean) {
+            bx.dx[3] = s->gT[0];
+            bx.f[bx.n] = bx.g[bx.n] = s->gT[1];
+            bx.n++;
+        }
         bx.kw = g.xw;
         {
             Scope sc(s, "fft_x_inv", 16. * g.nxppy * g.nz * 13);
@@ -1234,6 +1285,13 @@ static int convec_dev(iw_solver *s, int ispecfilternl, int mcstep = 0, int mcslo
         }
         FieldBatch b = batch({B[0], B[1], B[2], B[3], gr[0], gr[1], gr[2], gr[3], gr[4], gr[5], gr[6], gr[7],
                               gr[8]});
+        if (mean) {
+            launch_dx(g, B[3], s->gT[0], s->stream);
+            b.f[b.n] = b.g[b.n] = s->gT[0];
+            b.n++;
+            b.f[b.n] = b.g[b.n] = s->gT[1];
+            b.n++;
+        }
         for (int q = 0; q < 4; q++) b.g[q] = s->phys[q];
         fft_dev(s, b, 1, g.nz, kFftMain, mcstep > 0 ? &mf : nullptr);
     }
@@ -1249,7 +1307,15 @@ static int convec_dev(iw_solver *s, int ispecfilternl, int mcstep = 0, int mcslo
     }
     {
         Scope sc(s, "rhs_phys", 224. * nt);
-        launch_rhs_phys(g, s->P, s->t, ph, gr, S, s->ox, s->prod, s->stream);
+        MeanTerms mt;
+        if (mean) {
+            mt.mw = s->mw;
+            mt.ush = s->ush;
+            mt.dush = s->dush;
+            mt.dtdx = s->gT[0];
+            mt.dtdz = s->gT[1];
+        }
+        launch_rhs_phys(g, s->P, s->t, ph, gr, S, s->ox, s->prod, s->stream, &mt);
     }
     double **pr = s->prod;
     if (g.ndims == 3) {
@@ -1478,6 +1544,26 @@ static int mc_after(iw_solver *s, int slot)
     return 0;
 }
 
+extern "C" int iw_set_meanwave(iw_solver *s, const double *mw)
+{
+    if (!s || !mw) return fail("iw_set_meanwave: null argument");
+    const size_t n = 9 * (size_t)s->g.nxpp * s->g.nz;
+    HIPCHK(hipMemcpyAsync(s->mw, mw, sizeof(double) * n, hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    s->mw_on = false;
+    for (size_t i = 0; i < n && !s->mw_on; i++) s->mw_on = mw[i] != 0.0;
+    return 0;
+}
+
+extern "C" int iw_get_meanwave(iw_solver *s, double *mw)
+{
+    if (!s || !mw) return fail("iw_get_meanwave: null argument");
+    const size_t n = 9 * (size_t)s->g.nxpp * s->g.nz;
+    HIPCHK(hipMemcpyAsync(mw, s->mw, sizeof(double) * n, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return 0;
+}
+
 extern "C" int iw_maxcalc(iw_solver *s, int istep, double *rec)
 {
     if (!s) return fail("null solver");
@@ -1582,8 +1668,12 @@ static int host_tables(const iw_params *p, iw_solver &h)
     return 0;
 }
 
+// the mean-wave records (output.f:55-59 / restart.f:65-69): pairs of
+// /sol2dvar/ arrays interleaved per (i,k), or one array
+static const int kMwRec[5][2] = {{0, 1}, {2, 3}, {4, 5}, {6, -1}, {7, 8}};
+
 static int write_restart(const char *path, const iw_params *p, iw_solver &h, double t, double umax,
-                         const double *const *fields)
+                         const double *const *fields, const double *mw = nullptr)
 {
     const int nx = p->nx, ny = p->ny, nl = p->nzloc, ns = p->nsubd, nz = nl * ns, nzm = nl - 1;
     const int nxpp = nx + 2, nxhp = nx / 2 + 1;
@@ -1650,10 +1740,23 @@ static int write_restart(const char *path, const iw_params *p, iw_solver &h, dou
     r.put(&umax, 8);
     const double solks = 0.;
     r.put(&solks, 8);
-    for (int q = 0; q < 2; q++) r.putd(std::vector<double>(2 * (size_t)nxpp * nz, 0.)); // mean-wave fields
-    r.putd(std::vector<double>(2 * (size_t)nxpp * nz, 0.));
-    r.putd(std::vector<double>((size_t)nxpp * nz, 0.));
-    r.putd(std::vector<double>(2 * (size_t)nxpp * nz, 0.));
+    { // mean-wave fields (zero when none were given)
+        const size_t nm = (size_t)nxpp * nz;
+        for (int rr = 0; rr < 5; rr++) {
+            const int a = kMwRec[rr][0], b = kMwRec[rr][1];
+            std::vector<double> v(b < 0 ? nm : 2 * nm, 0.);
+            if (mw)
+                for (size_t m = 0; m < nm; m++) {
+                    if (b < 0) {
+                        v[m] = mw[a * nm + m];
+                    } else {
+                        v[2 * m] = mw[a * nm + m];
+                        v[2 * m + 1] = mw[b * nm + m];
+                    }
+                }
+            r.putd(v);
+        }
+    }
     for (int q = 0; q < 20; q++) r.put(fields[q], sizeof(double) * (size_t)ntot);
     if (!r.ok) return fail(std::string("write error on restart file ") + path);
     return 0;
@@ -1662,7 +1765,8 @@ static int write_restart(const char *path, const iw_params *p, iw_solver &h, dou
 // state index of restart field q (the file holds u..tnm, then sun..stnm)
 static int restart_state_index(int q) { return q < 12 ? q : q + 4; }
 
-static int read_restart(const char *path, const iw_params *p, double *t, double *umax, double *const *fields)
+static int read_restart(const char *path, const iw_params *p, double *t, double *umax, double *const *fields,
+                        double *mw = nullptr)
 {
     const int nx = p->nx, ny = p->ny, nl = p->nzloc, nz = nl * p->nsubd;
     const long ntot = (long)(nx + 2) * ny * nz;
@@ -1682,6 +1786,21 @@ static int read_restart(const char *path, const iw_params *p, double *t, double 
                 return fail("restart file: dimensions differ from the parameter block");
         }
         if (q == 22 && n == 8) memcpy(umax, b.data(), 8);
+        if (q >= 24 && q < 29 && mw) { // mean-wave fields
+            const size_t nm = (size_t)(nx + 2) * nz;
+            const int a = kMwRec[q - 24][0], c = kMwRec[q - 24][1];
+            if ((size_t)n != sizeof(double) * (c < 0 ? nm : 2 * nm))
+                return fail("restart file: bad mean-wave record");
+            const double *v = reinterpret_cast<const double *>(b.data());
+            for (size_t m = 0; m < nm; m++) {
+                if (c < 0) {
+                    mw[a * nm + m] = v[m];
+                } else {
+                    mw[a * nm + m] = v[2 * m];
+                    mw[c * nm + m] = v[2 * m + 1];
+                }
+            }
+        }
     }
     for (int q = 0; q < 20; q++) {
         const long n = r.get(b);
@@ -1723,8 +1842,10 @@ extern "C" int iw_save_restart(iw_solver *s, const char *path)
                               hipMemcpyDeviceToHost, s->stream));
         f[q] = host.data() + (size_t)q * s->g.ntot;
     }
+    std::vector<double> mw(9 * (size_t)s->g.nxpp * s->g.nz);
+    HIPCHK(hipMemcpyAsync(mw.data(), s->mw, sizeof(double) * mw.size(), hipMemcpyDeviceToHost, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
-    return write_restart(path, &s->P, *s, s->t, s->P.umax, f);
+    return write_restart(path, &s->P, *s, s->t, s->P.umax, f, mw.data());
 }
 
 extern "C" int iw_load_restart(iw_solver *s, const char *path, double cdt)
@@ -1736,7 +1857,9 @@ extern "C" int iw_load_restart(iw_solver *s, const char *path, double cdt)
     double *f[20];
     for (int q = 0; q < 20; q++) f[q] = host.data() + (size_t)q * s->g.ntot;
     double t = 0., umax = s->P.umax;
-    if (read_restart(path, &s->P, &t, &umax, f)) return -1;
+    std::vector<double> mw(9 * (size_t)s->g.nxpp * s->g.nz, 0.);
+    if (read_restart(path, &s->P, &t, &umax, f, mw.data())) return -1;
+    if (iw_set_meanwave(s, mw.data())) return -1; // restart.f:65-69
     for (int q = 0; q < 24; q++) HIPCHK(hipMemsetAsync(s->buf[q], 0, fb, s->stream));
     for (int q = 0; q < 20; q++)
         HIPCHK(hipMemcpyAsync(s->buf[restart_state_index(q)], f[q], fb, hipMemcpyHostToDevice, s->stream));

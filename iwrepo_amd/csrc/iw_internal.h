@@ -93,6 +93,8 @@ struct McFuse {
     double dt = 0, cx = 0, cy = 0; // dt/dx, dt/dy: the CFL numbers are |u| dt/dx ... to
                                    // within an ulp of the reference's (u dt)/dx
     double cny = 0;              // ny-1 for the Nyquist correction, 0: none
+    // mean-wave u, w, rho (nxpp,nz) and the shear current (nz), or nullptr
+    const double *umwv = nullptr, *wmwv = nullptr, *rhomwv = nullptr, *ush = nullptr;
 };
 // what the record kernels read after the transform
 struct McArgs {
@@ -104,6 +106,7 @@ struct McArgs {
     long nxppy;
     const double *part;
     int nblk;
+    const double *umwv;          // mean-wave u added for the umaxt chain, or nullptr
 };
 
 struct Prof {
@@ -154,6 +157,13 @@ struct iw_solver {
     std::vector<double> mc_hist;
     double Tso = 0.;
     int ivardt = 0, idtstep = 0;
+    // /sol2dvar/ mean-wave arrays (comsoliton.h:9-13), 9 x (nxpp,nz) in
+    // COMMON order, and the shear-current profiles uo fcshear fshear(z),
+    // uo fcshear dfsheardz(z) (forcing.f:190-214); with either active the
+    // terms of solver.f:848-931 need the physical dtdx, dtdz (gT)
+    double *mw = nullptr, *ush = nullptr, *dush = nullptr;
+    bool mw_on = false;
+    double *gT[2] = {nullptr, nullptr};
     // profiling
     bool profiling = false;
     iw::Prof prof;
@@ -174,9 +184,17 @@ void launch_dpdz(const Geo &g, const double *a, double *b, hipStream_t st);
 void launch_dpdz_conv(const Geo &g, double *const *uvwt, double *const *out4, hipStream_t st);
 void launch_dpdz3(const Geo &g, double *const *in3, double *const *out3, hipStream_t st);
 void launch_grad(const Geo &g, double *const *uvw, double *const *gr, hipStream_t st);
+// mean-wave and shear-current terms of rhsfm (solver.f:848-931); mw
+// nullptr: none
+struct MeanTerms {
+    const double *mw = nullptr;          // 9 x (nxpp,nz), COMMON order of /sol2dvar/
+    const double *ush = nullptr, *dush = nullptr; // nz
+    const double *dtdx = nullptr, *dtdz = nullptr; // physical
+};
 void launch_rhs_phys(const Geo &g, const iw_params &P, double t, double *const *uvwt,
                      double *const *gr, double *const *S, const double *ox, double *const *pr,
-                     hipStream_t st);
+                     hipStream_t st, const MeanTerms *mt = nullptr);
+void launch_dx(const Geo &g, const double *a, double *b, hipStream_t st);
 void launch_rhs_spec(const Geo &g, double *const *S, double *const *pr, hipStream_t st);
 void launch_wnl(const Geo &g, const double *sw, const double *swn, const double *swnm,
                 const double *fac, double *wnl, hipStream_t st);

@@ -376,19 +376,28 @@ __global__ __launch_bounds__(256) void k_fft_x(FieldBatch fb, int nrows, int ny,
                     xo = xo + mf.w00[k];
                 }
                 const int L = ((2 * m) * ny + j) * nz + k, L1 = L + ny * nz;
-                double ve = xe + 0.0, vo = xo + 0.0; // + umwv, wmwv, rhomwv (zero)
+                // + umwv, wmwv, rhomwv (zero unless set), + rhomean for rho
+                const double *mwf = f == 0 ? mf.umwv : f == 2 ? mf.wmwv : f == 3 ? mf.rhomwv : nullptr;
+                double ae = 0.0, ao = 0.0;
+                if (mwf) {
+                    const long mi = (long)k * nxpp + 2 * m;
+                    ae = mwf[mi];
+                    ao = mwf[mi + 1];
+                }
+                double ve = xe + ae, vo = xo + ao;
                 if (f == 3) {
-                    ve = ve + mf.rm[k]; // + rhomean
+                    ve = ve + mf.rm[k];
                     vo = vo + mf.rm[k];
                 }
                 mc_take(true, ve, L, b0, l0);
                 mc_take(true, vo, L1, b0, l0);
                 mc_take(false, ve, L, b1, l1);
                 mc_take(false, vo, L1, b1, l1);
-                if (f < 3) { // uo*fcshear = 0 (main.f:120); no CFLZ at the top point
+                if (f < 3) { // CFL numbers; the shear current advects in x; no CFLZ at the top point
                     const double cf = f == 0 ? mf.cx : f == 1 ? mf.cy : (k == nz - 1 ? 0.0 : mf.dt * mf.rdz[k]);
-                    mc_take(false, fabs(xe * cf), L, b2, l2);
-                    mc_take(false, fabs(xo * cf), L1, b2, l2);
+                    const double sh = (f == 0 && mf.ush) ? mf.ush[k] : 0.0;
+                    mc_take(false, fabs((sh + xe) * cf), L, b2, l2);
+                    mc_take(false, fabs((sh + xo) * cf), L1, b2, l2);
                 }
             }
 #pragma unroll
@@ -780,6 +789,9 @@ struct RhsArgs {
     double omz, bcoef, rg, omega;
     double t_now, xacrit_umax, umax_zdim, xkcrit, xomega, xmcrit, phit, rfac, inv_xkcrit;
     int paddle;
+    MeanTerms mt;
+    double xkappa;
+    int mean;
 };
 
 __global__ __launch_bounds__(256) void k_rhs_phys(RhsArgs A, long ntot, long nxppy, int nxpp, int nx,
@@ -798,8 +810,12 @@ __global__ __launch_bounds__(256) void k_rhs_phys(RhsArgs A, long ntot, long nxp
     sw = sw - 0.5 * (u * A.g[6][n] + v * A.g[7][n] + w * A.g[8][n]);
     sw = sw + A.bcoef * tp;
     const int k1 = k % nl, ks = k / nl;
+    // total vertical velocity at the interfaces, perturbation + mean wave
+    // (solver.f:723-724)
+    const long mi = (long)k * nxpp + i, nm = (long)nxpp * nz;
+    const double wmw = A.mean ? A.mt.mw[nm + mi] : 0.0;
     if (k1 == 0) {
-        double wb = (k == 0) ? 0. : w;
+        double wb = (k == 0) ? 0. : w + wmw;
         if (wb > 0.) {
             double tau1 = 0.5 / A.omega * (2. / zh[ks]);
             long m = n - nxppy;
@@ -810,7 +826,7 @@ __global__ __launch_bounds__(256) void k_rhs_phys(RhsArgs A, long ntot, long nxp
         }
     }
     if (k1 == nl - 1) {
-        double wt = (k == nz - 1) ? 0. : w;
+        double wt = (k == nz - 1) ? 0. : w + wmw;
         if (wt < 0.) {
             double g0 = fabs(wt);
             double tau2 = 0.5 / A.omega * (2. / zh[ks]);
@@ -820,6 +836,19 @@ __global__ __launch_bounds__(256) void k_rhs_phys(RhsArgs A, long ntot, long nxp
             sw = sw - tau2 * (g0 * w - g0 * A.w[m]);
             st = st - tau2 * (g0 * tp - g0 * A.t[m]);
         }
+    }
+    if (A.mean) { // mean-wave and shear-current terms, solver.f:848-931
+        const double *mw = A.mt.mw;
+        const double advmean = mw[mi] + A.mt.ush[k];
+        su = su - advmean * A.g[0][n] - wmw * A.g[2][n];
+        sv = sv - advmean * A.g[3][n] - wmw * A.g[5][n];
+        sw = sw - advmean * A.g[6][n] - wmw * A.g[8][n];
+        st = st - advmean * A.mt.dtdx[n] - wmw * A.mt.dtdz[n];
+        const double dusheardz = A.mt.dush[k];
+        su = su - u * mw[2 * nm + mi] - w * mw[3 * nm + mi] - w * dusheardz;
+        sw = sw - u * mw[4 * nm + mi] - w * mw[5 * nm + mi];
+        st = st - u * mw[7 * nm + mi] - w * mw[8 * nm + mi];
+        st = st + A.xkappa * 0.0; // d2rhomeandz2 = 0 (forcing.f:167-183)
     }
     if (A.paddle && i < nx) {
         double zz = zc[k], xx = xg[i];
@@ -845,9 +874,12 @@ __global__ __launch_bounds__(256) void k_rhs_phys(RhsArgs A, long ntot, long nxp
 
 void launch_rhs_phys(const Geo &g, const iw_params &P, double t, double *const *uvwt,
                      double *const *gr, double *const *S, const double *ox, double *const *pr,
-                     hipStream_t st)
+                     hipStream_t st, const MeanTerms *mt)
 {
     RhsArgs A;
+    A.mean = mt && mt->mw;
+    if (A.mean) A.mt = *mt;
+    A.xkappa = P.xkappa;
     A.u = uvwt[0]; A.v = uvwt[1]; A.w = uvwt[2]; A.t = uvwt[3];
     for (int q = 0; q < 9; q++) A.g[q] = gr[q];
     for (int q = 0; q < 4; q++) A.S[q] = S[q];
@@ -910,6 +942,24 @@ void launch_rhs_spec(const Geo &g, double *const *S, double *const *pr, hipStrea
     hipLaunchKernelGGL(k_rhs_spec, dim3((unsigned)((ncplx + 255) / 256)), dim3(256), 0, st, S[0], S[1],
                        S[2], S[3], pr[0], pr[1], pr[2], pr[3], pr[4], pr[5], pr[6], ncplx, g.nxhp, g.ny,
                        g.xw, g.yw);
+}
+
+// b = i kx a (spectral x derivative, solver.f:86-103)
+__global__ void k_dx(const double *__restrict__ a, double *b, long ncplx, int nxhp,
+                     const double *__restrict__ xw)
+{
+    long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncplx) return;
+    const double kx = xw[(int)(c % nxhp)];
+    const double re = a[2 * c], im = a[2 * c + 1];
+    b[2 * c] = -kx * im;
+    b[2 * c + 1] = kx * re;
+}
+
+void launch_dx(const Geo &g, const double *a, double *b, hipStream_t st)
+{
+    long ncplx = g.ntot / 2;
+    hipLaunchKernelGGL(k_dx, dim3((unsigned)((ncplx + 255) / 256)), dim3(256), 0, st, a, b, ncplx, g.nxhp, g.xw);
 }
 
 // wnl at bottom/top (solver.f:270-284)

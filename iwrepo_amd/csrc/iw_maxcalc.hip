@@ -106,6 +106,13 @@ __device__ __forceinline__ double mc_u(const McArgs &A, int i, int j, int k)
     return x;
 }
 
+// u + umwv as the umaxt chain compares it (various.f:757-758)
+__device__ __forceinline__ double mc_uu(const McArgs &A, int i, int j, int k)
+{
+    const double x = mc_u(A, i, j, k);
+    return A.umwv ? x + A.umwv[(long)k * A.nxpp + i] : x + 0.0;
+}
+
 // partials -> the record (layout of iw_maxcalc in iwgpu.h); entries the
 // reference keeps from the previous call (itmin, jtmin, dzmax) stay when
 // there is no new record
@@ -190,7 +197,7 @@ __global__ __launch_bounds__(256) void k_mc_colmax(McArgs A, double *cp)
     const int j = e / A.nx, i = e - j * A.nx;
     const int kc = blockIdx.y, k0 = kc * A.nz / kMcSlices, k1 = (kc + 1) * A.nz / kMcSlices;
     double m = -DBL_MAX;
-    for (int k = k0; k < k1; k++) m = fmax(m, mc_u(A, i, j, k) + 0.0);
+    for (int k = k0; k < k1; k++) m = fmax(m, mc_uu(A, i, j, k));
     cp[(size_t)kc * nxy + i * A.ny + j] = m;
 }
 
@@ -241,7 +248,7 @@ __global__ __launch_bounds__(256) void k_mc_walk(McArgs A, const double *__restr
         const int i = cc / ny, j = cc - i * ny;
         for (int k = kb; k < nz; k += 256) {
             const int kt = k + (int)threadIdx.x;
-            const bool f = kt < nz && (mc_u(A, i, j, kt) + 0.0) >= thr;
+            const bool f = kt < nz && mc_uu(A, i, j, kt) >= thr;
             const int r = mc_block_first(f);
             if (r < 256) return k + r;
         }
@@ -277,7 +284,7 @@ __global__ __launch_bounds__(256) void k_mc_walk(McArgs A, const double *__restr
         const int k = scan_column(c, k0);
         if (k < nz) {
             const int i = c / ny, j = c - i * ny;
-            thr = (mc_u(A, i, j, k) + 0.0) + umax;
+            thr = mc_uu(A, i, j, k) + umax;
             kmax = k + 1;
             k0 = k + 1;
             if (k0 == nz) {

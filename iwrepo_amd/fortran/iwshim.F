@@ -141,6 +141,11 @@ c     layout of iw_params (include/iwgpu.h)
           real(c_double) :: t, bdfv
           real(c_double) :: fac(3), bdf(3)
         end function
+        integer(c_int) function iw_set_meanwave(h, mw) bind(c)
+          import
+          type(c_ptr), value :: h
+          real(c_double) :: mw(*)
+        end function
         integer(c_int) function iw_maxcalc(h, istep, rec) bind(c)
           import
           type(c_ptr), value :: h
@@ -297,6 +302,8 @@ c     Tso kept between calls
         iwp%zh(ks) = zh(ks)
       enddo
       call iwshim_check(iw_create(iwp, iwh), 'iw_create')
+c     the nine /sol2dvar/ arrays follow umwv in the COMMON block
+      call iwshim_check(iw_set_meanwave(iwh, umwv), 'meanwave')
       end
 
 c     /flow/ and /matrix/ (comflow.h, commatrix.h) to the device

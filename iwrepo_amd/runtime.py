@@ -22,6 +22,7 @@ EXPORTS = [
     "iw_save_restart", "iw_load_restart", "iw_write_restart_file", "iw_read_restart_file", "iw_abvar1", "iw_abvar2", "iw_abvar3", "iw_bdfvarv",
     "iw_bdfvar1", "iw_bdfvar2", "iw_bdfvar3", "iw_set_profiling", "iw_kernel_stats",
     "iw_device_bytes", "iw_maxcalc", "iw_maxcalc_records", "iw_get_dtstate", "iw_set_dtstate",
+    "iw_set_meanwave", "iw_get_meanwave",
 ]
 
 MAXCALC_NREC = 42  # IW_MAXCALC_NREC
@@ -73,6 +74,8 @@ def lib():
     L.iw_maxcalc_records.argtypes = [P, P, I]
     L.iw_get_dtstate.argtypes = [P, P, P, P, P]
     L.iw_set_dtstate.argtypes = [P, D, I, I]
+    L.iw_set_meanwave.argtypes = [P, P]
+    L.iw_get_meanwave.argtypes = [P, P]
     for n in ("abvar1", "abvar2", "abvar3", "bdfvarv", "bdfvar1", "bdfvar2", "bdfvar3"):
         f = getattr(L, "iw_" + n)
         f.restype = D
@@ -218,6 +221,18 @@ class Solver:
         r = np.zeros((n, MAXCALC_NREC))
         _check(self.L.iw_maxcalc_records(self.h, _ptr(r), n))
         return r
+
+    def set_meanwave(self, mw):
+        """/sol2dvar/ arrays, 9 x (nz, nxpp) in COMMON order."""
+        a = np.ascontiguousarray(mw, dtype=np.float64)
+        if a.size != 9 * self.shape[0] * self.shape[2]:
+            raise ValueError("mean-wave arrays: expected 9 x nz x nxpp values")
+        _check(self.L.iw_set_meanwave(self.h, _ptr(a)))
+
+    def meanwave(self):
+        out = np.zeros((9, self.shape[0], self.shape[2]))
+        _check(self.L.iw_get_meanwave(self.h, _ptr(out)))
+        return out
 
     def dtstate(self):
         dt, cdt = ctypes.c_double(), ctypes.c_double()

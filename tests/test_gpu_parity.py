@@ -6,7 +6,7 @@ import os
 import numpy as np
 import pytest
 
-from helpers import SOLUTION, STATE, case, g1_params, g3_params, rel, routine_inputs
+from helpers import SOLUTION, STATE, case, g1_params, g3_params, meanwave_fields, rel, routine_inputs
 
 pytestmark = pytest.mark.gpu
 
@@ -442,3 +442,81 @@ def test_fortran_maxcalc_shim_under_the_reference_step_loop(oracle_mod):
     for q in SOLUTION:
         assert errs[STATE[q]] < (FIELD_TOL if q < 12 else HIST_TOL), STATE[q]
     assert abs(ts - tr) < 1e-15
+
+
+# ------------------------------------- mean-wave and shear-current terms
+
+
+@pytest.mark.parametrize("name", ["small", "c1"])
+def test_meanwave_and_shear_terms_match_oracle(rt, oracle_mod, name):
+    """The terms of solver.f:848-931 with fcshear = 0.5 and nonzero /sol2dvar/
+    fields (also in the interface upwind penalty and maxcalc), 10 steps."""
+    P, flds = case(name)
+    P.fcshear = 0.5
+    print("mean-wave terms", name, flush=True)
+    o, g = oracle_mod.Oracle(P), rt.Solver(P)
+    mw = meanwave_fields(P, o.table(7, P.nzloc * P.nsubd))
+    o.set_meanwave(mw)
+    g.set_meanwave(mw)
+    assert np.array_equal(g.meanwave(), mw)
+    o.set_initial(*flds)
+    g.set_initial(*flds)
+    o.run(10)
+    g.run(10, counts=False)
+    _cmp_records(g.maxcalc_records(10), o.maxcalc_records())
+    sg, so = g.state(), o.state()
+    for q in SOLUTION:
+        assert rel(sg[q], so[q]) < (FIELD_TOL if q < 12 else HIST_TOL), STATE[q]
+    g.close()
+
+
+def test_restart_with_meanwave_fields(rt, oracle_mod, tmp_path):
+    """A reference restart file that holds mean-wave fields (output.f:55-59):
+    the device reads them with the state and continues as the reference
+    restarting from the same file; the file it writes carries them back."""
+    if not oracle_mod.ref_available("small"):
+        pytest.skip("reference build small not present")
+    P, flds = case("small")
+    P.fcshear = 0.5
+    o = oracle_mod.Oracle(P)
+    mw = meanwave_fields(P, o.table(7, P.nzloc * P.nsubd))
+    _, t, _, rs = oracle_mod.run_ref("small", P, flds, nsteps=3, meanwave=mw, write_restart=True)
+    ref, tr, _ = oracle_mod.run_ref("small", P, flds, nsteps=4, start=rs, cdt=1.0)
+    path = tmp_path / "start.dat"
+    path.write_bytes(rs)
+    g = rt.Solver(P)
+    g.load_restart(str(path), 1.0)
+    assert np.array_equal(g.meanwave(), mw)
+    g.run(4, counts=False)
+    sg = g.state()
+    for q in SOLUTION:
+        assert rel(sg[q], ref[q]) < (FIELD_TOL if q < 12 else HIST_TOL), STATE[q]
+    out = tmp_path / "dev.dat"
+    g.save_restart(str(out))
+    h = rt.Solver(P)
+    h.load_restart(str(out), 1.0)
+    assert np.array_equal(h.meanwave(), mw)
+    g.close()
+    h.close()
+
+
+def test_fortran_shims_with_meanwave_and_shear(oracle_mod):
+    """The shimmed reference step loop (maxcalc included) with the shear
+    current and mean-wave fields on, against the reference alone."""
+    name, n = "c1", 20
+    for exe in ("iwref", "iwshim"):
+        if not os.path.exists(os.path.join(oracle_mod.REFDIR, name, exe)):
+            pytest.skip("%s/%s not built (built where the reference sources are)" % (name, exe))
+    if not oracle_mod.ref_has_maxcalc(name):
+        pytest.skip("reference build without various.f")
+    P, flds = case(name)
+    P.fcshear = 0.5
+    z = oracle_mod.Oracle(P).table(7, P.nzloc * P.nsubd)
+    mw = meanwave_fields(P, z)
+    print("shim mean-wave", name, n, flush=True)
+    sr, tr, _, mr = oracle_mod.run_ref(name, P, flds, nsteps=n, maxcalc=True, meanwave=mw)
+    ss, ts, _, ms = oracle_mod.run_ref(name, P, flds, nsteps=n, exe="iwshim", maxcalc=True, meanwave=mw)
+    assert np.array_equal(np.array(mr["dtlog"]), np.array(ms["dtlog"]))
+    assert rel(np.array(ms["cflz"])[:, 1], np.array(mr["cflz"])[:, 1]) < 1e-12
+    for q in SOLUTION:
+        assert rel(ss[q], sr[q]) < (FIELD_TOL if q < 12 else HIST_TOL), STATE[q]
