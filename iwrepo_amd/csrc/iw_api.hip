@@ -399,6 +399,7 @@ static int build_systems(iw_solver *s, bool refactor_p, int &first, int &count)
     SysTables &S = s->sys;
     if ((size_t)nsys > S.cap_sys) {
         if (S.AB) hipFree(S.AB);
+        if (S.FB) hipFree(S.FB);
         if (S.ipiv) hipFree(S.ipiv);
         if (S.coef) hipFree(S.coef);
         if (S.sysinfo) hipFree(S.sysinfo);
@@ -407,6 +408,9 @@ static int build_systems(iw_solver *s, bool refactor_p, int &first, int &count)
         // band positions outside the matrix are never written by the
         // factorisation and must read as zero in the lane solvers
         HIPCHK(hipMemsetAsync(S.AB, 0, sizeof(double) * cap * S.ldab * s->g.nz, s->stream));
+        const size_t fbn = (size_t)(fb_npf(P.nzloc) + fb_npb(P.nzloc)) * s->g.nz; // double2 per system
+        HIPCHK(hipMalloc((void **)&S.FB, sizeof(double) * 2 * cap * fbn));
+        s->dev_bytes += (long long)(sizeof(double) * 2 * cap * fbn);
         HIPCHK(hipMalloc((void **)&S.ipiv, sizeof(int) * cap * s->g.nz));
         HIPCHK(hipMalloc((void **)&S.coef, sizeof(double) * cap * 12 * ns));
         HIPCHK(hipMalloc((void **)&S.sysinfo, sizeof(double) * cap * 4));
@@ -414,6 +418,17 @@ static int build_systems(iw_solver *s, bool refactor_p, int &first, int &count)
                                     + sizeof(double) * cap * 12 * ns + sizeof(double) * cap * 4);
         S.cap_sys = cap;
         refactor_p = true;
+    }
+    {   // packed pressure factors
+        const int npc = std::max(np, 1);
+        if ((size_t)npc > S.cap_p) {
+            if (S.PB) hipFree(S.PB);
+            const size_t pbn = (size_t)(pb_nf(P.nzloc) + pb_nb(P.nzloc)) * s->g.nz;
+            HIPCHK(hipMalloc((void **)&S.PB, sizeof(double) * npc * pbn));
+            s->dev_bytes += (long long)(sizeof(double) * npc * pbn);
+            S.cap_p = npc;
+            refactor_p = true;
+        }
     }
     std::vector<double> coef((size_t)nsys * 12 * ns), info((size_t)nsys * 4);
     for (int q = 0; q < nsys; q++) {
@@ -485,6 +500,7 @@ static int ensure_factors(iw_solver *s)
     int first, count;
     if (build_systems(s, !s->p_factored, first, count)) return -1;
     launch_factor(s->g, s->sys, first, count, s->stream);
+    launch_compact(s->g, s->sys, first, count, s->stream);
     HIPCHK(hipGetLastError());
     // columns of systems factored without row interchanges go to the
     // lanes-as-columns solver; if any u/v, w or density system pivoted, all
@@ -941,6 +957,8 @@ extern "C" int iw_destroy(iw_solver *s)
     hipFree(s->mc_ring);
     hipHostFree(s->h_mc);
     hipFree(s->sys.AB);
+    hipFree(s->sys.FB);
+    hipFree(s->sys.PB);
     hipFree(s->sys.ipiv);
     hipFree(s->sys.coef);
     hipFree(s->sys.sysinfo);

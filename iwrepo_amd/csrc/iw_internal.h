@@ -51,6 +51,14 @@ struct SysTables {
     int np = 0; // pressure systems occupy ids 0..np-1
     int ldab = 0, kl = 0, kv = 0;
     double *AB = nullptr;   // nsys * ldab * nz, LAPACK band columns
+    // the parts of each factor column the no-pivot solves read, packed
+    // per system: nz forward segments (rows kv..kv+kl), then nz backward
+    // segments (rows kv-kl-1..kv+1), so both sweeps stream contiguous memory
+    double *FB = nullptr;
+    // the same for the pressure systems (ids < np, with interchanges): nz
+    // L segments (rows kv..kv+kl), then nz U columns (rows 0..kv+1)
+    double *PB = nullptr;
+    size_t cap_p = 0;
     int *ipiv = nullptr;    // nsys * nz (0-based)
     double *coef = nullptr; // nsys * nsubd * 12 assembly coefficients
     double *sysinfo = nullptr; // nsys * 4: eps, tau1(bottom), tau2(top), bc flags
@@ -208,6 +216,13 @@ void launch_filter3d(const Geo &g, double *f, int ifourier, int ilegen, hipStrea
 void launch_iface(const Geo &g, FieldBatch b, int *count, hipStream_t st);
 void launch_project(const Geo &g, FieldBatch b, hipStream_t st);
 void launch_factor(const Geo &g, const SysTables &S, int first, int count, hipStream_t st);
+void launch_compact(const Geo &g, const SysTables &S, int first, int count, hipStream_t st);
+// double2 per column of the packed forward / backward segments
+constexpr int fb_npf(int kl) { return (kl + 1) / 2; }
+constexpr int fb_npb(int kl) { return (kl + 3) / 2; }
+// doubles per column of the packed pressure segments
+constexpr int pb_nf(int kl) { return kl + 1; }
+constexpr int pb_nb(int kl) { return 2 * kl + 2; }
 void launch_pressure(const Geo &g, const iw_params &P, const SysTables &S, double *const *uvwt,
                      const double *wnl, const double *wviscrot, double dt, int cpen, hipStream_t st);
 void launch_viscous(const Geo &g, const iw_params &P, const SysTables &S, double *const *uvwt, double dt,

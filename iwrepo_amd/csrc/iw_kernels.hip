@@ -1431,6 +1431,49 @@ void launch_factor(const Geo &g, const SysTables &S, int first, int count, hipSt
                        S.coef, g.D, g.D2, S.AB, S.ipiv, S.sysinfo);
 }
 
+// packed copies of the factor rows the no-pivot solves read (SysTables::FB):
+// one block per system, one thread per column
+__global__ __launch_bounds__(256) void k_compact(const double *__restrict__ AB, double *FB, double *PB, int first,
+                                                 int np, int n, int ldab, int kv, int kl, int npf, int npb)
+{
+    const long sys = first + (long)blockIdx.x;
+    const double *ab = AB + sys * n * ldab;
+    if (sys < np) { // pressure system: L rows kv..kv+kl, U rows 0..kv+1 (k_solve_p)
+        const int nf = pb_nf(kl), nb = pb_nb(kl);
+        double *pf = PB + sys * n * (nf + nb);
+        for (int e = threadIdx.x; e < n * nf; e += blockDim.x) {
+            const int c = e / nf, r = e - c * nf;
+            pf[e] = ab[(long)c * ldab + kv + r];
+        }
+        double *pb = pf + (long)n * nf;
+        for (int e = threadIdx.x; e < n * nb; e += blockDim.x) {
+            const int c = e / nb, r = e - c * nb;
+            pb[e] = ab[(long)c * ldab + r];
+        }
+        return;
+    }
+    double *fw = FB + sys * n * 2 * (npf + npb);
+    const int nf = 2 * npf, nb = 2 * npb;
+    // consecutive threads copy consecutive packed entries (and nearby rows)
+    for (int e = threadIdx.x; e < n * nf; e += blockDim.x) {
+        const int c = e / nf, r = e - c * nf;
+        fw[e] = (kv + r < ldab) ? ab[(long)c * ldab + kv + r] : 0.;
+    }
+    double *bw = fw + (long)n * nf;
+    for (int e = threadIdx.x; e < n * nb; e += blockDim.x) {
+        const int c = e / nb, r = e - c * nb;
+        bw[e] = ab[(long)c * ldab + kv - kl - 1 + r];
+    }
+}
+
+void launch_compact(const Geo &g, const SysTables &S, int first, int count, hipStream_t st)
+{
+    if (count <= 0) return;
+    const int kl = g.nzloc;
+    hipLaunchKernelGGL(k_compact, dim3(count), dim3(256), 0, st, S.AB, S.FB, S.PB, first, S.np, g.nz, S.ldab,
+                       S.kv, kl, fb_npf(kl), fb_npb(kl));
+}
+
 // ---------------------------------------------------------------------------
 // Implicit step (calcuvwt.f:87-741 with update, calcuvwt.f:826-901) in two
 // kernels over the cached factors:
@@ -1939,7 +1982,7 @@ struct GrpArgs {
     double *f[4];
     const int *tasks;
     const int *modesys;
-    const double *AB;
+    const double *FB;
     const double *sysinfo;
     double den[4];
     long nxppy;
@@ -1964,21 +2007,22 @@ __global__ __launch_bounds__(64) void k_solve_grp(GrpArgs A)
     const int sys = A.modesys[4 * m + (fld < 2 ? 1 : fld == 2 ? 2 : 3)];
     const double eps = A.sysinfo[4 * sys], den = A.den[fld];
     const int n = A.n, kv = A.kv;
-    const long st = A.nxppy, cst = A.ldab / 2; // double2 stride between factor columns
-    const double2 *__restrict__ ab2 = reinterpret_cast<const double2 *>(A.AB + (long)sys * A.ldab * n);
+    const long st = A.nxppy;
+    static_assert(NPF == fb_npf(KL) && NPB == fb_npb(KL), "packed factor layout");
+    const double2 *__restrict__ fb2 = reinterpret_cast<const double2 *>(A.FB) + (long)sys * n * (NPF + NPB);
     double *__restrict__ b = A.f[fld] + col;
     double w[KL + 1];
     double rx[DP][PPL], ry[DP][PPL]; // ring of factor pieces (re/im halves of 16-byte loads)
     double bring[DP];
     // ---- forward: unit lower triangle (bandludcmp.f:1020-1030)
     {
-        const double2 *f0 = ab2 + kv / 2;
+        const double2 *f0 = fb2; // forward segments, NPF double2 per column
         auto ldp = [&](int sidx, double(&dx)[PPL], double(&dy)[PPL]) __attribute__((always_inline)) {
             const long c = sidx < n ? sidx : n - 1;
 #pragma unroll
             for (int q = 0; q < PPL; q++) {
                 const int p = rk + q * G;
-                const double2 v = f0[cst * c + (p < NPF ? p : 0)];
+                const double2 v = f0[NPF * c + (p < NPF ? p : 0)];
                 dx[q] = v.x;
                 dy[q] = v.y;
             }
@@ -2030,13 +2074,13 @@ __global__ __launch_bounds__(64) void k_solve_grp(GrpArgs A)
     }
     // ---- backward: upper band with KL superdiagonals (STBSV, ludcmp.f:4698-5046)
     {
-        const double2 *g0 = ab2 + (kv - KL - 1) / 2;
+        const double2 *g0 = fb2 + (long)n * NPF; // backward segments, NPB double2 per column
         auto ldp = [&](int sidx, double(&dx)[PPL], double(&dy)[PPL]) __attribute__((always_inline)) {
             const long c = sidx < n ? n - 1 - sidx : 0;
 #pragma unroll
             for (int q = 0; q < PPL; q++) {
                 const int p = rk + q * G;
-                const double2 v = g0[cst * c + (p < NPB ? p : 0)];
+                const double2 v = g0[NPB * c + (p < NPB ? p : 0)];
                 dx[q] = v.x;
                 dy[q] = v.y;
             }
@@ -2096,7 +2140,7 @@ void launch_solve_grp(const Geo &g, const iw_params &P, const SysTables &S, doub
     GrpArgs A;
     for (int q = 0; q < 4; q++) A.f[q] = uvwt[q];
     A.modesys = S.modesys;
-    A.AB = S.AB;
+    A.FB = S.FB;
     A.sysinfo = S.sysinfo;
     A.den[0] = A.den[1] = A.den[2] = P.xnu * dt;
     A.den[3] = dt * P.xkappa;
@@ -2242,7 +2286,7 @@ struct SolPArgs {
     double *phi;
     const int *tasks;
     const int *modesys;
-    const double *AB;
+    const double *PB;
     const int *ipiv;
     long nxppy;
     int n, ldab, kv;
@@ -2256,14 +2300,17 @@ __global__ __launch_bounds__(64) void k_solve_p(SolPArgs A)
     const int m = col >> 1;
     if (m == 0) return;
     const int sys = A.modesys[4 * m];
-    const int n = A.n, kv = A.kv, ldab = A.ldab;
+    const int n = A.n;
     const long st = A.nxppy;
-    const double *__restrict__ ab = A.AB + (long)sys * ldab * n;
-    const int *__restrict__ ip = A.ipiv + (long)sys * n;
-    double *__restrict__ b = A.phi + col;
     constexpr int NF = KL + 1;     // rows kv..kv+KL of column j (L multipliers)
     constexpr int KV = 2 * KL;     // superdiagonals of U after interchanges
     constexpr int NB = KV + 2;     // rows 0..kv+1 of column j (U column)
+    static_assert(NF == pb_nf(KL) && NB == pb_nb(KL), "packed pressure factor layout");
+    // packed factor: n L segments of NF, then n U columns of NB
+    const double *__restrict__ pf = A.PB + (long)sys * n * (NF + NB);
+    const double *__restrict__ pu = pf + (long)n * NF;
+    const int *__restrict__ ip = A.ipiv + (long)sys * n;
+    double *__restrict__ b = A.phi + col;
     // ---- forward: L with row interchanges (bandludcmp.f:1020-1030)
     {
         double w[KL + 1];
@@ -2275,13 +2322,13 @@ __global__ __launch_bounds__(64) void k_solve_p(SolPArgs A)
         double f0[NF], f1[NF];
         double r0, r1;
         int p0, p1;
-        ld_seg<NF>(ab + kv, f0);
+        ld_seg<NF>(pf, f0);
         r0 = b[(long)(KL + 1 < n ? KL + 1 : n - 1) * st];
         p0 = ip[0];
         int j = 0;
         auto step = [&](double (&fc)[NF], double (&fn)[NF], double &rc, double &rn, int &pc, int &pn) {
             const int jp = j + 1 < n ? j + 1 : n - 1;
-            ld_seg<NF>(ab + (long)ldab * jp + kv, fn);
+            ld_seg<NF>(pf + (long)NF * jp, fn);
             pn = ip[jp];
             const int rp = j + KL + 2 < n ? j + KL + 2 : n - 1;
             rn = b[(long)rp * st];
@@ -2316,12 +2363,12 @@ __global__ __launch_bounds__(64) void k_solve_p(SolPArgs A)
         }
         double g0[NB], g1[NB];
         double r0, r1;
-        ld_seg<NB>(ab + (long)ldab * (n - 1), g0);
+        ld_seg<NB>(pu + (long)NB * (n - 1), g0);
         r0 = b[(long)(n - 2 - KV >= 0 ? n - 2 - KV : 0) * st];
         int j = n - 1;
         auto step = [&](double (&gc)[NB], double (&gn)[NB], double &rc, double &rn) {
             const int jp = j - 1 >= 0 ? j - 1 : 0;
-            ld_seg<NB>(ab + (long)ldab * jp, gn);
+            ld_seg<NB>(pu + (long)NB * jp, gn);
             const int rp = j - KV - 2 >= 0 ? j - KV - 2 : 0;
             rn = b[(long)rp * st];
             const double dc = gc[KV];
@@ -2404,7 +2451,7 @@ void launch_pressure_nat(const Geo &g, const iw_params &P, const SysTables &S, d
     A.nxhp = g.nxhp;
     A.nz = g.nz;
     SolPArgs B;
-    B.phi = phi; B.tasks = S.np_tasks; B.modesys = S.modesys; B.AB = S.AB; B.ipiv = S.ipiv;
+    B.phi = phi; B.tasks = S.np_tasks; B.modesys = S.modesys; B.PB = S.PB; B.ipiv = S.ipiv;
     B.nxppy = g.nxppy; B.n = g.nz; B.ldab = S.ldab; B.kv = S.kv;
     dim3 gc((unsigned)((g.nxppy + 255) / 256), g.nsubd);
     dim3 gs(S.np_ntasks / 64);
