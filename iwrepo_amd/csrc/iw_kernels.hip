@@ -23,13 +23,15 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 b)
 // radix R has exactly 8/R butterflies per thread of a 256-thread block.
 // Passes are radix 8 while at least three factors of two remain, then one
 // radix-4 or radix-2 pass.  Each pass reads its butterflies into registers,
-// synchronises, and writes the outputs to their autosort positions; LDS
-// indices carry one pad slot per 16 values against bank conflicts.
+// synchronises, and writes the outputs to their autosort positions.  Row
+// layouts store value i at slot lp(i) = i ^ ((i >> 3) & 15), a permutation
+// within each 16-value chunk under which the radix-8 stride-8 writes and
+// aligned runs of 8 or 16 values hit distinct 16-byte bank groups.
 // sign -1: sum g_j e^{-2 pi i jk/N}; +1: e^{+...}.  tw[t] = e^{-2 pi i t/N}.
 // ---------------------------------------------------------------------------
 constexpr int kFE = 2048;
 
-__host__ __device__ __forceinline__ int lp(int i) { return i + (i >> 4); }
+__host__ __device__ __forceinline__ int lp(int i) { return i ^ ((i >> 3) & 15); }
 
 __device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
@@ -73,7 +75,7 @@ __device__ __forceinline__ void dft(double2 *a, int sign)
     }
 }
 
-// COL = false: sequence r occupies lp(r*N + i) and consecutive threads take
+// COL = false: sequence r occupies slots lp(r*N + i) and consecutive threads take
 // consecutive butterflies of one sequence.  COL = true: element i of
 // sequence r sits at i*nrows + r and consecutive threads take the same
 // butterfly of consecutive sequences (nrows a power of two), so every
@@ -209,7 +211,7 @@ __global__ __launch_bounds__(256) void k_fft_x(FieldBatch fb, int nrows, int ny,
 {
     extern __shared__ double2 smx[];
     double2 *buf = smx;
-    double2 *tw = smx + lp(R * M) + 1;
+    double2 *tw = smx + R * M;
     double2 *nyq = tw + M; // R entries: k = M term of each row (forward)
     const double *base = fb.f[blockIdx.y];
     double *obase = fb.g[blockIdx.y];
@@ -560,7 +562,7 @@ void launch_fft_x(const Geo &g, FieldBatch b, int nk, int dir, hipStream_t st, c
     const int R = kFE / g.M;
     int nrows = g.ny * nk;
     dim3 grid((nrows + R - 1) / R, b.n);
-    size_t sh = sizeof(double2) * ((size_t)lp(R * g.M) + 1 + g.M + R);
+    size_t sh = sizeof(double2) * ((size_t)R * g.M + g.M + R);
     McFuse none;
     if (dir < 0)
         hipLaunchKernelGGL(k_fft_x<-1>, grid, dim3(256), sh, st, b, nrows, g.ny, g.nxpp, g.M, g.logM, R,
