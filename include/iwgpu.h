@@ -39,6 +39,13 @@ typedef struct iw_params {
     int dtadapt;                   /* adaptive dt of various.f:911-985: 0 off
                                       (fixed dt), 1 with Tso kept from the last
                                       change, 2 with Tso read as 0 at every call  */
+    int n3doutput;                 /* every n3doutput steps the 3-D output routine
+                                      plotf takes u,v,w,temp through a horfft
+                                      round trip (main.f:996-1039); 0: never      */
+    int ncalclegspec;              /* every ncalclegspec steps speclegencalc
+                                      takes the Legendre spectra of u,v,w,temp
+                                      at j = ny/2 (main.f:905-911), a round
+                                      trip of each; 0: never                      */
     double xlen, ylen, zlen;
     double xnu, xkappa, brunt, grav, rho0, omz;
     double dt;
@@ -125,6 +132,28 @@ int iw_set_dtstate(iw_solver *s, double cdt, int ivardt, int idtstep);
 int iw_set_meanwave(iw_solver *s, const double *mw);
 int iw_get_meanwave(iw_solver *s, double *mw);
 
+/* Diagnostics.  Replaces postp(initenergy,t,brunt,tstart) (postp.f:9-76):
+ * the TKE budget of energycalc_postp (postp.f:415-732) of the device state,
+ * out = the sixteen values of the energy.dat line (postp.f:53-58) in its
+ * order: tketot, u1tke, u2tke, u3tke, prody, prodz, prod, bflux, epsilon,
+ * epssgs, epstot, chi, chisgs, chitot, tkemean, tempvar (out may be NULL).
+ * The state changes as postp's horfft round trips change it.  iw_run calls
+ * it every npostp steps (main.f:935-936) and iw_load_restart once
+ * (main.f:635). */
+int iw_postp(iw_solver *s, double *out);
+/* Replaces speclegencalc(filein,t,fin) (filtering.f:1110-1221) for field idx
+ * (0 u, 1 v, 2 w, 3 temp): out[kloc*nsubd + ks] = the x-average of the
+ * |Legendre coefficient kloc| of subdomain ks at j = ny/2, the numbers of
+ * the speclegen<c><t>.dat file; the field takes the horfft round trip.
+ * iw_run calls it for u,v,w,temp every ncalclegspec steps (main.f:905-911). */
+int iw_speclegen(iw_solver *s, int idx, double *out);
+/* The records of the last iw_run (or of iw_load_restart): postp records of
+ * 18 doubles (istep, t, the sixteen values) and speclegencalc records of
+ * 2 + 4*nz doubles (istep, t, the u, v, w, temp spectra).  Copies n records
+ * and returns how many there are (n = 0: just the count), or -1. */
+int iw_energy_records(iw_solver *s, double *out, int n);
+int iw_speclegen_records(iw_solver *s, double *out, int n);
+
 /* ---- reference routines on host arrays (parity / Fortran-shim mode) ---- */
 /* horfft.f:2    subroutine horfft(b,is,nk): is=-1 physical->spectral */
 int iw_horfft(iw_solver *s, double *b, int is, int nk);
@@ -147,6 +176,9 @@ int iw_filtering(iw_solver *s, int isgscalc);
 /* /comsgs/ (comsgs.h) tensor idx in physical space: 0 t11, 1 t12, 2 t13,
  * 3 t22, 4 t23, 5 t33, 6 th1, 7 th2, 8 th3 */
 int iw_get_sgs(iw_solver *s, int idx, double *host);
+/* The same tensors from the host, for callers whose /comsgs/ was written
+ * elsewhere (postp reads them, postp.f:622-665). */
+int iw_set_sgs(iw_solver *s, int idx, const double *host);
 
 /* ---- restart files in the record layout of output.f:27-104 and
  * restart.f:26-115 (Fortran sequential unformatted, REAL*8, INTEGER*4) ---- */

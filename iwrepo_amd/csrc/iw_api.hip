@@ -732,6 +732,8 @@ extern "C" int iw_create(const iw_params *p, iw_solver **out)
     if (lx < 2 || P.nx > 1024) return fail("nx must be a power of two in [4, 1024]");
     if (ly < 0 || P.ny > 1024) return fail("ny must be 1 or a power of two <= 1024");
     if (P.dt <= 0) return fail("dt must be positive");
+    if (P.ncalclegspec > 0 && P.ny < 2)
+        return fail("ncalclegspec needs ny >= 2 (speclegencalc reads the row j = ny/2)");
     iw_solver *s = new iw_solver();
     s->P = P;
     Geo &g = s->g;
@@ -956,6 +958,8 @@ extern "C" int iw_destroy(iw_solver *s)
     hipFree(s->mc_gmax);
     hipFree(s->mc_ring);
     hipHostFree(s->h_mc);
+    double *pp[8] = {s->pp_um, s->pp_tm, s->pp_dy, s->pp_dz, s->pp_part, s->pp_rec, s->e_ring, s->l_ring};
+    for (double *q : pp) hipFree(q);
     hipFree(s->sys.AB);
     hipFree(s->sys.FB);
     hipFree(s->sys.PB);
@@ -1562,6 +1566,155 @@ static int mc_after(iw_solver *s, int slot)
     return 0;
 }
 
+// ---------------------------------------------------------------------------
+// diagnostics: postp (postp.f:9-76) and speclegencalc (filtering.f:1110-1221)
+// ---------------------------------------------------------------------------
+static int pp_alloc(iw_solver *s)
+{
+    if (s->pp_um) return 0;
+    const size_t r = (size_t)s->g.ny * s->g.nz;
+    const size_t n[6] = {r, 6 * r, r, r, (size_t)pp_blocks(s->g) * 12, 18 + 2 + 4 * (size_t)s->g.nz};
+    double **p[6] = {&s->pp_um, &s->pp_tm, &s->pp_dy, &s->pp_dz, &s->pp_part, &s->pp_rec};
+    for (int q = 0; q < 6; q++) {
+        HIPCHK(hipMalloc((void **)p[q], sizeof(double) * n[q]));
+        s->dev_bytes += (long long)(sizeof(double) * n[q]);
+    }
+    return 0;
+}
+
+// grow a device record ring to hold n doubles (contents not kept)
+static int ring_reserve(iw_solver *s, double **ring, int *cap, size_t n)
+{
+    if ((size_t)*cap >= n) return 0;
+    hipFree(*ring);
+    *ring = nullptr;
+    *cap = 0;
+    HIPCHK(hipMalloc((void **)ring, sizeof(double) * n));
+    *cap = (int)n;
+    return 0;
+}
+
+// postp on the device state, out (device): istep, t and the sixteen values
+// of the energy.dat line.  The state changes as the reference's does: u,v,w
+// take two horfft round trips (postp.f:23-25, 47-49 and energycalc_postp's
+// own, postp.f:466-495), temp three (umeancalc's, postp.f:189-196, besides).
+// Scratch: phys, grad, prod[0..3] (free between steps).
+static int postp_dev(iw_solver *s, int istep, double *out)
+{
+    if (pp_alloc(s)) return -1;
+    const Geo &g = s->g;
+    double **B = s->buf, **gr = s->grad, **pr = s->prod, **ph = s->phys;
+    const int nz = g.nz;
+    { // u in physical space (postp.f:23); its x-means, taken out of it
+        FieldBatch b = batch({B[0]});
+        b.g[0] = ph[0];
+        fft_dev(s, b, 1, nz);
+    }
+    double *const *sgs = s->sgs[0] ? s->sgs : nullptr;
+    {
+        Scope sc(s, "postp_means");
+        launch_pp_means(g, ph[0], sgs, s->pp_um, s->pp_tm, s->pp_dy, s->pp_dz, 8. * atan(1.0) / s->P.ylen,
+                        s->stream);
+    }
+    { // the perturbation u' back to spectral space (postp.f:466)
+        FieldBatch b = batch({ph[0]});
+        b.g[0] = pr[0];
+        fft_dev(s, b, -1, nz);
+    }
+    // v, w, temp as energycalc_postp transforms them: after one round trip,
+    // temp after two
+    project_dev(s, B + 1, 3);
+    project_dev(s, B + 3, 1);
+    double *spec[4] = {pr[0], B[1], B[2], B[3]};
+    double *gx[4] = {gr[0], gr[3], gr[6], pr[1]}, *gy[4] = {gr[1], gr[4], gr[7], pr[2]};
+    double *gz[4] = {gr[2], gr[5], gr[8], pr[3]};
+    {
+        Scope sc(s, "postp_grad");
+        for (int q = 0; q < 4; q++) { // postp.f:471-489
+            launch_grad_xy(g, spec[q], gx[q], gy[q], s->stream);
+            launch_dpdz(g, spec[q], gz[q], s->stream);
+        }
+    }
+    FieldBatch b = batch({spec[0], spec[1], spec[2], spec[3], gx[0], gy[0], gz[0], gx[1], gy[1], gz[1], gx[2],
+                          gy[2], gz[2], gx[3], gy[3], gz[3]});
+    for (int q = 0; q < 4; q++) b.g[q] = ph[q];
+    fft_dev(s, b, 1, nz); // postp.f:493-508
+    double *g12[12] = {gx[0], gy[0], gz[0], gx[1], gy[1], gz[1], gx[2], gy[2], gz[2], gx[3], gy[3], gz[3]};
+    {
+        Scope sc(s, "postp_sums");
+        launch_pp_sums(g, ph, g12, sgs, s->pp_um, s->pp_dy, s->pp_dz, s->pp_tm, s->pp_part, s->P.xnu,
+                       s->P.xkappa, (double)istep, s->t, out, s->stream);
+    }
+    // the state: u,v,w after two round trips, temp after three
+    project_dev(s, B, 4);
+    project_dev(s, B, 1);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+// speclegencalc of spectral field f; out (device) nzloc*nsubd, hdr (device,
+// may be null) istep and t.  f takes the round trip of filtering.f:1152, 1206.
+static int speclegen_dev(iw_solver *s, double *f, int istep, double *hdr, double *out)
+{
+    FieldBatch b = batch({f});
+    b.g[0] = s->phys[0];
+    fft_dev(s, b, 1, s->g.nz);
+    {
+        Scope sc(s, "speclegen");
+        launch_speclegen(s->g, s->phys[0], (double)istep, s->t, hdr, out, s->stream);
+    }
+    project_dev(s, &f, 1);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int iw_postp(iw_solver *s, double *out)
+{
+    if (!s) return fail("null solver");
+    if (pp_alloc(s) || postp_dev(s, s->istep, s->pp_rec)) return -1;
+    double r[18];
+    HIPCHK(hipMemcpyAsync(r, s->pp_rec, sizeof r, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    if (out) memcpy(out, r + 2, 16 * sizeof(double));
+    return 0;
+}
+
+extern "C" int iw_speclegen(iw_solver *s, int idx, double *out)
+{
+    if (!s) return fail("null solver");
+    if (idx < 0 || idx > 3) return fail("iw_speclegen: field index must be 0..3 (u, v, w, temp)");
+    if (s->g.ny < 2) return fail("iw_speclegen: needs ny >= 2 (speclegencalc reads the row j = ny/2)");
+    if (pp_alloc(s)) return -1;
+    const int n = s->g.nz;
+    if (speclegen_dev(s, s->buf[idx], s->istep, nullptr, s->pp_rec + 20)) return -1;
+    std::vector<double> h(n);
+    HIPCHK(hipMemcpyAsync(h.data(), s->pp_rec + 20, sizeof(double) * n, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    if (out) memcpy(out, h.data(), sizeof(double) * n);
+    return 0;
+}
+
+extern "C" int iw_energy_records(iw_solver *s, double *out, int n)
+{
+    if (!s) return fail("null solver");
+    const int have = (int)(s->e_hist.size() / 18);
+    if (n < 0 || n > have) return fail("iw_energy_records: more records asked for than there are");
+    if (n > 0 && !out) return fail("iw_energy_records: null argument");
+    memcpy(out, s->e_hist.data(), sizeof(double) * 18 * (size_t)n);
+    return have;
+}
+
+extern "C" int iw_speclegen_records(iw_solver *s, double *out, int n)
+{
+    if (!s) return fail("null solver");
+    const size_t m = 2 + 4 * (size_t)s->g.nz;
+    const int have = (int)(s->l_hist.size() / m);
+    if (n < 0 || n > have) return fail("iw_speclegen_records: more records asked for than there are");
+    if (n > 0 && !out) return fail("iw_speclegen_records: null argument");
+    memcpy(out, s->l_hist.data(), sizeof(double) * m * (size_t)n);
+    return have;
+}
+
 extern "C" int iw_set_meanwave(iw_solver *s, const double *mw)
 {
     if (!s || !mw) return fail("iw_set_meanwave: null argument");
@@ -1634,6 +1787,7 @@ extern "C" int iw_set_initial(iw_solver *s, const double *u, const double *v, co
     for (int q = 0; q < 4; q++)
         HIPCHK(hipMemcpyAsync(s->buf[q], src[q], fb, hipMemcpyHostToDevice, s->stream));
     batch_fft(s, {s->buf[0], s->buf[1], s->buf[2], s->buf[3]}, -1);
+    project_dev(s, s->buf, 4); // plotf of the initial field (main.f:537-544)
     HIPCHK(hipStreamSynchronize(s->stream));
     s->t = 0.;
     s->fac[0] = 1.; s->fac[1] = 0.; s->fac[2] = 0.; // main.f:543-552
@@ -1892,6 +2046,17 @@ extern "C" int iw_load_restart(iw_solver *s, const char *path, double cdt)
     s->bdfv = iw_bdfvarv(dt, d2, d2);
     s->istep = 0;
     s->istart = 1;
+    // a restarted reference run has no SGS tensors yet (/comsgs/ is zero)
+    for (int q = 0; q < 9; q++)
+        if (s->sgs[q]) HIPCHK(hipMemsetAsync(s->sgs[q], 0, fb, s->stream));
+    project_dev(s, s->buf, 4); // plotf of the restart field (main.f:599-608)
+    // postp (main.f:634-635): its record is the one energy record until the
+    // next iw_run
+    if (ring_reserve(s, &s->e_ring, &s->e_cap, 18) || postp_dev(s, 0, s->e_ring)) return -1;
+    s->e_hist.assign(18, 0.);
+    HIPCHK(hipMemcpyAsync(s->e_hist.data(), s->e_ring, 18 * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    s->l_hist.clear();
     return 0;
 }
 
@@ -1906,6 +2071,20 @@ static int rho_mean(iw_solver *s)
     return 0;
 }
 
+// the nine /comsgs/ tensors (zero) and the level means of rho, on first use
+static int sgs_alloc(iw_solver *s)
+{
+    if (s->sgs[0]) return 0;
+    const size_t fb = sizeof(double) * (size_t)s->g.ntot;
+    for (int q = 0; q < 9; q++) {
+        HIPCHK(hipMalloc((void **)&s->sgs[q], fb));
+        HIPCHK(hipMemset(s->sgs[q], 0, fb));
+    }
+    HIPCHK(hipMalloc((void **)&s->tmean, sizeof(double) * s->g.nz));
+    s->dev_bytes += 9 * (long long)fb + sizeof(double) * s->g.nz;
+    return 0;
+}
+
 static int filtering_dev(iw_solver *s, int isgscalc)
 {
     const Geo &g = s->g;
@@ -1914,15 +2093,7 @@ static int filtering_dev(iw_solver *s, int isgscalc)
     double **B = s->buf;
     double *uvwt[4] = {B[0], B[1], B[2], B[3]};
     double *u0 = s->grad[0];
-    if (isgscalc && !s->sgs[0]) {
-        const size_t fb = sizeof(double) * (size_t)g.ntot;
-        for (int q = 0; q < 9; q++) {
-            HIPCHK(hipMalloc((void **)&s->sgs[q], fb));
-            HIPCHK(hipMemset(s->sgs[q], 0, fb));
-        }
-        HIPCHK(hipMalloc((void **)&s->tmean, sizeof(double) * g.nz));
-        s->dev_bytes += 9 * (long long)fb + sizeof(double) * g.nz;
-    }
+    if (isgscalc && sgs_alloc(s)) return -1;
     Scope sc(s, "estimation_filter");
     batch_fft(s, {B[0], B[1], B[2], B[3]}, 1);
     if (isgscalc) {
@@ -1966,6 +2137,17 @@ extern "C" int iw_get_sgs(iw_solver *s, int idx, double *host)
     return 0;
 }
 
+extern "C" int iw_set_sgs(iw_solver *s, int idx, const double *host)
+{
+    if (!s || !host) return fail("iw_set_sgs: null argument");
+    if (idx < 0 || idx > 8) return fail("iw_set_sgs: index must lie in [0, 8]");
+    if (sgs_alloc(s)) return -1;
+    HIPCHK(hipMemcpyAsync(s->sgs[idx], host, sizeof(double) * (size_t)s->g.ntot, hipMemcpyHostToDevice,
+                          s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return 0;
+}
+
 extern "C" int iw_run(iw_solver *s, int nsteps, int *counts)
 {
     if (!s || nsteps < 0) return fail("iw_run: bad argument");
@@ -1979,6 +2161,18 @@ extern "C" int iw_run(iw_solver *s, int nsteps, int *counts)
         HIPCHK(hipMalloc((void **)&s->mc_ring, sizeof(double) * kMcRec * (size_t)nsteps));
         s->mc_ring_cap = nsteps;
     }
+    // output routines of the step loop: speclegencalc (main.f:905-911),
+    // postp (main.f:935-936), plotf (main.f:996-1039); their records go to
+    // device rings, read back after the loop
+    const int npp = s->P.npostp, nleg = s->P.ncalclegspec, n3d = s->P.n3doutput;
+    const int ne = npp > 0 ? (s->istep + nsteps) / npp - s->istep / npp : 0;
+    const int nlg = nleg > 0 ? (s->istep + nsteps) / nleg - s->istep / nleg : 0;
+    const size_t lrec = 2 + 4 * (size_t)s->g.nz;
+    if (ring_reserve(s, &s->e_ring, &s->e_cap, 18 * (size_t)ne) ||
+        ring_reserve(s, &s->l_ring, &s->l_cap, lrec * (size_t)nlg))
+        return -1;
+    if ((ne > 0 || nlg > 0) && pp_alloc(s)) return -1;
+    int en = 0, ln = 0;
     for (int n = 0; n < nsteps; n++) {
         int istep = ++s->istep;
         // maxcalc (main.f:657): its horfft round trip (various.f:739-742,
@@ -2016,6 +2210,12 @@ extern "C" int iw_run(iw_solver *s, int nsteps, int *counts)
         if (s->P.ifilter == 1 && s->P.nstepfilter > 0 && istep % s->P.nstepfilter == 0) // main.f:880-895
             if (filtering_dev(s, s->P.npostp > 0 && istep % s->P.npostp == 0)) return -1;
         s->t = s->t + s->P.dt;
+        if (nleg > 0 && istep % nleg == 0) {
+            double *r = s->l_ring + lrec * (size_t)ln++;
+            for (int q = 0; q < 4; q++)
+                if (speclegen_dev(s, s->buf[q], istep, q == 0 ? r : nullptr, r + 2 + (size_t)q * s->g.nz))
+                    return -1;
+        }
         bool avg = s->P.navgintrfc > 0 && istep % s->P.navgintrfc == 0;
         if (avg) {
             if (iface_dev(s, s->buf, 4)) return -1;
@@ -2030,8 +2230,20 @@ extern "C" int iw_run(iw_solver *s, int nsteps, int *counts)
                 for (int q = 0; q < 4; q++) counts[4 * n + q] = 0;
             }
         }
+        if (npp > 0 && istep % npp == 0)
+            if (postp_dev(s, istep, s->e_ring + 18 * (size_t)en++)) return -1;
+        if (n3d > 0 && istep % n3d == 0) project_dev(s, s->buf, 4);
     }
     HIPCHK(hipGetLastError());
+    s->e_hist.assign(18 * (size_t)en, 0.);
+    s->l_hist.assign(lrec * (size_t)ln, 0.);
+    if (en > 0)
+        HIPCHK(hipMemcpyAsync(s->e_hist.data(), s->e_ring, sizeof(double) * s->e_hist.size(), hipMemcpyDeviceToHost,
+                              s->stream));
+    if (ln > 0)
+        HIPCHK(hipMemcpyAsync(s->l_hist.data(), s->l_ring, sizeof(double) * s->l_hist.size(), hipMemcpyDeviceToHost,
+                              s->stream));
+    if (en > 0 || ln > 0) HIPCHK(hipStreamSynchronize(s->stream));
     if (mc && !adapt && nsteps > 0) { // records of a fixed-dt run
         s->mc_hist.resize((size_t)nsteps * kMcRec);
         HIPCHK(hipMemcpyAsync(s->mc_hist.data(), s->mc_ring, sizeof(double) * kMcRec * (size_t)nsteps,

@@ -172,6 +172,15 @@ struct iw_solver {
     double *mw = nullptr, *ush = nullptr, *dush = nullptr;
     bool mw_on = false;
     double *gT[2] = {nullptr, nullptr};
+    // postp (postp.f:9-76) and speclegencalc (filtering.f:1110-1221): the
+    // x-means of u and of t11..t33 at every (y,z) line, d umean/dy and /dz,
+    // budget block partials and one record; the records of the last iw_run
+    // (device rings, host copies: istep, t, 16 values / istep, t, 4 spectra)
+    double *pp_um = nullptr, *pp_tm = nullptr, *pp_dy = nullptr, *pp_dz = nullptr, *pp_part = nullptr;
+    double *pp_rec = nullptr;
+    double *e_ring = nullptr, *l_ring = nullptr;
+    int e_cap = 0, l_cap = 0;
+    std::vector<double> e_hist, l_hist;
     // profiling
     bool profiling = false;
     iw::Prof prof;
@@ -203,6 +212,18 @@ void launch_rhs_phys(const Geo &g, const iw_params &P, double t, double *const *
                      double *const *gr, double *const *S, const double *ox, double *const *pr,
                      hipStream_t st, const MeanTerms *mt = nullptr);
 void launch_dx(const Geo &g, const double *a, double *b, hipStream_t st);
+// diagnostics (iw_diag.hip): postp's means and d/dy, d/dz of umean, its
+// TKE-budget sums (out: istep, t, the 16 energy.dat values), speclegencalc at
+// j = ny/2 (out[kloc nsubd + ks]; hdr, if not null, gets istep and t)
+int pp_blocks(const Geo &g);
+void launch_pp_means(const Geo &g, double *u, double *const *sgs6, double *um, double *tm, double *dumdy,
+                     double *dumdz, double beta, hipStream_t st);
+void launch_pp_sums(const Geo &g, double *const *uvwt, double *const *grad12, double *const *sgs9,
+                    const double *um, const double *dumdy, const double *dumdz, const double *tm, double *part,
+                    double pr, double xkappa, double istep, double t, double *out, hipStream_t st);
+void launch_speclegen(const Geo &g, const double *phys, double istep, double t, double *hdr, double *out,
+                      hipStream_t st);
+void launch_grad_xy(const Geo &g, const double *a, double *dx, double *dy, hipStream_t st);
 void launch_rhs_spec(const Geo &g, double *const *S, double *const *pr, hipStream_t st);
 void launch_wnl(const Geo &g, const double *sw, const double *swn, const double *swnm,
                 const double *fac, double *wnl, hipStream_t st);

@@ -30,7 +30,7 @@ c     layout of iw_params (include/iwgpu.h)
         integer(c_int) :: ispecfilternl, ispecfilterexp
         integer(c_int) :: nspecfilterexp, navgintrfc
         integer(c_int) :: ifilter, nstepfilter, npostp
-        integer(c_int) :: imaxcalc, dtadapt
+        integer(c_int) :: imaxcalc, dtadapt, n3doutput, ncalclegspec
         real(c_double) :: xlen, ylen, zlen
         real(c_double) :: xnu, xkappa, brunt, grav, rho0, omz
         real(c_double) :: dt
@@ -133,6 +133,23 @@ c     layout of iw_params (include/iwgpu.h)
           type(c_ptr), value :: h
           integer(c_int), value :: idx
           real(c_double) :: a(*)
+        end function
+        integer(c_int) function iw_set_sgs(h, idx, a) bind(c)
+          import
+          type(c_ptr), value :: h
+          integer(c_int), value :: idx
+          real(c_double) :: a(*)
+        end function
+        integer(c_int) function iw_postp(h, e) bind(c)
+          import
+          type(c_ptr), value :: h
+          real(c_double) :: e(*)
+        end function
+        integer(c_int) function iw_speclegen(h, idx, sp) bind(c)
+          import
+          type(c_ptr), value :: h
+          integer(c_int), value :: idx
+          real(c_double) :: sp(*)
         end function
         integer(c_int) function iw_get_coeffs(h, t, fac, bdf, bdfv)
      &       bind(c)
@@ -270,6 +287,9 @@ c     maxcalc is its own shim; adaptive dt as in various.f:911-985 with
 c     Tso kept between calls
       iwp%imaxcalc = 1
       iwp%dtadapt = 1
+c     the step loop (and its plotf calls) is main.f's own here
+      iwp%n3doutput = 0
+      iwp%ncalclegspec = 0
       iwp%xlen = xlen
       iwp%ylen = ylen
       iwp%zlen = zlen
@@ -572,6 +592,135 @@ c-----------------------------------------------------------------------
         call iwshim_check(iw_get_sgs(iwh, 7, th2), 'filtering')
         call iwshim_check(iw_get_sgs(iwh, 8, th3), 'filtering')
       endif
+      end
+
+c-----------------------------------------------------------------------
+c     postp.f:9   subroutine postp(initenergy,t,brunt,tstart)
+c     the TKE budget of energycalc_postp (postp.f:415-732) of /flow/
+c     u,v,w,temp with the /comsgs/ tensors, on the device: /energetics/
+c     and /energeticsw/ (comenergy.h, comenergyw.h), the state after
+c     postp's horfft round trips and the reference's output (units 60,
+c     61, 62 and the printed lines, postp.f:21, 53-73, 724-726).  The
+c     wake-interior values are 0, as in the reference, where the wake
+c     sizes of /lengthscales/ are never set (wakelengthcalc is not
+c     called, postp.f:39-41).  The x-means of /meanv/ (commeanv.h),
+c     read only by the relaxation procedure, are not returned.
+c-----------------------------------------------------------------------
+      subroutine postp(initenergy, t, brunt, tstart)
+      use iwshim
+      include 'dim.h'
+      include 'comflow.h'
+      include 'comsgs.h'
+      include 'comenergy.h'
+      include 'comenergyw.h'
+      include 'comlengthsc.h'
+      real(c_double) :: e(16)
+      call iwshim_init
+      write(*,*) 'POSTPROCESSING DATA'
+      call iwshim_check(iw_set_state(iwh, 0, u), 'postp')
+      call iwshim_check(iw_set_state(iwh, 1, v), 'postp')
+      call iwshim_check(iw_set_state(iwh, 2, w), 'postp')
+      call iwshim_check(iw_set_state(iwh, 3, temp), 'postp')
+      call iwshim_check(iw_set_sgs(iwh, 0, t11), 'postp')
+      call iwshim_check(iw_set_sgs(iwh, 1, t12), 'postp')
+      call iwshim_check(iw_set_sgs(iwh, 2, t13), 'postp')
+      call iwshim_check(iw_set_sgs(iwh, 3, t22), 'postp')
+      call iwshim_check(iw_set_sgs(iwh, 4, t23), 'postp')
+      call iwshim_check(iw_set_sgs(iwh, 5, t33), 'postp')
+      call iwshim_check(iw_set_sgs(iwh, 6, th1), 'postp')
+      call iwshim_check(iw_set_sgs(iwh, 7, th2), 'postp')
+      call iwshim_check(iw_set_sgs(iwh, 8, th3), 'postp')
+      call iwshim_check(iw_postp(iwh, e), 'postp')
+      call iwshim_check(iw_get_state(iwh, 0, u), 'postp')
+      call iwshim_check(iw_get_state(iwh, 1, v), 'postp')
+      call iwshim_check(iw_get_state(iwh, 2, w), 'postp')
+      call iwshim_check(iw_get_state(iwh, 3, temp), 'postp')
+      tketot = e(1)
+      u1tke = e(2)
+      u2tke = e(3)
+      u3tke = e(4)
+      prody = e(5)
+      prodz = e(6)
+      prod = e(7)
+      bflux = e(8)
+      epsilon = e(9)
+      epssgs = e(10)
+      epstot = e(11)
+      chi = e(12)
+      chisgs = e(13)
+      chitot = e(14)
+      tkemean = e(15)
+      tempvar = e(16)
+      tketotw = 0.
+      u1tkew = 0.
+      u2tkew = 0.
+      u3tkew = 0.
+      prodyw = 0.
+      prodzw = 0.
+      prodw = 0.
+      bfluxw = 0.
+      epsilonw = 0.
+      epssgsw = 0.
+      epstotw = 0.
+      tempvarw = 0.
+      chiw = 0.
+      chisgsw = 0.
+      chitotw = 0.
+      write(*,*) 'DISSIPATION RATES',epssgs,epsilon
+      write(*,*) 'Scalar DRs',chisgs,chi
+      write(60,'(1x,F10.4,17(2x,E12.6))')
+     &  (t-tstart),(t-tstart)*brunt,
+     &  tketot,u1tke,u2tke,u3tke,prody,prodz,prod,
+     &  bflux,epsilon,epssgs,epstot,
+     &  chi,chisgs,chitot,
+     &  tkemean,tempvar
+      write(61,'(1x,F10.4,17(2x,E12.6))')
+     &  (t-tstart),(t-tstart)*brunt,
+     &  tketotw,u1tkew,u2tkew,u3tkew,prodyw,prodzw,prodw,
+     &  bfluxw,epsilonw,epssgsw,epstotw,
+     &  chiw,chisgsw,chitotw,
+     &  tkemean,tempvarw
+      write(62,'(1x,F10.4,8(2x,E12.6))')
+     &  (t-tstart),(t-tstart)*brunt,
+     &  (udefy+udefz)/2.,udefy,udefz,wakely,wakelz,yc,zc
+      write(*,*) 'Dissipation rate estimate#2',epsilon,epssgs
+      end
+
+c-----------------------------------------------------------------------
+c     filtering.f:1110   subroutine speclegencalc(filein,t,fin)
+c     the x-averaged Legendre spectra of fin at j = ny/2 on the device,
+c     written to speclegen<filein><t>.dat in the reference's name and
+c     format (filtering.f:1131-1141, 1194-1216), and fin after its
+c     horfft round trip.  The device's u slot carries fin: every shim
+c     loads the state it reads, so nothing else depends on that slot.
+c-----------------------------------------------------------------------
+      subroutine speclegencalc(filein, t, fin)
+      use iwshim
+      include 'dim.h'
+      character*1 filein
+      character*30 fileout
+      character*10 fmt1
+      dimension fin(nxpp,ny,nz)
+      real(c_double) :: sp(nzloc*nsubd)
+      integer kloc, ks
+      call iwshim_init
+      call iwshim_check(iw_set_state(iwh, 0, fin), 'speclegencalc')
+      call iwshim_check(iw_speclegen(iwh, 0, sp), 'speclegencalc')
+      call iwshim_check(iw_get_state(iwh, 0, fin), 'speclegencalc')
+      fileout = 'speclegen'
+      write(unit=fileout(10:10),fmt='(a1)') filein
+      fmt1 = '(f5.3,a4)'
+      if (nint(t).ge.10) fmt1 = '(f6.3,a4)'
+      if (nint(t).ge.100) fmt1 = '(f7.3,a4)'
+      if (nint(t).ge.1000) fmt1 = '(f8.3,a4)'
+      write(unit=fileout(11:23),fmt=fmt1) t,'.dat'
+      write(*,*) fileout
+      open(405,file=fileout)
+      do kloc = 1, nzloc
+        write(405,150) kloc, (sp((kloc-1)*nsubd+ks), ks=1,nsubd)
+      enddo
+      close(405)
+ 150  format (1x,I2,2x,10(e12.6,2x))
       end
 
 c-----------------------------------------------------------------------

@@ -17,7 +17,8 @@ class IWParams(ctypes.Structure):
         ("ispecfilternl", ctypes.c_int), ("ispecfilterexp", ctypes.c_int),
         ("nspecfilterexp", ctypes.c_int), ("navgintrfc", ctypes.c_int),
         ("ifilter", ctypes.c_int), ("nstepfilter", ctypes.c_int), ("npostp", ctypes.c_int),
-        ("imaxcalc", ctypes.c_int), ("dtadapt", ctypes.c_int),
+        ("imaxcalc", ctypes.c_int), ("dtadapt", ctypes.c_int), ("n3doutput", ctypes.c_int),
+        ("ncalclegspec", ctypes.c_int),
         ("xlen", ctypes.c_double), ("ylen", ctypes.c_double), ("zlen", ctypes.c_double),
         ("xnu", ctypes.c_double), ("xkappa", ctypes.c_double), ("brunt", ctypes.c_double),
         ("grav", ctypes.c_double), ("rho0", ctypes.c_double), ("omz", ctypes.c_double),
@@ -54,7 +55,7 @@ def make_params(nx, ny, nsubd, order, xlen=2 * math.pi, ylen=math.pi / 2, zlen=1
                 ispecfilternl=1, ispecfilterexp=1, nspecfilterexp=1, zcen=0.13, xbcrit=0.84,
                 xacrit=0.0, umax=0.05, fwave=0.0, heights=None, grav=1.0, rho0=1.0,
                 ifilter=0, nstepfilter=5, npostp=100000000, fackcbv=0.0, imaxcalc=1, dtadapt=0,
-                fcshear=0.0, uo=0.07):
+                fcshear=0.0, uo=0.07, n3doutput=0, ncalclegspec=0):
     """Build a parameter block.  ``order`` is the GLL polynomial order p
     (nzloc = p+1); z in [0,zlen] is split into equal subdomains unless
     ``heights`` gives them (subd.dat, setup.f:297-330)."""
@@ -78,6 +79,7 @@ def make_params(nx, ny, nsubd, order, xlen=2 * math.pi, ylen=math.pi / 2, zlen=1
     P.ifilter, P.nstepfilter, P.npostp, P.fackcbv = ifilter, nstepfilter, npostp, fackcbv
     P.imaxcalc, P.dtadapt = imaxcalc, dtadapt
     P.fcshear, P.uo = fcshear, uo
+    P.n3doutput, P.ncalclegspec = n3doutput, ncalclegspec
     if heights is None:
         heights = [zlen / nsubd] * nsubd
     z = 0.0

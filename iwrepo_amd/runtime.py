@@ -22,7 +22,8 @@ EXPORTS = [
     "iw_save_restart", "iw_load_restart", "iw_write_restart_file", "iw_read_restart_file", "iw_abvar1", "iw_abvar2", "iw_abvar3", "iw_bdfvarv",
     "iw_bdfvar1", "iw_bdfvar2", "iw_bdfvar3", "iw_set_profiling", "iw_kernel_stats",
     "iw_device_bytes", "iw_maxcalc", "iw_maxcalc_records", "iw_get_dtstate", "iw_set_dtstate",
-    "iw_set_meanwave", "iw_get_meanwave",
+    "iw_set_meanwave", "iw_get_meanwave", "iw_postp", "iw_speclegen", "iw_set_sgs", "iw_energy_records",
+    "iw_speclegen_records",
 ]
 
 MAXCALC_NREC = 42  # IW_MAXCALC_NREC
@@ -76,6 +77,11 @@ def lib():
     L.iw_set_dtstate.argtypes = [P, D, I, I]
     L.iw_set_meanwave.argtypes = [P, P]
     L.iw_get_meanwave.argtypes = [P, P]
+    L.iw_postp.argtypes = [P, P]
+    L.iw_set_sgs.argtypes = [P, I, P]
+    L.iw_speclegen.argtypes = [P, I, P]
+    L.iw_energy_records.argtypes = [P, P, I]
+    L.iw_speclegen_records.argtypes = [P, P, I]
     for n in ("abvar1", "abvar2", "abvar3", "bdfvarv", "bdfvar1", "bdfvar2", "bdfvar3"):
         f = getattr(L, "iw_" + n)
         f.restype = D
@@ -210,6 +216,12 @@ class Solver:
         _check(self.L.iw_get_sgs(self.h, idx, _ptr(out)))
         return out
 
+    def set_sgs(self, idx, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        if a.size != int(np.prod(self.shape)):
+            raise ValueError("SGS tensor: expected nz x ny x nxpp values")
+        _check(self.L.iw_set_sgs(self.h, idx, _ptr(a)))
+
     def maxcalc(self, istep=1):
         """maxcalc (various.f:665-987) on the device state; returns its record."""
         r = np.zeros(MAXCALC_NREC)
@@ -220,6 +232,44 @@ class Solver:
         """(n, MAXCALC_NREC) records of the last run."""
         r = np.zeros((n, MAXCALC_NREC))
         _check(self.L.iw_maxcalc_records(self.h, _ptr(r), n))
+        return r
+
+    def postp(self):
+        """postp (postp.f:9-76) on the device state: the sixteen values of
+        the energy.dat line (tketot, u1tke, u2tke, u3tke, prody, prodz, prod,
+        bflux, epsilon, epssgs, epstot, chi, chisgs, chitot, tkemean,
+        tempvar); the state takes postp's horfft round trips."""
+        r = np.zeros(16)
+        _check(self.L.iw_postp(self.h, _ptr(r)))
+        return r
+
+    def speclegen(self, idx):
+        """speclegencalc (filtering.f:1110-1221) of field idx (0 u, 1 v, 2 w,
+        3 temp): (nzloc, nsubd) x-averaged |Legendre coefficients| at
+        j = ny/2; the field takes the horfft round trip."""
+        r = np.zeros(self.shape[0])
+        _check(self.L.iw_speclegen(self.h, idx, _ptr(r)))
+        return r.reshape(self.P.nzloc, self.P.nsubd)
+
+    def energy_records(self):
+        """(n, 18) postp records (istep, t, the sixteen values) of the last
+        iw_run (every npostp steps), or of iw_load_restart."""
+        n = self.L.iw_energy_records(self.h, None, 0)
+        _check(min(n, 0))
+        r = np.zeros((n, 18))
+        if n:
+            _check(min(self.L.iw_energy_records(self.h, _ptr(r), n), 0))
+        return r
+
+    def speclegen_records(self):
+        """(n, 2 + 4 nz) speclegencalc records (istep, t, the u, v, w, temp
+        spectra, each (nzloc, nsubd) row-major) of the last iw_run."""
+        m = 2 + 4 * self.shape[0]
+        n = self.L.iw_speclegen_records(self.h, None, 0)
+        _check(min(n, 0))
+        r = np.zeros((n, m))
+        if n:
+            _check(min(self.L.iw_speclegen_records(self.h, _ptr(r), n), 0))
         return r
 
     def set_meanwave(self, mw):

@@ -16,7 +16,9 @@ preprocessor renaming the three REAL loop counters of ``testdump``
 that routine is not called.  flang also rejects constant subscripts outside the
 arrays (``various.f:194-204`` for ny = 1, ``various.f:646, 745`` for nxpp < 32 or
 nz < 58), so builds where it does not compile leave it out and the harness
-supplies its ``zero``/``copy`` helpers instead.
+supplies its ``zero``/``copy`` helpers instead.  ``postp.f`` (the TKE budget
+of ``postp``, ``postp.f:9-76, 415-732``) is compiled the same way, and the
+harness calls ``postp`` where ``main.f:635, 935`` do when it is in the build.
 
 Outputs go only under ``oracle/_ref/`` (git-ignored).  Nothing here runs when
 ``/root/reference`` is absent.
@@ -94,6 +96,18 @@ def build(name, nx, ny, nzloc, nsubd, noslip=False, ref_dim=False, jobs=8, shim=
             os.remove(marker)
         print(name, ": various.f does not compile at these sizes; maxcalc left out")
     objs = [vo] if with_maxcalc else []
+    po = os.path.join(bdir, "postp.o")
+    r = subprocess.run([FLANG] + FFLAGS + ["-c", os.path.join(src, "postp.f"), "-o", po],
+                       stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+    with_postp = r.returncode == 0
+    pmark = os.path.join(bdir, "has_postp")
+    if with_postp:
+        open(pmark, "w").close()
+        objs.append(po)
+    else:
+        if os.path.exists(pmark):
+            os.remove(pmark)
+        print(name, ": postp.f does not compile at these sizes; postp left out")
     procs = []
     for s in srcs:
         o = os.path.join(bdir, s.replace(".f", ".o"))
@@ -104,34 +118,39 @@ def build(name, nx, ny, nzloc, nsubd, noslip=False, ref_dim=False, jobs=8, shim=
             _wait(procs)
     _wait(procs)
     ho = os.path.join(bdir, "ref_harness.o")
-    hdef = ["-DIW_MAXCALC"] if with_maxcalc else []
+    hdef = (["-DIW_MAXCALC"] if with_maxcalc else []) + (["-DIW_POSTP"] if with_postp else [])
     subprocess.run([FLANG] + FFLAGS + ["-cpp"] + hdef + ["-I", src, "-c", os.path.join(HERE, "ref_harness.f"),
                                                          "-o", ho], check=True)
     exe = os.path.join(bdir, "iwref")
-    link([FLANG] + FFLAGS + [ho] + objs + ["-o", exe], with_maxcalc)
+    link([FLANG] + FFLAGS + [ho] + objs + ["-o", exe])
     print("built", exe)
     if shim:
-        build_shim(bdir, src, ho, objs, noslip, with_maxcalc)
+        build_shim(bdir, src, ho, objs, noslip)
     return exe
 
 
-def link(cmd, with_maxcalc):
-    """Link; with various.f in the build, its timer routine timecall
-    (various.f:990-1010) references the ifort library routine time, which
-    flang's runtime lacks.  timecall is never called here, so that one
-    reference is left unresolved (reported as a warning); any other
-    unresolved symbol fails the build."""
-    if with_maxcalc:
-        cmd = cmd + ["-Wl,--warn-unresolved-symbols"]
+# Symbols that may stay unresolved: referenced only from routines the harness
+# never calls.  time: the ifort library routine behind timecall
+# (various.f:990-1010); curvefit: wakelengthcalc (postp.f:736-981, its call
+# at postp.f:39-41 commented out; curvefit.f is rejected by flang);
+# umeanprof: umeanreset (postp.f:364-412, relaxation only).
+UNRESOLVED_OK = ("time_", "curvefit_", "umeanprof_")
+
+
+def link(cmd):
+    """Link, leaving only the symbols of UNRESOLVED_OK unresolved (reported
+    as warnings); any other unresolved symbol fails the build."""
+    cmd = cmd + ["-Wl,--warn-unresolved-symbols"]
     r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
     out = r.stdout.decode()
-    bad = [l for l in out.splitlines() if "undefined symbol" in l and "time_" not in l]
+    bad = [l for l in out.splitlines()
+           if "undefined symbol" in l and not any(("symbol: " + u) in l for u in UNRESOLVED_OK)]
     if r.returncode != 0 or bad:
         sys.stderr.write(out)
         raise SystemExit("link failed")
 
 
-def build_shim(bdir, src, ho, objs, noslip, with_maxcalc=False):
+def build_shim(bdir, src, ho, objs, noslip):
     """The same harness with the product's Fortran shims (iwshim.F) linked in
     front of the reference objects: every call of horfft, dpdz, convec,
     calcuvwt, interface_avg3d, filter_3dlegenfourier and the multistep
@@ -151,7 +170,7 @@ def build_shim(bdir, src, ho, objs, noslip, with_maxcalc=False):
     rel = os.path.relpath(PKG, bdir)
     link([FLANG] + FFLAGS + [ho, so] + objs +
          ["-o", exe, "-Wl,--allow-multiple-definition", "-L" + PKG, "-liwgpu",
-          "-Wl,-rpath,$ORIGIN/" + rel, "-Wl,-rpath,/opt/rocm/lib"], with_maxcalc)
+          "-Wl,-rpath,$ORIGIN/" + rel, "-Wl,-rpath,/opt/rocm/lib"])
     print("built", exe)
     return exe
 

@@ -54,6 +54,10 @@ struct iwo_state {
     int ivardt, idtstep, itmin, jtmin;
     double *mcrec;
     int mcn, mccap;
+    double *erec; /* postp records of the last run */
+    int en;
+    double *lrec; /* speclegencalc records of the last run */
+    int ln;
 };
 
 #define U(s) ((s)->F[0])
@@ -369,6 +373,8 @@ void iwo_destroy(iwo_state *s)
     for (int f = 0; f < 8; f++) free(s->work[f]);
     free(s->wnl); free(s->wviscrot);
     free(s->mcrec);
+    free(s->erec);
+    free(s->lrec);
     for (int q = 0; q < 9; q++) free(s->mw[q]);
     free(s->ub); free(s->vb); free(s->wb); free(s->phi); free(s->dphidz);
     free(s->twx_c); free(s->twx_s); free(s->twy_c); free(s->twy_s);
@@ -1406,6 +1412,14 @@ static void calcuvwt(iwo_state *s, int cpenspecfilter)
 /* ------------------------------------------------------------------ */
 /* driver: main.f:653-1066 (istart=0, fixed dt)                           */
 /* ------------------------------------------------------------------ */
+/* plotf (output.f:125-260): u,v,w,temp to physical space for the dump and
+   back, the only effect on the state */
+static void plotf_trip(iwo_state *s)
+{
+    for (int q = 0; q < 4; q++) iwo_horfft(s, s->F[q], 1, s->nz);
+    for (int q = 0; q < 4; q++) iwo_horfft(s, s->F[q], -1, s->nz);
+}
+
 void iwo_set_initial(iwo_state *s, const double *u, const double *v, const double *w, const double *t)
 {
     long nb = sizeof(double) * (size_t)s->ntot;
@@ -1414,7 +1428,8 @@ void iwo_set_initial(iwo_state *s, const double *u, const double *v, const doubl
     memcpy(W(s), w, nb);
     memcpy(T(s), t, nb);
     for (int q = 4; q < 24; q++) memset(s->F[q], 0, nb);
-    for (int q = 0; q < 4; q++) iwo_horfft(s, s->F[q], -1, s->nz);
+    for (int q = 0; q < 4; q++) iwo_horfft(s, s->F[q], -1, s->nz); /* init.f:196-205 */
+    plotf_trip(s); /* output of the initial field, main.f:537-544 */
     s->t = 0.;
     s->istep = 0;
     s->istart = 0;
@@ -1661,6 +1676,23 @@ void iwo_set_restart(iwo_state *s, const double *state, double t, double cdt)
     s->bdfv = iwo_bdfvarv(d1, d2, d2);
     s->istep = 0;
     s->istart = 1;
+    for (int q = 0; q < 9; q++) /* a restarted run has no SGS tensors yet */
+        memset(s->sgs[q], 0, sizeof(double) * (size_t)s->ntot);
+    plotf_trip(s); /* output of the restart field, main.f:599-608 */
+    if (s->mccap < 1) { /* main.f:634-635: its record is the one energy record until the next run */
+        free(s->mcrec);
+        free(s->erec);
+        free(s->lrec);
+        s->mcrec = (double *)calloc(IWO_NREC, sizeof(double));
+        s->erec = (double *)calloc(IWO_NEREC, sizeof(double));
+        s->lrec = (double *)calloc((size_t)(2 + 4 * s->nz), sizeof(double));
+        s->mccap = 1;
+    }
+    s->erec[0] = 0.;
+    s->erec[1] = s->t;
+    iwo_postp(s, s->erec + 2);
+    s->en = 1;
+    s->ln = 0;
 }
 
 /* variable-step coefficients for the step sizes (dt1, dt2, dt3), the
@@ -1776,6 +1808,45 @@ void iwo_maxcalc(iwo_state *s, int istep, double *rec)
     if (rec) memcpy(rec, r, sizeof r);
 }
 
+double *iwo_speclegen_records(iwo_state *s, int *n)
+{
+    if (n) *n = s->ln;
+    return s->lrec;
+}
+
+/* speclegencalc (filtering.f:1110-1221) */
+void iwo_speclegen(iwo_state *s, double *f, double *out)
+{
+    const int nl = s->nzloc, nzm = nl - 1, nx = s->nx;
+    /* j = nyh (1-based); ny = 1 has no such row (the reference reads
+       outside the array there), row 1 is taken instead */
+    const long j = s->ny >= 2 ? s->ny / 2 - 1 : 0;
+    iwo_horfft(s, f, 1, s->nz);
+    double *acc = (double *)malloc(sizeof(double) * (size_t)nl);
+    for (int ks = 0; ks < s->nsubd; ks++) {
+        for (int kl = 0; kl < nl; kl++) acc[kl] = 0.;
+        for (int i = 0; i < nx; i++) {
+            for (int k = 0; k <= nzm; k++) {
+                double sum = 0.;
+                for (int k1 = 0; k1 < nl; k1++)
+                    sum = sum + s->wg[k1] * f[(long)(ks * nl + k1) * s->nxppy + j * s->nxpp + i]
+                                    * s->alpol[k * nl + k1];
+                const double a = (k == nzm) ? 0.5 * nzm * sum : (k + 0.5) * sum;
+                acc[k] = acc[k] + fabs(a);
+            }
+        }
+        for (int kl = 0; kl < nl; kl++) out[kl * s->nsubd + ks] = acc[kl] / nx;
+    }
+    free(acc);
+    iwo_horfft(s, f, -1, s->nz);
+}
+
+double *iwo_energy_records(iwo_state *s, int *n)
+{
+    if (n) *n = s->en;
+    return s->erec;
+}
+
 double *iwo_maxcalc_records(iwo_state *s, int *n)
 {
     if (n) *n = s->mcn;
@@ -1785,9 +1856,15 @@ double *iwo_maxcalc_records(iwo_state *s, int *n)
 void iwo_run(iwo_state *s, int nsteps, int *counts)
 {
     s->mcn = 0;
+    s->en = 0;
+    s->ln = 0;
     if (nsteps > s->mccap) {
         free(s->mcrec);
+        free(s->erec);
+        free(s->lrec);
         s->mcrec = (double *)calloc((size_t)nsteps * IWO_NREC, sizeof(double));
+        s->erec = (double *)calloc((size_t)nsteps * IWO_NEREC, sizeof(double));
+        s->lrec = (double *)calloc((size_t)nsteps * (2 + 4 * s->nz), sizeof(double));
         s->mccap = nsteps;
     }
     for (int n = 0; n < nsteps; n++) {
@@ -1830,11 +1907,177 @@ void iwo_run(iwo_state *s, int nsteps, int *counts)
         if (s->P.ifilter == 1 && s->P.nstepfilter > 0 && istep % s->P.nstepfilter == 0) /* main.f:880-895 */
             iwo_filtering(s, s->P.npostp > 0 && istep % s->P.npostp == 0);
         s->t = s->t + s->dt;
+        if (s->P.ncalclegspec > 0 && istep % s->P.ncalclegspec == 0) { /* main.f:905-911 */
+            double *e = s->lrec + (size_t)s->ln * (2 + 4 * s->nz);
+            e[0] = istep;
+            e[1] = s->t;
+            for (int q = 0; q < 4; q++) iwo_speclegen(s, s->F[q], e + 2 + (size_t)q * s->nz);
+            s->ln++;
+        }
         int ic[4] = {0, 0, 0, 0};
         if (s->P.navgintrfc > 0 && istep % s->P.navgintrfc == 0) /* main.f:920-932 */
             for (int q = 0; q < 4; q++) ic[q] = iwo_interface_avg3d(s, s->F[q]);
         if (counts)
             for (int q = 0; q < 4; q++) counts[4 * n + q] = ic[q];
+        if (s->P.npostp > 0 && istep % s->P.npostp == 0) { /* main.f:935-936 */
+            double *e = s->erec + (size_t)s->en * IWO_NEREC;
+            e[0] = istep;
+            e[1] = s->t;
+            iwo_postp(s, e + 2);
+            s->en++;
+        }
         memset(s->F[12], 0, sizeof(double) * (size_t)s->ntot); /* main.f:938-944 */
+        if (s->P.n3doutput > 0 && istep % s->P.n3doutput == 0) plotf_trip(s); /* main.f:996-1039 */
     }
+}
+
+/* ------------------------------------------------------------------ */
+/* postp (postp.f:9-76): umeancalc, dudzmeancalc, dudymeancalc,          */
+/* perturbcalc(0) and energycalc_postp (postp.f:129-363, 415-732)        */
+/* ------------------------------------------------------------------ */
+void iwo_postp(iwo_state *s, double *out)
+{
+    const int nx = s->nx, ny = s->ny, nz = s->nz, nxpp = s->nxpp, nl = s->nzloc;
+    const long nxppy = s->nxppy;
+    double *u = U(s), *v = V(s), *w = W(s), *tp = T(s);
+    double **g = s->g;
+    for (int q = 0; q < 4; q++) iwo_horfft(s, s->F[q], 1, nz); /* postp.f:23-26 */
+    /* umeancalc: x means at every (j,k), k >= 2 (k = 1 stays 0) */
+    double *um = (double *)calloc((size_t)ny * nz, sizeof(double));
+    double *tm = (double *)calloc((size_t)6 * ny * nz, sizeof(double));
+    for (int k = 1; k < nz; k++)
+        for (int j = 0; j < ny; j++) {
+            const long o = (long)k * nxppy + (long)j * nxpp;
+            double a = 0., t6[6] = {0., 0., 0., 0., 0., 0.};
+            for (int i = 0; i < nx; i++) {
+                a = a + u[o + i];
+                for (int q = 0; q < 6; q++)
+                    if (s->sgs[q]) t6[q] = t6[q] + s->sgs[q][o + i];
+            }
+            um[(long)k * ny + j] = a / nx;
+            for (int q = 0; q < 6; q++) tm[((long)q * nz + k) * ny + j] = t6[q] / nx;
+        }
+    /* tempmean through a round trip of temp (postp.f:189-196) */
+    iwo_horfft(s, tp, -1, nz);
+    iwo_horfft(s, tp, 1, nz);
+    /* dudzmeancalc: dpdzyzplane (postp.f:984-1030) */
+    double *umz = (double *)calloc((size_t)ny * nz, sizeof(double));
+    for (int j = 0; j < ny; j++)
+        for (int ks = 0; ks < s->nsubd; ks++)
+            for (int k1 = 0; k1 < nl; k1++) {
+                double sum = 0.0;
+                for (int k2 = 0; k2 < nl; k2++) sum = sum + s->d[k1 * nl + k2] * um[(long)(ks * nl + k2) * ny + j];
+                umz[(long)(ks * nl + k1) * ny + j] = sum * (2. / s->P.zh[ks]);
+            }
+    /* dudymeancalc (postp.f:237-296): spectral y derivative of the real
+       sequence umean(.,k) (FFT991: forward normalised by 1/ny, coefficient
+       m times i m beta, the Nyquist coefficient dropping out) */
+    double *umy = (double *)calloc((size_t)ny * nz, sizeof(double));
+    if (ny > 1) {
+        double *cr = (double *)calloc((size_t)ny, sizeof(double)), *ci = (double *)calloc((size_t)ny, sizeof(double));
+        for (int k = 0; k < nz; k++) {
+            const double *a = um + (long)k * ny;
+            for (int m = 1; m < ny / 2; m++) {
+                double sr = 0., si = 0.;
+                for (int q = 0; q < ny; q++) {
+                    const double ang = s->pi2 * (double)((long)m * q % ny) / ny;
+                    sr = sr + a[q] * cos(ang);
+                    si = si - a[q] * sin(ang);
+                }
+                const double kb = m * s->beta;
+                cr[m] = -kb * (si / ny); /* i m beta (a_m + i b_m) */
+                ci[m] = kb * (sr / ny);
+            }
+            for (int j = 0; j < ny; j++) {
+                double sum = 0.;
+                for (int m = 1; m < ny / 2; m++) {
+                    const double ang = s->pi2 * (double)((long)m * j % ny) / ny;
+                    sum = sum + 2. * (cr[m] * cos(ang) - ci[m] * sin(ang));
+                }
+                umy[(long)k * ny + j] = sum;
+            }
+        }
+        free(cr);
+        free(ci);
+    }
+    /* perturbcalc(0): u' = u - umean (postp.f:333-345) */
+    for (int k = 0; k < nz; k++)
+        for (int j = 0; j < ny; j++)
+            for (int i = 0; i < nx; i++) u[(long)k * nxppy + (long)j * nxpp + i] -= um[(long)k * ny + j];
+    /* energycalc_postp: gradients of the perturbation fields (postp.f:471-514) */
+    for (int q = 0; q < 4; q++) iwo_horfft(s, s->F[q], -1, nz);
+    iwo_dpdz(s, u, g[2]);
+    iwo_dpdz(s, v, g[5]);
+    iwo_dpdz(s, w, g[8]);
+    iwo_dpdz(s, tp, g[11]);
+    for (int k = 0; k < nz; k++)
+        for (int j = 0; j < ny; j++)
+            for (int ii = 0; ii < s->nxhp; ii++) {
+                const long o = (long)k * nxppy + (long)j * nxpp + 2 * ii;
+                const double xw = s->xw[ii], yw = s->yw[j];
+                double *src[4] = {u, v, w, tp};
+                const int dx[4] = {0, 3, 6, 9}, dy[4] = {1, 4, 7, 10};
+                for (int q = 0; q < 4; q++) {
+                    const double fr = src[q][o], fi = src[q][o + 1];
+                    g[dx[q]][o] = -xw * fi;
+                    g[dx[q]][o + 1] = xw * fr;
+                    g[dy[q]][o] = -yw * fi;
+                    g[dy[q]][o + 1] = yw * fr;
+                }
+            }
+    for (int q = 0; q < 4; q++) iwo_horfft(s, s->F[q], 1, nz);
+    for (int q = 0; q < 12; q++) iwo_horfft(s, g[q], 1, nz);
+    /* TKE budget sums over k >= 2, all j, i <= nx (postp.f:553-694); the
+       wake-interior variants are empty: wakelengthcalc is not called
+       (postp.f:39-41) and the zero wake sizes exclude every point */
+    double tketot = 0., u1 = 0., u2 = 0., u3 = 0., prody = 0., prodz = 0., bflux = 0., tempvar = 0.;
+    double eps = 0., epssgs = 0., chi = 0., chisgs = 0., tkemean = 0.;
+    for (int k = 1; k < nz; k++)
+        for (int j = 0; j < ny; j++) {
+            for (int i = 0; i < nx; i++) {
+                const long n = (long)k * nxppy + (long)j * nxpp + i;
+                const double uu = u[n], vv = v[n], ww = w[n], tt = tp[n];
+                tketot = tketot + (uu * uu + vv * vv + ww * ww);
+                u1 = u1 + uu * uu;
+                u2 = u2 + vv * vv;
+                u3 = u3 + ww * ww;
+                prody = prody - umy[(long)k * ny + j] * (uu * vv);
+                prodz = prodz - umz[(long)k * ny + j] * (uu * ww);
+                bflux = bflux + ww * tt;
+                tempvar = tempvar + tt * tt;
+                eps = eps + g[0][n] * g[0][n] + g[1][n] * g[1][n] + g[2][n] * g[2][n] + g[3][n] * g[3][n]
+                      + g[4][n] * g[4][n] + g[5][n] * g[5][n] + g[6][n] * g[6][n] + g[7][n] * g[7][n]
+                      + g[8][n] * g[8][n];
+                const double s11 = g[0][n], s22 = g[4][n], s33 = g[8][n];
+                const double s12 = 0.5 * (g[1][n] + g[3][n]), s13 = 0.5 * (g[2][n] + g[6][n]);
+                const double s23 = 0.5 * (g[5][n] + g[7][n]);
+                double f[6];
+                for (int q = 0; q < 6; q++)
+                    f[q] = (s->sgs[q] ? s->sgs[q][n] : 0.) - tm[((long)q * nz + k) * ny + j];
+                /* t11 t12 t13 t22 t23 t33 */
+                const double sg = f[0] * s11 + f[3] * s22 + f[5] * s33 + 2. * f[1] * s12 + 2. * f[2] * s13
+                                  + 2. * f[4] * s23;
+                epssgs = epssgs - sg;
+                chi = chi + g[9][n] * g[9][n] + g[10][n] * g[10][n] + g[11][n] * g[11][n];
+                const double th1 = s->sgs[6] ? s->sgs[6][n] : 0., th2 = s->sgs[7] ? s->sgs[7][n] : 0.;
+                const double th3 = s->sgs[8] ? s->sgs[8][n] : 0.;
+                chisgs = chisgs - (th1 * g[9][n] + th2 * g[10][n] + th3 * g[11][n]);
+            }
+            tkemean = tkemean + um[(long)k * ny + j] * um[(long)k * ny + j];
+        }
+    const double pr = s->P.xnu, ra = 0.; /* main.f:428; ra is 0 (output:73) */
+    tketot = tketot * 0.5;
+    tkemean = tkemean * 0.5 * (double)nx;
+    bflux = bflux * ra * pr;
+    eps = eps * pr;
+    chi = chi * s->P.xkappa;
+    const double r[16] = {tketot, u1, u2, u3, prody, prodz, prody + prodz, bflux, eps, epssgs, eps + epssgs,
+                          chi, chisgs, chisgs + chi, tkemean, tempvar};
+    if (out) memcpy(out, r, sizeof r);
+    /* the mean back into u (postp.f:712-719), then to spectral space */
+    for (int k = 1; k < nz; k++)
+        for (int j = 0; j < ny; j++)
+            for (int i = 0; i < nx; i++) u[(long)k * nxppy + (long)j * nxpp + i] += um[(long)k * ny + j];
+    for (int q = 0; q < 4; q++) iwo_horfft(s, s->F[q], -1, nz);
+    free(um); free(tm); free(umz); free(umy);
 }

@@ -26,6 +26,8 @@ typedef struct iwo_params {
     int ispecfilternl, ispecfilterexp, nspecfilterexp, navgintrfc;
     int ifilter, nstepfilter, npostp; /* estimation filter, main.f:880-895 */
     int imaxcalc, dtadapt;            /* maxcalc reductions, adaptive dt (various.f:665-987) */
+    int n3doutput;                    /* plotf round trip every n3doutput steps (main.f:996-1039) */
+    int ncalclegspec;                 /* speclegencalc every ncalclegspec steps (main.f:905-911) */
     double xlen, ylen, zlen;
     double xnu, xkappa, brunt, grav, rho0, omz;
     double dt;
@@ -76,6 +78,25 @@ void iwo_filtering(iwo_state *s, int isgscalc);
    5 t33, 6 th1, 7 th2, 8 th3 (comsgs.h) */
 double *iwo_sgs(iwo_state *s, int idx);
 
+/* postp (postp.f:9-76) on the spectral state: the TKE budget of
+   energycalc_postp in the energy.dat order of postp.f:53-58 (out, 16
+   values: tketot, u1tke, u2tke, u3tke, prody, prodz, prod, bflux, epsilon,
+   epssgs, epstot, chi, chisgs, chitot, tkemean, tempvar) and the state
+   change of its horfft round trips */
+void iwo_postp(iwo_state *s, double *out);
+/* postp calls of the last iwo_run (every npostp steps, main.f:935), or
+   the one of iwo_set_restart (main.f:635, istep 0): n records of IWO_NEREC
+   doubles, istep, t and the 16 values above */
+#define IWO_NEREC 18
+double *iwo_energy_records(iwo_state *s, int *n);
+/* speclegencalc (filtering.f:1110-1221) of spectral field f (in place: its
+   horfft round trip): the x-averaged |Legendre coefficients| at j = ny/2,
+   out[kloc * nsubd + ks] (the rows of speclegen<c><t>.dat) */
+void iwo_speclegen(iwo_state *s, double *f, double *out);
+/* speclegencalc calls of the last iwo_run (every ncalclegspec steps, for
+   u,v,w,temp, main.f:905-911): n records of 2 + 4 nzloc nsubd doubles,
+   istep, t and the four spectra */
+double *iwo_speclegen_records(iwo_state *s, int *n);
 /* the nine (nxpp,nz) arrays of /sol2dvar/ (comsoliton.h:9-13) in COMMON
    order: umwv, wmwv, dumdx, dumdz, dwmdx, dwmdz, rhomwv, drhomdx, drhomdz */
 void iwo_set_meanwave(iwo_state *s, const double *mw);

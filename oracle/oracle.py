@@ -49,6 +49,12 @@ def lib():
         L.iwo_set_meanwave.argtypes = [P, P]
         L.iwo_maxcalc_records.restype = ctypes.POINTER(D)
         L.iwo_maxcalc_records.argtypes = [P, ctypes.POINTER(ctypes.c_int)]
+        L.iwo_postp.argtypes = [P, P]
+        L.iwo_speclegen.argtypes = [P, P, P]
+        L.iwo_speclegen_records.restype = ctypes.POINTER(D)
+        L.iwo_speclegen_records.argtypes = [P, ctypes.POINTER(ctypes.c_int)]
+        L.iwo_energy_records.restype = ctypes.POINTER(D)
+        L.iwo_energy_records.argtypes = [P, ctypes.POINTER(ctypes.c_int)]
         L.iwo_sgs.restype = ctypes.POINTER(D)
         L.iwo_sgs.argtypes = [P, ctypes.c_int]
         for n in ("abvar1", "abvar2", "abvar3", "bdfvarv", "bdfvar1", "bdfvar2", "bdfvar3"):
@@ -62,6 +68,11 @@ def lib():
 def _ptr(a):
     assert a.dtype == np.float64 and a.flags.c_contiguous
     return a.ctypes.data_as(ctypes.c_void_p)
+
+
+# the values of postp's fort.60 line (postp.f:53-58) after t, t*brunt
+ENERGY_NAMES = ("tketot", "u1tke", "u2tke", "u3tke", "prody", "prodz", "prod", "bflux", "epsilon",
+                "epssgs", "epstot", "chi", "chisgs", "chitot", "tkemean", "tempvar")
 
 
 class Oracle:
@@ -164,6 +175,40 @@ class Oracle:
         self.L.iwo_maxcalc(self.h, istep, _ptr(r))
         return r
 
+    def postp(self):
+        """postp (postp.f:9-76) on the current state: the 16 TKE-budget
+        values in fort.60's order (ENERGY_NAMES); the state takes postp's
+        horfft round trips."""
+        r = np.zeros(16)
+        self.L.iwo_postp(self.h, _ptr(r))
+        return r
+
+    def speclegen(self, idx):
+        """speclegencalc (filtering.f:1110-1221) of state field idx (0-3):
+        (nzloc, nsubd) spectra; the field takes the horfft round trip."""
+        r = np.zeros(self.P.nzloc * self.P.nsubd)
+        self.L.iwo_speclegen(self.h, self.L.iwo_field(self.h, idx), _ptr(r))
+        return r.reshape(self.P.nzloc, self.P.nsubd)
+
+    def speclegen_records(self):
+        """(n, 2 + 4 nz) records (istep, t, the u,v,w,temp spectra, each
+        (nzloc, nsubd) row-major) of the speclegencalc calls of the last run."""
+        n = ctypes.c_int(0)
+        p = self.L.iwo_speclegen_records(self.h, ctypes.byref(n))
+        m = 2 + 4 * self.shape[0]
+        if n.value == 0:
+            return np.zeros((0, m))
+        return np.ctypeslib.as_array(p, shape=(n.value * m,)).reshape(n.value, m).copy()
+
+    def energy_records(self):
+        """(n, 18) records (istep, t, 16 values) of the postp calls of the
+        last run (every npostp steps, main.f:935)."""
+        n = ctypes.c_int(0)
+        p = self.L.iwo_energy_records(self.h, ctypes.byref(n))
+        if n.value == 0:
+            return np.zeros((0, 18))
+        return np.ctypeslib.as_array(p, shape=(n.value * 18,)).reshape(n.value, 18).copy()
+
     def maxcalc_records(self):
         """(nsteps, NREC) maxcalc records of the last run."""
         n = ctypes.c_int(0)
@@ -193,10 +238,37 @@ def _input_dat(P, nsteps, istart=0, cdt=1.0):
     irelax = 0, ispecfilternl = {P.ispecfilternl}, ispecfilterexp = {P.ispecfilterexp},
     nspecfilterexp = {P.nspecfilterexp}, amp = {P.amp!r}, p = {P.p!r}, pf = {P.pf!r},
     fackc = {P.fackc!r}, fackcbv = {P.fackcbv!r}, facdir = {P.facdir!r}, facneumn = {P.facneumn!r},
-    facrobin = {P.facrobin!r}, navgintrfc = {P.navgintrfc}, ncalclegspec = 100000000,
+    facrobin = {P.facrobin!r}, navgintrfc = {P.navgintrfc},
+    ncalclegspec = {P.ncalclegspec if P.ncalclegspec > 0 else 100000000},
     ifilter = {P.ifilter}, nstepfilter = {P.nstepfilter}, tstart = 0., npostp = {P.npostp}, itempreset = 0,
-    nrestartdump = 100000000, n3doutput = 100000000, i0 = 0 $
+    nrestartdump = 100000000, n3doutput = {P.n3doutput if P.n3doutput > 0 else 100000000}, i0 = 0 $
 """
+
+
+def ref_has_postp(build):
+    """True when the reference build includes postp.f."""
+    return os.path.exists(os.path.join(REFDIR, build, "has_postp"))
+
+
+def _parse_energy(d):
+    path = os.path.join(d, "energy.txt")
+    if not os.path.exists(path):
+        return np.zeros((0, 18))
+    rows = [[float(x) for x in l.split()] for l in open(path) if l.strip()]
+    return np.array(rows).reshape(-1, 18)
+
+
+def _parse_speclegen(d, nzloc, nsubd):
+    """speclegen<c><t>.dat files of speclegencalc (filtering.f:1110-1221):
+    {(c, t_text): (nzloc, nsubd) array}; E12.6 values."""
+    out = {}
+    for fn in os.listdir(d):
+        if not (fn.startswith("speclegen") and fn.endswith(".dat")):
+            continue
+        rows = [l.split() for l in open(os.path.join(d, fn)) if l.strip()]
+        a = np.array([[float(x) for x in r[1:]] for r in rows])
+        out[(fn[9], fn[10:-4])] = a.reshape(nzloc, nsubd)
+    return out
 
 
 def ref_has_maxcalc(build):
@@ -240,7 +312,7 @@ def _parse_maxcalc(d):
 
 
 def run_ref(build, P, fields, nsteps=1, mode=0, probe=None, exe="iwref", isgs=0, start=None, cdt=1.0,
-            write_restart=False, maxcalc=False, meanwave=None):
+            write_restart=False, maxcalc=False, meanwave=None, energy=False, speclegen=False):
     """Run the reference build on physical ``fields`` (u,v,w,rho').
 
     ``exe="iwshim"`` runs the same harness linked with the product's Fortran
@@ -251,8 +323,13 @@ def run_ref(build, P, fields, nsteps=1, mode=0, probe=None, exe="iwref", isgs=0,
     every step (adaptive dt included) and the parsed outputs
     (``_parse_maxcalc``) are returned as the last item.  ``P.fcshear`` sets
     the shear current; ``meanwave`` (9 arrays (nz, nxpp) in /sol2dvar/
-    order) the mean-wave fields.
-    Returns (state[24], t, diag_lines[, restart_bytes][, maxcalc_outputs])."""
+    order) the mean-wave fields.  With ``energy`` the postp records of
+    energy.txt (n, 18: istep, t, the 16 values of ENERGY_NAMES; builds with
+    postp.f) follow, and with ``speclegen`` the spectra of the
+    speclegencalc files (``_parse_speclegen``).  ``mode=6`` runs postp alone
+    on the fields.
+    Returns (state[24], t, diag_lines[, restart_bytes][, maxcalc_outputs]
+    [, energy])."""
     exe = os.path.join(REFDIR, build, exe)
     nz, ny, nxpp = P.nzloc * P.nsubd, P.ny, P.nx + 2
     with tempfile.TemporaryDirectory() as d:
@@ -294,4 +371,8 @@ def run_ref(build, P, fields, nsteps=1, mode=0, probe=None, exe="iwref", isgs=0,
                 res.append(f.read())
         if maxcalc:
             res.append(_parse_maxcalc(d))
+        if energy:
+            res.append(_parse_energy(d))
+        if speclegen:
+            res.append(_parse_speclegen(d, P.nzloc, P.nsubd))
     return tuple(res)

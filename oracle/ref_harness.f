@@ -15,6 +15,16 @@ c      imaxc=1 (builds compiled with -DIW_MAXCALC, where various.f is
 c      part of the build) the reference maxcalc itself runs, adaptive dt
 c      included, and t, dt, cdt after it go to dtlog.txt;
 c    * convec, calcuvwt, interface_avg3d are the reference routines;
+c    * the state-changing output routines run where main.f calls them:
+c      plotf (output.f:125) after the initial transform (main.f:543),
+c      after a restart (main.f:607) and every n3doutput steps
+c      (main.f:996-1039), its dump going to p_out.dat; in builds
+c      compiled with -DIW_POSTP, postp (postp.f:9) after a restart
+c      (main.f:635) and every npostp steps (main.f:935), the sixteen
+c      /energetics/ values (comenergy.h) going to energy.txt at full
+c      precision in the order of postp's fort.60 line; speclegencalc
+c      (filtering.f:1110) of u,v,w,temp every ncalclegspec steps
+c      (main.f:905-911), writing its speclegen*.dat files;
 c    * the SS/BDF coefficient selection follows main.f:660-664,
 c      700-777 and 813-856.
 c  Without -DIW_MAXCALC, various.f is not compiled (flang rejects it for
@@ -29,6 +39,8 @@ c    mode=3  filter_3dlegenfourier of the IC fields (spectral), dump
 c    mode=4  interface_avg3d of the IC fields (spectral), dump + counts
 c    mode=5  filtering(isgs) of the IC fields (spectral), dump + the
 c            nine SGS tensors of /comsgs/
+c    mode=6  postp of the IC fields (spectral; -DIW_POSTP builds),
+c            after filtering(1) when isgs=1, energies to energy.txt
 c  istart=1 (namelist &inputs) in mode 0 restarts from start.dat with
 c  restart.f and the first-step coefficients of main.f:619-634, 703-734
 c  and 818-825; iwrite=1 writes restart_out.dat with output.f at the end.
@@ -51,6 +63,7 @@ c=======================================================================
       include 'comcritlayer.h'
       include 'comscratch.h'
       include 'comsgs.h'
+      include 'comenergy.h'
 
       logical cpenspecfilter, ldtadapt
       character*256 icfile, outfile, diagfile
@@ -158,6 +171,15 @@ c-main.f:563-634: restart file, set-up again, first-step coefficients
         bdf3 = bdfvar3(dt1,dt2,dt3)
         bdfv = bdfvarv(dt1,dt2,dt3)
         open(12,file=diagfile,status='unknown')
+        open(3,file='p_out.dat',form='unformatted',status='unknown')
+        call plotf(3,iform,t)
+        close(3)
+#ifdef IW_POSTP
+        open(14,file='energy.txt',status='unknown')
+        initenergy = 1
+        call postp(initenergy,t,brunt,tstart)
+        call wenergy(0,t)
+#endif
         goto 500
       endif
 
@@ -209,6 +231,16 @@ c-Initial condition: 4 physical fields (nxpp,ny,nz), pads ignored
         write(12,'(4(1x,I10))') icu,icv,icw,ict
         goto 900
       endif
+#ifdef IW_POSTP
+      if (mode.eq.6) then
+        if (isgs.eq.1) call filtering(1)
+        open(14,file='energy.txt',status='unknown')
+        initenergy = 0
+        call postp(initenergy,t,brunt,tstart)
+        call wenergy(0,t)
+        goto 900
+      endif
+#endif
 
 c-mode 0: full run from a physical-space initial condition
       call horfft(u,-1,nz)
@@ -217,6 +249,12 @@ c-mode 0: full run from a physical-space initial condition
       call horfft(temp,-1,nz)
 
       t = 0.
+      open(3,file='p_out.dat',form='unformatted',status='unknown')
+      call plotf(3,iform,t)
+      close(3)
+#ifdef IW_POSTP
+      open(14,file='energy.txt',status='unknown')
+#endif
       fac1 = 1.0
       fac2 = 0.
       fac3 = 0.
@@ -332,6 +370,13 @@ c-main.f:880-895
 
         t = t + dt
 
+        if (mod(istep,ncalclegspec).eq.0) then
+          call speclegencalc('u',t,u)
+          call speclegencalc('v',t,v)
+          call speclegencalc('w',t,w)
+          call speclegencalc('T',t,temp)
+        endif
+
         if (mod(istep,navgintrfc).eq.0) then
           call interface_avg3d(u,icu)
           call interface_avg3d(v,icv)
@@ -339,7 +384,18 @@ c-main.f:880-895
           call interface_avg3d(temp,ict)
           write(12,'(I8,4(1x,I10))') istep,icu,icv,icw,ict
         endif
+#ifdef IW_POSTP
+        if (mod(istep,npostp).eq.0) then
+          call postp(initenergy,t,brunt,tstart)
+          call wenergy(istep,t)
+        endif
+#endif
         call zero(su,ntot)
+        if (mod(istep,n3doutput).eq.0) then
+          open(3,file='p_out.dat',form='unformatted',status='unknown')
+          call plotf(3,iform,t)
+          close(3)
+        endif
  1000 continue
 
  900  continue
@@ -361,6 +417,17 @@ c-main.f:880-895
       write(11) t
       if (mode.eq.5) write(11) t11,t12,t13,t22,t23,t33,th1,th2,th3
       close(11)
+      end
+
+c-----------------------------------------------------------------------
+c  /energetics/ (comenergy.h) at full precision, fort.60's order
+c-----------------------------------------------------------------------
+      subroutine wenergy(istep,t)
+      include 'comenergy.h'
+      write(14,'(I8,17(1x,E24.16))') istep,t,tketot,u1tke,u2tke,
+     >  u3tke,prody,prodz,prod,bflux,epsilon,epssgs,epstot,chi,chisgs,
+     >  chitot,tkemean,tempvar
+      return
       end
 
 #ifndef IW_MAXCALC

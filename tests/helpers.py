@@ -95,3 +95,79 @@ def meanwave_fields(P, z):
     z = np.asarray(z)
     return np.stack([a * fz(np.pi * z)[:, None] * fx(kx * x)[None, :] for a, fx, fz in spec])
 
+
+
+def postp_closed_form(P, x, z, amp=(0.3, 0.2, 0.25, 0.15)):
+    """Physical fields with a TKE budget known in closed form:
+    u = a cos(alpha x) + z^2 (1 + cos(beta y)), v = d cos(alpha x) sin(beta y),
+    w = b cos(alpha x), temp = e cos(alpha x).  The x-mean of u is
+    z^2 (1 + cos beta y) (zero at the bottom point z = 0), so the perturbation
+    is a cos(alpha x) everywhere, dumean/dz = 2 z (1 + cos beta y) and
+    dumean/dy = -z^2 beta sin(beta y).  Sums over the points k >= 2 of the
+    grid (nx, ny >= 4) give the sixteen values of the energy.dat line with
+    pr = xnu and the Rayleigh number 0; the SGS tensors are zero."""
+    a, b, d, e = amp
+    nx, ny, nz = P.nx, P.ny, len(z)
+    al, be = 2 * np.pi / P.xlen, 2 * np.pi / P.ylen
+    y = np.arange(ny) * (P.ylen / ny)
+    X = np.asarray(x)[None, None, :]
+    Y = y[None, :, None]
+    Z = np.asarray(z)[:, None, None]
+    shp = (nz, ny, nx + 2)
+    u = np.broadcast_to(a * np.cos(al * X) + Z ** 2 * (1 + np.cos(be * Y)), shp).copy()
+    v = np.broadcast_to(d * np.cos(al * X) * np.sin(be * Y), shp).copy()
+    w = np.broadcast_to(b * np.cos(al * X), shp).copy()
+    t = np.broadcast_to(e * np.cos(al * X), shp).copy()
+    for f in (u, v, w, t):
+        f[:, :, nx:] = 0.0
+    nk = nz - 1
+    zz = np.asarray(z)[1:]
+    hx, hy = nx / 2, ny / 2
+    u1, u2, u3 = a * a * hx * ny * nk, d * d * hx * hy * nk, b * b * hx * ny * nk
+    prody = a * d * be * hx * hy * np.sum(zz ** 2)
+    prodz = -a * b * hx * ny * np.sum(2 * zz)
+    eps = P.xnu * nk * (a * a * al * al * hx * ny + d * d * (al * al + be * be) * hx * hy
+                        + b * b * al * al * hx * ny)
+    chi = P.xkappa * e * e * al * al * hx * ny * nk
+    vals = [0.5 * (u1 + u2 + u3), u1, u2, u3, prody, prodz, prody + prodz, 0.0, eps, 0.0, eps, chi, 0.0, chi,
+            0.5 * nx * 1.5 * ny * np.sum(zz ** 4), e * e * hx * ny * nk]
+    return [u, v, w, t], np.array(vals)
+
+
+# groups of the energy.dat values that share a scale: (members, scale members)
+_EGROUPS = [((0, 1, 2, 3), (0,)), ((4, 5, 6), (4, 5)), ((7,), (0,)), ((8, 9, 10), (8, 9)),
+            ((11, 12, 13), (11, 12)), ((14,), (14,)), ((15,), (15,))]
+
+
+def energy_close(mine, ref, tol):
+    """Largest error of the sixteen postp values relative to the scale of
+    their group (the sum of the magnitudes of a budget's terms, so a
+    production or SGS flux that cancels to nearly zero is held to its
+    terms' size)."""
+    mine, ref = np.asarray(mine), np.asarray(ref)
+    worst = 0.0
+    for members, sc in _EGROUPS:
+        s = sum(abs(ref[i]) for i in sc) or 1.0
+        for i in members:
+            worst = max(worst, abs(mine[i] - ref[i]) / s)
+    return worst
+
+
+def legendre_closed_form(P, zpts, x, modes):
+    """A field whose Legendre content at j = ny/2 is one mode per subdomain:
+    P_m(zeta) (1 + cos(alpha x)/2) with m = modes[ks]; its speclegencalc
+    spectrum is 1 at (m, ks) and 0 elsewhere (GLL quadrature is exact for
+    P_m P_k, m + k < 2 nzm, and a(nzm) carries the factor nzm/2 that makes
+    the sum of P_nzm^2 = 2/nzm come out as 1)."""
+    from numpy.polynomial import legendre as Lg
+    nl = P.nzloc
+    g = 1 + 0.5 * np.cos(2 * np.pi / P.xlen * np.asarray(x))
+    f = np.zeros((nl * P.nsubd, P.ny, P.nx + 2))
+    want = np.zeros((nl, P.nsubd))
+    for ks, m in enumerate(modes):
+        c = np.zeros(m + 1)
+        c[m] = 1.0
+        pm = Lg.legval(np.asarray(zpts), c)
+        f[ks * nl:(ks + 1) * nl, :, :P.nx] = pm[:, None, None] * g[None, None, :P.nx]
+        want[m, ks] = 1.0
+    return f, want

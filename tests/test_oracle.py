@@ -6,7 +6,8 @@ import numpy as np
 import pytest
 from numpy.polynomial import legendre as L
 
-from helpers import SOLUTION, STATE, case, g1_params, g2_params, g3_params, meanwave_fields, rel, routine_inputs
+from helpers import (SOLUTION, STATE, case, energy_close, g1_params, g2_params, g3_params, legendre_closed_form,
+                     meanwave_fields, postp_closed_form, rel, routine_inputs)
 from iwrepo_amd.params import make_params
 
 # ---------------------------------------------------------------- GLL tables
@@ -471,3 +472,97 @@ def test_meanwave_and_shear_terms_match_compiled_reference(oracle_mod, name):
         assert np.array_equal(f51[:, [2, 3, 4, 6, 7, 8, 10, 11, 12]], rec[:, [23, 24, 25, 27, 28, 29, 31, 32, 33]])
         f50 = np.array(m["fort50"])
         assert np.abs(f50[:, 1:] - rec[:, 2:18]).max() < 5e-6
+
+
+# ------------------------- diagnostics: postp TKE budget, speclegencalc (f4)
+
+
+@pytest.mark.parametrize("name", ["small", "c1"])
+def test_postp_budget_closed_form(oracle_mod, name):
+    """energycalc_postp (postp.f:415-732) on fields whose budget is known in
+    closed form: each of the sixteen values (the half factors, pr = xnu and
+    xkappa scalings, the k >= 2 range, the x-mean removal and the spectral
+    dumean/dy) against its formula; the state comes back unchanged (smooth
+    fields: the round trips are exact)."""
+    P, _ = case(name)
+    o = oracle_mod.Oracle(P)
+    z, x = o.table(7, P.nzloc * P.nsubd), o.table(8, P.nx + 2)
+    flds, want = postp_closed_form(P, x, z)
+    spec = [o.horfft(f, -1) for f in flds]
+    for q in range(4):
+        o.set_field(q, spec[q])
+    got = o.postp()
+    assert energy_close(got, want, 0) < 1e-12
+    for q in range(4):
+        assert rel(o.field(q), spec[q]) < 1e-14
+
+
+@pytest.mark.parametrize("name", ["small", "c1"])
+def test_postp_matches_compiled_reference(oracle_mod, name):
+    """postp (postp.f:9-76) alone (harness mode 6): the /energetics/ values
+    and the state after its horfft round trips (two for u,v,w, three for
+    temp), without and with the SGS tensors of filtering(1)."""
+    if not oracle_mod.ref_available(name) or not oracle_mod.ref_has_postp(name):
+        pytest.skip("reference build %s with postp.f not present (needs /root/reference)" % name)
+    P, flds = case(name)
+    _, spec = routine_inputs(oracle_mod, P, flds)
+    for isgs in (0, 1):
+        st, _, _, en = oracle_mod.run_ref(name, P, spec, nsteps=0, mode=6, isgs=isgs, energy=True)
+        o = oracle_mod.Oracle(P)
+        for q in range(4):
+            o.set_field(q, spec[q])
+        if isgs:
+            o.filtering(1)
+        got = o.postp()
+        assert energy_close(got, en[0, 2:], 0) < 1e-12, isgs
+        for q in range(4):
+            assert rel(o.field(q), st[q]) < 1e-14, (isgs, q)
+        if isgs:
+            assert abs(got[9]) > 1e-3 * abs(got[8])  # the SGS flux is present
+
+
+def test_speclegen_closed_form(oracle_mod):
+    """speclegencalc (filtering.f:1110-1221) of a field with one Legendre mode
+    per subdomain, the last one the top mode nzm (its factor nzm/2)."""
+    P, _ = case("c1")
+    o = oracle_mod.Oracle(P)
+    x, zp = o.table(8, P.nx + 2), o.table(0, P.nzloc)
+    modes = [(2 * ks + 1) % P.nzloc for ks in range(P.nsubd)]
+    modes[-1] = P.nzloc - 1
+    f, want = legendre_closed_form(P, zp, x, modes)
+    sp = o.horfft(f, -1)
+    o.set_field(0, sp)
+    assert np.abs(o.speclegen(0) - want).max() < 1e-13
+    assert rel(o.field(0), sp) < 1e-14
+
+
+def test_step_loop_diagnostics_match_compiled_reference(oracle_mod):
+    """The output routines inside the step loop: speclegencalc every
+    ncalclegspec steps (main.f:905-911), postp every npostp steps with the
+    estimation filter's SGS tensors (main.f:888-893, 935) and plotf every
+    n3doutput steps (main.f:996-1039): the state, the /energetics/ records
+    and the speclegen*.dat spectra (E12.6 in the files)."""
+    if not oracle_mod.ref_available("small") or not oracle_mod.ref_has_postp("small"):
+        pytest.skip("reference build small with postp.f not present (needs /root/reference)")
+    P, flds = case("small")
+    P.ifilter, P.nstepfilter, P.npostp, P.n3doutput, P.ncalclegspec = 1, 3, 3, 2, 2
+    o = oracle_mod.Oracle(P)
+    o.set_initial(*flds)
+    o.run(6)
+    st, t, _, en, sl = oracle_mod.run_ref("small", P, flds, nsteps=6, energy=True, speclegen=True)
+    mine = o.state()
+    for q in SOLUTION:
+        assert rel(mine[q], st[q]) < (1e-11 if q < 12 else 1e-9), STATE[q]
+    er = o.energy_records()
+    assert er.shape == en.shape == (2, 18)
+    assert np.array_equal(er[:, 0], en[:, 0]) and np.abs(er[:, 1] - en[:, 1]).max() < 1e-15
+    for r in range(2):
+        assert energy_close(er[r, 2:], en[r, 2:], 0) < 1e-9, r
+    lr = o.speclegen_records()
+    nz = P.nzloc * P.nsubd
+    assert lr.shape == (3, 2 + 4 * nz) and len(sl) == 12
+    for r in lr:
+        for q, c in enumerate("uvwT"):
+            a = r[2 + q * nz:2 + (q + 1) * nz].reshape(P.nzloc, P.nsubd)
+            b = sl[(c, "%5.3f" % r[1])]
+            assert np.all(np.abs(a - b) <= 5e-6 * np.abs(b) + 1e-10 * np.abs(b).max()), (r[0], c)
