@@ -147,6 +147,14 @@ int iw_postp(iw_solver *s, double *out);
  * the speclegen<c><t>.dat file; the field takes the horfft round trip.
  * iw_run calls it for u,v,w,temp every ncalclegspec steps (main.f:905-911). */
 int iw_speclegen(iw_solver *s, int idx, double *out);
+/* Kinetic and available potential energy of the state, from the offline
+ * tool CalcE (CalcE/integration.f:14-226, KEh/KEv/PEh over 2-D dumps) taken
+ * to 3-D: out = KE_u, KE_v, KE_w, PE with KE_q = int rho0/2 q^2 dV and
+ * PE = int g^2 rho'^2/(2 rho0 N^2) dV; horizontal mean times xlen (times
+ * ylen when ny > 1; per unit span in 2-D as the tool), the z integral by
+ * integrcol (derv.f:479-512) per subdomain, PE without the tool's double
+ * count of i = 1 (integration.f:221).  The state is not changed. */
+int iw_kepe(iw_solver *s, double *out);
 /* The records of the last iw_run (or of iw_load_restart): postp records of
  * 18 doubles (istep, t, the sixteen values) and speclegencalc records of
  * 2 + 4*nz doubles (istep, t, the u, v, w, temp spectra).  Copies n records

@@ -958,7 +958,7 @@ extern "C" int iw_destroy(iw_solver *s)
     hipFree(s->mc_gmax);
     hipFree(s->mc_ring);
     hipHostFree(s->h_mc);
-    double *pp[8] = {s->pp_um, s->pp_tm, s->pp_dy, s->pp_dz, s->pp_part, s->pp_rec, s->e_ring, s->l_ring};
+    double *pp[9] = {s->pp_um, s->pp_tm, s->pp_dy, s->pp_dz, s->pp_part, s->pp_rec, s->e_ring, s->l_ring, s->kp};
     for (double *q : pp) hipFree(q);
     hipFree(s->sys.AB);
     hipFree(s->sys.FB);
@@ -1691,6 +1691,32 @@ extern "C" int iw_speclegen(iw_solver *s, int idx, double *out)
     HIPCHK(hipMemcpyAsync(h.data(), s->pp_rec + 20, sizeof(double) * n, hipMemcpyDeviceToHost, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
     if (out) memcpy(out, h.data(), sizeof(double) * n);
+    return 0;
+}
+
+// kinetic and potential energy of the device state (CalcE/integration.f:
+// 14-226 in 3-D, integrcol quadrature); the state is not changed
+extern "C" int iw_kepe(iw_solver *s, double *out)
+{
+    if (!s || !out) return fail("iw_kepe: null argument");
+    const Geo &g = s->g;
+    if (!s->kp) {
+        HIPCHK(hipMalloc((void **)&s->kp, sizeof(double) * (4 * (size_t)g.nz + 4)));
+        s->dev_bytes += (long long)(sizeof(double) * (4 * (size_t)g.nz + 4));
+    }
+    FieldBatch b = batch({s->buf[0], s->buf[1], s->buf[2], s->buf[3]});
+    for (int q = 0; q < 4; q++) b.g[q] = s->phys[q];
+    fft_dev(s, b, 1, g.nz);
+    const iw_params &P = s->P;
+    const double span = P.xlen * (g.ny > 1 ? P.ylen : 1.0);
+    const double cpe = 1. / 2. / P.rho0 * P.grav * P.grav / (P.brunt * P.brunt);
+    {
+        Scope sc(s, "kepe");
+        launch_kepe(g, s->phys, P.rho0 / 2., cpe, span, s->kp, s->kp + 4 * (size_t)g.nz, s->stream);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, s->kp + 4 * (size_t)g.nz, 4 * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
     return 0;
 }
 

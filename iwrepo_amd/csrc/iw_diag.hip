@@ -11,6 +11,11 @@
 // speclegencalc (filtering.f:1110-1221): at the row j = ny/2 the Legendre
 // coefficients of every z column of every subdomain, their magnitudes
 // averaged over x.
+//
+// Kinetic and available potential energy (CalcE/integration.f:14-226, the
+// offline tool, in 3-D): per z level the horizontal mean of rho0/2 q^2 and
+// g^2 rho'^2/(2 rho0 N^2) times the span, then the GLL quadrature of
+// integrcol (derv.f:479-512) per subdomain.
 #include "iw_internal.h"
 
 namespace iw {
@@ -236,7 +241,57 @@ __global__ __launch_bounds__(kPpThreads) void k_speclegen(const double *__restri
     }
 }
 
+// per z level: span * mean over (j, i < nx) of c q^2, for u, v, w (c = cke)
+// and rho' (c = cpe); lvl[q * nz + k]
+__global__ __launch_bounds__(kPpThreads) void k_kepe_level(const double *__restrict__ u,
+                                                           const double *__restrict__ v,
+                                                           const double *__restrict__ w,
+                                                           const double *__restrict__ t, int nx, int ny, int nz,
+                                                           int nxpp, long nxppy, double cke, double cpe,
+                                                           double span, double *lvl)
+{
+    __shared__ double lds[4 * 16];
+    const int k = blockIdx.x;
+    double s[4] = {0., 0., 0., 0.};
+    const long n = (long)nx * ny;
+    for (long p = threadIdx.x; p < n; p += blockDim.x) {
+        const int j = (int)(p / nx), i = (int)(p - (long)j * nx);
+        const long o = (long)k * nxppy + (long)j * nxpp + i;
+        const double a = u[o], b = v[o], c = w[o], d = t[o];
+        s[0] += cke * (a * a);
+        s[1] += cke * (b * b);
+        s[2] += cke * (c * c);
+        s[3] += cpe * (d * d);
+    }
+    block_sum<4>(s, lds);
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 4; q++) lvl[(long)q * nz + k] = span * s[q] / ((double)nx * ny);
+}
+
+// integrcol per subdomain, summed: out[q] for the four fields
+__global__ void k_kepe_final(const double *__restrict__ lvl, const double *__restrict__ wg,
+                             const double *__restrict__ zh, int nl, int nsubd, int nz, double *out)
+{
+    const int q = threadIdx.x;
+    if (q >= 4) return;
+    double tot = 0.;
+    for (int ks = 0; ks < nsubd; ks++) {
+        double sum = 0.;
+        for (int k = 0; k < nl; k++) sum = sum + wg[k] * lvl[(long)q * nz + ks * nl + k];
+        tot = tot + sum * zh[ks] / 2.;
+    }
+    out[q] = tot;
+}
+
 } // namespace
+
+void launch_kepe(const Geo &g, double *const *phys4, double cke, double cpe, double span, double *lvl, double *out,
+                 hipStream_t st)
+{
+    hipLaunchKernelGGL(k_kepe_level, dim3((unsigned)g.nz), dim3(kPpThreads), 0, st, phys4[0], phys4[1], phys4[2],
+                       phys4[3], g.nx, g.ny, g.nz, g.nxpp, g.nxppy, cke, cpe, span, lvl);
+    hipLaunchKernelGGL(k_kepe_final, dim3(1), dim3(64), 0, st, lvl, g.wg, g.zh, g.nzloc, g.nsubd, g.nz, out);
+}
 
 int pp_blocks(const Geo &g)
 {

@@ -178,6 +178,7 @@ struct iw_solver {
     // (device rings, host copies: istep, t, 16 values / istep, t, 4 spectra)
     double *pp_um = nullptr, *pp_tm = nullptr, *pp_dy = nullptr, *pp_dz = nullptr, *pp_part = nullptr;
     double *pp_rec = nullptr;
+    double *kp = nullptr; // KE/PE: 4 nz level integrals and the 4 totals
     double *e_ring = nullptr, *l_ring = nullptr;
     int e_cap = 0, l_cap = 0;
     std::vector<double> e_hist, l_hist;
@@ -224,6 +225,10 @@ void launch_pp_sums(const Geo &g, double *const *uvwt, double *const *grad12, do
 void launch_speclegen(const Geo &g, const double *phys, double istep, double t, double *hdr, double *out,
                       hipStream_t st);
 void launch_grad_xy(const Geo &g, const double *a, double *dx, double *dy, hipStream_t st);
+// kinetic energy of u, v, w and potential energy of rho' (physical fields;
+// lvl: 4 nz scratch, out: 4 values)
+void launch_kepe(const Geo &g, double *const *phys4, double cke, double cpe, double span, double *lvl, double *out,
+                 hipStream_t st);
 void launch_rhs_spec(const Geo &g, double *const *S, double *const *pr, hipStream_t st);
 void launch_wnl(const Geo &g, const double *sw, const double *swn, const double *swnm,
                 const double *fac, double *wnl, hipStream_t st);

@@ -22,7 +22,7 @@ EXPORTS = [
     "iw_save_restart", "iw_load_restart", "iw_write_restart_file", "iw_read_restart_file", "iw_abvar1", "iw_abvar2", "iw_abvar3", "iw_bdfvarv",
     "iw_bdfvar1", "iw_bdfvar2", "iw_bdfvar3", "iw_set_profiling", "iw_kernel_stats",
     "iw_device_bytes", "iw_maxcalc", "iw_maxcalc_records", "iw_get_dtstate", "iw_set_dtstate",
-    "iw_set_meanwave", "iw_get_meanwave", "iw_postp", "iw_speclegen", "iw_set_sgs", "iw_energy_records",
+    "iw_set_meanwave", "iw_get_meanwave", "iw_postp", "iw_speclegen", "iw_set_sgs", "iw_kepe", "iw_energy_records",
     "iw_speclegen_records",
 ]
 
@@ -79,6 +79,7 @@ def lib():
     L.iw_get_meanwave.argtypes = [P, P]
     L.iw_postp.argtypes = [P, P]
     L.iw_set_sgs.argtypes = [P, I, P]
+    L.iw_kepe.argtypes = [P, P]
     L.iw_speclegen.argtypes = [P, I, P]
     L.iw_energy_records.argtypes = [P, P, I]
     L.iw_speclegen_records.argtypes = [P, P, I]
@@ -241,6 +242,13 @@ class Solver:
         tempvar); the state takes postp's horfft round trips."""
         r = np.zeros(16)
         _check(self.L.iw_postp(self.h, _ptr(r)))
+        return r
+
+    def kepe(self):
+        """KE_u, KE_v, KE_w, PE of the device state (CalcE/integration.f:
+        14-226 in 3-D; integrcol quadrature); the state is not changed."""
+        r = np.zeros(4)
+        _check(self.L.iw_kepe(self.h, _ptr(r)))
         return r
 
     def speclegen(self, idx):

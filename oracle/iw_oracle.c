@@ -2081,3 +2081,39 @@ void iwo_postp(iwo_state *s, double *out)
     for (int q = 0; q < 4; q++) iwo_horfft(s, s->F[q], -1, nz);
     free(um); free(tm); free(umz); free(umy);
 }
+
+/* ------------------------------------------------------------------ */
+/* KE / PE (CalcE/integration.f:14-226; quadrature derv.f:479-512)    */
+/* ------------------------------------------------------------------ */
+void iwo_kepe(iwo_state *s, double *out)
+{
+    const int nx = s->nx, ny = s->ny, nz = s->nz, nl = s->nzloc;
+    const double span = s->P.xlen * (ny > 1 ? s->P.ylen : 1.0);
+    const double rho0 = s->P.rho0, g = s->P.grav, bn = s->P.brunt;
+    const double cke = rho0 / 2., cpe = 1. / 2. / rho0 * g * g / (bn * bn);
+    double *f = (double *)malloc(sizeof(double) * (size_t)s->ntot);
+    double *lvl = (double *)malloc(sizeof(double) * (size_t)nz);
+    for (int q = 0; q < 4; q++) {
+        memcpy(f, s->F[q], sizeof(double) * (size_t)s->ntot);
+        iwo_horfft(s, f, 1, nz);
+        const double c = q < 3 ? cke : cpe;
+        for (int k = 0; k < nz; k++) { /* KEh / PEh: the horizontal mean times the span */
+            double sum = 0.;
+            for (int j = 0; j < ny; j++)
+                for (int i = 0; i < nx; i++) {
+                    const double v = f[(long)k * s->nxppy + (long)j * s->nxpp + i];
+                    sum = sum + c * (v * v);
+                }
+            lvl[k] = span * sum / ((double)nx * ny);
+        }
+        double tot = 0.;
+        for (int ks = 0; ks < s->nsubd; ks++) { /* integrcol per subdomain */
+            double sum = 0.;
+            for (int k = 0; k < nl; k++) sum = sum + s->wg[k] * lvl[ks * nl + k];
+            tot = tot + sum * s->P.zh[ks] / 2.;
+        }
+        out[q] = tot;
+    }
+    free(f);
+    free(lvl);
+}

@@ -89,6 +89,15 @@ void iwo_postp(iwo_state *s, double *out);
    doubles, istep, t and the 16 values above */
 #define IWO_NEREC 18
 double *iwo_energy_records(iwo_state *s, int *n);
+/* Kinetic and available potential energy of the state (CalcE/
+   integration.f:14-226 over the x-z dumps, generalised to 3-D): out =
+   KE_u, KE_v, KE_w, PE with KE_q = int rho0/2 q^2 dV and PE = int g^2
+   rho'^2 / (2 rho0 N^2) dV.  Horizontal: mean over i < nx (and j) times xlen
+   (times ylen when ny > 1; per unit span in 2-D, as the 2-D tool does);
+   vertical: GLL quadrature per subdomain, integrcol (derv.f:479-512).  PE
+   without the tool's double count of i = 1 (integration.f:221).  The state
+   is not changed. */
+void iwo_kepe(iwo_state *s, double *out);
 /* speclegencalc (filtering.f:1110-1221) of spectral field f (in place: its
    horfft round trip): the x-averaged |Legendre coefficients| at j = ny/2,
    out[kloc * nsubd + ks] (the rows of speclegen<c><t>.dat) */

@@ -51,6 +51,7 @@ def lib():
         L.iwo_maxcalc_records.argtypes = [P, ctypes.POINTER(ctypes.c_int)]
         L.iwo_postp.argtypes = [P, P]
         L.iwo_speclegen.argtypes = [P, P, P]
+        L.iwo_kepe.argtypes = [P, P]
         L.iwo_speclegen_records.restype = ctypes.POINTER(D)
         L.iwo_speclegen_records.argtypes = [P, ctypes.POINTER(ctypes.c_int)]
         L.iwo_energy_records.restype = ctypes.POINTER(D)
@@ -181,6 +182,13 @@ class Oracle:
         horfft round trips."""
         r = np.zeros(16)
         self.L.iwo_postp(self.h, _ptr(r))
+        return r
+
+    def kepe(self):
+        """KE_u, KE_v, KE_w, PE of the state (CalcE/integration.f:14-226 in
+        3-D, integrcol quadrature, PE without the i = 1 double count)."""
+        r = np.zeros(4)
+        self.L.iwo_kepe(self.h, _ptr(r))
         return r
 
     def speclegen(self, idx):
@@ -327,7 +335,8 @@ def run_ref(build, P, fields, nsteps=1, mode=0, probe=None, exe="iwref", isgs=0,
     energy.txt (n, 18: istep, t, the 16 values of ENERGY_NAMES; builds with
     postp.f) follow, and with ``speclegen`` the spectra of the
     speclegencalc files (``_parse_speclegen``).  ``mode=6`` runs postp alone
-    on the fields.
+    on the fields; ``mode=7`` the kinetic and potential energies of the
+    physical fields (appended last).
     Returns (state[24], t, diag_lines[, restart_bytes][, maxcalc_outputs]
     [, energy])."""
     exe = os.path.join(REFDIR, build, exe)
@@ -373,6 +382,8 @@ def run_ref(build, P, fields, nsteps=1, mode=0, probe=None, exe="iwref", isgs=0,
             res.append(_parse_maxcalc(d))
         if energy:
             res.append(_parse_energy(d))
+        if mode == 7:
+            res.append(np.array([float(x) for x in open(os.path.join(d, "kepe.txt")).read().split()]))
         if speclegen:
             res.append(_parse_speclegen(d, P.nzloc, P.nsubd))
     return tuple(res)

@@ -41,6 +41,9 @@ c    mode=5  filtering(isgs) of the IC fields (spectral), dump + the
 c            nine SGS tensors of /comsgs/
 c    mode=6  postp of the IC fields (spectral; -DIW_POSTP builds),
 c            after filtering(1) when isgs=1, energies to energy.txt
+c    mode=7  kinetic and potential energy of the IC fields (physical),
+c            the z integral by the reference's integrcol (derv.f:479),
+c            to kepe.txt
 c  istart=1 (namelist &inputs) in mode 0 restarts from start.dat with
 c  restart.f and the first-step coefficients of main.f:619-634, 703-734
 c  and 818-825; iwrite=1 writes restart_out.dat with output.f at the end.
@@ -231,6 +234,12 @@ c-Initial condition: 4 physical fields (nxpp,ny,nz), pads ignored
         write(12,'(4(1x,I10))') icu,icv,icw,ict
         goto 900
       endif
+      if (mode.eq.7) then
+        open(14,file='kepe.txt',status='unknown')
+        call kepe(u,v,w,temp)
+        close(14)
+        goto 900
+      endif
 #ifdef IW_POSTP
       if (mode.eq.6) then
         if (isgs.eq.1) call filtering(1)
@@ -417,6 +426,45 @@ c-main.f:880-895
       write(11) t
       if (mode.eq.5) write(11) t11,t12,t13,t22,t23,t33,th1,th2,th3
       close(11)
+      end
+
+c-----------------------------------------------------------------------
+c  KE_u, KE_v, KE_w, PE of physical fields: rho0/2 q^2 and
+c  g^2 rho'^2/(2 rho0 N^2) (CalcE/integration.f:51, 216) averaged over
+c  x and y times xlen (times ylen in 3-D), then integrcol per subdomain
+c-----------------------------------------------------------------------
+      subroutine kepe(u,v,w,temp)
+      include 'dim.h'
+      include 'comsubd.h'
+      include 'comparamphys.h'
+      dimension u(nxpp,ny,nz),v(nxpp,ny,nz),w(nxpp,ny,nz)
+      dimension temp(nxpp,ny,nz),h(nz),e(4)
+      span = xlen
+      if (ny.gt.1) span = xlen*ylen
+      do iq = 1,4
+        c = rho0/2.
+        if (iq.eq.4) c = 1./2./rho0*grav**2/brunt**2
+        do k = 1,nz
+          s = 0.
+          do j = 1,ny
+            do i = 1,nx
+              if (iq.eq.1) f = u(i,j,k)
+              if (iq.eq.2) f = v(i,j,k)
+              if (iq.eq.3) f = w(i,j,k)
+              if (iq.eq.4) f = temp(i,j,k)
+              s = s + c*(f*f)
+            enddo
+          enddo
+          h(k) = span*s/float(nx*ny)
+        enddo
+        e(iq) = 0.
+        do ks = 1,nsubd
+          call integrcol(h((ks-1)*nzloc+1),outint,nzm,zh(ks))
+          e(iq) = e(iq) + outint
+        enddo
+      enddo
+      write(14,'(4(1x,E24.16))') e
+      return
       end
 
 c-----------------------------------------------------------------------

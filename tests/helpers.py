@@ -171,3 +171,33 @@ def legendre_closed_form(P, zpts, x, modes):
         f[ks * nl:(ks + 1) * nl, :, :P.nx] = pm[:, None, None] * g[None, None, :P.nx]
         want[m, ks] = 1.0
     return f, want
+
+
+def kepe_closed_form(P, x, z, amp=(0.3, 0.2, 0.25, 0.15)):
+    """Physical fields with known kinetic and potential energy:
+    u = a cos(alpha x) z^2, v = d sin(beta y) z, w = b cos(alpha x),
+    rho' = e (1 + z).  Every integrand is a polynomial of degree <= 4 in z,
+    which GLL quadrature with nzloc >= 4 integrates exactly; the horizontal
+    means of cos^2 and sin^2 are 1/2 (nx, ny >= 4).  Returns the fields and
+    (KE_u, KE_v, KE_w, PE) over [0, xlen] x [0, ylen] x [0, zlen] (per unit
+    span when ny = 1)."""
+    a, b, d, e = amp
+    nx, ny, nz = P.nx, P.ny, len(z)
+    al, be = 2 * np.pi / P.xlen, 2 * np.pi / P.ylen
+    y = np.arange(ny) * (P.ylen / ny)
+    X = np.asarray(x)[None, None, :]
+    Y = y[None, :, None]
+    Z = np.asarray(z)[:, None, None]
+    shp = (nz, ny, nx + 2)
+    u = np.broadcast_to(a * np.cos(al * X) * Z ** 2, shp).copy()
+    v = np.broadcast_to(d * np.sin(be * Y) * Z, shp).copy()
+    w = np.broadcast_to(b * np.cos(al * X) + 0 * Z, shp).copy()
+    t = np.broadcast_to(e * (1 + Z) + 0 * X, shp).copy()
+    for f in (u, v, w, t):
+        f[:, :, nx:] = 0.0
+    span = P.xlen * (P.ylen if ny > 1 else 1.0)
+    L = P.zlen
+    ck, cp = P.rho0 / 2, P.grav ** 2 / (2 * P.rho0 * P.brunt ** 2)
+    want = [ck * span * a * a / 2 * L ** 5 / 5, ck * span * d * d / 2 * L ** 3 / 3 if ny > 1 else 0.0,
+            ck * span * b * b / 2 * L, cp * span * e * e * ((1 + L) ** 3 - 1) / 3]
+    return [u, v, w, t], np.array(want)

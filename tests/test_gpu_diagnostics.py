@@ -5,8 +5,8 @@ the C-ABI, against the CPU oracle, against closed forms (also at the full
 import numpy as np
 import pytest
 
-from helpers import (SOLUTION, STATE, case, energy_close, legendre_closed_form, postp_closed_form, rel,
-                     routine_inputs)
+from helpers import (SOLUTION, STATE, case, energy_close, g3_params, kepe_closed_form, legendre_closed_form,
+                     postp_closed_form, rel, routine_inputs)
 
 pytestmark = pytest.mark.gpu
 
@@ -147,6 +147,35 @@ def test_restart_postp_record(rt, oracle_mod, tmp_path):
     for q in SOLUTION:
         assert rel(sh[q], so[q]) < (FIELD_TOL if q < 12 else HIST_TOL), STATE[q]
     h.close()
+
+
+@pytest.mark.parametrize("name", ["c1", "g3", "c2"])
+def test_kepe_on_device(rt, oracle_mod, name):
+    """iw_kepe against the oracle on the wave case and against the closed
+    form (c2: the full 256x128x(16x25) grid; g3: 2-D, unequal subdomains);
+    the state is left as it was."""
+    if name == "g3":
+        P, flds = g3_params()[0], None
+    else:
+        P, flds = case(name)
+    o, g = oracle_mod.Oracle(P), rt.Solver(P)
+    z, x = o.table(7, P.nzloc * P.nsubd), o.table(8, P.nx + 2)
+    if name == "c1":
+        print("kepe wave", name, flush=True)
+        o.set_initial(*flds)
+        g.set_initial(*flds)
+        ko, kg = o.kepe(), g.kepe()
+        assert np.all(np.abs(kg - ko) <= 1e-13 * np.abs(ko)), (ko, kg)
+    cf, want = kepe_closed_form(P, x, z)
+    spec = [g.horfft(f, -1) for f in cf]
+    for q in range(4):
+        g.set_state(q, spec[q])
+    print("kepe closed form", name, flush=True)
+    got = g.kepe()
+    assert np.all(np.abs(got - want) <= 1e-12 * np.abs(want).max()), (got, want)
+    for q in range(4):
+        assert np.array_equal(g.get_state(q), spec[q])
+    g.close()
 
 
 def test_fortran_postp_and_speclegen_shims_against_reference(oracle_mod):

@@ -6,8 +6,8 @@ import numpy as np
 import pytest
 from numpy.polynomial import legendre as L
 
-from helpers import (SOLUTION, STATE, case, energy_close, g1_params, g2_params, g3_params, legendre_closed_form,
-                     meanwave_fields, postp_closed_form, rel, routine_inputs)
+from helpers import (SOLUTION, STATE, case, energy_close, g1_params, g2_params, g3_params, kepe_closed_form,
+                     legendre_closed_form, meanwave_fields, postp_closed_form, rel, routine_inputs)
 from iwrepo_amd.params import make_params
 
 # ---------------------------------------------------------------- GLL tables
@@ -566,3 +566,38 @@ def test_step_loop_diagnostics_match_compiled_reference(oracle_mod):
             a = r[2 + q * nz:2 + (q + 1) * nz].reshape(P.nzloc, P.nsubd)
             b = sl[(c, "%5.3f" % r[1])]
             assert np.all(np.abs(a - b) <= 5e-6 * np.abs(b) + 1e-10 * np.abs(b).max()), (r[0], c)
+
+
+@pytest.mark.parametrize("name", ["small", "c1", "g3"])
+def test_kepe_closed_form(oracle_mod, name):
+    """Kinetic and potential energy of fields whose integrals are known: the
+    rho0/2 and g^2/(2 rho0 N^2) factors, the span, the per-subdomain GLL
+    weights and zh/2 of integrcol (derv.f:479-512); g3 is 2-D with twelve
+    unequal subdomains."""
+    if name == "g3":
+        P, _ = g3_params()
+    else:
+        P, _ = case(name)
+    o = oracle_mod.Oracle(P)
+    z, x = o.table(7, P.nzloc * P.nsubd), o.table(8, P.nx + 2)
+    flds, want = kepe_closed_form(P, x, z)
+    for q in range(4):
+        o.set_field(q, o.horfft(flds[q], -1))
+    got = o.kepe()
+    assert np.all(np.abs(got - want) <= 1e-13 * np.abs(want).max())
+
+
+@pytest.mark.parametrize("name", ["small", "c1"])
+def test_kepe_matches_compiled_reference(oracle_mod, name):
+    """The same energies with the z integral done by the reference's
+    integrcol after its own set-up (harness mode 7)."""
+    if not oracle_mod.ref_available(name):
+        pytest.skip("reference build %s not present (needs /root/reference)" % name)
+    P, flds = case(name)
+    o = oracle_mod.Oracle(P)
+    spec = [o.horfft(f, -1) for f in flds]
+    phys = [o.horfft(a, 1) for a in spec]  # the physical fields the state holds
+    for q in range(4):
+        o.set_field(q, spec[q])
+    *_, kr = oracle_mod.run_ref(name, P, phys, nsteps=0, mode=7)
+    assert np.all(np.abs(o.kepe() - kr) <= 1e-14 * np.abs(kr))
