@@ -1249,58 +1249,50 @@ static int convec_dev(iw_solver *s, int ispecfilternl, int mcstep = 0, int mcslo
         launch_zero_meanw(g, B[2], s->mc_w00, s->stream);
     }
     const McFuse mf = mc_fuse(s, g.ndims == 3 ? (double)(g.ny - 1) : 0.0, s->mc_w00);
-    {
-        Scope sc(s, "dpdz", 48. * nt);
-        double *o3[3] = {gr[2], gr[5], gr[8]};
-        launch_dpdz3(g, uvwt, o3, s->stream);
-    }
     // the mean-wave / shear-current terms also need dtdx, dtdz
-    // (solver.f:74-103): dtdz spectral here, both transformed below
+    // (solver.f:74-103)
     const bool mean = mean_active(s);
-    if (mean) {
-        if (!s->gT[0]) {
-            for (int q = 0; q < 2; q++) HIPCHK(hipMalloc((void **)&s->gT[q], sizeof(double) * (size_t)g.ntot));
-            s->dev_bytes += 2 * (long long)sizeof(double) * g.ntot;
-        }
-        Scope sc(s, "dpdz", 16. * nt);
-        launch_dpdz(g, B[3], s->gT[1], s->stream);
+    if (mean && !s->gT[0]) {
+        for (int q = 0; q < 2; q++) HIPCHK(hipMalloc((void **)&s->gT[q], sizeof(double) * (size_t)g.ntot));
+        s->dev_bytes += 2 * (long long)sizeof(double) * g.ntot;
     }
     if (g.ndims == 3) {
         // spectral-to-physical (solver.f:107-123) with the x/y gradients of
         // u,v,w (solver.f:86-103) formed inside the transforms: the y pass
-        // turns u,v,w,temp (into the physical buffers) and the three d/dz
-        // fields, and emits i ky f of u,v,w; the x pass turns all ten and
-        // emits i kx f of u,v,w
-        FieldBatch by = batch({B[0], B[1], B[2], B[3], gr[2], gr[5], gr[8]});
-        if (mean) {
-            by.f[by.n] = by.g[by.n] = s->gT[1];
-            by.n++;
-        }
+        // turns u,v,w,temp into the physical buffers and emits i ky f of
+        // u,v,w; the x pass turns all seven and emits i kx f of u,v,w (and
+        // temp).  The z gradients are taken after the transforms, from the
+        // physical fields, together with the vertical fluxes below.
+        FieldBatch by = batch({B[0], B[1], B[2], B[3]});
         for (int q = 0; q < 4; q++) by.g[q] = s->phys[q];
         by.dy[0] = gr[1];
         by.dy[1] = gr[4];
         by.dy[2] = gr[7];
         by.kw = g.yw;
         {
-            Scope sc(s, "fft_y_inv", 32. * g.nxh * g.ny * g.nz * 10);
+            Scope sc(s, "fft_y_inv", 32. * g.nxh * g.ny * g.nz * 7);
             launch_fft_y(g, by, g.nz, 1, s->stream);
         }
-        FieldBatch bx = batch({s->phys[0], s->phys[1], s->phys[2], s->phys[3], gr[2], gr[5], gr[8], gr[1], gr[4],
-                               gr[7]});
+        FieldBatch bx = batch({s->phys[0], s->phys[1], s->phys[2], s->phys[3], gr[1], gr[4], gr[7]});
         bx.dx[0] = gr[0];
         bx.dx[1] = gr[3];
         bx.dx[2] = gr[6];
-        if (mean) {
-            bx.dx[3] = s->gT[0];
-            bx.f[bx.n] = bx.g[bx.n] = s->gT[1];
-            bx.n++;
-        }
+        if (mean) bx.dx[3] = s->gT[0];
         bx.kw = g.xw;
         {
-            Scope sc(s, "fft_x_inv", 16. * g.nxppy * g.nz * 13);
+            Scope sc(s, "fft_x_inv", 16. * g.nxppy * g.nz * (mean ? 11 : 10));
             launch_fft_x(g, bx, g.nz, 1, s->stream, mcstep > 0 ? &mf : nullptr);
         }
     } else {
+        {
+            Scope sc(s, "dpdz", 48. * nt);
+            double *o3[3] = {gr[2], gr[5], gr[8]};
+            launch_dpdz3(g, uvwt, o3, s->stream);
+        }
+        if (mean) {
+            Scope sc(s, "dpdz", 16. * nt);
+            launch_dpdz(g, B[3], s->gT[1], s->stream);
+        }
         {
             Scope sc(s, "grad", 72. * nt);
             launch_grad(g, uvwt, gr, s->stream);
@@ -1322,7 +1314,12 @@ static int convec_dev(iw_solver *s, int ispecfilternl, int mcstep = 0, int mcslo
         if (mc_after(s, mcslot)) return -1;
     }
     double *ph[4] = {s->phys[0], s->phys[1], s->phys[2], s->phys[3]};
-    {
+    if (g.ndims == 3) { // fluxes and z gradients: 4 fields in, 7 (or 8) out
+        Scope sc(s, "dpdz_flux", (mean ? 96. : 88.) * nt);
+        double *o4[4] = {S[0], S[1], S[2], s->ox};
+        double *dz4[4] = {gr[2], gr[5], gr[8], mean ? s->gT[1] : nullptr};
+        launch_dpdz_conv_grad(g, ph, o4, dz4, s->stream);
+    } else {
         Scope sc(s, "dpdz_flux", 64. * nt);
         double *o4[4] = {S[0], S[1], S[2], s->ox};
         launch_dpdz_conv(g, ph, o4, s->stream);

@@ -201,6 +201,10 @@ void launch_fft_y(const Geo &g, FieldBatch b, int nk, int dir, hipStream_t st);
 void launch_dpdz(const Geo &g, const double *a, double *b, hipStream_t st);
 void launch_dpdz_conv(const Geo &g, double *const *uvwt, double *const *out4, hipStream_t st);
 void launch_dpdz3(const Geo &g, double *const *in3, double *const *out3, hipStream_t st);
+// launch_dpdz_conv's four flux derivatives plus du/dz, dv/dz, dw/dz (and
+// drho/dz when dz4[3] is not null) of the same physical fields in one pass
+void launch_dpdz_conv_grad(const Geo &g, double *const *uvwt, double *const *out4, double *const *dz4,
+                           hipStream_t st);
 void launch_grad(const Geo &g, double *const *uvw, double *const *gr, hipStream_t st);
 // mean-wave and shear-current terms of rhsfm (solver.f:848-931); mw
 // nullptr: none

@@ -715,6 +715,89 @@ void launch_dpdz_conv(const Geo &g, double *const *uvwt, double *const *out4, hi
     run_dpdz(g, A, st);
 }
 
+// The vertical flux derivatives of rhsfm above and, from the same columns,
+// the z-gradients of u, v, w (and optionally rho) that solver.f:74-77 forms
+// in spectral space before the inverse transforms: dpdz acts along z and
+// horfft along x and y, so the two commute and the gradients are taken here
+// from the physical fields instead of being transformed.  One pass: every
+// input column is read once; the products are formed as in k_dpdz_t.
+struct ConvArgs {
+    const double *u, *v, *w, *t; // physical u, v, w, rho
+    double *fu, *fv, *fw, *ft;   // d(-uw/2)/dz, d(-vw/2)/dz, d(-ww/2)/dz, d(-w rho)/dz
+    double *du, *dv, *dw, *dt;   // du/dz, dv/dz, dw/dz, drho/dz (dt may be null)
+};
+
+// (2/zh) D x of one column
+template <int NL>
+__device__ __forceinline__ void dz_col(const double *Ds, const double (&x)[NL], double *o, long nxppy, double fac)
+{
+#pragma unroll 1
+    for (int k1 = 0; k1 < NL; k1++) {
+        double sm = 0.0;
+#pragma unroll
+        for (int t = 0; t < NL; t++) sm = sm + Ds[k1 * NL + t] * x[t];
+        o[k1 * nxppy] = sm * fac;
+    }
+}
+
+template <int NL>
+__global__ __launch_bounds__(256) void k_dpdz_conv_t(ConvArgs A, long nxppy, const double *__restrict__ D,
+                                                     const double *__restrict__ zh)
+{
+    __shared__ double Ds[NL * NL];
+    for (int e = threadIdx.x; e < NL * NL; e += blockDim.x) Ds[e] = D[e];
+    __syncthreads();
+    const long ij = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (ij >= nxppy) return;
+    const int ks = blockIdx.y;
+    const double fac = 2. / zh[ks];
+    const long b0 = (long)ks * NL * nxppy + ij;
+    // each column: its own gradient, then in place the product with w
+    // ((c x) w, which for rho equals k_dpdz_t's (-w) rho exactly) and its
+    // gradient
+    double w[NL], c[NL];
+#pragma unroll
+    for (int t = 0; t < NL; t++) w[t] = A.w[b0 + t * nxppy];
+    dz_col<NL>(Ds, w, A.dw + b0, nxppy, fac);
+#pragma unroll
+    for (int t = 0; t < NL; t++) c[t] = (-0.5 * w[t]) * w[t];
+    dz_col<NL>(Ds, c, A.fw + b0, nxppy, fac);
+    const double *src[3] = {A.u, A.v, A.t};
+    double *dsts[3] = {A.du, A.dv, A.dt}, *fl[3] = {A.fu, A.fv, A.ft};
+#pragma unroll 1
+    for (int q = 0; q < 3; q++) {
+        const double cf = q < 2 ? -0.5 : -1.0;
+#pragma unroll
+        for (int t = 0; t < NL; t++) c[t] = src[q][b0 + t * nxppy];
+        if (dsts[q]) dz_col<NL>(Ds, c, dsts[q] + b0, nxppy, fac);
+#pragma unroll
+        for (int t = 0; t < NL; t++) c[t] = (cf * c[t]) * w[t];
+        dz_col<NL>(Ds, c, fl[q] + b0, nxppy, fac);
+    }
+}
+
+void launch_dpdz_conv_grad(const Geo &g, double *const *uvwt, double *const *out4, double *const *dz4,
+                           hipStream_t st)
+{
+    ConvArgs A;
+    A.u = uvwt[0]; A.v = uvwt[1]; A.w = uvwt[2]; A.t = uvwt[3];
+    A.fu = out4[0]; A.fv = out4[1]; A.fw = out4[2]; A.ft = out4[3];
+    A.du = dz4[0]; A.dv = dz4[1]; A.dw = dz4[2]; A.dt = dz4[3];
+    dim3 grid((unsigned)((g.nxppy + 255) / 256), g.nsubd);
+    switch (g.nzloc) {
+    case 9: hipLaunchKernelGGL(k_dpdz_conv_t<9>, grid, dim3(256), 0, st, A, g.nxppy, g.D, g.zh); break;
+    case 13: hipLaunchKernelGGL(k_dpdz_conv_t<13>, grid, dim3(256), 0, st, A, g.nxppy, g.D, g.zh); break;
+    case 17: hipLaunchKernelGGL(k_dpdz_conv_t<17>, grid, dim3(256), 0, st, A, g.nxppy, g.D, g.zh); break;
+    case 21: hipLaunchKernelGGL(k_dpdz_conv_t<21>, grid, dim3(256), 0, st, A, g.nxppy, g.D, g.zh); break;
+    case 25: hipLaunchKernelGGL(k_dpdz_conv_t<25>, grid, dim3(256), 0, st, A, g.nxppy, g.D, g.zh); break;
+    default: { // other orders: the products, then the plain gradients
+        launch_dpdz_conv(g, uvwt, out4, st);
+        launch_dpdz3(g, uvwt, dz4, st);
+        if (dz4[3]) launch_dpdz(g, uvwt[3], dz4[3], st);
+    }
+    }
+}
+
 // mean w = 0 (solver.f:66-69)
 // w(kx=0,ky=0) = 0 (solver.f:66-69); the removed real part is kept in w00
 // (maxcalc's values are recovered from convec's transform with it)
