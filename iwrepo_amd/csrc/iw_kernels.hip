@@ -18,9 +18,9 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 b)
 }
 
 // ---------------------------------------------------------------------------
-// In-place multi-radix Stockham FFT on LDS.  A block holds kFE = 2048 complex
-// values (nrows sequences of length N, N a power of two), so every pass of
-// radix R has exactly 8/R butterflies per thread of a 256-thread block.
+// In-place multi-radix Stockham FFT on LDS.  A block of kFT threads holds
+// kFE = 8 kFT complex values (nrows sequences of length N, N a power of two),
+// so every pass of radix R has exactly 8/R butterflies per thread.
 // Passes are radix 8 while at least three factors of two remain, then one
 // radix-4 or radix-2 pass.  Each pass reads its butterflies into registers,
 // synchronises, and writes the outputs to their autosort positions.  Row
@@ -29,7 +29,11 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 b)
 // aligned runs of 8 or 16 values hit distinct 16-byte bank groups.
 // sign -1: sum g_j e^{-2 pi i jk/N}; +1: e^{+...}.  tw[t] = e^{-2 pi i t/N}.
 // ---------------------------------------------------------------------------
-constexpr int kFE = 2048;
+#ifndef IW_FFT_THREADS
+#define IW_FFT_THREADS 256
+#endif
+constexpr int kFT = IW_FFT_THREADS;
+constexpr int kFE = 8 * kFT;
 
 __host__ __device__ __forceinline__ int lp(int i) { return i ^ ((i >> 3) & 15); }
 
@@ -91,7 +95,7 @@ __device__ __forceinline__ void lds_pass(double2 *buf, int logN, int logNs, int 
     int base[BPT], rr[BPT];
 #pragma unroll
     for (int i = 0; i < BPT; i++) {
-        const int b = threadIdx.x + i * 256;
+        const int b = threadIdx.x + i * kFT;
         base[i] = -1;
         rr[i] = 0;
         if (b < nb) {
@@ -205,7 +209,7 @@ __device__ __forceinline__ void mc_take(bool mn, double x, int lx, double &b, in
 }
 
 template <int DIR>
-__global__ __launch_bounds__(256) void k_fft_x(FieldBatch fb, int nrows, int ny, int nxpp, int M, int logM,
+__global__ __launch_bounds__(kFT) void k_fft_x(FieldBatch fb, int nrows, int ny, int nxpp, int M, int logM,
                                                int R, const double2 *__restrict__ twM,
                                                const double2 *__restrict__ twN, double scaley, McFuse mf)
 {
@@ -234,8 +238,8 @@ __global__ __launch_bounds__(256) void k_fft_x(FieldBatch fb, int nrows, int ny,
     // solver.f:86-103)
     auto load_inverse = [&](bool deriv) {
 #pragma unroll
-        for (int it = 0; it < kFE / 256; it++) {
-            const int idx = threadIdx.x + it * 256;
+        for (int it = 0; it < kFE / kFT; it++) {
+            const int idx = threadIdx.x + it * kFT;
             int r = idx >> logM, k = idx & (M - 1), row = row0 + r;
             double2 Z = make_double2(0., 0.);
             if (row < nrows) {
@@ -267,8 +271,8 @@ __global__ __launch_bounds__(256) void k_fft_x(FieldBatch fb, int nrows, int ny,
     };
     auto store_inverse = [&](double *ob) {
 #pragma unroll
-        for (int it = 0; it < kFE / 256; it++) {
-            const int idx = threadIdx.x + it * 256;
+        for (int it = 0; it < kFE / kFT; it++) {
+            const int idx = threadIdx.x + it * kFT;
             int r = idx >> logM, m = idx & (M - 1), row = row0 + r;
             if (row >= nrows) continue;
             *reinterpret_cast<double2 *>(ob + drow(row) + 2 * m) = buf[lp(idx)];
@@ -291,8 +295,8 @@ __global__ __launch_bounds__(256) void k_fft_x(FieldBatch fb, int nrows, int ny,
         const double sc = 1.0 / (double)N;
         auto load_fwd = [&](const double *src) {
 #pragma unroll
-            for (int it = 0; it < kFE / 256; it++) {
-                const int idx = threadIdx.x + it * 256;
+            for (int it = 0; it < kFE / kFT; it++) {
+                const int idx = threadIdx.x + it * kFT;
                 int r = idx >> logM, m = idx & (M - 1), row = row0 + r;
                 double2 z = make_double2(0., 0.);
                 if (row < nrows) z = *reinterpret_cast<const double2 *>(src + srow(row) + 2 * m);
@@ -316,15 +320,15 @@ __global__ __launch_bounds__(256) void k_fft_x(FieldBatch fb, int nrows, int ny,
             return make_double2(A, B);
         };
         const double *acc = fb.ax[blockIdx.y];
-        double2 pa[kFE / 256];
+        double2 pa[kFE / kFT];
         if (acc) { // transform of the flux product first: coefficients k < M
                    // kept in registers, k = M of each row in LDS
             load_fwd(acc);
             __syncthreads();
             fft_lds<false>(buf, logM, R, tw, DIR);
 #pragma unroll
-            for (int it = 0; it < kFE / 256; it++) {
-                const int idx = threadIdx.x + it * 256;
+            for (int it = 0; it < kFE / kFT; it++) {
+                const int idx = threadIdx.x + it * kFT;
                 const int r = idx >> logM, k = idx & (M - 1);
                 pa[it] = coef(r, k);
                 if (k == 0) nyq[r] = coef(r, M);
@@ -340,8 +344,8 @@ __global__ __launch_bounds__(256) void k_fft_x(FieldBatch fb, int nrows, int ny,
             return make_double2(c.x + (-kx * q.y), c.y + kx * q.x);
         };
 #pragma unroll
-        for (int it = 0; it < kFE / 256; it++) {
-            const int idx = threadIdx.x + it * 256;
+        for (int it = 0; it < kFE / kFT; it++) {
+            const int idx = threadIdx.x + it * kFT;
             int r = idx >> logM, k = idx & (M - 1), row = row0 + r;
             if (row >= nrows) continue;
             double *orow = obase + drow(row);
@@ -365,8 +369,8 @@ __global__ __launch_bounds__(256) void k_fft_x(FieldBatch fb, int nrows, int ny,
             double b0 = big, b1 = -big, b2 = -big; // min, max of the value; max of the CFL number
             int l0 = -1, l1 = -1, l2 = -1;
 #pragma unroll
-            for (int it = 0; it < kFE / 256; it++) {
-                const int idx = threadIdx.x + it * 256;
+            for (int it = 0; it < kFE / kFT; it++) {
+                const int idx = threadIdx.x + it * kFT;
                 const int r = idx >> logM, m = idx & (M - 1), row = row0 + r;
                 if (row >= nrows) continue;
                 const int k = row >> lny, j = row & (ny - 1);
@@ -410,8 +414,8 @@ __global__ __launch_bounds__(256) void k_fft_x(FieldBatch fb, int nrows, int ny,
                 x = __shfl_down(b1, off, 64); lx = __shfl_down(l1, off, 64); mc_take(false, x, lx, b1, l1);
                 x = __shfl_down(b2, off, 64); lx = __shfl_down(l2, off, 64); mc_take(false, x, lx, b2, l2);
             }
-            __shared__ double rv[4][3];
-            __shared__ int rl[4][3];
+            __shared__ double rv[kFT / 64][3];
+            __shared__ int rl[kFT / 64][3];
             const int wv = threadIdx.x >> 6;
             if ((threadIdx.x & 63) == 0) {
                 rv[wv][0] = b0; rv[wv][1] = b1; rv[wv][2] = b2;
@@ -419,7 +423,7 @@ __global__ __launch_bounds__(256) void k_fft_x(FieldBatch fb, int nrows, int ny,
             }
             __syncthreads();
             if (threadIdx.x == 0) {
-                for (int w = 1; w < 4; w++) {
+                for (int w = 1; w < kFT / 64; w++) {
                     mc_take(true, rv[w][0], rl[w][0], b0, l0);
                     mc_take(false, rv[w][1], rl[w][1], b1, l1);
                     mc_take(false, rv[w][2], rl[w][2], b2, l2);
@@ -446,7 +450,7 @@ __global__ __launch_bounds__(256) void k_fft_x(FieldBatch fb, int nrows, int ny,
 // scaled by 1/ny (horfft.f:28-30).  CB columns per block staged in LDS.
 // ---------------------------------------------------------------------------
 template <int DIR>
-__global__ __launch_bounds__(256) void k_fft_y(FieldBatch fb, int nxpp, int nxh, int ny, int logNy,
+__global__ __launch_bounds__(kFT) void k_fft_y(FieldBatch fb, int nxpp, int nxh, int ny, int logNy,
                                                int CB, const double2 *__restrict__ twY, double scaley)
 {
     extern __shared__ double2 smy[];
@@ -463,8 +467,8 @@ __global__ __launch_bounds__(256) void k_fft_y(FieldBatch fb, int nxpp, int nxh,
     // (the y gradient of solver.f:86-103)
     auto load = [&](const double *src, bool deriv) {
 #pragma unroll
-        for (int it = 0; it < kFE / 256; it++) {
-            const int idx = threadIdx.x + it * 256;
+        for (int it = 0; it < kFE / kFT; it++) {
+            const int idx = threadIdx.x + it * kFT;
             if (idx >= CB * ny) break;
             int j = idx >> lcb, c = idx & (CB - 1), ii = ii0 + c;
             double2 z = make_double2(0., 0.);
@@ -481,11 +485,11 @@ __global__ __launch_bounds__(256) void k_fft_y(FieldBatch fb, int nxpp, int nxh,
     const double *accp = (DIR < 0 && fb.ay[blockIdx.z])
                              ? fb.ay[blockIdx.z] + (long)(fb.smap ? fb.smap[lpl] : lpl) * nxpp * ny
                              : nullptr;
-    double2 pb[kFE / 256];
+    double2 pb[kFE / kFT];
     auto store = [&](double *op, bool add) {
 #pragma unroll
-        for (int it = 0; it < kFE / 256; it++) {
-            const int idx = threadIdx.x + it * 256;
+        for (int it = 0; it < kFE / kFT; it++) {
+            const int idx = threadIdx.x + it * kFT;
             if (idx >= CB * ny) break;
             int j = idx >> lcb, c = idx & (CB - 1), ii = ii0 + c;
             if (ii >= nxh) continue;
@@ -515,8 +519,8 @@ __global__ __launch_bounds__(256) void k_fft_y(FieldBatch fb, int nxpp, int nxh,
         __syncthreads();
         fft_lds<true>(buf, logNy, CB, tw, DIR);
 #pragma unroll
-        for (int it = 0; it < kFE / 256; it++) {
-            const int idx = threadIdx.x + it * 256;
+        for (int it = 0; it < kFE / kFT; it++) {
+            const int idx = threadIdx.x + it * kFT;
             double2 z = make_double2(0., 0.);
             if (idx < CB * ny) {
                 z = buf[((idx >> lcb) << lcb) + (idx & (CB - 1))];
@@ -565,10 +569,10 @@ void launch_fft_x(const Geo &g, FieldBatch b, int nk, int dir, hipStream_t st, c
     size_t sh = sizeof(double2) * ((size_t)R * g.M + g.M + R);
     McFuse none;
     if (dir < 0)
-        hipLaunchKernelGGL(k_fft_x<-1>, grid, dim3(256), sh, st, b, nrows, g.ny, g.nxpp, g.M, g.logM, R,
+        hipLaunchKernelGGL(k_fft_x<-1>, grid, dim3(kFT), sh, st, b, nrows, g.ny, g.nxpp, g.M, g.logM, R,
                            g.twM, g.twN, g.scaley, none);
     else
-        hipLaunchKernelGGL(k_fft_x<1>, grid, dim3(256), sh, st, b, nrows, g.ny, g.nxpp, g.M, g.logM, R,
+        hipLaunchKernelGGL(k_fft_x<1>, grid, dim3(kFT), sh, st, b, nrows, g.ny, g.nxpp, g.M, g.logM, R,
                            g.twM, g.twN, g.scaley, mf ? *mf : none);
 }
 
@@ -582,10 +586,10 @@ void launch_fft_y(const Geo &g, FieldBatch b, int nk, int dir, hipStream_t st)
     dim3 grid((g.nxh + CB - 1) / CB, nk, b.n);
     size_t sh = sizeof(double2) * ((size_t)CB * g.ny + g.ny);
     if (dir < 0)
-        hipLaunchKernelGGL(k_fft_y<-1>, grid, dim3(256), sh, st, b, g.nxpp, g.nxh, g.ny, g.logNy, CB,
+        hipLaunchKernelGGL(k_fft_y<-1>, grid, dim3(kFT), sh, st, b, g.nxpp, g.nxh, g.ny, g.logNy, CB,
                            g.twY, g.scaley);
     else
-        hipLaunchKernelGGL(k_fft_y<1>, grid, dim3(256), sh, st, b, g.nxpp, g.nxh, g.ny, g.logNy, CB,
+        hipLaunchKernelGGL(k_fft_y<1>, grid, dim3(kFT), sh, st, b, g.nxpp, g.nxh, g.ny, g.logNy, CB,
                            g.twY, g.scaley);
 }
 
