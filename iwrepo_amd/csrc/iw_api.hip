@@ -1392,15 +1392,16 @@ static int convec_dev(iw_solver *s, int ispecfilternl, int mcstep = 0, int mcslo
         // transforms: the x pass turns S_u,S_v,S_w,S_T adding i kx of
         // P0,P1,P3,P5 and writes the x transforms of P1,P2,P4,P6 to free
         // gradient buffers; the y pass turns S_u.. adding i ky of those
-        FieldBatch bx = batch({S[0], S[1], S[2], S[3], pr[1], pr[2], pr[4], pr[6]});
+        // P1 is both S_v's x flux (ax) and S_u's y flux: its transform in
+        // S_v's blocks is stored for the y pass as well (axo)
+        FieldBatch bx = batch({S[0], S[1], S[2], S[3], pr[2], pr[4], pr[6]});
         const int ia[4] = {0, 1, 3, 5};
-        for (int q = 0; q < 4; q++) {
-            bx.ax[q] = pr[ia[q]];
-            bx.g[4 + q] = gr[q];
-        }
+        for (int q = 0; q < 4; q++) bx.ax[q] = pr[ia[q]];
+        bx.axo[1] = gr[0];
+        for (int q = 1; q < 4; q++) bx.g[3 + q] = gr[q];
         bx.kw = g.xw;
         {
-            Scope sc(s, "fft_x_fwd", 16. * g.nxppy * g.nz * 10);
+            Scope sc(s, "fft_x_fwd", 16. * g.nxppy * g.nz * 9.5);
             launch_fft_x(g, bx, g.nz, -1, s->stream);
         }
         FieldBatch by = batch({S[0], S[1], S[2], S[3]});

@@ -27,6 +27,9 @@ struct FieldBatch {
     // horizontal flux terms of rhsfm); read with the plane map of f
     const double *ax[kMaxFields] = {};
     const double *ay[kMaxFields] = {};
+    // forward x pass only: where to store the plain transform of ax[q] as
+    // well (the y pass needs it for another field), or null
+    double *axo[kMaxFields] = {};
     const double *kw = nullptr; // ky (y pass) or kx (x pass) per row / column
     int n = 0;
     const int *smap = nullptr, *dmap = nullptr;

@@ -326,12 +326,18 @@ __global__ __launch_bounds__(kFT) void k_fft_x(FieldBatch fb, int nrows, int ny,
             load_fwd(acc);
             __syncthreads();
             fft_lds<false>(buf, logM, R, tw, DIR);
+            double *axo = fb.axo[blockIdx.y];
 #pragma unroll
             for (int it = 0; it < kFE / kFT; it++) {
                 const int idx = threadIdx.x + it * kFT;
-                const int r = idx >> logM, k = idx & (M - 1);
+                const int r = idx >> logM, k = idx & (M - 1), row = row0 + r;
                 pa[it] = coef(r, k);
                 if (k == 0) nyq[r] = coef(r, M);
+                if (axo && row < nrows) { // the plain transform of P, stored as well
+                    double *orow = axo + drow(row);
+                    *reinterpret_cast<double2 *>(orow + 2 * k) = pa[it];
+                    if (k == 0) *reinterpret_cast<double2 *>(orow + 2 * M) = nyq[r];
+                }
             }
             __syncthreads();
         }
