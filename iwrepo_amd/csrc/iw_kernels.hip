@@ -208,7 +208,7 @@ __device__ __forceinline__ void mc_take(bool mn, double x, int lx, double &b, in
     }
 }
 
-template <int DIR>
+template <int DIR, bool AX = false> // AX: the batch's fields have flux products (forward only)
 __global__ __launch_bounds__(kFT) void k_fft_x(FieldBatch fb, int nrows, int ny, int nxpp, int M, int logM,
                                                int R, const double2 *__restrict__ twM,
                                                const double2 *__restrict__ twN, double scaley, McFuse mf)
@@ -216,7 +216,7 @@ __global__ __launch_bounds__(kFT) void k_fft_x(FieldBatch fb, int nrows, int ny,
     extern __shared__ double2 smx[];
     double2 *buf = smx;
     double2 *tw = smx + R * M;
-    double2 *nyq = tw + M; // R entries: k = M term of each row (forward)
+    double2 *nyq = tw + M; // R entries: the kx-Nyquist coefficient of each row (inverse, for maxcalc)
     const double *base = fb.f[blockIdx.y];
     double *obase = fb.g[blockIdx.y];
     const int row0 = blockIdx.x * R;
@@ -319,49 +319,57 @@ __global__ __launch_bounds__(kFT) void k_fft_x(FieldBatch fb, int nrows, int ny,
             if (k == M) A = A * scaley; // horfft.f:28-30 on the untransformed Nyquist column
             return make_double2(A, B);
         };
-        const double *acc = fb.ax[blockIdx.y];
-        double2 pa[kFE / kFT];
-        if (acc) { // transform of the flux product first: coefficients k < M
-                   // kept in registers, k = M of each row in LDS
-            load_fwd(acc);
-            __syncthreads();
-            fft_lds<false>(buf, logM, R, tw, DIR);
-            double *axo = fb.axo[blockIdx.y];
-#pragma unroll
-            for (int it = 0; it < kFE / kFT; it++) {
-                const int idx = threadIdx.x + it * kFT;
-                const int r = idx >> logM, k = idx & (M - 1), row = row0 + r;
-                pa[it] = coef(r, k);
-                if (k == 0) nyq[r] = coef(r, M);
-                if (axo && row < nrows) { // the plain transform of P, stored as well
-                    double *orow = axo + drow(row);
-                    *reinterpret_cast<double2 *>(orow + 2 * k) = pa[it];
-                    if (k == 0) *reinterpret_cast<double2 *>(orow + 2 * M) = nyq[r];
-                }
-            }
-            __syncthreads();
-        }
-        load_fwd(base);
-        __syncthreads();
-        fft_lds<false>(buf, logM, R, tw, DIR);
         // + i kx P (the x part of the flux divergence, solver.f:985-1063, 1163-1196)
         auto addk = [&](double2 c, double2 q, int k) {
             const double kx = fb.kw[k];
             return make_double2(c.x + (-kx * q.y), c.y + kx * q.x);
         };
+        if (AX) {
+            // the tile holds H rows of the field and the same H rows of its
+            // flux product P: both are transformed by the same passes and
+            // combined from LDS
+            const int H = R >> 1, rowA = blockIdx.x * H;
+            const double *acc = fb.ax[blockIdx.y];
+            double *axo = fb.axo[blockIdx.y];
 #pragma unroll
-        for (int it = 0; it < kFE / kFT; it++) {
-            const int idx = threadIdx.x + it * kFT;
-            int r = idx >> logM, k = idx & (M - 1), row = row0 + r;
-            if (row >= nrows) continue;
-            double *orow = obase + drow(row);
-            double2 c = coef(r, k);
-            if (acc) c = addk(c, pa[it], k);
-            *reinterpret_cast<double2 *>(orow + 2 * k) = c;
-            if (k == 0) {
-                double2 cm = coef(r, M);
-                if (acc) cm = addk(cm, nyq[r], M);
-                *reinterpret_cast<double2 *>(orow + 2 * M) = cm;
+            for (int it = 0; it < kFE / kFT; it++) {
+                const int idx = threadIdx.x + it * kFT;
+                const int r = idx >> logM, m = idx & (M - 1);
+                const bool isp = r >= H;
+                const int row = rowA + (isp ? r - H : r);
+                double2 z = make_double2(0., 0.);
+                if (row < nrows) z = *reinterpret_cast<const double2 *>((isp ? acc : base) + srow(row) + 2 * m);
+                buf[lp(idx)] = z;
+            }
+            __syncthreads();
+            fft_lds<false>(buf, logM, R, tw, DIR);
+#pragma unroll
+            for (int it = 0; it < kFE / kFT; it++) {
+                const int idx = threadIdx.x + it * kFT;
+                const int r = idx >> logM, k = idx & (M - 1), row = rowA + r;
+                if (r >= H || row >= nrows) continue;
+                double *orow = obase + drow(row);
+                const double2 q = coef(r + H, k);
+                *reinterpret_cast<double2 *>(orow + 2 * k) = addk(coef(r, k), q, k);
+                if (axo) *reinterpret_cast<double2 *>(axo + drow(row) + 2 * k) = q; // P's own transform
+                if (k == 0) {
+                    const double2 qm = coef(r + H, M);
+                    *reinterpret_cast<double2 *>(orow + 2 * M) = addk(coef(r, M), qm, M);
+                    if (axo) *reinterpret_cast<double2 *>(axo + drow(row) + 2 * M) = qm;
+                }
+            }
+        } else {
+            load_fwd(base);
+            __syncthreads();
+            fft_lds<false>(buf, logM, R, tw, DIR);
+#pragma unroll
+            for (int it = 0; it < kFE / kFT; it++) {
+                const int idx = threadIdx.x + it * kFT;
+                int r = idx >> logM, k = idx & (M - 1), row = row0 + r;
+                if (row >= nrows) continue;
+                double *orow = obase + drow(row);
+                *reinterpret_cast<double2 *>(orow + 2 * k) = coef(r, k);
+                if (k == 0) *reinterpret_cast<double2 *>(orow + 2 * M) = coef(r, M);
             }
         }
     } else {
@@ -574,10 +582,30 @@ void launch_fft_x(const Geo &g, FieldBatch b, int nk, int dir, hipStream_t st, c
     dim3 grid((nrows + R - 1) / R, b.n);
     size_t sh = sizeof(double2) * ((size_t)R * g.M + g.M + R);
     McFuse none;
-    if (dir < 0)
-        hipLaunchKernelGGL(k_fft_x<-1>, grid, dim3(kFT), sh, st, b, nrows, g.ny, g.nxpp, g.M, g.logM, R,
-                           g.twM, g.twN, g.scaley, none);
-    else
+    if (dir < 0) {
+        // fields with a flux product hold its coefficients in registers (more
+        // VGPRs): they get their own launch so the others run at a higher
+        // occupancy
+        FieldBatch ba = b, bp = b;
+        ba.n = bp.n = 0;
+        for (int q = 0; q < b.n; q++) {
+            FieldBatch &t = b.ax[q] ? ba : bp;
+            t.f[t.n] = b.f[q];
+            t.g[t.n] = b.g[q];
+            t.ax[t.n] = b.ax[q];
+            t.axo[t.n] = b.axo[q];
+            t.ay[t.n] = b.ay[q];
+            t.dx[t.n] = b.dx[q];
+            t.dy[t.n] = b.dy[q];
+            t.n++;
+        }
+        if (ba.n)
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fft_x<-1, true>), dim3((nrows + R / 2 - 1) / (R / 2), ba.n), dim3(kFT), sh, st, ba, nrows,
+                               g.ny, g.nxpp, g.M, g.logM, R, g.twM, g.twN, g.scaley, none);
+        if (bp.n)
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fft_x<-1, false>), dim3(grid.x, bp.n), dim3(kFT), sh, st, bp,
+                               nrows, g.ny, g.nxpp, g.M, g.logM, R, g.twM, g.twN, g.scaley, none);
+    } else
         hipLaunchKernelGGL(k_fft_x<1>, grid, dim3(kFT), sh, st, b, nrows, g.ny, g.nxpp, g.M, g.logM, R,
                            g.twM, g.twN, g.scaley, mf ? *mf : none);
 }
