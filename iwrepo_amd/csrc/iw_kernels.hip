@@ -1373,18 +1373,21 @@ void launch_filter3d(const Geo &g, double *f, int ifourier, int ilegen, hipStrea
 }
 
 // ---------------------------------------------------------------------------
-// interface_avg3d physical-space pass (penalty.f:729-772): one thread per
-// (i,j) point and interface; counts by wavefront ballot + one atomic.
+// interface_avg3d physical-space pass (penalty.f:729-772): grid-stride
+// over the (i,j) points of every interface; the counts are summed per block
+// and added with one atomic per block (per-wavefront atomics on the four
+// counters serialised).
 // ---------------------------------------------------------------------------
 // The field holds, per interface, the four physical planes km-1, km, k, k+1
 // (compact: interface i at planes 4i..4i+3).
 __global__ __launch_bounds__(256) void k_iface(FieldBatch fb, int *count, long nxppy, int nl, int nint)
 {
-    long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ int ws[4];
     double *f = fb.f[blockIdx.y];
-    bool hit = false;
+    int hits = 0;
     (void)nl;
-    if (id < nxppy * nint) {
+    const long n = nxppy * nint;
+    for (long id = (long)blockIdx.x * blockDim.x + threadIdx.x; id < n; id += (long)gridDim.x * blockDim.x) {
         int it = (int)(id / nxppy);
         long ij = id % nxppy;
         long k = 4L * it + 2, km = 4L * it + 1;
@@ -1396,11 +1399,16 @@ __global__ __launch_bounds__(256) void k_iface(FieldBatch fb, int *count, long n
                         + f[(k + 1) * nxppy + ij] * 0.5;
             f[km * nxppy + ij] = ff;
             f[k * nxppy + ij] = ff;
-            hit = true;
+            hits++;
         }
     }
-    unsigned long long b = __ballot(hit);
-    if ((threadIdx.x & 63) == 0 && b) atomicAdd(&count[blockIdx.y], __popcll(b));
+    for (int off = 32; off > 0; off >>= 1) hits += __shfl_down(hits, off, 64);
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = hits;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int t = ws[0] + ws[1] + ws[2] + ws[3];
+        if (t) atomicAdd(&count[blockIdx.y], t);
+    }
 }
 
 void launch_iface(const Geo &g, FieldBatch b, int *count, hipStream_t st)
@@ -1408,7 +1416,9 @@ void launch_iface(const Geo &g, FieldBatch b, int *count, hipStream_t st)
     int nint = g.nsubd - 1;
     if (nint <= 0) return;
     long n = g.nxppy * nint;
-    dim3 grid((unsigned)((n + 255) / 256), b.n);
+    long nb = (n + 255) / 256;
+    if (nb > 512) nb = 512;
+    dim3 grid((unsigned)nb, b.n);
     hipLaunchKernelGGL(k_iface, grid, dim3(256), 0, st, b, count, g.nxppy, g.nzloc, nint);
 }
 
