@@ -812,6 +812,20 @@ extern "C" int iw_create(const iw_params *p, iw_solver **out)
     std::vector<double> sig(P.nzloc);
     for (int k = 0; k <= nzm; k++)
         sig[k] = (k >= kc) ? exp(-P.amp * pow((double)(k - kc) / (double)(nzm - kc), 2. * P.p)) : 1.;
+    // the same filter as one matrix: out[q] = sum_t fl[t nl + q] x[t] with
+    // fl = w_t sum_k P_k(x_t) c_k sig_k P_k(x_q), c_k = k + 1/2 (nzm/2 for
+    // k = nzm), the two sums of filtering.f:912-955 taken together
+    const int nl = P.nzloc;
+    std::vector<double> fl((size_t)nl * nl, 0.);
+    for (int t = 0; t < nl; t++)
+        for (int q = 0; q < nl; q++) {
+            double sum = 0.;
+            for (int k = 0; k < nl; k++) {
+                const double c = (k == nzm) ? 0.5 * nzm : (k + 0.5);
+                sum += s->alpol[k * nl + t] * c * sig[k] * s->alpol[k * nl + q];
+            }
+            fl[(size_t)t * nl + q] = s->wg[t] * sum;
+        }
     std::vector<double> fcf((size_t)g.nxhp * g.ny);
     double akmax = sqrt(s->xsq[g.nxhp - 1] + s->ysq[g.nyh]);
     for (int j = 0; j < g.ny; j++)
@@ -860,7 +874,7 @@ extern "C" int iw_create(const iw_params *p, iw_solver **out)
         || upload(&g.wg, s->wg, acc) || upload(&g.sig, sig, acc) || upload(&g.xw, s->xw, acc)
         || upload(&g.yw, s->yw, acc) || upload(&g.zh, zh, acc) || upload(&g.z, s->z, acc)
         || upload(&g.damp, damp, acc) || upload(&g.fcf, fcf, acc) || upload(&g.x, xg, acc)
-        || upload(&g.sigbv, sigbv, acc)
+        || upload(&g.sigbv, sigbv, acc) || upload(&g.FL, fl, acc)
         || upload(&g.twM, twM, acc) || upload(&g.twN, twN, acc) || upload(&g.twY, twY, acc)) {
         delete s;
         return -1;
@@ -970,7 +984,7 @@ extern "C" int iw_destroy(iw_solver *s)
     if (!s) return 0;
     hipStreamSynchronize(s->stream);
     Geo &g = s->g;
-    double *tabs[] = {g.D, g.D2, g.AL, g.wg, g.sig, g.xw, g.yw, g.zh, g.z, g.damp, g.fcf, g.x, g.sigbv};
+    double *tabs[] = {g.D, g.D2, g.AL, g.wg, g.sig, g.xw, g.yw, g.zh, g.z, g.damp, g.fcf, g.x, g.sigbv, g.FL};
     for (int q = 0; q < 9; q++) hipFree(s->sgs[q]);
     hipFree(s->tmean);
     for (double *t : tabs) hipFree(t);

@@ -44,6 +44,7 @@ struct Geo {
     // device tables
     double *D, *D2, *AL, *wg, *sig, *xw, *yw, *zh, *z, *damp, *fcf, *x;
     double *sigbv; // Boyd-Vandeven filter of the estimation model (filtering.f:1272-1310)
+    double *FL;    // the Legendre filter as one nl x nl matrix, out[q] = sum_t FL[t nl + q] x[t]
     double2 *twM;   // e^{-2 pi i t/M}, t < M/2
     double2 *twN;   // e^{-2 pi i k/nx}, k <= M
     double2 *twY;   // e^{-2 pi i t/ny}, t < ny/2

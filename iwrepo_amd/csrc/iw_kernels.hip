@@ -1254,11 +1254,11 @@ __global__ __launch_bounds__(256) void k_tile(TileArgs A, long ncplx_plane, long
 // row), the Fourier filter with one sincos, the Legendre tables in LDS.
 template <int NL, int MODE>
 __global__ __launch_bounds__(256, 2) void k_tile_t(TileArgs A, long nmode, long nxppy,
-                                                const double *__restrict__ fcf, const double *__restrict__ wg,
-                                                const double *__restrict__ AL, const double *__restrict__ sig)
+                                                const double *__restrict__ fcf, const double *__restrict__ FL)
 {
-    __shared__ double ALs[NL * NL], wgs[NL], sgs[NL];
-    stage_tables<NL>(wgs, ALs, sgs, wg, AL, sig);
+    __shared__ double FLs[NL * NL];
+    for (int e = threadIdx.x; e < NL * NL; e += blockDim.x) FLs[e] = FL[e];
+    __syncthreads();
     const long m = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const int ks = blockIdx.y, fz = blockIdx.z;
     if (m >= nmode) return;
@@ -1309,9 +1309,22 @@ __global__ __launch_bounds__(256, 2) void k_tile_t(TileArgs A, long nmode, long 
             xi[t] = gi;
         }
     }
-    if (A.ilegen) { // filtering.f:905-956
-        legfilter<NL>(xr, wgs, ALs, sgs);
-        legfilter<NL>(xi, wgs, ALs, sgs);
+    if (A.ilegen) { // filtering.f:905-956, the two sums as one matrix
+        auto apply = [&](double(&x)[NL]) __attribute__((always_inline)) {
+            double o[NL];
+#pragma unroll
+            for (int q = 0; q < NL; q++) o[q] = 0.;
+#pragma unroll
+            for (int t = 0; t < NL; t++) {
+                const double xt = x[t];
+#pragma unroll
+                for (int q = 0; q < NL; q++) o[q] = o[q] + FLs[t * NL + q] * xt;
+            }
+#pragma unroll
+            for (int q = 0; q < NL; q++) x[q] = o[q];
+        };
+        apply(xr);
+        apply(xi);
     }
 #pragma unroll
     for (int t = 0; t < NL; t++)
@@ -1325,10 +1338,10 @@ static void run_tile(const Geo &g, const TileArgs &A, int nf, hipStream_t st)
 #define TT(NL)                                                                                           \
     if (A.mode == 1)                                                                                     \
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_tile_t<NL, 1>), gt, dim3(256), 0, st, A, ncp, g.nxppy, g.fcf, \
-                           g.wg, g.AL, g.sig);                                                           \
+                           g.FL);                                                                        \
     else                                                                                                 \
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_tile_t<NL, 0>), gt, dim3(256), 0, st, A, ncp, g.nxppy, g.fcf, \
-                           g.wg, g.AL, g.sig)
+                           g.FL)
     switch (g.nzloc) {
     case 9: TT(9); return;
     case 13: TT(13); return;
