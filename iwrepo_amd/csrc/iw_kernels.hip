@@ -179,6 +179,32 @@ __device__ __forceinline__ void legfilter(double (&x)[NL], const double *wgs, co
     }
 }
 
+// the same filter as one matrix FL (nl x nl, in LDS): x[q] <- sum_t
+// FL[t nl + q] x[t], the two sums above taken together (half the
+// multiply-adds; equal to them up to rounding)
+template <int NL>
+__device__ __forceinline__ void legfilter_mat(double (&x)[NL], const double *FLs)
+{
+    double o[NL];
+#pragma unroll
+    for (int q = 0; q < NL; q++) o[q] = 0.;
+#pragma unroll
+    for (int t = 0; t < NL; t++) {
+        const double xt = x[t];
+#pragma unroll
+        for (int q = 0; q < NL; q++) o[q] = o[q] + FLs[t * NL + q] * xt;
+    }
+#pragma unroll
+    for (int q = 0; q < NL; q++) x[q] = o[q];
+}
+
+template <int NL>
+__device__ __forceinline__ void stage_fl(double *FLs, const double *FL)
+{
+    for (int e = threadIdx.x; e < NL * NL; e += blockDim.x) FLs[e] = FL[e];
+    __syncthreads();
+}
+
 template <int NL>
 __device__ __forceinline__ void stage_tables(double *wgs, double *ALs, double *sgs, const double *wg,
                                              const double *AL, const double *sig)
@@ -1310,21 +1336,8 @@ __global__ __launch_bounds__(256, 2) void k_tile_t(TileArgs A, long nmode, long 
         }
     }
     if (A.ilegen) { // filtering.f:905-956, the two sums as one matrix
-        auto apply = [&](double(&x)[NL]) __attribute__((always_inline)) {
-            double o[NL];
-#pragma unroll
-            for (int q = 0; q < NL; q++) o[q] = 0.;
-#pragma unroll
-            for (int t = 0; t < NL; t++) {
-                const double xt = x[t];
-#pragma unroll
-                for (int q = 0; q < NL; q++) o[q] = o[q] + FLs[t * NL + q] * xt;
-            }
-#pragma unroll
-            for (int q = 0; q < NL; q++) x[q] = o[q];
-        };
-        apply(xr);
-        apply(xi);
+        legfilter_mat<NL>(xr, FLs);
+        legfilter_mat<NL>(xi, FLs);
     }
 #pragma unroll
     for (int t = 0; t < NL; t++)
@@ -2320,13 +2333,11 @@ void launch_solve_grp(const Geo &g, const iw_params &P, const SysTables &S, doub
 // registers, the (uniform) tables read through the scalar cache.
 template <int NL>
 __global__ __launch_bounds__(256) void k_lfilter(double *u, double *v, double *w, double *t, long nxppy,
-                                                int cpen, const double *__restrict__ wg,
-                                                const double *__restrict__ AL,
-                                                const double *__restrict__ sig,
+                                                int cpen, const double *__restrict__ FL,
                                                 const double *__restrict__ damp)
 {
-    __shared__ double ALs[NL * NL], wgs[NL], sgs[NL];
-    stage_tables<NL>(wgs, ALs, sgs, wg, AL, sig);
+    __shared__ double FLs[NL * NL];
+    stage_fl<NL>(FLs, FL);
     const long col = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const int ks = blockIdx.y, fld = blockIdx.z;
     if (col >= nxppy) return;
@@ -2336,7 +2347,7 @@ __global__ __launch_bounds__(256) void k_lfilter(double *u, double *v, double *w
     double x[NL];
 #pragma unroll
     for (int k = 0; k < NL; k++) x[k] = f[(long)k * nxppy];
-    if (cpen) legfilter<NL>(x, wgs, ALs, sgs);
+    if (cpen) legfilter_mat<NL>(x, FLs);
     if (fld == 0 && m == 0)
 #pragma unroll
         for (int k = 0; k < NL; k++) x[k] = damp[ks * NL + k] * x[k];
@@ -2349,7 +2360,7 @@ void launch_lfilter(const Geo &g, double *const *uvwt, int cpen, hipStream_t st)
     dim3 grid((unsigned)((g.nxppy + 255) / 256), g.nsubd, 4);
 #define LF(NL)                                                                                             \
     hipLaunchKernelGGL(k_lfilter<NL>, grid, dim3(256), 0, st, uvwt[0], uvwt[1], uvwt[2], uvwt[3], g.nxppy, \
-                       cpen, g.wg, g.AL, g.sig, g.damp)
+                       cpen, g.FL, g.damp)
     switch (g.nzloc) {
     case 9: LF(9); break;
     case 13: LF(13); break;
@@ -2535,13 +2546,11 @@ __global__ __launch_bounds__(64) void k_solve_p(SolPArgs A)
 
 template <int NL>
 __global__ __launch_bounds__(256) void k_pproject(PArgs A, double *u, double *v, double *w, int cpen,
-                                                 int noslip, const double *__restrict__ wg,
-                                                 const double *__restrict__ AL,
-                                                 const double *__restrict__ sig)
+                                                 int noslip, const double *__restrict__ FL)
 {
-    __shared__ double ALs[NL * NL], wgs[NL], sgs[NL], Ds[NL * NL];
+    __shared__ double FLs[NL * NL], Ds[NL * NL];
     for (int e = threadIdx.x; e < NL * NL; e += blockDim.x) Ds[e] = A.D[e];
-    stage_tables<NL>(wgs, ALs, sgs, wg, AL, sig);
+    stage_fl<NL>(FLs, FL);
     const long col = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const int ks = blockIdx.y;
     const bool live = col < A.nxppy;
@@ -2551,7 +2560,7 @@ __global__ __launch_bounds__(256) void k_pproject(PArgs A, double *u, double *v,
     double x[NL];
 #pragma unroll
     for (int k = 0; k < NL; k++) x[k] = (m == 0) ? 0. : A.phi[base + (long)k * st + c];
-    if (cpen && m != 0) legfilter<NL>(x, wgs, ALs, sgs); // filter_legenpen of phi
+    if (cpen && m != 0) legfilter_mat<NL>(x, FLs); // filter_legenpen of phi
     const double xw = A.xw[m % A.nxhp], yw = A.yw[m / A.nxhp];
     const double cx = A.dt * xw, cy = A.dt * yw;
     const double fac = 2. / A.zh[ks];
@@ -2606,8 +2615,7 @@ void launch_pressure_nat(const Geo &g, const iw_params &P, const SysTables &S, d
     if (part == 0) hipLaunchKernelGGL(k_prhs<NL>, gc, dim3(256), 0, st, A);                               \
     if (part == 1) hipLaunchKernelGGL(k_solve_p<NL>, gs, dim3(64), 0, st, B);                             \
     if (part == 2)                                                                                        \
-        hipLaunchKernelGGL(k_pproject<NL>, gc, dim3(256), 0, st, A, uvwt[0], uvwt[1], uvwt[2], cpen, ns, g.wg, \
-                           g.AL, g.sig)
+        hipLaunchKernelGGL(k_pproject<NL>, gc, dim3(256), 0, st, A, uvwt[0], uvwt[1], uvwt[2], cpen, ns, g.FL)
     switch (g.nzloc) {
     case 9: PSTEP(9); break;
     case 13: PSTEP(13); break;
