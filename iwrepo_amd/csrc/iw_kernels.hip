@@ -198,6 +198,33 @@ __device__ __forceinline__ void legfilter_mat(double (&x)[NL], const double *FLs
     for (int q = 0; q < NL; q++) x[q] = o[q];
 }
 
+// two columns through the same matrix, each row of FL read once for both
+template <int NL>
+__device__ __forceinline__ void legfilter_mat2(double (&x)[NL], double (&y)[NL], const double *FLs)
+{
+    double o[NL], p[NL];
+#pragma unroll
+    for (int q = 0; q < NL; q++) {
+        o[q] = 0.;
+        p[q] = 0.;
+    }
+#pragma unroll
+    for (int t = 0; t < NL; t++) {
+        const double xt = x[t], yt = y[t];
+#pragma unroll
+        for (int q = 0; q < NL; q++) {
+            const double c = FLs[t * NL + q];
+            o[q] = o[q] + c * xt;
+            p[q] = p[q] + c * yt;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < NL; q++) {
+        x[q] = o[q];
+        y[q] = p[q];
+    }
+}
+
 template <int NL>
 __device__ __forceinline__ void stage_fl(double *FLs, const double *FL)
 {
@@ -791,16 +818,23 @@ struct ConvArgs {
     double *du, *dv, *dw, *dt;   // du/dz, dv/dz, dw/dz, drho/dz (dt may be null)
 };
 
-// (2/zh) D x of one column
+// (2/zh) D of a column x (into ox, may be null) and of its product with w,
+// (cf x) w (into op), both from each row of D read once
 template <int NL>
-__device__ __forceinline__ void dz_col(const double *Ds, const double (&x)[NL], double *o, long nxppy, double fac)
+__device__ __forceinline__ void dz_two(const double *Ds, const double (&x)[NL], const double (&w)[NL], double cf,
+                                       double *ox, double *op, long nxppy, double fac)
 {
 #pragma unroll 1
     for (int k1 = 0; k1 < NL; k1++) {
-        double sm = 0.0;
+        double s1 = 0.0, s2 = 0.0;
 #pragma unroll
-        for (int t = 0; t < NL; t++) sm = sm + Ds[k1 * NL + t] * x[t];
-        o[k1 * nxppy] = sm * fac;
+        for (int t = 0; t < NL; t++) {
+            const double d = Ds[k1 * NL + t];
+            s1 = s1 + d * x[t];
+            s2 = s2 + d * ((cf * x[t]) * w[t]);
+        }
+        if (ox) ox[k1 * nxppy] = s1 * fac;
+        op[k1 * nxppy] = s2 * fac;
     }
 }
 
@@ -816,27 +850,19 @@ __global__ __launch_bounds__(256) void k_dpdz_conv_t(ConvArgs A, long nxppy, con
     const int ks = blockIdx.y;
     const double fac = 2. / zh[ks];
     const long b0 = (long)ks * NL * nxppy + ij;
-    // each column: its own gradient, then in place the product with w
-    // ((c x) w, which for rho equals k_dpdz_t's (-w) rho exactly) and its
-    // gradient
+    // each column: its own gradient and that of its product with w
+    // ((c x) w, which for rho equals k_dpdz_t's (-w) rho exactly)
     double w[NL], c[NL];
 #pragma unroll
     for (int t = 0; t < NL; t++) w[t] = A.w[b0 + t * nxppy];
-    dz_col<NL>(Ds, w, A.dw + b0, nxppy, fac);
-#pragma unroll
-    for (int t = 0; t < NL; t++) c[t] = (-0.5 * w[t]) * w[t];
-    dz_col<NL>(Ds, c, A.fw + b0, nxppy, fac);
+    dz_two<NL>(Ds, w, w, -0.5, A.dw + b0, A.fw + b0, nxppy, fac);
     const double *src[3] = {A.u, A.v, A.t};
     double *dsts[3] = {A.du, A.dv, A.dt}, *fl[3] = {A.fu, A.fv, A.ft};
 #pragma unroll 1
     for (int q = 0; q < 3; q++) {
-        const double cf = q < 2 ? -0.5 : -1.0;
 #pragma unroll
         for (int t = 0; t < NL; t++) c[t] = src[q][b0 + t * nxppy];
-        if (dsts[q]) dz_col<NL>(Ds, c, dsts[q] + b0, nxppy, fac);
-#pragma unroll
-        for (int t = 0; t < NL; t++) c[t] = (cf * c[t]) * w[t];
-        dz_col<NL>(Ds, c, fl[q] + b0, nxppy, fac);
+        dz_two<NL>(Ds, c, w, q < 2 ? -0.5 : -1.0, dsts[q] ? dsts[q] + b0 : nullptr, fl[q] + b0, nxppy, fac);
     }
 }
 
@@ -1335,10 +1361,7 @@ __global__ __launch_bounds__(256, 2) void k_tile_t(TileArgs A, long nmode, long 
             xi[t] = gi;
         }
     }
-    if (A.ilegen) { // filtering.f:905-956, the two sums as one matrix
-        legfilter_mat<NL>(xr, FLs);
-        legfilter_mat<NL>(xi, FLs);
-    }
+    if (A.ilegen) legfilter_mat2<NL>(xr, xi, FLs); // filtering.f:905-956, the two sums as one matrix
 #pragma unroll
     for (int t = 0; t < NL; t++)
         *reinterpret_cast<double2 *>(outf + b0 + (long)t * nxppy) = make_double2(xr[t], xi[t]);
@@ -2329,8 +2352,8 @@ void launch_solve_grp(const Geo &g, const iw_params &P, const SysTables &S, doub
 
 // Legendre filter of every subdomain of the four solutions (filter_legenpen,
 // filtering.f:729-808) and the mean-u damping (calcuvwt.f:343-361), in place
-// on the native layout: lane = one column, the nl values of its subdomain in
-// registers, the (uniform) tables read through the scalar cache.
+// on the native layout: lane = the re and im columns of one mode, the nl
+// values of its subdomain in registers, the filter matrix in LDS.
 template <int NL>
 __global__ __launch_bounds__(256) void k_lfilter(double *u, double *v, double *w, double *t, long nxppy,
                                                 int cpen, const double *__restrict__ FL,
@@ -2338,26 +2361,32 @@ __global__ __launch_bounds__(256) void k_lfilter(double *u, double *v, double *w
 {
     __shared__ double FLs[NL * NL];
     stage_fl<NL>(FLs, FL);
-    const long col = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long m = (long)blockIdx.x * blockDim.x + threadIdx.x; // the re/im column pair of one mode
     const int ks = blockIdx.y, fld = blockIdx.z;
-    if (col >= nxppy) return;
-    const int m = (int)(col >> 1);
+    if (m >= nxppy / 2) return;
     if (fld == 2 && m == 0) return;
-    double *f = (fld == 0 ? u : fld == 1 ? v : fld == 2 ? w : t) + (long)ks * NL * nxppy + col;
-    double x[NL];
+    double *f = (fld == 0 ? u : fld == 1 ? v : fld == 2 ? w : t) + (long)ks * NL * nxppy + 2 * m;
+    double xr[NL], xi[NL];
 #pragma unroll
-    for (int k = 0; k < NL; k++) x[k] = f[(long)k * nxppy];
-    if (cpen) legfilter_mat<NL>(x, FLs);
+    for (int k = 0; k < NL; k++) {
+        const double2 q = *reinterpret_cast<const double2 *>(f + (long)k * nxppy);
+        xr[k] = q.x;
+        xi[k] = q.y;
+    }
+    if (cpen) legfilter_mat2<NL>(xr, xi, FLs);
     if (fld == 0 && m == 0)
 #pragma unroll
-        for (int k = 0; k < NL; k++) x[k] = damp[ks * NL + k] * x[k];
+        for (int k = 0; k < NL; k++) {
+            xr[k] = damp[ks * NL + k] * xr[k];
+            xi[k] = damp[ks * NL + k] * xi[k];
+        }
 #pragma unroll
-    for (int k = 0; k < NL; k++) f[(long)k * nxppy] = x[k];
+    for (int k = 0; k < NL; k++) *reinterpret_cast<double2 *>(f + (long)k * nxppy) = make_double2(xr[k], xi[k]);
 }
 
 void launch_lfilter(const Geo &g, double *const *uvwt, int cpen, hipStream_t st)
 {
-    dim3 grid((unsigned)((g.nxppy + 255) / 256), g.nsubd, 4);
+    dim3 grid((unsigned)((g.nxppy / 2 + 255) / 256), g.nsubd, 4);
 #define LF(NL)                                                                                             \
     hipLaunchKernelGGL(k_lfilter<NL>, grid, dim3(256), 0, st, uvwt[0], uvwt[1], uvwt[2], uvwt[3], g.nxppy, \
                        cpen, g.FL, g.damp)
