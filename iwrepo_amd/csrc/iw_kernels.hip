@@ -261,15 +261,26 @@ __device__ __forceinline__ void mc_take(bool mn, double x, int lx, double &b, in
     }
 }
 
-template <int DIR, bool AX = false> // AX: the batch's fields have flux products (forward only)
+// AX: the batch's fields have flux products (forward only); MAPS: the batch
+// has plane maps (else rows are addressed arithmetically, so no row address
+// waits on a table load and a thread's loads are all in flight together)
+template <int DIR, bool AX = false, bool MAPS = false>
 __global__ __launch_bounds__(kFT) void k_fft_x(FieldBatch fb, int nrows, int ny, int nxpp, int M, int logM,
                                                int R, const double2 *__restrict__ twM,
-                                               const double2 *__restrict__ twN, double scaley, McFuse mf)
+                                               const double2 *__restrict__ twNg, double scaley, McFuse mf)
 {
     extern __shared__ double2 smx[];
     double2 *buf = smx;
     double2 *tw = smx + R * M;
     double2 *nyq = tw + M; // R entries: the kx-Nyquist coefficient of each row (inverse, for maxcalc)
+    // the length-N twiddles and the wavenumbers kx of k = 0..M, staged in LDS
+    // with tw so that no per-element global table load sits between a
+    // thread's data loads or stores
+    double2 *twN = nyq + R;
+    double *kws = reinterpret_cast<double *>(twN + M + 1);
+    // maxcalc's per-level tables for the block's rows (w00, rm, rdz, ush),
+    // staged with the others so the epilogue after the stores reads LDS
+    double *mcv = kws + M + 1;
     const double *base = fb.f[blockIdx.y];
     double *obase = fb.g[blockIdx.y];
     const int row0 = blockIdx.x * R;
@@ -277,48 +288,70 @@ __global__ __launch_bounds__(kFT) void k_fft_x(FieldBatch fb, int nrows, int ny,
     const int lny = __builtin_ctz(ny); // ny is a power of two
     // logical row -> stored row of the source / destination (plane maps)
     auto srow = [&](int row) -> long {
+        if (!MAPS) return (long)row * nxpp;
         int lpl = row >> lny, j = row & (ny - 1);
         return (long)((fb.smap ? fb.smap[lpl] : lpl) * ny + j) * nxpp;
     };
     auto drow = [&](int row) -> long {
+        if (!MAPS) return (long)row * nxpp;
         int lpl = row >> lny, j = row & (ny - 1);
         return (long)((fb.dmap ? fb.dmap[lpl] : lpl) * ny + j) * nxpp;
     };
     for (int t = threadIdx.x; t < M; t += blockDim.x) tw[t] = twM[t];
+    for (int t = threadIdx.x; t <= M; t += blockDim.x) {
+        twN[t] = twNg[t];
+        kws[t] = fb.kw ? fb.kw[t] : 0.;
+    }
+    if (DIR > 0 && mf.part)
+        for (int r = threadIdx.x; r < R; r += blockDim.x) {
+            const int row = row0 + r, k = (row < nrows ? row : nrows - 1) >> lny;
+            mcv[r] = mf.w00 ? mf.w00[k] : 0.;
+            mcv[R + r] = mf.rm ? mf.rm[k] : 0.;
+            mcv[2 * R + r] = mf.rdz ? mf.rdz[k] : 0.;
+            mcv[3 * R + r] = mf.ush ? mf.ush[k] : 0.;
+        }
+    __syncthreads();
     // inverse pre-processing (FFT991 inverse, fft.f:1208-1281): the N/2+1
     // coefficients A_k + i B_k of a row -> the packed complex sequence; with
     // deriv the coefficients are first multiplied by i kx (the x gradient of
     // solver.f:86-103)
     auto load_inverse = [&](bool deriv) {
+        // all of a thread's loads first (rows past the end read the last row
+        // and are zeroed below), so they are in flight together
+        double2 va[kFE / kFT], vb[kFE / kFT];
 #pragma unroll
         for (int it = 0; it < kFE / kFT; it++) {
             const int idx = threadIdx.x + it * kFT;
-            int r = idx >> logM, k = idx & (M - 1), row = row0 + r;
-            double2 Z = make_double2(0., 0.);
-            if (row < nrows) {
-                const double *p = base + srow(row);
-                double2 va = *reinterpret_cast<const double2 *>(p + 2 * k);
-                double2 vb = *reinterpret_cast<const double2 *>(p + 2 * (M - k));
-                if (!deriv && k == 0) nyq[r] = vb; // kx-Nyquist coefficient of the row
-                if (deriv) {
-                    const double ka = fb.kw[k], kb = fb.kw[M - k];
-                    va = make_double2(-ka * va.y, ka * va.x);
-                    vb = make_double2(-kb * vb.y, kb * vb.x);
-                }
-                double ar, ai, br, bi;
-                if (k == 0) {
-                    ar = va.x; ai = 0.; br = vb.x; bi = 0.;
-                } else {
-                    ar = va.x; ai = va.y;
-                    br = vb.x; bi = -vb.y;
-                }
-                double er = ar + br, ei = ai + bi;
-                double dr = ar - br, di = ai - bi;
-                double2 w = twN[k];
-                double c = w.x, s = -w.y; // e^{+2 pi i k/N}
-                double orr = dr * c - di * s, oi = dr * s + di * c;
-                Z = make_double2(er - oi, ei + orr);
+            const int r = idx >> logM, k = idx & (M - 1), row = row0 + r;
+            const double *p = base + srow(row < nrows ? row : nrows - 1);
+            va[it] = *reinterpret_cast<const double2 *>(p + 2 * k);
+            vb[it] = *reinterpret_cast<const double2 *>(p + 2 * (M - k));
+        }
+#pragma unroll
+        for (int it = 0; it < kFE / kFT; it++) {
+            const int idx = threadIdx.x + it * kFT;
+            const int r = idx >> logM, k = idx & (M - 1), row = row0 + r;
+            double2 a = va[it], b = vb[it];
+            if (!deriv && k == 0) nyq[r] = b; // kx-Nyquist coefficient of the row
+            if (deriv) {
+                const double ka = kws[k], kb = kws[M - k];
+                a = make_double2(-ka * a.y, ka * a.x);
+                b = make_double2(-kb * b.y, kb * b.x);
             }
+            double ar, ai, br, bi;
+            if (k == 0) {
+                ar = a.x; ai = 0.; br = b.x; bi = 0.;
+            } else {
+                ar = a.x; ai = a.y;
+                br = b.x; bi = -b.y;
+            }
+            double er = ar + br, ei = ai + bi;
+            double dr = ar - br, di = ai - bi;
+            double2 w = twN[k];
+            double c = w.x, s = -w.y; // e^{+2 pi i k/N}
+            double orr = dr * c - di * s, oi = dr * s + di * c;
+            double2 Z = make_double2(er - oi, ei + orr);
+            if (row >= nrows) Z = make_double2(0., 0.);
             buf[lp(idx)] = Z;
         }
     };
@@ -351,8 +384,8 @@ __global__ __launch_bounds__(kFT) void k_fft_x(FieldBatch fb, int nrows, int ny,
             for (int it = 0; it < kFE / kFT; it++) {
                 const int idx = threadIdx.x + it * kFT;
                 int r = idx >> logM, m = idx & (M - 1), row = row0 + r;
-                double2 z = make_double2(0., 0.);
-                if (row < nrows) z = *reinterpret_cast<const double2 *>(src + srow(row) + 2 * m);
+                double2 z = *reinterpret_cast<const double2 *>(src + srow(row < nrows ? row : nrows - 1) + 2 * m);
+                if (row >= nrows) z = make_double2(0., 0.);
                 buf[lp(idx)] = z;
             }
         };
@@ -374,7 +407,7 @@ __global__ __launch_bounds__(kFT) void k_fft_x(FieldBatch fb, int nrows, int ny,
         };
         // + i kx P (the x part of the flux divergence, solver.f:985-1063, 1163-1196)
         auto addk = [&](double2 c, double2 q, int k) {
-            const double kx = fb.kw[k];
+            const double kx = kws[k];
             return make_double2(c.x + (-kx * q.y), c.y + kx * q.x);
         };
         if (AX) {
@@ -390,8 +423,9 @@ __global__ __launch_bounds__(kFT) void k_fft_x(FieldBatch fb, int nrows, int ny,
                 const int r = idx >> logM, m = idx & (M - 1);
                 const bool isp = r >= H;
                 const int row = rowA + (isp ? r - H : r);
-                double2 z = make_double2(0., 0.);
-                if (row < nrows) z = *reinterpret_cast<const double2 *>((isp ? acc : base) + srow(row) + 2 * m);
+                double2 z = *reinterpret_cast<const double2 *>((isp ? acc : base) +
+                                                               srow(row < nrows ? row : nrows - 1) + 2 * m);
+                if (row >= nrows) z = make_double2(0., 0.);
                 buf[lp(idx)] = z;
             }
             __syncthreads();
@@ -445,8 +479,8 @@ __global__ __launch_bounds__(kFT) void k_fft_x(FieldBatch fb, int nrows, int ny,
                 const double c = mf.cny * nyq[r].x;
                 double xe = z.x + c, xo = z.y - c; // points i = 2m, 2m+1
                 if (f == 2 && mf.w00) {
-                    xe = xe + mf.w00[k];
-                    xo = xo + mf.w00[k];
+                    xe = xe + mcv[r];
+                    xo = xo + mcv[r];
                 }
                 const int L = ((2 * m) * ny + j) * nz + k, L1 = L + ny * nz;
                 // + umwv, wmwv, rhomwv (zero unless set), + rhomean for rho
@@ -459,16 +493,16 @@ __global__ __launch_bounds__(kFT) void k_fft_x(FieldBatch fb, int nrows, int ny,
                 }
                 double ve = xe + ae, vo = xo + ao;
                 if (f == 3) {
-                    ve = ve + mf.rm[k];
-                    vo = vo + mf.rm[k];
+                    ve = ve + mcv[R + r];
+                    vo = vo + mcv[R + r];
                 }
                 mc_take(true, ve, L, b0, l0);
                 mc_take(true, vo, L1, b0, l0);
                 mc_take(false, ve, L, b1, l1);
                 mc_take(false, vo, L1, b1, l1);
                 if (f < 3) { // CFL numbers; the shear current advects in x; no CFLZ at the top point
-                    const double cf = f == 0 ? mf.cx : f == 1 ? mf.cy : (k == nz - 1 ? 0.0 : mf.dt * mf.rdz[k]);
-                    const double sh = (f == 0 && mf.ush) ? mf.ush[k] : 0.0;
+                    const double cf = f == 0 ? mf.cx : f == 1 ? mf.cy : (k == nz - 1 ? 0.0 : mf.dt * mcv[2 * R + r]);
+                    const double sh = (f == 0 && mf.ush) ? mcv[3 * R + r] : 0.0;
                     mc_take(false, fabs((sh + xe) * cf), L, b2, l2);
                     mc_take(false, fabs((sh + xo) * cf), L1, b2, l2);
                 }
@@ -523,13 +557,18 @@ __global__ __launch_bounds__(kFT) void k_fft_y(FieldBatch fb, int nxpp, int nxh,
     extern __shared__ double2 smy[];
     double2 *buf = smy;
     double2 *tw = smy + CB * ny;
+    double *kys = reinterpret_cast<double *>(tw + ny); // ky per row, staged like tw
     const int lpl = blockIdx.y;
     const double *plane = fb.f[blockIdx.z] + (long)(fb.smap ? fb.smap[lpl] : lpl) * nxpp * ny;
     const long opl = (long)(fb.dmap ? fb.dmap[lpl] : lpl) * nxpp * ny;
     double *oplane = fb.g[blockIdx.z] + opl;
     const int ii0 = blockIdx.x * CB;
     const int lcb = __builtin_ctz(CB); // CB is a power of two
-    for (int t = threadIdx.x; t < ny; t += blockDim.x) tw[t] = twY[t];
+    for (int t = threadIdx.x; t < ny; t += blockDim.x) {
+        tw[t] = twY[t];
+        kys[t] = fb.kw ? fb.kw[t] : 0.;
+    }
+    __syncthreads();
     // columns ii < nxh of the plane into LDS; with deriv multiplied by i ky
     // (the y gradient of solver.f:86-103)
     auto load = [&](const double *src, bool deriv) {
@@ -538,10 +577,10 @@ __global__ __launch_bounds__(kFT) void k_fft_y(FieldBatch fb, int nxpp, int nxh,
             const int idx = threadIdx.x + it * kFT;
             if (idx >= CB * ny) break;
             int j = idx >> lcb, c = idx & (CB - 1), ii = ii0 + c;
-            double2 z = make_double2(0., 0.);
-            if (ii < nxh) z = *reinterpret_cast<const double2 *>(src + (long)j * nxpp + 2 * ii);
+            double2 z = *reinterpret_cast<const double2 *>(src + (long)j * nxpp + 2 * (ii < nxh ? ii : nxh - 1));
+            if (ii >= nxh) z = make_double2(0., 0.);
             if (deriv) {
-                const double ky = fb.kw[j];
+                const double ky = kys[j];
                 z = make_double2(-ky * z.y, ky * z.x);
             }
             buf[(j << lcb) + c] = z;
@@ -565,7 +604,7 @@ __global__ __launch_bounds__(kFT) void k_fft_y(FieldBatch fb, int nxpp, int nxh,
                 z.x = scaley * z.x;
                 z.y = scaley * z.y;
                 if (add) {
-                    const double ky = fb.kw[j];
+                    const double ky = kys[j];
                     z.x = z.x + (-ky * pb[it].y);
                     z.y = z.y + ky * pb[it].x;
                 }
@@ -609,7 +648,7 @@ __global__ __launch_bounds__(kFT) void k_fft_y(FieldBatch fb, int nxpp, int nxh,
             const double *p = plane + (long)j * nxpp + 2 * nxh;
             double pr = p[0], pi = p[1];
             if (accp) {
-                const double ky = fb.kw[j];
+                const double ky = kys[j];
                 const double *a = accp + (long)j * nxpp + 2 * nxh;
                 pr = pr + (-ky * a[1]);
                 pi = pi + ky * a[0];
@@ -620,7 +659,7 @@ __global__ __launch_bounds__(kFT) void k_fft_y(FieldBatch fb, int nxpp, int nxh,
                 q[1] = pi;
             }
             if (dyp) {
-                const double ky = fb.kw[j];
+                const double ky = kys[j];
                 double *q = dyp + (long)j * nxpp + 2 * nxh;
                 q[0] = -ky * pi;
                 q[1] = ky * pr;
@@ -628,17 +667,17 @@ __global__ __launch_bounds__(kFT) void k_fft_y(FieldBatch fb, int nxpp, int nxh,
         }
 }
 
-void launch_fft_x(const Geo &g, FieldBatch b, int nk, int dir, hipStream_t st, const McFuse *mf)
+template <bool MAPS>
+static void fft_x_launch(const Geo &g, const FieldBatch &b, int nk, int dir, hipStream_t st, const McFuse *mf)
 {
     const int R = kFE / g.M;
     int nrows = g.ny * nk;
     dim3 grid((nrows + R - 1) / R, b.n);
-    size_t sh = sizeof(double2) * ((size_t)R * g.M + g.M + R);
+    size_t sh = sizeof(double2) * ((size_t)R * g.M + g.M + R + g.M + 1) + sizeof(double) * (g.M + 1 + 4 * R);
     McFuse none;
     if (dir < 0) {
-        // fields with a flux product hold its coefficients in registers (more
-        // VGPRs): they get their own launch so the others run at a higher
-        // occupancy
+        // fields with a flux product share the tile with it (H rows of
+        // each): they get their own launch
         FieldBatch ba = b, bp = b;
         ba.n = bp.n = 0;
         for (int q = 0; q < b.n; q++) {
@@ -653,14 +692,24 @@ void launch_fft_x(const Geo &g, FieldBatch b, int nk, int dir, hipStream_t st, c
             t.n++;
         }
         if (ba.n)
-            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fft_x<-1, true>), dim3((nrows + R / 2 - 1) / (R / 2), ba.n), dim3(kFT), sh, st, ba, nrows,
-                               g.ny, g.nxpp, g.M, g.logM, R, g.twM, g.twN, g.scaley, none);
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fft_x<-1, true, MAPS>), dim3((nrows + R / 2 - 1) / (R / 2), ba.n),
+                               dim3(kFT), sh, st, ba, nrows, g.ny, g.nxpp, g.M, g.logM, R, g.twM, g.twN, g.scaley,
+                               none);
         if (bp.n)
-            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fft_x<-1, false>), dim3(grid.x, bp.n), dim3(kFT), sh, st, bp,
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fft_x<-1, false, MAPS>), dim3(grid.x, bp.n), dim3(kFT), sh, st, bp,
                                nrows, g.ny, g.nxpp, g.M, g.logM, R, g.twM, g.twN, g.scaley, none);
-    } else
-        hipLaunchKernelGGL(k_fft_x<1>, grid, dim3(kFT), sh, st, b, nrows, g.ny, g.nxpp, g.M, g.logM, R,
-                           g.twM, g.twN, g.scaley, mf ? *mf : none);
+    } else {
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fft_x<1, false, MAPS>), grid, dim3(kFT), sh, st, b, nrows, g.ny,
+                           g.nxpp, g.M, g.logM, R, g.twM, g.twN, g.scaley, mf ? *mf : none);
+    }
+}
+
+void launch_fft_x(const Geo &g, FieldBatch b, int nk, int dir, hipStream_t st, const McFuse *mf)
+{
+    if (b.smap || b.dmap)
+        fft_x_launch<true>(g, b, nk, dir, st, mf);
+    else
+        fft_x_launch<false>(g, b, nk, dir, st, mf);
 }
 
 int mc_blocks(const Geo &g) { return (g.ny * g.nz + kFE / g.M - 1) / (kFE / g.M); }
@@ -671,7 +720,7 @@ void launch_fft_y(const Geo &g, FieldBatch b, int nk, int dir, hipStream_t st)
     int CB = kFE / g.ny;
     if (CB > g.nxh) CB = g.nxh;
     dim3 grid((g.nxh + CB - 1) / CB, nk, b.n);
-    size_t sh = sizeof(double2) * ((size_t)CB * g.ny + g.ny);
+    size_t sh = sizeof(double2) * ((size_t)CB * g.ny + g.ny) + sizeof(double) * g.ny;
     if (dir < 0)
         hipLaunchKernelGGL(k_fft_y<-1>, grid, dim3(kFT), sh, st, b, g.nxpp, g.nxh, g.ny, g.logNy, CB,
                            g.twY, g.scaley);
