@@ -572,12 +572,21 @@ __global__ __launch_bounds__(kFT) void k_fft_y(FieldBatch fb, int nxpp, int nxh,
     // columns ii < nxh of the plane into LDS; with deriv multiplied by i ky
     // (the y gradient of solver.f:86-103)
     auto load = [&](const double *src, bool deriv) {
+        // all loads first (columns past nxh are clamped and their values
+        // dropped), so they are in flight together; the tile is always
+        // CB x ny = kFE values
+        double2 v[kFE / kFT];
 #pragma unroll
         for (int it = 0; it < kFE / kFT; it++) {
             const int idx = threadIdx.x + it * kFT;
-            if (idx >= CB * ny) break;
+            const int j = idx >> lcb, ii = ii0 + (idx & (CB - 1));
+            v[it] = *reinterpret_cast<const double2 *>(src + (long)j * nxpp + 2 * (ii < nxh ? ii : nxh - 1));
+        }
+#pragma unroll
+        for (int it = 0; it < kFE / kFT; it++) {
+            const int idx = threadIdx.x + it * kFT;
             int j = idx >> lcb, c = idx & (CB - 1), ii = ii0 + c;
-            double2 z = *reinterpret_cast<const double2 *>(src + (long)j * nxpp + 2 * (ii < nxh ? ii : nxh - 1));
+            double2 z = v[it];
             if (ii >= nxh) z = make_double2(0., 0.);
             if (deriv) {
                 const double ky = kys[j];
@@ -596,7 +605,6 @@ __global__ __launch_bounds__(kFT) void k_fft_y(FieldBatch fb, int nxpp, int nxh,
 #pragma unroll
         for (int it = 0; it < kFE / kFT; it++) {
             const int idx = threadIdx.x + it * kFT;
-            if (idx >= CB * ny) break;
             int j = idx >> lcb, c = idx & (CB - 1), ii = ii0 + c;
             if (ii >= nxh) continue;
             double2 z = buf[(j << lcb) + c];
@@ -718,7 +726,6 @@ void launch_fft_y(const Geo &g, FieldBatch b, int nk, int dir, hipStream_t st)
 {
     if (g.ndims != 3) return;
     int CB = kFE / g.ny;
-    if (CB > g.nxh) CB = g.nxh;
     dim3 grid((g.nxh + CB - 1) / CB, nk, b.n);
     size_t sh = sizeof(double2) * ((size_t)CB * g.ny + g.ny) + sizeof(double) * g.ny;
     if (dir < 0)
@@ -2497,22 +2504,28 @@ __global__ __launch_bounds__(256) void k_prhs(PArgs A)
 #pragma unroll
     for (int t = 0; t < NL; t++) wc[t] = A.w[base + (long)t * st + col];
     const long pc = col ^ 1;
+    // every load before the arithmetic (no load behind a branch), so they are
+    // in flight together
+    const double pw0 = A.sysinfo[4 * sP + 1] * (A.wnl[col] + A.wviscrot[col]) * eps;
+    const double pw1 = A.sysinfo[4 * sP + 2] * (A.wnl[st + col] + A.wviscrot[st + col]) * eps;
+    double uu[NL], vv[NL];
+#pragma unroll
+    for (int k = 0; k < NL; k++) {
+        const long o = base + (long)k * st + pc;
+        uu[k] = A.u[o];
+        vv[k] = A.v[o];
+    }
 #pragma unroll
     for (int k = 0; k < NL; k++) {
         double sd = 0.;
 #pragma unroll
         for (int t = 0; t < NL; t++) sd = sd + Ds[k * NL + t] * wc[t];
-        const long o = base + (long)k * st;
-        double d;
-        if (comp == 0)
-            d = (0. + (-xw * A.u[o + pc])) + (-yw * A.v[o + pc]);
-        else
-            d = (0. + xw * A.u[o + pc]) + yw * A.v[o + pc];
+        double d = (comp == 0) ? (0. + (-xw * uu[k])) + (-yw * vv[k]) : (0. + xw * uu[k]) + yw * vv[k];
         d = d + sd * fac;
         double r = -(d / A.dt) * eps;
         const int kg = ks * NL + k;
-        if (kg == 0) r = r - A.sysinfo[4 * sP + 1] * (A.wnl[col] + A.wviscrot[col]) * eps;
-        if (kg == A.nz - 1) r = r + A.sysinfo[4 * sP + 2] * (A.wnl[st + col] + A.wviscrot[st + col]) * eps;
+        if (kg == 0) r = r - pw0;
+        if (kg == A.nz - 1) r = r + pw1;
         phi[(long)k * st] = r;
     }
 }
@@ -2642,30 +2655,52 @@ __global__ __launch_bounds__(256) void k_pproject(PArgs A, double *u, double *v,
     const double xw = A.xw[m % A.nxhp], yw = A.yw[m / A.nxhp];
     const double cx = A.dt * xw, cy = A.dt * yw;
     const double fac = 2. / A.zh[ks];
+    // a few levels at a time: their u, v, w loads first (no load behind a
+    // branch, so they are in flight together), then the updates
+    constexpr int KB = 5;
 #pragma unroll
-    for (int k = 0; k < NL; k++) {
-        const double other = __shfl_xor(x[k], 1); // phi of the partner component
-        double sd = 0.;
+    for (int k0 = 0; k0 < NL; k0 += KB) {
+        double uo[KB], vo[KB], wo[KB];
 #pragma unroll
-        for (int t = 0; t < NL; t++) sd = sd + Ds[k * NL + t] * x[t];
-        if (!live) continue;
-        const long o = base + (long)k * st + c;
-        const int kg = ks * NL + k;
-        if (noslip && kg == 0) { // rows the no-slip variant never assigns (calcuvwt_noslip.f:851)
-            u[o] = 0.;
-            v[o] = 0.;
-            w[o] = 0.;
-            continue;
+        for (int kk = 0; kk < KB; kk++) {
+            const int k = k0 + kk;
+            if (k < NL) {
+                const long o = base + (long)k * st + c;
+                uo[kk] = u[o];
+                vo[kk] = v[o];
+                wo[kk] = w[o];
+            }
         }
-        const double tr = comp == 0 ? x[k] : other, ti = comp == 0 ? other : x[k];
-        if (comp == 0) {
-            u[o] = u[o] - (0. * tr - cx * ti);
-            v[o] = v[o] - (0. * tr - cy * ti);
-        } else {
-            u[o] = u[o] - (0. * ti + cx * tr);
-            v[o] = v[o] - (0. * ti + cy * tr);
+#pragma unroll
+        for (int kk = 0; kk < KB; kk++) {
+            const int k = k0 + kk;
+            if (k >= NL) break;
+            const double other = __shfl_xor(x[k], 1); // phi of the partner component
+            double sd = 0.;
+#pragma unroll
+            for (int t = 0; t < NL; t++) sd = sd + Ds[k * NL + t] * x[t];
+            const double tr = comp == 0 ? x[k] : other, ti = comp == 0 ? other : x[k];
+            double nu, nv;
+            if (comp == 0) {
+                nu = uo[kk] - (0. * tr - cx * ti);
+                nv = vo[kk] - (0. * tr - cy * ti);
+            } else {
+                nu = uo[kk] - (0. * ti + cx * tr);
+                nv = vo[kk] - (0. * ti + cy * tr);
+            }
+            double nw = (m == 0) ? 0. : wo[kk] - A.dt * (sd * fac);
+            if (noslip && ks * NL + k == 0) { // rows the no-slip variant never assigns (calcuvwt_noslip.f:851)
+                nu = 0.;
+                nv = 0.;
+                nw = 0.;
+            }
+            if (live) {
+                const long o = base + (long)k * st + c;
+                u[o] = nu;
+                v[o] = nv;
+                w[o] = nw;
+            }
         }
-        w[o] = (m == 0) ? 0. : w[o] - A.dt * (sd * fac);
     }
 }
 
