@@ -286,22 +286,37 @@ __global__ __launch_bounds__(kFT) void k_fft_x(FieldBatch fb, int nrows, int ny,
     const int row0 = blockIdx.x * R;
     const int N = 2 * M;
     const int lny = __builtin_ctz(ny); // ny is a power of two
-    // logical row -> stored row of the source / destination (plane maps)
-    auto srow = [&](int row) -> long {
-        if (!MAPS) return (long)row * nxpp;
+    // logical row -> stored row of the source / destination (plane maps;
+    // with maps, the offsets of the block's rows are staged in LDS below)
+    auto srow_g = [&](int row) -> long {
         int lpl = row >> lny, j = row & (ny - 1);
         return (long)((fb.smap ? fb.smap[lpl] : lpl) * ny + j) * nxpp;
     };
-    auto drow = [&](int row) -> long {
-        if (!MAPS) return (long)row * nxpp;
+    auto drow_g = [&](int row) -> long {
         int lpl = row >> lny, j = row & (ny - 1);
         return (long)((fb.dmap ? fb.dmap[lpl] : lpl) * ny + j) * nxpp;
+    };
+    long *rmap = reinterpret_cast<long *>(kws + M + 1 + 4 * R); // 2 R: source, destination offsets
+    const bool staged = MAPS && !AX;
+    auto srow = [&](int row) -> long {
+        if (!MAPS) return (long)row * nxpp;
+        return staged ? rmap[row - row0] : srow_g(row);
+    };
+    auto drow = [&](int row) -> long {
+        if (!MAPS) return (long)row * nxpp;
+        return staged ? rmap[R + row - row0] : drow_g(row);
     };
     for (int t = threadIdx.x; t < M; t += blockDim.x) tw[t] = twM[t];
     for (int t = threadIdx.x; t <= M; t += blockDim.x) {
         twN[t] = twNg[t];
         kws[t] = fb.kw ? fb.kw[t] : 0.;
     }
+    if (staged)
+        for (int r = threadIdx.x; r < R; r += blockDim.x) {
+            const int row = row0 + r < nrows ? row0 + r : nrows - 1;
+            rmap[r] = srow_g(row);
+            rmap[R + r] = drow_g(row);
+        }
     if (DIR > 0 && mf.part)
         for (int r = threadIdx.x; r < R; r += blockDim.x) {
             const int row = row0 + r, k = (row < nrows ? row : nrows - 1) >> lny;
@@ -681,7 +696,8 @@ static void fft_x_launch(const Geo &g, const FieldBatch &b, int nk, int dir, hip
     const int R = kFE / g.M;
     int nrows = g.ny * nk;
     dim3 grid((nrows + R - 1) / R, b.n);
-    size_t sh = sizeof(double2) * ((size_t)R * g.M + g.M + R + g.M + 1) + sizeof(double) * (g.M + 1 + 4 * R);
+    size_t sh = sizeof(double2) * ((size_t)R * g.M + g.M + R + g.M + 1) + sizeof(double) * (g.M + 1 + 4 * R) +
+                sizeof(long) * 2 * R;
     McFuse none;
     if (dir < 0) {
         // fields with a flux product share the tile with it (H rows of
