@@ -1692,7 +1692,7 @@ __global__ __launch_bounds__(256) void k_compact(const double *__restrict__ AB, 
 {
     const long sys = first + (long)blockIdx.x;
     const double *ab = AB + sys * n * ldab;
-    if (sys < np) { // pressure system: L rows kv..kv+kl, U rows 0..kv+1 (k_solve_p)
+    if (sys < np) { // pressure system: L rows kv..kv+kl, U rows 0..kv+1 (k_solve_pg)
         const int nf = pb_nf(kl), nb = pb_nb(kl);
         double *pf = PB + sys * n * (nf + nb);
         for (int e = threadIdx.x; e < n * nf; e += blockDim.x) {
@@ -2479,8 +2479,8 @@ void launch_lfilter(const Geo &g, double *const *uvwt, int cpen, hipStream_t st)
 //   k_prhs:     divergence (divcolc, derv.f:35-85) of u*, scaled right-hand
 //               side -(div/dt) eps and the Neumann data wnl + wviscrot
 //               (calcuvwt.f:112-159, penalty.f:191-194, 583-598) -> phi;
-//   k_solve_p:  SGBTRS with row interchanges (bandludcmp.f:1020-1036), lane =
-//               one column, the same lane order as k_solve_nat;
+//   k_solve_pg: SGBTRS with row interchanges (bandludcmp.f:1020-1036), lane =
+//               one column, groups of lanes sharing a system;
 //   k_pproject: Legendre filter of phi, dphi/dz and u* - dt grad(phi).
 // ---------------------------------------------------------------------------
 struct PArgs {
@@ -2546,107 +2546,184 @@ __global__ __launch_bounds__(256) void k_prhs(PArgs A)
     }
 }
 
-struct SolPArgs {
+// SGBTRS with row interchanges in the grouped layout of k_solve_grp: the G
+// lanes of a group (+ky and -ky rows of one kx, re and im) share a pressure
+// system (it depends on k^2 only) and fetch its packed L row / U column
+// DPF / DPB rows ahead, one share each, assembled in LDS; the pivot row of
+// each step is fetched DPF rows ahead as well.  Per lane: a window w of the
+// right-hand side; each forward step swaps w[0] with w[ip[j] - j], emits
+// w[0] and shifts the window by one with the L multipliers; the backward
+// step divides by the diagonal of U and shifts with U's column.
+#ifndef IW_PG_DPF
+#define IW_PG_DPF 12 // L rows fetched ahead
+#endif
+#ifndef IW_PG_DPB
+#define IW_PG_DPB 6 // U columns fetched ahead
+#endif
+struct SolPGArgs {
     double *phi;
     const int *tasks;
     const int *modesys;
     const double *PB;
     const int *ipiv;
     long nxppy;
-    int n, ldab, kv;
+    int n;
 };
 
-template <int KL>
-__global__ __launch_bounds__(64) void k_solve_p(SolPArgs A)
+template <int KL, int G, int DPF, int DPB>
+__global__ __launch_bounds__(64) void k_solve_pg(SolPGArgs A)
 {
-    const int col = A.tasks[blockIdx.x * 64 + threadIdx.x];
-    if (col < 0) return;
+    static_assert(KL % 2 == 1, "16-byte factor rows need an even row length kl + 1");
+    constexpr int NG = 64 / G;
+    constexpr int NF2 = pb_nf(KL) / 2; // L row as double2
+    constexpr int NB2 = pb_nb(KL) / 2; // U column as double2
+    constexpr int KV = 2 * KL;
+    constexpr int PF = (NF2 + G - 1) / G, PU = (NB2 + G - 1) / G; // pieces per lane
+    __shared__ double2 seg[2][NG][NB2 + 1];
+    const int lane = threadIdx.x, grp = lane / G, rk = lane % G;
+    const int e = A.tasks[blockIdx.x * 64 + lane];
+    if (e < 0) return; // the whole group is empty
+    const bool idle = (e >> 29) & 1;
+    const int col = e & ((1 << 28) - 1);
     const int m = col >> 1;
-    if (m == 0) return;
+    if (m == 0) return; // no pressure for kx = ky = 0; its group holds only this mode
     const int sys = A.modesys[4 * m];
     const int n = A.n;
     const long st = A.nxppy;
-    constexpr int NF = KL + 1;     // rows kv..kv+KL of column j (L multipliers)
-    constexpr int KV = 2 * KL;     // superdiagonals of U after interchanges
-    constexpr int NB = KV + 2;     // rows 0..kv+1 of column j (U column)
-    static_assert(NF == pb_nf(KL) && NB == pb_nb(KL), "packed pressure factor layout");
-    // packed factor: n L segments of NF, then n U columns of NB
-    const double *__restrict__ pf = A.PB + (long)sys * n * (NF + NB);
-    const double *__restrict__ pu = pf + (long)n * NF;
+    const double2 *__restrict__ pf2 = reinterpret_cast<const double2 *>(A.PB) + (long)sys * n * (NF2 + NB2);
+    const double2 *__restrict__ pu2 = pf2 + (long)n * NF2;
     const int *__restrict__ ip = A.ipiv + (long)sys * n;
     double *__restrict__ b = A.phi + col;
     // ---- forward: L with row interchanges (bandludcmp.f:1020-1030)
     {
         double w[KL + 1];
+        double rx[DPF][PF], ry[DPF][PF];
+        double bring[DPF];
+        int pring[DPF];
+        auto ldp = [&](int j, double(&dx)[PF], double(&dy)[PF]) __attribute__((always_inline)) {
+            const long c = j < n ? j : n - 1;
+#pragma unroll
+            for (int q = 0; q < PF; q++) {
+                const int p = rk + q * G;
+                const double2 v = pf2[NF2 * c + (p < NF2 ? p : 0)];
+                dx[q] = v.x;
+                dy[q] = v.y;
+            }
+        };
+        auto stp = [&](int buf, const double(&sx)[PF], const double(&sy)[PF]) __attribute__((always_inline)) {
+#pragma unroll
+            for (int q = 0; q < PF; q++) {
+                const int p = rk + q * G;
+                if (p < NF2) seg[buf][grp][p] = make_double2(sx[q], sy[q]);
+            }
+        };
 #pragma unroll
         for (int t = 0; t <= KL; t++) {
             const double v = b[(long)(t < n ? t : n - 1) * st];
             w[t] = (t < n) ? v : 0.;
         }
-        double f0[NF], f1[NF];
-        double r0, r1;
-        int p0, p1;
-        ld_seg<NF>(pf, f0);
-        r0 = b[(long)(KL + 1 < n ? KL + 1 : n - 1) * st];
-        p0 = ip[0];
-        int j = 0;
-        auto step = [&](double (&fc)[NF], double (&fn)[NF], double &rc, double &rn, int &pc, int &pn) {
-            const int jp = j + 1 < n ? j + 1 : n - 1;
-            ld_seg<NF>(pf + (long)NF * jp, fn);
-            pn = ip[jp];
-            const int rp = j + KL + 2 < n ? j + KL + 2 : n - 1;
-            rn = b[(long)rp * st];
-            const int d = pc - j; // 0 <= d <= KL
-            double y = w[0];
+        static_for<0, DPF>([&](auto dc) __attribute__((always_inline)) {
+            constexpr int d = decltype(dc)::value;
+            ldp(d, rx[d], ry[d]);
+            const int r = KL + 1 + d;
+            bring[d] = b[(long)(r < n ? r : n - 1) * st];
+            pring[d] = ip[d < n ? d : n - 1];
+        });
+        stp(0, rx[0], ry[0]);
+        for (int j0 = 0; j0 < n; j0 += DPF) {
+            static_for<0, DPF>([&](auto dc) __attribute__((always_inline)) {
+                constexpr int d = decltype(dc)::value;
+                const int j = j0 + d;
+                if (j < n) {
+                    stp((j + 1) & 1, rx[(d + 1) % DPF], ry[(d + 1) % DPF]);
+                    ldp(j + DPF, rx[d], ry[d]);
+                    double fc[2 * NF2];
 #pragma unroll
-            for (int t = 1; t <= KL; t++) y = (d == t) ? w[t] : y;
-            const double w0 = w[0];
+                    for (int p = 0; p < NF2; p++) {
+                        const double2 v = seg[j & 1][grp][p];
+                        fc[2 * p] = v.x;
+                        fc[2 * p + 1] = v.y;
+                    }
+                    const double rc = bring[d];
+                    const int pc = pring[d];
+                    const int r = j + KL + 1 + DPF;
+                    bring[d] = b[(long)(r < n ? r : n - 1) * st];
+                    pring[d] = ip[j + DPF < n ? j + DPF : n - 1];
+                    const int dd = pc - j; // 0 <= dd <= KL
+                    double y = w[0];
 #pragma unroll
-            for (int t = 1; t <= KL; t++) {
-                const double wt = (d == t) ? w0 : w[t];
-                w[t - 1] = wt + fc[t] * (-y);
-            }
-            w[KL] = (j + KL + 1 < n) ? rc : 0.;
-            b[(long)j * st] = y;
-            j++;
-        };
-        while (j < n) {
-            step(f0, f1, r0, r1, p0, p1);
-            if (j >= n) break;
-            step(f1, f0, r1, r0, p1, p0);
+                    for (int t = 1; t <= KL; t++) y = (dd == t) ? w[t] : y;
+                    const double w0 = w[0];
+#pragma unroll
+                    for (int t = 1; t <= KL; t++) {
+                        const double wt = (dd == t) ? w0 : w[t];
+                        w[t - 1] = wt + fc[t] * (-y);
+                    }
+                    w[KL] = (j + KL + 1 < n) ? rc : 0.;
+                    if (!idle) b[(long)j * st] = y;
+                }
+            });
         }
     }
     // ---- backward: U with kv = 2 kl superdiagonals (STBSV, ludcmp.f:4698-5046)
     {
         double w[KV + 1];
+        double rx[DPB][PU], ry[DPB][PU];
+        double bring[DPB];
+        auto ldp = [&](int sidx, double(&dx)[PU], double(&dy)[PU]) __attribute__((always_inline)) {
+            const long c = sidx < n ? n - 1 - sidx : 0;
+#pragma unroll
+            for (int q = 0; q < PU; q++) {
+                const int p = rk + q * G;
+                const double2 v = pu2[NB2 * c + (p < NB2 ? p : 0)];
+                dx[q] = v.x;
+                dy[q] = v.y;
+            }
+        };
+        auto stp = [&](int buf, const double(&sx)[PU], const double(&sy)[PU]) __attribute__((always_inline)) {
+#pragma unroll
+            for (int q = 0; q < PU; q++) {
+                const int p = rk + q * G;
+                if (p < NB2) seg[buf][grp][p] = make_double2(sx[q], sy[q]);
+            }
+        };
 #pragma unroll
         for (int t = 0; t <= KV; t++) {
             const int r = n - 1 - t;
             const double v = b[(long)(r >= 0 ? r : 0) * st];
             w[t] = (r >= 0) ? v : 0.;
         }
-        double g0[NB], g1[NB];
-        double r0, r1;
-        ld_seg<NB>(pu + (long)NB * (n - 1), g0);
-        r0 = b[(long)(n - 2 - KV >= 0 ? n - 2 - KV : 0) * st];
-        int j = n - 1;
-        auto step = [&](double (&gc)[NB], double (&gn)[NB], double &rc, double &rn) {
-            const int jp = j - 1 >= 0 ? j - 1 : 0;
-            ld_seg<NB>(pu + (long)NB * jp, gn);
-            const int rp = j - KV - 2 >= 0 ? j - KV - 2 : 0;
-            rn = b[(long)rp * st];
-            const double dc = gc[KV];
-            const double x = (w[0] != 0.) ? w[0] / dc : w[0];
+        static_for<0, DPB>([&](auto dc) __attribute__((always_inline)) {
+            constexpr int d = decltype(dc)::value;
+            ldp(d, rx[d], ry[d]);
+            const int r = n - 2 - KV - d;
+            bring[d] = b[(long)(r >= 0 ? r : 0) * st];
+        });
+        stp(0, rx[0], ry[0]);
+        for (int s0 = 0; s0 < n; s0 += DPB) {
+            static_for<0, DPB>([&](auto dc) __attribute__((always_inline)) {
+                constexpr int d = decltype(dc)::value;
+                const int sidx = s0 + d, j = n - 1 - sidx;
+                if (sidx < n) {
+                    stp((sidx + 1) & 1, rx[(d + 1) % DPB], ry[(d + 1) % DPB]);
+                    ldp(sidx + DPB, rx[d], ry[d]);
+                    const double2 *gs = seg[sidx & 1][grp];
+                    const double rc = bring[d];
+                    const int r = j - KV - 1 - DPB;
+                    bring[d] = b[(long)(r >= 0 ? r : 0) * st];
+                    const double2 dd = gs[KV / 2];
+                    const double dc = dd.x; // gc[KV] (KV even)
+                    const double x = (w[0] != 0.) ? w[0] / dc : w[0];
 #pragma unroll
-            for (int t = 1; t <= KV; t++) w[t - 1] = w[t] - x * gc[KV - t];
-            w[KV] = (j - KV - 1 >= 0) ? rc : 0.;
-            b[(long)j * st] = x;
-            j--;
-        };
-        while (j >= 0) {
-            step(g0, g1, r0, r1);
-            if (j < 0) break;
-            step(g1, g0, r1, r0);
+                    for (int t = 1; t <= KV; t++) {
+                        const int gi = KV - t;
+                        const double2 gv = gs[gi >> 1];
+                        w[t - 1] = w[t] - x * ((gi & 1) ? gv.y : gv.x);
+                    }
+                    w[KV] = (j - KV - 1 >= 0) ? rc : 0.;
+                    if (!idle) b[(long)j * st] = x;
+                }
+            });
         }
     }
 }
@@ -2734,15 +2811,15 @@ void launch_pressure_nat(const Geo &g, const iw_params &P, const SysTables &S, d
     A.nxppy = g.nxppy;
     A.nxhp = g.nxhp;
     A.nz = g.nz;
-    SolPArgs B;
-    B.phi = phi; B.tasks = S.np_tasks; B.modesys = S.modesys; B.PB = S.PB; B.ipiv = S.ipiv;
-    B.nxppy = g.nxppy; B.n = g.nz; B.ldab = S.ldab; B.kv = S.kv;
     dim3 gc((unsigned)((g.nxppy + 255) / 256), g.nsubd);
-    dim3 gs(S.np_ntasks / 64);
+    SolPGArgs C;
+    C.phi = phi; C.tasks = S.gwt_tasks; C.modesys = S.modesys; C.PB = S.PB; C.ipiv = S.ipiv;
+    C.nxppy = g.nxppy; C.n = g.nz;
+    dim3 gg(S.gwt_ntasks / 64);
     const int ns = P.bc_uv_bottom == 1;
 #define PSTEP(NL)                                                                                         \
     if (part == 0) hipLaunchKernelGGL(k_prhs<NL>, gc, dim3(256), 0, st, A);                               \
-    if (part == 1) hipLaunchKernelGGL(k_solve_p<NL>, gs, dim3(64), 0, st, B);                             \
+    if (part == 1) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_solve_pg<NL, 4, IW_PG_DPF, IW_PG_DPB>), gg, dim3(64), 0, st, C);  \
     if (part == 2)                                                                                        \
         hipLaunchKernelGGL(k_pproject<NL>, gc, dim3(256), 0, st, A, uvwt[0], uvwt[1], uvwt[2], cpen, ns, g.FL)
     switch (g.nzloc) {
