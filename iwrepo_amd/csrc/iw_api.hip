@@ -409,7 +409,7 @@ static int build_systems(iw_solver *s, bool refactor_p, int &first, int &count)
         // band positions outside the matrix are never written by the
         // factorisation and must read as zero in the lane solvers
         HIPCHK(hipMemsetAsync(S.AB, 0, sizeof(double) * cap * S.ldab * s->g.nz, s->stream));
-        const size_t fbn = (size_t)(fb_npf(P.nzloc) + fb_npb(P.nzloc)) * s->g.nz; // double2 per system
+        const size_t fbn = (size_t)(tri_sf(P.nzloc) + tri_sb(P.nzloc, false)) * ns; // double2 per system
         HIPCHK(hipMalloc((void **)&S.FB, sizeof(double) * 2 * cap * fbn));
         s->dev_bytes += (long long)(sizeof(double) * 2 * cap * fbn);
         HIPCHK(hipMalloc((void **)&S.ipiv, sizeof(int) * cap * s->g.nz));
@@ -424,9 +424,9 @@ static int build_systems(iw_solver *s, bool refactor_p, int &first, int &count)
         const int npc = std::max(np, 1);
         if ((size_t)npc > S.cap_p) {
             if (S.PB) hipFree(S.PB);
-            const size_t pbn = (size_t)(pb_nf(P.nzloc) + pb_nb(P.nzloc)) * s->g.nz;
-            HIPCHK(hipMalloc((void **)&S.PB, sizeof(double) * npc * pbn));
-            s->dev_bytes += (long long)(sizeof(double) * npc * pbn);
+            const size_t pbn = (size_t)(tri_sf(P.nzloc) + tri_sb(P.nzloc, true)) * ns; // double2 per system
+            HIPCHK(hipMalloc((void **)&S.PB, sizeof(double) * 2 * npc * pbn));
+            s->dev_bytes += (long long)(sizeof(double) * 2 * npc * pbn);
             S.cap_p = npc;
             refactor_p = true;
         }
@@ -545,6 +545,10 @@ static int ensure_factors(iw_solver *s)
     if (s->factored_bdfv == s->bdfv && s->factored_dt == s->P.dt && s->p_factored) return 0;
     int first, count;
     if (build_systems(s, !s->p_factored, first, count)) return -1;
+    if (!s->sys.tri_bad) {
+        HIPCHK(hipMalloc((void **)&s->sys.tri_bad, sizeof(int)));
+        launch_zero_int(s->sys.tri_bad, 1, s->stream);
+    }
     launch_factor(s->g, s->sys, first, count, s->stream);
     launch_compact(s->g, s->sys, first, count, s->stream);
     HIPCHK(hipGetLastError());
@@ -558,7 +562,12 @@ static int ensure_factors(iw_solver *s)
     HIPCHK(hipStreamSynchronize(s->stream));
     std::vector<int> ms(4 * (size_t)nmode);
     HIPCHK(hipMemcpy(ms.data(), S.modesys, sizeof(int) * ms.size(), hipMemcpyDeviceToHost));
-    bool ok = solve_np_supported(s->P.nzloc, s->P.nsubd);
+    int tri_bad = 0;
+    HIPCHK(hipMemcpy(&tri_bad, S.tri_bad, sizeof(int), hipMemcpyDeviceToHost));
+    // the packed sweeps need every factor in the tri_* pattern; the band
+    // solvers on AB take over otherwise
+    S.tri_ok = tri_bad == 0;
+    bool ok = solve_np_supported(s->P.nzloc, s->P.nsubd) && S.tri_ok;
     for (int q = S.np; q < S.nsys && ok; q++)
         if (info[4 * q + 3] != 0.) ok = false;
     S.np_ok = ok;
@@ -1025,6 +1034,7 @@ extern "C" int iw_destroy(iw_solver *s)
     hipFree(s->sys.AB);
     hipFree(s->sys.FB);
     hipFree(s->sys.PB);
+    hipFree(s->sys.tri_bad);
     hipFree(s->sys.ipiv);
     hipFree(s->sys.coef);
     hipFree(s->sys.sysinfo);
@@ -1473,11 +1483,14 @@ static int calcuvwt_dev(iw_solver *s, int cpen)
     // algorithmic bytes: state columns in and out + every distinct cached factor once
     const double fcol = (double)s->sys.ldab * s->g.nz * 8. + s->g.nz * 4.;
     const double nt = (double)s->g.ntot;
-    if (s->sys.np_tasks) {
+    if (s->sys.np_tasks && s->sys.tri_ok) {
         const char *name[3] = {"p_rhs", "p_solve", "p_project"};
         // u,v,w in, phi out | phi in and out + each P factor once |
         // phi, u,v,w in, u,v,w out
-        const double bytes[3] = {32. * nt, 16. * nt + s->sys.np * fcol, 56. * nt};
+        // the pressure factor's structural nonzeros: per row nl - r below the
+        // diagonal, r + nl + 1 on and above it (2 nl + 1 in all), and the pivot
+        const double fpt = (double)(2 * s->P.nzloc + 1) * s->g.nz * 8. + s->g.nz * 4.;
+        const double bytes[3] = {32. * nt, 16. * nt + s->sys.np * fpt, 56. * nt};
         for (int part = 0; part < 3; part++) {
             Scope sc(s, name[part], bytes[part]);
             launch_pressure_nat(s->g, s->P, s->sys, uvwt, s->grad[0], s->wnl, s->wviscrot, s->P.dt, cpen, part,
@@ -1490,8 +1503,9 @@ static int calcuvwt_dev(iw_solver *s, int cpen)
     if (s->sys.np_ok) {
         {
             // right-hand sides in, solutions out (16 B per point and field)
-            // + each distinct no-interchange factor once ((2 kl + 1) rows)
-            const double fnp = (double)(2 * s->sys.kl + 1) * s->g.nz * 8.;
+            // + each distinct no-interchange factor's structural nonzeros
+            // once (per row nl - r below the diagonal, r + 2 on and above it)
+            const double fnp = (double)(s->P.nzloc + 2) * s->g.nz * 8.;
             Scope sc(s, "solve_nat", 4 * 16. * nt + (s->sys.nsys - s->sys.np) * fnp);
             launch_solve_grp(s->g, s->P, s->sys, uvwt, s->P.dt, s->stream);
         }

@@ -55,13 +55,14 @@ struct SysTables {
     int np = 0; // pressure systems occupy ids 0..np-1
     int ldab = 0, kl = 0, kv = 0;
     double *AB = nullptr;   // nsys * ldab * nz, LAPACK band columns
-    // the parts of each factor column the no-pivot solves read, packed
-    // per system: nz forward segments (rows kv..kv+kl), then nz backward
-    // segments (rows kv-kl-1..kv+1), so both sweeps stream contiguous memory
+    // the structurally nonzero parts of each factor column, packed per
+    // system (tri_* layout): nsubd forward blocks, then nsubd backward
+    // blocks, so both sweeps stream contiguous memory; FB for the systems
+    // without interchanges, PB for the pressure systems (ids < np)
     double *FB = nullptr;
-    // the same for the pressure systems (ids < np, with interchanges): nz
-    // L segments (rows kv..kv+kl), then nz U columns (rows 0..kv+1)
     double *PB = nullptr;
+    int *tri_bad = nullptr; // set by the packing when a factor breaks the tri_* pattern
+    bool tri_ok = false;
     size_t cap_p = 0;
     int *ipiv = nullptr;    // nsys * nz (0-based)
     double *coef = nullptr; // nsys * nsubd * 12 assembly coefficients
@@ -251,12 +252,33 @@ void launch_iface(const Geo &g, FieldBatch b, int *count, hipStream_t st);
 void launch_project(const Geo &g, FieldBatch b, hipStream_t st);
 void launch_factor(const Geo &g, const SysTables &S, int first, int count, hipStream_t st);
 void launch_compact(const Geo &g, const SysTables &S, int first, int count, hipStream_t st);
-// double2 per column of the packed forward / backward segments
-constexpr int fb_npf(int kl) { return (kl + 1) / 2; }
-constexpr int fb_npb(int kl) { return (kl + 3) / 2; }
-// doubles per column of the packed pressure segments
-constexpr int pb_nf(int kl) { return kl + 1; }
-constexpr int pb_nb(int kl) { return 2 * kl + 2; }
+// Packed factors of the z systems, one block of nl rows per subdomain.  The
+// penalty matrix couples a subdomain only to the interface rows of its
+// neighbours, so below the diagonal the L column of local row r holds
+// nl - r nonzeros (the rest of the subdomain and the next one's first row),
+// with or without row interchanges; above it the U column holds r + 1
+// (without interchanges: the rows above in the subdomain and the previous
+// one's last row) or r + nl (with interchanges: the previous subdomain's
+// rows as well).  Each segment is padded to whole double2.
+constexpr int tri_ef(int nl, int r) { return nl - r; }                    // L[1..ef]
+constexpr int tri_eb(int nl, int r, bool piv) { return r + (piv ? nl : 1); } // U[0..eb]
+constexpr int tri_npf(int nl, int r) { return (tri_ef(nl, r) + 1) / 2; }
+constexpr int tri_npb(int nl, int r, bool piv) { return (tri_eb(nl, r, piv) + 2) / 2; }
+constexpr int tri_off_f(int nl, int r)
+{
+    int s = 0;
+    for (int q = 0; q < r; q++) s += tri_npf(nl, q);
+    return s;
+}
+constexpr int tri_off_b(int nl, int r, bool piv)
+{
+    int s = 0;
+    for (int q = 0; q < r; q++) s += tri_npb(nl, q, piv);
+    return s;
+}
+// double2 per subdomain block: forward, backward
+constexpr int tri_sf(int nl) { return tri_off_f(nl, nl); }
+constexpr int tri_sb(int nl, bool piv) { return tri_off_b(nl, nl, piv); }
 void launch_pressure(const Geo &g, const iw_params &P, const SysTables &S, double *const *uvwt,
                      const double *wnl, const double *wviscrot, double dt, int cpen, hipStream_t st);
 void launch_viscous(const Geo &g, const iw_params &P, const SysTables &S, double *const *uvwt, double dt,

@@ -1685,47 +1685,49 @@ void launch_factor(const Geo &g, const SysTables &S, int first, int count, hipSt
                        S.coef, g.D, g.D2, S.AB, S.ipiv, S.sysinfo);
 }
 
-// packed copies of the factor rows the no-pivot solves read (SysTables::FB):
-// one block per system, one thread per column
-__global__ __launch_bounds__(256) void k_compact(const double *__restrict__ AB, double *FB, double *PB, int first,
-                                                 int np, int n, int ldab, int kv, int kl, int npf, int npb)
+// Packed copies of the structurally nonzero factor entries (tri_* layout in
+// SysTables::FB / PB): one block per system, one thread per column.  Entries
+// of rows outside the matrix are stored as zero.  A nonzero outside the
+// pattern, or a pressure interchange beyond the L column's reach, sets *bad:
+// the host then keeps the band solvers that read AB.
+__global__ __launch_bounds__(256) void k_compact(const double *__restrict__ AB, const int *__restrict__ ipiv,
+                                                 double *FB, double *PB, int first, int np, int n, int nl,
+                                                 int ldab, int kv, int kl, int *bad)
 {
     const long sys = first + (long)blockIdx.x;
     const double *ab = AB + sys * n * ldab;
-    if (sys < np) { // pressure system: L rows kv..kv+kl, U rows 0..kv+1 (k_solve_pg)
-        const int nf = pb_nf(kl), nb = pb_nb(kl);
-        double *pf = PB + sys * n * (nf + nb);
-        for (int e = threadIdx.x; e < n * nf; e += blockDim.x) {
-            const int c = e / nf, r = e - c * nf;
-            pf[e] = ab[(long)c * ldab + kv + r];
-        }
-        double *pb = pf + (long)n * nf;
-        for (int e = threadIdx.x; e < n * nb; e += blockDim.x) {
-            const int c = e / nb, r = e - c * nb;
-            pb[e] = ab[(long)c * ldab + r];
-        }
-        return;
+    const int *ip = ipiv + sys * n;
+    const bool piv = sys < np;
+    const int nsub = n / nl;
+    const int sf = tri_sf(nl), sb = tri_sb(nl, piv);
+    double *base = piv ? PB + sys * nsub * 2L * (sf + sb) : FB + sys * nsub * 2L * (sf + sb);
+    bool interchanges = false;
+    if (!piv)
+        for (int c = 0; c < n; c++) interchanges |= ip[c] != c;
+    int err = 0;
+    for (int c = threadIdx.x; c < n; c += blockDim.x) {
+        const int ks = c / nl, r = c - ks * nl;
+        const double *col = ab + (long)c * ldab;
+        const int ef = tri_ef(nl, r), eb = tri_eb(nl, r, piv);
+        double *f = base + 2L * ((long)ks * sf + tri_off_f(nl, r));
+        for (int t = 1; t <= ef; t++) f[t - 1] = (c + t < n) ? col[kv + t] : 0.;
+        if (ef & 1) f[ef] = 0.; // ef entries: pad an odd count
+        double *u = base + 2L * ((long)nsub * sf + (long)ks * sb + tri_off_b(nl, r, piv));
+        for (int t = 0; t <= eb; t++) u[t] = (c - t >= 0) ? col[kv - t] : 0.;
+        if ((eb + 1) & 1) u[eb + 1] = 0.; // eb + 1 entries: pad an odd count
+        if (interchanges) continue;
+        for (int t = ef + 1; t <= kl && c + t < n; t++) err |= col[kv + t] != 0.;
+        for (int t = eb + 1; t <= kv && c - t >= 0; t++) err |= col[kv - t] != 0.;
+        if (piv) err |= (ip[c] - c > ef) || (ip[c] < c);
     }
-    double *fw = FB + sys * n * 2 * (npf + npb);
-    const int nf = 2 * npf, nb = 2 * npb;
-    // consecutive threads copy consecutive packed entries (and nearby rows)
-    for (int e = threadIdx.x; e < n * nf; e += blockDim.x) {
-        const int c = e / nf, r = e - c * nf;
-        fw[e] = (kv + r < ldab) ? ab[(long)c * ldab + kv + r] : 0.;
-    }
-    double *bw = fw + (long)n * nf;
-    for (int e = threadIdx.x; e < n * nb; e += blockDim.x) {
-        const int c = e / nb, r = e - c * nb;
-        bw[e] = ab[(long)c * ldab + kv - kl - 1 + r];
-    }
+    if (err) atomicOr(bad, 1);
 }
 
 void launch_compact(const Geo &g, const SysTables &S, int first, int count, hipStream_t st)
 {
     if (count <= 0) return;
-    const int kl = g.nzloc;
-    hipLaunchKernelGGL(k_compact, dim3(count), dim3(256), 0, st, S.AB, S.FB, S.PB, first, S.np, g.nz, S.ldab,
-                       S.kv, kl, fb_npf(kl), fb_npb(kl));
+    hipLaunchKernelGGL(k_compact, dim3(count), dim3(256), 0, st, S.AB, S.ipiv, S.FB, S.PB, first, S.np, g.nz,
+                       g.nzloc, S.ldab, S.kv, S.kl, S.tri_bad);
 }
 
 // ---------------------------------------------------------------------------
@@ -2212,15 +2214,23 @@ void launch_solve_nat(const Geo &g, const iw_params &P, const SysTables &S, doub
 }
 
 // ---------------------------------------------------------------------------
-// Grouped variant of the same solves.  The G lanes of a group share one
-// system (host layout: +ky and -ky rows of one kx, and for G = 8 both u and
-// v).  Each lane fetches 1/G of the factor segment DP steps ahead into a
-// register ring; one step before use the pieces are put together in LDS and
-// every lane of the group reads the whole segment from there.  This keeps
-// DP steps of distinct factor data in flight per wavefront instead of one
-// step of G identical copies.  Task entry: col | field << 28 | idle << 29
-// (an idle lane helps load its group's factors but owns no column); -1
-// marks an empty group.
+// Band sweeps over the packed factors (tri_* layout), grouped lanes.  Lane =
+// one right-hand-side column on the native layout; the G lanes of a group
+// share one system (host layout: the +ky and -ky rows of one kx, re and im,
+// and for G = 8 both u and v).  Each lane fetches 1/G of a factor segment
+// DP rows ahead into a register ring; one row before use the pieces are put
+// together in LDS and every lane of the group reads the whole segment from
+// there.  The rows of one subdomain are unrolled, so every segment length,
+// ring slot and window update is fixed at compile time and only the
+// structurally nonzero entries are loaded and applied (an entry outside the
+// pattern is an exact zero, and w + 0 * y is w).
+//   forward (bandludcmp.f:1020-1030; SGBTRS with interchanges for the
+//     pressure): window w[t] = row j + t; y = w[0] (after the swap with
+//     w[ip[j] - j]), w[t-1] = w[t] - L[t] y;
+//   backward (STBSV, ludcmp.f:4698-5046): window w[t] = row j - t;
+//     x = w[0] / U[0], w[t-1] = w[t] - x U[t].
+// Task entry: col | field << 28 | idle << 29 (an idle lane helps load its
+// group's factors but owns no column); -1 marks an empty group.
 // ---------------------------------------------------------------------------
 // compile-time loop: f(std::integral_constant<int, I>) for I = B..E-1
 template <int B, int E, class F>
@@ -2231,186 +2241,256 @@ __device__ __forceinline__ void static_for(F &&f)
         static_for<B + 1, E>(f);
     }
 }
+template <int I>
+using IC = std::integral_constant<int, I>;
+// the row updates of a sweep stay in the step that makes them: without this
+// the compiler sinks each update into the step that first reads the value,
+// keeping every pending multiplier of a subdomain in registers
+__device__ __forceinline__ void pin(double &x) { asm volatile("" : "+v"(x)); }
 
-struct GrpArgs {
+struct SweepArgs {
     double *f[4];
     const int *tasks;
     const int *modesys;
-    const double *FB;
+    const double *PF; // FB, or PB for the pressure
+    const int *ipiv;
     const double *sysinfo;
     double den[4];
     long nxppy;
-    int n, ldab, kv, fld_base;
+    int n, fld_base;
 };
 
-template <int KL, int G, int DP>
-__global__ __launch_bounds__(64) void k_solve_grp(GrpArgs A)
+template <int NL, int G, int DP, bool PIV>
+__global__ __launch_bounds__(64) void k_sweep(SweepArgs A)
 {
-    constexpr int NG = 64 / G, PPL = 16 / G;
-    constexpr int NPF = (KL + 1) / 2; // forward segment, rows kv..kv+KL of a column
-    constexpr int NPB = (KL + 3) / 2; // backward segment, rows kv-KL-1..kv+1
-    __shared__ double2 seg[2][NG][17]; // 17: groups start on different banks
+    static_assert(NL % DP == 0, "ring slots follow the local row");
+    static_assert(DP >= 2, "a row is staged one step before use, from a slot filled at least one step earlier");
+    constexpr int NG = 64 / G;
+    constexpr int SF = tri_sf(NL), SB = tri_sb(NL, PIV);
+    constexpr int NPFM = tri_npf(NL, 0), NPBM = tri_npb(NL, NL - 1, PIV);
+    constexpr int NPM = NPFM > NPBM ? NPFM : NPBM;
+    constexpr int PPF = (NPFM + G - 1) / G, PPB = (NPBM + G - 1) / G; // pieces per lane
+    constexpr int EBM = tri_eb(NL, NL - 1, PIV);                     // widest U column
+    __shared__ double2 seg[2][NG][NPM + 1];
     const int lane = threadIdx.x, grp = lane / G, rk = lane % G;
     const int e = A.tasks[blockIdx.x * 64 + lane];
     if (e < 0) return; // the whole group is empty
     const bool idle = (e >> 29) & 1;
     const int col = e & ((1 << 28) - 1);
-    const int fld = A.fld_base + blockIdx.y + ((e >> 28) & 1);
     const int m = col >> 1;
-    if (fld == 2 && m == 0) return; // w(0,0) = 0 (calcuvwt.f:496-503); its group holds only this mode
-    const int sys = A.modesys[4 * m + (fld < 2 ? 1 : fld == 2 ? 2 : 3)];
-    const double eps = A.sysinfo[4 * sys], den = A.den[fld];
-    const int n = A.n, kv = A.kv;
+    int fld, sys;
+    if constexpr (PIV) {
+        fld = 0;
+        if (m == 0) return; // no pressure for kx = ky = 0; its group holds only this mode
+        sys = A.modesys[4 * m];
+    } else {
+        fld = A.fld_base + blockIdx.y + ((e >> 28) & 1);
+        if (fld == 2 && m == 0) return; // w(0,0) = 0 (calcuvwt.f:496-503); its group holds only this mode
+        sys = A.modesys[4 * m + (fld < 2 ? 1 : fld == 2 ? 2 : 3)];
+    }
+    const double eps = PIV ? 1. : A.sysinfo[4 * sys], den = PIV ? 1. : A.den[fld];
+    // right-hand side of the viscous systems: -(-f / den) eps (calcuvwt.f:390-402)
+    auto tr = [&](double v) __attribute__((always_inline)) { return PIV ? v : -(-v / den) * eps; };
+    const int n = A.n, nsub = n / NL;
     const long st = A.nxppy;
-    static_assert(NPF == fb_npf(KL) && NPB == fb_npb(KL), "packed factor layout");
-    const double2 *__restrict__ fb2 = reinterpret_cast<const double2 *>(A.FB) + (long)sys * n * (NPF + NPB);
+    const double2 *__restrict__ fw = reinterpret_cast<const double2 *>(A.PF) + (long)sys * nsub * (SF + SB);
+    const double2 *__restrict__ bw = fw + (long)nsub * SF;
+    const int *__restrict__ ip = A.ipiv + (PIV ? (long)sys * n : 0);
     double *__restrict__ b = A.f[fld] + col;
-    double w[KL + 1];
-    double rx[DP][PPL], ry[DP][PPL]; // ring of factor pieces (re/im halves of 16-byte loads)
-    double bring[DP];
-    // ---- forward: unit lower triangle (bandludcmp.f:1020-1030)
+    // ---- forward
     {
-        const double2 *f0 = fb2; // forward segments, NPF double2 per column
-        auto ldp = [&](int sidx, double(&dx)[PPL], double(&dy)[PPL]) __attribute__((always_inline)) {
-            const long c = sidx < n ? sidx : n - 1;
+        double w[NL + 1];
+        double rx[DP][PPF], ry[DP][PPF];
+        double bring[DP];
+        int pring[DP];
+        auto ldp = [&](auto Rc, int ks, double(&dx)[PPF], double(&dy)[PPF]) __attribute__((always_inline)) {
+            constexpr int R = decltype(Rc)::value, NP = tri_npf(NL, R);
+            const double2 *src = fw + (long)(ks < nsub ? ks : nsub - 1) * SF + tri_off_f(NL, R);
 #pragma unroll
-            for (int q = 0; q < PPL; q++) {
+            for (int q = 0; q < PPF; q++) {
                 const int p = rk + q * G;
-                const double2 v = f0[NPF * c + (p < NPF ? p : 0)];
-                dx[q] = v.x;
-                dy[q] = v.y;
+                if (q * G < NP) {
+                    const double2 v = src[p < NP ? p : 0];
+                    dx[q] = v.x;
+                    dy[q] = v.y;
+                }
             }
         };
-        auto stp = [&](int buf, const double(&sx)[PPL], const double(&sy)[PPL]) __attribute__((always_inline)) {
+        auto stp = [&](auto Rc, int buf, const double(&sx)[PPF], const double(&sy)[PPF]) __attribute__((always_inline)) {
+            constexpr int R = decltype(Rc)::value, NP = tri_npf(NL, R);
 #pragma unroll
-            for (int q = 0; q < PPL; q++) {
+            for (int q = 0; q < PPF; q++) {
                 const int p = rk + q * G;
-                if (p < NPF) seg[buf][grp][p] = make_double2(sx[q], sy[q]);
+                if (q * G < NP && p < NP) seg[buf][grp][p] = make_double2(sx[q], sy[q]);
             }
         };
 #pragma unroll
-        for (int t = 0; t <= KL; t++) {
+        for (int t = 0; t <= NL; t++) {
             const double v = b[(long)(t < n ? t : n - 1) * st];
-            w[t] = (t < n) ? -(-v / den) * eps : 0.;
+            w[t] = (t < n) ? tr(v) : 0.;
         }
         static_for<0, DP>([&](auto dc) __attribute__((always_inline)) {
             constexpr int d = decltype(dc)::value;
-            ldp(d, rx[d], ry[d]);
-            const int r = KL + 1 + d;
+            ldp(IC<d>{}, 0, rx[d], ry[d]);
+            const int r = NL + 1 + d;
             bring[d] = b[(long)(r < n ? r : n - 1) * st];
+            if constexpr (PIV) pring[d] = ip[d < n ? d : n - 1];
         });
-        stp(0, rx[0], ry[0]);
-        for (int j0 = 0; j0 < n; j0 += DP) {
-            static_for<0, DP>([&](auto dc) __attribute__((always_inline)) {
-                constexpr int d = decltype(dc)::value;
-                const int j = j0 + d;
-                if (j < n) {
-                stp((j + 1) & 1, rx[(d + 1) % DP], ry[(d + 1) % DP]);
-                ldp(j + DP, rx[d], ry[d]);
-                double fc[2 * NPF];
+        stp(IC<0>{}, 0, rx[0], ry[0]);
+        for (int ks = 0; ks < nsub; ks++) {
+            static_for<0, NL>([&](auto rc) __attribute__((always_inline)) {
+                constexpr int r = decltype(rc)::value, d = r % DP;
+                constexpr int EF = tri_ef(NL, r), NP = tri_npf(NL, r);
+                const int j = ks * NL + r;
+                if (j >= n) return; // never: one block per row step
+                stp(IC<(r + 1) % NL>{}, (j + 1) & 1, rx[(d + 1) % DP], ry[(d + 1) % DP]);
+                ldp(IC<(r + DP) % NL>{}, ks + (r + DP) / NL, rx[d], ry[d]);
+                double fc[2 * NP]; // fc[t - 1] = L[t]
 #pragma unroll
-                for (int p = 0; p < NPF; p++) {
+                for (int p = 0; p < NP; p++) {
                     const double2 v = seg[j & 1][grp][p];
                     fc[2 * p] = v.x;
                     fc[2 * p + 1] = v.y;
                 }
-                const double rc = bring[d];
-                const int r = j + KL + 1 + DP;
-                bring[d] = b[(long)(r < n ? r : n - 1) * st];
-                const double y = w[0];
-#pragma unroll
-                for (int t = 1; t <= KL; t++) w[t - 1] = w[t] + fc[t] * (-y);
-                w[KL] = (j + KL + 1 < n) ? -(-rc / den) * eps : 0.;
-                if (!idle) b[(long)j * st] = y;
+                const double rc_ = bring[d];
+                {
+                    const int rr = j + NL + 1 + DP;
+                    bring[d] = b[(long)(rr < n ? rr : n - 1) * st];
                 }
+                double y = w[0];
+                if constexpr (PIV) {
+                    const int dd = pring[d] - j; // 0 <= dd <= EF (checked when packed)
+                    {
+                        const int jj = j + DP;
+                        pring[d] = ip[jj < n ? jj : n - 1];
+                    }
+#pragma unroll
+                    for (int t = 1; t <= EF; t++) y = (dd == t) ? w[t] : y;
+                    const double w0 = w[0];
+#pragma unroll
+                    for (int t = 1; t <= NL; t++) {
+                        if (t <= EF) {
+                            const double wt = (dd == t) ? w0 : w[t];
+                            w[t - 1] = wt + fc[t - 1] * (-y);
+                            pin(w[t - 1]);
+                        } else {
+                            w[t - 1] = w[t];
+                        }
+                    }
+                } else {
+#pragma unroll
+                    for (int t = 1; t <= NL; t++) {
+                        w[t - 1] = (t <= EF) ? w[t] + fc[t - 1] * (-y) : w[t];
+                        if (t <= EF) pin(w[t - 1]);
+                    }
+                }
+                w[NL] = (j + NL + 1 < n) ? tr(rc_) : 0.;
+                if (!idle) b[(long)j * st] = y;
             });
         }
     }
-    // ---- backward: upper band with KL superdiagonals (STBSV, ludcmp.f:4698-5046)
+    // ---- backward
     {
-        const double2 *g0 = fb2 + (long)n * NPF; // backward segments, NPB double2 per column
-        auto ldp = [&](int sidx, double(&dx)[PPL], double(&dy)[PPL]) __attribute__((always_inline)) {
-            const long c = sidx < n ? n - 1 - sidx : 0;
+        double w[EBM + 1];
+        double rx[DP][PPB], ry[DP][PPB];
+        double bring[DP];
+        auto ldp = [&](auto Rc, int ks, double(&dx)[PPB], double(&dy)[PPB]) __attribute__((always_inline)) {
+            constexpr int R = decltype(Rc)::value, NP = tri_npb(NL, R, PIV);
+            const double2 *src = bw + (long)(ks >= 0 ? ks : 0) * SB + tri_off_b(NL, R, PIV);
 #pragma unroll
-            for (int q = 0; q < PPL; q++) {
+            for (int q = 0; q < PPB; q++) {
                 const int p = rk + q * G;
-                const double2 v = g0[NPB * c + (p < NPB ? p : 0)];
-                dx[q] = v.x;
-                dy[q] = v.y;
+                if (q * G < NP) {
+                    const double2 v = src[p < NP ? p : 0];
+                    dx[q] = v.x;
+                    dy[q] = v.y;
+                }
             }
         };
-        auto stp = [&](int buf, const double(&sx)[PPL], const double(&sy)[PPL]) __attribute__((always_inline)) {
+        auto stp = [&](auto Rc, int buf, const double(&sx)[PPB], const double(&sy)[PPB]) __attribute__((always_inline)) {
+            constexpr int R = decltype(Rc)::value, NP = tri_npb(NL, R, PIV);
 #pragma unroll
-            for (int q = 0; q < PPL; q++) {
+            for (int q = 0; q < PPB; q++) {
                 const int p = rk + q * G;
-                if (p < NPB) seg[buf][grp][p] = make_double2(sx[q], sy[q]);
+                if (q * G < NP && p < NP) seg[buf][grp][p] = make_double2(sx[q], sy[q]);
             }
         };
 #pragma unroll
-        for (int t = 0; t <= KL; t++) {
+        for (int t = 0; t <= EBM; t++) {
             const int r = n - 1 - t;
             const double v = b[(long)(r >= 0 ? r : 0) * st];
             w[t] = (r >= 0) ? v : 0.;
         }
         static_for<0, DP>([&](auto dc) __attribute__((always_inline)) {
             constexpr int d = decltype(dc)::value;
-            ldp(d, rx[d], ry[d]);
-            const int r = n - 2 - KL - d;
+            ldp(IC<NL - 1 - d>{}, nsub - 1, rx[d], ry[d]);
+            const int r = n - 2 - EBM - d;
             bring[d] = b[(long)(r >= 0 ? r : 0) * st];
         });
-        stp(0, rx[0], ry[0]);
-        for (int s0 = 0; s0 < n; s0 += DP) {
-            static_for<0, DP>([&](auto dc) __attribute__((always_inline)) {
-                constexpr int d = decltype(dc)::value;
-                const int sidx = s0 + d, j = n - 1 - sidx;
-                if (sidx < n) {
-                stp((sidx + 1) & 1, rx[(d + 1) % DP], ry[(d + 1) % DP]);
-                ldp(sidx + DP, rx[d], ry[d]);
-                double gc[2 * NPB];
+        stp(IC<NL - 1>{}, 0, rx[0], ry[0]);
+        for (int ks = nsub - 1; ks >= 0; ks--) {
+            static_for<0, NL>([&](auto qc) __attribute__((always_inline)) {
+                constexpr int q = decltype(qc)::value, r = NL - 1 - q, d = q % DP;
+                constexpr int EB = tri_eb(NL, r, PIV), NP = tri_npb(NL, r, PIV);
+                const int j = ks * NL + r, sidx = n - 1 - j;
+                if (j < 0) return; // always false; one block per row step
+                stp(IC<NL - 1 - (q + 1) % NL>{}, (sidx + 1) & 1, rx[(d + 1) % DP], ry[(d + 1) % DP]);
+                ldp(IC<NL - 1 - (q + DP) % NL>{}, ks - (q + DP) / NL, rx[d], ry[d]);
+                double gc[2 * NP]; // gc[t] = U[t], t = 0 the diagonal
 #pragma unroll
-                for (int p = 0; p < NPB; p++) {
+                for (int p = 0; p < NP; p++) {
                     const double2 v = seg[sidx & 1][grp][p];
                     gc[2 * p] = v.x;
                     gc[2 * p + 1] = v.y;
                 }
-                const double rc = bring[d];
-                const int r = j - KL - 1 - DP;
-                bring[d] = b[(long)(r >= 0 ? r : 0) * st];
-                const double dc = gc[KL + 1];
-                const double x = (w[0] != 0.) ? w[0] / dc : w[0];
-#pragma unroll
-                for (int t = 1; t <= KL; t++) w[t - 1] = w[t] - x * gc[KL + 1 - t];
-                w[KL] = (j - KL - 1 >= 0) ? rc : 0.;
-                if (!idle) b[(long)j * st] = x;
+                const double rc_ = bring[d];
+                {
+                    const int rr = j - EBM - 1 - DP;
+                    bring[d] = b[(long)(rr >= 0 ? rr : 0) * st];
                 }
+                const double x = (w[0] != 0.) ? w[0] / gc[0] : w[0];
+#pragma unroll
+                for (int t = 1; t <= EBM; t++) {
+                    w[t - 1] = (t <= EB) ? w[t] - x * gc[t] : w[t];
+                    if (t <= EB) pin(w[t - 1]);
+                }
+                w[EBM] = (j - EBM - 1 >= 0) ? rc_ : 0.;
+                if (!idle) b[(long)j * st] = x;
             });
         }
     }
 }
 
+// ring depth (rows fetched ahead) per subdomain order: a divisor of nl, as
+// deep as the registers allow
+template <int NL, bool PIV>
+constexpr int sweep_dp()
+{
+    return NL == 25 ? 5 : NL == 21 ? 7 : NL;
+}
+
 void launch_solve_grp(const Geo &g, const iw_params &P, const SysTables &S, double *const *uvwt, double dt,
                       hipStream_t st)
 {
-    GrpArgs A;
+    SweepArgs A;
     for (int q = 0; q < 4; q++) A.f[q] = uvwt[q];
     A.modesys = S.modesys;
-    A.FB = S.FB;
+    A.PF = S.FB;
+    A.ipiv = S.ipiv;
     A.sysinfo = S.sysinfo;
     A.den[0] = A.den[1] = A.den[2] = P.xnu * dt;
     A.den[3] = dt * P.xkappa;
     A.nxppy = g.nxppy;
     A.n = g.nz;
-    A.ldab = S.ldab;
-    A.kv = S.kv;
-    GrpArgs B = A;
+    SweepArgs B = A;
     A.tasks = S.guv_tasks;
     A.fld_base = 0;
     B.tasks = S.gwt_tasks;
     B.fld_base = 2;
     dim3 ga(S.guv_ntasks / 64, 1), gb(S.gwt_ntasks / 64, 2);
-#define GS(KL)                                                                                  \
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_solve_grp<KL, 8, 8>), ga, dim3(64), 0, st, A);         \
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_solve_grp<KL, 4, 4>), gb, dim3(64), 0, st, B)
+#define GS(NL)                                                                                                  \
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_sweep<NL, 8, sweep_dp<NL, false>(), false>), ga, dim3(64), 0, st, A); \
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_sweep<NL, 4, sweep_dp<NL, false>(), false>), gb, dim3(64), 0, st, B)
     switch (g.nzloc) {
     case 9: GS(9); break;
     case 13: GS(13); break;
@@ -2479,8 +2559,8 @@ void launch_lfilter(const Geo &g, double *const *uvwt, int cpen, hipStream_t st)
 //   k_prhs:     divergence (divcolc, derv.f:35-85) of u*, scaled right-hand
 //               side -(div/dt) eps and the Neumann data wnl + wviscrot
 //               (calcuvwt.f:112-159, penalty.f:191-194, 583-598) -> phi;
-//   k_solve_pg: SGBTRS with row interchanges (bandludcmp.f:1020-1036), lane =
-//               one column, groups of lanes sharing a system;
+//   k_sweep<.., true>: SGBTRS with row interchanges (bandludcmp.f:1020-1036),
+//               lane = one column, groups of lanes sharing a system;
 //   k_pproject: Legendre filter of phi, dphi/dz and u* - dt grad(phi).
 // ---------------------------------------------------------------------------
 struct PArgs {
@@ -2543,188 +2623,6 @@ __global__ __launch_bounds__(256) void k_prhs(PArgs A)
         if (kg == 0) r = r - pw0;
         if (kg == A.nz - 1) r = r + pw1;
         phi[(long)k * st] = r;
-    }
-}
-
-// SGBTRS with row interchanges in the grouped layout of k_solve_grp: the G
-// lanes of a group (+ky and -ky rows of one kx, re and im) share a pressure
-// system (it depends on k^2 only) and fetch its packed L row / U column
-// DPF / DPB rows ahead, one share each, assembled in LDS; the pivot row of
-// each step is fetched DPF rows ahead as well.  Per lane: a window w of the
-// right-hand side; each forward step swaps w[0] with w[ip[j] - j], emits
-// w[0] and shifts the window by one with the L multipliers; the backward
-// step divides by the diagonal of U and shifts with U's column.
-#ifndef IW_PG_DPF
-#define IW_PG_DPF 12 // L rows fetched ahead
-#endif
-#ifndef IW_PG_DPB
-#define IW_PG_DPB 6 // U columns fetched ahead
-#endif
-struct SolPGArgs {
-    double *phi;
-    const int *tasks;
-    const int *modesys;
-    const double *PB;
-    const int *ipiv;
-    long nxppy;
-    int n;
-};
-
-template <int KL, int G, int DPF, int DPB>
-__global__ __launch_bounds__(64) void k_solve_pg(SolPGArgs A)
-{
-    static_assert(KL % 2 == 1, "16-byte factor rows need an even row length kl + 1");
-    constexpr int NG = 64 / G;
-    constexpr int NF2 = pb_nf(KL) / 2; // L row as double2
-    constexpr int NB2 = pb_nb(KL) / 2; // U column as double2
-    constexpr int KV = 2 * KL;
-    constexpr int PF = (NF2 + G - 1) / G, PU = (NB2 + G - 1) / G; // pieces per lane
-    __shared__ double2 seg[2][NG][NB2 + 1];
-    const int lane = threadIdx.x, grp = lane / G, rk = lane % G;
-    const int e = A.tasks[blockIdx.x * 64 + lane];
-    if (e < 0) return; // the whole group is empty
-    const bool idle = (e >> 29) & 1;
-    const int col = e & ((1 << 28) - 1);
-    const int m = col >> 1;
-    if (m == 0) return; // no pressure for kx = ky = 0; its group holds only this mode
-    const int sys = A.modesys[4 * m];
-    const int n = A.n;
-    const long st = A.nxppy;
-    const double2 *__restrict__ pf2 = reinterpret_cast<const double2 *>(A.PB) + (long)sys * n * (NF2 + NB2);
-    const double2 *__restrict__ pu2 = pf2 + (long)n * NF2;
-    const int *__restrict__ ip = A.ipiv + (long)sys * n;
-    double *__restrict__ b = A.phi + col;
-    // ---- forward: L with row interchanges (bandludcmp.f:1020-1030)
-    {
-        double w[KL + 1];
-        double rx[DPF][PF], ry[DPF][PF];
-        double bring[DPF];
-        int pring[DPF];
-        auto ldp = [&](int j, double(&dx)[PF], double(&dy)[PF]) __attribute__((always_inline)) {
-            const long c = j < n ? j : n - 1;
-#pragma unroll
-            for (int q = 0; q < PF; q++) {
-                const int p = rk + q * G;
-                const double2 v = pf2[NF2 * c + (p < NF2 ? p : 0)];
-                dx[q] = v.x;
-                dy[q] = v.y;
-            }
-        };
-        auto stp = [&](int buf, const double(&sx)[PF], const double(&sy)[PF]) __attribute__((always_inline)) {
-#pragma unroll
-            for (int q = 0; q < PF; q++) {
-                const int p = rk + q * G;
-                if (p < NF2) seg[buf][grp][p] = make_double2(sx[q], sy[q]);
-            }
-        };
-#pragma unroll
-        for (int t = 0; t <= KL; t++) {
-            const double v = b[(long)(t < n ? t : n - 1) * st];
-            w[t] = (t < n) ? v : 0.;
-        }
-        static_for<0, DPF>([&](auto dc) __attribute__((always_inline)) {
-            constexpr int d = decltype(dc)::value;
-            ldp(d, rx[d], ry[d]);
-            const int r = KL + 1 + d;
-            bring[d] = b[(long)(r < n ? r : n - 1) * st];
-            pring[d] = ip[d < n ? d : n - 1];
-        });
-        stp(0, rx[0], ry[0]);
-        for (int j0 = 0; j0 < n; j0 += DPF) {
-            static_for<0, DPF>([&](auto dc) __attribute__((always_inline)) {
-                constexpr int d = decltype(dc)::value;
-                const int j = j0 + d;
-                if (j < n) {
-                    stp((j + 1) & 1, rx[(d + 1) % DPF], ry[(d + 1) % DPF]);
-                    ldp(j + DPF, rx[d], ry[d]);
-                    double fc[2 * NF2];
-#pragma unroll
-                    for (int p = 0; p < NF2; p++) {
-                        const double2 v = seg[j & 1][grp][p];
-                        fc[2 * p] = v.x;
-                        fc[2 * p + 1] = v.y;
-                    }
-                    const double rc = bring[d];
-                    const int pc = pring[d];
-                    const int r = j + KL + 1 + DPF;
-                    bring[d] = b[(long)(r < n ? r : n - 1) * st];
-                    pring[d] = ip[j + DPF < n ? j + DPF : n - 1];
-                    const int dd = pc - j; // 0 <= dd <= KL
-                    double y = w[0];
-#pragma unroll
-                    for (int t = 1; t <= KL; t++) y = (dd == t) ? w[t] : y;
-                    const double w0 = w[0];
-#pragma unroll
-                    for (int t = 1; t <= KL; t++) {
-                        const double wt = (dd == t) ? w0 : w[t];
-                        w[t - 1] = wt + fc[t] * (-y);
-                    }
-                    w[KL] = (j + KL + 1 < n) ? rc : 0.;
-                    if (!idle) b[(long)j * st] = y;
-                }
-            });
-        }
-    }
-    // ---- backward: U with kv = 2 kl superdiagonals (STBSV, ludcmp.f:4698-5046)
-    {
-        double w[KV + 1];
-        double rx[DPB][PU], ry[DPB][PU];
-        double bring[DPB];
-        auto ldp = [&](int sidx, double(&dx)[PU], double(&dy)[PU]) __attribute__((always_inline)) {
-            const long c = sidx < n ? n - 1 - sidx : 0;
-#pragma unroll
-            for (int q = 0; q < PU; q++) {
-                const int p = rk + q * G;
-                const double2 v = pu2[NB2 * c + (p < NB2 ? p : 0)];
-                dx[q] = v.x;
-                dy[q] = v.y;
-            }
-        };
-        auto stp = [&](int buf, const double(&sx)[PU], const double(&sy)[PU]) __attribute__((always_inline)) {
-#pragma unroll
-            for (int q = 0; q < PU; q++) {
-                const int p = rk + q * G;
-                if (p < NB2) seg[buf][grp][p] = make_double2(sx[q], sy[q]);
-            }
-        };
-#pragma unroll
-        for (int t = 0; t <= KV; t++) {
-            const int r = n - 1 - t;
-            const double v = b[(long)(r >= 0 ? r : 0) * st];
-            w[t] = (r >= 0) ? v : 0.;
-        }
-        static_for<0, DPB>([&](auto dc) __attribute__((always_inline)) {
-            constexpr int d = decltype(dc)::value;
-            ldp(d, rx[d], ry[d]);
-            const int r = n - 2 - KV - d;
-            bring[d] = b[(long)(r >= 0 ? r : 0) * st];
-        });
-        stp(0, rx[0], ry[0]);
-        for (int s0 = 0; s0 < n; s0 += DPB) {
-            static_for<0, DPB>([&](auto dc) __attribute__((always_inline)) {
-                constexpr int d = decltype(dc)::value;
-                const int sidx = s0 + d, j = n - 1 - sidx;
-                if (sidx < n) {
-                    stp((sidx + 1) & 1, rx[(d + 1) % DPB], ry[(d + 1) % DPB]);
-                    ldp(sidx + DPB, rx[d], ry[d]);
-                    const double2 *gs = seg[sidx & 1][grp];
-                    const double rc = bring[d];
-                    const int r = j - KV - 1 - DPB;
-                    bring[d] = b[(long)(r >= 0 ? r : 0) * st];
-                    const double2 dd = gs[KV / 2];
-                    const double dc = dd.x; // gc[KV] (KV even)
-                    const double x = (w[0] != 0.) ? w[0] / dc : w[0];
-#pragma unroll
-                    for (int t = 1; t <= KV; t++) {
-                        const int gi = KV - t;
-                        const double2 gv = gs[gi >> 1];
-                        w[t - 1] = w[t] - x * ((gi & 1) ? gv.y : gv.x);
-                    }
-                    w[KV] = (j - KV - 1 >= 0) ? rc : 0.;
-                    if (!idle) b[(long)j * st] = x;
-                }
-            });
-        }
     }
 }
 
@@ -2812,14 +2710,16 @@ void launch_pressure_nat(const Geo &g, const iw_params &P, const SysTables &S, d
     A.nxhp = g.nxhp;
     A.nz = g.nz;
     dim3 gc((unsigned)((g.nxppy + 255) / 256), g.nsubd);
-    SolPGArgs C;
-    C.phi = phi; C.tasks = S.gwt_tasks; C.modesys = S.modesys; C.PB = S.PB; C.ipiv = S.ipiv;
-    C.nxppy = g.nxppy; C.n = g.nz;
+    SweepArgs C;
+    C.f[0] = C.f[1] = C.f[2] = C.f[3] = phi;
+    C.tasks = S.gwt_tasks; C.modesys = S.modesys; C.PF = S.PB; C.ipiv = S.ipiv; C.sysinfo = S.sysinfo;
+    C.den[0] = C.den[1] = C.den[2] = C.den[3] = 1.;
+    C.nxppy = g.nxppy; C.n = g.nz; C.fld_base = 0;
     dim3 gg(S.gwt_ntasks / 64);
     const int ns = P.bc_uv_bottom == 1;
 #define PSTEP(NL)                                                                                         \
     if (part == 0) hipLaunchKernelGGL(k_prhs<NL>, gc, dim3(256), 0, st, A);                               \
-    if (part == 1) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_solve_pg<NL, 4, IW_PG_DPF, IW_PG_DPB>), gg, dim3(64), 0, st, C);  \
+    if (part == 1) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_sweep<NL, 4, sweep_dp<NL, true>(), true>), gg, dim3(64), 0, st, C);  \
     if (part == 2)                                                                                        \
         hipLaunchKernelGGL(k_pproject<NL>, gc, dim3(256), 0, st, A, uvwt[0], uvwt[1], uvwt[2], cpen, ns, g.FL)
     switch (g.nzloc) {
