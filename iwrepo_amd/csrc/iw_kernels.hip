@@ -1686,40 +1686,83 @@ void launch_factor(const Geo &g, const SysTables &S, int first, int count, hipSt
 }
 
 // Packed copies of the structurally nonzero factor entries (tri_* layout in
-// SysTables::FB / PB): one block per system, one thread per column.  Entries
-// of rows outside the matrix are stored as zero.  A nonzero outside the
-// pattern, or a pressure interchange beyond the L column's reach, sets *bad:
-// the host then keeps the band solvers that read AB.
+// SysTables::FB / PB): one block per system; an LDS map from a packed
+// position of a subdomain block to its (local row, t) lets consecutive
+// threads write consecutive entries.  Entries of rows outside the matrix are
+// stored as zero.  A nonzero outside the pattern, or a pressure interchange
+// beyond the L column's reach, sets *bad: the host then keeps the band
+// solvers that read AB.
 __global__ __launch_bounds__(256) void k_compact(const double *__restrict__ AB, const int *__restrict__ ipiv,
                                                  double *FB, double *PB, int first, int np, int n, int nl,
                                                  int ldab, int kv, int kl, int *bad)
 {
+    // packed double -> (local row, t, part) of one subdomain block, t < 0 a pad
+    __shared__ short map_r[2 * 1024], map_t[2 * 1024];
     const long sys = first + (long)blockIdx.x;
     const double *ab = AB + sys * n * ldab;
     const int *ip = ipiv + sys * n;
     const bool piv = sys < np;
     const int nsub = n / nl;
-    const int sf = tri_sf(nl), sb = tri_sb(nl, piv);
+    const int sf = tri_sf(nl), sb = tri_sb(nl, piv), blk = 2 * (sf + sb);
     double *base = piv ? PB + sys * nsub * 2L * (sf + sb) : FB + sys * nsub * 2L * (sf + sb);
-    bool interchanges = false;
-    if (!piv)
-        for (int c = 0; c < n; c++) interchanges |= ip[c] != c;
-    int err = 0;
-    for (int c = threadIdx.x; c < n; c += blockDim.x) {
-        const int ks = c / nl, r = c - ks * nl;
-        const double *col = ab + (long)c * ldab;
-        const int ef = tri_ef(nl, r), eb = tri_eb(nl, r, piv);
-        double *f = base + 2L * ((long)ks * sf + tri_off_f(nl, r));
-        for (int t = 1; t <= ef; t++) f[t - 1] = (c + t < n) ? col[kv + t] : 0.;
-        if (ef & 1) f[ef] = 0.; // ef entries: pad an odd count
-        double *u = base + 2L * ((long)nsub * sf + (long)ks * sb + tri_off_b(nl, r, piv));
-        for (int t = 0; t <= eb; t++) u[t] = (c - t >= 0) ? col[kv - t] : 0.;
-        if ((eb + 1) & 1) u[eb + 1] = 0.; // eb + 1 entries: pad an odd count
-        if (interchanges) continue;
-        for (int t = ef + 1; t <= kl && c + t < n; t++) err |= col[kv + t] != 0.;
-        for (int t = eb + 1; t <= kv && c - t >= 0; t++) err |= col[kv - t] != 0.;
-        if (piv) err |= (ip[c] - c > ef) || (ip[c] < c);
+    if (blk > 2 * 1024) { // orders beyond the map (nl > 32): band solvers only
+        if (threadIdx.x == 0) atomicOr(bad, 1);
+        return;
     }
+    for (int r = threadIdx.x; r < nl; r += blockDim.x) {
+        const int ef = tri_ef(nl, r), eb = tri_eb(nl, r, piv);
+        const int fo = 2 * tri_off_f(nl, r), bo = 2 * (sf + tri_off_b(nl, r, piv));
+        for (int q = 0; q < 2 * tri_npf(nl, r); q++) {
+            map_r[fo + q] = (short)r;
+            map_t[fo + q] = (short)(q < ef ? q + 1 : -1); // L[t] at t - 1
+        }
+        for (int q = 0; q < 2 * tri_npb(nl, r, piv); q++) {
+            map_r[bo + q] = (short)r;
+            map_t[bo + q] = (short)(q <= eb ? 1000 + q : -1); // U[t] at t
+        }
+    }
+    __syncthreads();
+    // consecutive threads write consecutive packed entries: block layout is
+    // nsub forward blocks then nsub backward blocks
+    for (long e = threadIdx.x; e < (long)nsub * blk; e += blockDim.x) {
+        const bool fwd = e < 2L * nsub * sf;
+        const long o = fwd ? e : e - 2L * nsub * sf;
+        const int per = fwd ? 2 * sf : 2 * sb;
+        const int ks = (int)(o / per), q = (int)(o - (long)ks * per) + (fwd ? 0 : 2 * sf);
+        const int r = map_r[q], tt = map_t[q], c = ks * nl + r;
+        double v = 0.;
+        if (tt > 0 && tt < 1000) {
+            if (c + tt < n) v = ab[(long)c * ldab + kv + tt];
+        } else if (tt >= 1000) {
+            const int t = tt - 1000;
+            if (c - t >= 0) v = ab[(long)c * ldab + kv - t];
+        }
+        base[e] = v;
+    }
+    // pattern check over the band (coalesced): zero outside it, and for the
+    // pressure every interchange within the L column's reach
+    int inter = 0;
+    if (!piv)
+        for (int c = threadIdx.x; c < n; c += blockDim.x) inter |= ip[c] != c;
+    if (__syncthreads_or(inter)) return; // such a system sends the solves to the band path anyway
+    int err = 0;
+    for (long e = threadIdx.x; e < (long)n * ldab; e += blockDim.x) {
+        const int c = (int)(e / ldab), row = (int)(e - (long)c * ldab), r = c % nl;
+        const double v = ab[e];
+        if (row > kv) { // L[t], t = row - kv
+            const int t = row - kv;
+            err |= (t > tri_ef(nl, r) && c + t < n && v != 0.);
+        } else { // U[t], t = kv - row
+            const int t = kv - row;
+            err |= (t > tri_eb(nl, r, piv) && c - t >= 0 && v != 0.);
+        }
+    }
+    if (piv)
+        for (int c = threadIdx.x; c < n; c += blockDim.x) {
+            const int d = ip[c] - c;
+            err |= (d < 0) || (d > tri_ef(nl, c % nl));
+        }
+    (void)kl;
     if (err) atomicOr(bad, 1);
 }
 
@@ -2466,6 +2509,7 @@ __global__ __launch_bounds__(64) void k_sweep(SweepArgs A)
 template <int NL, bool PIV>
 constexpr int sweep_dp()
 {
+    // nl = 25: 5 rows ahead measured faster than 25 (which halves occupancy)
     return NL == 25 ? 5 : NL == 21 ? 7 : NL;
 }
 
