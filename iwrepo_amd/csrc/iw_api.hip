@@ -552,9 +552,9 @@ static int ensure_factors(iw_solver *s)
     launch_factor(s->g, s->sys, first, count, s->stream);
     launch_compact(s->g, s->sys, first, count, s->stream);
     HIPCHK(hipGetLastError());
-    // columns of systems factored without row interchanges go to the
-    // lanes-as-columns solver; if any u/v, w or density system pivoted, all
-    // of them take the wavefront path
+    // the u/v, w and density systems go to the packed-factor sweeps (lane =
+    // one column); if any of them pivoted, or a factor breaks the packed
+    // pattern, all of them take the wavefront path
     SysTables &S = s->sys;
     const int nmode = s->g.nxhp * s->g.ny;
     std::vector<double> info((size_t)S.nsys * 4);
@@ -571,76 +571,11 @@ static int ensure_factors(iw_solver *s)
     for (int q = S.np; q < S.nsys && ok; q++)
         if (info[4 * q + 3] != 0.) ok = false;
     S.np_ok = ok;
-    if (solve_np_supported(s->P.nzloc, s->P.nsubd) && !S.np_tasks) {
-        // lane order of the native-layout solver: a wavefront takes 16
-        // consecutive kx of the rows +ky and -ky (equal k^2, so equal
-        // systems; each row segment is 4 whole 128-byte lines), then the
-        // self-paired rows ky = 0 and ky = -ny/2, then the kx-Nyquist column
+    if (solve_np_supported(s->P.nzloc, s->P.nsubd) && !S.gwt_tasks) {
         const Geo &g = s->g;
-        std::vector<int> cols;
-        auto push = [&](int jy, int ii) {
-            const int m = jy * g.nxhp + ii;
-            cols.push_back(2 * m);
-            cols.push_back(2 * m + 1);
-        };
-        const int B = 16;
-        for (int p = 1; p < (g.ny + 1) / 2; p++) {
-            const int q = g.ny - p;
-            if (q == p) continue;
-            for (int i0 = 0; i0 < g.nxh; i0 += B) {
-                for (int ii = i0; ii < std::min(i0 + B, g.nxh); ii++) push(p, ii);
-                for (int ii = i0; ii < std::min(i0 + B, g.nxh); ii++) push(q, ii);
-            }
-        }
+        // rows ky = 0 and ky = -ny/2 pair with themselves
         std::vector<int> single = {0};
         if (g.ny > 1) single.push_back(g.ny / 2);
-        for (int jy : single)
-            for (int ii = 0; ii < g.nxh; ii++) push(jy, ii);
-        for (int jy : single) push(jy, g.nxh);
-        for (int p = 1; p < (g.ny + 1) / 2; p++) {
-            const int q = g.ny - p;
-            if (q == p) continue;
-            push(p, g.nxh);
-            push(q, g.nxh);
-        }
-        if ((long)cols.size() != g.nxppy) return fail("internal: solver lane table does not cover the plane");
-        std::vector<int> tasks((cols.size() + 63) / 64 * 64, -1);
-        std::copy(cols.begin(), cols.end(), tasks.begin());
-        HIPCHK(hipMalloc((void **)&S.np_tasks, sizeof(int) * tasks.size()));
-        HIPCHK(hipMemcpy(S.np_tasks, tasks.data(), sizeof(int) * tasks.size(), hipMemcpyHostToDevice));
-        S.np_ntasks = (int)tasks.size();
-        // u and v share their systems: a wavefront takes 8 kx of the rows
-        // +ky and -ky for both fields, so 8 lanes use each factor segment
-        std::vector<int> uv;
-        auto push_uv = [&](int jy, int ii) {
-            const int m = jy * g.nxhp + ii;
-            for (int f = 0; f < 2; f++)
-                for (int c = 0; c < 2; c++) uv.push_back((2 * m + c) | (f << 28));
-        };
-        const int B8 = 8;
-        for (int p = 1; p < (g.ny + 1) / 2; p++) {
-            const int q = g.ny - p;
-            if (q == p) continue;
-            for (int i0 = 0; i0 < g.nxh; i0 += B8) {
-                for (int ii = i0; ii < std::min(i0 + B8, g.nxh); ii++) push_uv(p, ii);
-                for (int ii = i0; ii < std::min(i0 + B8, g.nxh); ii++) push_uv(q, ii);
-            }
-        }
-        for (int jy : single)
-            for (int ii = 0; ii < g.nxh; ii++) push_uv(jy, ii);
-        for (int jy : single) push_uv(jy, g.nxh);
-        for (int p = 1; p < (g.ny + 1) / 2; p++) {
-            const int q = g.ny - p;
-            if (q == p) continue;
-            push_uv(p, g.nxh);
-            push_uv(q, g.nxh);
-        }
-        if ((long)uv.size() != 2 * g.nxppy) return fail("internal: u/v lane table does not cover the plane");
-        std::vector<int> uvt((uv.size() + 63) / 64 * 64, -1);
-        std::copy(uv.begin(), uv.end(), uvt.begin());
-        HIPCHK(hipMalloc((void **)&S.uv_tasks, sizeof(int) * uvt.size()));
-        HIPCHK(hipMemcpy(S.uv_tasks, uvt.data(), sizeof(int) * uvt.size(), hipMemcpyHostToDevice));
-        S.uv_ntasks = (int)uvt.size();
         // grouped layouts: G lanes per system, groups of one kx on the rows
         // +ky and -ky (u and v together for G = 8); rows without a partner
         // fill their group with idle lanes
@@ -1041,8 +976,6 @@ extern "C" int iw_destroy(iw_solver *s)
     hipFree(s->sys.modesys);
     hipFree(s->sys.order_p);
     hipFree(s->sys.order_v);
-    hipFree(s->sys.np_tasks);
-    hipFree(s->sys.uv_tasks);
     hipFree(s->sys.guv_tasks);
     hipFree(s->sys.gwt_tasks);
     hipEventDestroy(s->ev0);
@@ -1483,7 +1416,7 @@ static int calcuvwt_dev(iw_solver *s, int cpen)
     // algorithmic bytes: state columns in and out + every distinct cached factor once
     const double fcol = (double)s->sys.ldab * s->g.nz * 8. + s->g.nz * 4.;
     const double nt = (double)s->g.ntot;
-    if (s->sys.np_tasks && s->sys.tri_ok) {
+    if (s->sys.gwt_tasks && s->sys.tri_ok) {
         const char *name[3] = {"p_rhs", "p_solve", "p_project"};
         // u,v,w in, phi out | phi in and out + each P factor once |
         // phi, u,v,w in, u,v,w out

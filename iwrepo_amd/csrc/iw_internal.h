@@ -73,12 +73,9 @@ struct SysTables {
     // from that XCD's L2 once for the whole group
     int *order_p = nullptr, *order_v = nullptr;
     int nblk_p = 0, nblk_v = 0;
-    // lanes-as-columns path for systems without row interchanges
-    int *np_tasks = nullptr;  // lane -> column (2 m + comp), -1 idle
-    int np_ntasks = 0;
-    int *uv_tasks = nullptr;  // lane -> column | field << 28 for the shared u/v systems
-    int uv_ntasks = 0;
-    int *guv_tasks = nullptr; // grouped layouts (8 lanes per u/v system, 4 per w/rho system)
+    // lane tables of the packed-factor sweeps (lane = one column): 8 lanes
+    // per u/v system, 4 per w/rho (and pressure) system
+    int *guv_tasks = nullptr;
     int guv_ntasks = 0;
     int *gwt_tasks = nullptr;
     int gwt_ntasks = 0;
@@ -286,8 +283,6 @@ void launch_viscous(const Geo &g, const iw_params &P, const SysTables &S, double
 void launch_zero_meanw(const Geo &g, double *w, double *w00, hipStream_t st);
 void launch_zero_int(int *p, int n, hipStream_t st);
 bool solve_np_supported(int nl, int nsubd);
-void launch_solve_nat(const Geo &g, const iw_params &P, const SysTables &S, double *const *uvwt, double dt,
-                      hipStream_t st);
 void launch_lfilter(const Geo &g, double *const *uvwt, int cpen, hipStream_t st);
 // estimation-model filter: modecut of f into out on levels kmin..kmax (u0 scratch)
 void launch_modecut(const Geo &g, const double *f, double *u0, double *out, int kmin, int kmax, hipStream_t st);

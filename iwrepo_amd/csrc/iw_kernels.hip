@@ -2096,167 +2096,6 @@ void launch_viscous(const Geo &g, const iw_params &P, const SysTables &S, double
 }
 
 // ---------------------------------------------------------------------------
-// Solves without row interchanges: the u/v, w and density operators are
-// diagonally dominant, so SGBTF2 never swaps rows for them (ipiv(j) = j) and
-// U keeps ku = kl superdiagonals.  Lane = one right-hand-side column
-// (re or im of one mode) of one field, read and written in place at
-// f[k*nxppy + col]; the host orders the lanes so that a wavefront holds 16
-// consecutive kx of the rows +ky and -ky (modes with equal k^2, hence equal
-// systems), so every row access is 4 contiguous 128-byte lines and lanes
-// sharing a system fetch the same factor segment.  The right-hand side
-// -(-f/(nu dt)) eps (calcuvwt.f:278-281, 612-615, penalty.f:191-194) is
-// formed as each row enters the band window.  Factor segments are read as
-// 16-byte loads one step ahead of use; band positions outside the matrix
-// are zero in AB, so no masking of the factors is needed.
-// ---------------------------------------------------------------------------
-struct NatArgs {
-    double *f[4];
-    const int *tasks;
-    const int *uv_tasks;
-    int ntask_blocks;
-    const int *modesys;
-    const double *AB;
-    const double *sysinfo;
-    double den[4];
-    long nxppy;
-    int n, ldab, kv;
-};
-
-template <int NV>
-__device__ __forceinline__ void ld_seg(const double *__restrict__ p, double (&c)[NV])
-{
-    const double2 *q = reinterpret_cast<const double2 *>(p);
-#pragma unroll
-    for (int i = 0; i < NV / 2; i++) {
-        const double2 v = q[i];
-        c[2 * i] = v.x;
-        c[2 * i + 1] = v.y;
-    }
-}
-
-template <int KL>
-__global__ __launch_bounds__(64) void k_solve_nat(NatArgs A)
-{
-    // blockIdx.y = 0: u and v lanes from the shared table; 1: w; 2: rho
-    int col, fld;
-    if (blockIdx.y == 0) {
-        const int e = A.uv_tasks[blockIdx.x * 64 + threadIdx.x];
-        if (e < 0) return;
-        col = e & ((1 << 28) - 1);
-        fld = e >> 28;
-    } else {
-        if ((int)blockIdx.x >= A.ntask_blocks) return;
-        col = A.tasks[blockIdx.x * 64 + threadIdx.x];
-        fld = blockIdx.y + 1;
-    }
-    if (col < 0) return;
-    const int m = col >> 1;
-    if (fld == 2 && m == 0) return; // w(0,0) = 0 (calcuvwt.f:496-503)
-    const int sys = A.modesys[4 * m + (fld < 2 ? 1 : fld == 2 ? 2 : 3)];
-    const double eps = A.sysinfo[4 * sys], den = A.den[fld];
-    const int n = A.n, kv = A.kv, ldab = A.ldab;
-    const long st = A.nxppy;
-    const double *__restrict__ ab = A.AB + (long)sys * ldab * n;
-    double *__restrict__ b = A.f[fld] + col;
-    constexpr int NF = KL + 1; // forward segment: rows kv..kv+KL of column j
-    constexpr int NB = KL + 3; // backward segment: rows kv-KL-1..kv+1 of column j
-    double w[KL + 1];
-    // ---- forward: unit lower triangle (bandludcmp.f:1020-1030); the factor
-    // segment and the entering right-hand-side row of step j+1 are loaded
-    // before the arithmetic of step j
-#pragma unroll
-    for (int t = 0; t <= KL; t++) {
-        const double v = b[(long)(t < n ? t : n - 1) * st];
-        w[t] = (t < n) ? -(-v / den) * eps : 0.;
-    }
-    {
-        double f0[NF], f1[NF];
-        double r0, r1;
-        ld_seg<NF>(ab + kv, f0);
-        r0 = b[(long)(KL + 1 < n ? KL + 1 : n - 1) * st];
-        int j = 0;
-        auto step = [&](double (&fc)[NF], double (&fn)[NF], double &rc, double &rn) {
-            const int jp = j + 1 < n ? j + 1 : n - 1;
-            ld_seg<NF>(ab + (long)ldab * jp + kv, fn);
-            const int rp = j + KL + 2 < n ? j + KL + 2 : n - 1;
-            rn = b[(long)rp * st];
-            const double y = w[0];
-#pragma unroll
-            for (int t = 1; t <= KL; t++) w[t - 1] = w[t] + fc[t] * (-y);
-            w[KL] = (j + KL + 1 < n) ? -(-rc / den) * eps : 0.;
-            b[(long)j * st] = y;
-            j++;
-        };
-        while (j < n) {
-            step(f0, f1, r0, r1);
-            if (j >= n) break;
-            step(f1, f0, r1, r0);
-        }
-    }
-    // ---- backward: upper band with KL superdiagonals (STBSV, ludcmp.f:4698-5046)
-#pragma unroll
-    for (int t = 0; t <= KL; t++) {
-        const int r = n - 1 - t;
-        const double v = b[(long)(r >= 0 ? r : 0) * st];
-        w[t] = (r >= 0) ? v : 0.;
-    }
-    {
-        double g0[NB], g1[NB];
-        double r0, r1;
-        const int o = kv - KL - 1;
-        ld_seg<NB>(ab + (long)ldab * (n - 1) + o, g0);
-        r0 = b[(long)(n - 2 - KL >= 0 ? n - 2 - KL : 0) * st];
-        int j = n - 1;
-        auto step = [&](double (&gc)[NB], double (&gn)[NB], double &rc, double &rn) {
-            const int jp = j - 1 >= 0 ? j - 1 : 0;
-            ld_seg<NB>(ab + (long)ldab * jp + o, gn);
-            const int rp = j - KL - 2 >= 0 ? j - KL - 2 : 0;
-            rn = b[(long)rp * st];
-            const double dc = gc[KL + 1];
-            const double x = (w[0] != 0.) ? w[0] / dc : w[0];
-#pragma unroll
-            for (int t = 1; t <= KL; t++) w[t - 1] = w[t] - x * gc[KL + 1 - t];
-            w[KL] = (j - KL - 1 >= 0) ? rc : 0.;
-            b[(long)j * st] = x;
-            j--;
-        };
-        while (j >= 0) {
-            step(g0, g1, r0, r1);
-            if (j < 0) break;
-            step(g1, g0, r1, r0);
-        }
-    }
-}
-
-void launch_solve_nat(const Geo &g, const iw_params &P, const SysTables &S, double *const *uvwt, double dt,
-                      hipStream_t st)
-{
-    NatArgs A;
-    for (int q = 0; q < 4; q++) A.f[q] = uvwt[q];
-    A.tasks = S.np_tasks;
-    A.uv_tasks = S.uv_tasks;
-    A.ntask_blocks = S.np_ntasks / 64;
-    A.modesys = S.modesys;
-    A.AB = S.AB;
-    A.sysinfo = S.sysinfo;
-    A.den[0] = A.den[1] = A.den[2] = P.xnu * dt;
-    A.den[3] = dt * P.xkappa;
-    A.nxppy = g.nxppy;
-    A.n = g.nz;
-    A.ldab = S.ldab;
-    A.kv = S.kv;
-    dim3 grid(S.uv_ntasks / 64, 3);
-    switch (g.nzloc) {
-    case 9: hipLaunchKernelGGL(k_solve_nat<9>, grid, dim3(64), 0, st, A); break;
-    case 13: hipLaunchKernelGGL(k_solve_nat<13>, grid, dim3(64), 0, st, A); break;
-    case 17: hipLaunchKernelGGL(k_solve_nat<17>, grid, dim3(64), 0, st, A); break;
-    case 21: hipLaunchKernelGGL(k_solve_nat<21>, grid, dim3(64), 0, st, A); break;
-    case 25: hipLaunchKernelGGL(k_solve_nat<25>, grid, dim3(64), 0, st, A); break;
-    default: break;
-    }
-}
-
-// ---------------------------------------------------------------------------
 // Band sweeps over the packed factors (tri_* layout), grouped lanes.  Lane =
 // one right-hand-side column on the native layout; the G lanes of a group
 // share one system (host layout: the +ky and -ky rows of one kx, re and im,
