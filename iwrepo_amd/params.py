@@ -112,6 +112,11 @@ CONFIGS = {
     # GLL order 10: its u,v,w,rho solves take the one-wavefront-per-system path
     "small11": dict(params=dict(nx=16, ny=8, nsubd=3, order=10, re=1e3, dt=2e-3),
                     wave=0.05, noise=1e-3, steps=10, seed=9),
+    # GLL orders 12 and 20 (13 and 21 points per subdomain) on the packed-factor sweeps
+    "small13": dict(params=dict(nx=16, ny=8, nsubd=3, order=12, re=1e3, dt=2e-3, noslip=True),
+                    wave=0.05, noise=1e-3, steps=10, seed=10),
+    "small21": dict(params=dict(nx=16, ny=8, nsubd=2, order=20, re=1e3, dt=2e-3),
+                    wave=0.05, noise=1e-3, steps=10, seed=11),
     "c1ns": dict(params=dict(nx=64, ny=16, nsubd=4, order=16, xlen=2 * math.pi, ylen=math.pi / 2,
                              re=1e4, pr=1.0, dt=1e-3, noslip=True), wave=0.05, noise=1e-4,
                  steps=200, seed=3),

@@ -76,8 +76,8 @@ def _run_pair(rt, O, P, flds, steps):
     return co, cg, so, sg, o.time()
 
 
-@pytest.mark.parametrize("name,steps", [("small", 10), ("smallns", 10), ("small11", 10), ("c1", 200),
-                                        ("c1ns", 200)])
+@pytest.mark.parametrize("name,steps", [("small", 10), ("smallns", 10), ("small11", 10), ("small13", 10),
+                                        ("small21", 10), ("c1", 200), ("c1ns", 200)])
 def test_step_matches_oracle(rt, oracle_mod, name, steps):
     P, flds = case(name)
     print("case", name, steps, flush=True)
