@@ -2228,6 +2228,10 @@ __global__ __launch_bounds__(64) void k_sweep(SweepArgs A)
                 stp(IC<(r + 1) % NL>{}, (j + 1) & 1, rx[(d + 1) % DP], ry[(d + 1) % DP]);
                 ldp(IC<(r + DP) % NL>{}, ks + (r + DP) / NL, rx[d], ry[d]);
                 double fc[2 * NP]; // fc[t - 1] = L[t]
+                // the other lanes' pieces were stored one step earlier; keep the
+                // compiler from moving these reads above any LDS store (the
+                // wave executes its LDS operations in order)
+                asm volatile("" ::: "memory");
 #pragma unroll
                 for (int p = 0; p < NP; p++) {
                     const double2 v = seg[j & 1][grp][p];
@@ -2319,6 +2323,7 @@ __global__ __launch_bounds__(64) void k_sweep(SweepArgs A)
                 stp(IC<NL - 1 - (q + 1) % NL>{}, (sidx + 1) & 1, rx[(d + 1) % DP], ry[(d + 1) % DP]);
                 ldp(IC<NL - 1 - (q + DP) % NL>{}, ks - (q + DP) / NL, rx[d], ry[d]);
                 double gc[2 * NP]; // gc[t] = U[t], t = 0 the diagonal
+                asm volatile("" ::: "memory");
 #pragma unroll
                 for (int p = 0; p < NP; p++) {
                     const double2 v = seg[sidx & 1][grp][p];
