@@ -357,6 +357,11 @@ static int build_systems(iw_solver *s, bool refactor_p, int &first, int &count)
     const iw_params &P = s->P;
     const int nxhp = s->g.nxhp, ny = s->g.ny, nmode = nxhp * ny, ns = P.nsubd;
     const double facvel = 1. / (P.xnu * P.dt), fact = 1. / (P.xkappa * P.dt);
+    // the tables below are rewritten by synchronous copies on the null
+    // stream, which does not wait for the solver's non-blocking stream: let
+    // the sweeps of the previous step (readers of sysinfo, modesys and the
+    // factors) finish first
+    HIPCHK(hipStreamSynchronize(s->stream));
     const int ubot = P.bc_uv_bottom == 1 ? BC_DIR : BC_NEU;
     typedef std::tuple<unsigned long long, int, int> Key;
     std::map<Key, int> idx;
@@ -545,10 +550,10 @@ static int ensure_factors(iw_solver *s)
     if (s->factored_bdfv == s->bdfv && s->factored_dt == s->P.dt && s->p_factored) return 0;
     int first, count;
     if (build_systems(s, !s->p_factored, first, count)) return -1;
-    if (!s->sys.tri_bad) {
-        HIPCHK(hipMalloc((void **)&s->sys.tri_bad, sizeof(int)));
-        launch_zero_int(s->sys.tri_bad, 1, s->stream);
-    }
+    if (!s->sys.tri_bad) HIPCHK(hipMalloc((void **)&s->sys.tri_bad, 2 * sizeof(int)));
+    // pattern flags of this packing: [0] pressure factors (only when they are
+    // packed again), [1] the u/v, w and density factors
+    launch_zero_int(first == 0 ? s->sys.tri_bad : s->sys.tri_bad + 1, first == 0 ? 2 : 1, s->stream);
     launch_factor(s->g, s->sys, first, count, s->stream);
     launch_compact(s->g, s->sys, first, count, s->stream);
     HIPCHK(hipGetLastError());
@@ -562,12 +567,14 @@ static int ensure_factors(iw_solver *s)
     HIPCHK(hipStreamSynchronize(s->stream));
     std::vector<int> ms(4 * (size_t)nmode);
     HIPCHK(hipMemcpy(ms.data(), S.modesys, sizeof(int) * ms.size(), hipMemcpyDeviceToHost));
-    int tri_bad = 0;
-    HIPCHK(hipMemcpy(&tri_bad, S.tri_bad, sizeof(int), hipMemcpyDeviceToHost));
-    // the packed sweeps need every factor in the tri_* pattern; the band
-    // solvers on AB take over otherwise
-    S.tri_ok = tri_bad == 0;
-    bool ok = solve_np_supported(s->P.nzloc, s->P.nsubd) && S.tri_ok;
+    int tri_bad[2] = {0, 0};
+    HIPCHK(hipMemcpy(tri_bad, S.tri_bad, 2 * sizeof(int), hipMemcpyDeviceToHost));
+    // the packed sweeps need every factor of their kind in the tri_*
+    // pattern; the band solvers on AB take over otherwise, for the pressure
+    // and the other systems separately
+    S.tri_ok_p = tri_bad[0] == 0;
+    S.tri_ok_v = tri_bad[1] == 0;
+    bool ok = solve_np_supported(s->P.nzloc, s->P.nsubd) && S.tri_ok_v;
     for (int q = S.np; q < S.nsys && ok; q++)
         if (info[4 * q + 3] != 0.) ok = false;
     S.np_ok = ok;
@@ -726,6 +733,10 @@ extern "C" int iw_create(const iw_params *p, iw_solver **out)
     if (P.dt <= 0) return fail("dt must be positive");
     if (P.ncalclegspec > 0 && P.ny < 2)
         return fail("ncalclegspec needs ny >= 2 (speclegencalc reads the row j = ny/2)");
+    // the estimation filter runs inside iw_run every nstepfilter steps: an
+    // order it lacks must fail here, not after steps of the run have advanced
+    if (P.ifilter == 1 && !solve_np_supported(P.nzloc, P.nsubd))
+        return fail("ifilter = 1: estimation filter GLL order must be one of 8, 12, 16, 20, 24");
     iw_solver *s = new iw_solver();
     s->P = P;
     Geo &g = s->g;
@@ -1416,7 +1427,7 @@ static int calcuvwt_dev(iw_solver *s, int cpen)
     // algorithmic bytes: state columns in and out + every distinct cached factor once
     const double fcol = (double)s->sys.ldab * s->g.nz * 8. + s->g.nz * 4.;
     const double nt = (double)s->g.ntot;
-    if (s->sys.gwt_tasks && s->sys.tri_ok) {
+    if (s->sys.gwt_tasks && s->sys.tri_ok_p) {
         const char *name[3] = {"p_rhs", "p_solve", "p_project"};
         // u,v,w in, phi out | phi in and out + each P factor once |
         // phi, u,v,w in, u,v,w out

@@ -61,8 +61,10 @@ struct SysTables {
     // without interchanges, PB for the pressure systems (ids < np)
     double *FB = nullptr;
     double *PB = nullptr;
-    int *tri_bad = nullptr; // set by the packing when a factor breaks the tri_* pattern
-    bool tri_ok = false;
+    // set by the packing when a factor breaks the tri_* pattern: [0] the
+    // pressure factors, [1] the others
+    int *tri_bad = nullptr;
+    bool tri_ok_p = false, tri_ok_v = false;
     size_t cap_p = 0;
     int *ipiv = nullptr;    // nsys * nz (0-based)
     double *coef = nullptr; // nsys * nsubd * 12 assembly coefficients

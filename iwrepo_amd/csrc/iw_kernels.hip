@@ -1690,8 +1690,8 @@ void launch_factor(const Geo &g, const SysTables &S, int first, int count, hipSt
 // position of a subdomain block to its (local row, t) lets consecutive
 // threads write consecutive entries.  Entries of rows outside the matrix are
 // stored as zero.  A nonzero outside the pattern, or a pressure interchange
-// beyond the L column's reach, sets *bad: the host then keeps the band
-// solvers that read AB.
+// beyond the L column's reach, sets bad[0] (pressure factors) or bad[1] (the
+// others): the host then keeps the band solvers that read AB for that kind.
 __global__ __launch_bounds__(256) void k_compact(const double *__restrict__ AB, const int *__restrict__ ipiv,
                                                  double *FB, double *PB, int first, int np, int n, int nl,
                                                  int ldab, int kv, int kl, int *bad)
@@ -1706,7 +1706,7 @@ __global__ __launch_bounds__(256) void k_compact(const double *__restrict__ AB, 
     const int sf = tri_sf(nl), sb = tri_sb(nl, piv), blk = 2 * (sf + sb);
     double *base = piv ? PB + sys * nsub * 2L * (sf + sb) : FB + sys * nsub * 2L * (sf + sb);
     if (blk > 2 * 1024) { // orders beyond the map (nl > 32): band solvers only
-        if (threadIdx.x == 0) atomicOr(bad, 1);
+        if (threadIdx.x == 0) atomicOr(bad + (piv ? 0 : 1), 1);
         return;
     }
     for (int r = threadIdx.x; r < nl; r += blockDim.x) {
@@ -1763,7 +1763,7 @@ __global__ __launch_bounds__(256) void k_compact(const double *__restrict__ AB, 
             err |= (d < 0) || (d > tri_ef(nl, c % nl));
         }
     (void)kl;
-    if (err) atomicOr(bad, 1);
+    if (err) atomicOr(bad + (piv ? 0 : 1), 1);
 }
 
 void launch_compact(const Geo &g, const SysTables &S, int first, int count, hipStream_t st)
@@ -2130,6 +2130,18 @@ using IC = std::integral_constant<int, I>;
 // keeping every pending multiplier of a subdomain in registers
 __device__ __forceinline__ void pin(double &x) { asm volatile("" : "+v"(x)); }
 
+// Lanes of one wavefront exchange data through LDS: a release/acquire fence
+// at wavefront scope with a wave barrier between them orders this lane's
+// earlier LDS stores before the other lanes' later loads.  A single
+// wavefront's LDS instructions execute in order, so the fences need no wait
+// counter and only fix the compiler's order.
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 struct SweepArgs {
     double *f[4];
     const int *tasks;
@@ -2228,10 +2240,9 @@ __global__ __launch_bounds__(64) void k_sweep(SweepArgs A)
                 stp(IC<(r + 1) % NL>{}, (j + 1) & 1, rx[(d + 1) % DP], ry[(d + 1) % DP]);
                 ldp(IC<(r + DP) % NL>{}, ks + (r + DP) / NL, rx[d], ry[d]);
                 double fc[2 * NP]; // fc[t - 1] = L[t]
-                // the other lanes' pieces were stored one step earlier; keep the
-                // compiler from moving these reads above any LDS store (the
-                // wave executes its LDS operations in order)
-                asm volatile("" ::: "memory");
+                // the other lanes' pieces were stored one step earlier by this
+                // wavefront: order those stores before these reads
+                wave_sync();
 #pragma unroll
                 for (int p = 0; p < NP; p++) {
                     const double2 v = seg[j & 1][grp][p];
@@ -2323,7 +2334,7 @@ __global__ __launch_bounds__(64) void k_sweep(SweepArgs A)
                 stp(IC<NL - 1 - (q + 1) % NL>{}, (sidx + 1) & 1, rx[(d + 1) % DP], ry[(d + 1) % DP]);
                 ldp(IC<NL - 1 - (q + DP) % NL>{}, ks - (q + DP) / NL, rx[d], ry[d]);
                 double gc[2 * NP]; // gc[t] = U[t], t = 0 the diagonal
-                asm volatile("" ::: "memory");
+                wave_sync();
 #pragma unroll
                 for (int p = 0; p < NP; p++) {
                     const double2 v = seg[sidx & 1][grp][p];

@@ -104,6 +104,15 @@ CONFIGS = {
     "c5": dict(params=dict(nx=512, ny=256, nsubd=32, order=24, xlen=4 * math.pi,
                            ylen=2 * math.pi, re=2e5, pr=7.0, dt=2.5e-4, noslip=True),
                wave=0.2, noise=1e-3, steps=500, seed=5),
+    # learned-closure fine run: 256x256x(16x25), Re 1e5, wave amplitude 0.15,
+    # 2000 steps (box, bottom condition, Pr and dt as in the single large instance)
+    "c4f": dict(params=dict(nx=256, ny=256, nsubd=16, order=24, xlen=4 * math.pi,
+                            ylen=2 * math.pi, re=1e5, pr=7.0, dt=5e-4, noslip=True),
+                wave=0.15, noise=1e-3, steps=2000, seed=4),
+    # its coarse counterpart: 64x64x(8x13)
+    "c4c": dict(params=dict(nx=64, ny=64, nsubd=8, order=12, xlen=4 * math.pi,
+                            ylen=2 * math.pi, re=1e5, pr=7.0, dt=5e-4, noslip=True),
+                wave=0.15, noise=1e-3, steps=2000, seed=4),
     # small cases for fast parity tests
     "small": dict(params=dict(nx=16, ny=8, nsubd=3, order=8, re=1e3, dt=2e-3),
                   wave=0.05, noise=1e-3, steps=10, seed=7),

@@ -145,6 +145,9 @@ class Solver:
         _check(self.L.iw_run(self.h, nsteps, c.ctypes.data_as(ctypes.c_void_p) if counts else None))
         return c[:4 * nsteps].reshape(nsteps, 4)
 
+    def set_dt(self, dt):
+        _check(self.L.iw_set_dt(self.h, ctypes.c_double(dt)))
+
     def sync(self):
         _check(self.L.iw_sync(self.h))
 

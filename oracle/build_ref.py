@@ -126,6 +126,7 @@ def build(name, nx, ny, nzloc, nsubd, noslip=False, ref_dim=False, jobs=8, shim=
     print("built", exe)
     if shim:
         build_shim(bdir, src, ho, objs, noslip)
+        build_shimdrv(name, src, noslip)
     return exe
 
 
@@ -171,6 +172,44 @@ def build_shim(bdir, src, ho, objs, noslip):
     link([FLANG] + FFLAGS + [ho, so] + objs +
          ["-o", exe, "-Wl,--allow-multiple-definition", "-L" + PKG, "-liwgpu",
           "-Wl,-rpath,$ORIGIN/" + rel, "-Wl,-rpath,/opt/rocm/lib"])
+    print("built", exe)
+    return exe
+
+
+SHIMOUT = os.path.join(HERE, "_shim")
+
+
+def build_shimdrv(name, src, noslip):
+    """The harness linked with the product's Fortran shims and
+    ``shimdrv_support.f`` only: no object of the reference is in the link
+    (its headers give the COMMON layout the shims are compiled against, as
+    in any caller's build).  Output ``oracle/_shim/<name>/shimdrv``, which
+    the GPU tests run against the oracle."""
+    lib = os.path.join(PKG, "libiwgpu.so")
+    if not os.path.exists(lib):
+        print("libiwgpu.so not built; skipping shim driver")
+        return None
+    odir = os.path.join(SHIMOUT, name)
+    os.makedirs(odir, exist_ok=True)
+    cmd = [FLANG] + FFLAGS + ["-cpp", "-DIW_SHIMDRV", "-DIW_MAXCALC", "-DIW_POSTP", "-I", src, "-J", odir]
+    if noslip:
+        cmd.append("-DIW_NOSLIP")
+    objs = []
+    for f in (os.path.join(PKG, "fortran", "iwshim.F"), os.path.join(HERE, "shimdrv_support.f"),
+              os.path.join(HERE, "ref_harness.f")):
+        o = os.path.join(odir, os.path.basename(f).split(".")[0] + ".o")
+        subprocess.run(cmd + ["-c", f, "-o", o], check=True)
+        objs.append(o)
+    exe = os.path.join(odir, "shimdrv")
+    rel = os.path.relpath(PKG, odir)
+    r = subprocess.run([FLANG] + FFLAGS + objs + ["-o", exe, "-L" + PKG, "-liwgpu",
+                                                  "-Wl,-rpath,$ORIGIN/" + rel, "-Wl,-rpath,/opt/rocm/lib"],
+                       stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout.decode())
+        raise SystemExit("shim driver link failed")
+    for o in objs:
+        os.remove(o)
     print("built", exe)
     return exe
 

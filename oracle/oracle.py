@@ -13,6 +13,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "_ref", "liboracle.so")
 REFDIR = os.path.join(HERE, "_ref")
+SHIMDIR = os.path.join(HERE, "_shim")
 
 _lib = None
 NREC = 42  # maxcalc record length (IWO_NREC)
@@ -229,6 +230,12 @@ class Oracle:
 # ---------------------------------------------------------------------------
 # the compiled reference (oracle/_ref/<build>/iwref), driven by ref_harness.f
 # ---------------------------------------------------------------------------
+def shim_available(build):
+    """True when the shim driver (the harness with the product's Fortran
+    shims and no reference object, oracle/build_ref.py) is built."""
+    return os.path.exists(os.path.join(SHIMDIR, build, "shimdrv"))
+
+
 def ref_available(build):
     return os.path.exists(os.path.join(REFDIR, build, "iwref"))
 
@@ -339,7 +346,7 @@ def run_ref(build, P, fields, nsteps=1, mode=0, probe=None, exe="iwref", isgs=0,
     physical fields (appended last).
     Returns (state[24], t, diag_lines[, restart_bytes][, maxcalc_outputs]
     [, energy])."""
-    exe = os.path.join(REFDIR, build, exe)
+    exe = os.path.join(SHIMDIR if exe == "shimdrv" else REFDIR, build, exe)
     nz, ny, nxpp = P.nzloc * P.nsubd, P.ny, P.nx + 2
     with tempfile.TemporaryDirectory() as d:
         if start is not None:

@@ -62,3 +62,12 @@ def test_initial_condition_is_seeded_and_continuous():
         assert np.all(f[:, :, P.nx:] == 0.0)  # pads
     # SplitMix64 known answer for seed 0 (first output of the published generator)
     assert int(splitmix64(0, 1)[0]) == 0xE220A8397B1DCDAF
+
+
+def test_create_rejects_estimation_filter_at_an_order_it_lacks():
+    """ifilter = 1 at a GLL order without the estimation-filter kernels fails
+    at creation (before any step), with a message naming the filter."""
+    P = config_params("small")
+    P.nzloc, P.ifilter, P.nstepfilter = 11, 1, 2
+    with pytest.raises(runtime.IWError, match="estimation filter"):
+        runtime.Solver(P)

@@ -177,38 +177,3 @@ def test_kepe_on_device(rt, oracle_mod, name):
         assert np.array_equal(g.get_state(q), spec[q])
     g.close()
 
-
-def test_fortran_postp_and_speclegen_shims_against_reference(oracle_mod):
-    """postp and speclegencalc reached through the Fortran shims inside the
-    reference step loop (speclegencalc every 2 steps, the filter and postp
-    every 3, plotf every 2), against the harness built from the reference
-    alone: the state, the /energetics/ values the shim leaves in COMMON
-    and the speclegen*.dat files; then postp alone (harness mode 6) with
-    the SGS tensors."""
-    import os
-    for exe in ("iwref", "iwshim"):
-        if not os.path.exists(os.path.join(oracle_mod.REFDIR, "small", exe)):
-            pytest.skip("small/%s not built (built where the reference sources are)" % exe)
-    if not oracle_mod.ref_has_postp("small"):
-        pytest.skip("reference build small without postp.f")
-    P, flds = case("small")
-    P.ifilter, P.nstepfilter, P.npostp, P.n3doutput, P.ncalclegspec = 1, 3, 3, 2, 2
-    print("shim diagnostics loop", flush=True)
-    sr, tr, _, er, lr = oracle_mod.run_ref("small", P, flds, nsteps=6, energy=True, speclegen=True)
-    ss, ts, _, es, ls = oracle_mod.run_ref("small", P, flds, nsteps=6, energy=True, speclegen=True, exe="iwshim")
-    for q in SOLUTION:
-        assert rel(ss[q], sr[q]) < (FIELD_TOL if q < 12 else HIST_TOL), STATE[q]
-    assert er.shape == es.shape == (2, 18)
-    for r in range(2):
-        assert energy_close(es[r, 2:], er[r, 2:], 0) < 1e-9, r
-    assert sorted(lr) == sorted(ls) and len(lr) == 12
-    for k in lr:
-        a, b = ls[k], lr[k]
-        # both rounded to E12.6: at most one unit of the sixth digit apart
-        assert np.all(np.abs(a - b) <= 1.1e-5 * np.abs(b) + 1e-10 * np.abs(b).max()), k
-    _, spec = routine_inputs(oracle_mod, P, flds)
-    sr, _, _, er = oracle_mod.run_ref("small", P, spec, nsteps=0, mode=6, isgs=1, energy=True)
-    ss, _, _, es = oracle_mod.run_ref("small", P, spec, nsteps=0, mode=6, isgs=1, energy=True, exe="iwshim")
-    assert energy_close(es[0, 2:], er[0, 2:], 0) < 1e-11
-    for q in range(4):
-        assert rel(ss[q], sr[q]) < 1e-13, q
