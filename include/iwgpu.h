@@ -155,6 +155,13 @@ int iw_speclegen(iw_solver *s, int idx, double *out);
  * integrcol (derv.f:479-512) per subdomain, PE without the tool's double
  * count of i = 1 (integration.f:221).  The state is not changed. */
 int iw_kepe(iw_solver *s, double *out);
+/* iw_kepe without allocating: no counterpart in the reference.  For callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  The per-level partial
+ * sums live in a static device array of 8192 doubles per GPU (nz up to
+ * 2047); larger nz is rejected.  Concurrent calls on one GPU share that
+ * array and race.  Same result as iw_kepe. */
+int iw_kepe_noalloc(iw_solver *s, double *out);
 /* The records of the last iw_run (or of iw_load_restart): postp records of
  * 18 doubles (istep, t, the sixteen values) and speclegencalc records of
  * 2 + 4*nz doubles (istep, t, the u, v, w, temp spectra).  Copies n records

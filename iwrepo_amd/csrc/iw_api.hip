@@ -10,6 +10,7 @@
 #include <cstring>
 #include <algorithm>
 #include <map>
+#include <mutex>
 #include <set>
 #include <string>
 #include <tuple>
@@ -721,6 +722,8 @@ static void prof_resolve(iw_solver *s)
 // ---------------------------------------------------------------------------
 // creation
 // ---------------------------------------------------------------------------
+static int scratch_init(int dev);
+
 extern "C" int iw_create(const iw_params *p, iw_solver **out)
 {
     if (!p || !out) return fail("null argument");
@@ -757,6 +760,10 @@ extern "C" int iw_create(const iw_params *p, iw_solver **out)
     g.logNy = ly;
     g.scaley = 1. / (double)P.ny;
     HIPCHK(hipGetDevice(&s->device));
+    if (scratch_init(s->device)) {
+        delete s;
+        return -1;
+    }
     HIPCHK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
     setup_tables(s);
     long long &acc = s->dev_bytes;
@@ -1734,6 +1741,58 @@ extern "C" int iw_kepe(iw_solver *s, double *out)
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(out, s->kp + 4 * (size_t)g.nz, 4 * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// KE/PE without allocation.  The per-level partial sums and the four results
+// live in a file-scope device array (one copy per GPU, held until the process
+// ends); its address on each device is looked up once, at the first
+// iw_create on that device, and only that address reaches kernels and
+// copies.  Every slot the kernels read is written by them first in the same
+// call, so nothing needs clearing.  Calls that overlap on one GPU share the
+// array and race.
+// ---------------------------------------------------------------------------
+constexpr int kKepeScratch = 8192; // doubles: 4 nz + 4 for nz <= 2047
+static __device__ __align__(256) double scratch[kKepeScratch];
+constexpr int kMaxDev = 64;
+static double *g_scratch_ptr[kMaxDev] = {};
+static std::mutex g_scratch_mu;
+
+static int scratch_init(int dev)
+{
+    if (dev < 0 || dev >= kMaxDev) return fail("device index beyond the scratch table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[dev]) return 0;
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) return fail(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = static_cast<double *>(ptr);
+    return 0;
+}
+
+extern "C" int iw_kepe_noalloc(iw_solver *s, double *out)
+{
+    if (!s || !out) return fail("iw_kepe_noalloc: null argument");
+    const Geo &g = s->g;
+    if (4 * (long)g.nz + 4 > kKepeScratch)
+        return fail("iw_kepe_noalloc: nz = " + std::to_string(g.nz) + " needs " + std::to_string(4 * g.nz + 4) +
+                    " doubles of scratch, beyond the static " + std::to_string(kKepeScratch));
+    double *ptr = (s->device >= 0 && s->device < kMaxDev) ? g_scratch_ptr[s->device] : nullptr;
+    if (!ptr) return fail("iw_kepe_noalloc: no scratch address for this device");
+    FieldBatch b = batch({s->buf[0], s->buf[1], s->buf[2], s->buf[3]});
+    for (int q = 0; q < 4; q++) b.g[q] = s->phys[q];
+    fft_dev(s, b, 1, g.nz);
+    const iw_params &P = s->P;
+    const double span = P.xlen * (g.ny > 1 ? P.ylen : 1.0);
+    const double cpe = 1. / 2. / P.rho0 * P.grav * P.grav / (P.brunt * P.brunt);
+    {
+        Scope sc(s, "kepe");
+        launch_kepe(g, s->phys, P.rho0 / 2., cpe, span, ptr, ptr + 4 * (size_t)g.nz, s->stream);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, ptr + 4 * (size_t)g.nz, 4 * sizeof(double), hipMemcpyDeviceToHost, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
     return 0;
 }

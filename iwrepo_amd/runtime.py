@@ -22,7 +22,7 @@ EXPORTS = [
     "iw_save_restart", "iw_load_restart", "iw_write_restart_file", "iw_read_restart_file", "iw_abvar1", "iw_abvar2", "iw_abvar3", "iw_bdfvarv",
     "iw_bdfvar1", "iw_bdfvar2", "iw_bdfvar3", "iw_set_profiling", "iw_kernel_stats",
     "iw_device_bytes", "iw_maxcalc", "iw_maxcalc_records", "iw_get_dtstate", "iw_set_dtstate",
-    "iw_set_meanwave", "iw_get_meanwave", "iw_postp", "iw_speclegen", "iw_set_sgs", "iw_kepe", "iw_energy_records",
+    "iw_set_meanwave", "iw_get_meanwave", "iw_postp", "iw_speclegen", "iw_set_sgs", "iw_kepe", "iw_kepe_noalloc", "iw_energy_records",
     "iw_speclegen_records",
 ]
 
@@ -80,6 +80,7 @@ def lib():
     L.iw_postp.argtypes = [P, P]
     L.iw_set_sgs.argtypes = [P, I, P]
     L.iw_kepe.argtypes = [P, P]
+    L.iw_kepe_noalloc.argtypes = [P, P]
     L.iw_speclegen.argtypes = [P, I, P]
     L.iw_energy_records.argtypes = [P, P, I]
     L.iw_speclegen_records.argtypes = [P, P, I]
