@@ -169,6 +169,16 @@ int iw_kepe_noalloc(iw_solver *s, double *out);
 int iw_energy_records(iw_solver *s, double *out, int n);
 int iw_speclegen_records(iw_solver *s, double *out, int n);
 
+/* Mixed precision (no counterpart in the reference, whose arithmetic is FP64
+ * throughout).  on = 1: the physical-space part of convec (solver.f:6-514,
+ * 523-1206) -- the inverse and forward horizontal transforms, the z
+ * derivatives of the fluxes and of u, v, w (on FP32 matrix cores) and the
+ * nonlinear products -- keeps its working set in FP32; the state, the SS/BDF
+ * assembly, the filters, the penalty solves and the projection stay FP64.
+ * 3-D runs without maxcalc records, adaptive dt or mean-wave terms; orders
+ * 8, 12, 16, 20, 24.  on = 0: the plain FP64 path (the default). */
+int iw_set_mixed(iw_solver *s, int on);
+
 /* ---- reference routines on host arrays (parity / Fortran-shim mode) ---- */
 /* horfft.f:2    subroutine horfft(b,is,nk): is=-1 physical->spectral */
 int iw_horfft(iw_solver *s, double *b, int is, int nk);

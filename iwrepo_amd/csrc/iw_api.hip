@@ -1254,6 +1254,7 @@ static void mc_record(iw_solver *s, int istep, double cny)
 }
 
 static int mc_after(iw_solver *s, int slot);
+static int convec_tail(iw_solver *s, int ispecfilternl);
 
 // convec (solver.f:6-514) on the device state.  mcstep > 0: maxcalc of that
 // step (its record from convec's inverse transforms, then the adaptive step
@@ -1391,6 +1392,18 @@ static int convec_dev(iw_solver *s, int ispecfilternl, int mcstep = 0, int mcslo
         Scope sc(s, "rhs_spec", 120. * nt);
         launch_rhs_spec(g, S, pr, s->stream);
     }
+    return convec_tail(s, ispecfilternl);
+}
+
+// The spectral part of convec after the right-hand sides S are formed: the
+// round trip of u,v,w,temp, the pressure BC terms, SS/BDF assembly with the
+// filters, history rotation.
+static int convec_tail(iw_solver *s, int ispecfilternl)
+{
+    const Geo &g = s->g;
+    double **B = s->buf;
+    double *uvwt[4] = {B[0], B[1], B[2], B[3]};
+    const double nt = (double)g.ntot;
     // u,v,w,temp back to spectral space (solver.f:1180, 1201-1203): their
     // physical values were not changed, so the round trip reduces to its
     // spectral effect on the kx=0 and kx-Nyquist columns
@@ -1425,6 +1438,89 @@ static int convec_dev(iw_solver *s, int ispecfilternl, int mcstep = 0, int mcslo
         B[12 + q] = fr;
     }
     return hipGetLastError() == hipSuccess ? 0 : fail("convec launch failed");
+}
+
+// convec on the mixed path: the same sequence as convec_dev (3-D, no
+// maxcalc reductions, no mean-wave terms) with the physical-space working set
+// in FP32.  The inverse y pass reads the FP64 spectral state and writes FP32;
+// the x passes, the z contraction (FP32 matrix cores) and the products run on
+// FP32 arrays; the forward y pass writes the FP64 spectral right-hand sides.
+// From there on (round-trip projection, pressure BC terms, SS/BDF assembly,
+// filters and the implicit step) everything is FP64 as in the plain path.
+// The FP32 arrays live in the FP64 scratch of the plain path: phys[q] holds
+// the physical field (first half) and its right-hand side (second half),
+// grad[q], prod[q] and ox one FP32 array each.
+static int convec_mixed(iw_solver *s, int ispecfilternl)
+{
+    const Geo &g = s->g;
+    double **B = s->buf;
+    double *uvwt[4] = {B[0], B[1], B[2], B[3]};
+    double *S[4] = {B[12], B[13], B[14], B[15]};
+    float *ph[4], *S32[4], *g32[9], *pr32[7];
+    for (int q = 0; q < 4; q++) {
+        ph[q] = reinterpret_cast<float *>(s->phys[q]);
+        S32[q] = ph[q] + g.ntot;
+    }
+    for (int q = 0; q < 9; q++) g32[q] = reinterpret_cast<float *>(s->grad[q]);
+    for (int q = 0; q < 7; q++) pr32[q] = reinterpret_cast<float *>(s->prod[q]);
+    float *ox32 = reinterpret_cast<float *>(s->ox);
+    auto D = [](float *p) { return reinterpret_cast<double *>(p); };
+    const double nt = (double)g.ntot, nc = (double)g.nxh * g.ny * g.nz;
+    {
+        Scope sc(s, "zero_meanw");
+        launch_zero_meanw(g, B[2], s->mc_w00, s->stream);
+    }
+    {   // u,v,w,temp (FP64 spectral) -> FP32, with i ky f of u,v,w
+        FieldBatch by = batch({B[0], B[1], B[2], B[3]});
+        for (int q = 0; q < 4; q++) by.g[q] = D(ph[q]);
+        by.dy[0] = D(g32[1]);
+        by.dy[1] = D(g32[4]);
+        by.dy[2] = D(g32[7]);
+        by.kw = g.yw;
+        Scope sc(s, "m_fft_y_inv", (4 * 16. + 7 * 8.) * nc);
+        launch_fft_y(g, by, g.nz, 1, s->stream, kPrecF32);
+    }
+    {   // x pass in place on the seven FP32 arrays, with i kx f of u,v,w
+        FieldBatch bx = batch({D(ph[0]), D(ph[1]), D(ph[2]), D(ph[3]), D(g32[1]), D(g32[4]), D(g32[7])});
+        bx.dx[0] = D(g32[0]);
+        bx.dx[1] = D(g32[3]);
+        bx.dx[2] = D(g32[6]);
+        bx.kw = g.xw;
+        Scope sc(s, "m_fft_x_inv", 17 * 4. * (double)g.nxppy * g.nz);
+        launch_fft_x(g, bx, g.nz, 1, s->stream, nullptr, kPrecF32);
+    }
+    {   // vertical fluxes and z gradients on the matrix cores
+        float *o4[4] = {S32[0], S32[1], S32[2], ox32};
+        float *dz3[3] = {g32[2], g32[5], g32[8]};
+        Scope sc(s, "m_dpdz_mfma", 11 * 4. * nt);
+        if (!launch_dpdz_conv_grad_mfma(g, ph, o4, dz3, s->stream))
+            return fail("mixed path: no matrix-core z contraction for this GLL order");
+    }
+    {
+        Scope sc(s, "m_rhs_phys", 28 * 4. * nt);
+        launch_rhs_phys_f32(g, s->P, s->t, ph, g32, S32, ox32, pr32, s->stream);
+    }
+    {   // forward x pass: as convec_dev, on FP32 arrays
+        FieldBatch bx = batch({D(S32[0]), D(S32[1]), D(S32[2]), D(S32[3]), D(pr32[2]), D(pr32[4]), D(pr32[6])});
+        const int ia[4] = {0, 1, 3, 5};
+        for (int q = 0; q < 4; q++) bx.ax[q] = D(pr32[ia[q]]);
+        bx.axo[1] = D(g32[0]);
+        for (int q = 1; q < 4; q++) bx.g[3 + q] = D(g32[q]);
+        bx.kw = g.xw;
+        Scope sc(s, "m_fft_x_fwd", 19 * 4. * (double)g.nxppy * g.nz);
+        launch_fft_x(g, bx, g.nz, -1, s->stream, nullptr, kPrecF32);
+    }
+    {   // forward y pass: FP32 in, FP64 spectral right-hand sides out
+        FieldBatch by = batch({D(S32[0]), D(S32[1]), D(S32[2]), D(S32[3])});
+        for (int q = 0; q < 4; q++) {
+            by.g[q] = S[q];
+            by.ay[q] = D(g32[q]);
+        }
+        by.kw = g.yw;
+        Scope sc(s, "m_fft_y_fwd", (8 * 8. + 4 * 16.) * nc);
+        launch_fft_y(g, by, g.nz, -1, s->stream, kPrecF32);
+    }
+    return convec_tail(s, ispecfilternl);
 }
 
 static int calcuvwt_dev(iw_solver *s, int cpen)
@@ -1794,6 +1890,22 @@ extern "C" int iw_kepe_noalloc(iw_solver *s, double *out)
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(out, ptr + 4 * (size_t)g.nz, 4 * sizeof(double), hipMemcpyDeviceToHost, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
+    return 0;
+}
+
+extern "C" int iw_set_mixed(iw_solver *s, int on)
+{
+    if (!s) return fail("iw_set_mixed: null solver");
+    if (on) {
+        const Geo &g = s->g;
+        if (g.ndims != 3) return fail("iw_set_mixed: the mixed path is 3-D only (ny >= 2)");
+        if (!solve_np_supported(g.nzloc, g.nsubd))
+            return fail("iw_set_mixed: GLL order must be one of 8, 12, 16, 20, 24");
+        if (s->P.imaxcalc || s->P.dtadapt)
+            return fail("iw_set_mixed: set imaxcalc = 0 and dtadapt = 0 (no maxcalc records on the mixed path)");
+        if (mean_active(s)) return fail("iw_set_mixed: mean-wave and shear-current terms are FP64 only");
+    }
+    s->mixed = on != 0;
     return 0;
 }
 
@@ -2255,6 +2367,8 @@ extern "C" int iw_run(iw_solver *s, int nsteps, int *counts)
 {
     if (!s || nsteps < 0) return fail("iw_run: bad argument");
     const bool mc = s->P.imaxcalc || s->P.dtadapt;
+    if (s->mixed && (mc || mean_active(s)))
+        return fail("iw_run: the mixed path runs without maxcalc records, adaptive dt and mean-wave terms");
     const bool adapt = s->P.dtadapt != 0;
     s->mc_hist.clear();
     if (mc && !adapt && nsteps > s->mc_ring_cap) { // device ring of the step records
@@ -2285,7 +2399,9 @@ extern "C" int iw_run(iw_solver *s, int nsteps, int *counts)
         // taken there before dt or the coefficients are read
         project_dev(s, s->buf, 4);
         if (!mc && s->ivardt == 1) s->idtstep++; // main.f:660-664
-        if (convec_dev(s, s->P.ispecfilternl, mc ? istep : 0, n)) return -1;
+        if (s->mixed ? convec_mixed(s, s->P.ispecfilternl)
+                     : convec_dev(s, s->P.ispecfilternl, mc ? istep : 0, n))
+            return -1;
         const bool ldtadapt = s->ivardt == 1 && s->idtstep < 2;
         if (istep < 2 && s->istart == 0) { // main.f:700-777
             s->fac[0] = 2.; s->fac[1] = -1.; s->fac[2] = 0.;

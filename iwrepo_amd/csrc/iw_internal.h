@@ -175,6 +175,8 @@ struct iw_solver {
     // terms of solver.f:848-931 need the physical dtdx, dtdz (gT)
     double *mw = nullptr, *ush = nullptr, *dush = nullptr;
     bool mw_on = false;
+    // mixed path (iw_set_mixed): FP32 physical-space working set in convec
+    bool mixed = false;
     double *gT[2] = {nullptr, nullptr};
     // postp (postp.f:9-76) and speclegencalc (filtering.f:1110-1221): the
     // x-means of u and of t11..t33 at every (y,z) line, d umean/dy and /dz,
@@ -200,8 +202,13 @@ void launch_maxcalc_final(const Geo &g, const McArgs &A, int istep, double t, do
                           hipStream_t st);
 void launch_maxcalc_umaxt(const Geo &g, const McArgs &A, double umax, double *colmax, double *groupmax,
                           double *rec, hipStream_t st);
-void launch_fft_x(const Geo &g, FieldBatch b, int nk, int dir, hipStream_t st, const McFuse *mf = nullptr);
-void launch_fft_y(const Geo &g, FieldBatch b, int nk, int dir, hipStream_t st);
+// storage of the physical-space side of a transform: kPrecF64 everything
+// FP64; kPrecF32 (mixed path) x pass FP32 in and out, y pass FP64 spectral
+// in and FP32 out (inverse) or FP32 in and FP64 spectral out (forward)
+constexpr int kPrecF64 = 0, kPrecF32 = 1;
+void launch_fft_x(const Geo &g, FieldBatch b, int nk, int dir, hipStream_t st, const McFuse *mf = nullptr,
+                  int prec = kPrecF64);
+void launch_fft_y(const Geo &g, FieldBatch b, int nk, int dir, hipStream_t st, int prec = kPrecF64);
 void launch_dpdz(const Geo &g, const double *a, double *b, hipStream_t st);
 void launch_dpdz_conv(const Geo &g, double *const *uvwt, double *const *out4, hipStream_t st);
 void launch_dpdz3(const Geo &g, double *const *in3, double *const *out3, hipStream_t st);
@@ -220,6 +227,14 @@ struct MeanTerms {
 void launch_rhs_phys(const Geo &g, const iw_params &P, double t, double *const *uvwt,
                      double *const *gr, double *const *S, const double *ox, double *const *pr,
                      hipStream_t st, const MeanTerms *mt = nullptr);
+// the same on FP32 arrays, FP32 arithmetic (mixed path; no mean-wave terms)
+void launch_rhs_phys_f32(const Geo &g, const iw_params &P, double t, float *const *uvwt, float *const *gr,
+                         float *const *S, const float *ox, float *const *pr, hipStream_t st);
+// mixed path (iw_mixed.hip): launch_dpdz_conv_grad on FP32 fields with the
+// contraction by D on FP32 matrix cores (MFMA); false if the order has no
+// instantiation
+bool launch_dpdz_conv_grad_mfma(const Geo &g, float *const *uvwt, float *const *out4, float *const *dz3,
+                                hipStream_t st);
 void launch_dx(const Geo &g, const double *a, double *b, hipStream_t st);
 // diagnostics (iw_diag.hip): postp's means and d/dy, d/dz of umean, its
 // TKE-budget sums (out: istep, t, the 16 energy.dat values), speclegencalc at

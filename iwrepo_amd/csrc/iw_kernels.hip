@@ -253,6 +253,20 @@ __device__ __forceinline__ void stage_tables(double *wgs, double *ALs, double *s
 // even/odd-packed row with LDS-staged radix-2 passes.
 // ---------------------------------------------------------------------------
 // maxcalc extrema of one field (various.f:749-851), ties to the later point
+// complex loads and stores of a field stored in FP64 or FP32 (the mixed
+// path keeps the physical-space working set in FP32); arithmetic stays FP64
+__device__ __forceinline__ double2 ld2(const double *p) { return *reinterpret_cast<const double2 *>(p); }
+__device__ __forceinline__ double2 ld2(const float *p)
+{
+    const float2 v = *reinterpret_cast<const float2 *>(p);
+    return make_double2((double)v.x, (double)v.y);
+}
+__device__ __forceinline__ void st2(double *p, double2 v) { *reinterpret_cast<double2 *>(p) = v; }
+__device__ __forceinline__ void st2(float *p, double2 v)
+{
+    *reinterpret_cast<float2 *>(p) = make_float2((float)v.x, (float)v.y);
+}
+
 __device__ __forceinline__ void mc_take(bool mn, double x, int lx, double &b, int &lb)
 {
     if (mn ? (x < b || (x == b && lx > lb)) : (x > b || (x == b && lx > lb))) {
@@ -263,8 +277,9 @@ __device__ __forceinline__ void mc_take(bool mn, double x, int lx, double &b, in
 
 // AX: the batch's fields have flux products (forward only); MAPS: the batch
 // has plane maps (else rows are addressed arithmetically, so no row address
-// waits on a table load and a thread's loads are all in flight together)
-template <int DIR, bool AX = false, bool MAPS = false>
+// waits on a table load and a thread's loads are all in flight together).
+// TI, TO: storage type of the inputs (f, ax) and of the outputs (g, dx, axo)
+template <int DIR, bool AX = false, bool MAPS = false, typename TI = double, typename TO = double>
 __global__ __launch_bounds__(kFT) void k_fft_x(FieldBatch fb, int nrows, int ny, int nxpp, int M, int logM,
                                                int R, const double2 *__restrict__ twM,
                                                const double2 *__restrict__ twNg, double scaley, McFuse mf)
@@ -281,8 +296,8 @@ __global__ __launch_bounds__(kFT) void k_fft_x(FieldBatch fb, int nrows, int ny,
     // maxcalc's per-level tables for the block's rows (w00, rm, rdz, ush),
     // staged with the others so the epilogue after the stores reads LDS
     double *mcv = kws + M + 1;
-    const double *base = fb.f[blockIdx.y];
-    double *obase = fb.g[blockIdx.y];
+    const TI *base = reinterpret_cast<const TI *>(fb.f[blockIdx.y]);
+    TO *obase = reinterpret_cast<TO *>(fb.g[blockIdx.y]);
     const int row0 = blockIdx.x * R;
     const int N = 2 * M;
     const int lny = __builtin_ctz(ny); // ny is a power of two
@@ -338,9 +353,9 @@ __global__ __launch_bounds__(kFT) void k_fft_x(FieldBatch fb, int nrows, int ny,
         for (int it = 0; it < kFE / kFT; it++) {
             const int idx = threadIdx.x + it * kFT;
             const int r = idx >> logM, k = idx & (M - 1), row = row0 + r;
-            const double *p = base + srow(row < nrows ? row : nrows - 1);
-            va[it] = *reinterpret_cast<const double2 *>(p + 2 * k);
-            vb[it] = *reinterpret_cast<const double2 *>(p + 2 * (M - k));
+            const TI *p = base + srow(row < nrows ? row : nrows - 1);
+            va[it] = ld2(p + 2 * k);
+            vb[it] = ld2(p + 2 * (M - k));
         }
 #pragma unroll
         for (int it = 0; it < kFE / kFT; it++) {
@@ -370,17 +385,17 @@ __global__ __launch_bounds__(kFT) void k_fft_x(FieldBatch fb, int nrows, int ny,
             buf[lp(idx)] = Z;
         }
     };
-    auto store_inverse = [&](double *ob) {
+    auto store_inverse = [&](TO *ob) {
 #pragma unroll
         for (int it = 0; it < kFE / kFT; it++) {
             const int idx = threadIdx.x + it * kFT;
             int r = idx >> logM, m = idx & (M - 1), row = row0 + r;
             if (row >= nrows) continue;
-            *reinterpret_cast<double2 *>(ob + drow(row) + 2 * m) = buf[lp(idx)];
+            st2(ob + drow(row) + 2 * m, buf[lp(idx)]);
         }
         for (int r = threadIdx.x; r < R; r += blockDim.x) {
             int row = row0 + r;
-            if (row < nrows) *reinterpret_cast<double2 *>(ob + drow(row) + N) = make_double2(0., 0.);
+            if (row < nrows) st2(ob + drow(row) + N, make_double2(0., 0.));
         }
     };
     if (DIR > 0 && fb.dx[blockIdx.y]) {
@@ -388,18 +403,18 @@ __global__ __launch_bounds__(kFT) void k_fft_x(FieldBatch fb, int nrows, int ny,
         load_inverse(true);
         __syncthreads();
         fft_lds<false>(buf, logM, R, tw, DIR);
-        store_inverse(fb.dx[blockIdx.y]);
+        store_inverse(reinterpret_cast<TO *>(fb.dx[blockIdx.y]));
         __syncthreads();
     }
     if (DIR < 0) {
         // forward: FFT991 normalised by 1/N with B_0 = B_M = 0 (fft.f:1293-1305)
         const double sc = 1.0 / (double)N;
-        auto load_fwd = [&](const double *src) {
+        auto load_fwd = [&](const TI *src) {
 #pragma unroll
             for (int it = 0; it < kFE / kFT; it++) {
                 const int idx = threadIdx.x + it * kFT;
                 int r = idx >> logM, m = idx & (M - 1), row = row0 + r;
-                double2 z = *reinterpret_cast<const double2 *>(src + srow(row < nrows ? row : nrows - 1) + 2 * m);
+                double2 z = ld2(src + srow(row < nrows ? row : nrows - 1) + 2 * m);
                 if (row >= nrows) z = make_double2(0., 0.);
                 buf[lp(idx)] = z;
             }
@@ -430,16 +445,15 @@ __global__ __launch_bounds__(kFT) void k_fft_x(FieldBatch fb, int nrows, int ny,
             // flux product P: both are transformed by the same passes and
             // combined from LDS
             const int H = R >> 1, rowA = blockIdx.x * H;
-            const double *acc = fb.ax[blockIdx.y];
-            double *axo = fb.axo[blockIdx.y];
+            const TI *acc = reinterpret_cast<const TI *>(fb.ax[blockIdx.y]);
+            TO *axo = reinterpret_cast<TO *>(fb.axo[blockIdx.y]);
 #pragma unroll
             for (int it = 0; it < kFE / kFT; it++) {
                 const int idx = threadIdx.x + it * kFT;
                 const int r = idx >> logM, m = idx & (M - 1);
                 const bool isp = r >= H;
                 const int row = rowA + (isp ? r - H : r);
-                double2 z = *reinterpret_cast<const double2 *>((isp ? acc : base) +
-                                                               srow(row < nrows ? row : nrows - 1) + 2 * m);
+                double2 z = ld2((isp ? acc : base) + srow(row < nrows ? row : nrows - 1) + 2 * m);
                 if (row >= nrows) z = make_double2(0., 0.);
                 buf[lp(idx)] = z;
             }
@@ -450,14 +464,14 @@ __global__ __launch_bounds__(kFT) void k_fft_x(FieldBatch fb, int nrows, int ny,
                 const int idx = threadIdx.x + it * kFT;
                 const int r = idx >> logM, k = idx & (M - 1), row = rowA + r;
                 if (r >= H || row >= nrows) continue;
-                double *orow = obase + drow(row);
+                TO *orow = obase + drow(row);
                 const double2 q = coef(r + H, k);
-                *reinterpret_cast<double2 *>(orow + 2 * k) = addk(coef(r, k), q, k);
-                if (axo) *reinterpret_cast<double2 *>(axo + drow(row) + 2 * k) = q; // P's own transform
+                st2(orow + 2 * k, addk(coef(r, k), q, k));
+                if (axo) st2(axo + drow(row) + 2 * k, q); // P's own transform
                 if (k == 0) {
                     const double2 qm = coef(r + H, M);
-                    *reinterpret_cast<double2 *>(orow + 2 * M) = addk(coef(r, M), qm, M);
-                    if (axo) *reinterpret_cast<double2 *>(axo + drow(row) + 2 * M) = qm;
+                    st2(orow + 2 * M, addk(coef(r, M), qm, M));
+                    if (axo) st2(axo + drow(row) + 2 * M, qm);
                 }
             }
         } else {
@@ -469,9 +483,9 @@ __global__ __launch_bounds__(kFT) void k_fft_x(FieldBatch fb, int nrows, int ny,
                 const int idx = threadIdx.x + it * kFT;
                 int r = idx >> logM, k = idx & (M - 1), row = row0 + r;
                 if (row >= nrows) continue;
-                double *orow = obase + drow(row);
-                *reinterpret_cast<double2 *>(orow + 2 * k) = coef(r, k);
-                if (k == 0) *reinterpret_cast<double2 *>(orow + 2 * M) = coef(r, M);
+                TO *orow = obase + drow(row);
+                st2(orow + 2 * k, coef(r, k));
+                if (k == 0) st2(orow + 2 * M, coef(r, M));
             }
         }
     } else {
@@ -565,7 +579,8 @@ __global__ __launch_bounds__(kFT) void k_fft_x(FieldBatch fb, int nrows, int ny,
 // transform of columns ii < nxh (the kx-Nyquist column is skipped); forward
 // scaled by 1/ny (horfft.f:28-30).  CB columns per block staged in LDS.
 // ---------------------------------------------------------------------------
-template <int DIR>
+// TI, TO: storage type of the inputs (f, ay) and of the outputs (g, dy)
+template <int DIR, typename TI = double, typename TO = double>
 __global__ __launch_bounds__(kFT) void k_fft_y(FieldBatch fb, int nxpp, int nxh, int ny, int logNy,
                                                int CB, const double2 *__restrict__ twY, double scaley)
 {
@@ -574,9 +589,9 @@ __global__ __launch_bounds__(kFT) void k_fft_y(FieldBatch fb, int nxpp, int nxh,
     double2 *tw = smy + CB * ny;
     double *kys = reinterpret_cast<double *>(tw + ny); // ky per row, staged like tw
     const int lpl = blockIdx.y;
-    const double *plane = fb.f[blockIdx.z] + (long)(fb.smap ? fb.smap[lpl] : lpl) * nxpp * ny;
+    const TI *plane = reinterpret_cast<const TI *>(fb.f[blockIdx.z]) + (long)(fb.smap ? fb.smap[lpl] : lpl) * nxpp * ny;
     const long opl = (long)(fb.dmap ? fb.dmap[lpl] : lpl) * nxpp * ny;
-    double *oplane = fb.g[blockIdx.z] + opl;
+    TO *oplane = reinterpret_cast<TO *>(fb.g[blockIdx.z]) + opl;
     const int ii0 = blockIdx.x * CB;
     const int lcb = __builtin_ctz(CB); // CB is a power of two
     for (int t = threadIdx.x; t < ny; t += blockDim.x) {
@@ -586,7 +601,7 @@ __global__ __launch_bounds__(kFT) void k_fft_y(FieldBatch fb, int nxpp, int nxh,
     __syncthreads();
     // columns ii < nxh of the plane into LDS; with deriv multiplied by i ky
     // (the y gradient of solver.f:86-103)
-    auto load = [&](const double *src, bool deriv) {
+    auto load = [&](const TI *src, bool deriv) {
         // all loads first (columns past nxh are clamped and their values
         // dropped), so they are in flight together; the tile is always
         // CB x ny = kFE values
@@ -595,7 +610,7 @@ __global__ __launch_bounds__(kFT) void k_fft_y(FieldBatch fb, int nxpp, int nxh,
         for (int it = 0; it < kFE / kFT; it++) {
             const int idx = threadIdx.x + it * kFT;
             const int j = idx >> lcb, ii = ii0 + (idx & (CB - 1));
-            v[it] = *reinterpret_cast<const double2 *>(src + (long)j * nxpp + 2 * (ii < nxh ? ii : nxh - 1));
+            v[it] = ld2(src + (long)j * nxpp + 2 * (ii < nxh ? ii : nxh - 1));
         }
 #pragma unroll
         for (int it = 0; it < kFE / kFT; it++) {
@@ -612,11 +627,11 @@ __global__ __launch_bounds__(kFT) void k_fft_y(FieldBatch fb, int nxpp, int nxh,
     };
     // forward with a second input P: + i ky P-hat (the y part of the flux
     // divergence, solver.f:985-1063, 1163-1196), P-hat kept in registers
-    const double *accp = (DIR < 0 && fb.ay[blockIdx.z])
-                             ? fb.ay[blockIdx.z] + (long)(fb.smap ? fb.smap[lpl] : lpl) * nxpp * ny
-                             : nullptr;
+    const TI *accp = (DIR < 0 && fb.ay[blockIdx.z])
+                         ? reinterpret_cast<const TI *>(fb.ay[blockIdx.z]) + (long)(fb.smap ? fb.smap[lpl] : lpl) * nxpp * ny
+                         : nullptr;
     double2 pb[kFE / kFT];
-    auto store = [&](double *op, bool add) {
+    auto store = [&](TO *op, bool add) {
 #pragma unroll
         for (int it = 0; it < kFE / kFT; it++) {
             const int idx = threadIdx.x + it * kFT;
@@ -632,10 +647,10 @@ __global__ __launch_bounds__(kFT) void k_fft_y(FieldBatch fb, int nxpp, int nxh,
                     z.y = z.y + ky * pb[it].x;
                 }
             }
-            *reinterpret_cast<double2 *>(op + (long)j * nxpp + 2 * ii) = z;
+            st2(op + (long)j * nxpp + 2 * ii, z);
         }
     };
-    double *dyp = (DIR > 0 && fb.dy[blockIdx.z]) ? fb.dy[blockIdx.z] + opl : nullptr;
+    TO *dyp = (DIR > 0 && fb.dy[blockIdx.z]) ? reinterpret_cast<TO *>(fb.dy[blockIdx.z]) + opl : nullptr;
     if (dyp) { // i ky f first: a transform in place overwrites its input
         load(plane, true);
         __syncthreads();
@@ -666,31 +681,26 @@ __global__ __launch_bounds__(kFT) void k_fft_y(FieldBatch fb, int nxpp, int nxh,
     store(oplane, accp != nullptr);
     // the untransformed kx-Nyquist column: copied when out of place, plus
     // i ky P when accumulating, and times i ky into the derivative output
-    if (blockIdx.x == 0 && (plane != oplane || dyp || accp))
+    const bool inplace = (const void *)plane == (const void *)oplane;
+    if (blockIdx.x == 0 && (!inplace || dyp || accp))
         for (int j = threadIdx.x; j < ny; j += blockDim.x) {
-            const double *p = plane + (long)j * nxpp + 2 * nxh;
+            const TI *p = plane + (long)j * nxpp + 2 * nxh;
             double pr = p[0], pi = p[1];
             if (accp) {
                 const double ky = kys[j];
-                const double *a = accp + (long)j * nxpp + 2 * nxh;
-                pr = pr + (-ky * a[1]);
-                pi = pi + ky * a[0];
+                const TI *a = accp + (long)j * nxpp + 2 * nxh;
+                pr = pr + (-ky * (double)a[1]);
+                pi = pi + ky * (double)a[0];
             }
-            if (plane != oplane || accp) {
-                double *q = oplane + (long)j * nxpp + 2 * nxh;
-                q[0] = pr;
-                q[1] = pi;
-            }
+            if (!inplace || accp) st2(oplane + (long)j * nxpp + 2 * nxh, make_double2(pr, pi));
             if (dyp) {
                 const double ky = kys[j];
-                double *q = dyp + (long)j * nxpp + 2 * nxh;
-                q[0] = -ky * pi;
-                q[1] = ky * pr;
+                st2(dyp + (long)j * nxpp + 2 * nxh, make_double2(-ky * pi, ky * pr));
             }
         }
 }
 
-template <bool MAPS>
+template <bool MAPS, typename TI, typename TO>
 static void fft_x_launch(const Geo &g, const FieldBatch &b, int nk, int dir, hipStream_t st, const McFuse *mf)
 {
     const int R = kFE / g.M;
@@ -716,40 +726,49 @@ static void fft_x_launch(const Geo &g, const FieldBatch &b, int nk, int dir, hip
             t.n++;
         }
         if (ba.n)
-            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fft_x<-1, true, MAPS>), dim3((nrows + R / 2 - 1) / (R / 2), ba.n),
-                               dim3(kFT), sh, st, ba, nrows, g.ny, g.nxpp, g.M, g.logM, R, g.twM, g.twN, g.scaley,
-                               none);
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fft_x<-1, true, MAPS, TI, TO>),
+                               dim3((nrows + R / 2 - 1) / (R / 2), ba.n), dim3(kFT), sh, st, ba, nrows, g.ny, g.nxpp,
+                               g.M, g.logM, R, g.twM, g.twN, g.scaley, none);
         if (bp.n)
-            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fft_x<-1, false, MAPS>), dim3(grid.x, bp.n), dim3(kFT), sh, st, bp,
-                               nrows, g.ny, g.nxpp, g.M, g.logM, R, g.twM, g.twN, g.scaley, none);
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fft_x<-1, false, MAPS, TI, TO>), dim3(grid.x, bp.n), dim3(kFT), sh,
+                               st, bp, nrows, g.ny, g.nxpp, g.M, g.logM, R, g.twM, g.twN, g.scaley, none);
     } else {
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fft_x<1, false, MAPS>), grid, dim3(kFT), sh, st, b, nrows, g.ny,
-                           g.nxpp, g.M, g.logM, R, g.twM, g.twN, g.scaley, mf ? *mf : none);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fft_x<1, false, MAPS, TI, TO>), grid, dim3(kFT), sh, st, b, nrows,
+                           g.ny, g.nxpp, g.M, g.logM, R, g.twM, g.twN, g.scaley, mf ? *mf : none);
     }
 }
 
-void launch_fft_x(const Geo &g, FieldBatch b, int nk, int dir, hipStream_t st, const McFuse *mf)
+void launch_fft_x(const Geo &g, FieldBatch b, int nk, int dir, hipStream_t st, const McFuse *mf, int prec)
 {
-    if (b.smap || b.dmap)
-        fft_x_launch<true>(g, b, nk, dir, st, mf);
+    if (prec == kPrecF32) // FP32 in and out (mixed path; no plane maps)
+        fft_x_launch<false, float, float>(g, b, nk, dir, st, mf);
+    else if (b.smap || b.dmap)
+        fft_x_launch<true, double, double>(g, b, nk, dir, st, mf);
     else
-        fft_x_launch<false>(g, b, nk, dir, st, mf);
+        fft_x_launch<false, double, double>(g, b, nk, dir, st, mf);
 }
 
 int mc_blocks(const Geo &g) { return (g.ny * g.nz + kFE / g.M - 1) / (kFE / g.M); }
 
-void launch_fft_y(const Geo &g, FieldBatch b, int nk, int dir, hipStream_t st)
+void launch_fft_y(const Geo &g, FieldBatch b, int nk, int dir, hipStream_t st, int prec)
 {
     if (g.ndims != 3) return;
     int CB = kFE / g.ny;
     dim3 grid((g.nxh + CB - 1) / CB, nk, b.n);
     size_t sh = sizeof(double2) * ((size_t)CB * g.ny + g.ny) + sizeof(double) * g.ny;
-    if (dir < 0)
-        hipLaunchKernelGGL(k_fft_y<-1>, grid, dim3(kFT), sh, st, b, g.nxpp, g.nxh, g.ny, g.logNy, CB,
-                           g.twY, g.scaley);
+    // mixed path: spectral FP64 <-> FP32 intermediate
+    if (dir < 0 && prec == kPrecF32)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fft_y<-1, float, double>), grid, dim3(kFT), sh, st, b, g.nxpp, g.nxh,
+                           g.ny, g.logNy, CB, g.twY, g.scaley);
+    else if (prec == kPrecF32)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fft_y<1, double, float>), grid, dim3(kFT), sh, st, b, g.nxpp, g.nxh,
+                           g.ny, g.logNy, CB, g.twY, g.scaley);
+    else if (dir < 0)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fft_y<-1>), grid, dim3(kFT), sh, st, b, g.nxpp, g.nxh, g.ny, g.logNy,
+                           CB, g.twY, g.scaley);
     else
-        hipLaunchKernelGGL(k_fft_y<1>, grid, dim3(kFT), sh, st, b, g.nxpp, g.nxh, g.ny, g.logNy, CB,
-                           g.twY, g.scaley);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fft_y<1>), grid, dim3(kFT), sh, st, b, g.nxpp, g.nxh, g.ny, g.logNy,
+                           CB, g.twY, g.scaley);
 }
 
 // ---------------------------------------------------------------------------
@@ -1041,6 +1060,9 @@ struct RhsArgs {
     int mean;
 };
 
+// T: storage and arithmetic type (double for the plain path, float for the
+// mixed path, which has no mean-wave terms); the pointers of A hold T arrays
+template <typename T>
 __global__ __launch_bounds__(256) void k_rhs_phys(RhsArgs A, long ntot, long nxppy, int nxpp, int nx,
                                                   int nz, int nl, const double *__restrict__ zh,
                                                   const double *__restrict__ zc,
@@ -1048,90 +1070,92 @@ __global__ __launch_bounds__(256) void k_rhs_phys(RhsArgs A, long ntot, long nxp
 {
     long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= ntot) return;
+    const T *Au = reinterpret_cast<const T *>(A.u), *Av = reinterpret_cast<const T *>(A.v);
+    const T *Aw = reinterpret_cast<const T *>(A.w), *At = reinterpret_cast<const T *>(A.t);
+    const T *G[9];
+#pragma unroll
+    for (int q = 0; q < 9; q++) G[q] = reinterpret_cast<const T *>(A.g[q]);
+    T *S0 = reinterpret_cast<T *>(A.S[0]), *S1 = reinterpret_cast<T *>(A.S[1]);
+    T *S2 = reinterpret_cast<T *>(A.S[2]), *S3 = reinterpret_cast<T *>(A.S[3]);
+    const T half = T(0.5), four = T(4.), omz = T(A.omz);
     const int k = (int)(n / nxppy);
     const int i = (int)(n % nxpp);
-    double u = A.u[n], v = A.v[n], w = A.w[n], tp = A.t[n];
-    double su = A.S[0][n], sv = A.S[1][n], sw = A.S[2][n], st = 0.0;
-    su = su - 0.5 * (u * A.g[0][n] + v * (A.g[1][n] - 4. * A.omz) + w * A.g[2][n]);
-    sv = sv - 0.5 * (u * (A.g[3][n] + 4. * A.omz) + v * A.g[4][n] + w * A.g[5][n]);
-    sw = sw - 0.5 * (u * A.g[6][n] + v * A.g[7][n] + w * A.g[8][n]);
-    sw = sw + A.bcoef * tp;
+    T u = Au[n], v = Av[n], w = Aw[n], tp = At[n];
+    T su = S0[n], sv = S1[n], sw = S2[n], st = T(0.);
+    su = su - half * (u * G[0][n] + v * (G[1][n] - four * omz) + w * G[2][n]);
+    sv = sv - half * (u * (G[3][n] + four * omz) + v * G[4][n] + w * G[5][n]);
+    sw = sw - half * (u * G[6][n] + v * G[7][n] + w * G[8][n]);
+    sw = sw + T(A.bcoef) * tp;
     const int k1 = k % nl, ks = k / nl;
     // total vertical velocity at the interfaces, perturbation + mean wave
     // (solver.f:723-724)
     const long mi = (long)k * nxpp + i, nm = (long)nxpp * nz;
-    const double wmw = A.mean ? A.mt.mw[nm + mi] : 0.0;
+    const T wmw = A.mean ? T(A.mt.mw[nm + mi]) : T(0.);
     if (k1 == 0) {
-        double wb = (k == 0) ? 0. : w + wmw;
-        if (wb > 0.) {
-            double tau1 = 0.5 / A.omega * (2. / zh[ks]);
+        T wb = (k == 0) ? T(0.) : w + wmw;
+        if (wb > T(0.)) {
+            T tau1 = T(0.5 / A.omega * (2. / zh[ks]));
             long m = n - nxppy;
-            su = su - tau1 * (wb * u - wb * A.u[m]);
-            sv = sv - tau1 * (wb * v - wb * A.v[m]);
-            sw = sw - tau1 * (wb * w - wb * A.w[m]);
-            st = st - tau1 * (wb * tp - wb * A.t[m]);
+            su = su - tau1 * (wb * u - wb * Au[m]);
+            sv = sv - tau1 * (wb * v - wb * Av[m]);
+            sw = sw - tau1 * (wb * w - wb * Aw[m]);
+            st = st - tau1 * (wb * tp - wb * At[m]);
         }
     }
     if (k1 == nl - 1) {
-        double wt = (k == nz - 1) ? 0. : w + wmw;
-        if (wt < 0.) {
-            double g0 = fabs(wt);
-            double tau2 = 0.5 / A.omega * (2. / zh[ks]);
+        T wt = (k == nz - 1) ? T(0.) : w + wmw;
+        if (wt < T(0.)) {
+            T g0 = fabs(wt);
+            T tau2 = T(0.5 / A.omega * (2. / zh[ks]));
             long m = n + nxppy;
-            su = su - tau2 * (g0 * u - g0 * A.u[m]);
-            sv = sv - tau2 * (g0 * v - g0 * A.v[m]);
-            sw = sw - tau2 * (g0 * w - g0 * A.w[m]);
-            st = st - tau2 * (g0 * tp - g0 * A.t[m]);
+            su = su - tau2 * (g0 * u - g0 * Au[m]);
+            sv = sv - tau2 * (g0 * v - g0 * Av[m]);
+            sw = sw - tau2 * (g0 * w - g0 * Aw[m]);
+            st = st - tau2 * (g0 * tp - g0 * At[m]);
         }
     }
-    if (A.mean) { // mean-wave and shear-current terms, solver.f:848-931
-        const double *mw = A.mt.mw;
-        const double advmean = mw[mi] + A.mt.ush[k];
-        su = su - advmean * A.g[0][n] - wmw * A.g[2][n];
-        sv = sv - advmean * A.g[3][n] - wmw * A.g[5][n];
-        sw = sw - advmean * A.g[6][n] - wmw * A.g[8][n];
-        st = st - advmean * A.mt.dtdx[n] - wmw * A.mt.dtdz[n];
-        const double dusheardz = A.mt.dush[k];
-        su = su - u * mw[2 * nm + mi] - w * mw[3 * nm + mi] - w * dusheardz;
-        sw = sw - u * mw[4 * nm + mi] - w * mw[5 * nm + mi];
-        st = st - u * mw[7 * nm + mi] - w * mw[8 * nm + mi];
-        st = st + A.xkappa * 0.0; // d2rhomeandz2 = 0 (forcing.f:167-183)
+    if constexpr (std::is_same<T, double>::value) {
+        if (A.mean) { // mean-wave and shear-current terms, solver.f:848-931
+            const double *mw = A.mt.mw;
+            const double advmean = mw[mi] + A.mt.ush[k];
+            su = su - advmean * G[0][n] - wmw * G[2][n];
+            sv = sv - advmean * G[3][n] - wmw * G[5][n];
+            sw = sw - advmean * G[6][n] - wmw * G[8][n];
+            st = st - advmean * A.mt.dtdx[n] - wmw * A.mt.dtdz[n];
+            const double dusheardz = A.mt.dush[k];
+            su = su - u * mw[2 * nm + mi] - w * mw[3 * nm + mi] - w * dusheardz;
+            sw = sw - u * mw[4 * nm + mi] - w * mw[5 * nm + mi];
+            st = st - u * mw[7 * nm + mi] - w * mw[8 * nm + mi];
+            st = st + A.xkappa * 0.0; // d2rhomeandz2 = 0 (forcing.f:167-183)
+        }
     }
     if (A.paddle && i < nx) {
         double zz = zc[k], xx = xg[i];
         double arg = A.xkcrit * xx - A.xomega * A.t_now;
         double wzp = A.xmcrit * cos(A.xmcrit * zz), wz = sin(A.xmcrit * zz);
-        su = su + (-A.inv_xkcrit) * A.xacrit_umax * wzp * sin(arg) * A.umax_zdim * A.phit;
-        sw = sw + A.xacrit_umax * wz * cos(arg) * A.umax_zdim * A.phit;
-        st = st + A.xacrit_umax * wz * sin(arg) * A.rfac * A.umax_zdim * A.phit;
+        su = su + T((-A.inv_xkcrit) * A.xacrit_umax * wzp * sin(arg) * A.umax_zdim * A.phit);
+        sw = sw + T(A.xacrit_umax * wz * cos(arg) * A.umax_zdim * A.phit);
+        st = st + T(A.xacrit_umax * wz * sin(arg) * A.rfac * A.umax_zdim * A.phit);
     }
-    st = st + A.ox[n] - w * A.rg;
-    A.S[0][n] = su;
-    A.S[1][n] = sv;
-    A.S[2][n] = sw;
-    A.S[3][n] = st;
-    A.pr[0][n] = (-0.5 * u) * u;
-    A.pr[1][n] = (-0.5 * u) * v;
-    A.pr[2][n] = (-0.5 * v) * v;
-    A.pr[3][n] = (-0.5 * u) * w;
-    A.pr[4][n] = (-0.5 * v) * w;
-    A.pr[5][n] = -(u * tp);
-    A.pr[6][n] = -(v * tp);
+    st = st + reinterpret_cast<const T *>(A.ox)[n] - w * T(A.rg);
+    S0[n] = su;
+    S1[n] = sv;
+    S2[n] = sw;
+    S3[n] = st;
+    T *const *pr = reinterpret_cast<T *const *>(A.pr);
+    pr[0][n] = (-half * u) * u;
+    pr[1][n] = (-half * u) * v;
+    pr[2][n] = (-half * v) * v;
+    pr[3][n] = (-half * u) * w;
+    pr[4][n] = (-half * v) * w;
+    pr[5][n] = -(u * tp);
+    pr[6][n] = -(v * tp);
 }
 
-void launch_rhs_phys(const Geo &g, const iw_params &P, double t, double *const *uvwt,
-                     double *const *gr, double *const *S, const double *ox, double *const *pr,
-                     hipStream_t st, const MeanTerms *mt)
+// the scalar parameters of rhsfm's physical-space part
+static void rhs_args(const Geo &g, const iw_params &P, double t, RhsArgs &A)
 {
-    RhsArgs A;
-    A.mean = mt && mt->mw;
-    if (A.mean) A.mt = *mt;
     A.xkappa = P.xkappa;
-    A.u = uvwt[0]; A.v = uvwt[1]; A.w = uvwt[2]; A.t = uvwt[3];
-    for (int q = 0; q < 9; q++) A.g[q] = gr[q];
-    for (int q = 0; q < 4; q++) A.S[q] = S[q];
-    A.ox = ox;
-    for (int q = 0; q < 7; q++) A.pr[q] = pr[q];
     double rg = -P.rho0 * (P.brunt * P.brunt) / P.grav; // forcing.f:115-120
     A.omz = P.omz;
     A.bcoef = (P.brunt * P.brunt) / rg;                 // solver.f:693-696
@@ -1150,7 +1174,40 @@ void launch_rhs_phys(const Geo &g, const iw_params &P, double t, double *const *
     A.rfac = -(1. / xomega) * P.brunt * P.brunt * (P.rho0 / P.grav);
     A.phit = (t <= P.fwave * (pi2 / xomega)) ? 1. : 0.; // forcing.f:424-447
     A.paddle = (P.xacrit != 0.);
-    hipLaunchKernelGGL(k_rhs_phys, dim3((unsigned)((g.ntot + 255) / 256)), dim3(256), 0, st, A, g.ntot,
+}
+
+void launch_rhs_phys(const Geo &g, const iw_params &P, double t, double *const *uvwt,
+                     double *const *gr, double *const *S, const double *ox, double *const *pr,
+                     hipStream_t st, const MeanTerms *mt)
+{
+    RhsArgs A;
+    rhs_args(g, P, t, A);
+    A.mean = mt && mt->mw;
+    if (A.mean) A.mt = *mt;
+    A.u = uvwt[0]; A.v = uvwt[1]; A.w = uvwt[2]; A.t = uvwt[3];
+    for (int q = 0; q < 9; q++) A.g[q] = gr[q];
+    for (int q = 0; q < 4; q++) A.S[q] = S[q];
+    A.ox = ox;
+    for (int q = 0; q < 7; q++) A.pr[q] = pr[q];
+    hipLaunchKernelGGL(k_rhs_phys<double>, dim3((unsigned)((g.ntot + 255) / 256)), dim3(256), 0, st, A, g.ntot,
+                       g.nxppy, g.nxpp, g.nx, g.nz, g.nzloc, g.zh, g.z, g.x);
+}
+
+void launch_rhs_phys_f32(const Geo &g, const iw_params &P, double t, float *const *uvwt, float *const *gr,
+                         float *const *S, const float *ox, float *const *pr, hipStream_t st)
+{
+    RhsArgs A;
+    rhs_args(g, P, t, A);
+    A.mean = 0;
+    A.u = reinterpret_cast<const double *>(uvwt[0]);
+    A.v = reinterpret_cast<const double *>(uvwt[1]);
+    A.w = reinterpret_cast<const double *>(uvwt[2]);
+    A.t = reinterpret_cast<const double *>(uvwt[3]);
+    for (int q = 0; q < 9; q++) A.g[q] = reinterpret_cast<const double *>(gr[q]);
+    for (int q = 0; q < 4; q++) A.S[q] = reinterpret_cast<double *>(S[q]);
+    A.ox = reinterpret_cast<const double *>(ox);
+    for (int q = 0; q < 7; q++) A.pr[q] = reinterpret_cast<double *>(pr[q]);
+    hipLaunchKernelGGL(k_rhs_phys<float>, dim3((unsigned)((g.ntot + 255) / 256)), dim3(256), 0, st, A, g.ntot,
                        g.nxppy, g.nxpp, g.nx, g.nz, g.nzloc, g.zh, g.z, g.x);
 }
 

@@ -22,7 +22,7 @@ EXPORTS = [
     "iw_save_restart", "iw_load_restart", "iw_write_restart_file", "iw_read_restart_file", "iw_abvar1", "iw_abvar2", "iw_abvar3", "iw_bdfvarv",
     "iw_bdfvar1", "iw_bdfvar2", "iw_bdfvar3", "iw_set_profiling", "iw_kernel_stats",
     "iw_device_bytes", "iw_maxcalc", "iw_maxcalc_records", "iw_get_dtstate", "iw_set_dtstate",
-    "iw_set_meanwave", "iw_get_meanwave", "iw_postp", "iw_speclegen", "iw_set_sgs", "iw_kepe", "iw_kepe_noalloc", "iw_energy_records",
+    "iw_set_meanwave", "iw_get_meanwave", "iw_postp", "iw_speclegen", "iw_set_sgs", "iw_kepe", "iw_kepe_noalloc", "iw_set_mixed", "iw_energy_records",
     "iw_speclegen_records",
 ]
 
@@ -81,6 +81,7 @@ def lib():
     L.iw_set_sgs.argtypes = [P, I, P]
     L.iw_kepe.argtypes = [P, P]
     L.iw_kepe_noalloc.argtypes = [P, P]
+    L.iw_set_mixed.argtypes = [P, I]
     L.iw_speclegen.argtypes = [P, I, P]
     L.iw_energy_records.argtypes = [P, P, I]
     L.iw_speclegen_records.argtypes = [P, P, I]
@@ -145,6 +146,10 @@ class Solver:
         c = np.zeros(4 * max(nsteps, 1), dtype=np.int32)
         _check(self.L.iw_run(self.h, nsteps, c.ctypes.data_as(ctypes.c_void_p) if counts else None))
         return c[:4 * nsteps].reshape(nsteps, 4)
+
+    def set_mixed(self, on=True):
+        """FP32 physical-space working set in convec (iw_set_mixed)."""
+        _check(self.L.iw_set_mixed(self.h, 1 if on else 0))
 
     def set_dt(self, dt):
         _check(self.L.iw_set_dt(self.h, ctypes.c_double(dt)))

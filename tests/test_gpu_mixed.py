@@ -1,0 +1,79 @@
+"""The mixed-precision path (iw_set_mixed): FP32 physical-space working set in
+convec, with the z contraction on FP32 matrix cores, against the FP64 oracle.
+The bound is the one the mixed path is built for: kinetic and potential
+energy within 1e-4 relative of the FP64 run (here far tighter); every field
+within 1e-5 relative after the steps below.  It must differ from the plain
+path (FP32 rounding is visible), and the plain path must stay untouched."""
+import numpy as np
+import pytest
+
+from helpers import SOLUTION, STATE, case, rel
+
+pytestmark = pytest.mark.gpu
+
+ENERGY_TOL = 1e-4   # the mixed path's bound on KE and PE
+FIELD_TOL = 1e-5
+
+
+@pytest.fixture(scope="module")
+def rt():
+    from iwrepo_amd import runtime
+    runtime.lib()
+    return runtime
+
+
+def _params(name):
+    P, flds = case(name)
+    P.imaxcalc = 0
+    P.dtadapt = 0
+    return P, flds
+
+
+@pytest.mark.parametrize("name,steps", [("small", 20), ("small13", 20), ("small21", 20), ("c1", 200),
+                                        ("c1ns", 200)])
+def test_mixed_path_against_fp64_oracle(rt, oracle_mod, name, steps):
+    P, flds = _params(name)
+    print("mixed", name, steps, flush=True)
+    o = oracle_mod.Oracle(P)
+    o.set_initial(*flds)
+    g = rt.Solver(P)
+    g.set_mixed(True)
+    g.set_initial(*flds)
+    eo, eg = [], []
+    for n in range(steps // 10):
+        o.run(10)
+        g.run(10, counts=False)
+        eo.append(o.kepe())
+        eg.append(g.kepe())
+    eo, eg = np.array(eo), np.array(eg)
+    errs = {STATE[q]: rel(g.get_state(q), o.field(q)) for q in SOLUTION}
+    ke = [rel(eg[:, q], eo[:, q]) for q in range(4)]
+    print("fields", errs, "KE/PE series", ke, flush=True)
+    for q in range(4):
+        assert ke[q] < ENERGY_TOL, q
+    for q in range(12):
+        assert errs[STATE[q]] < FIELD_TOL, STATE[q]
+    assert max(errs[STATE[q]] for q in range(4)) > 1e-13  # FP32 rounding is there
+    g.close()
+
+
+def test_mixed_path_switches_back_to_plain(rt, oracle_mod):
+    """iw_set_mixed(0) restores the plain path exactly (1e-10 vs the oracle),
+    and the mixed path refuses what it does not cover."""
+    P, flds = _params("small")
+    g = rt.Solver(P)
+    g.set_mixed(True)
+    g.set_mixed(False)
+    g.set_initial(*flds)
+    g.run(10)
+    o = oracle_mod.Oracle(P)
+    o.set_initial(*flds)
+    o.run(10)
+    for q in range(12):
+        assert rel(g.get_state(q), o.field(q)) < 1e-10, STATE[q]
+    g.close()
+    P2, _ = case("small")  # imaxcalc = 1: maxcalc records are FP64-path only
+    h = rt.Solver(P2)
+    with pytest.raises(rt.IWError, match="imaxcalc"):
+        h.set_mixed(True)
+    h.close()
