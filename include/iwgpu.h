@@ -175,8 +175,9 @@ int iw_speclegen_records(iw_solver *s, double *out, int n);
  * derivatives of the fluxes and of u, v, w (on FP32 matrix cores) and the
  * nonlinear products -- keeps its working set in FP32; the state, the SS/BDF
  * assembly, the filters, the penalty solves and the projection stay FP64.
- * 3-D runs without maxcalc records, adaptive dt or mean-wave terms; orders
- * 8, 12, 16, 20, 24.  on = 0: the plain FP64 path (the default). */
+ * maxcalc's extrema come from the FP64 values of the inverse transform, its
+ * umaxt chain from FP32 u.  3-D runs without mean-wave terms; orders 8, 12,
+ * 16, 20, 24.  on = 0: the plain FP64 path (the default). */
 int iw_set_mixed(iw_solver *s, int on);
 
 /* ---- reference routines on host arrays (parity / Fortran-shim mode) ---- */

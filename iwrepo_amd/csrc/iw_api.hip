@@ -1231,11 +1231,12 @@ static McFuse mc_fuse(iw_solver *s, double cny, const double *w00)
 }
 
 // the maxcalc record (s->mc_rec) from the partials of that transform
-static void mc_record(iw_solver *s, int istep, double cny)
+static void mc_record(iw_solver *s, int istep, double cny, const float *u32 = nullptr)
 {
     const Geo &g = s->g;
     McArgs A;
     A.u = s->phys[0];
+    A.u32 = u32;
     A.spec = s->buf[0];
     A.cny = cny;
     A.dz = s->mc_dz;
@@ -1450,7 +1451,7 @@ static int convec_tail(iw_solver *s, int ispecfilternl)
 // The FP32 arrays live in the FP64 scratch of the plain path: phys[q] holds
 // the physical field (first half) and its right-hand side (second half),
 // grad[q], prod[q] and ox one FP32 array each.
-static int convec_mixed(iw_solver *s, int ispecfilternl)
+static int convec_mixed(iw_solver *s, int ispecfilternl, int mcstep = 0, int mcslot = 0)
 {
     const Geo &g = s->g;
     double **B = s->buf;
@@ -1470,6 +1471,9 @@ static int convec_mixed(iw_solver *s, int ispecfilternl)
         Scope sc(s, "zero_meanw");
         launch_zero_meanw(g, B[2], s->mc_w00, s->stream);
     }
+    // maxcalc's reductions in the inverse x pass read the FP64 values of the
+    // transform before they are stored in FP32; the umaxt chain reads FP32 u
+    const McFuse mf = mc_fuse(s, (double)(g.ny - 1), s->mc_w00);
     {   // u,v,w,temp (FP64 spectral) -> FP32, with i ky f of u,v,w
         FieldBatch by = batch({B[0], B[1], B[2], B[3]});
         for (int q = 0; q < 4; q++) by.g[q] = D(ph[q]);
@@ -1487,7 +1491,11 @@ static int convec_mixed(iw_solver *s, int ispecfilternl)
         bx.dx[2] = D(g32[6]);
         bx.kw = g.xw;
         Scope sc(s, "m_fft_x_inv", 17 * 4. * (double)g.nxppy * g.nz);
-        launch_fft_x(g, bx, g.nz, 1, s->stream, nullptr, kPrecF32);
+        launch_fft_x(g, bx, g.nz, 1, s->stream, mcstep > 0 ? &mf : nullptr, kPrecF32);
+    }
+    if (mcstep > 0) {
+        mc_record(s, mcstep, mf.cny, ph[0]);
+        if (mc_after(s, mcslot)) return -1;
     }
     {   // vertical fluxes and z gradients on the matrix cores
         float *o4[4] = {S32[0], S32[1], S32[2], ox32};
@@ -1901,8 +1909,6 @@ extern "C" int iw_set_mixed(iw_solver *s, int on)
         if (g.ndims != 3) return fail("iw_set_mixed: the mixed path is 3-D only (ny >= 2)");
         if (!solve_np_supported(g.nzloc, g.nsubd))
             return fail("iw_set_mixed: GLL order must be one of 8, 12, 16, 20, 24");
-        if (s->P.imaxcalc || s->P.dtadapt)
-            return fail("iw_set_mixed: set imaxcalc = 0 and dtadapt = 0 (no maxcalc records on the mixed path)");
         if (mean_active(s)) return fail("iw_set_mixed: mean-wave and shear-current terms are FP64 only");
     }
     s->mixed = on != 0;
@@ -2367,8 +2373,7 @@ extern "C" int iw_run(iw_solver *s, int nsteps, int *counts)
 {
     if (!s || nsteps < 0) return fail("iw_run: bad argument");
     const bool mc = s->P.imaxcalc || s->P.dtadapt;
-    if (s->mixed && (mc || mean_active(s)))
-        return fail("iw_run: the mixed path runs without maxcalc records, adaptive dt and mean-wave terms");
+    if (s->mixed && mean_active(s)) return fail("iw_run: the mixed path runs without mean-wave terms");
     const bool adapt = s->P.dtadapt != 0;
     s->mc_hist.clear();
     if (mc && !adapt && nsteps > s->mc_ring_cap) { // device ring of the step records
@@ -2399,7 +2404,7 @@ extern "C" int iw_run(iw_solver *s, int nsteps, int *counts)
         // taken there before dt or the coefficients are read
         project_dev(s, s->buf, 4);
         if (!mc && s->ivardt == 1) s->idtstep++; // main.f:660-664
-        if (s->mixed ? convec_mixed(s, s->P.ispecfilternl)
+        if (s->mixed ? convec_mixed(s, s->P.ispecfilternl, mc ? istep : 0, n)
                      : convec_dev(s, s->P.ispecfilternl, mc ? istep : 0, n))
             return -1;
         const bool ldtadapt = s->ivardt == 1 && s->idtstep < 2;

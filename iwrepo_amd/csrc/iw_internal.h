@@ -111,6 +111,7 @@ struct McFuse {
 // what the record kernels read after the transform
 struct McArgs {
     const double *u;             // physical u as transformed (convec's or maxcalc's)
+    const float *u32 = nullptr;  // the same in FP32 (mixed path), read instead of u when set
     const double *spec;          // spectral u (kx-Nyquist coefficients), used when cny != 0
     double cny;
     const double *dz;

@@ -98,7 +98,7 @@ __device__ void mc_block_reduce(McAcc &a)
 __device__ __forceinline__ double mc_u(const McArgs &A, int i, int j, int k)
 {
     const long o = (long)k * A.nxppy + (long)j * A.nxpp;
-    double x = A.u[o + i];
+    double x = A.u32 ? (double)A.u32[o + i] : A.u[o + i];
     if (A.cny != 0.0) {
         const double c = A.cny * A.spec[o + A.nx];
         x = (i & 1) ? x - c : x + c;
@@ -181,7 +181,7 @@ __global__ __launch_bounds__(kMcThreads) void k_mc_final(McArgs A, int istep, do
     auto probe = [&](long p) {
         if (p >= ntot) return __builtin_nan("");
         const int i = (int)(p % A.nxpp), j = (int)((p / A.nxpp) % ny), k = (int)(p / A.nxppy);
-        return i < A.nx ? mc_u(A, i, j, k) : A.u[p]; // the pads hold zeros
+        return i < A.nx ? mc_u(A, i, j, k) : (A.u32 ? (double)A.u32[p] : A.u[p]); // the pads hold zeros
     };
     rec[40] = probe(p1);
     rec[41] = probe(p2);

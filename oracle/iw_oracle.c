@@ -74,6 +74,15 @@ int iwo_nthreads(void)
     return 1;
 #endif
 }
+/* threads of the following calls (the 1-core baseline timing) */
+void iwo_set_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
 
 /* ------------------------------------------------------------------ */
 /* multistep.f:7-89 -- variable-step AB3/BDF3 coefficients             */

@@ -71,6 +71,7 @@ void iwo_run(iwo_state *s, int nsteps, int *counts);
 /* restart from 24 state arrays at time t with cdtin = cdt (main.f:563-634) */
 void iwo_set_restart(iwo_state *s, const double *state, double t, double cdt);
 int  iwo_nthreads(void);
+void iwo_set_threads(int n);
 /* estimation-model filter of u,v,w (filtering.f:8-93) on the spectral
    state; isgscalc also computes the SGS tensors (sgscalc.f:4-223) */
 void iwo_filtering(iwo_state *s, int isgscalc);

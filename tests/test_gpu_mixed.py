@@ -7,12 +7,29 @@ path (FP32 rounding is visible), and the plain path must stay untouched."""
 import numpy as np
 import pytest
 
-from helpers import SOLUTION, STATE, case, rel
+from helpers import STATE, case, rel
 
 pytestmark = pytest.mark.gpu
 
 ENERGY_TOL = 1e-4   # the mixed path's bound on KE and PE
-FIELD_TOL = 1e-5
+FIELD_TOL = 1e-5    # per field, relative to the velocity scale for u, v, w
+
+
+def _field_errs(g, o, qs):
+    """Error of each array: u, v, w (and their histories) relative to the
+    norm of the whole velocity of that level (v is noise-level in these
+    waves, so its own norm is no scale for FP32 rounding), rho relative to
+    its own norm."""
+    out = {}
+    for q in qs:
+        a, b = g.get_state(q), o.field(q)
+        lev = q - q % 4
+        if q % 4 < 3:
+            scale = np.sqrt(sum(np.linalg.norm(o.field(lev + c)) ** 2 for c in range(3)))
+        else:
+            scale = np.linalg.norm(b)
+        out[STATE[q]] = float(np.linalg.norm(a - b) / scale)
+    return out
 
 
 @pytest.fixture(scope="module")
@@ -46,7 +63,7 @@ def test_mixed_path_against_fp64_oracle(rt, oracle_mod, name, steps):
         eo.append(o.kepe())
         eg.append(g.kepe())
     eo, eg = np.array(eo), np.array(eg)
-    errs = {STATE[q]: rel(g.get_state(q), o.field(q)) for q in SOLUTION}
+    errs = _field_errs(g, o, range(12))
     ke = [rel(eg[:, q], eo[:, q]) for q in range(4)]
     print("fields", errs, "KE/PE series", ke, flush=True)
     for q in range(4):
@@ -72,8 +89,34 @@ def test_mixed_path_switches_back_to_plain(rt, oracle_mod):
     for q in range(12):
         assert rel(g.get_state(q), o.field(q)) < 1e-10, STATE[q]
     g.close()
-    P2, _ = case("small")  # imaxcalc = 1: maxcalc records are FP64-path only
+    P2, _ = case("small")  # mean-wave / shear terms: FP64 path only
+    P2.fcshear = 0.5
     h = rt.Solver(P2)
-    with pytest.raises(rt.IWError, match="imaxcalc"):
+    with pytest.raises(rt.IWError, match="mean-wave"):
         h.set_mixed(True)
     h.close()
+
+
+def test_mixed_path_maxcalc_records_and_adaptive_dt(rt, oracle_mod):
+    """maxcalc on the mixed path: the extrema and CFL maxima with their
+    positions (from the FP64 values of the inverse transform), the umaxt
+    chain on FP32 u, and the adaptive step: c1 with dt changes at steps 22
+    and 43 as on the FP64 path."""
+    P, flds = case("c1")
+    P.dtadapt = 1
+    o, g = oracle_mod.Oracle(P), rt.Solver(P)
+    g.set_mixed(True)
+    o.set_initial(*flds)
+    g.set_initial(*flds)
+    o.run(50)
+    g.run(50, counts=False)
+    ro, rg = o.maxcalc_records(), g.maxcalc_records(50)
+    assert [int(r[0]) for r in rg if r[37] == 1] == [int(r[0]) for r in ro if r[37] == 1] == [22, 43]
+    assert np.array_equal(rg[:, 35], ro[:, 35])  # dt after every call
+    for c in (2, 3, 6, 7, 10, 11, 14, 15, 22, 26, 30):  # extrema and CFL maxima
+        assert np.abs(rg[:, c] - ro[:, c]).max() <= 1e-5 * np.abs(ro[:, c]).max(), c
+    errs = _field_errs(g, o, range(4))
+    print(errs, flush=True)
+    for q in range(4):
+        assert errs[STATE[q]] < FIELD_TOL, q
+    g.close()
