@@ -180,6 +180,48 @@ int iw_speclegen_records(iw_solver *s, double *out, int n);
  * 16, 20, 24.  on = 0: the plain FP64 path (the default). */
 int iw_set_mixed(iw_solver *s, int on);
 
+/* ---- learned sub-grid closure (no counterpart in the reference, whose
+ * sub-grid model is the estimation filter with sgscalc's tensors,
+ * filtering.f:8-93, sgscalc.f:125-219) ----
+ * A per-point network 12 -> 64 -> 64 -> 64 -> 9 (tanh; BF16 matrix-core
+ * forward with FP32 accumulation, FP32 backward, Adam) predicts the SGS
+ * stresses t11 t12 t13 t22 t23 t33 and buoyancy fluxes th1 th2 th3 of a
+ * coarse grid from its filtered velocity gradients (dudx dudy dudz dvdx dvdy
+ * dvdz dwdx dwdy dwdz), d rho/dz, |grad_h rho| and z.  Training samples come
+ * from fine runs coarse-grained onto the coarse grid (sharp spectral cut-off
+ * in x and y, Legendre L2 projection in z onto coarse subdomains made of
+ * whole fine subdomains): tau_ij = f(u_i u_j) - f(u_i) f(u_j), theta_i =
+ * f(u_i rho) - f(u_i) f(rho).  In a coarse run (iw_set_closure) the
+ * predicted tensors enter rhsfm's flux form (solver.f:975-1063, 1155-1161). */
+typedef struct iw_mlp iw_mlp;
+/* parameters + normalisation floats: W1[64][16] b1[64] W2[64][64] b2[64]
+ * W3[64][64] b3[64] W4[16][64] b4[16] (unused pads zero), then the mean and
+ * deviation of the 12 inputs and of the 9 outputs */
+int iw_mlp_nparams(void);
+int iw_mlp_create(unsigned long long seed, iw_mlp **out);
+int iw_mlp_destroy(iw_mlp *m);
+int iw_mlp_get_params(iw_mlp *m, float *out);
+int iw_mlp_set_params(iw_mlp *m, const float *in);
+/* host arrays: raw inputs x (n x 12) -> predicted tensors y (n x 9) */
+int iw_mlp_forward(iw_mlp *m, const float *x, int n, float *y);
+/* mean squared error of the normalised outputs for targets t (n x 9) and its
+ * gradient with respect to the first nparams - 42 parameters */
+int iw_mlp_gradient(iw_mlp *m, const float *x, const float *t, int n, float *grad, float *loss);
+/* coarse-grain the fine instance's u,v,w,rho into the coarse instance's
+ * state and (m not null) append the coarse grid's nx ny nz samples to m's
+ * training set */
+int iw_closure_sample(iw_mlp *m, iw_solver *fine, iw_solver *coarse);
+/* size of the training set; copies the first n samples when n > 0 */
+long long iw_closure_dataset(iw_mlp *m, float *x, float *t, long long n);
+/* normalisation from the training set (unless set), then `steps` Adam steps
+ * on minibatches of nb samples; loss_hist (may be null) per step */
+int iw_mlp_train(iw_mlp *m, int steps, int nb, float lr, unsigned long long seed, float *loss_hist);
+/* the network's tensors at the coarse instance's sample points (nx ny nz x 9) */
+int iw_closure_predict(iw_mlp *m, iw_solver *coarse, float *tau);
+/* use the network in the coarse instance's steps (m null: off; plain FP64
+ * path, 3-D, no mean-wave terms) */
+int iw_set_closure(iw_solver *coarse, iw_mlp *m);
+
 /* ---- reference routines on host arrays (parity / Fortran-shim mode) ---- */
 /* horfft.f:2    subroutine horfft(b,is,nk): is=-1 physical->spectral */
 int iw_horfft(iw_solver *s, double *b, int is, int nk);

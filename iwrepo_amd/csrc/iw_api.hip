@@ -28,6 +28,8 @@ static int fail(const std::string &m)
     return -1;
 }
 
+int iw_fail(const std::string &m) { return fail(m); }
+
 #define HIPCHK(x)                                                                          \
     do {                                                                                   \
         hipError_t e_ = (x);                                                               \
@@ -988,6 +990,8 @@ extern "C" int iw_destroy(iw_solver *s)
     hipFree(s->sys.FB);
     hipFree(s->sys.PB);
     hipFree(s->sys.tri_bad);
+    for (double *q : s->cl_buf)
+        if (q) hipFree(q);
     hipFree(s->sys.ipiv);
     hipFree(s->sys.coef);
     hipFree(s->sys.sysinfo);
@@ -1178,6 +1182,21 @@ static void fft_dev(iw_solver *s, FieldBatch b, int is, int nk, int use = kFftMa
 
 static void horfft_dev(iw_solver *s, FieldBatch b, int is, int nk) { fft_dev(s, b, is, nk); }
 
+void iw_fft_batch(iw_solver *s, const std::vector<double *> &f, const std::vector<double *> &g, int is)
+{
+    for (size_t i = 0; i < f.size(); i += kMaxFields) {
+        FieldBatch b;
+        b.n = 0;
+        b.smap = b.dmap = nullptr;
+        for (size_t q = i; q < f.size() && q < i + kMaxFields; q++) {
+            b.f[b.n] = f[q];
+            b.g[b.n] = g[q];
+            b.n++;
+        }
+        fft_dev(s, b, is, s->g.nz);
+    }
+}
+
 static void batch_fft(iw_solver *s, std::vector<double *> v, int is)
 {
     for (size_t i = 0; i < v.size(); i += kMaxFields) {
@@ -1293,6 +1312,7 @@ static int convec_dev(iw_solver *s, int ispecfilternl, int mcstep = 0, int mcslo
         by.dy[0] = gr[1];
         by.dy[1] = gr[4];
         by.dy[2] = gr[7];
+        if (s->closure) by.dy[3] = s->cl_buf[1]; // d rho/dy for the closure's features
         by.kw = g.yw;
         {
             Scope sc(s, "fft_y_inv", 32. * g.nxh * g.ny * g.nz * 7);
@@ -1303,6 +1323,11 @@ static int convec_dev(iw_solver *s, int ispecfilternl, int mcstep = 0, int mcslo
         bx.dx[1] = gr[3];
         bx.dx[2] = gr[6];
         if (mean) bx.dx[3] = s->gT[0];
+        if (s->closure) { // d rho/dy to physical space, d rho/dx
+            bx.f[bx.n] = bx.g[bx.n] = s->cl_buf[1];
+            bx.n++;
+            bx.dx[3] = s->cl_buf[0];
+        }
         bx.kw = g.xw;
         {
             Scope sc(s, "fft_x_inv", 16. * g.nxppy * g.nz * (mean ? 11 : 10));
@@ -1342,7 +1367,7 @@ static int convec_dev(iw_solver *s, int ispecfilternl, int mcstep = 0, int mcslo
     if (g.ndims == 3) { // fluxes and z gradients: 4 fields in, 7 (or 8) out
         Scope sc(s, "dpdz_flux", (mean ? 96. : 88.) * nt);
         double *o4[4] = {S[0], S[1], S[2], s->ox};
-        double *dz4[4] = {gr[2], gr[5], gr[8], mean ? s->gT[1] : nullptr};
+        double *dz4[4] = {gr[2], gr[5], gr[8], mean ? s->gT[1] : (s->closure ? s->cl_buf[2] : nullptr)};
         launch_dpdz_conv_grad(g, ph, o4, dz4, s->stream);
     } else {
         Scope sc(s, "dpdz_flux", 64. * nt);
@@ -1360,6 +1385,10 @@ static int convec_dev(iw_solver *s, int ispecfilternl, int mcstep = 0, int mcslo
             mt.dtdz = s->gT[1];
         }
         launch_rhs_phys(g, s->P, s->t, ph, gr, S, s->ox, s->prod, s->stream, &mt);
+    }
+    if (s->closure) {
+        Scope sc(s, "closure");
+        if (g.ndims != 3 || iw_closure_apply(s, S, s->prod)) return -1;
     }
     double **pr = s->prod;
     if (g.ndims == 3) {
@@ -1910,6 +1939,7 @@ extern "C" int iw_set_mixed(iw_solver *s, int on)
         if (!solve_np_supported(g.nzloc, g.nsubd))
             return fail("iw_set_mixed: GLL order must be one of 8, 12, 16, 20, 24");
         if (mean_active(s)) return fail("iw_set_mixed: mean-wave and shear-current terms are FP64 only");
+        if (s->closure) return fail("iw_set_mixed: the learned closure runs on the plain FP64 path");
     }
     s->mixed = on != 0;
     return 0;

@@ -178,6 +178,10 @@ struct iw_solver {
     bool mw_on = false;
     // mixed path (iw_set_mixed): FP32 physical-space working set in convec
     bool mixed = false;
+    // learned closure (iw_set_closure): the network and 8 scratch fields
+    // (rho gradients x, y, z; the four vertical SGS fluxes; one more)
+    struct iw_mlp *closure = nullptr;
+    double *cl_buf[8] = {};
     double *gT[2] = {nullptr, nullptr};
     // postp (postp.f:9-76) and speclegencalc (filtering.f:1110-1221): the
     // x-means of u and of t11..t33 at every (y,z) line, d umean/dy and /dz,
@@ -195,6 +199,14 @@ struct iw_solver {
     hipEvent_t ev0, ev1;
     long long dev_bytes = 0;
 };
+
+// host helpers of iw_api.hip shared with iw_closure.hip: record a failure
+// for iw_last_error (returns -1); transform the fields f (spectral, is = 1,
+// or physical, is = -1) into g (may equal f) with horfft's conventions
+int iw_fail(const std::string &m);
+void iw_fft_batch(iw_solver *s, const std::vector<double *> &f, const std::vector<double *> &g, int is);
+// the closure's term in convec (iw_closure.hip)
+int iw_closure_apply(iw_solver *c, double *const *S, double *const *pr);
 
 namespace iw {
 // kernels launchers (iw_kernels.hip)

@@ -24,6 +24,9 @@ EXPORTS = [
     "iw_device_bytes", "iw_maxcalc", "iw_maxcalc_records", "iw_get_dtstate", "iw_set_dtstate",
     "iw_set_meanwave", "iw_get_meanwave", "iw_postp", "iw_speclegen", "iw_set_sgs", "iw_kepe", "iw_kepe_noalloc", "iw_set_mixed", "iw_energy_records",
     "iw_speclegen_records",
+    "iw_mlp_nparams", "iw_mlp_create", "iw_mlp_destroy", "iw_mlp_get_params", "iw_mlp_set_params",
+    "iw_mlp_forward", "iw_mlp_gradient", "iw_closure_sample", "iw_closure_dataset", "iw_mlp_train",
+    "iw_closure_predict", "iw_set_closure",
 ]
 
 MAXCALC_NREC = 42  # IW_MAXCALC_NREC
@@ -93,6 +96,18 @@ def lib():
     L.iw_kernel_stats.argtypes = [P, ctypes.c_char_p, I, P, P, P, I]
     L.iw_device_bytes.restype = ctypes.c_longlong
     L.iw_device_bytes.argtypes = [P]
+    L.iw_mlp_create.argtypes = [ctypes.c_ulonglong, ctypes.POINTER(P)]
+    L.iw_mlp_destroy.argtypes = [P]
+    L.iw_mlp_get_params.argtypes = [P, P]
+    L.iw_mlp_set_params.argtypes = [P, P]
+    L.iw_mlp_forward.argtypes = [P, P, I, P]
+    L.iw_mlp_gradient.argtypes = [P, P, P, I, P, P]
+    L.iw_closure_sample.argtypes = [P, P, P]
+    L.iw_closure_dataset.restype = ctypes.c_longlong
+    L.iw_closure_dataset.argtypes = [P, P, P, ctypes.c_longlong]
+    L.iw_mlp_train.argtypes = [P, I, I, ctypes.c_float, ctypes.c_ulonglong, P]
+    L.iw_closure_predict.argtypes = [P, P, P]
+    L.iw_set_closure.argtypes = [P, P]
     if L.iw_params_size() != ctypes.sizeof(IWParams):
         raise IWError("iw_params layout mismatch")
     _lib = L
@@ -342,3 +357,96 @@ def read_restart_file(path, P):
     _check(lib().iw_read_restart_file(os.fsencode(path), ctypes.byref(P), ctypes.byref(t), ctypes.byref(umax),
                                       _ptr(st)))
     return st.reshape((24, P.nzloc * P.nsubd, P.ny, P.nx + 2)), t.value, umax.value
+
+
+def _fptr(a):
+    if a.dtype != np.float32 or not a.flags.c_contiguous:
+        raise ValueError("arrays must be C-contiguous float32")
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class Closure:
+    """The learned sub-grid closure network (iw_mlp_*, iw_closure_*)."""
+
+    NIN, NOUT = 12, 9
+
+    def __init__(self, seed=1):
+        self.L = lib()
+        h = ctypes.c_void_p()
+        _check(self.L.iw_mlp_create(ctypes.c_ulonglong(seed), ctypes.byref(h)))
+        self.h = h
+        self.nparams = self.L.iw_mlp_nparams()
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.iw_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def params(self):
+        out = np.zeros(self.nparams, dtype=np.float32)
+        _check(self.L.iw_mlp_get_params(self.h, _fptr(out)))
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        if p.size != self.nparams:
+            raise ValueError("expected %d values" % self.nparams)
+        _check(self.L.iw_mlp_set_params(self.h, _fptr(p)))
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.zeros((len(x), self.NOUT), dtype=np.float32)
+        _check(self.L.iw_mlp_forward(self.h, _fptr(x), len(x), _fptr(y)))
+        return y
+
+    def gradient(self, x, t):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        g = np.zeros(self.nparams - 2 * (self.NIN + self.NOUT), dtype=np.float32)
+        loss = ctypes.c_float()
+        _check(self.L.iw_mlp_gradient(self.h, _fptr(x), _fptr(t), len(x), _fptr(g), ctypes.byref(loss)))
+        return loss.value, g
+
+    def sample(self, fine, coarse):
+        """Coarse-grain fine's state into coarse's and add coarse's samples."""
+        _check(self.L.iw_closure_sample(self.h, fine.h, coarse.h))
+
+    def dataset(self, n=None):
+        m = self.L.iw_closure_dataset(self.h, None, None, 0)
+        if m < 0:
+            raise IWError(self.L.iw_last_error().decode())
+        n = m if n is None else n
+        x = np.zeros((n, self.NIN), dtype=np.float32)
+        t = np.zeros((n, self.NOUT), dtype=np.float32)
+        if n:
+            if self.L.iw_closure_dataset(self.h, _fptr(x), _fptr(t), n) < 0:
+                raise IWError(self.L.iw_last_error().decode())
+        return x, t
+
+    def train(self, steps, batch=8192, lr=1e-3, seed=7):
+        hist = np.zeros(max(steps, 1), dtype=np.float32)
+        _check(self.L.iw_mlp_train(self.h, steps, batch, ctypes.c_float(lr), ctypes.c_ulonglong(seed), _fptr(hist)))
+        return hist[:steps]
+
+    def predict(self, coarse):
+        """tau at coarse's sample points: (nz, ny, nx, 9)."""
+        P = coarse.P
+        out = np.zeros((P.nzloc * P.nsubd * P.ny * P.nx, self.NOUT), dtype=np.float32)
+        _check(self.L.iw_closure_predict(self.h, coarse.h, _fptr(out)))
+        return out.reshape(P.nzloc * P.nsubd, P.ny, P.nx, self.NOUT)
+
+
+def coarse_grain(fine, coarse):
+    """Coarse-grain fine's u,v,w,rho into coarse's state (no samples kept)."""
+    _check(lib().iw_closure_sample(None, fine.h, coarse.h))
+
+
+def set_closure(coarse, closure):
+    _check(lib().iw_set_closure(coarse.h, closure.h if closure is not None else None))
+
