@@ -169,6 +169,14 @@ int iw_kepe_noalloc(iw_solver *s, double *out);
 int iw_energy_records(iw_solver *s, double *out, int n);
 int iw_speclegen_records(iw_solver *s, double *out, int n);
 
+/* Threads per block of the horizontal transforms (64, 128 or 256, the
+ * default): every block holds 8 values per thread, i.e. 8 ft / (nx/2) rows
+ * of the x transform or 8 ft / ny columns of the y transform; smaller blocks
+ * give small grids more blocks.  The results do not depend on it (each
+ * sequence takes the same passes).  A tuning knob with no counterpart in the
+ * reference (iw_ensemble_tune sets it per member). */
+int iw_set_fft_threads(iw_solver *s, int ft);
+
 /* Mixed precision (no counterpart in the reference, whose arithmetic is FP64
  * throughout).  on = 1: the physical-space part of convec (solver.f:6-514,
  * 523-1206) -- the inverse and forward horizontal transforms, the z
@@ -221,6 +229,28 @@ int iw_closure_predict(iw_mlp *m, iw_solver *coarse, float *tau);
 /* use the network in the coarse instance's steps (m null: off; plain FP64
  * path, 3-D, no mean-wave terms) */
 int iw_set_closure(iw_solver *coarse, iw_mlp *m);
+
+/* ---- ensembles (no counterpart in the reference, which sizes one run at
+ * compile time, dim.h:12-28) ----
+ * Independent instances of uneven sizes (each from iw_create and
+ * iw_set_initial, on one device) advanced together: after their start-up
+ * steps, three steps of every member are captured once into one HIP graph
+ * (members dealt to independent chains) and replayed, one launch per three
+ * steps.  Members run without maxcalc records, adaptive dt, the estimation
+ * filter, the output routines and the paddle; their state after
+ * iw_ensemble_run equals that of iw_run on each member. */
+typedef struct iw_ensemble iw_ensemble;
+int iw_ensemble_create(iw_solver **members, int n, iw_ensemble **out);
+int iw_ensemble_destroy(iw_ensemble *e);
+int iw_ensemble_run(iw_ensemble *e, int nsteps);
+/* Autotuner: per member the transform block size (iw_set_fft_threads), then
+ * the number of graph chains (or no graph), each timed over `steps` steps on
+ * the members' own state, which is restored.  out (may be null): chosen
+ * chains (0: no graph), ms per ensemble step chosen, ms with all chains and
+ * 256 threads. */
+int iw_ensemble_tune(iw_ensemble *e, int steps, double *out);
+int iw_ensemble_config(iw_ensemble *e, int *lanes, int *ft);
+int iw_ensemble_set_lanes(iw_ensemble *e, int lanes);
 
 /* ---- reference routines on host arrays (parity / Fortran-shim mode) ---- */
 /* horfft.f:2    subroutine horfft(b,is,nk): is=-1 physical->spectral */

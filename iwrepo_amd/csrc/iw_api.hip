@@ -905,7 +905,8 @@ extern "C" int iw_create(const iw_params *p, iw_solver **out)
         for (int k = 0; k < g.nz; k++) rdz[k] = 1. / dz[k];
         if (upload(&s->mc_rm, rm, acc) || upload(&s->mc_dz, dz, acc) || upload(&s->mc_rdz, rdz, acc)) return -1;
         const int nxy = P.nx * P.ny;
-        HIPCHK(hipMalloc((void **)&s->mc_part, sizeof(double) * 2 * kMcQ * (size_t)mc_blocks(g)));
+        // sized for the smallest transform block (the most blocks)
+        HIPCHK(hipMalloc((void **)&s->mc_part, sizeof(double) * 2 * kMcQ * (size_t)mc_blocks(g, 64)));
         HIPCHK(hipMalloc((void **)&s->mc_rec, sizeof(double) * kMcRec));
         HIPCHK(hipMemset(s->mc_rec, 0, sizeof(double) * kMcRec));
         HIPCHK(hipMalloc((void **)&s->mc_w00, sizeof(double) * g.nz));
@@ -1927,6 +1928,27 @@ extern "C" int iw_kepe_noalloc(iw_solver *s, double *out)
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(out, ptr + 4 * (size_t)g.nz, 4 * sizeof(double), hipMemcpyDeviceToHost, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
+    return 0;
+}
+
+// The thread count of the horizontal transforms' blocks (each holds 8
+// values per thread, so it bounds the rows and columns per block).
+bool iw::fft_threads_ok(const Geo &g, int ft)
+{
+    if (ft != 64 && ft != 128 && ft != 256) return false;
+    if (2 * g.M > 8 * ft) return false;                      // two rows per block (forward with flux products)
+    if (g.ndims == 3 && g.ny > 8 * ft) return false;         // one column per block
+    return true;
+}
+
+extern "C" int iw_set_fft_threads(iw_solver *s, int ft)
+{
+    if (!s) return fail("iw_set_fft_threads: null solver");
+    if (!fft_threads_ok(s->g, ft))
+        return fail("iw_set_fft_threads: " + std::to_string(ft) + " threads do not fit this grid (64, 128 or 256 with "
+                    "8 ft >= nx and 8 ft >= ny)");
+    HIPCHK(hipStreamSynchronize(s->stream));
+    s->g.ft = ft;
     return 0;
 }
 

@@ -41,6 +41,7 @@ struct Geo {
     long nxppy, ntot;
     int M, logM, logNy; // x complex length nx/2 and logs
     double scaley;
+    int ft = 256; // threads per transform block (64, 128 or 256), 8 values each
     // device tables
     double *D, *D2, *AL, *wg, *sig, *xw, *yw, *zh, *z, *damp, *fcf, *x;
     double *sigbv; // Boyd-Vandeven filter of the estimation model (filtering.f:1272-1310)
@@ -210,7 +211,9 @@ int iw_closure_apply(iw_solver *c, double *const *S, double *const *pr);
 
 namespace iw {
 // kernels launchers (iw_kernels.hip)
-int mc_blocks(const Geo &g);
+// blocks of the inverse x transform (maxcalc partials); ft > 0: for that
+// block size instead of g.ft
+int mc_blocks(const Geo &g, int ft = 0);
 void launch_maxcalc_final(const Geo &g, const McArgs &A, int istep, double t, double umax, double *rec,
                           hipStream_t st);
 void launch_maxcalc_umaxt(const Geo &g, const McArgs &A, double umax, double *colmax, double *groupmax,
@@ -313,6 +316,8 @@ void launch_viscous(const Geo &g, const iw_params &P, const SysTables &S, double
 void launch_zero_meanw(const Geo &g, double *w, double *w00, hipStream_t st);
 void launch_zero_int(int *p, int n, hipStream_t st);
 bool solve_np_supported(int nl, int nsubd);
+// whether ft threads per transform block fit the grid (iw_set_fft_threads)
+bool fft_threads_ok(const Geo &g, int ft);
 void launch_lfilter(const Geo &g, double *const *uvwt, int cpen, hipStream_t st);
 // estimation-model filter: modecut of f into out on levels kmin..kmax (u0 scratch)
 void launch_modecut(const Geo &g, const double *f, double *u0, double *out, int kmin, int kmax, hipStream_t st);

@@ -18,8 +18,8 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 b)
 }
 
 // ---------------------------------------------------------------------------
-// In-place multi-radix Stockham FFT on LDS.  A block of kFT threads holds
-// kFE = 8 kFT complex values (nrows sequences of length N, N a power of two),
+// In-place multi-radix Stockham FFT on LDS.  A block of T threads holds
+// 8 T complex values (nrows sequences of length N, N a power of two),
 // so every pass of radix R has exactly 8/R butterflies per thread.
 // Passes are radix 8 while at least three factors of two remain, then one
 // radix-4 or radix-2 pass.  Each pass reads its butterflies into registers,
@@ -29,11 +29,9 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 b)
 // aligned runs of 8 or 16 values hit distinct 16-byte bank groups.
 // sign -1: sum g_j e^{-2 pi i jk/N}; +1: e^{+...}.  tw[t] = e^{-2 pi i t/N}.
 // ---------------------------------------------------------------------------
-#ifndef IW_FFT_THREADS
-#define IW_FFT_THREADS 256
-#endif
-constexpr int kFT = IW_FFT_THREADS;
-constexpr int kFE = 8 * kFT;
+// The block size is a run-time choice (Geo::ft: 64, 128 or 256 threads); a
+// block always holds 8 values per thread.
+constexpr int kFT = 256; // largest block
 
 __host__ __device__ __forceinline__ int lp(int i) { return i ^ ((i >> 3) & 15); }
 
@@ -95,7 +93,7 @@ __device__ __forceinline__ void lds_pass(double2 *buf, int logN, int logNs, int 
     int base[BPT], rr[BPT];
 #pragma unroll
     for (int i = 0; i < BPT; i++) {
-        const int b = threadIdx.x + i * kFT;
+        const int b = threadIdx.x + i * (int)blockDim.x;
         base[i] = -1;
         rr[i] = 0;
         if (b < nb) {
@@ -348,18 +346,18 @@ __global__ __launch_bounds__(kFT) void k_fft_x(FieldBatch fb, int nrows, int ny,
     auto load_inverse = [&](bool deriv) {
         // all of a thread's loads first (rows past the end read the last row
         // and are zeroed below), so they are in flight together
-        double2 va[kFE / kFT], vb[kFE / kFT];
+        double2 va[8], vb[8];
 #pragma unroll
-        for (int it = 0; it < kFE / kFT; it++) {
-            const int idx = threadIdx.x + it * kFT;
+        for (int it = 0; it < 8; it++) {
+            const int idx = threadIdx.x + it * (int)blockDim.x;
             const int r = idx >> logM, k = idx & (M - 1), row = row0 + r;
             const TI *p = base + srow(row < nrows ? row : nrows - 1);
             va[it] = ld2(p + 2 * k);
             vb[it] = ld2(p + 2 * (M - k));
         }
 #pragma unroll
-        for (int it = 0; it < kFE / kFT; it++) {
-            const int idx = threadIdx.x + it * kFT;
+        for (int it = 0; it < 8; it++) {
+            const int idx = threadIdx.x + it * (int)blockDim.x;
             const int r = idx >> logM, k = idx & (M - 1), row = row0 + r;
             double2 a = va[it], b = vb[it];
             if (!deriv && k == 0) nyq[r] = b; // kx-Nyquist coefficient of the row
@@ -387,8 +385,8 @@ __global__ __launch_bounds__(kFT) void k_fft_x(FieldBatch fb, int nrows, int ny,
     };
     auto store_inverse = [&](TO *ob) {
 #pragma unroll
-        for (int it = 0; it < kFE / kFT; it++) {
-            const int idx = threadIdx.x + it * kFT;
+        for (int it = 0; it < 8; it++) {
+            const int idx = threadIdx.x + it * (int)blockDim.x;
             int r = idx >> logM, m = idx & (M - 1), row = row0 + r;
             if (row >= nrows) continue;
             st2(ob + drow(row) + 2 * m, buf[lp(idx)]);
@@ -411,8 +409,8 @@ __global__ __launch_bounds__(kFT) void k_fft_x(FieldBatch fb, int nrows, int ny,
         const double sc = 1.0 / (double)N;
         auto load_fwd = [&](const TI *src) {
 #pragma unroll
-            for (int it = 0; it < kFE / kFT; it++) {
-                const int idx = threadIdx.x + it * kFT;
+            for (int it = 0; it < 8; it++) {
+                const int idx = threadIdx.x + it * (int)blockDim.x;
                 int r = idx >> logM, m = idx & (M - 1), row = row0 + r;
                 double2 z = ld2(src + srow(row < nrows ? row : nrows - 1) + 2 * m);
                 if (row >= nrows) z = make_double2(0., 0.);
@@ -448,8 +446,8 @@ __global__ __launch_bounds__(kFT) void k_fft_x(FieldBatch fb, int nrows, int ny,
             const TI *acc = reinterpret_cast<const TI *>(fb.ax[blockIdx.y]);
             TO *axo = reinterpret_cast<TO *>(fb.axo[blockIdx.y]);
 #pragma unroll
-            for (int it = 0; it < kFE / kFT; it++) {
-                const int idx = threadIdx.x + it * kFT;
+            for (int it = 0; it < 8; it++) {
+                const int idx = threadIdx.x + it * (int)blockDim.x;
                 const int r = idx >> logM, m = idx & (M - 1);
                 const bool isp = r >= H;
                 const int row = rowA + (isp ? r - H : r);
@@ -460,8 +458,8 @@ __global__ __launch_bounds__(kFT) void k_fft_x(FieldBatch fb, int nrows, int ny,
             __syncthreads();
             fft_lds<false>(buf, logM, R, tw, DIR);
 #pragma unroll
-            for (int it = 0; it < kFE / kFT; it++) {
-                const int idx = threadIdx.x + it * kFT;
+            for (int it = 0; it < 8; it++) {
+                const int idx = threadIdx.x + it * (int)blockDim.x;
                 const int r = idx >> logM, k = idx & (M - 1), row = rowA + r;
                 if (r >= H || row >= nrows) continue;
                 TO *orow = obase + drow(row);
@@ -479,8 +477,8 @@ __global__ __launch_bounds__(kFT) void k_fft_x(FieldBatch fb, int nrows, int ny,
             __syncthreads();
             fft_lds<false>(buf, logM, R, tw, DIR);
 #pragma unroll
-            for (int it = 0; it < kFE / kFT; it++) {
-                const int idx = threadIdx.x + it * kFT;
+            for (int it = 0; it < 8; it++) {
+                const int idx = threadIdx.x + it * (int)blockDim.x;
                 int r = idx >> logM, k = idx & (M - 1), row = row0 + r;
                 if (row >= nrows) continue;
                 TO *orow = obase + drow(row);
@@ -499,8 +497,8 @@ __global__ __launch_bounds__(kFT) void k_fft_x(FieldBatch fb, int nrows, int ny,
             double b0 = big, b1 = -big, b2 = -big; // min, max of the value; max of the CFL number
             int l0 = -1, l1 = -1, l2 = -1;
 #pragma unroll
-            for (int it = 0; it < kFE / kFT; it++) {
-                const int idx = threadIdx.x + it * kFT;
+            for (int it = 0; it < 8; it++) {
+                const int idx = threadIdx.x + it * (int)blockDim.x;
                 const int r = idx >> logM, m = idx & (M - 1), row = row0 + r;
                 if (row >= nrows) continue;
                 const int k = row >> lny, j = row & (ny - 1);
@@ -553,7 +551,7 @@ __global__ __launch_bounds__(kFT) void k_fft_x(FieldBatch fb, int nrows, int ny,
             }
             __syncthreads();
             if (threadIdx.x == 0) {
-                for (int w = 1; w < kFT / 64; w++) {
+                for (int w = 1; w < (int)blockDim.x / 64; w++) {
                     mc_take(true, rv[w][0], rl[w][0], b0, l0);
                     mc_take(false, rv[w][1], rl[w][1], b1, l1);
                     mc_take(false, rv[w][2], rl[w][2], b2, l2);
@@ -604,17 +602,17 @@ __global__ __launch_bounds__(kFT) void k_fft_y(FieldBatch fb, int nxpp, int nxh,
     auto load = [&](const TI *src, bool deriv) {
         // all loads first (columns past nxh are clamped and their values
         // dropped), so they are in flight together; the tile is always
-        // CB x ny = kFE values
-        double2 v[kFE / kFT];
+        // CB x ny = 8 blockDim values
+        double2 v[8];
 #pragma unroll
-        for (int it = 0; it < kFE / kFT; it++) {
-            const int idx = threadIdx.x + it * kFT;
+        for (int it = 0; it < 8; it++) {
+            const int idx = threadIdx.x + it * (int)blockDim.x;
             const int j = idx >> lcb, ii = ii0 + (idx & (CB - 1));
             v[it] = ld2(src + (long)j * nxpp + 2 * (ii < nxh ? ii : nxh - 1));
         }
 #pragma unroll
-        for (int it = 0; it < kFE / kFT; it++) {
-            const int idx = threadIdx.x + it * kFT;
+        for (int it = 0; it < 8; it++) {
+            const int idx = threadIdx.x + it * (int)blockDim.x;
             int j = idx >> lcb, c = idx & (CB - 1), ii = ii0 + c;
             double2 z = v[it];
             if (ii >= nxh) z = make_double2(0., 0.);
@@ -630,11 +628,11 @@ __global__ __launch_bounds__(kFT) void k_fft_y(FieldBatch fb, int nxpp, int nxh,
     const TI *accp = (DIR < 0 && fb.ay[blockIdx.z])
                          ? reinterpret_cast<const TI *>(fb.ay[blockIdx.z]) + (long)(fb.smap ? fb.smap[lpl] : lpl) * nxpp * ny
                          : nullptr;
-    double2 pb[kFE / kFT];
+    double2 pb[8];
     auto store = [&](TO *op, bool add) {
 #pragma unroll
-        for (int it = 0; it < kFE / kFT; it++) {
-            const int idx = threadIdx.x + it * kFT;
+        for (int it = 0; it < 8; it++) {
+            const int idx = threadIdx.x + it * (int)blockDim.x;
             int j = idx >> lcb, c = idx & (CB - 1), ii = ii0 + c;
             if (ii >= nxh) continue;
             double2 z = buf[(j << lcb) + c];
@@ -663,8 +661,8 @@ __global__ __launch_bounds__(kFT) void k_fft_y(FieldBatch fb, int nxpp, int nxh,
         __syncthreads();
         fft_lds<true>(buf, logNy, CB, tw, DIR);
 #pragma unroll
-        for (int it = 0; it < kFE / kFT; it++) {
-            const int idx = threadIdx.x + it * kFT;
+        for (int it = 0; it < 8; it++) {
+            const int idx = threadIdx.x + it * (int)blockDim.x;
             double2 z = make_double2(0., 0.);
             if (idx < CB * ny) {
                 z = buf[((idx >> lcb) << lcb) + (idx & (CB - 1))];
@@ -703,7 +701,7 @@ __global__ __launch_bounds__(kFT) void k_fft_y(FieldBatch fb, int nxpp, int nxh,
 template <bool MAPS, typename TI, typename TO>
 static void fft_x_launch(const Geo &g, const FieldBatch &b, int nk, int dir, hipStream_t st, const McFuse *mf)
 {
-    const int R = kFE / g.M;
+    const int R = 8 * g.ft / g.M;
     int nrows = g.ny * nk;
     dim3 grid((nrows + R - 1) / R, b.n);
     size_t sh = sizeof(double2) * ((size_t)R * g.M + g.M + R + g.M + 1) + sizeof(double) * (g.M + 1 + 4 * R) +
@@ -727,13 +725,13 @@ static void fft_x_launch(const Geo &g, const FieldBatch &b, int nk, int dir, hip
         }
         if (ba.n)
             hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fft_x<-1, true, MAPS, TI, TO>),
-                               dim3((nrows + R / 2 - 1) / (R / 2), ba.n), dim3(kFT), sh, st, ba, nrows, g.ny, g.nxpp,
+                               dim3((nrows + R / 2 - 1) / (R / 2), ba.n), dim3(g.ft), sh, st, ba, nrows, g.ny, g.nxpp,
                                g.M, g.logM, R, g.twM, g.twN, g.scaley, none);
         if (bp.n)
-            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fft_x<-1, false, MAPS, TI, TO>), dim3(grid.x, bp.n), dim3(kFT), sh,
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fft_x<-1, false, MAPS, TI, TO>), dim3(grid.x, bp.n), dim3(g.ft), sh,
                                st, bp, nrows, g.ny, g.nxpp, g.M, g.logM, R, g.twM, g.twN, g.scaley, none);
     } else {
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fft_x<1, false, MAPS, TI, TO>), grid, dim3(kFT), sh, st, b, nrows,
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fft_x<1, false, MAPS, TI, TO>), grid, dim3(g.ft), sh, st, b, nrows,
                            g.ny, g.nxpp, g.M, g.logM, R, g.twM, g.twN, g.scaley, mf ? *mf : none);
     }
 }
@@ -748,26 +746,30 @@ void launch_fft_x(const Geo &g, FieldBatch b, int nk, int dir, hipStream_t st, c
         fft_x_launch<false, double, double>(g, b, nk, dir, st, mf);
 }
 
-int mc_blocks(const Geo &g) { return (g.ny * g.nz + kFE / g.M - 1) / (kFE / g.M); }
+int mc_blocks(const Geo &g, int ft)
+{
+    const int R = 8 * (ft > 0 ? ft : g.ft) / g.M;
+    return (g.ny * g.nz + R - 1) / R;
+}
 
 void launch_fft_y(const Geo &g, FieldBatch b, int nk, int dir, hipStream_t st, int prec)
 {
     if (g.ndims != 3) return;
-    int CB = kFE / g.ny;
+    int CB = 8 * g.ft / g.ny;
     dim3 grid((g.nxh + CB - 1) / CB, nk, b.n);
     size_t sh = sizeof(double2) * ((size_t)CB * g.ny + g.ny) + sizeof(double) * g.ny;
     // mixed path: spectral FP64 <-> FP32 intermediate
     if (dir < 0 && prec == kPrecF32)
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fft_y<-1, float, double>), grid, dim3(kFT), sh, st, b, g.nxpp, g.nxh,
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fft_y<-1, float, double>), grid, dim3(g.ft), sh, st, b, g.nxpp, g.nxh,
                            g.ny, g.logNy, CB, g.twY, g.scaley);
     else if (prec == kPrecF32)
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fft_y<1, double, float>), grid, dim3(kFT), sh, st, b, g.nxpp, g.nxh,
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fft_y<1, double, float>), grid, dim3(g.ft), sh, st, b, g.nxpp, g.nxh,
                            g.ny, g.logNy, CB, g.twY, g.scaley);
     else if (dir < 0)
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fft_y<-1>), grid, dim3(kFT), sh, st, b, g.nxpp, g.nxh, g.ny, g.logNy,
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fft_y<-1>), grid, dim3(g.ft), sh, st, b, g.nxpp, g.nxh, g.ny, g.logNy,
                            CB, g.twY, g.scaley);
     else
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fft_y<1>), grid, dim3(kFT), sh, st, b, g.nxpp, g.nxh, g.ny, g.logNy,
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fft_y<1>), grid, dim3(g.ft), sh, st, b, g.nxpp, g.nxh, g.ny, g.logNy,
                            CB, g.twY, g.scaley);
 }
 

@@ -134,3 +134,25 @@ CONFIGS = {
 
 def config_params(name):
     return make_params(**CONFIGS[name]["params"])
+
+
+def c3_members(n=64, seed=0x1D5EC0F1 + 3):
+    """The batched-ensemble workload: n independent instances drawn from one
+    seed (NumPy PCG64): Nx, Ny uniform in {32, 64, 128}, 2-8 subdomains of GLL
+    order 8-24, wave amplitude uniform in [0.01, 0.1], Re log-uniform in
+    [1e3, 1e5]; the box, Pr, dt and the no-slip bottom of the single large
+    instance, dt 1e-3; noise 1e-3.  Returns [(params, wave, noise, ic_seed)]."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n):
+        nx, ny = (int(v) for v in rng.choice([32, 64, 128], size=2))
+        nsubd = int(rng.integers(2, 9))
+        order = int(rng.integers(8, 25))
+        wave = float(rng.uniform(0.01, 0.1))
+        re = float(10 ** rng.uniform(3.0, 5.0))
+        P = make_params(nx=nx, ny=ny, nsubd=nsubd, order=order, xlen=4 * math.pi, ylen=2 * math.pi, re=re,
+                        pr=7.0, dt=1e-3, noslip=True, imaxcalc=0)
+        out.append((P, wave, 1e-3, 1000 + k))
+    return out
+

@@ -22,11 +22,13 @@ EXPORTS = [
     "iw_save_restart", "iw_load_restart", "iw_write_restart_file", "iw_read_restart_file", "iw_abvar1", "iw_abvar2", "iw_abvar3", "iw_bdfvarv",
     "iw_bdfvar1", "iw_bdfvar2", "iw_bdfvar3", "iw_set_profiling", "iw_kernel_stats",
     "iw_device_bytes", "iw_maxcalc", "iw_maxcalc_records", "iw_get_dtstate", "iw_set_dtstate",
-    "iw_set_meanwave", "iw_get_meanwave", "iw_postp", "iw_speclegen", "iw_set_sgs", "iw_kepe", "iw_kepe_noalloc", "iw_set_mixed", "iw_energy_records",
+    "iw_set_meanwave", "iw_get_meanwave", "iw_postp", "iw_speclegen", "iw_set_sgs", "iw_kepe", "iw_kepe_noalloc", "iw_set_mixed", "iw_set_fft_threads", "iw_energy_records",
     "iw_speclegen_records",
     "iw_mlp_nparams", "iw_mlp_create", "iw_mlp_destroy", "iw_mlp_get_params", "iw_mlp_set_params",
     "iw_mlp_forward", "iw_mlp_gradient", "iw_closure_sample", "iw_closure_dataset", "iw_mlp_train",
     "iw_closure_predict", "iw_set_closure",
+    "iw_ensemble_create", "iw_ensemble_destroy", "iw_ensemble_run", "iw_ensemble_tune", "iw_ensemble_config",
+    "iw_ensemble_set_lanes",
 ]
 
 MAXCALC_NREC = 42  # IW_MAXCALC_NREC
@@ -85,6 +87,7 @@ def lib():
     L.iw_kepe.argtypes = [P, P]
     L.iw_kepe_noalloc.argtypes = [P, P]
     L.iw_set_mixed.argtypes = [P, I]
+    L.iw_set_fft_threads.argtypes = [P, I]
     L.iw_speclegen.argtypes = [P, I, P]
     L.iw_energy_records.argtypes = [P, P, I]
     L.iw_speclegen_records.argtypes = [P, P, I]
@@ -108,6 +111,12 @@ def lib():
     L.iw_mlp_train.argtypes = [P, I, I, ctypes.c_float, ctypes.c_ulonglong, P]
     L.iw_closure_predict.argtypes = [P, P, P]
     L.iw_set_closure.argtypes = [P, P]
+    L.iw_ensemble_create.argtypes = [P, I, ctypes.POINTER(P)]
+    L.iw_ensemble_destroy.argtypes = [P]
+    L.iw_ensemble_run.argtypes = [P, I]
+    L.iw_ensemble_tune.argtypes = [P, I, P]
+    L.iw_ensemble_config.argtypes = [P, P, P]
+    L.iw_ensemble_set_lanes.argtypes = [P, I]
     if L.iw_params_size() != ctypes.sizeof(IWParams):
         raise IWError("iw_params layout mismatch")
     _lib = L
@@ -165,6 +174,9 @@ class Solver:
     def set_mixed(self, on=True):
         """FP32 physical-space working set in convec (iw_set_mixed)."""
         _check(self.L.iw_set_mixed(self.h, 1 if on else 0))
+
+    def set_fft_threads(self, ft):
+        _check(self.L.iw_set_fft_threads(self.h, ft))
 
     def set_dt(self, dt):
         _check(self.L.iw_set_dt(self.h, ctypes.c_double(dt)))
@@ -449,4 +461,44 @@ def coarse_grain(fine, coarse):
 
 def set_closure(coarse, closure):
     _check(lib().iw_set_closure(coarse.h, closure.h if closure is not None else None))
+
+
+class Ensemble:
+    """Members (Solver objects, initialised) advanced together (iw_ensemble_*)."""
+
+    def __init__(self, members):
+        self.L = lib()
+        self.members = list(members)
+        arr = (ctypes.c_void_p * len(self.members))(*[m.h.value for m in self.members])
+        h = ctypes.c_void_p()
+        _check(self.L.iw_ensemble_create(arr, len(self.members), ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.iw_ensemble_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self, nsteps):
+        _check(self.L.iw_ensemble_run(self.h, nsteps))
+
+    def tune(self, steps=9):
+        out = np.zeros(3)
+        _check(self.L.iw_ensemble_tune(self.h, steps, out.ctypes.data_as(ctypes.c_void_p)))
+        return {"lanes": int(out[0]), "ms_per_step": float(out[1]), "default_ms_per_step": float(out[2])}
+
+    def config(self):
+        lanes = ctypes.c_int()
+        ft = np.zeros(len(self.members), dtype=np.int32)
+        _check(self.L.iw_ensemble_config(self.h, ctypes.byref(lanes), ft.ctypes.data_as(ctypes.c_void_p)))
+        return lanes.value, ft
+
+    def set_lanes(self, lanes):
+        _check(self.L.iw_ensemble_set_lanes(self.h, lanes))
 
