@@ -184,8 +184,8 @@ int iw_set_fft_threads(iw_solver *s, int ft);
  * nonlinear products -- keeps its working set in FP32; the state, the SS/BDF
  * assembly, the filters, the penalty solves and the projection stay FP64.
  * maxcalc's extrema come from the FP64 values of the inverse transform, its
- * umaxt chain from FP32 u.  3-D runs without mean-wave terms; orders 8, 12,
- * 16, 20, 24.  on = 0: the plain FP64 path (the default). */
+ * umaxt chain from FP32 u.  3-D runs without mean-wave terms; GLL orders
+ * 8 to 24.  on = 0: the plain FP64 path (the default). */
 int iw_set_mixed(iw_solver *s, int on);
 
 /* ---- learned sub-grid closure (no counterpart in the reference, whose

@@ -741,7 +741,7 @@ extern "C" int iw_create(const iw_params *p, iw_solver **out)
     // the estimation filter runs inside iw_run every nstepfilter steps: an
     // order it lacks must fail here, not after steps of the run have advanced
     if (P.ifilter == 1 && !solve_np_supported(P.nzloc, P.nsubd))
-        return fail("ifilter = 1: estimation filter GLL order must be one of 8, 12, 16, 20, 24");
+        return fail("ifilter = 1: estimation filter GLL order must lie in 8..24");
     iw_solver *s = new iw_solver();
     s->P = P;
     Geo &g = s->g;
@@ -1959,7 +1959,7 @@ extern "C" int iw_set_mixed(iw_solver *s, int on)
         const Geo &g = s->g;
         if (g.ndims != 3) return fail("iw_set_mixed: the mixed path is 3-D only (ny >= 2)");
         if (!solve_np_supported(g.nzloc, g.nsubd))
-            return fail("iw_set_mixed: GLL order must be one of 8, 12, 16, 20, 24");
+            return fail("iw_set_mixed: GLL order must lie in 8..24");
         if (mean_active(s)) return fail("iw_set_mixed: mean-wave and shear-current terms are FP64 only");
         if (s->closure) return fail("iw_set_mixed: the learned closure runs on the plain FP64 path");
     }
@@ -2362,7 +2362,7 @@ static int filtering_dev(iw_solver *s, int isgscalc)
 {
     const Geo &g = s->g;
     if (!solve_np_supported(s->P.nzloc, s->P.nsubd))
-        return fail("estimation filter: GLL order must be one of 8, 12, 16, 20, 24");
+        return fail("estimation filter: GLL order must lie in 8..24");
     double **B = s->buf;
     double *uvwt[4] = {B[0], B[1], B[2], B[3]};
     double *u0 = s->grad[0];

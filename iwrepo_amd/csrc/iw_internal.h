@@ -10,6 +10,27 @@
 namespace iw {
 
 constexpr int kMaxNl = 32;     // GLL points per subdomain supported by the kernels
+// GLL points per subdomain (orders 8..24) with kernels specialised at
+// compile time; other orders take the generic kernels
+#define IW_NL_SWITCH(M)                                                                                       \
+    case 9: M(9); break;                                                                                       \
+    case 10: M(10); break;                                                                                     \
+    case 11: M(11); break;                                                                                     \
+    case 12: M(12); break;                                                                                     \
+    case 13: M(13); break;                                                                                     \
+    case 14: M(14); break;                                                                                     \
+    case 15: M(15); break;                                                                                     \
+    case 16: M(16); break;                                                                                     \
+    case 17: M(17); break;                                                                                     \
+    case 18: M(18); break;                                                                                     \
+    case 19: M(19); break;                                                                                     \
+    case 20: M(20); break;                                                                                     \
+    case 21: M(21); break;                                                                                     \
+    case 22: M(22); break;                                                                                     \
+    case 23: M(23); break;                                                                                     \
+    case 24: M(24); break;                                                                                     \
+    case 25: M(25); break;
+constexpr bool nl_specialised(int nl) { return nl >= 9 && nl <= 25; }
 constexpr int kMaxFields = 16; // fields per batched transform launch
 
 // Field pointer batch passed by value to batched kernels: source and

@@ -854,14 +854,12 @@ __global__ __launch_bounds__(256) void k_dpdz_t(DpdzArgs A, long nxppy, const do
 static void run_dpdz(const Geo &g, const DpdzArgs &a, hipStream_t st)
 {
     dim3 grid((unsigned)((g.nxppy + 255) / 256), g.nsubd);
+#define DZ(NL) hipLaunchKernelGGL(k_dpdz_t<NL>, grid, dim3(256), 0, st, a, g.nxppy, g.D, g.zh)
     switch (g.nzloc) {
-    case 9: hipLaunchKernelGGL(k_dpdz_t<9>, grid, dim3(256), 0, st, a, g.nxppy, g.D, g.zh); break;
-    case 13: hipLaunchKernelGGL(k_dpdz_t<13>, grid, dim3(256), 0, st, a, g.nxppy, g.D, g.zh); break;
-    case 17: hipLaunchKernelGGL(k_dpdz_t<17>, grid, dim3(256), 0, st, a, g.nxppy, g.D, g.zh); break;
-    case 21: hipLaunchKernelGGL(k_dpdz_t<21>, grid, dim3(256), 0, st, a, g.nxppy, g.D, g.zh); break;
-    case 25: hipLaunchKernelGGL(k_dpdz_t<25>, grid, dim3(256), 0, st, a, g.nxppy, g.D, g.zh); break;
+        IW_NL_SWITCH(DZ)
     default: hipLaunchKernelGGL(k_dpdz, grid, dim3(256), 0, st, a, g.nxppy, g.nzloc, g.D, g.zh); break;
     }
+#undef DZ
 }
 
 void launch_dpdz(const Geo &g, const double *a, double *b, hipStream_t st)
@@ -967,18 +965,16 @@ void launch_dpdz_conv_grad(const Geo &g, double *const *uvwt, double *const *out
     A.fu = out4[0]; A.fv = out4[1]; A.fw = out4[2]; A.ft = out4[3];
     A.du = dz4[0]; A.dv = dz4[1]; A.dw = dz4[2]; A.dt = dz4[3];
     dim3 grid((unsigned)((g.nxppy + 255) / 256), g.nsubd);
+#define DC(NL) hipLaunchKernelGGL(k_dpdz_conv_t<NL>, grid, dim3(256), 0, st, A, g.nxppy, g.D, g.zh)
     switch (g.nzloc) {
-    case 9: hipLaunchKernelGGL(k_dpdz_conv_t<9>, grid, dim3(256), 0, st, A, g.nxppy, g.D, g.zh); break;
-    case 13: hipLaunchKernelGGL(k_dpdz_conv_t<13>, grid, dim3(256), 0, st, A, g.nxppy, g.D, g.zh); break;
-    case 17: hipLaunchKernelGGL(k_dpdz_conv_t<17>, grid, dim3(256), 0, st, A, g.nxppy, g.D, g.zh); break;
-    case 21: hipLaunchKernelGGL(k_dpdz_conv_t<21>, grid, dim3(256), 0, st, A, g.nxppy, g.D, g.zh); break;
-    case 25: hipLaunchKernelGGL(k_dpdz_conv_t<25>, grid, dim3(256), 0, st, A, g.nxppy, g.D, g.zh); break;
+        IW_NL_SWITCH(DC)
     default: { // other orders: the products, then the plain gradients
         launch_dpdz_conv(g, uvwt, out4, st);
         launch_dpdz3(g, uvwt, dz4, st);
         if (dz4[3]) launch_dpdz(g, uvwt[3], dz4[3], st);
     }
     }
+#undef DC
 }
 
 // mean w = 0 (solver.f:66-69)
@@ -1509,13 +1505,12 @@ static void run_tile(const Geo &g, const TileArgs &A, int nf, hipStream_t st)
     else                                                                                                 \
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_tile_t<NL, 0>), gt, dim3(256), 0, st, A, ncp, g.nxppy, g.fcf, \
                            g.FL)
-    switch (g.nzloc) {
-    case 9: TT(9); return;
-    case 13: TT(13); return;
-    case 17: TT(17); return;
-    case 21: TT(21); return;
-    case 25: TT(25); return;
-    default: break;
+    if (nl_specialised(g.nzloc)) {
+        switch (g.nzloc) {
+            IW_NL_SWITCH(TT)
+        default: break;
+        }
+        return;
     }
 #undef TT
     dim3 grid((unsigned)((ncp + kMB - 1) / kMB), g.nsubd, nf);
@@ -2423,8 +2418,15 @@ __global__ __launch_bounds__(64) void k_sweep(SweepArgs A)
 template <int NL, bool PIV>
 constexpr int sweep_dp()
 {
-    // nl = 25: 5 rows ahead measured faster than 25 (which halves occupancy)
-    return NL == 25 ? 5 : NL == 21 ? 7 : NL;
+    // nl = 25: 5 rows ahead measured faster than 25 (which halves occupancy);
+    // 9, 13, 17: the whole subdomain; otherwise the largest divisor of nl in
+    // 4..11, or nl itself when there is none
+    if (NL == 25) return 5;
+    if (NL == 21) return 7;
+    if (NL == 9 || NL == 13 || NL == 17) return NL;
+    for (int d = 11; d >= 4; d--)
+        if (NL % d == 0) return d;
+    return NL;
 }
 
 void launch_solve_grp(const Geo &g, const iw_params &P, const SysTables &S, double *const *uvwt, double dt,
@@ -2450,11 +2452,7 @@ void launch_solve_grp(const Geo &g, const iw_params &P, const SysTables &S, doub
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_sweep<NL, 8, sweep_dp<NL, false>(), false>), ga, dim3(64), 0, st, A); \
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_sweep<NL, 4, sweep_dp<NL, false>(), false>), gb, dim3(64), 0, st, B)
     switch (g.nzloc) {
-    case 9: GS(9); break;
-    case 13: GS(13); break;
-    case 17: GS(17); break;
-    case 21: GS(21); break;
-    case 25: GS(25); break;
+        IW_NL_SWITCH(GS)
     default: break;
     }
 #undef GS
@@ -2501,11 +2499,7 @@ void launch_lfilter(const Geo &g, double *const *uvwt, int cpen, hipStream_t st)
     hipLaunchKernelGGL(k_lfilter<NL>, grid, dim3(256), 0, st, uvwt[0], uvwt[1], uvwt[2], uvwt[3], g.nxppy, \
                        cpen, g.FL, g.damp)
     switch (g.nzloc) {
-    case 9: LF(9); break;
-    case 13: LF(13); break;
-    case 17: LF(17); break;
-    case 21: LF(21); break;
-    case 25: LF(25); break;
+        IW_NL_SWITCH(LF)
     default: break;
     }
 #undef LF
@@ -2681,11 +2675,7 @@ void launch_pressure_nat(const Geo &g, const iw_params &P, const SysTables &S, d
     if (part == 2)                                                                                        \
         hipLaunchKernelGGL(k_pproject<NL>, gc, dim3(256), 0, st, A, uvwt[0], uvwt[1], uvwt[2], cpen, ns, g.FL)
     switch (g.nzloc) {
-    case 9: PSTEP(9); break;
-    case 13: PSTEP(13); break;
-    case 17: PSTEP(17); break;
-    case 21: PSTEP(21); break;
-    case 25: PSTEP(25); break;
+        IW_NL_SWITCH(PSTEP)
     default: break;
     }
 #undef PSTEP
@@ -2694,7 +2684,7 @@ void launch_pressure_nat(const Geo &g, const iw_params &P, const SysTables &S, d
 bool solve_np_supported(int nl, int nsubd)
 {
     (void)nsubd;
-    return nl == 9 || nl == 13 || nl == 17 || nl == 21 || nl == 25;
+    return nl_specialised(nl);
 }
 
 // ---------------------------------------------------------------------------
@@ -2862,11 +2852,7 @@ void launch_modecut(const Geo &g, const double *f, double *u0, double *out, int 
     hipLaunchKernelGGL(k_sgs_z<NL>, gz, dim3(256), 0, st, u0, out, g.nx, g.ny, g.nxpp, kmin, kmax, g.wg, \
                        g.AL, g.sigbv)
     switch (g.nzloc) {
-    case 9: SZ(9); break;
-    case 13: SZ(13); break;
-    case 17: SZ(17); break;
-    case 21: SZ(21); break;
-    case 25: SZ(25); break;
+        IW_NL_SWITCH(SZ)
     default: break;
     }
 #undef SZ

@@ -131,13 +131,10 @@ bool launch_dpdz_conv_grad_mfma(const Geo &g, float *const *uvwt, float *const *
     dim3 blk(kMW * kMWaves);
 #define MC(NL) hipLaunchKernelGGL(k_dpdz_conv_mfma<NL>, grid, blk, 0, st, A, g.nxppy, g.D, g.zh)
     switch (g.nzloc) {
-    case 9: MC(9); return true;
-    case 13: MC(13); return true;
-    case 17: MC(17); return true;
-    case 21: MC(21); return true;
-    case 25: MC(25); return true;
+        IW_NL_SWITCH(MC)
     default: return false;
     }
+    return true;
 #undef MC
 }
 

@@ -126,6 +126,11 @@ CONFIGS = {
                     wave=0.05, noise=1e-3, steps=10, seed=10),
     "small21": dict(params=dict(nx=16, ny=8, nsubd=2, order=20, re=1e3, dt=2e-3),
                     wave=0.05, noise=1e-3, steps=10, seed=11),
+    # every other order 8..24 on the specialised kernels (orders 9, 11, 14,
+    # 15, 18, 19, 22, 23 here; 10, 12, 16, 20 above)
+    **{"small%d" % (o + 1): dict(params=dict(nx=16, ny=8, nsubd=2, order=o, re=1e3, dt=2e-3, noslip=o % 2 == 1),
+                                 wave=0.05, noise=1e-3, steps=10, seed=20 + o)
+       for o in (9, 11, 14, 15, 18, 19, 22, 23)},
     "c1ns": dict(params=dict(nx=64, ny=16, nsubd=4, order=16, xlen=2 * math.pi, ylen=math.pi / 2,
                              re=1e4, pr=1.0, dt=1e-3, noslip=True), wave=0.05, noise=1e-4,
                  steps=200, seed=3),

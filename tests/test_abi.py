@@ -68,6 +68,6 @@ def test_create_rejects_estimation_filter_at_an_order_it_lacks():
     """ifilter = 1 at a GLL order without the estimation-filter kernels fails
     at creation (before any step), with a message naming the filter."""
     P = config_params("small")
-    P.nzloc, P.ifilter, P.nstepfilter = 11, 1, 2
+    P.nzloc, P.ifilter, P.nstepfilter = 27, 1, 2
     with pytest.raises(runtime.IWError, match="estimation filter"):
         runtime.Solver(P)

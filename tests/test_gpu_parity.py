@@ -77,7 +77,9 @@ def _run_pair(rt, O, P, flds, steps):
 
 
 @pytest.mark.parametrize("name,steps", [("small", 10), ("smallns", 10), ("small11", 10), ("small13", 10),
-                                        ("small21", 10), ("c1", 200), ("c1ns", 200)])
+                                        ("small21", 10)] +
+                         [("small%d" % (o + 1), 10) for o in (9, 11, 14, 15, 18, 19, 22, 23)] +
+                         [("c1", 200), ("c1ns", 200)])
 def test_step_matches_oracle(rt, oracle_mod, name, steps):
     P, flds = case(name)
     print("case", name, steps, flush=True)
