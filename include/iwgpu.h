@@ -67,7 +67,10 @@ const char *iw_last_error(void);
 /* Set-up of one solver instance on the current HIP device: GLL nodes,
  * weights, Costa-Don matrices, wavenumbers and grid (replaces setup,
  * setup.f:6-267, and fftfax/cftfax, main.f:210-218) and the device mirror
- * of the COMMON blocks /flow/ and /matrix/ (comflow.h, commatrix.h). */
+ * of the COMMON blocks /flow/ and /matrix/ (comflow.h, commatrix.h).
+ * nx and ny (ny > 1) even, of the form 2^a 3^b 5^c (the lengths FFTFAX and
+ * CFTFAX factor, fft.f:407-500, 1503-1622), nx in [4, 1024], ny <= 1024;
+ * ny = 1 is the 2-D case. */
 int iw_create(const iw_params *p, iw_solver **out);
 int iw_destroy(iw_solver *s);
 

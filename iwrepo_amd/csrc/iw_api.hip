@@ -262,6 +262,28 @@ static int ilog2(int v)
     return (1 << l) == v ? l : -1;
 }
 
+// The passes of a transform of length n = 2^a 3^b 5^c (the factors FFTFAX /
+// CFTFAX accept, fft.f:407-500, 1503-1622): radix 8 while 8 divides what
+// remains, then 4 or 2, then the fives and threes; 4 bits per pass, first
+// pass lowest.  0 when n has another prime factor (or n = 1).
+static unsigned fft_plan(int n)
+{
+    unsigned plan = 0;
+    int shift = 0;
+    auto take = [&](int r) {
+        plan |= (unsigned)r << shift;
+        shift += 4;
+        n /= r;
+    };
+    if (n < 2) return 0;
+    while (n % 8 == 0) take(8);
+    if (n % 4 == 0) take(4);
+    if (n % 2 == 0) take(2);
+    while (n % 5 == 0) take(5);
+    while (n % 3 == 0) take(3);
+    return (n == 1 && shift <= 32) ? plan : 0;
+}
+
 // ---------------------------------------------------------------------------
 // penalty parameters (penalty.f:9-157) and per-subdomain assembly
 // coefficients of matrix_elmsetup (penalty.f:403-634)
@@ -732,9 +754,13 @@ extern "C" int iw_create(const iw_params *p, iw_solver **out)
     const iw_params &P = *p;
     if (P.nsubd < 1 || P.nsubd > IW_MAXSUB) return fail("nsubd out of range");
     if (P.nzloc < 3 || P.nzloc > kMaxNl - 1) return fail("nzloc must lie in [3, 31]");
-    int lx = ilog2(P.nx), ly = ilog2(P.ny);
-    if (lx < 2 || P.nx > 1024) return fail("nx must be a power of two in [4, 1024]");
-    if (ly < 0 || P.ny > 1024) return fail("ny must be 1 or a power of two <= 1024");
+    // FFT991 / CFFT99 lengths: nx and ny even and free of prime factors
+    // above 5 (fft.f:532 rejects the others); ny = 1 is the 2-D case
+    const unsigned planM = (P.nx % 2 == 0) ? fft_plan(P.nx / 2) : 0;
+    const unsigned planY = (P.ny % 2 == 0) ? fft_plan(P.ny) : 0;
+    if (P.nx < 4 || P.nx > 1024 || !planM) return fail("nx must be even, 2^a 3^b 5^c, in [4, 1024]");
+    if (P.ny < 1 || P.ny > 1024 || (P.ny > 1 && !planY)) return fail("ny must be 1 or even, 2^a 3^b 5^c, <= 1024");
+    const int lx = ilog2(P.nx), ly = ilog2(P.ny);
     if (P.dt <= 0) return fail("dt must be positive");
     if (P.ncalclegspec > 0 && P.ny < 2)
         return fail("ncalclegspec needs ny >= 2 (speclegencalc reads the row j = ny/2)");
@@ -758,8 +784,11 @@ extern "C" int iw_create(const iw_params *p, iw_solver **out)
     g.nxppy = (long)g.nxpp * g.ny;
     g.ntot = g.nxppy * g.nz;
     g.M = P.nx / 2;
-    g.logM = lx - 1;
-    g.logNy = ly;
+    g.p2 = lx > 0 && ly >= 0;
+    g.logM = g.p2 ? lx - 1 : 0;
+    g.logNy = g.p2 ? ly : 0;
+    g.planM = planM;
+    g.planY = planY;
     g.scaley = 1. / (double)P.ny;
     HIPCHK(hipGetDevice(&s->device));
     if (scratch_init(s->device)) {

@@ -60,16 +60,18 @@ struct FieldBatch {
 struct Geo {
     int nx, ny, nz, nzloc, nsubd, nxpp, nxh, nxhp, nyh, ndims;
     long nxppy, ntot;
-    int M, logM, logNy; // x complex length nx/2 and logs
+    int M, logM, logNy; // x complex length nx/2 and logs (powers of two)
+    bool p2;            // M and ny powers of two: shift-and-mask transforms
+    unsigned planM, planY; // radices of the passes for M and ny, 4 bits each, first pass lowest
     double scaley;
     int ft = 256; // threads per transform block (64, 128 or 256), 8 values each
     // device tables
     double *D, *D2, *AL, *wg, *sig, *xw, *yw, *zh, *z, *damp, *fcf, *x;
     double *sigbv; // Boyd-Vandeven filter of the estimation model (filtering.f:1272-1310)
     double *FL;    // the Legendre filter as one nl x nl matrix, out[q] = sum_t FL[t nl + q] x[t]
-    double2 *twM;   // e^{-2 pi i t/M}, t < M/2
+    double2 *twM;   // e^{-2 pi i t/M}, t < M
     double2 *twN;   // e^{-2 pi i k/nx}, k <= M
-    double2 *twY;   // e^{-2 pi i t/ny}, t < ny/2
+    double2 *twY;   // e^{-2 pi i t/ny}, t < ny
 };
 
 struct SysTables {

@@ -20,7 +20,8 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 b)
 // ---------------------------------------------------------------------------
 // In-place multi-radix Stockham FFT on LDS.  A block of T threads holds
 // 8 T complex values (nrows sequences of length N, N a power of two),
-// so every pass of radix R has exactly 8/R butterflies per thread.
+// so every pass of radix R has exactly 8/R butterflies per thread.  (Other
+// lengths N = 2^a 3^b 5^c: fft_lds_g below, radix 2, 3, 4, 5 and 8.)
 // Passes are radix 8 while at least three factors of two remain, then one
 // radix-4 or radix-2 pass.  Each pass reads its butterflies into registers,
 // synchronises, and writes the outputs to their autosort positions.  Row
@@ -57,6 +58,31 @@ __device__ __forceinline__ void dft(double2 *a, int sign)
         a[2] = csub(t0, t2);
         a[1] = cadd(t1, t3);
         a[3] = csub(t1, t3);
+    } else if (R == 3) {
+        // y_q = sum_m a_m e^{sign 2 pi i m q / 3}
+        constexpr double h = 0.86602540378443864676372317075293618; // sqrt(3)/2
+        const double2 t1 = cadd(a[1], a[2]);
+        const double2 t2 = make_double2(a[0].x - 0.5 * t1.x, a[0].y - 0.5 * t1.y);
+        const double2 d = cmuli(csub(a[1], a[2]), sign); // sign i (a1 - a2)
+        a[0] = cadd(a[0], t1);
+        a[1] = make_double2(t2.x + h * d.x, t2.y + h * d.y);
+        a[2] = make_double2(t2.x - h * d.x, t2.y - h * d.y);
+    } else if (R == 5) {
+        // cos and sin of 2 pi/5 and 4 pi/5: (sqrt5 - 1)/4, sqrt((5 + sqrt5)/8),
+        // -(sqrt5 + 1)/4, sqrt((5 - sqrt5)/8)
+        constexpr double c1 = 0.30901699437494742410229341718281906, s1 = 0.95105651629515357211643933337938214;
+        constexpr double c2 = -0.80901699437494742410229341718281906, s2 = 0.58778525229247312916870595463907277;
+        const double2 b1 = cadd(a[1], a[4]), b2 = cadd(a[2], a[3]);
+        const double2 d1 = cmuli(csub(a[1], a[4]), sign), d2 = cmuli(csub(a[2], a[3]), sign);
+        const double2 p1 = make_double2(a[0].x + c1 * b1.x + c2 * b2.x, a[0].y + c1 * b1.y + c2 * b2.y);
+        const double2 p2 = make_double2(a[0].x + c2 * b1.x + c1 * b2.x, a[0].y + c2 * b1.y + c1 * b2.y);
+        const double2 q1 = make_double2(s1 * d1.x + s2 * d2.x, s1 * d1.y + s2 * d2.y);
+        const double2 q2 = make_double2(s2 * d1.x - s1 * d2.x, s2 * d1.y - s1 * d2.y);
+        a[0] = cadd(a[0], cadd(b1, b2));
+        a[1] = cadd(p1, q1);
+        a[4] = csub(p1, q1);
+        a[2] = cadd(p2, q2);
+        a[3] = csub(p2, q2);
     } else { // R == 8: two 4-point DFTs of the even and odd inputs
         constexpr double c = 0.70710678118654752440084436210484904; // sqrt(2)/2
         double2 e[4] = {a[0], a[2], a[4], a[6]}, o[4] = {a[1], a[3], a[5], a[7]};
@@ -148,6 +174,76 @@ __device__ void fft_lds(double2 *buf, int logN, int nrows, const double2 *tw, in
     }
     if (left == 2) lds_pass<4, 2, COL>(buf, logN, lNs, nrows, lrows, tw, sign);
     if (left == 1) lds_pass<2, 1, COL>(buf, logN, lNs, nrows, lrows, tw, sign);
+}
+
+// Sequences of any length N = 2^a 3^b 5^c (the factors of fft.f:501-837):
+// the same self-sorting passes with the radices of `plan` (4 bits each,
+// first pass lowest) and division in place of shifts.  A block's tile holds
+// nrows whole sequences (nrows N <= 8 blockDim values).
+template <int R, bool COL>
+__device__ __forceinline__ void lds_pass_g(double2 *buf, int N, int Ns, int nrows, const double2 *tw, int sign)
+{
+    constexpr int BPT = (8 + R - 1) / R;
+    const int NR = N / R, nb = nrows * NR, step = N / (Ns * R);
+    double2 a[BPT][R];
+    int base[BPT], rr[BPT];
+#pragma unroll
+    for (int i = 0; i < BPT; i++) {
+        const int b = threadIdx.x + i * (int)blockDim.x;
+        base[i] = -1;
+        rr[i] = 0;
+        if (b < nb) {
+            int r, j;
+            if (COL) {
+                r = b % nrows;
+                j = b / nrows;
+            } else {
+                r = b / NR;
+                j = b - r * NR;
+            }
+            const int k = j % Ns;
+#pragma unroll
+            for (int m = 0; m < R; m++)
+                a[i][m] = COL ? buf[(j + m * NR) * nrows + r] : buf[lp(r * N + j + m * NR)];
+#pragma unroll
+            for (int m = 1; m < R; m++) {
+                double2 w = tw[m * k * step];
+                if (sign > 0) w.y = -w.y;
+                a[i][m] = cmul(a[i][m], w);
+            }
+            dft<R>(a[i], sign);
+            base[i] = (j - k) * R + k;
+            rr[i] = r;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < BPT; i++)
+        if (base[i] >= 0)
+#pragma unroll
+            for (int q = 0; q < R; q++) {
+                const int e = base[i] + q * Ns;
+                if (COL)
+                    buf[e * nrows + rr[i]] = a[i][q];
+                else
+                    buf[lp(rr[i] * N + e)] = a[i][q];
+            }
+    __syncthreads();
+}
+
+template <bool COL>
+__device__ void fft_lds_g(double2 *buf, int N, unsigned plan, int nrows, const double2 *tw, int sign)
+{
+    int Ns = 1;
+    for (; plan; plan >>= 4) {
+        switch (plan & 15) {
+        case 8: lds_pass_g<8, COL>(buf, N, Ns, nrows, tw, sign); Ns *= 8; break;
+        case 5: lds_pass_g<5, COL>(buf, N, Ns, nrows, tw, sign); Ns *= 5; break;
+        case 4: lds_pass_g<4, COL>(buf, N, Ns, nrows, tw, sign); Ns *= 4; break;
+        case 3: lds_pass_g<3, COL>(buf, N, Ns, nrows, tw, sign); Ns *= 3; break;
+        default: lds_pass_g<2, COL>(buf, N, Ns, nrows, tw, sign); Ns *= 2; break;
+        }
+    }
 }
 
 // filter_legenpen (filtering.f:729-808) of the NL values x of one column in
@@ -276,15 +372,21 @@ __device__ __forceinline__ void mc_take(bool mn, double x, int lx, double &b, in
 // AX: the batch's fields have flux products (forward only); MAPS: the batch
 // has plane maps (else rows are addressed arithmetically, so no row address
 // waits on a table load and a thread's loads are all in flight together).
-// TI, TO: storage type of the inputs (f, ax) and of the outputs (g, dx, axo)
-template <int DIR, bool AX = false, bool MAPS = false, typename TI = double, typename TO = double>
+// TI, TO: storage type of the inputs (f, ax) and of the outputs (g, dx, axo).
+// P2: M and ny are powers of two (shifts and masks, radix-8/4/2 passes);
+// else any M = 2^a 3^b 5^c with the radices of `plan`, and a tile of
+// R = floor(8 blockDim / M) rows that may leave slots of the block unused.
+template <int DIR, bool AX = false, bool MAPS = false, typename TI = double, typename TO = double, bool P2 = true>
 __global__ __launch_bounds__(kFT) void k_fft_x(FieldBatch fb, int nrows, int ny, int nxpp, int M, int logM,
-                                               int R, const double2 *__restrict__ twM,
+                                               unsigned plan, int R, const double2 *__restrict__ twM,
                                                const double2 *__restrict__ twNg, double scaley, McFuse mf)
 {
     extern __shared__ double2 smx[];
     double2 *buf = smx;
-    double2 *tw = smx + R * M;
+    // the tile: 8 blockDim slots (R M of them used; lp permutes within
+    // aligned runs of 128, so a partial tile's slots stay inside the block)
+    double2 *tw = smx + 8 * (int)blockDim.x;
+    const int nval = R * M;
     double2 *nyq = tw + M; // R entries: the kx-Nyquist coefficient of each row (inverse, for maxcalc)
     // the length-N twiddles and the wavenumbers kx of k = 0..M, staged in LDS
     // with tw so that no per-element global table load sits between a
@@ -298,15 +400,26 @@ __global__ __launch_bounds__(kFT) void k_fft_x(FieldBatch fb, int nrows, int ny,
     TO *obase = reinterpret_cast<TO *>(fb.g[blockIdx.y]);
     const int row0 = blockIdx.x * R;
     const int N = 2 * M;
-    const int lny = __builtin_ctz(ny); // ny is a power of two
+    const int lny = P2 ? __builtin_ctz(ny) : 0;
+    // value idx of the tile -> (row r, index k); row -> (plane, row j)
+    auto rowof = [&](int idx) -> int { return P2 ? idx >> logM : idx / M; };
+    auto colof = [&](int idx, int r) -> int { return P2 ? idx & (M - 1) : idx - r * M; };
+    auto planeof = [&](int row) -> int { return P2 ? row >> lny : row / ny; };
+    auto jof = [&](int row, int pl) -> int { return P2 ? row & (ny - 1) : row - pl * ny; };
+    auto fft_rows = [&](int nr) {
+        if (P2)
+            fft_lds<false>(buf, logM, nr, tw, DIR);
+        else
+            fft_lds_g<false>(buf, M, plan, nr, tw, DIR);
+    };
     // logical row -> stored row of the source / destination (plane maps;
     // with maps, the offsets of the block's rows are staged in LDS below)
     auto srow_g = [&](int row) -> long {
-        int lpl = row >> lny, j = row & (ny - 1);
+        int lpl = planeof(row), j = jof(row, lpl);
         return (long)((fb.smap ? fb.smap[lpl] : lpl) * ny + j) * nxpp;
     };
     auto drow_g = [&](int row) -> long {
-        int lpl = row >> lny, j = row & (ny - 1);
+        int lpl = planeof(row), j = jof(row, lpl);
         return (long)((fb.dmap ? fb.dmap[lpl] : lpl) * ny + j) * nxpp;
     };
     long *rmap = reinterpret_cast<long *>(kws + M + 1 + 4 * R); // 2 R: source, destination offsets
@@ -332,7 +445,7 @@ __global__ __launch_bounds__(kFT) void k_fft_x(FieldBatch fb, int nrows, int ny,
         }
     if (DIR > 0 && mf.part)
         for (int r = threadIdx.x; r < R; r += blockDim.x) {
-            const int row = row0 + r, k = (row < nrows ? row : nrows - 1) >> lny;
+            const int row = row0 + r, k = planeof(row < nrows ? row : nrows - 1);
             mcv[r] = mf.w00 ? mf.w00[k] : 0.;
             mcv[R + r] = mf.rm ? mf.rm[k] : 0.;
             mcv[2 * R + r] = mf.rdz ? mf.rdz[k] : 0.;
@@ -350,7 +463,7 @@ __global__ __launch_bounds__(kFT) void k_fft_x(FieldBatch fb, int nrows, int ny,
 #pragma unroll
         for (int it = 0; it < 8; it++) {
             const int idx = threadIdx.x + it * (int)blockDim.x;
-            const int r = idx >> logM, k = idx & (M - 1), row = row0 + r;
+            const int r = rowof(idx), k = colof(idx, r), row = row0 + r;
             const TI *p = base + srow(row < nrows ? row : nrows - 1);
             va[it] = ld2(p + 2 * k);
             vb[it] = ld2(p + 2 * (M - k));
@@ -358,7 +471,8 @@ __global__ __launch_bounds__(kFT) void k_fft_x(FieldBatch fb, int nrows, int ny,
 #pragma unroll
         for (int it = 0; it < 8; it++) {
             const int idx = threadIdx.x + it * (int)blockDim.x;
-            const int r = idx >> logM, k = idx & (M - 1), row = row0 + r;
+            if (!P2 && idx >= nval) continue;
+            const int r = rowof(idx), k = colof(idx, r), row = row0 + r;
             double2 a = va[it], b = vb[it];
             if (!deriv && k == 0) nyq[r] = b; // kx-Nyquist coefficient of the row
             if (deriv) {
@@ -387,7 +501,8 @@ __global__ __launch_bounds__(kFT) void k_fft_x(FieldBatch fb, int nrows, int ny,
 #pragma unroll
         for (int it = 0; it < 8; it++) {
             const int idx = threadIdx.x + it * (int)blockDim.x;
-            int r = idx >> logM, m = idx & (M - 1), row = row0 + r;
+            if (!P2 && idx >= nval) continue;
+            int r = rowof(idx), m = colof(idx, r), row = row0 + r;
             if (row >= nrows) continue;
             st2(ob + drow(row) + 2 * m, buf[lp(idx)]);
         }
@@ -400,7 +515,7 @@ __global__ __launch_bounds__(kFT) void k_fft_x(FieldBatch fb, int nrows, int ny,
         // i kx f first: a transform in place overwrites its input rows
         load_inverse(true);
         __syncthreads();
-        fft_lds<false>(buf, logM, R, tw, DIR);
+        fft_rows(R);
         store_inverse(reinterpret_cast<TO *>(fb.dx[blockIdx.y]));
         __syncthreads();
     }
@@ -411,7 +526,8 @@ __global__ __launch_bounds__(kFT) void k_fft_x(FieldBatch fb, int nrows, int ny,
 #pragma unroll
             for (int it = 0; it < 8; it++) {
                 const int idx = threadIdx.x + it * (int)blockDim.x;
-                int r = idx >> logM, m = idx & (M - 1), row = row0 + r;
+                if (!P2 && idx >= nval) continue;
+                int r = rowof(idx), m = colof(idx, r), row = row0 + r;
                 double2 z = ld2(src + srow(row < nrows ? row : nrows - 1) + 2 * m);
                 if (row >= nrows) z = make_double2(0., 0.);
                 buf[lp(idx)] = z;
@@ -420,8 +536,9 @@ __global__ __launch_bounds__(kFT) void k_fft_x(FieldBatch fb, int nrows, int ny,
         // coefficient k from Z_k and Z_{M-k}; the thread of k = 0 also
         // forms k = M (both from Z_0)
         auto coef = [&](int r, int k) -> double2 {
-            double2 Zk = buf[lp((r << logM) + (k == M ? 0 : k))];
-            double2 Zm = buf[lp((r << logM) + (k == 0 ? 0 : M - k))];
+            const int r0 = P2 ? r << logM : r * M;
+            double2 Zk = buf[lp(r0 + (k == M ? 0 : k))];
+            double2 Zm = buf[lp(r0 + (k == 0 ? 0 : M - k))];
             double er = 0.5 * (Zk.x + Zm.x), ei = 0.5 * (Zk.y - Zm.y);
             double dr = Zk.x - Zm.x, di = Zk.y + Zm.y;
             double orr = 0.5 * di, oi = -0.5 * dr;
@@ -441,26 +558,27 @@ __global__ __launch_bounds__(kFT) void k_fft_x(FieldBatch fb, int nrows, int ny,
         if (AX) {
             // the tile holds H rows of the field and the same H rows of its
             // flux product P: both are transformed by the same passes and
-            // combined from LDS
+            // combined from LDS (with R odd the last row of the tile is unused)
             const int H = R >> 1, rowA = blockIdx.x * H;
             const TI *acc = reinterpret_cast<const TI *>(fb.ax[blockIdx.y]);
             TO *axo = reinterpret_cast<TO *>(fb.axo[blockIdx.y]);
 #pragma unroll
             for (int it = 0; it < 8; it++) {
                 const int idx = threadIdx.x + it * (int)blockDim.x;
-                const int r = idx >> logM, m = idx & (M - 1);
+                if (!P2 && idx >= nval) continue;
+                const int r = rowof(idx), m = colof(idx, r);
                 const bool isp = r >= H;
                 const int row = rowA + (isp ? r - H : r);
                 double2 z = ld2((isp ? acc : base) + srow(row < nrows ? row : nrows - 1) + 2 * m);
-                if (row >= nrows) z = make_double2(0., 0.);
+                if (row >= nrows || (!P2 && r >= 2 * H)) z = make_double2(0., 0.);
                 buf[lp(idx)] = z;
             }
             __syncthreads();
-            fft_lds<false>(buf, logM, R, tw, DIR);
+            fft_rows(R);
 #pragma unroll
             for (int it = 0; it < 8; it++) {
                 const int idx = threadIdx.x + it * (int)blockDim.x;
-                const int r = idx >> logM, k = idx & (M - 1), row = rowA + r;
+                const int r = rowof(idx), k = colof(idx, r), row = rowA + r;
                 if (r >= H || row >= nrows) continue;
                 TO *orow = obase + drow(row);
                 const double2 q = coef(r + H, k);
@@ -475,11 +593,12 @@ __global__ __launch_bounds__(kFT) void k_fft_x(FieldBatch fb, int nrows, int ny,
         } else {
             load_fwd(base);
             __syncthreads();
-            fft_lds<false>(buf, logM, R, tw, DIR);
+            fft_rows(R);
 #pragma unroll
             for (int it = 0; it < 8; it++) {
                 const int idx = threadIdx.x + it * (int)blockDim.x;
-                int r = idx >> logM, k = idx & (M - 1), row = row0 + r;
+                if (!P2 && idx >= nval) continue;
+                int r = rowof(idx), k = colof(idx, r), row = row0 + r;
                 if (row >= nrows) continue;
                 TO *orow = obase + drow(row);
                 st2(orow + 2 * k, coef(r, k));
@@ -489,19 +608,20 @@ __global__ __launch_bounds__(kFT) void k_fft_x(FieldBatch fb, int nrows, int ny,
     } else {
         load_inverse(false);
         __syncthreads();
-        fft_lds<false>(buf, logM, R, tw, DIR);
+        fft_rows(R);
         store_inverse(obase);
         if (mf.part && blockIdx.y < 4) { // maxcalc reductions of u, v, w, temp
-            const int f = blockIdx.y, nz = nrows >> lny;
+            const int f = blockIdx.y, nz = planeof(nrows);
             const double big = 1.79769313486231570e308;
             double b0 = big, b1 = -big, b2 = -big; // min, max of the value; max of the CFL number
             int l0 = -1, l1 = -1, l2 = -1;
 #pragma unroll
             for (int it = 0; it < 8; it++) {
                 const int idx = threadIdx.x + it * (int)blockDim.x;
-                const int r = idx >> logM, m = idx & (M - 1), row = row0 + r;
+                if (!P2 && idx >= nval) continue;
+                const int r = rowof(idx), m = colof(idx, r), row = row0 + r;
                 if (row >= nrows) continue;
-                const int k = row >> lny, j = row & (ny - 1);
+                const int k = planeof(row), j = jof(row, k);
                 const double2 z = buf[lp(idx)];
                 const double c = mf.cny * nyq[r].x;
                 double xe = z.x + c, xo = z.y - c; // points i = 2m, 2m+1
@@ -577,9 +697,11 @@ __global__ __launch_bounds__(kFT) void k_fft_x(FieldBatch fb, int nrows, int ny,
 // transform of columns ii < nxh (the kx-Nyquist column is skipped); forward
 // scaled by 1/ny (horfft.f:28-30).  CB columns per block staged in LDS.
 // ---------------------------------------------------------------------------
-// TI, TO: storage type of the inputs (f, ay) and of the outputs (g, dy)
-template <int DIR, typename TI = double, typename TO = double>
-__global__ __launch_bounds__(kFT) void k_fft_y(FieldBatch fb, int nxpp, int nxh, int ny, int logNy,
+// TI, TO: storage type of the inputs (f, ay) and of the outputs (g, dy).
+// P2: ny is a power of two; else ny = 2^a 3^b 5^c with the radices of
+// `plan` and CB = floor(8 blockDim / ny) columns per block.
+template <int DIR, typename TI = double, typename TO = double, bool P2 = true>
+__global__ __launch_bounds__(kFT) void k_fft_y(FieldBatch fb, int nxpp, int nxh, int ny, int logNy, unsigned plan,
                                                int CB, const double2 *__restrict__ twY, double scaley)
 {
     extern __shared__ double2 smy[];
@@ -591,7 +713,16 @@ __global__ __launch_bounds__(kFT) void k_fft_y(FieldBatch fb, int nxpp, int nxh,
     const long opl = (long)(fb.dmap ? fb.dmap[lpl] : lpl) * nxpp * ny;
     TO *oplane = reinterpret_cast<TO *>(fb.g[blockIdx.z]) + opl;
     const int ii0 = blockIdx.x * CB;
-    const int lcb = __builtin_ctz(CB); // CB is a power of two
+    const int nval = CB * ny; // values of the tile: value idx is column idx % CB of row idx / CB
+    const int lcb = P2 ? __builtin_ctz(CB) : 0;
+    auto rowof = [&](int idx) -> int { return P2 ? idx >> lcb : idx / CB; };
+    auto colof = [&](int idx, int j) -> int { return P2 ? idx & (CB - 1) : idx - j * CB; };
+    auto fft_cols = [&]() {
+        if (P2)
+            fft_lds<true>(buf, logNy, CB, tw, DIR);
+        else
+            fft_lds_g<true>(buf, ny, plan, CB, tw, DIR);
+    };
     for (int t = threadIdx.x; t < ny; t += blockDim.x) {
         tw[t] = twY[t];
         kys[t] = fb.kw ? fb.kw[t] : 0.;
@@ -601,26 +732,29 @@ __global__ __launch_bounds__(kFT) void k_fft_y(FieldBatch fb, int nxpp, int nxh,
     // (the y gradient of solver.f:86-103)
     auto load = [&](const TI *src, bool deriv) {
         // all loads first (columns past nxh are clamped and their values
-        // dropped), so they are in flight together; the tile is always
-        // CB x ny = 8 blockDim values
+        // dropped), so they are in flight together; with P2 the tile is
+        // CB x ny = 8 blockDim values, else values idx >= CB ny are unused
         double2 v[8];
 #pragma unroll
         for (int it = 0; it < 8; it++) {
             const int idx = threadIdx.x + it * (int)blockDim.x;
-            const int j = idx >> lcb, ii = ii0 + (idx & (CB - 1));
+            int j = rowof(idx);
+            const int ii = ii0 + colof(idx, j);
+            if (!P2 && j >= ny) j = ny - 1;
             v[it] = ld2(src + (long)j * nxpp + 2 * (ii < nxh ? ii : nxh - 1));
         }
 #pragma unroll
         for (int it = 0; it < 8; it++) {
             const int idx = threadIdx.x + it * (int)blockDim.x;
-            int j = idx >> lcb, c = idx & (CB - 1), ii = ii0 + c;
+            if (!P2 && idx >= nval) continue;
+            int j = rowof(idx), c = colof(idx, j), ii = ii0 + c;
             double2 z = v[it];
             if (ii >= nxh) z = make_double2(0., 0.);
             if (deriv) {
                 const double ky = kys[j];
                 z = make_double2(-ky * z.y, ky * z.x);
             }
-            buf[(j << lcb) + c] = z;
+            buf[idx] = z;
         }
     };
     // forward with a second input P: + i ky P-hat (the y part of the flux
@@ -633,9 +767,10 @@ __global__ __launch_bounds__(kFT) void k_fft_y(FieldBatch fb, int nxpp, int nxh,
 #pragma unroll
         for (int it = 0; it < 8; it++) {
             const int idx = threadIdx.x + it * (int)blockDim.x;
-            int j = idx >> lcb, c = idx & (CB - 1), ii = ii0 + c;
+            if (!P2 && idx >= nval) continue;
+            int j = rowof(idx), c = colof(idx, j), ii = ii0 + c;
             if (ii >= nxh) continue;
-            double2 z = buf[(j << lcb) + c];
+            double2 z = buf[idx];
             if (DIR < 0) {
                 z.x = scaley * z.x;
                 z.y = scaley * z.y;
@@ -652,20 +787,20 @@ __global__ __launch_bounds__(kFT) void k_fft_y(FieldBatch fb, int nxpp, int nxh,
     if (dyp) { // i ky f first: a transform in place overwrites its input
         load(plane, true);
         __syncthreads();
-        fft_lds<true>(buf, logNy, CB, tw, DIR);
+        fft_cols();
         store(dyp, false);
         __syncthreads();
     }
     if (accp) {
         load(accp, false);
         __syncthreads();
-        fft_lds<true>(buf, logNy, CB, tw, DIR);
+        fft_cols();
 #pragma unroll
         for (int it = 0; it < 8; it++) {
             const int idx = threadIdx.x + it * (int)blockDim.x;
             double2 z = make_double2(0., 0.);
-            if (idx < CB * ny) {
-                z = buf[((idx >> lcb) << lcb) + (idx & (CB - 1))];
+            if (idx < nval) {
+                z = buf[idx];
                 z.x = scaley * z.x;
                 z.y = scaley * z.y;
             }
@@ -675,7 +810,7 @@ __global__ __launch_bounds__(kFT) void k_fft_y(FieldBatch fb, int nxpp, int nxh,
     }
     load(plane, false);
     __syncthreads();
-    fft_lds<true>(buf, logNy, CB, tw, DIR);
+    fft_cols();
     store(oplane, accp != nullptr);
     // the untransformed kx-Nyquist column: copied when out of place, plus
     // i ky P when accumulating, and times i ky into the derivative output
@@ -698,13 +833,13 @@ __global__ __launch_bounds__(kFT) void k_fft_y(FieldBatch fb, int nxpp, int nxh,
         }
 }
 
-template <bool MAPS, typename TI, typename TO>
+template <bool MAPS, typename TI, typename TO, bool P2>
 static void fft_x_launch(const Geo &g, const FieldBatch &b, int nk, int dir, hipStream_t st, const McFuse *mf)
 {
-    const int R = 8 * g.ft / g.M;
+    const int R = 8 * g.ft / g.M; // rows per block (fft_threads_ok: R >= 2)
     int nrows = g.ny * nk;
     dim3 grid((nrows + R - 1) / R, b.n);
-    size_t sh = sizeof(double2) * ((size_t)R * g.M + g.M + R + g.M + 1) + sizeof(double) * (g.M + 1 + 4 * R) +
+    size_t sh = sizeof(double2) * ((size_t)8 * g.ft + g.M + R + g.M + 1) + sizeof(double) * (g.M + 1 + 4 * R) +
                 sizeof(long) * 2 * R;
     McFuse none;
     if (dir < 0) {
@@ -723,27 +858,38 @@ static void fft_x_launch(const Geo &g, const FieldBatch &b, int nk, int dir, hip
             t.dy[t.n] = b.dy[q];
             t.n++;
         }
+        const int H = R / 2;
         if (ba.n)
-            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fft_x<-1, true, MAPS, TI, TO>),
-                               dim3((nrows + R / 2 - 1) / (R / 2), ba.n), dim3(g.ft), sh, st, ba, nrows, g.ny, g.nxpp,
-                               g.M, g.logM, R, g.twM, g.twN, g.scaley, none);
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fft_x<-1, true, MAPS, TI, TO, P2>), dim3((nrows + H - 1) / H, ba.n),
+                               dim3(g.ft), sh, st, ba, nrows, g.ny, g.nxpp, g.M, g.logM, g.planM, R, g.twM, g.twN,
+                               g.scaley, none);
         if (bp.n)
-            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fft_x<-1, false, MAPS, TI, TO>), dim3(grid.x, bp.n), dim3(g.ft), sh,
-                               st, bp, nrows, g.ny, g.nxpp, g.M, g.logM, R, g.twM, g.twN, g.scaley, none);
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fft_x<-1, false, MAPS, TI, TO, P2>), dim3(grid.x, bp.n), dim3(g.ft),
+                               sh, st, bp, nrows, g.ny, g.nxpp, g.M, g.logM, g.planM, R, g.twM, g.twN, g.scaley, none);
     } else {
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fft_x<1, false, MAPS, TI, TO>), grid, dim3(g.ft), sh, st, b, nrows,
-                           g.ny, g.nxpp, g.M, g.logM, R, g.twM, g.twN, g.scaley, mf ? *mf : none);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fft_x<1, false, MAPS, TI, TO, P2>), grid, dim3(g.ft), sh, st, b, nrows,
+                           g.ny, g.nxpp, g.M, g.logM, g.planM, R, g.twM, g.twN, g.scaley, mf ? *mf : none);
     }
+}
+
+template <bool P2>
+static void fft_x_dispatch(const Geo &g, const FieldBatch &b, int nk, int dir, hipStream_t st, const McFuse *mf,
+                           int prec)
+{
+    if (prec == kPrecF32) // FP32 in and out (mixed path; no plane maps)
+        fft_x_launch<false, float, float, P2>(g, b, nk, dir, st, mf);
+    else if (b.smap || b.dmap)
+        fft_x_launch<true, double, double, P2>(g, b, nk, dir, st, mf);
+    else
+        fft_x_launch<false, double, double, P2>(g, b, nk, dir, st, mf);
 }
 
 void launch_fft_x(const Geo &g, FieldBatch b, int nk, int dir, hipStream_t st, const McFuse *mf, int prec)
 {
-    if (prec == kPrecF32) // FP32 in and out (mixed path; no plane maps)
-        fft_x_launch<false, float, float>(g, b, nk, dir, st, mf);
-    else if (b.smap || b.dmap)
-        fft_x_launch<true, double, double>(g, b, nk, dir, st, mf);
+    if (g.p2)
+        fft_x_dispatch<true>(g, b, nk, dir, st, mf, prec);
     else
-        fft_x_launch<false, double, double>(g, b, nk, dir, st, mf);
+        fft_x_dispatch<false>(g, b, nk, dir, st, mf, prec);
 }
 
 int mc_blocks(const Geo &g, int ft)
@@ -752,25 +898,34 @@ int mc_blocks(const Geo &g, int ft)
     return (g.ny * g.nz + R - 1) / R;
 }
 
-void launch_fft_y(const Geo &g, FieldBatch b, int nk, int dir, hipStream_t st, int prec)
+template <bool P2>
+static void fft_y_dispatch(const Geo &g, const FieldBatch &b, int nk, int dir, hipStream_t st, int prec)
 {
-    if (g.ndims != 3) return;
-    int CB = 8 * g.ft / g.ny;
+    const int CB = 8 * g.ft / g.ny; // columns per block (fft_threads_ok: CB >= 1)
     dim3 grid((g.nxh + CB - 1) / CB, nk, b.n);
     size_t sh = sizeof(double2) * ((size_t)CB * g.ny + g.ny) + sizeof(double) * g.ny;
     // mixed path: spectral FP64 <-> FP32 intermediate
     if (dir < 0 && prec == kPrecF32)
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fft_y<-1, float, double>), grid, dim3(g.ft), sh, st, b, g.nxpp, g.nxh,
-                           g.ny, g.logNy, CB, g.twY, g.scaley);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fft_y<-1, float, double, P2>), grid, dim3(g.ft), sh, st, b, g.nxpp,
+                           g.nxh, g.ny, g.logNy, g.planY, CB, g.twY, g.scaley);
     else if (prec == kPrecF32)
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fft_y<1, double, float>), grid, dim3(g.ft), sh, st, b, g.nxpp, g.nxh,
-                           g.ny, g.logNy, CB, g.twY, g.scaley);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fft_y<1, double, float, P2>), grid, dim3(g.ft), sh, st, b, g.nxpp,
+                           g.nxh, g.ny, g.logNy, g.planY, CB, g.twY, g.scaley);
     else if (dir < 0)
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fft_y<-1>), grid, dim3(g.ft), sh, st, b, g.nxpp, g.nxh, g.ny, g.logNy,
-                           CB, g.twY, g.scaley);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fft_y<-1, double, double, P2>), grid, dim3(g.ft), sh, st, b, g.nxpp,
+                           g.nxh, g.ny, g.logNy, g.planY, CB, g.twY, g.scaley);
     else
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fft_y<1>), grid, dim3(g.ft), sh, st, b, g.nxpp, g.nxh, g.ny, g.logNy,
-                           CB, g.twY, g.scaley);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fft_y<1, double, double, P2>), grid, dim3(g.ft), sh, st, b, g.nxpp,
+                           g.nxh, g.ny, g.logNy, g.planY, CB, g.twY, g.scaley);
+}
+
+void launch_fft_y(const Geo &g, FieldBatch b, int nk, int dir, hipStream_t st, int prec)
+{
+    if (g.ndims != 3) return;
+    if (g.p2)
+        fft_y_dispatch<true>(g, b, nk, dir, st, prec);
+    else
+        fft_y_dispatch<false>(g, b, nk, dir, st, prec);
 }
 
 // ---------------------------------------------------------------------------

@@ -131,6 +131,16 @@ CONFIGS = {
     **{"small%d" % (o + 1): dict(params=dict(nx=16, ny=8, nsubd=2, order=o, re=1e3, dt=2e-3, noslip=o % 2 == 1),
                                  wave=0.05, noise=1e-3, steps=10, seed=20 + o)
        for o in (9, 11, 14, 15, 18, 19, 22, 23)},
+    # lengths with factors 3 and 5 (the radix-3/5 passes): x lengths
+    # M = nx/2 = 24, 48, 30, 45 and y lengths 24, 40, 18
+    "r48": dict(params=dict(nx=48, ny=24, nsubd=2, order=8, re=1e3, dt=2e-3),
+                wave=0.05, noise=1e-3, steps=10, seed=40),
+    "r96": dict(params=dict(nx=96, ny=40, nsubd=3, order=12, re=1e3, dt=2e-3, noslip=True),
+                wave=0.05, noise=1e-3, steps=10, seed=41),
+    "r60": dict(params=dict(nx=60, ny=18, nsubd=2, order=10, re=1e3, dt=2e-3),
+                wave=0.05, noise=1e-3, steps=10, seed=42),
+    "r90": dict(params=dict(nx=90, ny=1, nsubd=3, order=8, re=1e3, dt=2e-3, noslip=True),
+                wave=0.05, noise=1e-3, steps=10, seed=43),
     "c1ns": dict(params=dict(nx=64, ny=16, nsubd=4, order=16, xlen=2 * math.pi, ylen=math.pi / 2,
                              re=1e4, pr=1.0, dt=1e-3, noslip=True), wave=0.05, noise=1e-4,
                  steps=200, seed=3),

@@ -47,7 +47,7 @@ def _params(name):
 
 
 @pytest.mark.parametrize("name,steps", [("small", 20), ("small13", 20), ("small21", 20), ("c1", 200),
-                                        ("c1ns", 200)])
+                                        ("c1ns", 200), ("r48", 20), ("r96", 20)])
 def test_mixed_path_against_fp64_oracle(rt, oracle_mod, name, steps):
     P, flds = _params(name)
     print("mixed", name, steps, flush=True)

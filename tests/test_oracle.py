@@ -88,8 +88,11 @@ def _dft_forward(phys, nx, ny):
     return out
 
 
-def test_horfft_matches_brute_force_dft(oracle_mod):
-    P, flds = case("small")
+@pytest.mark.parametrize("name", ["small", "r48", "r60", "r90"])
+def test_horfft_matches_brute_force_dft(oracle_mod, name):
+    """Powers of two and lengths with factors 3 and 5 (x lengths 16, 48, 60,
+    90; y lengths 8, 24, 18, 1)."""
+    P, flds = case(name)
     o = oracle_mod.Oracle(P)
     spec = o.horfft(flds[0], -1)
     for k in (0, 5, P.nzloc * P.nsubd - 1):
