@@ -7,6 +7,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <algorithm>
 #include <map>
@@ -431,6 +432,7 @@ static int build_systems(iw_solver *s, bool refactor_p, int &first, int &count)
     if ((size_t)nsys > S.cap_sys) {
         if (S.AB) hipFree(S.AB);
         if (S.FB) hipFree(S.FB);
+        if (S.SPK) hipFree(S.SPK);
         if (S.ipiv) hipFree(S.ipiv);
         if (S.coef) hipFree(S.coef);
         if (S.sysinfo) hipFree(S.sysinfo);
@@ -442,6 +444,9 @@ static int build_systems(iw_solver *s, bool refactor_p, int &first, int &count)
         const size_t fbn = (size_t)(tri_sf(P.nzloc) + tri_sb(P.nzloc, false)) * ns; // double2 per system
         HIPCHK(hipMalloc((void **)&S.FB, sizeof(double) * 2 * cap * fbn));
         s->dev_bytes += (long long)(sizeof(double) * 2 * cap * fbn);
+        const size_t spn = (size_t)spk_stride(P.nzloc) * ns; // doubles per system
+        HIPCHK(hipMalloc((void **)&S.SPK, sizeof(double) * cap * spn));
+        s->dev_bytes += (long long)(sizeof(double) * cap * spn);
         HIPCHK(hipMalloc((void **)&S.ipiv, sizeof(int) * cap * s->g.nz));
         HIPCHK(hipMalloc((void **)&S.coef, sizeof(double) * cap * 12 * ns));
         HIPCHK(hipMalloc((void **)&S.sysinfo, sizeof(double) * cap * 4));
@@ -581,6 +586,7 @@ static int ensure_factors(iw_solver *s)
     launch_zero_int(first == 0 ? s->sys.tri_bad : s->sys.tri_bad + 1, first == 0 ? 2 : 1, s->stream);
     launch_factor(s->g, s->sys, first, count, s->stream);
     launch_compact(s->g, s->sys, first, count, s->stream);
+    launch_spike_pre(s->g, s->sys, first, count, s->stream);
     HIPCHK(hipGetLastError());
     // the u/v, w and density systems go to the packed-factor sweeps (lane =
     // one column); if any of them pivoted, or a factor breaks the packed
@@ -668,6 +674,11 @@ static int ensure_factors(iw_solver *s)
         HIPCHK(hipMalloc((void **)&S.gwt_tasks, sizeof(int) * gwt.size()));
         HIPCHK(hipMemcpy(S.gwt_tasks, gwt.data(), sizeof(int) * gwt.size(), hipMemcpyHostToDevice));
         S.gwt_ntasks = (int)gwt.size();
+        // coupling scalars of the subdomain-parallel solve: 4 fields x nsubd x columns, twice
+        const size_t nal = (size_t)4 * g.nsubd * g.nxppy;
+        HIPCHK(hipMalloc((void **)&S.spk_al, sizeof(double) * nal));
+        HIPCHK(hipMalloc((void **)&S.spk_au, sizeof(double) * nal));
+        s->dev_bytes += (long long)(2 * sizeof(double) * nal);
     }
     s->factored_bdfv = s->bdfv;
     s->factored_dt = s->P.dt;
@@ -770,6 +781,10 @@ extern "C" int iw_create(const iw_params *p, iw_solver **out)
         return fail("ifilter = 1: estimation filter GLL order must lie in 8..24");
     iw_solver *s = new iw_solver();
     s->P = P;
+    {
+        const char *e = getenv("IW_SWEEPS");
+        s->seq_sweeps = e && !strcmp(e, "seq");
+    }
     Geo &g = s->g;
     g.nx = P.nx;
     g.ny = P.ny;
@@ -1019,6 +1034,9 @@ extern "C" int iw_destroy(iw_solver *s)
     hipFree(s->sys.AB);
     hipFree(s->sys.FB);
     hipFree(s->sys.PB);
+    hipFree(s->sys.SPK);
+    hipFree(s->sys.spk_al);
+    hipFree(s->sys.spk_au);
     hipFree(s->sys.tri_bad);
     for (double *q : s->cl_buf)
         if (q) hipFree(q);
@@ -1620,12 +1638,25 @@ static int calcuvwt_dev(iw_solver *s, int cpen)
             // + each distinct no-interchange factor's structural nonzeros
             // once (per row nl - r below the diagonal, r + 2 on and above it)
             const double fnp = (double)(s->P.nzloc + 2) * s->g.nz * 8.;
-            Scope sc(s, "solve_nat", 4 * 16. * nt + (s->sys.nsys - s->sys.np) * fnp);
-            launch_solve_grp(s->g, s->P, s->sys, uvwt, s->P.dt, s->stream);
+            // + the coupling scalars written (4 fields x nsubd x columns x 2)
+            const double cpl = 2. * 4. * s->g.nsubd * s->g.nxppy * 8.;
+            Scope sc(s, "solve_nat", 4 * 16. * nt + (s->sys.nsys - s->sys.np) * fnp + (s->seq_sweeps ? 0. : cpl));
+            if (s->seq_sweeps)
+                launch_solve_grp(s->g, s->P, s->sys, uvwt, s->P.dt, s->stream);
+            else
+                launch_solve_spike(s->g, s->P, s->sys, uvwt, s->P.dt, s->stream);
         }
-        {
+        if (s->seq_sweeps) {
             Scope sc(s, "lfilter", 4 * 16. * nt);
             launch_lfilter(s->g, uvwt, cpen, s->stream);
+        } else {
+            // + the coupling scalars (the recurrences read both and write
+            // both, the correction reads both) and each distinct system's
+            // k, h, beta, gamma, delta once
+            const double fix = 6. * 4. * s->g.nsubd * s->g.nxppy * 8. +
+                               (s->sys.nsys - s->sys.np) * (double)spk_stride(s->P.nzloc) * s->P.nsubd * 8.;
+            Scope sc(s, "lfilter", 4 * 16. * nt + fix);
+            launch_spike_finish(s->g, s->sys, uvwt, cpen, s->stream);
         }
     } else {
         Scope sc(s, "viscous", 64. * nt + (s->sys.nsys - s->sys.np) * fcol);

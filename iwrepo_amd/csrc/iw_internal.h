@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "iwgpu.h"
@@ -85,6 +86,12 @@ struct SysTables {
     // without interchanges, PB for the pressure systems (ids < np)
     double *FB = nullptr;
     double *PB = nullptr;
+    // subdomain-parallel solve of the FB systems: per (system, subdomain)
+    // k[nl], h[nl], beta, gamma, delta, 0 (spk_stride doubles); per (field,
+    // subdomain, column) the coupling scalars alpha_s (then c_{s-1}) and
+    // a_{s-1}
+    double *SPK = nullptr;
+    double *spk_al = nullptr, *spk_au = nullptr;
     // set by the packing when a factor breaks the tri_* pattern: [0] the
     // pressure factors, [1] the others
     int *tri_bad = nullptr;
@@ -202,6 +209,9 @@ struct iw_solver {
     bool mw_on = false;
     // mixed path (iw_set_mixed): FP32 physical-space working set in convec
     bool mixed = false;
+    // the u/v, w and rho solves by the sequential band sweeps (k_sweep)
+    // instead of the subdomain-parallel solve: IW_SWEEPS=seq at iw_create
+    bool seq_sweeps = false;
     // learned closure (iw_set_closure): the network and 8 scratch fields
     // (rho gradients x, y, z; the four vertical SGS fluxes; one more)
     struct iw_mlp *closure = nullptr;
@@ -332,6 +342,8 @@ constexpr int tri_off_b(int nl, int r, bool piv)
 // double2 per subdomain block: forward, backward
 constexpr int tri_sf(int nl) { return tri_off_f(nl, nl); }
 constexpr int tri_sb(int nl, bool piv) { return tri_off_b(nl, nl, piv); }
+// doubles per (system, subdomain) of SysTables::SPK
+constexpr int spk_stride(int nl) { return 2 * nl + 4; }
 void launch_pressure(const Geo &g, const iw_params &P, const SysTables &S, double *const *uvwt,
                      const double *wnl, const double *wviscrot, double dt, int cpen, hipStream_t st);
 void launch_viscous(const Geo &g, const iw_params &P, const SysTables &S, double *const *uvwt, double dt,
@@ -351,8 +363,99 @@ void launch_sgs_post(const Geo &g, double *const *uvwt, const double *tm, double
                      double *const *tmp, double *ox, int part, hipStream_t st);
 void launch_solve_grp(const Geo &g, const iw_params &P, const SysTables &S, double *const *uvwt, double dt,
                       hipStream_t st);
+// subdomain-parallel solve of the u/v, w and rho systems (k_spike), and
+// what completes it in place of launch_lfilter: the coupling recurrences and
+// correction, the Legendre filter and the mean-u damping (k_spike_finish)
+void launch_solve_spike(const Geo &g, const iw_params &P, const SysTables &S, double *const *uvwt, double dt,
+                        hipStream_t st);
+void launch_spike_finish(const Geo &g, const SysTables &S, double *const *uvwt, int cpen, hipStream_t st);
+// its per-(system, subdomain) vectors, after launch_compact
+void launch_spike_pre(const Geo &g, const SysTables &S, int first, int count, hipStream_t st);
 // part 0: right-hand side, 1: solve, 2: filter + projection
 void launch_pressure_nat(const Geo &g, const iw_params &P, const SysTables &S, double *const *uvwt, double *phi,
                          const double *wnl, const double *wviscrot, double dt, int cpen, int part,
                          hipStream_t st);
+
+// ---- device helpers shared by the kernel files
+// compile-time loop: f(std::integral_constant<int, I>) for I = B..E-1
+template <int B, int E, class F>
+__device__ __forceinline__ void static_for(F &&f)
+{
+    if constexpr (B < E) {
+        f(std::integral_constant<int, B>{});
+        static_for<B + 1, E>(f);
+    }
+}
+template <int I>
+using IC = std::integral_constant<int, I>;
+// the row updates of a sweep stay in the step that makes them: without this
+// the compiler sinks each update into the step that first reads the value,
+// keeping every pending multiplier of a subdomain in registers
+__device__ __forceinline__ void pin(double &x) { asm volatile("" : "+v"(x)); }
+
+// Lanes of one wavefront exchange data through LDS: a release/acquire fence
+// at wavefront scope with a wave barrier between them orders this lane's
+// earlier LDS stores before the other lanes' later loads.  A single
+// wavefront's LDS instructions execute in order, so the fences need no wait
+// counter and only fix the compiler's order.
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// the same filter as one matrix FL (nl x nl, in LDS): x[q] <- sum_t
+// FL[t nl + q] x[t], the two sums above taken together (half the
+// multiply-adds; equal to them up to rounding)
+template <int NL>
+__device__ __forceinline__ void legfilter_mat(double (&x)[NL], const double *FLs)
+{
+    double o[NL];
+#pragma unroll
+    for (int q = 0; q < NL; q++) o[q] = 0.;
+#pragma unroll
+    for (int t = 0; t < NL; t++) {
+        const double xt = x[t];
+#pragma unroll
+        for (int q = 0; q < NL; q++) o[q] = o[q] + FLs[t * NL + q] * xt;
+    }
+#pragma unroll
+    for (int q = 0; q < NL; q++) x[q] = o[q];
+}
+
+// two columns through the same matrix, each row of FL read once for both
+template <int NL>
+__device__ __forceinline__ void legfilter_mat2(double (&x)[NL], double (&y)[NL], const double *FLs)
+{
+    double o[NL], p[NL];
+#pragma unroll
+    for (int q = 0; q < NL; q++) {
+        o[q] = 0.;
+        p[q] = 0.;
+    }
+#pragma unroll
+    for (int t = 0; t < NL; t++) {
+        const double xt = x[t], yt = y[t];
+#pragma unroll
+        for (int q = 0; q < NL; q++) {
+            const double c = FLs[t * NL + q];
+            o[q] = o[q] + c * xt;
+            p[q] = p[q] + c * yt;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < NL; q++) {
+        x[q] = o[q];
+        y[q] = p[q];
+    }
+}
+
+template <int NL>
+__device__ __forceinline__ void stage_fl(double *FLs, const double *FL)
+{
+    for (int e = threadIdx.x; e < NL * NL; e += blockDim.x) FLs[e] = FL[e];
+    __syncthreads();
+}
+
 } // namespace iw

@@ -273,59 +273,6 @@ __device__ __forceinline__ void legfilter(double (&x)[NL], const double *wgs, co
     }
 }
 
-// the same filter as one matrix FL (nl x nl, in LDS): x[q] <- sum_t
-// FL[t nl + q] x[t], the two sums above taken together (half the
-// multiply-adds; equal to them up to rounding)
-template <int NL>
-__device__ __forceinline__ void legfilter_mat(double (&x)[NL], const double *FLs)
-{
-    double o[NL];
-#pragma unroll
-    for (int q = 0; q < NL; q++) o[q] = 0.;
-#pragma unroll
-    for (int t = 0; t < NL; t++) {
-        const double xt = x[t];
-#pragma unroll
-        for (int q = 0; q < NL; q++) o[q] = o[q] + FLs[t * NL + q] * xt;
-    }
-#pragma unroll
-    for (int q = 0; q < NL; q++) x[q] = o[q];
-}
-
-// two columns through the same matrix, each row of FL read once for both
-template <int NL>
-__device__ __forceinline__ void legfilter_mat2(double (&x)[NL], double (&y)[NL], const double *FLs)
-{
-    double o[NL], p[NL];
-#pragma unroll
-    for (int q = 0; q < NL; q++) {
-        o[q] = 0.;
-        p[q] = 0.;
-    }
-#pragma unroll
-    for (int t = 0; t < NL; t++) {
-        const double xt = x[t], yt = y[t];
-#pragma unroll
-        for (int q = 0; q < NL; q++) {
-            const double c = FLs[t * NL + q];
-            o[q] = o[q] + c * xt;
-            p[q] = p[q] + c * yt;
-        }
-    }
-#pragma unroll
-    for (int q = 0; q < NL; q++) {
-        x[q] = o[q];
-        y[q] = p[q];
-    }
-}
-
-template <int NL>
-__device__ __forceinline__ void stage_fl(double *FLs, const double *FL)
-{
-    for (int e = threadIdx.x; e < NL * NL; e += blockDim.x) FLs[e] = FL[e];
-    __syncthreads();
-}
-
 template <int NL>
 __device__ __forceinline__ void stage_tables(double *wgs, double *ALs, double *sgs, const double *wg,
                                              const double *AL, const double *sig)
@@ -2323,34 +2270,6 @@ void launch_viscous(const Geo &g, const iw_params &P, const SysTables &S, double
 // Task entry: col | field << 28 | idle << 29 (an idle lane helps load its
 // group's factors but owns no column); -1 marks an empty group.
 // ---------------------------------------------------------------------------
-// compile-time loop: f(std::integral_constant<int, I>) for I = B..E-1
-template <int B, int E, class F>
-__device__ __forceinline__ void static_for(F &&f)
-{
-    if constexpr (B < E) {
-        f(std::integral_constant<int, B>{});
-        static_for<B + 1, E>(f);
-    }
-}
-template <int I>
-using IC = std::integral_constant<int, I>;
-// the row updates of a sweep stay in the step that makes them: without this
-// the compiler sinks each update into the step that first reads the value,
-// keeping every pending multiplier of a subdomain in registers
-__device__ __forceinline__ void pin(double &x) { asm volatile("" : "+v"(x)); }
-
-// Lanes of one wavefront exchange data through LDS: a release/acquire fence
-// at wavefront scope with a wave barrier between them orders this lane's
-// earlier LDS stores before the other lanes' later loads.  A single
-// wavefront's LDS instructions execute in order, so the fences need no wait
-// counter and only fix the compiler's order.
-__device__ __forceinline__ void wave_sync()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 struct SweepArgs {
     double *f[4];
     const int *tasks;
@@ -2611,53 +2530,6 @@ void launch_solve_grp(const Geo &g, const iw_params &P, const SysTables &S, doub
     default: break;
     }
 #undef GS
-}
-
-// Legendre filter of every subdomain of the four solutions (filter_legenpen,
-// filtering.f:729-808) and the mean-u damping (calcuvwt.f:343-361), in place
-// on the native layout: lane = the re and im columns of one mode, the nl
-// values of its subdomain in registers, the filter matrix in LDS.
-template <int NL>
-__global__ __launch_bounds__(256) void k_lfilter(double *u, double *v, double *w, double *t, long nxppy,
-                                                int cpen, const double *__restrict__ FL,
-                                                const double *__restrict__ damp)
-{
-    __shared__ double FLs[NL * NL];
-    stage_fl<NL>(FLs, FL);
-    const long m = (long)blockIdx.x * blockDim.x + threadIdx.x; // the re/im column pair of one mode
-    const int ks = blockIdx.y, fld = blockIdx.z;
-    if (m >= nxppy / 2) return;
-    if (fld == 2 && m == 0) return;
-    double *f = (fld == 0 ? u : fld == 1 ? v : fld == 2 ? w : t) + (long)ks * NL * nxppy + 2 * m;
-    double xr[NL], xi[NL];
-#pragma unroll
-    for (int k = 0; k < NL; k++) {
-        const double2 q = *reinterpret_cast<const double2 *>(f + (long)k * nxppy);
-        xr[k] = q.x;
-        xi[k] = q.y;
-    }
-    if (cpen) legfilter_mat2<NL>(xr, xi, FLs);
-    if (fld == 0 && m == 0)
-#pragma unroll
-        for (int k = 0; k < NL; k++) {
-            xr[k] = damp[ks * NL + k] * xr[k];
-            xi[k] = damp[ks * NL + k] * xi[k];
-        }
-#pragma unroll
-    for (int k = 0; k < NL; k++) *reinterpret_cast<double2 *>(f + (long)k * nxppy) = make_double2(xr[k], xi[k]);
-}
-
-void launch_lfilter(const Geo &g, double *const *uvwt, int cpen, hipStream_t st)
-{
-    dim3 grid((unsigned)((g.nxppy / 2 + 255) / 256), g.nsubd, 4);
-#define LF(NL)                                                                                             \
-    hipLaunchKernelGGL(k_lfilter<NL>, grid, dim3(256), 0, st, uvwt[0], uvwt[1], uvwt[2], uvwt[3], g.nxppy, \
-                       cpen, g.FL, g.damp)
-    switch (g.nzloc) {
-        IW_NL_SWITCH(LF)
-    default: break;
-    }
-#undef LF
 }
 
 // ---------------------------------------------------------------------------
