@@ -433,6 +433,7 @@ static int build_systems(iw_solver *s, bool refactor_p, int &first, int &count)
         if (S.AB) hipFree(S.AB);
         if (S.FB) hipFree(S.FB);
         if (S.SPK) hipFree(S.SPK);
+        if (S.SPB) hipFree(S.SPB);
         if (S.ipiv) hipFree(S.ipiv);
         if (S.coef) hipFree(S.coef);
         if (S.sysinfo) hipFree(S.sysinfo);
@@ -446,7 +447,8 @@ static int build_systems(iw_solver *s, bool refactor_p, int &first, int &count)
         s->dev_bytes += (long long)(sizeof(double) * 2 * cap * fbn);
         const size_t spn = (size_t)spk_stride(P.nzloc) * ns; // doubles per system
         HIPCHK(hipMalloc((void **)&S.SPK, sizeof(double) * cap * spn));
-        s->dev_bytes += (long long)(sizeof(double) * cap * spn);
+        HIPCHK(hipMalloc((void **)&S.SPB, sizeof(double) * cap * 4 * ns));
+        s->dev_bytes += (long long)(sizeof(double) * cap * (spn + 4 * ns));
         HIPCHK(hipMalloc((void **)&S.ipiv, sizeof(int) * cap * s->g.nz));
         HIPCHK(hipMalloc((void **)&S.coef, sizeof(double) * cap * 12 * ns));
         HIPCHK(hipMalloc((void **)&S.sysinfo, sizeof(double) * cap * 4));
@@ -1035,6 +1037,7 @@ extern "C" int iw_destroy(iw_solver *s)
     hipFree(s->sys.FB);
     hipFree(s->sys.PB);
     hipFree(s->sys.SPK);
+    hipFree(s->sys.SPB);
     hipFree(s->sys.spk_al);
     hipFree(s->sys.spk_au);
     hipFree(s->sys.tri_bad);
@@ -1652,9 +1655,9 @@ static int calcuvwt_dev(iw_solver *s, int cpen)
         } else {
             // + the coupling scalars (the recurrences read both and write
             // both, the correction reads both) and each distinct system's
-            // k, h, beta, gamma, delta once
+            // k, h and beta, gamma, delta once
             const double fix = 6. * 4. * s->g.nsubd * s->g.nxppy * 8. +
-                               (s->sys.nsys - s->sys.np) * (double)spk_stride(s->P.nzloc) * s->P.nsubd * 8.;
+                               (s->sys.nsys - s->sys.np) * (2. * s->P.nzloc + 4.) * s->P.nsubd * 8.;
             Scope sc(s, "lfilter", 4 * 16. * nt + fix);
             launch_spike_finish(s->g, s->sys, uvwt, cpen, s->stream);
         }

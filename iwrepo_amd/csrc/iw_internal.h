@@ -89,8 +89,9 @@ struct SysTables {
     // subdomain-parallel solve of the FB systems: per (system, subdomain)
     // k[nl], h[nl], beta, gamma, delta, 0 (spk_stride doubles); per (field,
     // subdomain, column) the coupling scalars alpha_s (then c_{s-1}) and
-    // a_{s-1}
-    double *SPK = nullptr;
+    // a_{s-1}; SPB: beta, gamma, delta, 0 per (system, subdomain) again,
+    // packed for the recurrences
+    double *SPK = nullptr, *SPB = nullptr;
     double *spk_al = nullptr, *spk_au = nullptr;
     // set by the packing when a factor breaks the tri_* pattern: [0] the
     // pressure factors, [1] the others
@@ -342,6 +343,28 @@ constexpr int tri_off_b(int nl, int r, bool piv)
 // double2 per subdomain block: forward, backward
 constexpr int tri_sf(int nl) { return tri_off_f(nl, nl); }
 constexpr int tri_sb(int nl, bool piv) { return tri_off_b(nl, nl, piv); }
+// The Fourier filter of one spectral value (filtering.f:852-901): magnitude
+// |f|, phase +-acos(f_r/|f|) (the sign of f_i), then |f| sigma (cos, sin) of
+// the phase.  cos(acos c) = c and sin(acos c) = sqrt((1 - c)(1 + c)) >= 0
+// give the same two values without the transcendental calls; c is the same
+// rounded quotient, so the loss of the reference's form for nearly real
+// modes (sin of the phase from 1 - c, c rounded) is kept, and the results
+// agree with the acos/sin/cos evaluation to a few ulps of |f| sigma.
+__device__ __forceinline__ void fourier_filter_value(double &fr, double &fi, double sigma)
+{
+    const double fmag = sqrt(fr * fr + fi * fi);
+    if (fmag != 0.) {
+        const double cosc = fr / fmag, sinc = fi / fmag;
+        const double fmagf = fmag * sigma;
+        const double sn = sqrt((1. - cosc) * (1. + cosc));
+        fr = fmagf * cosc;
+        fi = (sinc >= 0.) ? fmagf * sn : -(fmagf * sn);
+    } else {
+        fr = 0.;
+        fi = 0.;
+    }
+}
+
 // doubles per (system, subdomain) of SysTables::SPK
 constexpr int spk_stride(int nl) { return 2 * nl + 4; }
 void launch_pressure(const Geo &g, const iw_params &P, const SysTables &S, double *const *uvwt,

@@ -1490,17 +1490,7 @@ __global__ __launch_bounds__(256) void k_tile(TileArgs A, long ncplx_plane, long
             int t = idx / kMB, cc = idx - t * kMB;
             if (cc >= nmb) continue;
             double fr = tile[t * W2 + 2 * cc], fi = tile[t * W2 + 2 * cc + 1];
-            double fmag = sqrt(fr * fr + fi * fi);
-            double cosc = fr / fmag, sinc = fi / fmag;
-            if (fmag != 0.) {
-                double phase = (sinc >= 0.) ? acos(cosc) : -acos(cosc);
-                double fmagf = fmag * fcf[c0 + cc];
-                fr = fmagf * cos(phase);
-                fi = fmagf * sin(phase);
-            } else {
-                fr = 0.;
-                fi = 0.;
-            }
+            fourier_filter_value(fr, fi, fcf[c0 + cc]);
             tile[t * W2 + 2 * cc] = fr;
             tile[t * W2 + 2 * cc + 1] = fi;
         }
@@ -1573,22 +1563,7 @@ __global__ __launch_bounds__(256, 2) void k_tile_t(TileArgs A, long nmode, long 
     if (A.ifourier) { // Fourier magnitude/phase filter, filtering.f:852-901
         const double sigf = fcf[m];
 #pragma unroll
-        for (int t = 0; t < NL; t++) {
-            const double fr = xr[t], fi = xi[t];
-            const double fmag = sqrt(fr * fr + fi * fi);
-            double gr = 0., gi = 0.;
-            if (fmag != 0.) {
-                const double cosc = fr / fmag, sinc = fi / fmag;
-                const double phase = (sinc >= 0.) ? acos(cosc) : -acos(cosc);
-                const double fmagf = fmag * sigf;
-                double sn, cs;
-                sincos(phase, &sn, &cs);
-                gr = fmagf * cs;
-                gi = fmagf * sn;
-            }
-            xr[t] = gr;
-            xi[t] = gi;
-        }
+        for (int t = 0; t < NL; t++) fourier_filter_value(xr[t], xi[t], sigf);
     }
     if (A.ilegen) legfilter_mat2<NL>(xr, xi, FLs); // filtering.f:905-956, the two sums as one matrix
 #pragma unroll

@@ -43,8 +43,8 @@ namespace iw {
 // ---------------------------------------------------------------------------
 
 template <int NL>
-__global__ __launch_bounds__(64) void k_spike_pre(const double *__restrict__ FB, double *SPK, int first, int count,
-                                                  int np, int nsub)
+__global__ __launch_bounds__(64) void k_spike_pre(const double *__restrict__ FB, double *SPK, double *SPB, int first,
+                                                  int count, int np, int nsub)
 {
     constexpr int SF = tri_sf(NL), SB = tri_sb(NL, false), ST = spk_stride(NL);
     const long id = (long)blockIdx.x * 64 + threadIdx.x;
@@ -98,6 +98,15 @@ __global__ __launch_bounds__(64) void k_spike_pre(const double *__restrict__ FB,
         o[2 * NL + 1 - ST] = gam;
         o[2 * NL + 2 - ST] = del;
     }
+    // the same three per subdomain, 4 doubles apart, for the recurrences
+    double *p = SPB + ((long)sys * nsub + s) * 4;
+    p[0] = beta;
+    p[3] = 0.;
+    if (s == nsub - 1) p[1] = p[2] = 0.;
+    if (s > 0) {
+        p[1 - 4] = gam;
+        p[2 - 4] = del;
+    }
 }
 
 void launch_spike_pre(const Geo &g, const SysTables &S, int first, int count, hipStream_t st)
@@ -106,7 +115,7 @@ void launch_spike_pre(const Geo &g, const SysTables &S, int first, int count, hi
     const long nt = (long)count * g.nsubd;
     const dim3 grid((unsigned)((nt + 63) / 64));
 #define SP(NL) \
-    hipLaunchKernelGGL(k_spike_pre<NL>, grid, dim3(64), 0, st, S.FB, S.SPK, first, count, S.np, g.nsubd)
+    hipLaunchKernelGGL(k_spike_pre<NL>, grid, dim3(64), 0, st, S.FB, S.SPK, S.SPB, first, count, S.np, g.nsubd)
     switch (g.nzloc) {
         IW_NL_SWITCH(SP)
     default: break;
@@ -280,9 +289,10 @@ void launch_solve_spike(const Geo &g, const iw_params &P, const SysTables &S, do
 #undef GS
 }
 
-// the two scalar recurrences per column (one thread per column and field)
+// the two scalar recurrences per column (one thread per column and field);
+// SPB holds beta, gamma, delta of (system, subdomain) at (system nsub + s) 4
 __global__ __launch_bounds__(256) void k_spike_couple(double *al, double *au, const int *__restrict__ modesys,
-                                                      const double *__restrict__ SPK, long nxppy, int nsub, int nl)
+                                                      const double *__restrict__ SPB, long nxppy, int nsub)
 {
     const long col = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const int fld = blockIdx.y;
@@ -290,22 +300,26 @@ __global__ __launch_bounds__(256) void k_spike_couple(double *al, double *au, co
     const int m = (int)(col >> 1);
     if (fld == 2 && m == 0) return;
     const int sys = modesys[4 * m + (fld < 2 ? 1 : fld == 2 ? 2 : 3)];
-    const int stp = spk_stride(nl);
-    const double *bgd = SPK + (long)sys * nsub * stp + 2 * nl; // beta, gamma, delta of subdomain s at s stp
+    const double4 *bgd = reinterpret_cast<const double4 *>(SPB) + (long)sys * nsub;
     double *a1 = al + (long)fld * nsub * nxppy + col, *a2 = au + (long)fld * nsub * nxppy + col;
     // c_s = alpha_s - beta_s c_{s-1}; slot s receives c_{s-1}
     double c = 0.;
+#pragma unroll 4
     for (int s = 0; s < nsub; s++) {
         const double alpha = a1[(long)s * nxppy];
         a1[(long)s * nxppy] = c;
-        c = alpha - bgd[s * stp] * c;
+        c = alpha - bgd[s].x * c;
     }
     // d_s = a_s - gamma_s c_s - delta_s d_{s+1}, a_s = a_{(s+1)-1} from block
     // s+1, c_s in slot s+1; slot s receives d_s
     double d = 0., anext = 0.;
+#pragma unroll 4
     for (int s = nsub - 1; s >= 0; s--) {
         const double ap = a2[(long)s * nxppy];
-        if (s < nsub - 1) d = anext - bgd[s * stp + 1] * a1[(long)(s + 1) * nxppy] - bgd[s * stp + 2] * d;
+        if (s < nsub - 1) {
+            const double4 q = bgd[s];
+            d = anext - q.y * a1[(long)(s + 1) * nxppy] - q.z * d;
+        }
         a2[(long)s * nxppy] = d;
         anext = ap;
     }
@@ -378,7 +392,7 @@ __global__ __launch_bounds__(256) void k_spike_finish(FinishArgs A)
 void launch_spike_finish(const Geo &g, const SysTables &S, double *const *uvwt, int cpen, hipStream_t st)
 {
     hipLaunchKernelGGL(k_spike_couple, dim3((unsigned)((g.nxppy + 255) / 256), 4), dim3(256), 0, st, S.spk_al,
-                       S.spk_au, S.modesys, S.SPK, g.nxppy, g.nsubd, g.nzloc);
+                       S.spk_au, S.modesys, S.SPB, g.nxppy, g.nsubd);
     FinishArgs A;
     for (int q = 0; q < 4; q++) A.f[q] = uvwt[q];
     A.modesys = S.modesys;
