@@ -16,6 +16,30 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 b)
 {
     return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b)
+{
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// the LDS tile of the transforms holds FP64 values, or FP32 values on the
+// mixed path (C = float2); the pre- and post-processing stays FP64
+__device__ __forceinline__ double2 tod(double2 v) { return v; }
+__device__ __forceinline__ double2 tod(float2 v) { return make_double2((double)v.x, (double)v.y); }
+template <typename C>
+__device__ __forceinline__ C toc(double2 v);
+template <>
+__device__ __forceinline__ double2 toc<double2>(double2 v)
+{
+    return v;
+}
+template <>
+__device__ __forceinline__ float2 toc<float2>(double2 v)
+{
+    return make_float2((float)v.x, (float)v.y);
+}
+// tile type of a transform with storage types TI (in) and TO (out)
+template <typename TI, typename TO>
+using tile_t = typename std::conditional<std::is_same<TI, float>::value || std::is_same<TO, float>::value, float2,
+                                         double2>::type;
 
 // ---------------------------------------------------------------------------
 // In-place multi-radix Stockham FFT on LDS.  A block of T threads holds
@@ -38,60 +62,79 @@ __host__ __device__ __forceinline__ int lp(int i) { return i ^ ((i >> 3) & 15); 
 
 __device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
 // multiply by sign*i
 __device__ __forceinline__ double2 cmuli(double2 a, int sign)
 {
     return sign < 0 ? make_double2(a.y, -a.x) : make_double2(-a.y, a.x);
 }
-
-template <int R>
-__device__ __forceinline__ void dft(double2 *a, int sign)
+__device__ __forceinline__ float2 cmuli(float2 a, int sign)
 {
+    return sign < 0 ? make_float2(a.y, -a.x) : make_float2(-a.y, a.x);
+}
+template <typename C>
+__device__ __forceinline__ C cmake(double x, double y);
+template <>
+__device__ __forceinline__ double2 cmake<double2>(double x, double y)
+{
+    return make_double2(x, y);
+}
+template <>
+__device__ __forceinline__ float2 cmake<float2>(double x, double y)
+{
+    return make_float2((float)x, (float)y);
+}
+
+template <int R, typename C>
+__device__ __forceinline__ void dft(C *a, int sign)
+{
+    using S = decltype(a[0].x);
     if (R == 2) {
-        const double2 t = a[0];
+        const C t = a[0];
         a[0] = cadd(t, a[1]);
         a[1] = csub(t, a[1]);
     } else if (R == 4) {
-        const double2 t0 = cadd(a[0], a[2]), t1 = csub(a[0], a[2]);
-        const double2 t2 = cadd(a[1], a[3]), t3 = cmuli(csub(a[1], a[3]), sign);
+        const C t0 = cadd(a[0], a[2]), t1 = csub(a[0], a[2]);
+        const C t2 = cadd(a[1], a[3]), t3 = cmuli(csub(a[1], a[3]), sign);
         a[0] = cadd(t0, t2);
         a[2] = csub(t0, t2);
         a[1] = cadd(t1, t3);
         a[3] = csub(t1, t3);
     } else if (R == 3) {
         // y_q = sum_m a_m e^{sign 2 pi i m q / 3}
-        constexpr double h = 0.86602540378443864676372317075293618; // sqrt(3)/2
-        const double2 t1 = cadd(a[1], a[2]);
-        const double2 t2 = make_double2(a[0].x - 0.5 * t1.x, a[0].y - 0.5 * t1.y);
-        const double2 d = cmuli(csub(a[1], a[2]), sign); // sign i (a1 - a2)
+        constexpr S h = (S)0.86602540378443864676372317075293618; // sqrt(3)/2
+        const C t1 = cadd(a[1], a[2]);
+        const C t2 = cmake<C>(a[0].x - (S)0.5 * t1.x, a[0].y - (S)0.5 * t1.y);
+        const C d = cmuli(csub(a[1], a[2]), sign); // sign i (a1 - a2)
         a[0] = cadd(a[0], t1);
-        a[1] = make_double2(t2.x + h * d.x, t2.y + h * d.y);
-        a[2] = make_double2(t2.x - h * d.x, t2.y - h * d.y);
+        a[1] = cmake<C>(t2.x + h * d.x, t2.y + h * d.y);
+        a[2] = cmake<C>(t2.x - h * d.x, t2.y - h * d.y);
     } else if (R == 5) {
         // cos and sin of 2 pi/5 and 4 pi/5: (sqrt5 - 1)/4, sqrt((5 + sqrt5)/8),
         // -(sqrt5 + 1)/4, sqrt((5 - sqrt5)/8)
-        constexpr double c1 = 0.30901699437494742410229341718281906, s1 = 0.95105651629515357211643933337938214;
-        constexpr double c2 = -0.80901699437494742410229341718281906, s2 = 0.58778525229247312916870595463907277;
-        const double2 b1 = cadd(a[1], a[4]), b2 = cadd(a[2], a[3]);
-        const double2 d1 = cmuli(csub(a[1], a[4]), sign), d2 = cmuli(csub(a[2], a[3]), sign);
-        const double2 p1 = make_double2(a[0].x + c1 * b1.x + c2 * b2.x, a[0].y + c1 * b1.y + c2 * b2.y);
-        const double2 p2 = make_double2(a[0].x + c2 * b1.x + c1 * b2.x, a[0].y + c2 * b1.y + c1 * b2.y);
-        const double2 q1 = make_double2(s1 * d1.x + s2 * d2.x, s1 * d1.y + s2 * d2.y);
-        const double2 q2 = make_double2(s2 * d1.x - s1 * d2.x, s2 * d1.y - s1 * d2.y);
+        constexpr S c1 = (S)0.30901699437494742410229341718281906, s1 = (S)0.95105651629515357211643933337938214;
+        constexpr S c2 = (S)-0.80901699437494742410229341718281906, s2 = (S)0.58778525229247312916870595463907277;
+        const C b1 = cadd(a[1], a[4]), b2 = cadd(a[2], a[3]);
+        const C d1 = cmuli(csub(a[1], a[4]), sign), d2 = cmuli(csub(a[2], a[3]), sign);
+        const C p1 = cmake<C>(a[0].x + c1 * b1.x + c2 * b2.x, a[0].y + c1 * b1.y + c2 * b2.y);
+        const C p2 = cmake<C>(a[0].x + c2 * b1.x + c1 * b2.x, a[0].y + c2 * b1.y + c1 * b2.y);
+        const C q1 = cmake<C>(s1 * d1.x + s2 * d2.x, s1 * d1.y + s2 * d2.y);
+        const C q2 = cmake<C>(s2 * d1.x - s1 * d2.x, s2 * d1.y - s1 * d2.y);
         a[0] = cadd(a[0], cadd(b1, b2));
         a[1] = cadd(p1, q1);
         a[4] = csub(p1, q1);
         a[2] = cadd(p2, q2);
         a[3] = csub(p2, q2);
     } else { // R == 8: two 4-point DFTs of the even and odd inputs
-        constexpr double c = 0.70710678118654752440084436210484904; // sqrt(2)/2
-        double2 e[4] = {a[0], a[2], a[4], a[6]}, o[4] = {a[1], a[3], a[5], a[7]};
+        constexpr S c = (S)0.70710678118654752440084436210484904; // sqrt(2)/2
+        C e[4] = {a[0], a[2], a[4], a[6]}, o[4] = {a[1], a[3], a[5], a[7]};
         dft<4>(e, sign);
         dft<4>(o, sign);
         // o_q *= e^{sign 2 pi i q/8}
-        const double2 o1 = make_double2(c * (o[1].x - sign * o[1].y), c * (o[1].y + sign * o[1].x));
-        const double2 o2 = cmuli(o[2], sign);
-        const double2 o3 = make_double2(c * (-o[3].x - sign * o[3].y), c * (-o[3].y + sign * o[3].x));
+        const C o1 = cmake<C>(c * (o[1].x - (S)sign * o[1].y), c * (o[1].y + (S)sign * o[1].x));
+        const C o2 = cmuli(o[2], sign);
+        const C o3 = cmake<C>(c * (-o[3].x - (S)sign * o[3].y), c * (-o[3].y + (S)sign * o[3].x));
         a[0] = cadd(e[0], o[0]);
         a[4] = csub(e[0], o[0]);
         a[1] = cadd(e[1], o1);
@@ -108,14 +151,13 @@ __device__ __forceinline__ void dft(double2 *a, int sign)
 // sequence r sits at i*nrows + r and consecutive threads take the same
 // butterfly of consecutive sequences (nrows a power of two), so every
 // 16-lane access is 16 consecutive slots.
-template <int R, int LR, bool COL>
-__device__ __forceinline__ void lds_pass(double2 *buf, int logN, int logNs, int nrows, int lrows,
-                                         const double2 *tw, int sign)
+template <int R, int LR, bool COL, typename C>
+__device__ __forceinline__ void lds_pass(C *buf, int logN, int logNs, int nrows, int lrows, const C *tw, int sign)
 {
     constexpr int BPT = 8 / R;
     const int lNR = logN - LR, NR = 1 << lNR, Ns = 1 << logNs, nb = nrows << lNR;
     const int lstep = logN - logNs - LR;
-    double2 a[BPT][R];
+    C a[BPT][R];
     int base[BPT], rr[BPT];
 #pragma unroll
     for (int i = 0; i < BPT; i++) {
@@ -137,7 +179,7 @@ __device__ __forceinline__ void lds_pass(double2 *buf, int logN, int logNs, int 
                 a[i][m] = COL ? buf[((j + m * NR) << lrows) + r] : buf[lp((r << logN) + j + m * NR)];
 #pragma unroll
             for (int m = 1; m < R; m++) {
-                double2 w = tw[(m * k) << lstep];
+                C w = tw[(m * k) << lstep];
                 if (sign > 0) w.y = -w.y;
                 a[i][m] = cmul(a[i][m], w);
             }
@@ -162,8 +204,8 @@ __device__ __forceinline__ void lds_pass(double2 *buf, int logN, int logNs, int 
 }
 
 // nrows sequences of length N = 2^logN in buf, in place
-template <bool COL>
-__device__ void fft_lds(double2 *buf, int logN, int nrows, const double2 *tw, int sign)
+template <bool COL, typename C>
+__device__ void fft_lds(C *buf, int logN, int nrows, const C *tw, int sign)
 {
     const int lrows = COL ? __builtin_ctz(nrows) : 0;
     int lNs = 0, left = logN;
@@ -180,12 +222,12 @@ __device__ void fft_lds(double2 *buf, int logN, int nrows, const double2 *tw, in
 // the same self-sorting passes with the radices of `plan` (4 bits each,
 // first pass lowest) and division in place of shifts.  A block's tile holds
 // nrows whole sequences (nrows N <= 8 blockDim values).
-template <int R, bool COL>
-__device__ __forceinline__ void lds_pass_g(double2 *buf, int N, int Ns, int nrows, const double2 *tw, int sign)
+template <int R, bool COL, typename C>
+__device__ __forceinline__ void lds_pass_g(C *buf, int N, int Ns, int nrows, const C *tw, int sign)
 {
     constexpr int BPT = (8 + R - 1) / R;
     const int NR = N / R, nb = nrows * NR, step = N / (Ns * R);
-    double2 a[BPT][R];
+    C a[BPT][R];
     int base[BPT], rr[BPT];
 #pragma unroll
     for (int i = 0; i < BPT; i++) {
@@ -207,7 +249,7 @@ __device__ __forceinline__ void lds_pass_g(double2 *buf, int N, int Ns, int nrow
                 a[i][m] = COL ? buf[(j + m * NR) * nrows + r] : buf[lp(r * N + j + m * NR)];
 #pragma unroll
             for (int m = 1; m < R; m++) {
-                double2 w = tw[m * k * step];
+                C w = tw[m * k * step];
                 if (sign > 0) w.y = -w.y;
                 a[i][m] = cmul(a[i][m], w);
             }
@@ -231,8 +273,8 @@ __device__ __forceinline__ void lds_pass_g(double2 *buf, int N, int Ns, int nrow
     __syncthreads();
 }
 
-template <bool COL>
-__device__ void fft_lds_g(double2 *buf, int N, unsigned plan, int nrows, const double2 *tw, int sign)
+template <bool COL, typename C>
+__device__ void fft_lds_g(C *buf, int N, unsigned plan, int nrows, const C *tw, int sign)
 {
     int Ns = 1;
     for (; plan; plan >>= 4) {
@@ -328,13 +370,16 @@ __global__ __launch_bounds__(kFT) void k_fft_x(FieldBatch fb, int nrows, int ny,
                                                unsigned plan, int R, const double2 *__restrict__ twM,
                                                const double2 *__restrict__ twNg, double scaley, McFuse mf)
 {
+    using C = tile_t<TI, TO>;
     extern __shared__ double2 smx[];
-    double2 *buf = smx;
     // the tile: 8 blockDim slots (R M of them used; lp permutes within
     // aligned runs of 128, so a partial tile's slots stay inside the block)
-    double2 *tw = smx + 8 * (int)blockDim.x;
+    C *buf = reinterpret_cast<C *>(smx);
+    C *tw = buf + 8 * (int)blockDim.x;
     const int nval = R * M;
-    double2 *nyq = tw + M; // R entries: the kx-Nyquist coefficient of each row (inverse, for maxcalc)
+    // R entries: the kx-Nyquist coefficient of each row (inverse, for
+    // maxcalc), 16-byte aligned after the M twiddles
+    double2 *nyq = reinterpret_cast<double2 *>(tw + (sizeof(C) == 8 ? (M + 1) & ~1 : M));
     // the length-N twiddles and the wavenumbers kx of k = 0..M, staged in LDS
     // with tw so that no per-element global table load sits between a
     // thread's data loads or stores
@@ -379,7 +424,7 @@ __global__ __launch_bounds__(kFT) void k_fft_x(FieldBatch fb, int nrows, int ny,
         if (!MAPS) return (long)row * nxpp;
         return staged ? rmap[R + row - row0] : drow_g(row);
     };
-    for (int t = threadIdx.x; t < M; t += blockDim.x) tw[t] = twM[t];
+    for (int t = threadIdx.x; t < M; t += blockDim.x) tw[t] = toc<C>(twM[t]);
     for (int t = threadIdx.x; t <= M; t += blockDim.x) {
         twN[t] = twNg[t];
         kws[t] = fb.kw ? fb.kw[t] : 0.;
@@ -441,7 +486,7 @@ __global__ __launch_bounds__(kFT) void k_fft_x(FieldBatch fb, int nrows, int ny,
             double orr = dr * c - di * s, oi = dr * s + di * c;
             double2 Z = make_double2(er - oi, ei + orr);
             if (row >= nrows) Z = make_double2(0., 0.);
-            buf[lp(idx)] = Z;
+            buf[lp(idx)] = toc<C>(Z);
         }
     };
     auto store_inverse = [&](TO *ob) {
@@ -451,7 +496,7 @@ __global__ __launch_bounds__(kFT) void k_fft_x(FieldBatch fb, int nrows, int ny,
             if (!P2 && idx >= nval) continue;
             int r = rowof(idx), m = colof(idx, r), row = row0 + r;
             if (row >= nrows) continue;
-            st2(ob + drow(row) + 2 * m, buf[lp(idx)]);
+            st2(ob + drow(row) + 2 * m, tod(buf[lp(idx)]));
         }
         for (int r = threadIdx.x; r < R; r += blockDim.x) {
             int row = row0 + r;
@@ -477,15 +522,15 @@ __global__ __launch_bounds__(kFT) void k_fft_x(FieldBatch fb, int nrows, int ny,
                 int r = rowof(idx), m = colof(idx, r), row = row0 + r;
                 double2 z = ld2(src + srow(row < nrows ? row : nrows - 1) + 2 * m);
                 if (row >= nrows) z = make_double2(0., 0.);
-                buf[lp(idx)] = z;
+                buf[lp(idx)] = toc<C>(z);
             }
         };
         // coefficient k from Z_k and Z_{M-k}; the thread of k = 0 also
         // forms k = M (both from Z_0)
         auto coef = [&](int r, int k) -> double2 {
             const int r0 = P2 ? r << logM : r * M;
-            double2 Zk = buf[lp(r0 + (k == M ? 0 : k))];
-            double2 Zm = buf[lp(r0 + (k == 0 ? 0 : M - k))];
+            double2 Zk = tod(buf[lp(r0 + (k == M ? 0 : k))]);
+            double2 Zm = tod(buf[lp(r0 + (k == 0 ? 0 : M - k))]);
             double er = 0.5 * (Zk.x + Zm.x), ei = 0.5 * (Zk.y - Zm.y);
             double dr = Zk.x - Zm.x, di = Zk.y + Zm.y;
             double orr = 0.5 * di, oi = -0.5 * dr;
@@ -518,7 +563,7 @@ __global__ __launch_bounds__(kFT) void k_fft_x(FieldBatch fb, int nrows, int ny,
                 const int row = rowA + (isp ? r - H : r);
                 double2 z = ld2((isp ? acc : base) + srow(row < nrows ? row : nrows - 1) + 2 * m);
                 if (row >= nrows || (!P2 && r >= 2 * H)) z = make_double2(0., 0.);
-                buf[lp(idx)] = z;
+                buf[lp(idx)] = toc<C>(z);
             }
             __syncthreads();
             fft_rows(R);
@@ -569,7 +614,7 @@ __global__ __launch_bounds__(kFT) void k_fft_x(FieldBatch fb, int nrows, int ny,
                 const int r = rowof(idx), m = colof(idx, r), row = row0 + r;
                 if (row >= nrows) continue;
                 const int k = planeof(row), j = jof(row, k);
-                const double2 z = buf[lp(idx)];
+                const double2 z = tod(buf[lp(idx)]);
                 const double c = mf.cny * nyq[r].x;
                 double xe = z.x + c, xo = z.y - c; // points i = 2m, 2m+1
                 if (f == 2 && mf.w00) {
@@ -651,9 +696,10 @@ template <int DIR, typename TI = double, typename TO = double, bool P2 = true>
 __global__ __launch_bounds__(kFT) void k_fft_y(FieldBatch fb, int nxpp, int nxh, int ny, int logNy, unsigned plan,
                                                int CB, const double2 *__restrict__ twY, double scaley)
 {
+    using C = tile_t<TI, TO>;
     extern __shared__ double2 smy[];
-    double2 *buf = smy;
-    double2 *tw = smy + CB * ny;
+    C *buf = reinterpret_cast<C *>(smy);
+    C *tw = buf + CB * ny;
     double *kys = reinterpret_cast<double *>(tw + ny); // ky per row, staged like tw
     const int lpl = blockIdx.y;
     const TI *plane = reinterpret_cast<const TI *>(fb.f[blockIdx.z]) + (long)(fb.smap ? fb.smap[lpl] : lpl) * nxpp * ny;
@@ -671,7 +717,7 @@ __global__ __launch_bounds__(kFT) void k_fft_y(FieldBatch fb, int nxpp, int nxh,
             fft_lds_g<true>(buf, ny, plan, CB, tw, DIR);
     };
     for (int t = threadIdx.x; t < ny; t += blockDim.x) {
-        tw[t] = twY[t];
+        tw[t] = toc<C>(twY[t]);
         kys[t] = fb.kw ? fb.kw[t] : 0.;
     }
     __syncthreads();
@@ -701,7 +747,7 @@ __global__ __launch_bounds__(kFT) void k_fft_y(FieldBatch fb, int nxpp, int nxh,
                 const double ky = kys[j];
                 z = make_double2(-ky * z.y, ky * z.x);
             }
-            buf[idx] = z;
+            buf[idx] = toc<C>(z);
         }
     };
     // forward with a second input P: + i ky P-hat (the y part of the flux
@@ -717,7 +763,7 @@ __global__ __launch_bounds__(kFT) void k_fft_y(FieldBatch fb, int nxpp, int nxh,
             if (!P2 && idx >= nval) continue;
             int j = rowof(idx), c = colof(idx, j), ii = ii0 + c;
             if (ii >= nxh) continue;
-            double2 z = buf[idx];
+            double2 z = tod(buf[idx]);
             if (DIR < 0) {
                 z.x = scaley * z.x;
                 z.y = scaley * z.y;
@@ -747,7 +793,7 @@ __global__ __launch_bounds__(kFT) void k_fft_y(FieldBatch fb, int nxpp, int nxh,
             const int idx = threadIdx.x + it * (int)blockDim.x;
             double2 z = make_double2(0., 0.);
             if (idx < nval) {
-                z = buf[idx];
+                z = tod(buf[idx]);
                 z.x = scaley * z.x;
                 z.y = scaley * z.y;
             }
@@ -786,8 +832,10 @@ static void fft_x_launch(const Geo &g, const FieldBatch &b, int nk, int dir, hip
     const int R = 8 * g.ft / g.M; // rows per block (fft_threads_ok: R >= 2)
     int nrows = g.ny * nk;
     dim3 grid((nrows + R - 1) / R, b.n);
-    size_t sh = sizeof(double2) * ((size_t)8 * g.ft + g.M + R + g.M + 1) + sizeof(double) * (g.M + 1 + 4 * R) +
-                sizeof(long) * 2 * R;
+    using C = std::conditional_t<std::is_same<TI, float>::value || std::is_same<TO, float>::value, float2, double2>;
+    const size_t mt = sizeof(C) == 8 ? (size_t)((g.M + 1) & ~1) : (size_t)g.M; // twiddles, padded to 16 bytes
+    size_t sh = sizeof(C) * ((size_t)8 * g.ft + mt) + sizeof(double2) * ((size_t)R + g.M + 1) +
+                sizeof(double) * (g.M + 1 + 4 * R) + sizeof(long) * 2 * R;
     McFuse none;
     if (dir < 0) {
         // fields with a flux product share the tile with it (H rows of
@@ -850,7 +898,9 @@ static void fft_y_dispatch(const Geo &g, const FieldBatch &b, int nk, int dir, h
 {
     const int CB = 8 * g.ft / g.ny; // columns per block (fft_threads_ok: CB >= 1)
     dim3 grid((g.nxh + CB - 1) / CB, nk, b.n);
-    size_t sh = sizeof(double2) * ((size_t)CB * g.ny + g.ny) + sizeof(double) * g.ny;
+    // the mixed path's tile and twiddles are FP32
+    const size_t cs = prec == kPrecF32 ? sizeof(float2) : sizeof(double2);
+    size_t sh = cs * ((size_t)CB * g.ny + g.ny) + sizeof(double) * g.ny;
     // mixed path: spectral FP64 <-> FP32 intermediate
     if (dir < 0 && prec == kPrecF32)
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fft_y<-1, float, double, P2>), grid, dim3(g.ft), sh, st, b, g.nxpp,
