@@ -1573,65 +1573,71 @@ __global__ __launch_bounds__(256) void k_tile(TileArgs A, long ncplx_plane, long
 // The same per (mode, subdomain) with the GLL order fixed at compile time:
 // lane = one mode, its re/im columns in registers (16-byte loads of each
 // row), the Fourier filter with one sincos, the Legendre tables in LDS.
+// One column (the re or the im part of a mode) per lane, consecutive lanes
+// consecutive columns, so that a wavefront reads and writes whole rows; the
+// Fourier filter needs both parts of a value and takes the partner lane's
+// with a lane exchange (lanes 2i and 2i+1 hold re and im of one mode).  The
+// Legendre filter is applied one output row at a time from the inputs in
+// registers (the sums in the order of legfilter_mat).
 template <int NL, int MODE>
-__global__ __launch_bounds__(256, 2) void k_tile_t(TileArgs A, long nmode, long nxppy,
-                                                const double *__restrict__ fcf, const double *__restrict__ FL)
+__global__ __launch_bounds__(256) void k_tile_t(TileArgs A, long nxppy, const double *__restrict__ fcf,
+                                                const double *__restrict__ FL)
 {
     __shared__ double FLs[NL * NL];
     for (int e = threadIdx.x; e < NL * NL; e += blockDim.x) FLs[e] = FL[e];
     __syncthreads();
-    const long m = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long col = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const int ks = blockIdx.y, fz = blockIdx.z;
-    if (m >= nmode) return;
+    if (col >= nxppy) return; // nxppy is even: both lanes of a mode stay or leave together
+    const bool im = col & 1;
     const double *const *Sf = A.S[fz], *const *Ff = A.F[fz];
     double *outf = A.out[fz];
-    const long b0 = (long)ks * NL * nxppy + 2 * m;
-    double xr[NL], xi[NL];
+    const long b0 = (long)ks * NL * nxppy + col;
+    double x[NL];
     if (MODE == 1) {
 #pragma unroll
         for (int t = 0; t < NL; t++) {
             const long o = b0 + (long)t * nxppy;
-            const double2 s0 = *reinterpret_cast<const double2 *>(Sf[0] + o);
-            const double2 s1 = *reinterpret_cast<const double2 *>(Sf[1] + o);
-            const double2 s2 = *reinterpret_cast<const double2 *>(Sf[2] + o);
-            const double2 f0 = *reinterpret_cast<const double2 *>(Ff[0] + o);
-            const double2 f1 = *reinterpret_cast<const double2 *>(Ff[1] + o);
-            const double2 f2 = *reinterpret_cast<const double2 *>(Ff[2] + o);
-            const double sr = A.f1 * s0.x + A.f2 * s1.x + A.f3 * s2.x;
-            const double si = A.f1 * s0.y + A.f2 * s1.y + A.f3 * s2.y;
-            xr[t] = A.dt * sr + A.b1 * f0.x + A.b2 * f1.x + A.b3 * f2.x;
-            xi[t] = A.dt * si + A.b1 * f0.y + A.b2 * f1.y + A.b3 * f2.y;
+            const double s0 = Sf[0][o], s1 = Sf[1][o], s2 = Sf[2][o];
+            const double f0 = Ff[0][o], f1 = Ff[1][o], f2 = Ff[2][o];
+            const double sr = A.f1 * s0 + A.f2 * s1 + A.f3 * s2;
+            x[t] = A.dt * sr + A.b1 * f0 + A.b2 * f1 + A.b3 * f2;
         }
     } else {
 #pragma unroll
-        for (int t = 0; t < NL; t++) {
-            const double2 v = *reinterpret_cast<const double2 *>(outf + b0 + (long)t * nxppy);
-            xr[t] = v.x;
-            xi[t] = v.y;
-        }
+        for (int t = 0; t < NL; t++) x[t] = outf[b0 + (long)t * nxppy];
     }
     if (A.ifourier) { // Fourier magnitude/phase filter, filtering.f:852-901
-        const double sigf = fcf[m];
+        const double sigf = fcf[col >> 1];
 #pragma unroll
-        for (int t = 0; t < NL; t++) fourier_filter_value(xr[t], xi[t], sigf);
+        for (int t = 0; t < NL; t++) {
+            const double other = __shfl_xor(x[t], 1, 64);
+            double fr = im ? other : x[t], fi = im ? x[t] : other;
+            fourier_filter_value(fr, fi, sigf);
+            x[t] = im ? fi : fr;
+        }
     }
-    if (A.ilegen) legfilter_mat2<NL>(xr, xi, FLs); // filtering.f:905-956, the two sums as one matrix
 #pragma unroll
-    for (int t = 0; t < NL; t++)
-        *reinterpret_cast<double2 *>(outf + b0 + (long)t * nxppy) = make_double2(xr[t], xi[t]);
+    for (int q = 0; q < NL; q++) {
+        double o = x[q];
+        if (A.ilegen) { // filtering.f:905-956, the two sums as one matrix
+            o = 0.;
+#pragma unroll
+            for (int t = 0; t < NL; t++) o = o + FLs[t * NL + q] * x[t];
+        }
+        outf[b0 + (long)q * nxppy] = o;
+    }
 }
 
 static void run_tile(const Geo &g, const TileArgs &A, int nf, hipStream_t st)
 {
     long ncp = (long)g.nxhp * g.ny;
-    dim3 gt((unsigned)((ncp + 255) / 256), g.nsubd, nf);
-#define TT(NL)                                                                                           \
-    if (A.mode == 1)                                                                                     \
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_tile_t<NL, 1>), gt, dim3(256), 0, st, A, ncp, g.nxppy, g.fcf, \
-                           g.FL);                                                                        \
-    else                                                                                                 \
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_tile_t<NL, 0>), gt, dim3(256), 0, st, A, ncp, g.nxppy, g.fcf, \
-                           g.FL)
+    dim3 gt((unsigned)((g.nxppy + 255) / 256), g.nsubd, nf);
+#define TT(NL)                                                                                                 \
+    if (A.mode == 1)                                                                                           \
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_tile_t<NL, 1>), gt, dim3(256), 0, st, A, g.nxppy, g.fcf, g.FL); \
+    else                                                                                                       \
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_tile_t<NL, 0>), gt, dim3(256), 0, st, A, g.nxppy, g.fcf, g.FL)
     if (nl_specialised(g.nzloc)) {
         switch (g.nzloc) {
             IW_NL_SWITCH(TT)

@@ -30,7 +30,7 @@ namespace iw {
 //                   place, alpha_s and a_{s-1} to per-column scratch;
 //   k_spike_couple: one thread per column: the two scalar recurrences, c_{s-1}
 //                   and d_s written over alpha_s and a_{s-1};
-//   k_spike_finish: one lane per (mode, subdomain), re and im together:
+//   k_spike_finish: one lane per (column, subdomain):
 //                   x_s = x0_s - c_{s-1} k_s - d_s h_s, then the Legendre
 //                   filter and the mean-u damping that calcuvwt applies to
 //                   every solution (filter_legenpen).
@@ -327,9 +327,11 @@ __global__ __launch_bounds__(256) void k_spike_couple(double *al, double *au, co
 
 // The coupling correction and the Legendre filter of every subdomain of the
 // four solutions (filter_legenpen, filtering.f:729-808) with the mean-u
-// damping (calcuvwt.f:343-361), in place on the native layout: lane = the re
-// and im columns of one mode in one subdomain (consecutive lanes read
-// consecutive 16-byte pairs of every row), the filter matrix in LDS.
+// damping (calcuvwt.f:343-361), in place on the native layout: lane = one
+// column (re or im of a mode) in one subdomain, consecutive lanes consecutive
+// columns, the filter matrix in LDS; the filter is applied one output row at
+// a time from the inputs in registers (the sums in the order of
+// legfilter_mat).
 struct FinishArgs {
     double *f[4];
     const int *modesys;
@@ -345,47 +347,34 @@ __global__ __launch_bounds__(256) void k_spike_finish(FinishArgs A)
 {
     __shared__ double FLs[NL * NL];
     stage_fl<NL>(FLs, A.FL);
-    const long m = (long)blockIdx.x * blockDim.x + threadIdx.x; // the re/im column pair of one mode
+    const long col = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const int s = blockIdx.y, fld = blockIdx.z, nsub = A.nsub;
     const long st = A.nxppy;
-    if (m >= st / 2) return;
+    if (col >= st) return;
+    const long m = col >> 1;
     if (fld == 2 && m == 0) return;
     const int sys = A.modesys[4 * m + (fld < 2 ? 1 : fld == 2 ? 2 : 3)];
     const double2 *kh = reinterpret_cast<const double2 *>(A.SPK + ((long)sys * nsub + s) * spk_stride(NL));
-    const long o = ((long)fld * nsub + s) * st + 2 * m;
-    const double2 c = *reinterpret_cast<const double2 *>(A.al + o);
-    const double2 d = *reinterpret_cast<const double2 *>(A.au + o);
-    double *fs = A.f[fld] + 2 * m + (long)s * NL * st;
-    double xr[NL], xi[NL];
+    const long o = ((long)fld * nsub + s) * st + col;
+    const double c = A.al[o], d = A.au[o];
+    double *fs = A.f[fld] + col + (long)s * NL * st;
+    double x[NL];
 #pragma unroll
     for (int k = 0; k < NL; k++) {
-        const double2 q = *reinterpret_cast<const double2 *>(fs + (long)k * st);
         const double2 v = kh[k];
-        xr[k] = q.x - c.x * v.x - d.x * v.y;
-        xi[k] = q.y - c.y * v.x - d.y * v.y;
+        x[k] = fs[(long)k * st] - c * v.x - d * v.y;
     }
-    // one output row at a time, stored at once: the inputs stay in
-    // registers, one accumulator pair (the sums run in the order of
-    // legfilter_mat2, so the results are the same)
     const bool damp = fld == 0 && m == 0;
 #pragma unroll
     for (int q = 0; q < NL; q++) {
-        double orr = xr[q], oi = xi[q];
+        double r = x[q];
         if (A.cpen) {
-            orr = 0.;
-            oi = 0.;
+            r = 0.;
 #pragma unroll
-            for (int t = 0; t < NL; t++) {
-                const double cf = FLs[t * NL + q];
-                orr = orr + cf * xr[t];
-                oi = oi + cf * xi[t];
-            }
+            for (int t = 0; t < NL; t++) r = r + FLs[t * NL + q] * x[t];
         }
-        if (damp) {
-            orr = A.damp[s * NL + q] * orr;
-            oi = A.damp[s * NL + q] * oi;
-        }
-        *reinterpret_cast<double2 *>(fs + (long)q * st) = make_double2(orr, oi);
+        if (damp) r = A.damp[s * NL + q] * r;
+        fs[(long)q * st] = r;
     }
 }
 
@@ -404,7 +393,7 @@ void launch_spike_finish(const Geo &g, const SysTables &S, double *const *uvwt, 
     A.nxppy = g.nxppy;
     A.nsub = g.nsubd;
     A.cpen = cpen;
-    dim3 grid((unsigned)((g.nxppy / 2 + 255) / 256), g.nsubd, 4);
+    dim3 grid((unsigned)((g.nxppy + 255) / 256), g.nsubd, 4);
 #define FN(NL) hipLaunchKernelGGL(k_spike_finish<NL>, grid, dim3(256), 0, st, A)
     switch (g.nzloc) {
         IW_NL_SWITCH(FN)
