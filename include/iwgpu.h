@@ -260,6 +260,22 @@ int iw_ensemble_set_lanes(iw_ensemble *e, int lanes);
 int iw_horfft(iw_solver *s, double *b, int is, int nk);
 /* derv.f:212    subroutine dpdz(a,b) */
 int iw_dpdz(iw_solver *s, const double *a, double *b);
+/* penalty.f:9, 162-235  subroutine penalty_setup(epsilon) and the boundary
+ * part of penalty_main: the /penaltyparams/ alpha, beta, gamma, delta, tau1,
+ * tau2 of every subdomain (nsubd values each) for (epsilon, ifbcbot,
+ * ifbctop); boundary types 0 Dirichlet, 1 Neumann, 2 Robin (parambctype.h) */
+int iw_penalty_params(iw_solver *s, double eps, int ifbcbot, int ifbctop, double *alpha, double *beta,
+                      double *gamma, double *delta, double *tau1, double *tau2);
+/* penalty.f:57   subroutine param_set(ifbc,alpha0,beta0,tau0,epsilon,ybar,ks) */
+int iw_param_set(iw_solver *s, int ifbc, double eps, double ybar, double *alpha0, double *beta0, double *tau0);
+/* penalty.f:255  subroutine helmv_penalty(fvr,fvi,epsilon,rhsinr,rhsini,
+ * bcrtop,bcitop,bcrbot,bcibot,ifbctop,ifbcbot,ifuncreal) for one column pair
+ * (with the boundary-row terms of matrix_elmsetup, penalty.f:583-604, and the
+ * banded LU of bandlu_solve, band_lu.f:15-52, on the device); rhsr, rhsi as
+ * penalty_main hands them over (already times -epsilon), the penalty
+ * parameters those of iw_penalty_params; ifbcbot, ifbctop 0 or 1 */
+int iw_penalty_solve(iw_solver *s, double eps, int ifbcbot, int ifbctop, double bcrbot, double bcibot,
+                     double bcrtop, double bcitop, const double *rhsr, const double *rhsi, double *fr, double *fi);
 /* filtering.f:811  subroutine filter_3dlegenfourier(fin3d,ifourier,ilegen,iout) */
 int iw_filter_3dlegenfourier(iw_solver *s, double *f, int ifourier, int ilegen);
 /* penalty.f:714  subroutine interface_avg3d(fin,ic) */

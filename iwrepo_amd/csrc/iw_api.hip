@@ -1036,6 +1036,8 @@ extern "C" int iw_destroy(iw_solver *s)
     hipFree(s->sys.AB);
     hipFree(s->sys.FB);
     hipFree(s->sys.PB);
+    hipFree(s->pen_buf);
+    hipFree(s->pen_piv);
     hipFree(s->sys.SPK);
     hipFree(s->sys.SPB);
     hipFree(s->sys.spk_al);
@@ -2607,6 +2609,106 @@ extern "C" int iw_horfft(iw_solver *s, double *b, int is, int nk)
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(b, s->hbuf, bytes, hipMemcpyDeviceToHost, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
+    return 0;
+}
+
+// penalty_setup + the boundary part of penalty_main (penalty.f:9-55,
+// 183-235): alpha, beta, gamma, delta, tau1, tau2 of every subdomain
+extern "C" int iw_penalty_params(iw_solver *s, double eps, int ifbcbot, int ifbctop, double *alpha, double *beta,
+                                 double *gamma, double *delta, double *tau1, double *tau2)
+{
+    if (!s || !alpha || !beta || !gamma || !delta || !tau1 || !tau2) return fail("iw_penalty_params: null argument");
+    if (ifbcbot < 0 || ifbcbot > 2 || ifbctop < 0 || ifbctop > 2)
+        return fail("iw_penalty_params: boundary types must be 0 (Dirichlet), 1 (Neumann) or 2 (Robin)");
+    const Pen p = penalty_params(s->P, eps, ifbcbot, ifbctop);
+    for (int k = 0; k < s->P.nsubd; k++) {
+        alpha[k] = p.alpha[k];
+        beta[k] = p.beta[k];
+        gamma[k] = p.gamma[k];
+        delta[k] = p.delta[k];
+        tau1[k] = p.tau1[k];
+        tau2[k] = p.tau2[k];
+    }
+    return 0;
+}
+
+// param_set (penalty.f:57-157): the parameters of one boundary of subdomain
+// height ybar
+extern "C" int iw_param_set(iw_solver *s, int ifbc, double eps, double ybar, double *alpha0, double *beta0,
+                            double *tau0)
+{
+    if (!s || !alpha0 || !beta0 || !tau0) return fail("iw_param_set: null argument");
+    if (ifbc < 0 || ifbc > 2) return fail("iw_param_set: ifbc must be 0, 1 or 2");
+    const double omega = 2. / ((s->P.nzloc + 1) * (s->P.nzloc + 2));
+    param_set(s->P, omega, ifbc, eps, ybar, *alpha0, *beta0, *tau0);
+    return 0;
+}
+
+// helmv_penalty (penalty.f:255-400) on one column pair: the boundary rows of
+// the right-hand side take tau (bc) (Dirichlet) or -tau (bc) eps / +tau (bc)
+// eps (Neumann, bottom / top) as in penalty.f:583-604, the matrix of
+// matrix_elmsetup is assembled and factored with row interchanges
+// (SGBTF2 order, k_factor) and both columns solved (SGBTRS) on the device.
+// rhsr, rhsi are the right-hand sides as helmv_penalty receives them
+// (penalty_main has multiplied them by -eps); the penalty parameters are
+// those of penalty_setup + penalty_main for (eps, ifbcbot, ifbctop).
+extern "C" int iw_penalty_solve(iw_solver *s, double eps, int ifbcbot, int ifbctop, double bcrbot, double bcibot,
+                                double bcrtop, double bcitop, const double *rhsr, const double *rhsi, double *fr,
+                                double *fi)
+{
+    if (!s || !rhsr || !rhsi || !fr || !fi) return fail("iw_penalty_solve: null argument");
+    if (ifbcbot < 0 || ifbcbot > 1 || ifbctop < 0 || ifbctop > 1)
+        return fail("iw_penalty_solve: boundary types must be 0 (Dirichlet) or 1 (Neumann)");
+    if (!(eps > 0.)) return fail("iw_penalty_solve: epsilon must be positive");
+    const iw_params &P = s->P;
+    const int nz = s->g.nz, ns = P.nsubd, ldab = s->sys.ldab;
+    const Pen p = penalty_params(P, eps, ifbcbot, ifbctop);
+    std::vector<double> coef((size_t)12 * ns), info(4), x((size_t)2 * nz);
+    assembly_coef(P, p, eps, coef.data());
+    info[0] = eps;
+    info[1] = p.tau1[0];
+    info[2] = p.tau2[ns - 1];
+    info[3] = ifbcbot * 4 + ifbctop;
+    for (int k = 0; k < nz; k++) {
+        x[k] = rhsr[k];
+        x[nz + k] = rhsi[k];
+    }
+    if (ifbcbot == BC_DIR) {
+        x[0] = rhsr[0] + p.tau1[0] * bcrbot;
+        x[nz] = rhsi[0] + p.tau1[0] * bcibot;
+    } else {
+        x[0] = rhsr[0] - p.tau1[0] * bcrbot * eps;
+        x[nz] = rhsi[0] - p.tau1[0] * bcibot * eps;
+    }
+    if (ifbctop == BC_DIR) {
+        x[nz - 1] = rhsr[nz - 1] + p.tau2[ns - 1] * bcrtop;
+        x[2 * nz - 1] = rhsi[nz - 1] + p.tau2[ns - 1] * bcitop;
+    } else {
+        x[nz - 1] = rhsr[nz - 1] + p.tau2[ns - 1] * bcrtop * eps;
+        x[2 * nz - 1] = rhsi[nz - 1] + p.tau2[ns - 1] * bcitop * eps;
+    }
+    if (!s->pen_buf) {
+        // coef, sysinfo, band, solution | pivots
+        const size_t nd = (size_t)12 * ns + 4 + (size_t)ldab * nz + 2 * (size_t)nz;
+        HIPCHK(hipMalloc((void **)&s->pen_buf, sizeof(double) * nd));
+        HIPCHK(hipMalloc((void **)&s->pen_piv, sizeof(int) * (size_t)nz));
+        s->dev_bytes += (long long)(sizeof(double) * nd + sizeof(int) * nz);
+    }
+    double *dcoef = s->pen_buf, *dinfo = dcoef + 12 * ns, *dab = dinfo + 4, *dx = dab + (size_t)ldab * nz;
+    HIPCHK(hipMemcpyAsync(dcoef, coef.data(), sizeof(double) * coef.size(), hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipMemcpyAsync(dinfo, info.data(), sizeof(double) * 4, hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipMemcpyAsync(dx, x.data(), sizeof(double) * x.size(), hipMemcpyHostToDevice, s->stream));
+    // band positions the factorisation does not write read as zero
+    HIPCHK(hipMemsetAsync(dab, 0, sizeof(double) * (size_t)ldab * nz, s->stream));
+    launch_factor_one(s->g, ldab, dcoef, dinfo, dab, s->pen_piv, s->stream);
+    launch_band_solve1(s->g, ldab, dab, s->pen_piv, dx, s->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(x.data(), dx, sizeof(double) * x.size(), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    for (int k = 0; k < nz; k++) {
+        fr[k] = x[k];
+        fi[k] = x[nz + k];
+    }
     return 0;
 }
 

@@ -228,6 +228,10 @@ struct iw_solver {
     double *e_ring = nullptr, *l_ring = nullptr;
     int e_cap = 0, l_cap = 0;
     std::vector<double> e_hist, l_hist;
+    // the single-column penalty solve (iw_penalty_solve): coefficients,
+    // system table, band and solution; pivots (allocated on first use)
+    double *pen_buf = nullptr;
+    int *pen_piv = nullptr;
     // profiling
     bool profiling = false;
     iw::Prof prof;
@@ -316,6 +320,11 @@ void launch_iface(const Geo &g, FieldBatch b, int *count, hipStream_t st);
 void launch_project(const Geo &g, FieldBatch b, hipStream_t st);
 void launch_factor(const Geo &g, const SysTables &S, int first, int count, hipStream_t st);
 void launch_compact(const Geo &g, const SysTables &S, int first, int count, hipStream_t st);
+// one penalty system (coef: 12 nsubd, sysinfo: 4) factored into AB/ipiv,
+// and the re and im columns x[0..nz), x[nz..2nz) solved with it
+void launch_factor_one(const Geo &g, int ldab, const double *coef, double *sysinfo, double *AB, int *ipiv,
+                       hipStream_t st);
+void launch_band_solve1(const Geo &g, int ldab, const double *AB, const int *ipiv, double *x, hipStream_t st);
 // Packed factors of the z systems, one block of nl rows per subdomain.  The
 // penalty matrix couples a subdomain only to the interface rows of its
 // neighbours, so below the diagonal the L column of local row r holds

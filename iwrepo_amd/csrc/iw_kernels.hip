@@ -1872,6 +1872,49 @@ void launch_factor(const Geo &g, const SysTables &S, int first, int count, hipSt
                        S.coef, g.D, g.D2, S.AB, S.ipiv, S.sysinfo);
 }
 
+// One system, for the single-column solve of iw_penalty_solve: the factor
+// of k_factor from its own coefficient and system tables (index 0), then
+// SGBTRS with the interchanges (bandludcmp.f:1020-1036, STBSV
+// ludcmp.f:4698-5046) of the re and im columns x[0..n), x[n..2n) on one
+// lane (a test-size call; the step's solves are the batched sweeps).
+void launch_factor_one(const Geo &g, int ldab, const double *coef, double *sysinfo, double *AB, int *ipiv,
+                       hipStream_t st)
+{
+    size_t sh = sizeof(double) * ((size_t)kWR * kWS + 2 * g.nzloc * g.nzloc + 12 * g.nsubd);
+    hipLaunchKernelGGL(k_factor, dim3(1), dim3(64), sh, st, 0, g.nz, g.nzloc, g.nsubd, ldab, coef, g.D, g.D2, AB,
+                       ipiv, sysinfo);
+}
+
+__global__ void k_band_solve1(const double *__restrict__ AB, const int *__restrict__ ipiv, double *x, int n, int ldab,
+                              int kl)
+{
+    if (threadIdx.x >= 2) return;
+    double *b = x + (long)threadIdx.x * n;
+    const int kv = 2 * kl;
+    for (int j = 0; j < n - 1; j++) {
+        const int lm = kl < n - 1 - j ? kl : n - 1 - j;
+        const int jp = ipiv[j];
+        if (jp != j) {
+            const double t = b[j];
+            b[j] = b[jp];
+            b[jp] = t;
+        }
+        const double y = b[j];
+        for (int t = 1; t <= lm; t++) b[j + t] = b[j + t] + AB[(long)j * ldab + kv + t] * (-y);
+    }
+    for (int j = n - 1; j >= 0; j--) {
+        if (b[j] != 0.) b[j] = b[j] / AB[(long)j * ldab + kv];
+        const double xj = b[j];
+        const int lm = kv < j ? kv : j;
+        for (int t = 1; t <= lm; t++) b[j - t] = b[j - t] - xj * AB[(long)j * ldab + kv - t];
+    }
+}
+
+void launch_band_solve1(const Geo &g, int ldab, const double *AB, const int *ipiv, double *x, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_band_solve1, dim3(1), dim3(64), 0, st, AB, ipiv, x, g.nz, ldab, g.nzloc);
+}
+
 // Packed copies of the structurally nonzero factor entries (tri_* layout in
 // SysTables::FB / PB): one block per system; an LDS map from a packed
 // position of a subdomain block to its (local row, t) lets consecutive

@@ -100,6 +100,30 @@ c     layout of iw_params (include/iwgpu.h)
           type(c_ptr), value :: h
           real(c_double) :: a(*), b(*)
         end function
+        integer(c_int) function iw_penalty_params(h, eps, ibot, itop,
+     &       al, be, ga, de, t1, t2) bind(c)
+          import
+          type(c_ptr), value :: h
+          real(c_double), value :: eps
+          integer(c_int), value :: ibot, itop
+          real(c_double) :: al(*), be(*), ga(*), de(*), t1(*), t2(*)
+        end function
+        integer(c_int) function iw_param_set(h, ifbc, eps, ybar,
+     &       a0, b0, t0) bind(c)
+          import
+          type(c_ptr), value :: h
+          integer(c_int), value :: ifbc
+          real(c_double), value :: eps, ybar
+          real(c_double) :: a0, b0, t0
+        end function
+        integer(c_int) function iw_penalty_solve(h, eps, ibot, itop,
+     &       brb, bib, brt, bit, rr, ri, fr, fi) bind(c)
+          import
+          type(c_ptr), value :: h
+          real(c_double), value :: eps, brb, bib, brt, bit
+          integer(c_int), value :: ibot, itop
+          real(c_double) :: rr(*), ri(*), fr(*), fi(*)
+        end function
         integer(c_int) function iw_filter_3dlegenfourier(h, f, ifo, ile)
      &       bind(c)
           import
@@ -726,6 +750,119 @@ c-----------------------------------------------------------------------
 c-----------------------------------------------------------------------
 c     multistep.f:7-89 variable-step AB3/BDF3 coefficients
 c-----------------------------------------------------------------------
+c-----------------------------------------------------------------------
+c     penalty.f:9   subroutine penalty_setup(epsilon): the penalty
+c     parameters of the interior subdomains into /penaltyparams/ and
+c     omega into /comomega/ (comparampen.h)
+c-----------------------------------------------------------------------
+      subroutine penalty_setup(epsilon)
+      use iwshim
+      include 'dim.h'
+      include 'comparampen.h'
+      real(c_double) :: pa(nsubd), pb(nsubd), pg(nsubd), pd(nsubd)
+      real(c_double) :: pt1(nsubd), pt2(nsubd)
+      integer ks
+      call iwshim_init
+      call iwshim_check(iw_penalty_params(iwh, epsilon, 2, 2, pa, pb,
+     &     pg, pd, pt1, pt2), 'penalty_setup')
+      omega = 2./((nzloc+1)*(nzloc+2))
+      do ks = 2, nsubd - 1
+        alpha(ks) = pa(ks)
+        beta(ks) = pb(ks)
+        tau1(ks) = pt1(ks)
+        gamma(ks) = pg(ks)
+        delta(ks) = pd(ks)
+        tau2(ks) = pt2(ks)
+      enddo
+      end
+
+c-----------------------------------------------------------------------
+c     penalty.f:57  subroutine param_set(ifbc,alpha0,beta0,tau0,
+c     epsilon,ybar,ks)
+c-----------------------------------------------------------------------
+      subroutine param_set(ifbc, alpha0, beta0, tau0, epsilon, ybar, ks)
+      use iwshim
+      call iwshim_init
+      call iwshim_check(iw_param_set(iwh, ifbc, epsilon, ybar, alpha0,
+     &     beta0, tau0), 'param_set')
+      end
+
+c-----------------------------------------------------------------------
+c     penalty.f:162 subroutine penalty_main(epsilon,ifuncreal,bcrbot,
+c     bcibot,bcrtop,bcitop,ifbcbot,ifbctop,rhsinr,rhsini,fr,fi): the
+c     right-hand sides times -epsilon in place, the boundary parameters
+c     into /penaltyparams/, then helmv_penalty
+c-----------------------------------------------------------------------
+      subroutine penalty_main(epsilon, ifuncreal, bcrbot, bcibot,
+     &     bcrtop, bcitop, ifbcbot, ifbctop, rhsinr, rhsini, fr, fi)
+      use iwshim
+      include 'dim.h'
+      include 'comparampen.h'
+      dimension fr(nz), fi(nz), rhsinr(nz), rhsini(nz)
+      real(c_double) :: pa(nsubd), pb(nsubd), pg(nsubd), pd(nsubd)
+      real(c_double) :: pt1(nsubd), pt2(nsubd)
+      call iwshim_init
+      do k = 1, nz
+        rhsinr(k) = -rhsinr(k)*epsilon
+        rhsini(k) = -rhsini(k)*epsilon
+      enddo
+      call iwshim_check(iw_penalty_params(iwh, epsilon, ifbcbot,
+     &     ifbctop, pa, pb, pg, pd, pt1, pt2), 'penalty_main')
+      alpha(1) = pa(1)
+      beta(1) = pb(1)
+      tau1(1) = pt1(1)
+      if (nsubd .gt. 1) then
+        gamma(1) = pg(1)
+        delta(1) = pd(1)
+        tau2(1) = pt2(1)
+      endif
+      gamma(nsubd) = pg(nsubd)
+      delta(nsubd) = pd(nsubd)
+      tau2(nsubd) = pt2(nsubd)
+      if (nsubd .gt. 1) then
+        alpha(nsubd) = pa(nsubd)
+        beta(nsubd) = pb(nsubd)
+        tau1(nsubd) = pt1(nsubd)
+      endif
+      call helmv_penalty(fr, fi, epsilon, rhsinr, rhsini, bcrtop,
+     &     bcitop, bcrbot, bcibot, ifbctop, ifbcbot, ifuncreal)
+      end
+
+c-----------------------------------------------------------------------
+c     penalty.f:255 subroutine helmv_penalty(fvr,fvi,epsilon,rhsinr,
+c     rhsini,bcrtop,bcitop,bcrbot,bcibot,ifbctop,ifbcbot,ifuncreal):
+c     assembly (matrix_elmsetup), banded LU and solve (bandlu_solve) on
+c     the device.  With ifuncreal = 1 only the real column is solved and
+c     fvi keeps the right-hand side with its boundary terms
+c     (band_lu.f:41-43, penalty.f:583-604).
+c-----------------------------------------------------------------------
+      subroutine helmv_penalty(fvr, fvi, epsilon, rhsinr, rhsini,
+     &     bcrtop, bcitop, bcrbot, bcibot, ifbctop, ifbcbot, ifuncreal)
+      use iwshim
+      include 'dim.h'
+      include 'comparampen.h'
+      dimension fvr(nz), fvi(nz), rhsinr(nz), rhsini(nz)
+      call iwshim_init
+      call iwshim_check(iw_penalty_solve(iwh, epsilon, ifbcbot,
+     &     ifbctop, bcrbot, bcibot, bcrtop, bcitop, rhsinr, rhsini,
+     &     fvr, fvi), 'helmv_penalty')
+      if (ifuncreal .eq. 1) then
+        do k = 1, nz
+          fvi(k) = rhsini(k)
+        enddo
+        if (ifbcbot .eq. 0) then
+          fvi(1) = rhsini(1) + tau1(1)*bcibot
+        else
+          fvi(1) = rhsini(1) - tau1(1)*bcibot*epsilon
+        endif
+        if (ifbctop .eq. 0) then
+          fvi(nz) = rhsini(nz) + tau2(nsubd)*bcitop
+        else
+          fvi(nz) = rhsini(nz) + tau2(nsubd)*bcitop*epsilon
+        endif
+      endif
+      end
+
       real function abvar1(dt1, dt2, dt3)
       use iwshim
       abvar1 = iw_abvar1(dt1, dt2, dt3)

@@ -17,7 +17,8 @@ LIBPATH = os.path.join(PKG, "libiwgpu.so")
 EXPORTS = [
     "iw_params_size", "iw_last_error", "iw_create", "iw_destroy", "iw_set_state", "iw_get_state",
     "iw_get_pressbc", "iw_set_pressbc", "iw_set_dt", "iw_set_coeffs", "iw_get_coeffs", "iw_set_initial", "iw_run", "iw_time",
-    "iw_sync", "iw_horfft", "iw_dpdz", "iw_filter_3dlegenfourier", "iw_interface_avg3d",
+    "iw_sync", "iw_horfft", "iw_dpdz", "iw_penalty_params", "iw_param_set", "iw_penalty_solve",
+    "iw_filter_3dlegenfourier", "iw_interface_avg3d",
     "iw_convec", "iw_calcuvwt", "iw_filtering", "iw_get_sgs",
     "iw_save_restart", "iw_load_restart", "iw_write_restart_file", "iw_read_restart_file", "iw_abvar1", "iw_abvar2", "iw_abvar3", "iw_bdfvarv",
     "iw_bdfvar1", "iw_bdfvar2", "iw_bdfvar3", "iw_set_profiling", "iw_kernel_stats",
@@ -66,6 +67,9 @@ def lib():
     L.iw_sync.argtypes = [P]
     L.iw_horfft.argtypes = [P, P, I, I]
     L.iw_dpdz.argtypes = [P, P, P]
+    L.iw_penalty_params.argtypes = [P, D, I, I, P, P, P, P, P, P]
+    L.iw_param_set.argtypes = [P, I, D, D, P, P, P]
+    L.iw_penalty_solve.argtypes = [P, D, I, I, D, D, D, D, P, P, P, P]
     L.iw_filter_3dlegenfourier.argtypes = [P, P, I, I]
     L.iw_interface_avg3d.argtypes = [P, P, ctypes.POINTER(I)]
     L.iw_convec.argtypes = [P, I, I]
@@ -221,6 +225,28 @@ class Solver:
         b = np.empty_like(a)
         _check(self.L.iw_dpdz(self.h, _ptr(a), _ptr(b)))
         return b
+
+    def penalty_params(self, eps, ifbcbot, ifbctop):
+        """alpha, beta, gamma, delta, tau1, tau2 per subdomain (penalty.f:9-235)."""
+        out = [np.zeros(self.P.nsubd) for _ in range(6)]
+        _check(self.L.iw_penalty_params(self.h, eps, ifbcbot, ifbctop, *[_ptr(a) for a in out]))
+        return out
+
+    def param_set(self, ifbc, eps, ybar):
+        """(alpha0, beta0, tau0) of one boundary (penalty.f:57-157)."""
+        v = [ctypes.c_double() for _ in range(3)]
+        _check(self.L.iw_param_set(self.h, ifbc, eps, ybar, *[ctypes.byref(x) for x in v]))
+        return tuple(x.value for x in v)
+
+    def penalty_solve(self, eps, ifbcbot, ifbctop, rhsr, rhsi, bc=(0.0, 0.0, 0.0, 0.0)):
+        """helmv_penalty on one column pair (rhs already times -eps), on the
+        device; bc = (bcrbot, bcibot, bcrtop, bcitop)."""
+        rr = np.ascontiguousarray(rhsr, dtype=np.float64)
+        ri = np.ascontiguousarray(rhsi, dtype=np.float64)
+        fr, fi = np.empty_like(rr), np.empty_like(ri)
+        _check(self.L.iw_penalty_solve(self.h, eps, ifbcbot, ifbctop, bc[0], bc[1], bc[2], bc[3], _ptr(rr),
+                                       _ptr(ri), _ptr(fr), _ptr(fi)))
+        return fr, fi
 
     def filter3d(self, f, ifourier=1, ilegen=1):
         f = self._arr(f).copy()

@@ -44,6 +44,9 @@ c            after filtering(1) when isgs=1, energies to energy.txt
 c    mode=7  kinetic and potential energy of the IC fields (physical),
 c            the z integral by the reference's integrcol (derv.f:479),
 c            to kepe.txt
+c    mode=8  penalty_setup + penalty_main (testhelm.f:60-100) of IC
+c            columns for four (epsilon, boundary types and values),
+c            solutions and scaled right-hand sides to the diagnostics file
 c  istart=1 (namelist &inputs) in mode 0 restarts from start.dat with
 c  restart.f and the first-step coefficients of main.f:619-634, 703-734
 c  and 818-825; iwrite=1 writes restart_out.dat with output.f at the end.
@@ -71,6 +74,7 @@ c=======================================================================
       logical cpenspecfilter, ldtadapt
       character*256 icfile, outfile, diagfile
       dimension wrk(nxpp,ny,nz)
+      dimension rhr8(nz), rhi8(nz), fr8(nz), fi8(nz)
 
       common /Dump/ Tdump,Tx
       common /anim/ianim,ianimc
@@ -232,6 +236,58 @@ c-Initial condition: 4 physical fields (nxpp,ny,nz), pads ignored
         call interface_avg3d(w,icw)
         call interface_avg3d(temp,ict)
         write(12,'(4(1x,I10))') icu,icv,icw,ict
+        goto 900
+      endif
+      if (mode.eq.8) then
+c-testhelm.f:60-100: penalty_setup, then penalty_main of the column
+c-i=icase+1, j=1 of u (real) and v (imaginary), for four (epsilon,
+c-boundary types, boundary values); solution and the right-hand sides
+c-as penalty_main leaves them to the diagnostics file
+        do icase = 1, 4
+          if (icase.eq.1) then
+            eps8 = 0.01
+            ifbb8 = 0
+            ifbt8 = 0
+            bcrb8 = 0.
+            bcib8 = 0.
+            bcrt8 = 0.
+            bcit8 = 0.
+          elseif (icase.eq.2) then
+            eps8 = 1./17.
+            ifbb8 = 1
+            ifbt8 = 1
+            bcrb8 = 0.3
+            bcib8 = -0.2
+            bcrt8 = 0.1
+            bcit8 = 0.05
+          elseif (icase.eq.3) then
+            eps8 = 1./(17.+11./6./(xnu*dt))
+            ifbb8 = 0
+            ifbt8 = 1
+            bcrb8 = 1.
+            bcib8 = 0.5
+            bcrt8 = -0.25
+            bcit8 = 0.75
+          else
+            eps8 = 1./(5.+11./6./(xkappa*dt))
+            ifbb8 = 1
+            ifbt8 = 0
+            bcrb8 = 0.
+            bcib8 = 0.
+            bcrt8 = 0.2
+            bcit8 = -0.1
+          endif
+          do k = 1, nz
+            rhr8(k) = u(icase+1,1,k)
+            rhi8(k) = v(icase+1,1,k)
+          enddo
+          call penalty_setup(eps8)
+          call penalty_main(eps8,0,bcrb8,bcib8,bcrt8,bcit8,ifbb8,ifbt8,
+     >                      rhr8,rhi8,fr8,fi8)
+          do k = 1, nz
+            write(12,'(4(1x,E25.17))') fr8(k),fi8(k),rhr8(k),rhi8(k)
+          enddo
+        enddo
         goto 900
       endif
       if (mode.eq.7) then

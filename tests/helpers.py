@@ -201,3 +201,28 @@ def kepe_closed_form(P, x, z, amp=(0.3, 0.2, 0.25, 0.15)):
     want = [ck * span * a * a / 2 * L ** 5 / 5, ck * span * d * d / 2 * L ** 3 / 3 if ny > 1 else 0.0,
             ck * span * b * b / 2 * L, cp * span * e * e * ((1 + L) ** 3 - 1) / 3]
     return [u, v, w, t], np.array(want)
+
+
+def penalty_cases(P):
+    """The (epsilon, bottom type, top type, (bcrbot, bcibot, bcrtop, bcitop))
+    of harness mode 8 (oracle/ref_harness.f): testhelm's epsilon with
+    homogeneous Dirichlet ends, a pressure-like Neumann system, and
+    velocity- and density-like systems with non-zero boundary values.
+    Types: 0 Dirichlet, 1 Neumann (parambctype.h)."""
+    return [(0.01, 0, 0, (0.0, 0.0, 0.0, 0.0)),
+            (1.0 / 17.0, 1, 1, (0.3, -0.2, 0.1, 0.05)),
+            (1.0 / (17.0 + 11.0 / 6.0 / (P.xnu * P.dt)), 0, 1, (1.0, 0.5, -0.25, 0.75)),
+            (1.0 / (5.0 + 11.0 / 6.0 / (P.xkappa * P.dt)), 1, 0, (0.0, 0.0, 0.2, -0.1))]
+
+
+def penalty_columns(flds, icase):
+    """The right-hand sides of mode 8, case icase (0-based): u and v at
+    i = icase + 1, j = 0 of the initial fields."""
+    return flds[0][:, 0, icase + 1].copy(), flds[1][:, 0, icase + 1].copy()
+
+
+def parse_penalty_diag(diag, nz):
+    """Mode 8's diagnostics: per case nz lines of fr, fi and the right-hand
+    sides as penalty_main leaves them (times -epsilon)."""
+    rows = np.array([[float(x) for x in l.split()] for l in diag if l.strip()])
+    return rows.reshape(4, nz, 4)

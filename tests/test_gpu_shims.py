@@ -117,6 +117,56 @@ def test_fortran_shim_routines(rt, oracle_mod, name):
             assert [int(x) for x in ds[0].split()] == counts
 
 
+@pytest.mark.parametrize("name", ["small", "c1", "c2"])
+def test_penalty_solve_on_the_device_matches_oracle(rt, oracle_mod, name):
+    """iw_penalty_solve (helmv_penalty on one column pair: boundary-row terms,
+    assembly, banded LU with interchanges and SGBTRS on the device) and
+    iw_penalty_params against the oracle's penalty solve, for the four
+    (epsilon, boundary types and values) of harness mode 8 on the z grids of
+    the small case, C1 and 256x128x(16x25), within the parity bar (1e-10):
+    the Neumann-Neumann system of the 400-point grid is ill-conditioned
+    enough that the rounding of the two factorisations (FMA on the device,
+    none in the oracle) shows at 7e-12."""
+    from helpers import penalty_cases
+    P, _ = case(name)
+    o, g = oracle_mod.Oracle(P), rt.Solver(P)
+    rng = np.random.default_rng(5)
+    for c, (eps, bot, top, bc) in enumerate(penalty_cases(P)):
+        print("penalty solve", name, "case", c, flush=True)
+        rr, ri = rng.standard_normal(P.nz), rng.standard_normal(P.nz)
+        fr, fi = o.penalty_solve(eps, bot, top, rr, ri, bc)
+        gr, gi = g.penalty_solve(eps, bot, top, (-rr) * eps, (-ri) * eps, bc)
+        assert rel(gr, fr) < 1e-10 and rel(gi, fi) < 1e-10, c
+        al, be, ga, de, t1, t2 = g.penalty_params(eps, bot, top)
+        assert be[0] == (0.0 if bot == 0 else 1.0) and de[-1] == (0.0 if top == 0 else 1.0)
+        assert all(x > 0 for x in t1) and all(x > 0 for x in t2)
+    with pytest.raises(rt.IWError, match="boundary types"):
+        g.penalty_solve(0.01, 2, 0, np.zeros(P.nz), np.zeros(P.nz))
+    g.close()
+
+
+def test_fortran_penalty_shims_as_testhelm_calls_them(rt, oracle_mod):
+    """penalty_setup and penalty_main through the Fortran shims in the shim
+    driver (no reference object linked), called as testhelm.f:60-100 calls
+    them (harness mode 8), against the oracle: solutions within 1e-12, the
+    right-hand sides left times -epsilon in place."""
+    from helpers import parse_penalty_diag, penalty_cases, penalty_columns
+    _need(oracle_mod, "small")
+    for name in ("small", "c1ns"):
+        if not oracle_mod.shim_available(name) and name != "small":
+            continue
+        P, flds = case(name)
+        o = oracle_mod.Oracle(P)
+        print("penalty shims", name, flush=True)
+        _, _, diag = oracle_mod.run_ref(name, P, flds, nsteps=0, mode=8, exe="shimdrv")
+        got = parse_penalty_diag(diag, P.nz)
+        for c, (eps, bot, top, bc) in enumerate(penalty_cases(P)):
+            rr, ri = penalty_columns(flds, c)
+            fr, fi = o.penalty_solve(eps, bot, top, rr, ri, bc)
+            assert rel(got[c, :, 0], fr) < 1e-12 and rel(got[c, :, 1], fi) < 1e-12, (name, c)
+            assert np.array_equal(got[c, :, 2], -rr * eps) and np.array_equal(got[c, :, 3], -ri * eps)
+
+
 def test_fortran_shim_filtering(rt, oracle_mod):
     """filtering(isgscalc) through the shim: the fields and the /comsgs/
     tensors against the oracle."""

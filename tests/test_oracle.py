@@ -265,6 +265,28 @@ def test_routines_match_compiled_reference(oracle_mod, name):
         assert rel(f, st[q]) < 1e-13
 
 
+@pytest.mark.parametrize("name", ["small", "c1ns"])
+def test_penalty_main_matches_compiled_reference(oracle_mod, name):
+    """penalty_setup + penalty_main (testhelm.f:60-100, harness mode 8) of
+    the compiled reference for four (epsilon, boundary types and values),
+    non-homogeneous Dirichlet and Neumann ends included, against the
+    oracle's penalty solve; the right-hand sides come back times -epsilon,
+    as penalty_main leaves them."""
+    from helpers import parse_penalty_diag, penalty_cases, penalty_columns
+    if not oracle_mod.ref_available(name):
+        pytest.skip("reference build %s not present (needs /root/reference)" % name)
+    P, flds = case(name)
+    o = oracle_mod.Oracle(P)
+    _, _, diag = oracle_mod.run_ref(name, P, flds, nsteps=0, mode=8)
+    got = parse_penalty_diag(diag, P.nz)
+    for c, (eps, bot, top, bc) in enumerate(penalty_cases(P)):
+        rr, ri = penalty_columns(flds, c)
+        fr, fi = o.penalty_solve(eps, bot, top, rr, ri, bc)
+        assert rel(got[c, :, 0], fr) < 1e-12, (c, "re")
+        assert rel(got[c, :, 1], fi) < 1e-12, (c, "im")
+        assert np.array_equal(got[c, :, 2], -rr * eps) and np.array_equal(got[c, :, 3], -ri * eps)
+
+
 @pytest.mark.parametrize("name,fackcbv", [("small", 0.0), ("small", 0.5), ("c1", 0.0)])
 def test_estimation_filter_matches_compiled_reference(oracle_mod, name, fackcbv):
     """filtering(isgscalc) (filtering.f:8-93): modecut, the x/y top-hat
