@@ -582,13 +582,22 @@ static int ensure_factors(iw_solver *s)
     if (s->factored_bdfv == s->bdfv && s->factored_dt == s->P.dt && s->p_factored) return 0;
     int first, count;
     if (build_systems(s, !s->p_factored, first, count)) return -1;
-    if (!s->sys.tri_bad) HIPCHK(hipMalloc((void **)&s->sys.tri_bad, 2 * sizeof(int)));
+    // [0], [1]: pattern flags of the pressure and the other factors; [2]:
+    // the u/v and w factor blocks below the top subdomain differ
+    if (!s->sys.tri_bad) HIPCHK(hipMalloc((void **)&s->sys.tri_bad, 3 * sizeof(int)));
     // pattern flags of this packing: [0] pressure factors (only when they are
     // packed again), [1] the u/v, w and density factors
     launch_zero_int(first == 0 ? s->sys.tri_bad : s->sys.tri_bad + 1, first == 0 ? 2 : 1, s->stream);
     launch_factor(s->g, s->sys, first, count, s->stream);
     launch_compact(s->g, s->sys, first, count, s->stream);
     launch_spike_pre(s->g, s->sys, first, count, s->stream);
+    // u, v and w below the top subdomain over one factor (no-slip bottom:
+    // the u/v and w systems differ only in the top boundary condition)
+    const bool uvw = s->P.bc_uv_bottom == 1 && s->P.nsubd > 1;
+    if (uvw) {
+        launch_zero_int(s->sys.tri_bad + 2, 1, s->stream);
+        launch_share_check(s->g, s->sys, s->sys.tri_bad + 2, s->stream);
+    }
     HIPCHK(hipGetLastError());
     // the u/v, w and density systems go to the packed-factor sweeps (lane =
     // one column); if any of them pivoted, or a factor breaks the packed
@@ -600,8 +609,8 @@ static int ensure_factors(iw_solver *s)
     HIPCHK(hipStreamSynchronize(s->stream));
     std::vector<int> ms(4 * (size_t)nmode);
     HIPCHK(hipMemcpy(ms.data(), S.modesys, sizeof(int) * ms.size(), hipMemcpyDeviceToHost));
-    int tri_bad[2] = {0, 0};
-    HIPCHK(hipMemcpy(tri_bad, S.tri_bad, 2 * sizeof(int), hipMemcpyDeviceToHost));
+    int tri_bad[3] = {0, 0, 1};
+    HIPCHK(hipMemcpy(tri_bad, S.tri_bad, (uvw ? 3 : 2) * sizeof(int), hipMemcpyDeviceToHost));
     // the packed sweeps need every factor of their kind in the tri_*
     // pattern; the band solvers on AB take over otherwise, for the pressure
     // and the other systems separately
@@ -668,6 +677,58 @@ static int ensure_factors(iw_solver *s)
         std::vector<int> guv, gwt;
         if (groups(8, true, guv) || groups(4, false, gwt))
             return fail("internal: grouped lane tables do not cover the plane");
+        if (uvw) {
+            // groups of 12 lanes: u, v of +ky and -ky, then w of +ky and
+            // -ky (re, im each), five groups per 64 lanes; field in bits
+            // 28-29, idle in bit 30; w(0,0) is not solved (idle)
+            std::vector<int> out, blk;
+            auto entf = [&](int jy, int ii, int f, std::vector<int> &v) {
+                const int m = jy * g.nxhp + ii;
+                for (int c = 0; c < 2; c++)
+                    v.push_back((f == 2 && m == 0) ? ((2 * m + c) | (1 << 30)) : ((2 * m + c) | (f << 28)));
+            };
+            auto add = [&](std::vector<int> &v) {
+                const int first = v[0] & ((1 << 28) - 1);
+                while ((int)v.size() < 12) v.push_back(first | (1 << 30));
+                if (blk.size() + 12 > 64) {
+                    while (blk.size() < 64) blk.push_back(-1);
+                    out.insert(out.end(), blk.begin(), blk.end());
+                    blk.clear();
+                }
+                blk.insert(blk.end(), v.begin(), v.end());
+            };
+            auto pair = [&](int p, int q, int ii) {
+                std::vector<int> v;
+                entf(p, ii, 0, v);
+                entf(p, ii, 1, v);
+                if (q >= 0) {
+                    entf(q, ii, 0, v);
+                    entf(q, ii, 1, v);
+                }
+                entf(p, ii, 2, v);
+                if (q >= 0) entf(q, ii, 2, v);
+                add(v);
+            };
+            for (int p = 1; p < (g.ny + 1) / 2; p++)
+                if (g.ny - p != p)
+                    for (int ii = 0; ii < g.nxh; ii++) pair(p, g.ny - p, ii);
+            for (int jy : single)
+                for (int ii = 0; ii < g.nxh; ii++) pair(jy, -1, ii);
+            for (int jy : single) pair(jy, -1, g.nxh);
+            for (int p = 1; p < (g.ny + 1) / 2; p++)
+                if (g.ny - p != p) pair(p, g.ny - p, g.nxh);
+            if (!blk.empty()) {
+                while (blk.size() < 64) blk.push_back(-1);
+                out.insert(out.end(), blk.begin(), blk.end());
+            }
+            long live = 0;
+            for (int e : out) live += (e >= 0 && !((e >> 30) & 1)) ? 1 : 0;
+            if (live != 3 * g.nxppy - 2) return fail("internal: u/v/w lane table does not cover the plane");
+            out = xcd_place(out, ms, 1);
+            HIPCHK(hipMalloc((void **)&S.guvw_tasks, sizeof(int) * out.size()));
+            HIPCHK(hipMemcpy(S.guvw_tasks, out.data(), sizeof(int) * out.size(), hipMemcpyHostToDevice));
+            S.guvw_ntasks = (int)out.size();
+        }
         guv = xcd_place(guv, ms, 1);
         gwt = xcd_place(gwt, ms, 2);
         HIPCHK(hipMalloc((void **)&S.guv_tasks, sizeof(int) * guv.size()));
@@ -682,6 +743,7 @@ static int ensure_factors(iw_solver *s)
         HIPCHK(hipMalloc((void **)&S.spk_au, sizeof(double) * nal));
         s->dev_bytes += (long long)(2 * sizeof(double) * nal);
     }
+    S.share_uvw = uvw && tri_bad[2] == 0 && S.np_ok && S.guvw_tasks;
     s->factored_bdfv = s->bdfv;
     s->factored_dt = s->P.dt;
     s->p_factored = true;
@@ -1053,6 +1115,7 @@ extern "C" int iw_destroy(iw_solver *s)
     hipFree(s->sys.order_v);
     hipFree(s->sys.guv_tasks);
     hipFree(s->sys.gwt_tasks);
+    hipFree(s->sys.guvw_tasks);
     hipEventDestroy(s->ev0);
     hipEventDestroy(s->ev1);
     hipStreamDestroy(s->stream);

@@ -113,6 +113,12 @@ struct SysTables {
     int guv_ntasks = 0;
     int *gwt_tasks = nullptr;
     int gwt_ntasks = 0;
+    // u, v and w of a (kx, +-ky) pair in one group of 12 lanes (no-slip
+    // bottom only), and whether their factor blocks below the top subdomain
+    // are the same bits for the u/v and the w system (checked at each packing)
+    int *guvw_tasks = nullptr;
+    int guvw_ntasks = 0;
+    bool share_uvw = false;
     bool np_ok = false;
     size_t cap_sys = 0;
 };
@@ -403,6 +409,8 @@ void launch_solve_spike(const Geo &g, const iw_params &P, const SysTables &S, do
 void launch_spike_finish(const Geo &g, const SysTables &S, double *const *uvwt, int cpen, hipStream_t st);
 // its per-(system, subdomain) vectors, after launch_compact
 void launch_spike_pre(const Geo &g, const SysTables &S, int first, int count, hipStream_t st);
+// sets *flag when the u/v and w factor blocks below the top subdomain differ
+void launch_share_check(const Geo &g, const SysTables &S, int *flag, hipStream_t st);
 // part 0: right-hand side, 1: solve, 2: filter + projection
 void launch_pressure_nat(const Geo &g, const iw_params &P, const SysTables &S, double *const *uvwt, double *phi,
                          const double *wnl, const double *wviscrot, double dt, int cpen, int part,

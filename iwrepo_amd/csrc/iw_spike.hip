@@ -133,6 +133,7 @@ struct SpikeArgs {
     double den[4];
     long nxppy;
     int nsub, fld_base;
+    int ks0; // subdomain of grid row 0
 };
 
 // rows of the 2 nl sweep steps of one block: forward rows 0..nl-1, then
@@ -143,11 +144,16 @@ constexpr int spike_np(int i)
     return i < NL ? tri_npf(NL, i) : tri_npb(NL, 2 * NL - 1 - i, false);
 }
 
-template <int NL, int G, int DP>
+// UVW: the u, v and w columns of a (kx, +-ky) pair in one group of G = 12
+// lanes (lane table entry: column | field << 28 | idle << 30), all solved
+// with the u/v system's factor blocks; used for the subdomains below the
+// top one when those blocks of the u/v and w systems are bit for bit the same
+// (a no-slip bottom: the two differ only in the top boundary condition)
+template <int NL, int G, int DP, bool UVW = false>
 __global__ __launch_bounds__(64) void k_spike(SpikeArgs A)
 {
     static_assert(DP >= 2 && DP <= 2 * NL, "a row is staged one step before use, from a slot filled earlier");
-    constexpr int NG = 64 / G, NS = 2 * NL;
+    constexpr int NG = (64 + G - 1) / G, NS = 2 * NL;
     constexpr int SF = tri_sf(NL), SB = tri_sb(NL, false);
     constexpr int NPM = tri_npf(NL, 0) > tri_npb(NL, NL - 1, false) ? tri_npf(NL, 0) : tri_npb(NL, NL - 1, false);
     constexpr int PP = (NPM + G - 1) / G; // pieces per lane
@@ -155,14 +161,16 @@ __global__ __launch_bounds__(64) void k_spike(SpikeArgs A)
     const int lane = threadIdx.x, grp = lane / G, rk = lane % G;
     const int e = A.tasks[blockIdx.x * 64 + lane];
     if (e < 0) return; // the whole group is empty
-    const bool idle = (e >> 29) & 1;
+    const bool idle = UVW ? (e >> 30) & 1 : (e >> 29) & 1;
     const int col = e & ((1 << 28) - 1);
     const int m = col >> 1;
-    const int fld = A.fld_base + blockIdx.z + ((e >> 28) & 1);
-    if (fld == 2 && m == 0) return; // w(0,0) = 0 (calcuvwt.f:496-503); its group holds only this mode
-    const int sys = A.modesys[4 * m + (fld < 2 ? 1 : fld == 2 ? 2 : 3)];
+    const int fld = UVW ? (e >> 28) & 3 : A.fld_base + blockIdx.z + ((e >> 28) & 1);
+    // w(0,0) = 0 (calcuvwt.f:496-503): its group holds only this mode (in
+    // the u/v/w groups its lanes are idle)
+    if (!UVW && fld == 2 && m == 0) return;
+    const int sys = A.modesys[4 * m + (UVW || fld < 2 ? 1 : fld == 2 ? 2 : 3)];
     const double eps = A.sysinfo[4 * sys], den = A.den[fld];
-    const int ks = blockIdx.y, nsub = A.nsub;
+    const int ks = A.ks0 + blockIdx.y, nsub = A.nsub;
     const long st = A.nxppy;
     const double2 *fw = reinterpret_cast<const double2 *>(A.FB) + (long)sys * nsub * (SF + SB) + (long)ks * SF;
     const double2 *bw = reinterpret_cast<const double2 *>(A.FB) + (long)sys * nsub * (SF + SB) + (long)nsub * SF +
@@ -250,11 +258,12 @@ __global__ __launch_bounds__(64) void k_spike(SpikeArgs A)
 // ring depth of k_spike (rows fetched ahead): measured at nl = 25, 8 for the
 // u/v blocks (G = 8, two pieces per lane) and 5 for w and rho (G = 4, four
 // pieces per lane); deeper rings or a register cap that allows a third
-// wavefront per SIMD spill and run slower
+// wavefront per SIMD spill and run slower.  The u/v/w groups (G = 12, two
+// pieces per lane) take the depth of the u/v ones.
 template <int NL, int G>
 constexpr int spike_dp()
 {
-    return (G == 8 ? 8 : 5) < 2 * NL ? (G == 8 ? 8 : 5) : 2 * NL;
+    return (G >= 8 ? 8 : 5) < 2 * NL ? (G >= 8 ? 8 : 5) : 2 * NL;
 }
 
 void launch_solve_spike(const Geo &g, const iw_params &P, const SysTables &S, double *const *uvwt, double dt,
@@ -271,6 +280,7 @@ void launch_solve_spike(const Geo &g, const iw_params &P, const SysTables &S, do
     A.den[3] = dt * P.xkappa;
     A.nxppy = g.nxppy;
     A.nsub = g.nsubd;
+    A.ks0 = 0;
     SpikeArgs B = A;
     A.tasks = S.guv_tasks;
     A.fld_base = 0;
@@ -278,6 +288,28 @@ void launch_solve_spike(const Geo &g, const iw_params &P, const SysTables &S, do
     B.fld_base = 2;
     // subdomain-major grid: the blocks of one subdomain run together, and
     // block x keeps the XCD (x mod 8) the task placement chose for it
+    if (S.share_uvw) {
+        // below the top subdomain u, v and w in the u/v/w groups over the
+        // u/v factor blocks; the top subdomain of u/v and of w and every
+        // subdomain of rho in their own groups
+        SpikeArgs C = A, W = B, T = B;
+        C.tasks = S.guvw_tasks;
+        A.ks0 = W.ks0 = g.nsubd - 1;
+        T.fld_base = 3;
+        dim3 gc(S.guvw_ntasks / 64, g.nsubd - 1, 1), ga(S.guv_ntasks / 64, 1, 1), gw(S.gwt_ntasks / 64, 1, 1),
+            gt(S.gwt_ntasks / 64, g.nsubd, 1);
+#define GS3(NL)                                                                                                  \
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_spike<NL, 12, spike_dp<NL, 12>(), true>), gc, dim3(64), 0, st, C);   \
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_spike<NL, 8, spike_dp<NL, 8>()>), ga, dim3(64), 0, st, A);           \
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_spike<NL, 4, spike_dp<NL, 4>()>), gw, dim3(64), 0, st, W);           \
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_spike<NL, 4, spike_dp<NL, 4>()>), gt, dim3(64), 0, st, T)
+        switch (g.nzloc) {
+            IW_NL_SWITCH(GS3)
+        default: break;
+        }
+#undef GS3
+        return;
+    }
     dim3 ga(S.guv_ntasks / 64, g.nsubd, 1), gb(S.gwt_ntasks / 64, g.nsubd, 2);
 #define GS(NL)                                                                                          \
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_spike<NL, 8, spike_dp<NL, 8>()>), ga, dim3(64), 0, st, A); \
@@ -287,6 +319,35 @@ void launch_solve_spike(const Geo &g, const iw_params &P, const SysTables &S, do
     default: break;
     }
 #undef GS
+}
+
+// Bit-for-bit comparison of the u/v and w factor blocks below the top
+// subdomain, forward and backward, for every mode (one thread per packed
+// double2); any difference sets *flag.  Run after each packing when the
+// bottom condition is the same for both systems.
+__global__ __launch_bounds__(256) void k_share_check(const double *__restrict__ FB, const int *__restrict__ modesys,
+                                                     long nmode, int nsub, int sf, int sb, int *flag)
+{
+    const long per = (long)(nsub - 1) * (sf + sb);
+    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= (nmode - 1) * per) return;
+    const long m = 1 + id / per, q = id - (m - 1) * per;
+    const long off = q < (long)(nsub - 1) * sf ? q : (long)nsub * sf + (q - (long)(nsub - 1) * sf);
+    const long su = modesys[4 * m + 1], sw = modesys[4 * m + 2];
+    const double2 a = reinterpret_cast<const double2 *>(FB)[su * nsub * (sf + sb) + off];
+    const double2 b = reinterpret_cast<const double2 *>(FB)[sw * nsub * (sf + sb) + off];
+    if (__double_as_longlong(a.x) != __double_as_longlong(b.x) || __double_as_longlong(a.y) != __double_as_longlong(b.y))
+        atomicOr(flag, 1);
+}
+
+void launch_share_check(const Geo &g, const SysTables &S, int *flag, hipStream_t st)
+{
+    const long nmode = (long)g.nxhp * g.ny;
+    const int sf = tri_sf(g.nzloc), sb = tri_sb(g.nzloc, false);
+    const long n = (nmode - 1) * (long)(g.nsubd - 1) * (sf + sb);
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_share_check, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, S.FB, S.modesys, nmode,
+                       g.nsubd, sf, sb, flag);
 }
 
 // the two scalar recurrences per column (one thread per column and field);
