@@ -429,6 +429,8 @@ static int build_systems(iw_solver *s, bool refactor_p, int &first, int &count)
     }
     const int nsys = (int)eps_of.size();
     SysTables &S = s->sys;
+    S.nsys_w = 0; // Dirichlet at both ends: the w systems
+    for (int q = np; q < nsys; q++) S.nsys_w += (bb[q] == BC_DIR && bt[q] == BC_DIR) ? 1 : 0;
     if ((size_t)nsys > S.cap_sys) {
         if (S.AB) hipFree(S.AB);
         if (S.FB) hipFree(S.FB);
@@ -1706,9 +1708,14 @@ static int calcuvwt_dev(iw_solver *s, int cpen)
             // + each distinct no-interchange factor's structural nonzeros
             // once (per row nl - r below the diagonal, r + 2 on and above it)
             const double fnp = (double)(s->P.nzloc + 2) * s->g.nz * 8.;
-            // + the coupling scalars written (4 fields x nsubd x columns x 2)
+            // + the coupling scalars written (4 fields x nsubd x columns x 2);
+            // with the u/v/w groups the w factors below the top subdomain are
+            // the u/v ones, read once
             const double cpl = 2. * 4. * s->g.nsubd * s->g.nxppy * 8.;
-            Scope sc(s, "solve_nat", 4 * 16. * nt + (s->sys.nsys - s->sys.np) * fnp + (s->seq_sweeps ? 0. : cpl));
+            const double nfac = (s->sys.nsys - s->sys.np) -
+                                ((!s->seq_sweeps && s->sys.share_uvw)
+                                     ? s->sys.nsys_w * (s->P.nsubd - 1.) / s->P.nsubd : 0.);
+            Scope sc(s, "solve_nat", 4 * 16. * nt + nfac * fnp + (s->seq_sweeps ? 0. : cpl));
             if (s->seq_sweeps)
                 launch_solve_grp(s->g, s->P, s->sys, uvwt, s->P.dt, s->stream);
             else

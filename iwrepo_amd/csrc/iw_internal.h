@@ -119,6 +119,7 @@ struct SysTables {
     int *guvw_tasks = nullptr;
     int guvw_ntasks = 0;
     bool share_uvw = false;
+    int nsys_w = 0; // systems with Dirichlet ends (w), among the nsys
     bool np_ok = false;
     size_t cap_sys = 0;
 };
