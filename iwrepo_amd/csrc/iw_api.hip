@@ -534,48 +534,24 @@ static int build_systems(iw_solver *s, bool refactor_p, int &first, int &count)
     return 0;
 }
 
-// Place the 64-lane blocks of a lane table on the XCDs.  Modes with equal
-// k^2 share a system; many such pairs are not in one block (k^2 = (i alpha)^2
-// + (j beta)^2 coincides for unrelated (i, j)), so two blocks read the same
-// factor rows.  Blocks are dealt to the 8 XCDs (block b runs on XCD b mod 8)
-// so that blocks sharing systems land on the same XCD, where the second
-// read of a row can come from that XCD's L2 while the first block streams
-// it.  Greedy: each block goes to the XCD, not yet full, whose blocks
-// already use most of its systems (ties: the least loaded).  Lanes within a
-// block keep their order, so row accesses stay coalesced.  slot: the
-// modesys column that names the block's systems; padding blocks are -1.
-static std::vector<int> xcd_place(const std::vector<int> &tasks, const std::vector<int> &ms, int slot)
+// Place the 64-lane blocks of a lane table on the XCDs: XCD x gets the
+// x-th eighth of the table, in order (block b of the grid runs on XCD b mod 8,
+// the hardware's round-robin dispatch).  Consecutive blocks hold consecutive
+// kx of the same ky rows, so the cache lines of a state row that two blocks
+// share are fetched into one XCD's L2 by blocks that run at the same time;
+// dealing blocks to XCDs by the systems they share instead (modes with equal
+// k^2) splits those lines between XCDs and measured slower.  Padding blocks
+// are -1.
+static std::vector<int> xcd_place(const std::vector<int> &tasks)
 {
     const int nx = 8, nblk = (int)(tasks.size() / 64);
     const int cap = (nblk + nx - 1) / nx;
-    std::vector<std::vector<int>> q(nx);
-    std::vector<std::set<int>> seen(nx);
-    for (int b = 0; b < nblk; b++) {
-        std::set<int> sys;
-        for (int l = 0; l < 64; l++) {
-            const int e = tasks[(size_t)b * 64 + l];
-            if (e < 0) continue;
-            const int m = (e & 0x0FFFFFFF) >> 1, id = ms[4 * (size_t)m + slot];
-            if (id >= 0) sys.insert(id);
-        }
-        int best = -1, bs = -1;
-        for (int x = 0; x < nx; x++) {
-            if ((int)q[x].size() >= cap) continue;
-            int sc = 0;
-            for (int id : sys) sc += (int)seen[x].count(id);
-            if (sc > bs || (sc == bs && q[x].size() < q[best].size())) {
-                bs = sc;
-                best = x;
-            }
-        }
-        q[best].push_back(b);
-        seen[best].insert(sys.begin(), sys.end());
-    }
     std::vector<int> out((size_t)cap * nx * 64, -1);
-    for (int x = 0; x < nx; x++)
-        for (size_t i = 0; i < q[x].size(); i++)
-            std::copy(tasks.begin() + (size_t)q[x][i] * 64, tasks.begin() + (size_t)q[x][i] * 64 + 64,
-                      out.begin() + ((size_t)i * nx + x) * 64);
+    for (int b = 0; b < nblk; b++) {
+        const int x = b / cap, i = b % cap;
+        std::copy(tasks.begin() + (size_t)b * 64, tasks.begin() + (size_t)b * 64 + 64,
+                  out.begin() + ((size_t)i * nx + x) * 64);
+    }
     return out;
 }
 
@@ -609,8 +585,6 @@ static int ensure_factors(iw_solver *s)
     std::vector<double> info((size_t)S.nsys * 4);
     HIPCHK(hipMemcpyAsync(info.data(), S.sysinfo, sizeof(double) * info.size(), hipMemcpyDeviceToHost, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
-    std::vector<int> ms(4 * (size_t)nmode);
-    HIPCHK(hipMemcpy(ms.data(), S.modesys, sizeof(int) * ms.size(), hipMemcpyDeviceToHost));
     int tri_bad[3] = {0, 0, 1};
     HIPCHK(hipMemcpy(tri_bad, S.tri_bad, (uvw ? 3 : 2) * sizeof(int), hipMemcpyDeviceToHost));
     // the packed sweeps need every factor of their kind in the tri_*
@@ -726,13 +700,13 @@ static int ensure_factors(iw_solver *s)
             long live = 0;
             for (int e : out) live += (e >= 0 && !((e >> 30) & 1)) ? 1 : 0;
             if (live != 3 * g.nxppy - 2) return fail("internal: u/v/w lane table does not cover the plane");
-            out = xcd_place(out, ms, 1);
+            out = xcd_place(out);
             HIPCHK(hipMalloc((void **)&S.guvw_tasks, sizeof(int) * out.size()));
             HIPCHK(hipMemcpy(S.guvw_tasks, out.data(), sizeof(int) * out.size(), hipMemcpyHostToDevice));
             S.guvw_ntasks = (int)out.size();
         }
-        guv = xcd_place(guv, ms, 1);
-        gwt = xcd_place(gwt, ms, 2);
+        guv = xcd_place(guv);
+        gwt = xcd_place(gwt);
         HIPCHK(hipMalloc((void **)&S.guv_tasks, sizeof(int) * guv.size()));
         HIPCHK(hipMemcpy(S.guv_tasks, guv.data(), sizeof(int) * guv.size(), hipMemcpyHostToDevice));
         S.guv_ntasks = (int)guv.size();

@@ -144,6 +144,7 @@ struct McFuse {
     double dt = 0, cx = 0, cy = 0; // dt/dx, dt/dy: the CFL numbers are |u| dt/dx ... to
                                    // within an ulp of the reference's (u dt)/dx
     double cny = 0;              // ny-1 for the Nyquist correction, 0: none
+    int fid0 = 0;                // field index (u, v, w, temp = 0..3) of the batch's first field
     // mean-wave u, w, rho (nxpp,nz) and the shear current (nz), or nullptr
     const double *umwv = nullptr, *wmwv = nullptr, *rhomwv = nullptr, *ush = nullptr;
 };

@@ -358,7 +358,10 @@ __device__ __forceinline__ void mc_take(bool mn, double x, int lx, double &b, in
     }
 }
 
-// AX: the batch's fields have flux products (forward only); MAPS: the batch
+// AX: forward, the batch's fields have flux products; inverse (P2 only), the
+// batch's fields have x gradients, and the tile holds R/2 rows of the field
+// and the same rows of i kx f, transformed by the same passes from one load
+// of the coefficients.  MAPS: the batch
 // has plane maps (else rows are addressed arithmetically, so no row address
 // waits on a table load and a thread's loads are all in flight together).
 // TI, TO: storage type of the inputs (f, ax) and of the outputs (g, dx, axo).
@@ -390,7 +393,7 @@ __global__ __launch_bounds__(kFT) void k_fft_x(FieldBatch fb, int nrows, int ny,
     double *mcv = kws + M + 1;
     const TI *base = reinterpret_cast<const TI *>(fb.f[blockIdx.y]);
     TO *obase = reinterpret_cast<TO *>(fb.g[blockIdx.y]);
-    const int row0 = blockIdx.x * R;
+    const int row0 = blockIdx.x * (AX ? R >> 1 : R); // first row of the block's field rows
     const int N = 2 * M;
     const int lny = P2 ? __builtin_ctz(ny) : 0;
     // value idx of the tile -> (row r, index k); row -> (plane, row j)
@@ -448,6 +451,31 @@ __global__ __launch_bounds__(kFT) void k_fft_x(FieldBatch fb, int nrows, int ny,
     // coefficients A_k + i B_k of a row -> the packed complex sequence; with
     // deriv the coefficients are first multiplied by i kx (the x gradient of
     // solver.f:86-103)
+    // coefficients a = A_k + i B_k, b = A_{M-k} + i B_{M-k} of row `row`
+    // (tile row r) -> element k of the packed sequence
+    auto pre_inverse = [&](double2 a, double2 b, int k, int r, int row, bool deriv) -> double2 {
+        if (!deriv && k == 0) nyq[r] = b; // kx-Nyquist coefficient of the row
+        if (deriv) {
+            const double ka = kws[k], kb = kws[M - k];
+            a = make_double2(-ka * a.y, ka * a.x);
+            b = make_double2(-kb * b.y, kb * b.x);
+        }
+        double ar, ai, br, bi;
+        if (k == 0) {
+            ar = a.x; ai = 0.; br = b.x; bi = 0.;
+        } else {
+            ar = a.x; ai = a.y;
+            br = b.x; bi = -b.y;
+        }
+        double er = ar + br, ei = ai + bi;
+        double dr = ar - br, di = ai - bi;
+        double2 w = twN[k];
+        double c = w.x, s = -w.y; // e^{+2 pi i k/N}
+        double orr = dr * c - di * s, oi = dr * s + di * c;
+        double2 Z = make_double2(er - oi, ei + orr);
+        if (row >= nrows) Z = make_double2(0., 0.);
+        return Z;
+    };
     auto load_inverse = [&](bool deriv) {
         // all of a thread's loads first (rows past the end read the last row
         // and are zeroed below), so they are in flight together
@@ -465,28 +493,7 @@ __global__ __launch_bounds__(kFT) void k_fft_x(FieldBatch fb, int nrows, int ny,
             const int idx = threadIdx.x + it * (int)blockDim.x;
             if (!P2 && idx >= nval) continue;
             const int r = rowof(idx), k = colof(idx, r), row = row0 + r;
-            double2 a = va[it], b = vb[it];
-            if (!deriv && k == 0) nyq[r] = b; // kx-Nyquist coefficient of the row
-            if (deriv) {
-                const double ka = kws[k], kb = kws[M - k];
-                a = make_double2(-ka * a.y, ka * a.x);
-                b = make_double2(-kb * b.y, kb * b.x);
-            }
-            double ar, ai, br, bi;
-            if (k == 0) {
-                ar = a.x; ai = 0.; br = b.x; bi = 0.;
-            } else {
-                ar = a.x; ai = a.y;
-                br = b.x; bi = -b.y;
-            }
-            double er = ar + br, ei = ai + bi;
-            double dr = ar - br, di = ai - bi;
-            double2 w = twN[k];
-            double c = w.x, s = -w.y; // e^{+2 pi i k/N}
-            double orr = dr * c - di * s, oi = dr * s + di * c;
-            double2 Z = make_double2(er - oi, ei + orr);
-            if (row >= nrows) Z = make_double2(0., 0.);
-            buf[lp(idx)] = toc<C>(Z);
+            buf[lp(idx)] = toc<C>(pre_inverse(va[it], vb[it], k, r, row, deriv));
         }
     };
     auto store_inverse = [&](TO *ob) {
@@ -503,7 +510,7 @@ __global__ __launch_bounds__(kFT) void k_fft_x(FieldBatch fb, int nrows, int ny,
             if (row < nrows) st2(ob + drow(row) + N, make_double2(0., 0.));
         }
     };
-    if (DIR > 0 && fb.dx[blockIdx.y]) {
+    if (DIR > 0 && !AX && fb.dx[blockIdx.y]) {
         // i kx f first: a transform in place overwrites its input rows
         load_inverse(true);
         __syncthreads();
@@ -598,17 +605,58 @@ __global__ __launch_bounds__(kFT) void k_fft_x(FieldBatch fb, int nrows, int ny,
             }
         }
     } else {
-        load_inverse(false);
-        __syncthreads();
-        fft_rows(R);
-        store_inverse(obase);
-        if (mf.part && blockIdx.y < 4) { // maxcalc reductions of u, v, w, temp
-            const int f = blockIdx.y, nz = planeof(nrows);
+        // field rows of the tile: the first 8 value slots of each thread, or
+        // the first 4 when the other 4 hold the rows' i kx f
+        constexpr int NIT = AX ? 4 : 8;
+        if (AX) {
+            TO *dxo = reinterpret_cast<TO *>(fb.dx[blockIdx.y]);
+            const int H = R >> 1;
+            const int half = 4 * (int)blockDim.x; // value slot idx + half: tile row r + H, same k
+            double2 va[4], vb[4];
+#pragma unroll
+            for (int it = 0; it < 4; it++) {
+                const int idx = threadIdx.x + it * (int)blockDim.x;
+                const int r = rowof(idx), k = colof(idx, r), row = row0 + r;
+                const TI *p = base + srow(row < nrows ? row : nrows - 1);
+                va[it] = ld2(p + 2 * k);
+                vb[it] = ld2(p + 2 * (M - k));
+            }
+#pragma unroll
+            for (int it = 0; it < 4; it++) {
+                const int idx = threadIdx.x + it * (int)blockDim.x;
+                const int r = rowof(idx), k = colof(idx, r), row = row0 + r;
+                buf[lp(idx)] = toc<C>(pre_inverse(va[it], vb[it], k, r, row, false));
+                buf[lp(idx + half)] = toc<C>(pre_inverse(va[it], vb[it], k, r + H, row, true));
+            }
+            __syncthreads();
+            fft_rows(R);
+#pragma unroll
+            for (int it = 0; it < 8; it++) {
+                const int idx = threadIdx.x + it * (int)blockDim.x;
+                const int rr = rowof(idx), m = colof(idx, rr);
+                const bool d = rr >= H;
+                const int row = row0 + (d ? rr - H : rr);
+                if (row >= nrows) continue;
+                st2((d ? dxo : obase) + drow(row) + 2 * m, tod(buf[lp(idx)]));
+            }
+            for (int r = threadIdx.x; r < R; r += blockDim.x) {
+                const bool d = r >= H;
+                const int row = row0 + (d ? r - H : r);
+                if (row < nrows) st2((d ? dxo : obase) + drow(row) + N, make_double2(0., 0.));
+            }
+        } else {
+            load_inverse(false);
+            __syncthreads();
+            fft_rows(R);
+            store_inverse(obase);
+        }
+        if (mf.part && mf.fid0 + (int)blockIdx.y < 4) { // maxcalc reductions of u, v, w, temp
+            const int f = mf.fid0 + blockIdx.y, nz = planeof(nrows);
             const double big = 1.79769313486231570e308;
             double b0 = big, b1 = -big, b2 = -big; // min, max of the value; max of the CFL number
             int l0 = -1, l1 = -1, l2 = -1;
 #pragma unroll
-            for (int it = 0; it < 8; it++) {
+            for (int it = 0; it < NIT; it++) {
                 const int idx = threadIdx.x + it * (int)blockDim.x;
                 if (!P2 && idx >= nval) continue;
                 const int r = rowof(idx), m = colof(idx, r), row = row0 + r;
@@ -670,14 +718,19 @@ __global__ __launch_bounds__(kFT) void k_fft_x(FieldBatch fb, int nrows, int ny,
                 }
                 // quantities: 0 umin 1 umax 2 vmin 3 vmax 4 wmin 5 wmax 6 tmin
                 // 7 tmax 8 cflx 9 cfly 10 cflz
-                double *p = mf.part + (size_t)blockIdx.x * (2 * kMcQ);
-                p[4 * f] = b0;
-                p[4 * f + 1] = (double)l0;
-                p[4 * f + 2] = b1;
-                p[4 * f + 3] = (double)l1;
-                if (f < 3) {
-                    p[2 * (8 + f)] = b2;
-                    p[2 * (8 + f) + 1] = (double)l2;
+                // partial slots per R/2 rows with P2 (the paired tiles' rows):
+                // a block of R rows fills two, the second with no candidate
+                const int slot = (!AX && P2) ? 2 * (int)blockIdx.x : (int)blockIdx.x;
+                for (int q = 0; q < ((!AX && P2) ? 2 : 1); q++) {
+                    double *p = mf.part + (size_t)(slot + q) * (2 * kMcQ);
+                    p[4 * f] = q ? big : b0;
+                    p[4 * f + 1] = q ? -1.0 : (double)l0;
+                    p[4 * f + 2] = q ? -big : b1;
+                    p[4 * f + 3] = q ? -1.0 : (double)l1;
+                    if (f < 3) {
+                        p[2 * (8 + f)] = q ? -big : b2;
+                        p[2 * (8 + f) + 1] = q ? -1.0 : (double)l2;
+                    }
                 }
             }
         }
@@ -862,8 +915,36 @@ static void fft_x_launch(const Geo &g, const FieldBatch &b, int nk, int dir, hip
             hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fft_x<-1, false, MAPS, TI, TO, P2>), dim3(grid.x, bp.n), dim3(g.ft),
                                sh, st, bp, nrows, g.ny, g.nxpp, g.M, g.logM, g.planM, R, g.twM, g.twN, g.scaley, none);
     } else {
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fft_x<1, false, MAPS, TI, TO, P2>), grid, dim3(g.ft), sh, st, b, nrows,
-                           g.ny, g.nxpp, g.M, g.logM, g.planM, R, g.twM, g.twN, g.scaley, mf ? *mf : none);
+        McFuse m = mf ? *mf : none;
+        // fields with an x gradient (a leading run of the batch) share their
+        // tiles with it (R/2 rows of each), the rest take whole tiles; with
+        // P2 the maxcalc partials come per R/2 rows either way (mc_blocks)
+        int nd = 0;
+        while (nd < b.n && b.dx[nd]) nd++;
+        bool lead = P2 && nd > 0;
+        for (int q = nd; q < b.n; q++) lead = lead && !b.dx[q];
+        if (lead) {
+            FieldBatch bd = b, br = b;
+            bd.n = nd;
+            br.n = b.n - nd;
+            for (int q = 0; q < br.n; q++) {
+                br.f[q] = b.f[nd + q];
+                br.g[q] = b.g[nd + q];
+                br.dx[q] = nullptr;
+            }
+            McFuse mr = m;
+            mr.fid0 = m.fid0 + nd;
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fft_x<1, true, MAPS, TI, TO, P2>), dim3(2 * grid.x, bd.n),
+                               dim3(g.ft), sh, st, bd, nrows, g.ny, g.nxpp, g.M, g.logM, g.planM, R, g.twM, g.twN,
+                               g.scaley, m);
+            if (br.n)
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fft_x<1, false, MAPS, TI, TO, P2>), dim3(grid.x, br.n),
+                                   dim3(g.ft), sh, st, br, nrows, g.ny, g.nxpp, g.M, g.logM, g.planM, R, g.twM,
+                                   g.twN, g.scaley, mr);
+        } else {
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fft_x<1, false, MAPS, TI, TO, P2>), grid, dim3(g.ft), sh, st, b,
+                               nrows, g.ny, g.nxpp, g.M, g.logM, g.planM, R, g.twM, g.twN, g.scaley, m);
+        }
     }
 }
 
@@ -887,10 +968,12 @@ void launch_fft_x(const Geo &g, FieldBatch b, int nk, int dir, hipStream_t st, c
         fft_x_dispatch<false>(g, b, nk, dir, st, mf, prec);
 }
 
+// maxcalc partial slots of the inverse x transform: one per R rows, two with
+// P2 (the tiles that pair a field with its x gradient hold R/2 of its rows)
 int mc_blocks(const Geo &g, int ft)
 {
     const int R = 8 * (ft > 0 ? ft : g.ft) / g.M;
-    return (g.ny * g.nz + R - 1) / R;
+    return (g.p2 ? 2 : 1) * ((g.ny * g.nz + R - 1) / R);
 }
 
 template <bool P2>
