@@ -1046,6 +1046,7 @@ extern "C" int iw_destroy(iw_solver *s)
     hipFree(s->ox);
     hipFree(s->hbuf);
     for (int q = 0; q < 4; q++) hipFree(s->phys[q]);
+    for (int q = 0; q < 12; q++) hipFree(s->s32[q]);
     hipFree(s->icomp);
     hipFree(s->iface_src);
     hipFree(s->iface_mid_src);
@@ -1532,14 +1533,27 @@ static int convec_tail(iw_solver *s, int ispecfilternl)
     project_dev(s, uvwt, 4);
     {
         Scope sc(s, "wnl", 64. * (double)g.nxppy);
-        launch_wnl(g, B[14], B[18], B[22], s->fac, s->wnl, s->stream);
+        if (s->mixed)
+            launch_wnl(g, s->s32[2], s->s32[6], s->s32[10], s->fac, s->wnl, s->stream);
+        else
+            launch_wnl(g, B[14], B[18], B[22], s->fac, s->wnl, s->stream);
     }
     {
         Scope sc(s, "rotvisc");
         double *lev[9] = {B[0], B[1], B[2], B[4], B[5], B[6], B[8], B[9], B[10]};
         launch_rotvisc(g, s->P, lev, s->fac, s->wviscrot, s->stream);
     }
-    {
+    if (s->mixed) { // FP32 right-hand-side histories
+        Scope sc(s, "bdf_filter", 4 * 44. * nt);
+        float *S12[12];
+        double *F12[12], *out4[4];
+        for (int q = 0; q < 4; q++) {
+            S12[3 * q] = s->s32[q]; S12[3 * q + 1] = s->s32[4 + q]; S12[3 * q + 2] = s->s32[8 + q];
+            F12[3 * q] = B[q]; F12[3 * q + 1] = B[4 + q]; F12[3 * q + 2] = B[8 + q];
+            out4[q] = B[8 + q];
+        }
+        launch_bdf_filter(g, S12, F12, out4, s->fac, s->bdf, s->P.dt, ispecfilternl == 1, s->stream);
+    } else {
         Scope sc(s, "bdf_filter", 4 * 56. * nt);
         double *S12[12], *F12[12], *out4[4];
         for (int q = 0; q < 4; q++) {
@@ -1554,6 +1568,13 @@ static int convec_tail(iw_solver *s, int ispecfilternl)
         B[8 + q] = B[4 + q];
         B[4 + q] = B[q];
         B[q] = nw;
+        if (s->mixed) { // the FP64 S, Sn, Snm are written when the mixed run ends
+            float *fr = s->s32[8 + q];
+            s->s32[8 + q] = s->s32[4 + q];
+            s->s32[4 + q] = s->s32[q];
+            s->s32[q] = fr;
+            continue;
+        }
         double *fr = B[20 + q];
         B[20 + q] = B[16 + q];
         B[16 + q] = B[12 + q];
@@ -1566,9 +1587,11 @@ static int convec_tail(iw_solver *s, int ispecfilternl)
 // maxcalc reductions, no mean-wave terms) with the physical-space working set
 // in FP32.  The inverse y pass reads the FP64 spectral state and writes FP32;
 // the x passes, the z contraction (FP32 matrix cores) and the products run on
-// FP32 arrays; the forward y pass writes the FP64 spectral right-hand sides.
-// From there on (round-trip projection, pressure BC terms, SS/BDF assembly,
-// filters and the implicit step) everything is FP64 as in the plain path.
+// FP32 arrays; the forward y pass writes the spectral right-hand sides in
+// FP32 (s32: the products they come from carry FP32 rounding already), which
+// the SS/BDF assembly widens.  From there on (round-trip projection, pressure
+// BC terms, SS/BDF assembly, filters and the implicit step) everything is FP64
+// as in the plain path.
 // The FP32 arrays live in the FP64 scratch of the plain path: phys[q] holds
 // the physical field (first half) and its right-hand side (second half),
 // grad[q], prod[q] and ox one FP32 array each.
@@ -1576,8 +1599,6 @@ static int convec_mixed(iw_solver *s, int ispecfilternl, int mcstep = 0, int mcs
 {
     const Geo &g = s->g;
     double **B = s->buf;
-    double *uvwt[4] = {B[0], B[1], B[2], B[3]};
-    double *S[4] = {B[12], B[13], B[14], B[15]};
     float *ph[4], *S32[4], *g32[9], *pr32[7];
     for (int q = 0; q < 4; q++) {
         ph[q] = reinterpret_cast<float *>(s->phys[q]);
@@ -1639,15 +1660,15 @@ static int convec_mixed(iw_solver *s, int ispecfilternl, int mcstep = 0, int mcs
         Scope sc(s, "m_fft_x_fwd", 19 * 4. * (double)g.nxppy * g.nz);
         launch_fft_x(g, bx, g.nz, -1, s->stream, nullptr, kPrecF32);
     }
-    {   // forward y pass: FP32 in, FP64 spectral right-hand sides out
+    {   // forward y pass: FP32 in, FP32 spectral right-hand sides out
         FieldBatch by = batch({D(S32[0]), D(S32[1]), D(S32[2]), D(S32[3])});
         for (int q = 0; q < 4; q++) {
-            by.g[q] = S[q];
+            by.g[q] = D(s->s32[q]);
             by.ay[q] = D(g32[q]);
         }
         by.kw = g.yw;
-        Scope sc(s, "m_fft_y_fwd", (8 * 8. + 4 * 16.) * nc);
-        launch_fft_y(g, by, g.nz, -1, s->stream, kPrecF32);
+        Scope sc(s, "m_fft_y_fwd", 12 * 8. * nc);
+        launch_fft_y(g, by, g.nz, -1, s->stream, kPrecF32S);
     }
     return convec_tail(s, ispecfilternl);
 }
@@ -2072,9 +2093,31 @@ extern "C" int iw_set_mixed(iw_solver *s, int on)
         if (mean_active(s)) return fail("iw_set_mixed: mean-wave and shear-current terms are FP64 only");
         if (s->closure) return fail("iw_set_mixed: the learned closure runs on the plain FP64 path");
     }
+    if (on && !s->s32[0]) {
+        for (int q = 0; q < 12; q++) HIPCHK(hipMalloc((void **)&s->s32[q], sizeof(float) * (size_t)s->g.ntot));
+        s->dev_bytes += 12 * (long long)sizeof(float) * s->g.ntot;
+    }
     s->mixed = on != 0;
     return 0;
 }
+
+// Mixed runs keep S, Sn, Snm in FP32 (s32); outside a run the FP64 buffers
+// hold them.  At the start of a mixed run Sn, Snm are rounded to FP32 (S is
+// formed anew in the step); at its end, on every return, all three are
+// widened back into buf[12..23].
+struct MixedHistories {
+    iw_solver *s;
+    explicit MixedHistories(iw_solver *s_) : s(s_)
+    {
+        if (!s->mixed) return;
+        for (int i = 0; i < 8; i++) launch_cvt(s->buf[16 + i], s->s32[4 + i], s->g.ntot, s->stream);
+    }
+    ~MixedHistories()
+    {
+        if (!s->mixed) return;
+        for (int i = 0; i < 12; i++) launch_cvt(s->s32[i], s->buf[12 + i], s->g.ntot, s->stream);
+    }
+};
 
 extern "C" int iw_energy_records(iw_solver *s, double *out, int n)
 {
@@ -2535,6 +2578,7 @@ extern "C" int iw_run(iw_solver *s, int nsteps, int *counts)
     if (!s || nsteps < 0) return fail("iw_run: bad argument");
     const bool mc = s->P.imaxcalc || s->P.dtadapt;
     if (s->mixed && mean_active(s)) return fail("iw_run: the mixed path runs without mean-wave terms");
+    const MixedHistories mixed_hist(s);
     const bool adapt = s->P.dtadapt != 0;
     s->mc_hist.clear();
     if (mc && !adapt && nsteps > s->mc_ring_cap) { // device ring of the step records

@@ -218,6 +218,11 @@ struct iw_solver {
     bool mw_on = false;
     // mixed path (iw_set_mixed): FP32 physical-space working set in convec
     bool mixed = false;
+    // the mixed path's right-hand-side histories S, Sn, Snm of u, v, w, temp
+    // in FP32 (the products they come from are FP32): filled from buf[16..23]
+    // when a mixed run starts and written back to buf[12..23] when it ends,
+    // so that between runs the state is the FP64 one
+    float *s32[12] = {};
     // the u/v, w and rho solves by the sequential band sweeps (k_sweep)
     // instead of the subdomain-parallel solve: IW_SWEEPS=seq at iw_create
     bool seq_sweeps = false;
@@ -268,6 +273,9 @@ void launch_maxcalc_umaxt(const Geo &g, const McArgs &A, double umax, double *co
 // FP64; kPrecF32 (mixed path) x pass FP32 in and out, y pass FP64 spectral
 // in and FP32 out (inverse) or FP32 in and FP64 spectral out (forward)
 constexpr int kPrecF64 = 0, kPrecF32 = 1;
+// forward y pass only: FP32 in and FP32 spectral out (the mixed path's
+// right-hand sides)
+constexpr int kPrecF32S = 2;
 void launch_fft_x(const Geo &g, FieldBatch b, int nk, int dir, hipStream_t st, const McFuse *mf = nullptr,
                   int prec = kPrecF64);
 void launch_fft_y(const Geo &g, FieldBatch b, int nk, int dir, hipStream_t st, int prec = kPrecF64);
@@ -317,11 +325,19 @@ void launch_kepe(const Geo &g, double *const *phys4, double cke, double cpe, dou
 void launch_rhs_spec(const Geo &g, double *const *S, double *const *pr, hipStream_t st);
 void launch_wnl(const Geo &g, const double *sw, const double *swn, const double *swnm,
                 const double *fac, double *wnl, hipStream_t st);
+void launch_wnl(const Geo &g, const float *sw, const float *swn, const float *swnm, const double *fac,
+                double *wnl, hipStream_t st);
+// n values FP64 -> FP32 (round to nearest) or FP32 -> FP64
+void launch_cvt(const double *a, float *b, long n, hipStream_t st);
+void launch_cvt(const float *a, double *b, long n, hipStream_t st);
 void launch_rotvisc(const Geo &g, const iw_params &P, double *const *lev /*9*/, const double *fac,
                     double *wviscrot, hipStream_t st);
 // SS extrapolation + BDF assembly + filters of u,v,w,rho in one launch:
 // S12/F12 = (S, Sn, Snm) and (f, fn, fnm) per field, out4 the results
 void launch_bdf_filter(const Geo &g, double *const *S12, double *const *F12, double *const *out4, const double *fac,
+                       const double *bdf, double dt, int filt, hipStream_t st);
+// the same with the right-hand-side histories in FP32 (mixed path)
+void launch_bdf_filter(const Geo &g, float *const *S12, double *const *F12, double *const *out4, const double *fac,
                        const double *bdf, double dt, int filt, hipStream_t st);
 void launch_filter3d(const Geo &g, double *f, int ifourier, int ilegen, hipStream_t st);
 void launch_iface(const Geo &g, FieldBatch b, int *count, hipStream_t st);

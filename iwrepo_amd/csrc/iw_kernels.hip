@@ -982,10 +982,13 @@ static void fft_y_dispatch(const Geo &g, const FieldBatch &b, int nk, int dir, h
     const int CB = 8 * g.ft / g.ny; // columns per block (fft_threads_ok: CB >= 1)
     dim3 grid((g.nxh + CB - 1) / CB, nk, b.n);
     // the mixed path's tile and twiddles are FP32
-    const size_t cs = prec == kPrecF32 ? sizeof(float2) : sizeof(double2);
+    const size_t cs = prec != kPrecF64 ? sizeof(float2) : sizeof(double2);
     size_t sh = cs * ((size_t)CB * g.ny + g.ny) + sizeof(double) * g.ny;
-    // mixed path: spectral FP64 <-> FP32 intermediate
-    if (dir < 0 && prec == kPrecF32)
+    // mixed path: spectral FP64 <-> FP32 intermediate, or FP32 right-hand sides
+    if (dir < 0 && prec == kPrecF32S)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fft_y<-1, float, float, P2>), grid, dim3(g.ft), sh, st, b, g.nxpp,
+                           g.nxh, g.ny, g.logNy, g.planY, CB, g.twY, g.scaley);
+    else if (dir < 0 && prec == kPrecF32)
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fft_y<-1, float, double, P2>), grid, dim3(g.ft), sh, st, b, g.nxpp,
                            g.nxh, g.ny, g.logNy, g.planY, CB, g.twY, g.scaley);
     else if (prec == kPrecF32)
@@ -1500,20 +1503,28 @@ void launch_dx(const Geo &g, const double *a, double *b, hipStream_t st)
 }
 
 // wnl at bottom/top (solver.f:270-284)
-__global__ void k_wnl(const double *sw, const double *swn, const double *swnm, double f1, double f2,
-                      double f3, double *wnl, long nxppy, int nz)
+template <typename T>
+__global__ void k_wnl(const T *sw, const T *swn, const T *swnm, double f1, double f2, double f3, double *wnl,
+                      long nxppy, int nz)
 {
     long ij = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (ij >= nxppy) return;
     long kt = (long)(nz - 1) * nxppy + ij;
-    wnl[ij] = f1 * sw[ij] + f2 * swn[ij] + f3 * swnm[ij];
-    wnl[nxppy + ij] = f1 * sw[kt] + f2 * swn[kt] + f3 * swnm[kt];
+    wnl[ij] = f1 * (double)sw[ij] + f2 * (double)swn[ij] + f3 * (double)swnm[ij];
+    wnl[nxppy + ij] = f1 * (double)sw[kt] + f2 * (double)swn[kt] + f3 * (double)swnm[kt];
 }
 
 void launch_wnl(const Geo &g, const double *sw, const double *swn, const double *swnm, const double *fac,
                 double *wnl, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_wnl, dim3((unsigned)((g.nxppy + 255) / 256)), dim3(256), 0, st, sw, swn, swnm,
+    hipLaunchKernelGGL(k_wnl<double>, dim3((unsigned)((g.nxppy + 255) / 256)), dim3(256), 0, st, sw, swn, swnm,
+                       fac[0], fac[1], fac[2], wnl, g.nxppy, g.nz);
+}
+
+void launch_wnl(const Geo &g, const float *sw, const float *swn, const float *swnm, const double *fac,
+                double *wnl, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_wnl<float>, dim3((unsigned)((g.nxppy + 255) / 256)), dim3(256), 0, st, sw, swn, swnm,
                        fac[0], fac[1], fac[2], wnl, g.nxppy, g.nz);
 }
 
@@ -1583,6 +1594,7 @@ constexpr int kMB = 32;
 
 struct TileArgs {
     const double *S[4][3]; // per field (blockIdx.z): S, Sn, Snm
+    const float *S32[4][3]; // the same in FP32 (k_tile_t<.., 1, true>)
     const double *F[4][3]; // f, fn, fnm
     double *out[4];
     double f1, f2, f3, b1, b2, b3, dt;
@@ -1662,7 +1674,8 @@ __global__ __launch_bounds__(256) void k_tile(TileArgs A, long ncplx_plane, long
 // with a lane exchange (lanes 2i and 2i+1 hold re and im of one mode).  The
 // Legendre filter is applied one output row at a time from the inputs in
 // registers (the sums in the order of legfilter_mat).
-template <int NL, int MODE>
+// S32: the right-hand-side histories are FP32 (widened exactly before use)
+template <int NL, int MODE, bool S32 = false>
 __global__ __launch_bounds__(256) void k_tile_t(TileArgs A, long nxppy, const double *__restrict__ fcf,
                                                 const double *__restrict__ FL)
 {
@@ -1681,7 +1694,16 @@ __global__ __launch_bounds__(256) void k_tile_t(TileArgs A, long nxppy, const do
 #pragma unroll
         for (int t = 0; t < NL; t++) {
             const long o = b0 + (long)t * nxppy;
-            const double s0 = Sf[0][o], s1 = Sf[1][o], s2 = Sf[2][o];
+            double s0, s1, s2;
+            if (S32) {
+                s0 = (double)A.S32[fz][0][o];
+                s1 = (double)A.S32[fz][1][o];
+                s2 = (double)A.S32[fz][2][o];
+            } else {
+                s0 = Sf[0][o];
+                s1 = Sf[1][o];
+                s2 = Sf[2][o];
+            }
             const double f0 = Ff[0][o], f1 = Ff[1][o], f2 = Ff[2][o];
             const double sr = A.f1 * s0 + A.f2 * s1 + A.f3 * s2;
             x[t] = A.dt * sr + A.b1 * f0 + A.b2 * f1 + A.b3 * f2;
@@ -1717,7 +1739,10 @@ static void run_tile(const Geo &g, const TileArgs &A, int nf, hipStream_t st)
     long ncp = (long)g.nxhp * g.ny;
     dim3 gt((unsigned)((g.nxppy + 255) / 256), g.nsubd, nf);
 #define TT(NL)                                                                                                 \
-    if (A.mode == 1)                                                                                           \
+    if (A.mode == 1 && A.S32[0][0])                                                                            \
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_tile_t<NL, 1, true>), gt, dim3(256), 0, st, A, g.nxppy, g.fcf,  \
+                           g.FL);                                                                              \
+    else if (A.mode == 1)                                                                                      \
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_tile_t<NL, 1>), gt, dim3(256), 0, st, A, g.nxppy, g.fcf, g.FL); \
     else                                                                                                       \
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_tile_t<NL, 0>), gt, dim3(256), 0, st, A, g.nxppy, g.fcf, g.FL)
@@ -1751,6 +1776,49 @@ void launch_bdf_filter(const Geo &g, double *const *S12, double *const *F12, dou
     A.ifourier = filt;
     A.ilegen = filt;
     run_tile(g, A, 4, st);
+}
+
+// only the specialised orders (those of the mixed path) read FP32 histories
+void launch_bdf_filter(const Geo &g, float *const *S12, double *const *F12, double *const *out4, const double *fac,
+                       const double *bdf, double dt, int filt, hipStream_t st)
+{
+    TileArgs A = {};
+    for (int f = 0; f < 4; f++) {
+        for (int q = 0; q < 3; q++) {
+            A.S32[f][q] = S12[3 * f + q];
+            A.F[f][q] = F12[3 * f + q];
+        }
+        A.out[f] = out4[f];
+    }
+    A.f1 = fac[0]; A.f2 = fac[1]; A.f3 = fac[2];
+    A.b1 = bdf[0]; A.b2 = bdf[1]; A.b3 = bdf[2];
+    A.dt = dt;
+    A.mode = 1;
+    A.ifourier = filt;
+    A.ilegen = filt;
+    if (nl_specialised(g.nzloc)) run_tile(g, A, 4, st);
+}
+
+__global__ __launch_bounds__(256) void k_cvt_df(const double *__restrict__ a, float *__restrict__ b, long n)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (float)a[i];
+}
+
+__global__ __launch_bounds__(256) void k_cvt_fd(const float *__restrict__ a, double *__restrict__ b, long n)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (double)a[i];
+}
+
+void launch_cvt(const double *a, float *b, long n, hipStream_t st)
+{
+    if (n > 0) hipLaunchKernelGGL(k_cvt_df, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a, b, n);
+}
+
+void launch_cvt(const float *a, double *b, long n, hipStream_t st)
+{
+    if (n > 0) hipLaunchKernelGGL(k_cvt_fd, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a, b, n);
 }
 
 void launch_filter3d(const Geo &g, double *f, int ifourier, int ilegen, hipStream_t st)

@@ -74,6 +74,29 @@ def test_mixed_path_against_fp64_oracle(rt, oracle_mod, name, steps):
     g.close()
 
 
+def test_mixed_run_split_over_calls_is_bitwise_one_call(rt):
+    """The mixed path keeps the right-hand-side histories S, Sn, Snm in FP32
+    during a run and in the FP64 state between runs: 12 steps as 5 + 7 calls
+    equal 12 steps in one call bit for bit (all 24 state arrays), and the
+    histories between runs are FP32 values."""
+    P, flds = _params("small")
+    out = []
+    for chunks in ((12,), (5, 7)):
+        g = rt.Solver(P)
+        g.set_mixed(True)
+        g.set_initial(*flds)
+        for n in chunks:
+            g.run(n, counts=False)
+        out.append([g.get_state(q) for q in range(24)])
+        g.close()
+    for q in range(24):
+        assert np.array_equal(out[0][q], out[1][q]), STATE[q]
+    for q in range(16, 24):
+        a = out[0][q]
+        assert np.array_equal(a.astype(np.float32).astype(np.float64), a), STATE[q]
+        assert np.any(a != 0.0), STATE[q]
+
+
 def test_mixed_path_switches_back_to_plain(rt, oracle_mod):
     """iw_set_mixed(0) restores the plain path exactly (1e-10 vs the oracle),
     and the mixed path refuses what it does not cover."""
