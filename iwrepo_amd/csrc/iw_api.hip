@@ -990,7 +990,8 @@ extern "C" int iw_create(const iw_params *p, iw_solver **out)
         if (upload(&s->mc_rm, rm, acc) || upload(&s->mc_dz, dz, acc) || upload(&s->mc_rdz, rdz, acc)) return -1;
         const int nxy = P.nx * P.ny;
         // sized for the smallest transform block (the most blocks)
-        HIPCHK(hipMalloc((void **)&s->mc_part, sizeof(double) * 2 * kMcQ * (size_t)mc_blocks(g, 64)));
+        // + kMcPre slots after the largest count (block size 64) for the two-level reduction
+        HIPCHK(hipMalloc((void **)&s->mc_part, sizeof(double) * 2 * kMcQ * ((size_t)mc_blocks(g, 64) + kMcPre)));
         HIPCHK(hipMalloc((void **)&s->mc_rec, sizeof(double) * kMcRec));
         HIPCHK(hipMemset(s->mc_rec, 0, sizeof(double) * kMcRec));
         HIPCHK(hipMalloc((void **)&s->mc_w00, sizeof(double) * g.nz));
@@ -1355,6 +1356,7 @@ static void mc_record(iw_solver *s, int istep, double cny, const float *u32 = nu
     A.nx = g.nx; A.ny = g.ny; A.nz = g.nz; A.nxpp = g.nxpp; A.nxppy = g.nxppy;
     A.part = s->mc_part;
     A.nblk = mc_blocks(g);
+    A.part2 = s->mc_part + (size_t)2 * kMcQ * mc_blocks(g, 64);
     A.umwv = s->mw_on ? s->mw : nullptr;
     {
         Scope sc(s, "maxcalc");

@@ -160,7 +160,9 @@ struct McArgs {
     const double *part;
     int nblk;
     const double *umwv;          // mean-wave u added for the umaxt chain, or nullptr
+    double *part2 = nullptr;     // kMcPre partial slots: the partials reduced in kMcPre groups first
 };
+constexpr int kMcPre = 64;
 
 struct Prof {
     std::vector<std::string> names;

@@ -304,9 +304,38 @@ __global__ __launch_bounds__(256) void k_mc_walk(McArgs A, const double *__restr
 
 } // namespace
 
+// the partials of kMcPre groups of blocks, each reduced by one work-group
+// (the reduction is order-free: extreme value, largest L on a tie)
+__global__ __launch_bounds__(kMcThreads) void k_mc_pre(const double *__restrict__ part, int nblk, double *part2)
+{
+    McAcc acc;
+    acc.init();
+    const int per = (nblk + (int)gridDim.x - 1) / (int)gridDim.x;
+    const int b0 = (int)blockIdx.x * per, b1 = b0 + per < nblk ? b0 + per : nblk;
+    for (int b = b0 + (int)threadIdx.x; b < b1; b += kMcThreads) {
+        const double *p = part + (size_t)b * (2 * kMcQ);
+        for (int q = 0; q < kMcQ; q++) acc.add(q, p[2 * q], (int)p[2 * q + 1]);
+    }
+    mc_block_reduce(acc);
+    if (threadIdx.x != 0) return;
+    double *o = part2 + (size_t)blockIdx.x * (2 * kMcQ);
+    for (int q = 0; q < kMcQ; q++) {
+        o[2 * q] = acc.v[q];
+        o[2 * q + 1] = (double)acc.l[q];
+    }
+}
+
 void launch_maxcalc_final(const Geo &g, const McArgs &A, int istep, double t, double umax, double *rec,
                           hipStream_t st)
 {
+    if (A.part2 && A.nblk > kMcPre) {
+        hipLaunchKernelGGL(k_mc_pre, dim3(kMcPre), dim3(kMcThreads), 0, st, A.part, A.nblk, A.part2);
+        McArgs B = A;
+        B.part = A.part2;
+        B.nblk = kMcPre;
+        hipLaunchKernelGGL(k_mc_final, dim3(1), dim3(kMcThreads), 0, st, B, istep, t, umax, rec);
+        return;
+    }
     hipLaunchKernelGGL(k_mc_final, dim3(1), dim3(kMcThreads), 0, st, A, istep, t, umax, rec);
 }
 
