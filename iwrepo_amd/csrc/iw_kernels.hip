@@ -745,20 +745,30 @@ __global__ __launch_bounds__(kFT) void k_fft_x(FieldBatch fb, int nrows, int ny,
 // TI, TO: storage type of the inputs (f, ay) and of the outputs (g, dy).
 // P2: ny is a power of two; else ny = 2^a 3^b 5^c with the radices of
 // `plan` and CB = floor(8 blockDim / ny) columns per block.
+// One-dimensional grid: ncb column blocks per plane, npl planes per field,
+// the (plane, field) units padded to a multiple of 8.
 template <int DIR, typename TI = double, typename TO = double, bool P2 = true>
 __global__ __launch_bounds__(kFT) void k_fft_y(FieldBatch fb, int nxpp, int nxh, int ny, int logNy, unsigned plan,
-                                               int CB, const double2 *__restrict__ twY, double scaley)
+                                               int CB, const double2 *__restrict__ twY, double scaley, int ncb,
+                                               int npl)
 {
+    // workgroup -> (column block, plane, field) such that the column blocks
+    // of one plane are consecutive workgroups of one XCD (workgroup w runs on
+    // XCD w mod 8): the cache lines that neighbouring column blocks share in
+    // every row are then read into one L2, by workgroups that run together
+    const int wg = blockIdx.x, xcd = wg & 7, slot = wg >> 3;
+    const int cbi = slot % ncb, unit = (slot / ncb) * 8 + xcd;
+    if (unit >= npl * fb.n) return;
+    const int lpl = unit % npl, fz = unit / npl;
     using C = tile_t<TI, TO>;
     extern __shared__ double2 smy[];
     C *buf = reinterpret_cast<C *>(smy);
     C *tw = buf + CB * ny;
     double *kys = reinterpret_cast<double *>(tw + ny); // ky per row, staged like tw
-    const int lpl = blockIdx.y;
-    const TI *plane = reinterpret_cast<const TI *>(fb.f[blockIdx.z]) + (long)(fb.smap ? fb.smap[lpl] : lpl) * nxpp * ny;
+    const TI *plane = reinterpret_cast<const TI *>(fb.f[fz]) + (long)(fb.smap ? fb.smap[lpl] : lpl) * nxpp * ny;
     const long opl = (long)(fb.dmap ? fb.dmap[lpl] : lpl) * nxpp * ny;
-    TO *oplane = reinterpret_cast<TO *>(fb.g[blockIdx.z]) + opl;
-    const int ii0 = blockIdx.x * CB;
+    TO *oplane = reinterpret_cast<TO *>(fb.g[fz]) + opl;
+    const int ii0 = cbi * CB;
     const int nval = CB * ny; // values of the tile: value idx is column idx % CB of row idx / CB
     const int lcb = P2 ? __builtin_ctz(CB) : 0;
     auto rowof = [&](int idx) -> int { return P2 ? idx >> lcb : idx / CB; };
@@ -805,8 +815,8 @@ __global__ __launch_bounds__(kFT) void k_fft_y(FieldBatch fb, int nxpp, int nxh,
     };
     // forward with a second input P: + i ky P-hat (the y part of the flux
     // divergence, solver.f:985-1063, 1163-1196), P-hat kept in registers
-    const TI *accp = (DIR < 0 && fb.ay[blockIdx.z])
-                         ? reinterpret_cast<const TI *>(fb.ay[blockIdx.z]) + (long)(fb.smap ? fb.smap[lpl] : lpl) * nxpp * ny
+    const TI *accp = (DIR < 0 && fb.ay[fz])
+                         ? reinterpret_cast<const TI *>(fb.ay[fz]) + (long)(fb.smap ? fb.smap[lpl] : lpl) * nxpp * ny
                          : nullptr;
     double2 pb[8];
     auto store = [&](TO *op, bool add) {
@@ -829,7 +839,7 @@ __global__ __launch_bounds__(kFT) void k_fft_y(FieldBatch fb, int nxpp, int nxh,
             st2(op + (long)j * nxpp + 2 * ii, z);
         }
     };
-    TO *dyp = (DIR > 0 && fb.dy[blockIdx.z]) ? reinterpret_cast<TO *>(fb.dy[blockIdx.z]) + opl : nullptr;
+    TO *dyp = (DIR > 0 && fb.dy[fz]) ? reinterpret_cast<TO *>(fb.dy[fz]) + opl : nullptr;
     if (dyp) { // i ky f first: a transform in place overwrites its input
         load(plane, true);
         __syncthreads();
@@ -861,7 +871,7 @@ __global__ __launch_bounds__(kFT) void k_fft_y(FieldBatch fb, int nxpp, int nxh,
     // the untransformed kx-Nyquist column: copied when out of place, plus
     // i ky P when accumulating, and times i ky into the derivative output
     const bool inplace = (const void *)plane == (const void *)oplane;
-    if (blockIdx.x == 0 && (!inplace || dyp || accp))
+    if (cbi == 0 && (!inplace || dyp || accp))
         for (int j = threadIdx.x; j < ny; j += blockDim.x) {
             const TI *p = plane + (long)j * nxpp + 2 * nxh;
             double pr = p[0], pi = p[1];
@@ -980,26 +990,28 @@ template <bool P2>
 static void fft_y_dispatch(const Geo &g, const FieldBatch &b, int nk, int dir, hipStream_t st, int prec)
 {
     const int CB = 8 * g.ft / g.ny; // columns per block (fft_threads_ok: CB >= 1)
-    dim3 grid((g.nxh + CB - 1) / CB, nk, b.n);
+    const int ncb = (g.nxh + CB - 1) / CB;
+    const long units = ((long)nk * b.n + 7) / 8 * 8;
+    dim3 grid((unsigned)(ncb * units));
     // the mixed path's tile and twiddles are FP32
     const size_t cs = prec != kPrecF64 ? sizeof(float2) : sizeof(double2);
     size_t sh = cs * ((size_t)CB * g.ny + g.ny) + sizeof(double) * g.ny;
     // mixed path: spectral FP64 <-> FP32 intermediate, or FP32 right-hand sides
     if (dir < 0 && prec == kPrecF32S)
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fft_y<-1, float, float, P2>), grid, dim3(g.ft), sh, st, b, g.nxpp,
-                           g.nxh, g.ny, g.logNy, g.planY, CB, g.twY, g.scaley);
+                           g.nxh, g.ny, g.logNy, g.planY, CB, g.twY, g.scaley, ncb, nk);
     else if (dir < 0 && prec == kPrecF32)
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fft_y<-1, float, double, P2>), grid, dim3(g.ft), sh, st, b, g.nxpp,
-                           g.nxh, g.ny, g.logNy, g.planY, CB, g.twY, g.scaley);
+                           g.nxh, g.ny, g.logNy, g.planY, CB, g.twY, g.scaley, ncb, nk);
     else if (prec == kPrecF32)
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fft_y<1, double, float, P2>), grid, dim3(g.ft), sh, st, b, g.nxpp,
-                           g.nxh, g.ny, g.logNy, g.planY, CB, g.twY, g.scaley);
+                           g.nxh, g.ny, g.logNy, g.planY, CB, g.twY, g.scaley, ncb, nk);
     else if (dir < 0)
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fft_y<-1, double, double, P2>), grid, dim3(g.ft), sh, st, b, g.nxpp,
-                           g.nxh, g.ny, g.logNy, g.planY, CB, g.twY, g.scaley);
+                           g.nxh, g.ny, g.logNy, g.planY, CB, g.twY, g.scaley, ncb, nk);
     else
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fft_y<1, double, double, P2>), grid, dim3(g.ft), sh, st, b, g.nxpp,
-                           g.nxh, g.ny, g.logNy, g.planY, CB, g.twY, g.scaley);
+                           g.nxh, g.ny, g.logNy, g.planY, CB, g.twY, g.scaley, ncb, nk);
 }
 
 void launch_fft_y(const Geo &g, FieldBatch b, int nk, int dir, hipStream_t st, int prec)
