@@ -205,7 +205,7 @@ __device__ __forceinline__ void lds_pass(C *buf, int logN, int logNs, int nrows,
 
 // nrows sequences of length N = 2^logN in buf, in place
 template <bool COL, typename C>
-__device__ void fft_lds(C *buf, int logN, int nrows, const C *tw, int sign)
+__device__ __forceinline__ void fft_lds(C *buf, int logN, int nrows, const C *tw, int sign)
 {
     const int lrows = COL ? __builtin_ctz(nrows) : 0;
     int lNs = 0, left = logN;
@@ -274,7 +274,7 @@ __device__ __forceinline__ void lds_pass_g(C *buf, int N, int Ns, int nrows, con
 }
 
 template <bool COL, typename C>
-__device__ void fft_lds_g(C *buf, int N, unsigned plan, int nrows, const C *tw, int sign)
+__device__ __forceinline__ void fft_lds_g(C *buf, int N, unsigned plan, int nrows, const C *tw, int sign)
 {
     int Ns = 1;
     for (; plan; plan >>= 4) {
