@@ -350,12 +350,13 @@ __device__ __forceinline__ void st2(float *p, double2 v)
     *reinterpret_cast<float2 *>(p) = make_float2((float)v.x, (float)v.y);
 }
 
+// (x, lx) replaces (b, lb) when it is more extreme, or equal and later;
+// selects, not a branch (no divergent region per comparison)
 __device__ __forceinline__ void mc_take(bool mn, double x, int lx, double &b, int &lb)
 {
-    if (mn ? (x < b || (x == b && lx > lb)) : (x > b || (x == b && lx > lb))) {
-        b = x;
-        lb = lx;
-    }
+    const bool take = (mn ? x < b : x > b) | ((x == b) & (lx > lb));
+    b = take ? x : b;
+    lb = take ? lx : lb;
 }
 
 // AX: forward, the batch's fields have flux products; inverse (P2 only), the
