@@ -1545,8 +1545,12 @@ static int convec_tail(iw_solver *s, int ispecfilternl)
         double *lev[9] = {B[0], B[1], B[2], B[4], B[5], B[6], B[8], B[9], B[10]};
         launch_rotvisc(g, s->P, lev, s->fac, s->wviscrot, s->stream);
     }
+    // with the packed pressure path set up (from the first step on), the
+    // same pass forms the pressure right-hand side for this step's calcuvwt
+    const bool prhs = s->fuse_prhs && s->sys.gwt_tasks && s->sys.tri_ok_p && s->p_factored;
+    s->phi_ready = false;
     if (s->mixed) { // FP32 right-hand-side histories
-        Scope sc(s, "bdf_filter", 4 * 44. * nt);
+        Scope sc(s, "bdf_filter", (prhs ? 4 * 44. + 8. : 4 * 44.) * nt);
         float *S12[12];
         double *F12[12], *out4[4];
         for (int q = 0; q < 4; q++) {
@@ -1554,16 +1558,24 @@ static int convec_tail(iw_solver *s, int ispecfilternl)
             F12[3 * q] = B[q]; F12[3 * q + 1] = B[4 + q]; F12[3 * q + 2] = B[8 + q];
             out4[q] = B[8 + q];
         }
-        launch_bdf_filter(g, S12, F12, out4, s->fac, s->bdf, s->P.dt, ispecfilternl == 1, s->stream);
+        if (prhs)
+            s->phi_ready = launch_bdf_filter_prhs(g, S12, F12, out4, s->fac, s->bdf, s->P.dt, ispecfilternl == 1,
+                                                  s->sys, s->grad[0], s->wnl, s->wviscrot, s->stream);
+        if (!s->phi_ready)
+            launch_bdf_filter(g, S12, F12, out4, s->fac, s->bdf, s->P.dt, ispecfilternl == 1, s->stream);
     } else {
-        Scope sc(s, "bdf_filter", 4 * 56. * nt);
+        Scope sc(s, "bdf_filter", (prhs ? 4 * 56. + 8. : 4 * 56.) * nt);
         double *S12[12], *F12[12], *out4[4];
         for (int q = 0; q < 4; q++) {
             S12[3 * q] = B[12 + q]; S12[3 * q + 1] = B[16 + q]; S12[3 * q + 2] = B[20 + q];
             F12[3 * q] = B[q]; F12[3 * q + 1] = B[4 + q]; F12[3 * q + 2] = B[8 + q];
             out4[q] = B[8 + q];
         }
-        launch_bdf_filter(g, S12, F12, out4, s->fac, s->bdf, s->P.dt, ispecfilternl == 1, s->stream);
+        if (prhs)
+            s->phi_ready = launch_bdf_filter_prhs(g, S12, F12, out4, s->fac, s->bdf, s->P.dt, ispecfilternl == 1,
+                                                  s->sys, s->grad[0], s->wnl, s->wviscrot, s->stream);
+        if (!s->phi_ready)
+            launch_bdf_filter(g, S12, F12, out4, s->fac, s->bdf, s->P.dt, ispecfilternl == 1, s->stream);
     }
     for (int q = 0; q < 4; q++) { // history rotation (solver.f:447-478) by pointer swap
         double *nw = B[8 + q];
@@ -1691,6 +1703,7 @@ static int calcuvwt_dev(iw_solver *s, int cpen)
         const double fpt = (double)(2 * s->P.nzloc + 1) * s->g.nz * 8. + s->g.nz * 4.;
         const double bytes[3] = {32. * nt, 16. * nt + s->sys.np * fpt, 56. * nt};
         for (int part = 0; part < 3; part++) {
+            if (part == 0 && s->phi_ready) continue; // formed by convec's SS/BDF pass
             Scope sc(s, name[part], bytes[part]);
             launch_pressure_nat(s->g, s->P, s->sys, uvwt, s->grad[0], s->wnl, s->wviscrot, s->P.dt, cpen, part,
                                 s->stream);
@@ -1699,6 +1712,7 @@ static int calcuvwt_dev(iw_solver *s, int cpen)
         Scope sc(s, "pressure", 56. * nt + s->sys.np * fcol); // u,v,w in; u,v,w out; w re-read
         launch_pressure(s->g, s->P, s->sys, uvwt, s->wnl, s->wviscrot, s->P.dt, cpen, s->stream);
     }
+    s->phi_ready = false;
     if (s->sys.np_ok) {
         {
             // right-hand sides in, solutions out (16 B per point and field)
@@ -2581,6 +2595,11 @@ extern "C" int iw_run(iw_solver *s, int nsteps, int *counts)
     const bool mc = s->P.imaxcalc || s->P.dtadapt;
     if (s->mixed && mean_active(s)) return fail("iw_run: the mixed path runs without mean-wave terms");
     const MixedHistories mixed_hist(s);
+    struct FuseScope { // convec may form the pressure right-hand side for the calcuvwt of its step
+        iw_solver *s;
+        explicit FuseScope(iw_solver *s_) : s(s_) { s->fuse_prhs = true; }
+        ~FuseScope() { s->fuse_prhs = s->phi_ready = false; }
+    } fuse_scope(s);
     const bool adapt = s->P.dtadapt != 0;
     s->mc_hist.clear();
     if (mc && !adapt && nsteps > s->mc_ring_cap) { // device ring of the step records

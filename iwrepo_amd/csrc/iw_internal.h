@@ -225,6 +225,9 @@ struct iw_solver {
     // when a mixed run starts and written back to buf[12..23] when it ends,
     // so that between runs the state is the FP64 one
     float *s32[12] = {};
+    // inside iw_run's step loop: convec's SS/BDF pass also forms the pressure
+    // right-hand side (phi_ready) for the calcuvwt of the same step
+    bool fuse_prhs = false, phi_ready = false;
     // the u/v, w and rho solves by the sequential band sweeps (k_sweep)
     // instead of the subdomain-parallel solve: IW_SWEEPS=seq at iw_create
     bool seq_sweeps = false;
@@ -338,6 +341,15 @@ void launch_rotvisc(const Geo &g, const iw_params &P, double *const *lev /*9*/, 
 // S12/F12 = (S, Sn, Snm) and (f, fn, fnm) per field, out4 the results
 void launch_bdf_filter(const Geo &g, double *const *S12, double *const *F12, double *const *out4, const double *fac,
                        const double *bdf, double dt, int filt, hipStream_t st);
+// bdf_filter fused with the pressure right-hand side (k_prhs) of the packed
+// pressure path: phi receives what k_prhs would write from the filtered u, v,
+// w; false (nothing launched) for orders without specialised kernels
+bool launch_bdf_filter_prhs(const Geo &g, double *const *S12, double *const *F12, double *const *out4,
+                            const double *fac, const double *bdf, double dt, int filt, const SysTables &S,
+                            double *phi, const double *wnl, const double *wviscrot, hipStream_t st);
+bool launch_bdf_filter_prhs(const Geo &g, float *const *S12, double *const *F12, double *const *out4,
+                            const double *fac, const double *bdf, double dt, int filt, const SysTables &S,
+                            double *phi, const double *wnl, const double *wviscrot, hipStream_t st);
 // the same with the right-hand-side histories in FP32 (mixed path)
 void launch_bdf_filter(const Geo &g, float *const *S12, double *const *F12, double *const *out4, const double *fac,
                        const double *bdf, double dt, int filt, hipStream_t st);

@@ -2946,6 +2946,161 @@ void launch_pressure_nat(const Geo &g, const iw_params &P, const SysTables &S, d
 #undef PSTEP
 }
 
+// bdf_filter and the pressure right-hand side in one pass.  A work-group is
+// 64 columns (re and im of 32 modes) of one subdomain; wavefront f takes
+// field f (u, v, w, rho) through the SS/BDF assembly and the filters of
+// k_tile_t<NL,1,S32> and writes it.  The filtered u, v, w also go to LDS,
+// and after a barrier the four wavefronts form the pressure right-hand side
+// of k_prhs from them (the same sums in the same order), so the pressure
+// pass reads no u, v, w back.
+template <int NL, bool S32>
+__global__ __launch_bounds__(256) void k_tile_prhs(TileArgs A, PArgs P, long nxppy, const double *__restrict__ fcf,
+                                                   const double *__restrict__ FL)
+{
+    __shared__ double FLs[NL * NL], Ds[NL * NL];
+    __shared__ double fv[3][NL][64]; // filtered u, v, w of the work-group's columns
+    for (int e = threadIdx.x; e < NL * NL; e += blockDim.x) {
+        FLs[e] = FL[e];
+        Ds[e] = P.D[e];
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, fz = threadIdx.x >> 6;
+    const long col = (long)blockIdx.x * 64 + lane;
+    const int ks = blockIdx.y;
+    const bool live = col < nxppy; // nxppy is even: both lanes of a mode are live or not
+    const bool im = col & 1;
+    const long b0 = (long)ks * NL * nxppy + (live ? col : 0);
+    if (live) {
+        const double *const *Ff = A.F[fz];
+        double *outf = A.out[fz];
+        const double sigf = fcf[col >> 1];
+        double x[NL];
+#pragma unroll
+        for (int t = 0; t < NL; t++) {
+            const long o = b0 + (long)t * nxppy;
+            double s0, s1, s2;
+            if (S32) {
+                s0 = (double)A.S32[fz][0][o];
+                s1 = (double)A.S32[fz][1][o];
+                s2 = (double)A.S32[fz][2][o];
+            } else {
+                s0 = A.S[fz][0][o];
+                s1 = A.S[fz][1][o];
+                s2 = A.S[fz][2][o];
+            }
+            const double f0 = Ff[0][o], f1 = Ff[1][o], f2 = Ff[2][o];
+            const double sr = A.f1 * s0 + A.f2 * s1 + A.f3 * s2;
+            x[t] = A.dt * sr + A.b1 * f0 + A.b2 * f1 + A.b3 * f2;
+        }
+        if (A.ifourier) {
+#pragma unroll
+            for (int t = 0; t < NL; t++) {
+                const double other = __shfl_xor(x[t], 1, 64);
+                double fr = im ? other : x[t], fi = im ? x[t] : other;
+                fourier_filter_value(fr, fi, sigf);
+                x[t] = im ? fi : fr;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < NL; q++) {
+            double o = x[q];
+            if (A.ilegen) {
+                o = 0.;
+#pragma unroll
+                for (int t = 0; t < NL; t++) o = o + FLs[t * NL + q] * x[t];
+            }
+            outf[b0 + (long)q * nxppy] = o;
+            if (fz < 3) fv[fz][q][lane] = o;
+        }
+    }
+    __syncthreads();
+    if (!live) return;
+    // k_prhs for this column, rows fz, fz + 4, ... (kx = ky = 0: phi = 0)
+    const int m = (int)(col >> 1), pl = lane ^ 1;
+    double eps = 0., pw0 = 0., pw1 = 0., xw = 0., yw = 0.;
+    if (m != 0) {
+        const int sP = P.modesys[4 * m];
+        eps = P.sysinfo[4 * sP];
+        xw = P.xw[m % P.nxhp];
+        yw = P.yw[m / P.nxhp];
+        pw0 = P.sysinfo[4 * sP + 1] * (P.wnl[col] + P.wviscrot[col]) * eps;
+        pw1 = P.sysinfo[4 * sP + 2] * (P.wnl[nxppy + col] + P.wviscrot[nxppy + col]) * eps;
+    }
+    const double fac = 2. / P.zh[ks];
+    for (int k = fz; k < NL; k += 4) {
+        double sd = 0.;
+#pragma unroll
+        for (int t = 0; t < NL; t++) sd = sd + Ds[k * NL + t] * fv[2][t][lane];
+        const double uu = fv[0][k][pl], vv = fv[1][k][pl];
+        double d = im ? (0. + xw * uu) + yw * vv : (0. + (-xw * uu)) + (-yw * vv);
+        d = d + sd * fac;
+        double r = -(d / P.dt) * eps;
+        const int kg = ks * NL + k;
+        if (kg == 0) r = r - pw0;
+        if (kg == P.nz - 1) r = r + pw1;
+        P.phi[b0 + (long)k * nxppy] = (m == 0) ? 0. : r;
+    }
+}
+
+template <typename TS>
+static bool bdf_prhs_launch(const Geo &g, TS *const *S12, double *const *F12, double *const *out4, const double *fac,
+                            const double *bdf, double dt, int filt, const SysTables &S, double *phi,
+                            const double *wnl, const double *wviscrot, hipStream_t st)
+{
+    if (!nl_specialised(g.nzloc) || g.ndims != 3) return false;
+    TileArgs A = {};
+    for (int f = 0; f < 4; f++) {
+        for (int q = 0; q < 3; q++) {
+            if constexpr (std::is_same<TS, float>::value)
+                A.S32[f][q] = S12[3 * f + q];
+            else
+                A.S[f][q] = S12[3 * f + q];
+            A.F[f][q] = F12[3 * f + q];
+        }
+        A.out[f] = out4[f];
+    }
+    A.f1 = fac[0]; A.f2 = fac[1]; A.f3 = fac[2];
+    A.b1 = bdf[0]; A.b2 = bdf[1]; A.b3 = bdf[2];
+    A.dt = dt;
+    A.mode = 1;
+    A.ifourier = filt;
+    A.ilegen = filt;
+    PArgs P;
+    P.u = P.v = P.w = nullptr;
+    P.phi = phi;
+    P.wnl = wnl; P.wviscrot = wviscrot;
+    P.modesys = S.modesys; P.sysinfo = S.sysinfo;
+    P.D = g.D; P.zh = g.zh; P.xw = g.xw; P.yw = g.yw;
+    P.dt = dt;
+    P.nxppy = g.nxppy;
+    P.nxhp = g.nxhp;
+    P.nz = g.nz;
+    constexpr bool F32 = std::is_same<TS, float>::value;
+    dim3 gt((unsigned)((g.nxppy + 63) / 64), g.nsubd);
+#define TP(NL) \
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_tile_prhs<NL, F32>), gt, dim3(256), 0, st, A, P, g.nxppy, g.fcf, g.FL)
+    switch (g.nzloc) {
+        IW_NL_SWITCH(TP)
+    default: return false;
+    }
+#undef TP
+    return true;
+}
+
+bool launch_bdf_filter_prhs(const Geo &g, double *const *S12, double *const *F12, double *const *out4,
+                            const double *fac, const double *bdf, double dt, int filt, const SysTables &S,
+                            double *phi, const double *wnl, const double *wviscrot, hipStream_t st)
+{
+    return bdf_prhs_launch<double>(g, S12, F12, out4, fac, bdf, dt, filt, S, phi, wnl, wviscrot, st);
+}
+
+bool launch_bdf_filter_prhs(const Geo &g, float *const *S12, double *const *F12, double *const *out4,
+                            const double *fac, const double *bdf, double dt, int filt, const SysTables &S,
+                            double *phi, const double *wnl, const double *wviscrot, hipStream_t st)
+{
+    return bdf_prhs_launch<float>(g, S12, F12, out4, fac, bdf, dt, filt, S, phi, wnl, wviscrot, st);
+}
+
 bool solve_np_supported(int nl, int nsubd)
 {
     (void)nsubd;
