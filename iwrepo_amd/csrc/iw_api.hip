@@ -851,6 +851,9 @@ extern "C" int iw_create(const iw_params *p, iw_solver **out)
         return -1;
     }
     HIPCHK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    HIPCHK(hipStreamCreateWithFlags(&s->stream2, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&s->ev_fork, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&s->ev_join, hipEventDisableTiming));
     setup_tables(s);
     long long &acc = s->dev_bytes;
     // filter tables: Legendre sigma (filtering.f:1225-1243), Fourier factor
@@ -1097,6 +1100,9 @@ extern "C" int iw_destroy(iw_solver *s)
     hipEventDestroy(s->ev0);
     hipEventDestroy(s->ev1);
     hipStreamDestroy(s->stream);
+    if (s->stream2) hipStreamDestroy(s->stream2);
+    if (s->ev_fork) hipEventDestroy(s->ev_fork);
+    if (s->ev_join) hipEventDestroy(s->ev_join);
     delete s;
     return 0;
 }
@@ -1691,6 +1697,17 @@ static int calcuvwt_dev(iw_solver *s, int cpen)
 {
     if (ensure_factors(s)) return -1;
     double *uvwt[4] = {s->buf[0], s->buf[1], s->buf[2], s->buf[3]};
+    // the rho solve needs nothing the pressure step changes: with the u/v/w
+    // groups it runs on the second stream beside the pressure step and the
+    // u/v/w solves, joined before the coupling pass (not when profiling, so
+    // that the per-class times stay those of one stream)
+    const bool rho_side = !s->seq_sweeps && s->sys.np_ok && s->sys.share_uvw && s->stream2 && !s->profiling;
+    if (rho_side) {
+        HIPCHK(hipEventRecord(s->ev_fork, s->stream));
+        HIPCHK(hipStreamWaitEvent(s->stream2, s->ev_fork, 0));
+        launch_solve_spike(s->g, s->P, s->sys, uvwt, s->P.dt, s->stream2, 2);
+        HIPCHK(hipEventRecord(s->ev_join, s->stream2));
+    }
     // algorithmic bytes: state columns in and out + every distinct cached factor once
     const double fcol = (double)s->sys.ldab * s->g.nz * 8. + s->g.nz * 4.;
     const double nt = (double)s->g.ntot;
@@ -1730,8 +1747,9 @@ static int calcuvwt_dev(iw_solver *s, int cpen)
             if (s->seq_sweeps)
                 launch_solve_grp(s->g, s->P, s->sys, uvwt, s->P.dt, s->stream);
             else
-                launch_solve_spike(s->g, s->P, s->sys, uvwt, s->P.dt, s->stream);
+                launch_solve_spike(s->g, s->P, s->sys, uvwt, s->P.dt, s->stream, rho_side ? 1 : 0);
         }
+        if (rho_side) HIPCHK(hipStreamWaitEvent(s->stream, s->ev_join, 0));
         if (s->seq_sweeps) {
             Scope sc(s, "lfilter", 4 * 16. * nt);
             launch_lfilter(s->g, uvwt, cpen, s->stream);

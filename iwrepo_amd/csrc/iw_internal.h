@@ -254,6 +254,10 @@ struct iw_solver {
     bool profiling = false;
     iw::Prof prof;
     hipEvent_t ev0, ev1;
+    // second stream (default priority) for the rho solve beside the pressure
+    // step, and the events that fork it from and join it to the solver stream
+    hipStream_t stream2 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     long long dev_bytes = 0;
 };
 
@@ -436,8 +440,12 @@ void launch_solve_grp(const Geo &g, const iw_params &P, const SysTables &S, doub
 // subdomain-parallel solve of the u/v, w and rho systems (k_spike), and
 // what completes it in place of launch_lfilter: the coupling recurrences and
 // correction, the Legendre filter and the mean-u damping (k_spike_finish)
+// part 0: every system; with the u/v/w groups (S.share_uvw) part 1 launches
+// all but the rho solve and part 2 the rho solve alone (it needs nothing the
+// pressure step changes, so it can run beside it); without them part 1 is
+// everything and part 2 nothing
 void launch_solve_spike(const Geo &g, const iw_params &P, const SysTables &S, double *const *uvwt, double dt,
-                        hipStream_t st);
+                        hipStream_t st, int part = 0);
 void launch_spike_finish(const Geo &g, const SysTables &S, double *const *uvwt, int cpen, hipStream_t st);
 // its per-(system, subdomain) vectors, after launch_compact
 void launch_spike_pre(const Geo &g, const SysTables &S, int first, int count, hipStream_t st);

@@ -267,7 +267,7 @@ constexpr int spike_dp()
 }
 
 void launch_solve_spike(const Geo &g, const iw_params &P, const SysTables &S, double *const *uvwt, double dt,
-                        hipStream_t st)
+                        hipStream_t st, int part)
 {
     SpikeArgs A;
     for (int q = 0; q < 4; q++) A.f[q] = uvwt[q];
@@ -299,10 +299,12 @@ void launch_solve_spike(const Geo &g, const iw_params &P, const SysTables &S, do
         dim3 gc(S.guvw_ntasks / 64, g.nsubd - 1, 1), ga(S.guv_ntasks / 64, 1, 1), gw(S.gwt_ntasks / 64, 1, 1),
             gt(S.gwt_ntasks / 64, g.nsubd, 1);
 #define GS3(NL)                                                                                                  \
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_spike<NL, 12, spike_dp<NL, 12>(), true>), gc, dim3(64), 0, st, C);   \
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_spike<NL, 8, spike_dp<NL, 8>()>), ga, dim3(64), 0, st, A);           \
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_spike<NL, 4, spike_dp<NL, 4>()>), gw, dim3(64), 0, st, W);           \
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_spike<NL, 4, spike_dp<NL, 4>()>), gt, dim3(64), 0, st, T)
+    if (part != 2) {                                                                                             \
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_spike<NL, 12, spike_dp<NL, 12>(), true>), gc, dim3(64), 0, st, C); \
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_spike<NL, 8, spike_dp<NL, 8>()>), ga, dim3(64), 0, st, A);         \
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_spike<NL, 4, spike_dp<NL, 4>()>), gw, dim3(64), 0, st, W);         \
+    }                                                                                                            \
+    if (part != 1) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_spike<NL, 4, spike_dp<NL, 4>()>), gt, dim3(64), 0, st, T)
         switch (g.nzloc) {
             IW_NL_SWITCH(GS3)
         default: break;
@@ -310,6 +312,7 @@ void launch_solve_spike(const Geo &g, const iw_params &P, const SysTables &S, do
 #undef GS3
         return;
     }
+    if (part == 2) return; // without the u/v/w groups rho is solved with w
     dim3 ga(S.guv_ntasks / 64, g.nsubd, 1), gb(S.gwt_ntasks / 64, g.nsubd, 2);
 #define GS(NL)                                                                                          \
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_spike<NL, 8, spike_dp<NL, 8>()>), ga, dim3(64), 0, st, A); \
