@@ -824,6 +824,8 @@ extern "C" int iw_create(const iw_params *p, iw_solver **out)
     {
         const char *e = getenv("IW_SWEEPS");
         s->seq_sweeps = e && !strcmp(e, "seq");
+        const char *m = getenv("IW_MIXED_MFMA");
+        s->mixed_mfma = m && !strcmp(m, "1");
     }
     Geo &g = s->g;
     g.nx = P.nx;
@@ -1465,27 +1467,38 @@ static int convec_dev(iw_solver *s, int ispecfilternl, int mcstep = 0, int mcslo
         if (mc_after(s, mcslot)) return -1;
     }
     double *ph[4] = {s->phys[0], s->phys[1], s->phys[2], s->phys[3]};
-    if (g.ndims == 3) { // fluxes and z gradients: 4 fields in, 7 (or 8) out
-        Scope sc(s, "dpdz_flux", (mean ? 96. : 88.) * nt);
-        double *o4[4] = {S[0], S[1], S[2], s->ox};
-        double *dz4[4] = {gr[2], gr[5], gr[8], mean ? s->gT[1] : (s->closure ? s->cl_buf[2] : nullptr)};
-        launch_dpdz_conv_grad(g, ph, o4, dz4, s->stream);
-    } else {
-        Scope sc(s, "dpdz_flux", 64. * nt);
-        double *o4[4] = {S[0], S[1], S[2], s->ox};
-        launch_dpdz_conv(g, ph, o4, s->stream);
+    // z flux derivatives, z gradients and the physical right-hand sides in
+    // one pass when nothing else needs the z gradients (no mean-wave terms,
+    // no closure): 10 fields in and 11 out instead of 4 in and 7 out, then 18
+    // in and 11 out
+    bool fused = false;
+    if (g.ndims == 3 && !mean && !s->closure) {
+        Scope sc(s, "conv_rhs", 168. * nt);
+        fused = launch_conv_rhs(g, s->P, s->t, ph, gr, S, s->prod, s->stream);
     }
-    {
-        Scope sc(s, "rhs_phys", 224. * nt);
-        MeanTerms mt;
-        if (mean) {
-            mt.mw = s->mw;
-            mt.ush = s->ush;
-            mt.dush = s->dush;
-            mt.dtdx = s->gT[0];
-            mt.dtdz = s->gT[1];
+    if (!fused) {
+        if (g.ndims == 3) { // fluxes and z gradients: 4 fields in, 7 (or 8) out
+            Scope sc(s, "dpdz_flux", (mean ? 96. : 88.) * nt);
+            double *o4[4] = {S[0], S[1], S[2], s->ox};
+            double *dz4[4] = {gr[2], gr[5], gr[8], mean ? s->gT[1] : (s->closure ? s->cl_buf[2] : nullptr)};
+            launch_dpdz_conv_grad(g, ph, o4, dz4, s->stream);
+        } else {
+            Scope sc(s, "dpdz_flux", 64. * nt);
+            double *o4[4] = {S[0], S[1], S[2], s->ox};
+            launch_dpdz_conv(g, ph, o4, s->stream);
         }
-        launch_rhs_phys(g, s->P, s->t, ph, gr, S, s->ox, s->prod, s->stream, &mt);
+        {
+            Scope sc(s, "rhs_phys", 224. * nt);
+            MeanTerms mt;
+            if (mean) {
+                mt.mw = s->mw;
+                mt.ush = s->ush;
+                mt.dush = s->dush;
+                mt.dtdx = s->gT[0];
+                mt.dtdz = s->gT[1];
+            }
+            launch_rhs_phys(g, s->P, s->t, ph, gr, S, s->ox, s->prod, s->stream, &mt);
+        }
     }
     if (s->closure) {
         Scope sc(s, "closure");
@@ -1659,16 +1672,22 @@ static int convec_mixed(iw_solver *s, int ispecfilternl, int mcstep = 0, int mcs
         mc_record(s, mcstep, mf.cny, ph[0]);
         if (mc_after(s, mcslot)) return -1;
     }
-    {   // vertical fluxes and z gradients on the matrix cores
-        float *o4[4] = {S32[0], S32[1], S32[2], ox32};
-        float *dz3[3] = {g32[2], g32[5], g32[8]};
-        Scope sc(s, "m_dpdz_mfma", 11 * 4. * nt);
-        if (!launch_dpdz_conv_grad_mfma(g, ph, o4, dz3, s->stream))
-            return fail("mixed path: no matrix-core z contraction for this GLL order");
-    }
-    {
-        Scope sc(s, "m_rhs_phys", 28 * 4. * nt);
-        launch_rhs_phys_f32(g, s->P, s->t, ph, g32, S32, ox32, pr32, s->stream);
+    if (s->mixed_mfma) { // the z contraction on the matrix cores, then the products
+        {
+            float *o4[4] = {S32[0], S32[1], S32[2], ox32};
+            float *dz3[3] = {g32[2], g32[5], g32[8]};
+            Scope sc(s, "m_dpdz_mfma", 11 * 4. * nt);
+            if (!launch_dpdz_conv_grad_mfma(g, ph, o4, dz3, s->stream))
+                return fail("mixed path: no matrix-core z contraction for this GLL order");
+        }
+        {
+            Scope sc(s, "m_rhs_phys", 28 * 4. * nt);
+            launch_rhs_phys_f32(g, s->P, s->t, ph, g32, S32, ox32, pr32, s->stream);
+        }
+    } else { // vertical fluxes, z gradients and the products in one pass (the z sums on the VALU from LDS)
+        Scope sc(s, "m_conv_rhs", 21 * 4. * nt);
+        if (!launch_conv_rhs(g, s->P, s->t, ph, g32, S32, pr32, s->stream))
+            return fail("mixed path: no fused z contraction for this GLL order");
     }
     {   // forward x pass: as convec_dev, on FP32 arrays
         FieldBatch bx = batch({D(S32[0]), D(S32[1]), D(S32[2]), D(S32[3]), D(pr32[2]), D(pr32[4]), D(pr32[6])});

@@ -220,6 +220,10 @@ struct iw_solver {
     bool mw_on = false;
     // mixed path (iw_set_mixed): FP32 physical-space working set in convec
     bool mixed = false;
+    // mixed path: the z contraction on the matrix cores (k_dpdz_conv_mfma)
+    // followed by the products, instead of the fused pass: IW_MIXED_MFMA=1
+    // at iw_create
+    bool mixed_mfma = false;
     // the mixed path's right-hand-side histories S, Sn, Snm of u, v, w, temp
     // in FP32 (the products they come from are FP32): filled from buf[16..23]
     // when a mixed run starts and written back to buf[12..23] when it ends,
@@ -332,6 +336,16 @@ void launch_grad_xy(const Geo &g, const double *a, double *dx, double *dy, hipSt
 void launch_kepe(const Geo &g, double *const *phys4, double cke, double cpe, double span, double *lvl, double *out,
                  hipStream_t st);
 void launch_rhs_spec(const Geo &g, double *const *S, double *const *pr, hipStream_t st);
+// the z flux derivatives and z gradients (launch_dpdz_conv_grad) and rhsfm's
+// physical part (launch_rhs_phys) in one pass, 3-D plain path without
+// mean-wave terms: S and pr as launch_rhs_phys writes them; the z gradients
+// (gr[2], gr[5], gr[8]) and d(-w rho)/dz are not stored.  false (nothing
+// launched) for orders without specialised kernels or 2-D
+bool launch_conv_rhs(const Geo &g, const iw_params &P, double t, double *const *uvwt, double *const *gr,
+                     double *const *S, double *const *pr, hipStream_t st);
+// the same on the mixed path's FP32 arrays (VALU sums in FP32)
+bool launch_conv_rhs(const Geo &g, const iw_params &P, double t, float *const *uvwt, float *const *gr,
+                     float *const *S, float *const *pr, hipStream_t st);
 void launch_wnl(const Geo &g, const double *sw, const double *swn, const double *swnm,
                 const double *fac, double *wnl, hipStream_t st);
 void launch_wnl(const Geo &g, const float *sw, const float *swn, const float *swnm, const double *fac,

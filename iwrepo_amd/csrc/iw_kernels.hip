@@ -1309,31 +1309,27 @@ struct RhsArgs {
     int mean;
 };
 
-// T: storage and arithmetic type (double for the plain path, float for the
-// mixed path, which has no mean-wave terms); the pointers of A hold T arrays
+// rhsfm's physical-space part at one point n (plane k, x index i): u, v, w,
+// tp and the nine gradients G (du/dx, du/dy, du/dz, dv/.., dw/..) at the
+// point, su, sv, sw the vertical flux derivatives, oxv d(-w rho)/dz; writes
+// S_u..S_T and the seven flux products.  T: storage and arithmetic type
+// (double for the plain path, float for the mixed path, which has no
+// mean-wave terms); the pointers of A hold T arrays.
 template <typename T>
-__global__ __launch_bounds__(256) void k_rhs_phys(RhsArgs A, long ntot, long nxppy, int nxpp, int nx,
-                                                  int nz, int nl, const double *__restrict__ zh,
-                                                  const double *__restrict__ zc,
-                                                  const double *__restrict__ xg)
+__device__ __forceinline__ void rhs_point(const RhsArgs &A, long n, int k, int i, long nxppy, int nxpp, int nx,
+                                          int nz, int nl, const double *__restrict__ zh,
+                                          const double *__restrict__ zc, const double *__restrict__ xg, T u, T v,
+                                          T w, T tp, const T (&G)[9], T su, T sv, T sw, T oxv)
 {
-    long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= ntot) return;
     const T *Au = reinterpret_cast<const T *>(A.u), *Av = reinterpret_cast<const T *>(A.v);
     const T *Aw = reinterpret_cast<const T *>(A.w), *At = reinterpret_cast<const T *>(A.t);
-    const T *G[9];
-#pragma unroll
-    for (int q = 0; q < 9; q++) G[q] = reinterpret_cast<const T *>(A.g[q]);
     T *S0 = reinterpret_cast<T *>(A.S[0]), *S1 = reinterpret_cast<T *>(A.S[1]);
     T *S2 = reinterpret_cast<T *>(A.S[2]), *S3 = reinterpret_cast<T *>(A.S[3]);
     const T half = T(0.5), four = T(4.), omz = T(A.omz);
-    const int k = (int)(n / nxppy);
-    const int i = (int)(n % nxpp);
-    T u = Au[n], v = Av[n], w = Aw[n], tp = At[n];
-    T su = S0[n], sv = S1[n], sw = S2[n], st = T(0.);
-    su = su - half * (u * G[0][n] + v * (G[1][n] - four * omz) + w * G[2][n]);
-    sv = sv - half * (u * (G[3][n] + four * omz) + v * G[4][n] + w * G[5][n]);
-    sw = sw - half * (u * G[6][n] + v * G[7][n] + w * G[8][n]);
+    T st = T(0.);
+    su = su - half * (u * G[0] + v * (G[1] - four * omz) + w * G[2]);
+    sv = sv - half * (u * (G[3] + four * omz) + v * G[4] + w * G[5]);
+    sw = sw - half * (u * G[6] + v * G[7] + w * G[8]);
     sw = sw + T(A.bcoef) * tp;
     const int k1 = k % nl, ks = k / nl;
     // total vertical velocity at the interfaces, perturbation + mean wave
@@ -1367,9 +1363,9 @@ __global__ __launch_bounds__(256) void k_rhs_phys(RhsArgs A, long ntot, long nxp
         if (A.mean) { // mean-wave and shear-current terms, solver.f:848-931
             const double *mw = A.mt.mw;
             const double advmean = mw[mi] + A.mt.ush[k];
-            su = su - advmean * G[0][n] - wmw * G[2][n];
-            sv = sv - advmean * G[3][n] - wmw * G[5][n];
-            sw = sw - advmean * G[6][n] - wmw * G[8][n];
+            su = su - advmean * G[0] - wmw * G[2];
+            sv = sv - advmean * G[3] - wmw * G[5];
+            sw = sw - advmean * G[6] - wmw * G[8];
             st = st - advmean * A.mt.dtdx[n] - wmw * A.mt.dtdz[n];
             const double dusheardz = A.mt.dush[k];
             su = su - u * mw[2 * nm + mi] - w * mw[3 * nm + mi] - w * dusheardz;
@@ -1386,7 +1382,7 @@ __global__ __launch_bounds__(256) void k_rhs_phys(RhsArgs A, long ntot, long nxp
         sw = sw + T(A.xacrit_umax * wz * cos(arg) * A.umax_zdim * A.phit);
         st = st + T(A.xacrit_umax * wz * sin(arg) * A.rfac * A.umax_zdim * A.phit);
     }
-    st = st + reinterpret_cast<const T *>(A.ox)[n] - w * T(A.rg);
+    st = st + oxv - w * T(A.rg);
     S0[n] = su;
     S1[n] = sv;
     S2[n] = sw;
@@ -1399,6 +1395,94 @@ __global__ __launch_bounds__(256) void k_rhs_phys(RhsArgs A, long ntot, long nxp
     pr[4][n] = (-half * v) * w;
     pr[5][n] = -(u * tp);
     pr[6][n] = -(v * tp);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_rhs_phys(RhsArgs A, long ntot, long nxppy, int nxpp, int nx,
+                                                  int nz, int nl, const double *__restrict__ zh,
+                                                  const double *__restrict__ zc,
+                                                  const double *__restrict__ xg)
+{
+    long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= ntot) return;
+    const T *Au = reinterpret_cast<const T *>(A.u), *Av = reinterpret_cast<const T *>(A.v);
+    const T *Aw = reinterpret_cast<const T *>(A.w), *At = reinterpret_cast<const T *>(A.t);
+    T G[9];
+#pragma unroll
+    for (int q = 0; q < 9; q++) G[q] = reinterpret_cast<const T *>(A.g[q])[n];
+    const T *S0 = reinterpret_cast<const T *>(A.S[0]), *S1 = reinterpret_cast<const T *>(A.S[1]);
+    const T *S2 = reinterpret_cast<const T *>(A.S[2]);
+    const int k = (int)(n / nxppy);
+    const int i = (int)(n % nxpp);
+    rhs_point<T>(A, n, k, i, nxppy, nxpp, nx, nz, nl, zh, zc, xg, Au[n], Av[n], Aw[n], At[n], G, S0[n], S1[n],
+                 S2[n], reinterpret_cast<const T *>(A.ox)[n]);
+}
+
+// The z flux derivatives and z gradients of k_dpdz_conv_t and rhsfm's
+// physical part of k_rhs_phys in one pass (plain path, no mean-wave terms, no
+// closure).  A work-group is 64 columns of one subdomain: their u, v, w, rho go
+// to LDS, then wavefront q takes rows q, q + 4, ... of every column, forms the
+// seven sums of dz_two for the row (same terms, same order) and hands them to
+// rhs_point, so neither the sums nor the gradients go through HBM.  T: the
+// storage and arithmetic type as in k_rhs_phys (float: the mixed path, whose
+// separate z contraction runs on the matrix cores with other rounding).
+template <int NL, typename T>
+__global__ __launch_bounds__(256) void k_conv_rhs(RhsArgs A, long nxppy, int nxpp, int nx, int nz,
+                                                  const double *__restrict__ D, const double *__restrict__ zh,
+                                                  const double *__restrict__ zc, const double *__restrict__ xg)
+{
+    __shared__ T Ds[NL * NL];
+    __shared__ T cu[NL][64], cv[NL][64], cw[NL][64], ct[NL][64];
+    for (int e = threadIdx.x; e < NL * NL; e += blockDim.x) Ds[e] = T(D[e]);
+    const T *Au = reinterpret_cast<const T *>(A.u), *Av = reinterpret_cast<const T *>(A.v);
+    const T *Aw = reinterpret_cast<const T *>(A.w), *At = reinterpret_cast<const T *>(A.t);
+    const int lane = threadIdx.x & 63, wq = threadIdx.x >> 6;
+    const long ij = (long)blockIdx.x * 64 + lane;
+    const int ks = blockIdx.y;
+    const bool live = ij < nxppy;
+    const long b0 = (long)ks * NL * nxppy + (live ? ij : 0);
+    for (int t = wq; t < NL; t += 4) {
+        const long o = b0 + (long)t * nxppy;
+        cu[t][lane] = Au[o];
+        cv[t][lane] = Av[o];
+        cw[t][lane] = Aw[o];
+        ct[t][lane] = At[o];
+    }
+    __syncthreads();
+    if (!live) return;
+    const T fac = T(2. / zh[ks]);
+    const T h = T(-0.5), one = T(-1.0);
+    const int i = (int)(ij % nxpp);
+    for (int k1 = wq; k1 < NL; k1 += 4) {
+        // dz_two of w (-0.5), u (-0.5), v (-0.5) and rho (-1.0, gradient unused)
+        T gw = T(0.), fw = T(0.), gu = T(0.), fu = T(0.), gv = T(0.), fv = T(0.), ft = T(0.);
+        // a few rows of the columns in registers at a time
+#pragma unroll 5
+        for (int t = 0; t < NL; t++) {
+            const T d = Ds[k1 * NL + t];
+            const T wt = cw[t][lane], ut = cu[t][lane], vt = cv[t][lane], tt = ct[t][lane];
+            gw = gw + d * wt;
+            fw = fw + d * ((h * wt) * wt);
+            gu = gu + d * ut;
+            fu = fu + d * ((h * ut) * wt);
+            gv = gv + d * vt;
+            fv = fv + d * ((h * vt) * wt);
+            ft = ft + d * ((one * tt) * wt);
+        }
+        const long n = b0 + (long)k1 * nxppy;
+        T G[9];
+        G[0] = reinterpret_cast<const T *>(A.g[0])[n];
+        G[1] = reinterpret_cast<const T *>(A.g[1])[n];
+        G[2] = gu * fac;
+        G[3] = reinterpret_cast<const T *>(A.g[3])[n];
+        G[4] = reinterpret_cast<const T *>(A.g[4])[n];
+        G[5] = gv * fac;
+        G[6] = reinterpret_cast<const T *>(A.g[6])[n];
+        G[7] = reinterpret_cast<const T *>(A.g[7])[n];
+        G[8] = gw * fac;
+        rhs_point<T>(A, n, ks * NL + k1, i, nxppy, nxpp, nx, nz, NL, zh, zc, xg, cu[k1][lane], cv[k1][lane],
+                     cw[k1][lane], ct[k1][lane], G, fu * fac, fv * fac, fw * fac, ft * fac);
+    }
 }
 
 // the scalar parameters of rhsfm's physical-space part
@@ -1440,6 +1524,44 @@ void launch_rhs_phys(const Geo &g, const iw_params &P, double t, double *const *
     for (int q = 0; q < 7; q++) A.pr[q] = pr[q];
     hipLaunchKernelGGL(k_rhs_phys<double>, dim3((unsigned)((g.ntot + 255) / 256)), dim3(256), 0, st, A, g.ntot,
                        g.nxppy, g.nxpp, g.nx, g.nz, g.nzloc, g.zh, g.z, g.x);
+}
+
+template <typename T>
+static bool conv_rhs_launch(const Geo &g, const iw_params &P, double t, T *const *uvwt, T *const *gr, T *const *S,
+                            T *const *pr, hipStream_t st)
+{
+    if (g.ndims != 3 || !nl_specialised(g.nzloc)) return false;
+    RhsArgs A;
+    rhs_args(g, P, t, A);
+    A.mean = 0;
+    A.u = reinterpret_cast<const double *>(uvwt[0]); A.v = reinterpret_cast<const double *>(uvwt[1]);
+    A.w = reinterpret_cast<const double *>(uvwt[2]); A.t = reinterpret_cast<const double *>(uvwt[3]);
+    for (int q = 0; q < 9; q++) A.g[q] = reinterpret_cast<const double *>(gr[q]);
+    for (int q = 0; q < 4; q++) A.S[q] = reinterpret_cast<double *>(S[q]);
+    A.ox = nullptr;
+    for (int q = 0; q < 7; q++) A.pr[q] = reinterpret_cast<double *>(pr[q]);
+    dim3 grid((unsigned)((g.nxppy + 63) / 64), g.nsubd);
+#define CR(NL)                                                                                                   \
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_conv_rhs<NL, T>), grid, dim3(256), 0, st, A, g.nxppy, g.nxpp, g.nx, g.nz, \
+                       g.D, g.zh, g.z, g.x)
+    switch (g.nzloc) {
+        IW_NL_SWITCH(CR)
+    default: return false;
+    }
+#undef CR
+    return true;
+}
+
+bool launch_conv_rhs(const Geo &g, const iw_params &P, double t, double *const *uvwt, double *const *gr,
+                     double *const *S, double *const *pr, hipStream_t st)
+{
+    return conv_rhs_launch<double>(g, P, t, uvwt, gr, S, pr, st);
+}
+
+bool launch_conv_rhs(const Geo &g, const iw_params &P, double t, float *const *uvwt, float *const *gr,
+                     float *const *S, float *const *pr, hipStream_t st)
+{
+    return conv_rhs_launch<float>(g, P, t, uvwt, gr, S, pr, st);
 }
 
 void launch_rhs_phys_f32(const Geo &g, const iw_params &P, double t, float *const *uvwt, float *const *gr,

@@ -74,6 +74,32 @@ def test_mixed_path_against_fp64_oracle(rt, oracle_mod, name, steps):
     g.close()
 
 
+def test_mixed_path_with_matrix_core_contraction(rt, oracle_mod, monkeypatch):
+    """IW_MIXED_MFMA=1 at creation: the z contraction on FP32 matrix cores
+    (k_dpdz_conv_mfma) and the products in their own pass instead of the
+    fused VALU pass; the same bounds against the FP64 oracle, and the two
+    within FP32 rounding of each other (the matrix cores accumulate the same
+    products in the same order, so they often agree bit for bit)."""
+    P, flds = _params("c1")
+    o = oracle_mod.Oracle(P)
+    o.set_initial(*flds)
+    o.run(50)
+    out = {}
+    for mfma in ("1", "0"):
+        monkeypatch.setenv("IW_MIXED_MFMA", mfma)
+        g = rt.Solver(P)
+        g.set_mixed(True)
+        g.set_initial(*flds)
+        g.run(50, counts=False)
+        errs = _field_errs(g, o, range(12))
+        print("mfma", mfma, errs, flush=True)
+        for q in range(12):
+            assert errs[STATE[q]] < FIELD_TOL, (mfma, STATE[q])
+        out[mfma] = g.get_state(0)
+        g.close()
+    assert rel(out["1"], out["0"]) < 1e-5
+
+
 def test_mixed_run_split_over_calls_is_bitwise_one_call(rt):
     """The mixed path keeps the right-hand-side histories S, Sn, Snm in FP32
     during a run and in the FP64 state between runs: 12 steps as 5 + 7 calls
