@@ -82,15 +82,22 @@ def cpu_baseline(P, fields, nsteps, threads, label):
                       "OpenMP thread(s) of %s" % (nsteps, label, dt, n, cpu_model())}
 
 
-def load_traffic(kernel, workload):
-    path = os.path.join(ROOT, "profiles", "traffic_%s_%s.json" % (kernel, workload))
-    if not os.path.exists(path):
-        return None, None
-    try:
-        d = json.load(open(path))
-        return float(d["hbm_bytes_per_launch"]), "profiles/traffic_%s_%s.json" % (kernel, workload)
-    except Exception:
-        return None, None
+def load_traffic(kernel, workload, mixed=False):
+    """HBM bytes per launch of a kernel class from its committed counter
+    summary; on the mixed path the mixed form's file when the class has one
+    (bdf_filter reads FP32 histories there)."""
+    names = (["traffic_%s_mixed_%s.json" % (kernel, workload)] if mixed else []) + \
+        ["traffic_%s_%s.json" % (kernel, workload)]
+    for name in names:
+        path = os.path.join(ROOT, "profiles", name)
+        if not os.path.exists(path):
+            continue
+        try:
+            d = json.load(open(path))
+            return float(d["hbm_bytes_per_launch"]), "profiles/" + name
+        except Exception:
+            return None, None
+    return None, None
 
 
 def time_path(Solver, P, fields, mixed, a):
@@ -186,7 +193,7 @@ def main():
     dom = max(tot, key=tot.get)
     dstat = stats[dom]
     achieved = dstat["avg_bytes"] / (dstat["avg_ms"] * 1e-3) / 1e9
-    traffic, tsrc = load_traffic(dom, a.config)
+    traffic, tsrc = load_traffic(dom, a.config, mixed=head == "mixed")
     step_gbs = rates[head] * STATE_BYTES_PER_POINT / 1e9
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                 "peak": MI355X_HBM_GBS, "unit": "GB/s", "frac": round(achieved / MI355X_HBM_GBS, 4),

@@ -1431,12 +1431,13 @@ __global__ __launch_bounds__(256) void k_conv_rhs(RhsArgs A, long nxppy, int nxp
                                                   const double *__restrict__ D, const double *__restrict__ zh,
                                                   const double *__restrict__ zc, const double *__restrict__ xg)
 {
-    __shared__ T Ds[NL * NL];
     __shared__ T cu[NL][64], cv[NL][64], cw[NL][64], ct[NL][64];
-    for (int e = threadIdx.x; e < NL * NL; e += blockDim.x) Ds[e] = T(D[e]);
     const T *Au = reinterpret_cast<const T *>(A.u), *Av = reinterpret_cast<const T *>(A.v);
     const T *Aw = reinterpret_cast<const T *>(A.w), *At = reinterpret_cast<const T *>(A.t);
-    const int lane = threadIdx.x & 63, wq = threadIdx.x >> 6;
+    // the row a wavefront works on is the same in all its lanes, so D's
+    // entries come through scalar loads (no LDS for D: three work-groups fit
+    // a CU)
+    const int lane = threadIdx.x & 63, wq = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const long ij = (long)blockIdx.x * 64 + lane;
     const int ks = blockIdx.y;
     const bool live = ij < nxppy;
@@ -1459,7 +1460,7 @@ __global__ __launch_bounds__(256) void k_conv_rhs(RhsArgs A, long nxppy, int nxp
         // a few rows of the columns in registers at a time
 #pragma unroll 5
         for (int t = 0; t < NL; t++) {
-            const T d = Ds[k1 * NL + t];
+            const T d = T(D[k1 * NL + t]);
             const T wt = cw[t][lane], ut = cu[t][lane], vt = cv[t][lane], tt = ct[t][lane];
             gw = gw + d * wt;
             fw = fw + d * ((h * wt) * wt);
@@ -3079,14 +3080,12 @@ template <int NL, bool S32>
 __global__ __launch_bounds__(256) void k_tile_prhs(TileArgs A, PArgs P, long nxppy, const double *__restrict__ fcf,
                                                    const double *__restrict__ FL)
 {
-    __shared__ double FLs[NL * NL], Ds[NL * NL];
     __shared__ double fv[3][NL][64]; // filtered u, v, w of the work-group's columns
-    for (int e = threadIdx.x; e < NL * NL; e += blockDim.x) {
-        FLs[e] = FL[e];
-        Ds[e] = P.D[e];
-    }
-    __syncthreads();
-    const int lane = threadIdx.x & 63, fz = threadIdx.x >> 6;
+    // the field is the same in all lanes of a wavefront, so the field's
+    // pointers and the filter and D entries (their indices fixed at compile
+    // time) come through scalar loads
+    const int lane = threadIdx.x & 63, fz = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const double *__restrict__ FLs = FL;
     const long col = (long)blockIdx.x * 64 + lane;
     const int ks = blockIdx.y;
     const bool live = col < nxppy; // nxppy is even: both lanes of a mode are live or not
@@ -3152,7 +3151,7 @@ __global__ __launch_bounds__(256) void k_tile_prhs(TileArgs A, PArgs P, long nxp
     for (int k = fz; k < NL; k += 4) {
         double sd = 0.;
 #pragma unroll
-        for (int t = 0; t < NL; t++) sd = sd + Ds[k * NL + t] * fv[2][t][lane];
+        for (int t = 0; t < NL; t++) sd = sd + P.D[k * NL + t] * fv[2][t][lane];
         const double uu = fv[0][k][pl], vv = fv[1][k][pl];
         double d = im ? (0. + xw * uu) + yw * vv : (0. + (-xw * uu)) + (-yw * vv);
         d = d + sd * fac;
