@@ -404,6 +404,7 @@ struct FinishArgs {
     const double *FL, *damp;
     long nxppy;
     int nsub, cpen;
+    int share_uvw; // w's blocks below the top subdomain are the u/v ones: read their k, h
 };
 
 template <int NL>
@@ -411,13 +412,18 @@ __global__ __launch_bounds__(256) void k_spike_finish(FinishArgs A)
 {
     __shared__ double FLs[NL * NL];
     stage_fl<NL>(FLs, A.FL);
-    const long col = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int s = blockIdx.y, fld = blockIdx.z, nsub = A.nsub;
+    // wavefront f = field f of the work-group's 64 columns: u, v (and w below
+    // the top subdomain with the u/v/w groups) read the same k, h, once from
+    // HBM for the work-group
+    const int fld = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const long col = (long)blockIdx.x * 64 + (threadIdx.x & 63);
+    const int s = blockIdx.y, nsub = A.nsub;
     const long st = A.nxppy;
     if (col >= st) return;
     const long m = col >> 1;
     if (fld == 2 && m == 0) return;
-    const int sys = A.modesys[4 * m + (fld < 2 ? 1 : fld == 2 ? 2 : 3)];
+    const bool uvk = fld < 2 || (fld == 2 && A.share_uvw && s < nsub - 1);
+    const int sys = A.modesys[4 * m + (uvk ? 1 : fld == 2 ? 2 : 3)];
     const double2 *kh = reinterpret_cast<const double2 *>(A.SPK + ((long)sys * nsub + s) * spk_stride(NL));
     const long o = ((long)fld * nsub + s) * st + col;
     const double c = A.al[o], d = A.au[o];
@@ -457,7 +463,8 @@ void launch_spike_finish(const Geo &g, const SysTables &S, double *const *uvwt, 
     A.nxppy = g.nxppy;
     A.nsub = g.nsubd;
     A.cpen = cpen;
-    dim3 grid((unsigned)((g.nxppy + 255) / 256), g.nsubd, 4);
+    A.share_uvw = S.share_uvw ? 1 : 0;
+    dim3 grid((unsigned)((g.nxppy + 63) / 64), g.nsubd);
 #define FN(NL) hipLaunchKernelGGL(k_spike_finish<NL>, grid, dim3(256), 0, st, A)
     switch (g.nzloc) {
         IW_NL_SWITCH(FN)
