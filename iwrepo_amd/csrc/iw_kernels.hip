@@ -1670,11 +1670,15 @@ struct RotArgs {
     const double *lev[9]; // u v w | un vn wn | unm vnm wnm
 };
 
+// NL > 0: the GLL order at compile time, the column loops unrolled so that a
+// thread's 300 loads are in flight together (the same sums in the same order)
+template <int NL>
 __global__ void k_rotvisc(RotArgs A, double f1, double f2, double f3, double xnu, double *wviscrot,
-                          int nmode, int nxhp, int ny, int nl, int nsubd, long nxppy,
+                          int nmode, int nxhp, int ny, int nl_, int nsubd, long nxppy,
                           const double *__restrict__ D, const double *__restrict__ zh,
                           const double *__restrict__ xw, const double *__restrict__ yw)
 {
+    const int nl = NL > 0 ? NL : nl_;
     int id = blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= 2 * nmode) return;
     int side = id / nmode, m = id - side * nmode;
@@ -1685,10 +1689,12 @@ __global__ void k_rotvisc(RotArgs A, double f1, double f2, double f3, double xnu
     double kx = xw[ii], ky = yw[j];
     double fac2z = 2. / zh[ks];
     double rr[3], ri[3];
+#pragma unroll
     for (int L = 0; L < 3; L++) {
         const double *u = A.lev[3 * L], *v = A.lev[3 * L + 1], *w = A.lev[3 * L + 2];
         double sur = 0., sui = 0., svr = 0., svi = 0.;
-        for (int t = 0; t < nl; t++) {
+#pragma unroll
+        for (int t = 0; t < (NL > 0 ? NL : nl); t++) {
             long n = (long)(ks * nl + t) * nxppy + o;
             double dd = D[k0 * nl + t];
             sur = sur + dd * u[n];
@@ -1714,9 +1720,14 @@ void launch_rotvisc(const Geo &g, const iw_params &P, double *const *lev, const 
     RotArgs A;
     for (int q = 0; q < 9; q++) A.lev[q] = lev[q];
     int nmode = g.nxhp * g.ny;
-    hipLaunchKernelGGL(k_rotvisc, dim3((2 * nmode + 127) / 128), dim3(128), 0, st, A, fac[0], fac[1],
-                       fac[2], P.xnu, wviscrot, nmode, g.nxhp, g.ny, g.nzloc, g.nsubd, g.nxppy, g.D,
-                       g.zh, g.xw, g.yw);
+#define RV(NL)                                                                                                     \
+    hipLaunchKernelGGL(k_rotvisc<NL>, dim3((2 * nmode + 127) / 128), dim3(128), 0, st, A, fac[0], fac[1], fac[2], \
+                       P.xnu, wviscrot, nmode, g.nxhp, g.ny, g.nzloc, g.nsubd, g.nxppy, g.D, g.zh, g.xw, g.yw)
+    switch (g.nzloc) {
+        IW_NL_SWITCH(RV)
+    default: RV(0);
+    }
+#undef RV
 }
 
 // ---------------------------------------------------------------------------
