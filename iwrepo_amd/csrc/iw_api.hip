@@ -1717,14 +1717,16 @@ static int calcuvwt_dev(iw_solver *s, int cpen)
     if (ensure_factors(s)) return -1;
     double *uvwt[4] = {s->buf[0], s->buf[1], s->buf[2], s->buf[3]};
     // the rho solve needs nothing the pressure step changes: with the u/v/w
-    // groups it runs on the second stream beside the pressure step and the
-    // u/v/w solves, joined before the coupling pass (not when profiling, so
-    // that the per-class times stay those of one stream)
+    // groups it runs with its coupling pass and filter on the second stream
+    // beside the pressure step and the u/v/w solves, joined after the u/v/w
+    // coupling pass (not when profiling, so that the per-class times stay
+    // those of one stream)
     const bool rho_side = !s->seq_sweeps && s->sys.np_ok && s->sys.share_uvw && s->stream2 && !s->profiling;
     if (rho_side) {
         HIPCHK(hipEventRecord(s->ev_fork, s->stream));
         HIPCHK(hipStreamWaitEvent(s->stream2, s->ev_fork, 0));
         launch_solve_spike(s->g, s->P, s->sys, uvwt, s->P.dt, s->stream2, 2);
+        launch_spike_finish(s->g, s->sys, uvwt, cpen, s->stream2, 2);
         HIPCHK(hipEventRecord(s->ev_join, s->stream2));
     }
     // algorithmic bytes: state columns in and out + every distinct cached factor once
@@ -1768,7 +1770,6 @@ static int calcuvwt_dev(iw_solver *s, int cpen)
             else
                 launch_solve_spike(s->g, s->P, s->sys, uvwt, s->P.dt, s->stream, rho_side ? 1 : 0);
         }
-        if (rho_side) HIPCHK(hipStreamWaitEvent(s->stream, s->ev_join, 0));
         if (s->seq_sweeps) {
             Scope sc(s, "lfilter", 4 * 16. * nt);
             launch_lfilter(s->g, uvwt, cpen, s->stream);
@@ -1779,8 +1780,9 @@ static int calcuvwt_dev(iw_solver *s, int cpen)
             const double fix = 6. * 4. * s->g.nsubd * s->g.nxppy * 8. +
                                (s->sys.nsys - s->sys.np) * (2. * s->P.nzloc + 4.) * s->P.nsubd * 8.;
             Scope sc(s, "lfilter", 4 * 16. * nt + fix);
-            launch_spike_finish(s->g, s->sys, uvwt, cpen, s->stream);
+            launch_spike_finish(s->g, s->sys, uvwt, cpen, s->stream, rho_side ? 1 : 0);
         }
+        if (rho_side) HIPCHK(hipStreamWaitEvent(s->stream, s->ev_join, 0));
     } else {
         Scope sc(s, "viscous", 64. * nt + (s->sys.nsys - s->sys.np) * fcol);
         launch_viscous(s->g, s->P, s->sys, uvwt, s->P.dt, cpen, s->stream);

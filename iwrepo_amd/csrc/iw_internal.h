@@ -460,7 +460,8 @@ void launch_solve_grp(const Geo &g, const iw_params &P, const SysTables &S, doub
 // everything and part 2 nothing
 void launch_solve_spike(const Geo &g, const iw_params &P, const SysTables &S, double *const *uvwt, double dt,
                         hipStream_t st, int part = 0);
-void launch_spike_finish(const Geo &g, const SysTables &S, double *const *uvwt, int cpen, hipStream_t st);
+void launch_spike_finish(const Geo &g, const SysTables &S, double *const *uvwt, int cpen, hipStream_t st,
+                         int part = 0);
 // its per-(system, subdomain) vectors, after launch_compact
 void launch_spike_pre(const Geo &g, const SysTables &S, int first, int count, hipStream_t st);
 // sets *flag when the u/v and w factor blocks below the top subdomain differ

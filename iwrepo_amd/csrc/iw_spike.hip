@@ -356,10 +356,11 @@ void launch_share_check(const Geo &g, const SysTables &S, int *flag, hipStream_t
 // the two scalar recurrences per column (one thread per column and field);
 // SPB holds beta, gamma, delta of (system, subdomain) at (system nsub + s) 4
 __global__ __launch_bounds__(256) void k_spike_couple(double *al, double *au, const int *__restrict__ modesys,
-                                                      const double *__restrict__ SPB, long nxppy, int nsub)
+                                                      const double *__restrict__ SPB, long nxppy, int nsub,
+                                                      int fld0)
 {
     const long col = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int fld = blockIdx.y;
+    const int fld = fld0 + (int)blockIdx.y;
     if (col >= nxppy) return;
     const int m = (int)(col >> 1);
     if (fld == 2 && m == 0) return;
@@ -405,6 +406,7 @@ struct FinishArgs {
     long nxppy;
     int nsub, cpen;
     int share_uvw; // w's blocks below the top subdomain are the u/v ones: read their k, h
+    int fld0;      // field of the first wavefront
 };
 
 template <int NL>
@@ -415,7 +417,7 @@ __global__ __launch_bounds__(256) void k_spike_finish(FinishArgs A)
     // wavefront f = field f of the work-group's 64 columns: u, v (and w below
     // the top subdomain with the u/v/w groups) read the same k, h, once from
     // HBM for the work-group
-    const int fld = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int fld = A.fld0 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const long col = (long)blockIdx.x * 64 + (threadIdx.x & 63);
     const int s = blockIdx.y, nsub = A.nsub;
     const long st = A.nxppy;
@@ -448,10 +450,13 @@ __global__ __launch_bounds__(256) void k_spike_finish(FinishArgs A)
     }
 }
 
-void launch_spike_finish(const Geo &g, const SysTables &S, double *const *uvwt, int cpen, hipStream_t st)
+// part 0: the four fields; 1: u, v, w; 2: rho (beside the pressure step on
+// a second stream, after the rho solve of launch_solve_spike part 2)
+void launch_spike_finish(const Geo &g, const SysTables &S, double *const *uvwt, int cpen, hipStream_t st, int part)
 {
-    hipLaunchKernelGGL(k_spike_couple, dim3((unsigned)((g.nxppy + 255) / 256), 4), dim3(256), 0, st, S.spk_al,
-                       S.spk_au, S.modesys, S.SPB, g.nxppy, g.nsubd);
+    const int f0 = part == 2 ? 3 : 0, nf = part == 0 ? 4 : part == 1 ? 3 : 1;
+    hipLaunchKernelGGL(k_spike_couple, dim3((unsigned)((g.nxppy + 255) / 256), nf), dim3(256), 0, st, S.spk_al,
+                       S.spk_au, S.modesys, S.SPB, g.nxppy, g.nsubd, f0);
     FinishArgs A;
     for (int q = 0; q < 4; q++) A.f[q] = uvwt[q];
     A.modesys = S.modesys;
@@ -464,8 +469,9 @@ void launch_spike_finish(const Geo &g, const SysTables &S, double *const *uvwt, 
     A.nsub = g.nsubd;
     A.cpen = cpen;
     A.share_uvw = S.share_uvw ? 1 : 0;
+    A.fld0 = f0;
     dim3 grid((unsigned)((g.nxppy + 63) / 64), g.nsubd);
-#define FN(NL) hipLaunchKernelGGL(k_spike_finish<NL>, grid, dim3(256), 0, st, A)
+#define FN(NL) hipLaunchKernelGGL(k_spike_finish<NL>, grid, dim3(64 * nf), 0, st, A)
     switch (g.nzloc) {
         IW_NL_SWITCH(FN)
     default: break;
