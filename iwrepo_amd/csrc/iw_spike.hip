@@ -407,6 +407,7 @@ struct FinishArgs {
     int nsub, cpen;
     int share_uvw; // w's blocks below the top subdomain are the u/v ones: read their k, h
     int fld0;      // field of the first wavefront
+    int nf;        // fields of the launch; with one, a work-group covers blockDim.x columns of it
 };
 
 template <int NL>
@@ -414,11 +415,13 @@ __global__ __launch_bounds__(256) void k_spike_finish(FinishArgs A)
 {
     __shared__ double FLs[NL * NL];
     stage_fl<NL>(FLs, A.FL);
-    // wavefront f = field f of the work-group's 64 columns: u, v (and w below
-    // the top subdomain with the u/v/w groups) read the same k, h, once from
-    // HBM for the work-group
-    const int fld = A.fld0 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const long col = (long)blockIdx.x * 64 + (threadIdx.x & 63);
+    // wavefront f = field fld0 + f of the work-group's 64 columns: u, v (and
+    // w below the top subdomain with the u/v/w groups) read the same k, h,
+    // once from HBM for the work-group; a one-field launch covers blockDim.x
+    // columns of that field per work-group
+    const bool one = A.nf == 1;
+    const int fld = one ? A.fld0 : A.fld0 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const long col = one ? (long)blockIdx.x * blockDim.x + threadIdx.x : (long)blockIdx.x * 64 + (threadIdx.x & 63);
     const int s = blockIdx.y, nsub = A.nsub;
     const long st = A.nxppy;
     if (col >= st) return;
@@ -470,8 +473,10 @@ void launch_spike_finish(const Geo &g, const SysTables &S, double *const *uvwt, 
     A.cpen = cpen;
     A.share_uvw = S.share_uvw ? 1 : 0;
     A.fld0 = f0;
-    dim3 grid((unsigned)((g.nxppy + 63) / 64), g.nsubd);
-#define FN(NL) hipLaunchKernelGGL(k_spike_finish<NL>, grid, dim3(64 * nf), 0, st, A)
+    A.nf = nf;
+    const int cpw = nf == 1 ? 256 : 64; // columns per work-group
+    dim3 grid((unsigned)((g.nxppy + cpw - 1) / cpw), g.nsubd);
+#define FN(NL) hipLaunchKernelGGL(k_spike_finish<NL>, grid, dim3(nf == 1 ? 256 : 64 * nf), 0, st, A)
     switch (g.nzloc) {
         IW_NL_SWITCH(FN)
     default: break;
