@@ -187,8 +187,25 @@ __global__ __launch_bounds__(kMcThreads) void k_mc_final(McArgs A, int istep, do
     rec[41] = probe(p2);
 }
 
+// mc_uu with its three cases fixed at compile time (FP32 u, the Nyquist
+// term, the mean-wave term): the same operations in the same order
+template <bool F32, bool CNY, bool MW>
+__device__ __forceinline__ double mc_uu_t(const McArgs &A, int i, int j, int k)
+{
+    const long o = (long)k * A.nxppy + (long)j * A.nxpp;
+    double x = F32 ? (double)A.u32[o + i] : A.u[o + i];
+    if (CNY) {
+        const double c = A.cny * A.spec[o + A.nx];
+        x = (i & 1) ? x - c : x + c;
+    }
+    return MW ? x + A.umwv[(long)k * A.nxpp + i] : x + 0.0;
+}
+
 // partial column maxima of u over kMcSlices slices of z:
-// cp[kc][i ny + j] = max over the slice of u(i,j,k)
+// cp[kc][i ny + j] = max over the slice of u(i,j,k).  Four loads in flight
+// per step; the maximum does not depend on the order (fmax skips NaN, and
+// -0/+0 compare equal wherever the maxima are used)
+template <bool F32, bool CNY, bool MW>
 __global__ __launch_bounds__(256) void k_mc_colmax(McArgs A, double *cp)
 {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
@@ -197,7 +214,13 @@ __global__ __launch_bounds__(256) void k_mc_colmax(McArgs A, double *cp)
     const int j = e / A.nx, i = e - j * A.nx;
     const int kc = blockIdx.y, k0 = kc * A.nz / kMcSlices, k1 = (kc + 1) * A.nz / kMcSlices;
     double m = -DBL_MAX;
-    for (int k = k0; k < k1; k++) m = fmax(m, mc_uu(A, i, j, k));
+    int k = k0;
+    for (; k + 4 <= k1; k += 4) {
+        const double a = mc_uu_t<F32, CNY, MW>(A, i, j, k), b = mc_uu_t<F32, CNY, MW>(A, i, j, k + 1);
+        const double c = mc_uu_t<F32, CNY, MW>(A, i, j, k + 2), d = mc_uu_t<F32, CNY, MW>(A, i, j, k + 3);
+        m = fmax(m, fmax(fmax(a, b), fmax(c, d)));
+    }
+    for (; k < k1; k++) m = fmax(m, mc_uu_t<F32, CNY, MW>(A, i, j, k));
     cp[(size_t)kc * nxy + i * A.ny + j] = m;
 }
 
@@ -345,7 +368,17 @@ void launch_maxcalc_umaxt(const Geo &g, const McArgs &A, double umax, double *co
 {
     const int nxy = g.nx * g.ny;
     double *cm = colmax + (size_t)kMcSlices * nxy;
-    hipLaunchKernelGGL(k_mc_colmax, dim3((nxy + 255) / 256, kMcSlices), dim3(256), 0, st, A, colmax);
+    const dim3 grid((nxy + 255) / 256, kMcSlices);
+#define CM(F, C, M) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_mc_colmax<F, C, M>), grid, dim3(256), 0, st, A, colmax)
+    const bool c = A.cny != 0.0, w = A.umwv != nullptr;
+    if (A.u32) {
+        if (c) { if (w) CM(true, true, true); else CM(true, true, false); }
+        else { if (w) CM(true, false, true); else CM(true, false, false); }
+    } else {
+        if (c) { if (w) CM(false, true, true); else CM(false, true, false); }
+        else { if (w) CM(false, false, true); else CM(false, false, false); }
+    }
+#undef CM
     hipLaunchKernelGGL(k_mc_groupmax, dim3((nxy + 255) / 256), dim3(256), 0, st, colmax, nxy, cm, groupmax);
     hipLaunchKernelGGL(k_mc_walk, dim3(1), dim3(256), 0, st, A, cm, groupmax, umax, rec);
 }
