@@ -241,7 +241,10 @@ int iw_set_closure(iw_solver *coarse, iw_mlp *m);
  * (members dealt to independent chains) and replayed, one launch per three
  * steps.  Members run without maxcalc records, adaptive dt, the estimation
  * filter, the output routines and the paddle; their state after
- * iw_ensemble_run equals that of iw_run on each member. */
+ * iw_ensemble_run equals that of iw_run on each member.  While a solver is a
+ * member, iw_run keeps all of its launches on the solver's own stream (no
+ * second stream for the rho solve); iw_ensemble_destroy restores it, so the
+ * members must outlive the ensemble. */
 typedef struct iw_ensemble iw_ensemble;
 int iw_ensemble_create(iw_solver **members, int n, iw_ensemble **out);
 int iw_ensemble_destroy(iw_ensemble *e);

@@ -1721,7 +1721,8 @@ static int calcuvwt_dev(iw_solver *s, int cpen)
     // beside the pressure step and the u/v/w solves, joined after the u/v/w
     // coupling pass (not when profiling, so that the per-class times stay
     // those of one stream)
-    const bool rho_side = !s->seq_sweeps && s->sys.np_ok && s->sys.share_uvw && s->stream2 && !s->profiling;
+    const bool rho_side = !s->seq_sweeps && s->sys.np_ok && s->sys.share_uvw && s->stream2 && !s->profiling &&
+                          !s->one_stream;
     if (rho_side) {
         HIPCHK(hipEventRecord(s->ev_fork, s->stream));
         HIPCHK(hipStreamWaitEvent(s->stream2, s->ev_fork, 0));

@@ -97,6 +97,7 @@ extern "C" int iw_ensemble_create(iw_solver **members, int n, iw_ensemble **out)
             iw_ensemble_destroy(e);
             return iw_fail("iw_ensemble_create: events");
         }
+    for (iw_solver *s : e->m) s->one_stream = true;
     *out = e;
     return 0;
 }
@@ -105,6 +106,7 @@ extern "C" int iw_ensemble_destroy(iw_ensemble *e)
 {
     if (!e) return 0;
     drop_graph(e);
+    for (iw_solver *s : e->m) s->one_stream = false;
     for (hipEvent_t v : e->ev)
         if (v) (void)hipEventDestroy(v);
     if (e->cs) (void)hipStreamDestroy(e->cs);

@@ -261,6 +261,10 @@ struct iw_solver {
     // second stream (default priority) for the rho solve beside the pressure
     // step, and the events that fork it from and join it to the solver stream
     hipStream_t stream2 = nullptr;
+    // set while the solver is a member of an ensemble: every launch stays on
+    // the solver stream (the ensemble's chains are the concurrency, and the
+    // extra branches of a second stream per member slow its graph down)
+    bool one_stream = false;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     long long dev_bytes = 0;
 };
